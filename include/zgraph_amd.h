@@ -1,0 +1,218 @@
+/*
+ * zgraph_amd.h -- C ABI of the MI355X-native ASAP neighbour-sampling
+ * estimator library (libzgraph_amd.so).
+ *
+ * Drop-in boundary for ZGraph's estimator path (jy-yuan/sampling-graph-mining):
+ * the reference runs `ZGraphInstance<E>::calculate` (src/ZGraphInstance.cpp:256-297)
+ * over user `estimate()` callbacks on a per-socket CSR (include/Graph.hpp:148-166).
+ * Here the CSR lives in HBM on one GPU, each estimate() body is a device
+ * "plan", and calculate() is one kernel launch over estimator ids
+ * [first_id, first_id + count), each drawing from its own Philox4x32-10
+ * stream keyed by (seed, estimator id).  Plain pointers and sizes only; every
+ * function returns 0 on success or a negative ZG_E* code, with a message
+ * available from zg_last_error().  Calls on one graph handle are serialised
+ * on that handle's stream.
+ */
+#ifndef ZGRAPH_AMD_H
+#define ZGRAPH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define ZG_OK 0
+#define ZG_EINVAL (-1)      /* bad argument (self-loop, id >= n, size limits) */
+#define ZG_EHIP (-2)        /* HIP runtime error (message has the HIP string) */
+#define ZG_ENOMEM (-3)      /* device allocation failed */
+#define ZG_EOVERFLOW (-4)   /* an estimator value exceeded 128 bits */
+#define ZG_ESTATE (-5)      /* no GPU / library not initialised / empty result */
+
+typedef struct zg_graph zg_graph;
+typedef struct zg_plan zg_plan;
+
+/* Copies the last error message of the calling thread (NUL-terminated). */
+int zg_last_error(char *buf, size_t len);
+
+/* Library/device initialisation (device 0).  Called implicitly by the first
+ * graph upload; fails with ZG_ESTATE when no HIP device is present. */
+int zg_init(void);
+/* Device properties used by the launch heuristics: CU count, LDS per CU. */
+int zg_device_info(int *num_cus, int *lds_per_cu, int *warp_size);
+
+/* ---- graphs ------------------------------------------------------------ *
+ * Replaces DistGraphLoader<E>::load_graph + CSRConstructor<E>::
+ * construct_bitmap_assisted_csr (src/Graph.cpp:214-377, :386-615) at one
+ * worker (W = 1: no partitioning, no cross-partition edge dropping).
+ * Input: m edges in file order (src[i], dst[i]) over vertices [0, n), an
+ * optional float weight per edge, and a shuffle seed.  Edge ranks ("edge
+ * ids") are a keyed random permutation of the input order; every vertex's
+ * adjacency is ascending by rank, as the reference asserts
+ * (src/Graph.cpp:324-334).  Self-loops are rejected (src/Graph.cpp:454).
+ * Requires m < 2^32.  Host pointers. */
+int zg_graph_create(const uint32_t *src, const uint32_t *dst, const float *weights,
+                    uint64_t m, uint32_t n, uint64_t shuffle_seed, zg_graph **out);
+/* Same, with the edge list already in device memory (not consumed). */
+int zg_graph_create_device(const uint32_t *d_src, const uint32_t *d_dst, const float *d_weights,
+                           uint64_t m, uint32_t n, uint64_t shuffle_seed, zg_graph **out);
+int zg_graph_destroy(zg_graph *g);
+
+typedef struct {
+    uint32_t n;
+    uint64_t m;            /* stored edges ("num_local_edges") */
+    uint64_t num_edges;    /* graph.num_edges (ELP bound input) */
+    uint64_t max_degree;   /* graph.max_degree */
+    uint64_t device_bytes; /* HBM held by the handle */
+    int has_weights;
+} zg_graph_info;
+int zg_graph_info_get(const zg_graph *g, zg_graph_info *out);
+
+/* Copies the device CSR back for inspection.  Any pointer may be NULL.
+ * src/dst/w: [m] by rank; off: [n+1]; adj_rank/adj_nbr: [2m], each list
+ * ascending by rank (the reference's AdjUnit edge_id / dst). */
+int zg_graph_export(const zg_graph *g, uint32_t *src, uint32_t *dst, float *w, uint64_t *off,
+                    uint32_t *adj_rank, uint32_t *adj_nbr);
+
+/* GraphSampler<E>::sample_graph (src/Graph.cpp:628-780): keeps each edge iff
+ * 1 + U[0, 2^30) <= floor(rate * 2^30) drawn from Philox (seed, edge rank),
+ * rebuilds the CSR with a fresh rank permutation (same seed). */
+int zg_graph_subsample(const zg_graph *in, double rate, uint64_t seed, zg_graph **out);
+/* GraphFilter<E>::generate_graph (src/GraphFilters.cpp:16-177) with the
+ * predicate `w >= threshold` (keep_ge != 0) or `w < threshold`. */
+int zg_graph_filter(const zg_graph *in, float threshold, int keep_ge, uint64_t shuffle_seed,
+                    zg_graph **out);
+
+/* ---- seeded synthetic graphs (device generators) ----------------------- *
+ * No reference counterpart (the reference reads SNAP files); these build
+ * the benchmark inputs in HBM.  Outputs are device arrays owned by the
+ * library, freed with zg_edges_free.  All generators are deterministic in
+ * (parameters, seed), de-duplicate undirected pairs and drop self-loops. */
+typedef struct {
+    uint32_t *d_src, *d_dst;   /* device pointers, m entries, src < dst */
+    float *d_w;                /* device pointer or NULL */
+    uint64_t m;
+    uint32_t n;
+} zg_edges;
+/* Erdos-Renyi G(n, m): m distinct uniform pairs. */
+int zg_gen_erdos_renyi(uint32_t n, uint64_t m, uint64_t seed, zg_edges *out);
+/* Chung-Lu: weights w_i = (i + i0)^(-1/(beta-1)); pairs drawn from w x w
+ * until m distinct pairs; i0 chosen so the expected top degree is max_deg. */
+int zg_gen_chung_lu(uint32_t n, uint64_t m, double beta, double max_deg, uint64_t seed, zg_edges *out);
+/* R-MAT (a, b, c, 1-a-b-c) at 2^scale vertices until m distinct pairs. */
+int zg_gen_rmat(int scale, uint64_t m, double a, double b, double c, uint64_t seed, zg_edges *out);
+/* Attach float weights U[0,1) (Philox keyed by (seed, edge index)). */
+int zg_edges_add_uniform_weights(zg_edges *e, uint64_t seed);
+/* Copy device edges to host buffers (any may be NULL). */
+int zg_edges_download(const zg_edges *e, uint32_t *src, uint32_t *dst, float *w);
+int zg_edges_free(zg_edges *e);
+
+/* ---- plans: the estimate() bodies -------------------------------------- */
+enum {
+    ZG_PLAN_TRIANGLE = 1,     /* TriangleCoutingInstance (applications/Triangle.cpp:48-73) */
+    ZG_PLAN_THREE_CHAIN = 2,  /* ThreeChainCountingInstance (applications/ThreeChain.cpp:22-37) */
+    ZG_PLAN_THREE_MOTIF = 3,  /* ThreeMotif.cpp:48-121 fused: channel 0 = 3-chain, 1 = triangle */
+    ZG_PLAN_CHAIN = 4,        /* ChainMiningInstance(order) (applications/ChainMining.cpp:45-84) */
+    ZG_PLAN_FOUR_CHAIN = 5,   /* FourChainCountingInstance (applications/FourChain.cpp:24-124) */
+    ZG_PLAN_PATTERN = 6,      /* SamplerGenerator<Empty>(Pattern) (src/SamplerGenerator.cpp:378-464) */
+    ZG_PLAN_K4_CLOSE = 7      /* 4-clique: three spanning shapes + closing checks (unbiased;
+                                 replaces the biased applications/FourClique.cpp:41-76) */
+};
+
+#define ZG_MAX_PATTERN_EDGES 15
+typedef struct {
+    int kind;
+    int order;                               /* ZG_PLAN_CHAIN: vertices in the path (>= 3) */
+    int pattern_vertices;                    /* ZG_PLAN_PATTERN: k (2..6) */
+    int pattern_edges;                       /* ZG_PLAN_PATTERN: E (1..15) */
+    uint32_t pattern_src[ZG_MAX_PATTERN_EDGES];
+    uint32_t pattern_dst[ZG_MAX_PATTERN_EDGES];
+} zg_plan_desc;
+
+int zg_plan_create(const zg_plan_desc *desc, zg_plan **out);
+int zg_plan_destroy(zg_plan *p);
+/* outputs per estimator (2 for ZG_PLAN_THREE_MOTIF, else 1) */
+int zg_plan_channels(const zg_plan *p);
+/* conditional_sample_edge calls per successful estimate
+ * (get_num_conditional_sample_edge_calls, src/ZGraphInstance.cpp:470-488;
+ * SamplerGenerator override src/SamplerGenerator.cpp:471-474) */
+double zg_plan_cond_calls(const zg_plan *p);
+/* SamplerGenerator unique relabelings (src/SamplerGenerator.cpp:336-359). */
+int zg_plan_unique_patterns(const zg_plan *p, int *unique, int *total);
+
+/* ---- runs (calculate) -------------------------------------------------- */
+typedef struct {
+    uint64_t sum[3];      /* exact sum of values * 2^log2_den, 192-bit little-endian */
+    int32_t log2_den;     /* the estimate is sum / 2^log2_den / count */
+    int32_t status;       /* 0 ok, ZG_EOVERFLOW if some value exceeded 128 bits */
+    uint64_t successes;   /* estimators with a nonzero value */
+    uint64_t count;       /* estimators run */
+    double sumsq;         /* sum of squared values (for the CLT interval) */
+    double mean;          /* sum / 2^log2_den / count */
+} zg_sums;
+
+/* Per-estimator record (parity dumps). */
+typedef struct {
+    uint64_t num_lo, num_hi; /* value = num / 2^log2den */
+    uint32_t e0;             /* rank of the first sampled edge */
+    uint8_t status;          /* 0 fail, 1 success, 3 overflow */
+    uint8_t log2den;
+    uint8_t ncond;           /* conditional_sample_edge calls made */
+    uint8_t nwords;          /* Philox words consumed (saturating at 255) */
+} zg_outcome;
+
+typedef struct {
+    int block_size;     /* threads per workgroup (multiple of 64; 0 = default 256) */
+    int blocks_per_cu;  /* workgroups per CU in the grid (0 = default) */
+} zg_launch;
+
+/* The calculate() loop: runs ids [first_id, first_id+count) and reduces
+ * exactly.  sums has zg_plan_channels() entries.  kernel_ms (may be NULL)
+ * receives the estimator kernel's duration from HIP events on the handle's
+ * stream.  Blocks until done. */
+int zg_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_id, uint64_t count,
+           const zg_launch *launch, zg_sums *sums, float *kernel_ms);
+/* Same, and copies every estimator's outcome(s) to host memory
+ * (count * channels records, channel-minor). */
+int zg_run_outcomes(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_id,
+                    uint64_t count, zg_outcome *host_out, zg_sums *sums);
+
+/* Algorithmic read count of a run: the number of dependent global reads the
+ * estimators of ids [first_id, first_id+count) issue (each one 64-byte
+ * sector), for the roofline's algorithmic bytes.  Same outcomes as zg_run;
+ * slower (it counts with atomics).  No reference counterpart. */
+int zg_count_reads(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_id, uint64_t count,
+                   uint64_t *reads);
+
+/* ---- error-latency profile (src/ErrorLatencyProfile.cpp) --------------- */
+/* ErrorProfile<E>::get_num_estimators (:71-207) on the device: subsample at
+ * rate 0.1, 2^14 pilot batches of 16 estimators plus 2^23 more, EMA error
+ * curve, through-origin regression K, N = bound(eps, 1-conf, a, b, f, Delta, m) * K^2.
+ * a = 1 (:449-452), b = zg_plan_cond_calls.  Writes N (rounded down), K and the
+ * rough estimate f.  `pilot_scale` divides the pilot sizes by 2^pilot_scale (0 = reference sizes). */
+typedef struct {
+    double num_estimators;   /* N before truncation */
+    double K;                /* regression slope */
+    double rough_estimate;   /* F / 0.1^(a+b) */
+    double sampled_estimate; /* F on the subsampled graph */
+    uint64_t sampled_m, sampled_max_degree;
+    double elapsed_ms;
+} zg_elp_result;
+int zg_elp_regression(zg_graph *g, const zg_plan *p, double eps, double conf, uint64_t seed,
+                      int pilot_scale, zg_elp_result *out);
+/* The bound alone (ErrorProfile::calculate_bound, :51-59). */
+double zg_elp_bound(double eps, double delta, double a, double b, double est, uint64_t max_degree,
+                    uint64_t num_edges);
+/* UnbiasedLinearRegression::solve (src/Regression.cpp:17-28) on host arrays. */
+long double zg_regression_k(const long double *x, const long double *y, unsigned len);
+/* The arithmetic of get_num_estimators (:127-200) given pilot running means. */
+double zg_elp_from_pilots(const long double *est, uint64_t npts, long double F, double a, double b,
+                          uint64_t samp_maxdeg, uint64_t samp_m, uint64_t maxdeg, uint64_t m,
+                          double eps, double conf, long double *K_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -29,6 +29,8 @@
 #include <string.h>
 #include <omp.h>
 
+_Static_assert(sizeof(og_outcome) == 24, "og_outcome must match the 24-byte zg_outcome record");
+
 /* ===================================================================== */
 /* Philox4x32-10 (Salmon et al., SC'11), replacing include/Random.hpp     */
 /* ===================================================================== */
@@ -192,6 +194,35 @@ int og_graph_build(og_graph *g, uint32_t n, uint64_t m, const uint32_t *src,
     return 0;
 }
 
+int og_graph_from_csr(og_graph *g, uint32_t n, uint64_t m, const uint32_t *src, const uint32_t *dst,
+                      const uint64_t *off, const uint32_t *adj_rank, const uint32_t *adj_nbr)
+{
+    memset(g, 0, sizeof(*g));
+    g->n = n; g->m = m; g->num_edges = m;
+    g->src = (uint32_t *)malloc(4 * (m ? m : 1));
+    g->dst = (uint32_t *)malloc(4 * (m ? m : 1));
+    g->off = (uint64_t *)malloc(8 * ((size_t)n + 1));
+    g->adj_rank = (uint64_t *)malloc(8 * (m ? 2 * m : 1));
+    g->adj_src = (uint32_t *)malloc(4 * (m ? 2 * m : 1));
+    g->adj_dst = (uint32_t *)malloc(4 * (m ? 2 * m : 1));
+    memcpy(g->src, src, 4 * m);
+    memcpy(g->dst, dst, 4 * m);
+    memcpy(g->off, off, 8 * ((size_t)n + 1));
+    uint64_t maxdeg = 0;
+    #pragma omp parallel for schedule(dynamic, 4096) reduction(max:maxdeg)
+    for (int64_t v = 0; v < (int64_t)n; ++v) {
+        uint64_t d = off[v + 1] - off[v];
+        if (d > maxdeg) maxdeg = d;
+        for (uint64_t x = off[v]; x < off[v + 1]; ++x) {
+            g->adj_rank[x] = adj_rank[x];
+            g->adj_src[x] = (uint32_t)v;
+            g->adj_dst[x] = adj_nbr[x];
+        }
+    }
+    g->max_degree = maxdeg;
+    return 0;
+}
+
 /* keep-list rebuild shared by the sampler and the filter */
 static int rebuild_from_kept(const og_graph *in, og_graph *out, const uint8_t *keep, uint64_t seed)
 {
@@ -237,6 +268,148 @@ int og_graph_filter(const og_graph *in, og_graph *out, float thr, int keep_ge, u
     int rc = rebuild_from_kept(in, out, keep, seed);
     free(keep);
     return rc;
+}
+
+/* ===================================================================== */
+/* Seeded generators                                                       */
+/* ===================================================================== */
+void og_free(void *p) { free(p); }
+
+/* open-addressing set of u64 keys */
+typedef struct { uint64_t *k; uint64_t cap; } og_set;
+static int set_insert(og_set *s, uint64_t key)
+{
+    uint64_t h = key * 0x9E3779B97F4A7C15ull;
+    uint64_t i = h & (s->cap - 1);
+    while (s->k[i] != ~0ull) {
+        if (s->k[i] == key) return 0;
+        i = (i + 1) & (s->cap - 1);
+    }
+    s->k[i] = key;
+    return 1;
+}
+
+typedef uint64_t (*og_pair_fn)(og_stream *st, const void *ctx);
+
+static int gen_first_distinct(uint64_t m, uint64_t seed, uint64_t maxc, og_pair_fn fn, const void *ctx,
+                              uint32_t **src, uint32_t **dst)
+{
+    og_set set;
+    set.cap = 1;
+    while (set.cap < 2 * m + 16) set.cap <<= 1;
+    set.k = (uint64_t *)malloc(8 * set.cap);
+    memset(set.k, 0xff, 8 * set.cap);
+    uint32_t *s = (uint32_t *)malloc(4 * (m ? m : 1)), *d = (uint32_t *)malloc(4 * (m ? m : 1));
+    uint64_t got = 0;
+    for (uint64_t i = 0; i < maxc && got < m; ++i) {
+        og_stream st;
+        og_stream_init(&st, seed, OG_TAG_GEN, i);
+        uint64_t key = fn(&st, ctx);
+        if (key == ~0ull) continue;
+        if (set_insert(&set, key)) { s[got] = (uint32_t)(key >> 32); d[got] = (uint32_t)key; ++got; }
+    }
+    free(set.k);
+    if (got < m) { free(s); free(d); return -1; }
+    *src = s; *dst = d;
+    return 0;
+}
+
+static uint64_t pair_key(uint32_t u, uint32_t v)
+{
+    if (u == v) return ~0ull;
+    uint32_t a = u < v ? u : v, b = u < v ? v : u;
+    return ((uint64_t)a << 32) | b;
+}
+
+static uint64_t er_pair(og_stream *st, const void *ctx)
+{
+    uint32_t n = *(const uint32_t *)ctx;
+    uint32_t u = (uint32_t)og_uniform(st, n);
+    uint32_t v = (uint32_t)og_uniform(st, n);
+    return pair_key(u, v);
+}
+
+int og_gen_er(uint32_t n, uint64_t m, uint64_t seed, uint64_t maxc, uint32_t **src, uint32_t **dst)
+{
+    return gen_first_distinct(m, seed, maxc, er_pair, &n, src, dst);
+}
+
+typedef struct { uint32_t n; double *prob; uint32_t *alias; } cl_ctx;
+
+static uint32_t cl_draw(og_stream *st, const cl_ctx *c)
+{
+    uint32_t j = (uint32_t)og_uniform(st, c->n);
+    uint64_t hi = og_next_u32(st), lo = og_next_u32(st);
+    double coin = (double)((hi << 21) ^ (lo >> 11)) * 0x1p-53;
+    return coin < c->prob[j] ? j : c->alias[j];
+}
+
+static uint64_t cl_pair(og_stream *st, const void *ctx)
+{
+    const cl_ctx *c = (const cl_ctx *)ctx;
+    uint32_t u = cl_draw(st, c), v = cl_draw(st, c);
+    return pair_key(u, v);
+}
+
+/* Chung-Lu weights (i + i0)^(-1/(beta-1)) with i0 set by bisection so that
+ * 2m w_0 / sum w = max_deg; Vose's alias table in double precision. */
+int og_gen_chung_lu(uint32_t n, uint64_t m, double beta, double max_deg, uint64_t seed, uint64_t maxc,
+                    uint32_t **src, uint32_t **dst)
+{
+    const double alpha = 1.0 / (beta - 1.0);
+    const double target = max_deg / (2.0 * (double)m);
+    double lo = 1e-3, hi = (double)n;
+    for (int it = 0; it < 60; ++it) {
+        double mid = sqrt(lo * hi), sm = 0;
+        for (uint32_t i = 0; i < n; ++i) sm += pow(i + mid, -alpha);
+        if (pow(mid, -alpha) / sm > target) lo = mid; else hi = mid;
+    }
+    const double i0 = sqrt(lo * hi);
+    double *w = (double *)malloc(8 * (size_t)n), *prob = (double *)malloc(8 * (size_t)n);
+    uint32_t *alias = (uint32_t *)malloc(4 * (size_t)n);
+    uint32_t *small = (uint32_t *)malloc(4 * (size_t)n), *large = (uint32_t *)malloc(4 * (size_t)n);
+    uint32_t ns = 0, nl = 0;
+    double sm = 0;
+    for (uint32_t i = 0; i < n; ++i) { w[i] = pow(i + i0, -alpha); sm += w[i]; }
+    for (uint32_t i = 0; i < n; ++i) {
+        w[i] = w[i] * n / sm;
+        if (w[i] < 1.0) small[ns++] = i; else large[nl++] = i;
+    }
+    while (ns && nl) {
+        uint32_t l = small[--ns];
+        uint32_t g = large[nl - 1];
+        prob[l] = w[l]; alias[l] = g;
+        w[g] = (w[g] + w[l]) - 1.0;
+        if (w[g] < 1.0) { --nl; small[ns++] = g; }
+    }
+    for (uint32_t i = 0; i < nl; ++i) { prob[large[i]] = 1.0; alias[large[i]] = large[i]; }
+    for (uint32_t i = 0; i < ns; ++i) { prob[small[i]] = 1.0; alias[small[i]] = small[i]; }
+    cl_ctx c = {n, prob, alias};
+    int rc = gen_first_distinct(m, seed, maxc, cl_pair, &c, src, dst);
+    free(w); free(prob); free(alias); free(small); free(large);
+    return rc;
+}
+
+typedef struct { int scale; double a, ab, abc; } rmat_ctx;
+
+static uint64_t rmat_pair(og_stream *st, const void *ctx)
+{
+    const rmat_ctx *c = (const rmat_ctx *)ctx;
+    uint32_t u = 0, v = 0;
+    for (int l = 0; l < c->scale; ++l) {
+        double t = (double)og_next_u32(st) * 0x1p-32;
+        uint32_t bu = t >= c->ab, bv = (t >= c->a && t < c->ab) || t >= c->abc;
+        u = (u << 1) | bu;
+        v = (v << 1) | bv;
+    }
+    return pair_key(u, v);
+}
+
+int og_gen_rmat(int scale, uint64_t m, double a, double b, double c, uint64_t seed, uint64_t maxc,
+                uint32_t **src, uint32_t **dst)
+{
+    rmat_ctx x = {scale, a, a + b, a + b + c};
+    return gen_first_distinct(m, seed, maxc, rmat_pair, &x, src, dst);
 }
 
 /* ===================================================================== */
@@ -435,7 +608,8 @@ static int prim_close_sub(og_ctx *c, const og_sub *sampled, const og_sub *wait)
 static void set_ok(og_outcome *o, const og_ctx *c)
 {
     o->status = c->overflow ? 3 : 1;
-    o->num = c->num;
+    o->num_lo = (uint64_t)c->num;
+    o->num_hi = (uint64_t)(c->num >> 64);
     o->log2den = (uint8_t)c->log2den;
 }
 
@@ -915,7 +1089,7 @@ static void estimate_with(const og_graph *g, const og_plan *p, const og_patset *
         out[i].e0 = c.e0;
         out[i].ncond = (uint8_t)c.ncond;
         out[i].nwords = nw;
-        if (!out[i].status) { out[i].num = 0; out[i].log2den = 0; }
+        if (!out[i].status) { out[i].num_lo = out[i].num_hi = 0; out[i].log2den = 0; }
     }
 }
 
@@ -932,13 +1106,13 @@ static void acc_add(og_sums *s, const og_outcome *o)
     if (o->status != 1) return;
     s->successes++;
     int sh = s->K - o->log2den;
-    og_u128 v = o->num;
+    og_u128 v = ((og_u128)o->num_hi << 64) | o->num_lo;
     og_u128 lo = v << sh;
     uint64_t hi = sh ? (uint64_t)(v >> (128 - sh)) : 0;
     og_u128 prev = s->sum_lo;
     s->sum_lo += lo;
     s->sum_hi += hi + (s->sum_lo < prev);
-    long double x = (long double)o->num / ldexpl(1.0L, o->log2den);
+    long double x = (long double)v / ldexpl(1.0L, o->log2den);
     s->sumsq += (double)(x * x);
 }
 
@@ -1029,7 +1203,8 @@ double og_expectation(const og_graph *g, const og_plan *p, int channel, uint64_t
         if (channel < nch && o[channel].status == 1) {
             long double wgt = 1.0L;
             for (int i = 0; i < en.len; ++i) wgt /= (long double)en.n[i];
-            E += (long double)o[channel].num / ldexpl(1.0L, o[channel].log2den) * wgt;
+            og_u128 v = ((og_u128)o[channel].num_hi << 64) | o[channel].num_lo;
+            E += (long double)v / ldexpl(1.0L, o[channel].log2den) * wgt;
         }
         /* odometer */
         int j = en.len - 1;

@@ -73,6 +73,7 @@ def lib():
         L.og_graph_build.argtypes = [P(Graph), C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_uint64]
         L.og_graph_free.argtypes = [P(Graph)]
+        L.og_graph_from_csr.argtypes = [P(Graph), C.c_uint32, C.c_uint64] + [C.c_void_p] * 5
         L.og_graph_subsample.argtypes = [P(Graph), P(Graph), C.c_double, C.c_uint64]
         L.og_graph_filter.argtypes = [P(Graph), P(Graph), C.c_float, C.c_int, C.c_uint64]
         L.og_estimate.argtypes = [P(Graph), P(Plan), C.c_uint64, C.c_uint64, P(Outcome)]
@@ -114,8 +115,46 @@ def lib():
                                             C.c_double, C.c_uint64, C.c_uint64, C.c_uint64,
                                             C.c_uint64, C.c_double, C.c_double,
                                             P(C.c_longdouble)]
+        P2 = P(P(C.c_uint32))
+        L.og_gen_er.argtypes = [C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, P2, P2]
+        L.og_gen_chung_lu.argtypes = [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_uint64,
+                                      C.c_uint64, P2, P2]
+        L.og_gen_rmat.argtypes = [C.c_int, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                  C.c_uint64, P2, P2]
+        L.og_free.argtypes = [C.c_void_p]
         _lib = L
     return _lib
+
+
+def _take(ps, pd, m):
+    s = np.ctypeslib.as_array(ps, (max(m, 1),))[:m].copy()
+    d = np.ctypeslib.as_array(pd, (max(m, 1),))[:m].copy()
+    lib().og_free(ps)
+    lib().og_free(pd)
+    return s, d
+
+
+def gen_er(n, m, seed, max_candidates=None):
+    ps, pd = C.POINTER(C.c_uint32)(), C.POINTER(C.c_uint32)()
+    if lib().og_gen_er(n, m, seed, max_candidates or 64 * m + 1024, C.byref(ps), C.byref(pd)):
+        raise ValueError("not enough candidates")
+    return _take(ps, pd, m)
+
+
+def gen_chung_lu(n, m, beta, max_deg, seed, max_candidates=None):
+    ps, pd = C.POINTER(C.c_uint32)(), C.POINTER(C.c_uint32)()
+    if lib().og_gen_chung_lu(n, m, beta, max_deg, seed, max_candidates or 64 * m + 1024, C.byref(ps),
+                             C.byref(pd)):
+        raise ValueError("not enough candidates")
+    return _take(ps, pd, m)
+
+
+def gen_rmat(scale, m, a, b, c, seed, max_candidates=None):
+    ps, pd = C.POINTER(C.c_uint32)(), C.POINTER(C.c_uint32)()
+    if lib().og_gen_rmat(scale, m, a, b, c, seed, max_candidates or 64 * m + 1024, C.byref(ps),
+                         C.byref(pd)):
+        raise ValueError("not enough candidates")
+    return _take(ps, pd, m)
 
 
 def philox(ctr, key):
@@ -170,6 +209,15 @@ class OGraph:
         if g.w:
             out["w"] = np.ctypeslib.as_array(g.w, (max(m, 1),))[:m].copy()
         return out
+
+    @classmethod
+    def from_csr(cls, n, src, dst, off, adj_rank, adj_nbr):
+        o = cls(0, None, None, _raw=True)
+        arrs = [np.ascontiguousarray(a) for a in (src, dst, off, adj_rank, adj_nbr)]
+        rc = lib().og_graph_from_csr(C.byref(o.g), n, len(arrs[0]), *[a.ctypes.data for a in arrs])
+        if rc:
+            raise ValueError(rc)
+        return o
 
     def subsample(self, rate, seed):
         o = OGraph(0, None, None, _raw=True)

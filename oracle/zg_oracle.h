@@ -72,11 +72,27 @@ typedef struct {
 int og_graph_build(og_graph *g, uint32_t n, uint64_t m, const uint32_t *src,
                    const uint32_t *dst, const float *w, uint64_t shuffle_seed);
 void og_graph_free(og_graph *g);
+/* Adopts a CSR built elsewhere (copies it): src/dst by rank [m], off [n+1],
+ * per-list ranks/neighbours [2m].  Used to time the CPU path on a graph the
+ * device built; tests pin og_graph_build against the device build separately. */
+int og_graph_from_csr(og_graph *g, uint32_t n, uint64_t m, const uint32_t *src, const uint32_t *dst,
+                      const uint64_t *off, const uint32_t *adj_rank, const uint32_t *adj_nbr);
 /* GraphSampler::sample_graph (src/Graph.cpp:628-780) at W=1 */
 int og_graph_subsample(const og_graph *in, og_graph *out, double rate, uint64_t seed);
 /* GraphFilter::generate_graph (src/GraphFilters.cpp:16-177); keeps edges with
  * (w >= thr) when keep_ge != 0, else (w < thr). */
 int og_graph_filter(const og_graph *in, og_graph *out, float thr, int keep_ge, uint64_t shuffle_seed);
+
+/* ---- seeded generators (restating the library's device generators) ---- *
+ * Candidate pair i draws from stream (seed, OG_TAG_GEN, i); the output is the
+ * first m distinct non-loop pairs (min, max) in candidate order.  Caller frees
+ * *src and *dst.  Returns 0, or -1 if max_candidates were not enough. */
+int og_gen_er(uint32_t n, uint64_t m, uint64_t seed, uint64_t max_candidates, uint32_t **src, uint32_t **dst);
+int og_gen_chung_lu(uint32_t n, uint64_t m, double beta, double max_deg, uint64_t seed,
+                    uint64_t max_candidates, uint32_t **src, uint32_t **dst);
+int og_gen_rmat(int scale, uint64_t m, double a, double b, double c, uint64_t seed,
+                uint64_t max_candidates, uint32_t **src, uint32_t **dst);
+void og_free(void *p);
 
 /* ---- plans (estimate() bodies) ---------------------------------------- */
 enum {
@@ -101,7 +117,7 @@ typedef struct {
 } og_plan;
 
 typedef struct {
-    og_u128 num;               /* value = num / 2^log2den */
+    uint64_t num_lo, num_hi;   /* value = num / 2^log2den (num as two 64-bit limbs: 24-byte record) */
     uint32_t e0;               /* rank of the first sampled edge (diagnostic) */
     uint8_t status;            /* 1 = success */
     uint8_t log2den;

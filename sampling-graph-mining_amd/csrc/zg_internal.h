@@ -1,0 +1,88 @@
+// zg_internal.h -- host-side structures shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "zg_device.h"
+#include "../../include/zgraph_amd.h"
+
+namespace zg {
+
+// per-channel reduction record (block partials and final result)
+struct Acc {
+    uint64_t dig[5];   // sum of 32-bit digits of value * 2^(K - log2den)
+    uint64_t succ;
+    double sq;
+    uint64_t ovf;
+};
+static_assert(sizeof(Acc) == 64, "Acc layout");
+static_assert(sizeof(zg_outcome) == 24, "zg_outcome is a 24-byte record");
+
+struct PlanView {
+    int kind, order, pk, pe, K, nch;
+    const uint32_t *accept;   // ZG_PLAN_PATTERN: bit table over relabeled edge masks
+};
+
+}  // namespace zg
+
+struct zg_graph {
+    uint32_t n = 0;
+    uint64_t m = 0, num_edges = 0, max_degree = 0;
+    uint4 *edges = nullptr;
+    uint64_t *off = nullptr, *adj = nullptr, *sadj = nullptr;
+    float *w = nullptr;
+    uint64_t bytes = 0;
+    hipStream_t stream = nullptr;
+    zg::Acc *partials = nullptr;  // scratch for block partials
+    size_t partials_cap = 0;
+    zg::Acc *result = nullptr;    // device result (2 channels)
+    zg_outcome *outbuf = nullptr;
+    size_t outbuf_cap = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    zg::GView view() const { return zg::GView{edges, off, adj, sadj, m, n, nullptr}; }
+};
+
+struct zg_plan {
+    zg_plan_desc desc;
+    int nch = 1, K = 0;
+    double cond_calls = 0;
+    int nuniq = 0, ntotal = 0;
+    uint32_t *d_accept = nullptr;
+    zg::PlanView view() const
+    {
+        return zg::PlanView{desc.kind, desc.order, desc.pattern_vertices, desc.pattern_edges, K, nch, d_accept};
+    }
+};
+
+namespace zg {
+
+// error plumbing (thread-local message)
+int set_error(int code, const std::string &msg);
+int hip_error(hipError_t e, const char *what);
+#define ZG_HIP(call)                                        \
+    do {                                                    \
+        hipError_t _e = (call);                             \
+        if (_e != hipSuccess) return zg::hip_error(_e, #call); \
+    } while (0)
+
+int ensure_init();
+int device_cus();
+
+// graph construction from device edge arrays (zg_graph.hip)
+int build_graph(const uint32_t *d_src, const uint32_t *d_dst, const float *d_w, uint64_t m, uint32_t n,
+                uint64_t seed, zg_graph **out);
+int subsample_graph(const zg_graph *in, double rate, uint64_t seed, zg_graph **out);
+int filter_graph(const zg_graph *in, float thr, int keep_ge, uint64_t seed, zg_graph **out);
+void free_graph(zg_graph *g);
+
+// estimator launch (zg_kernels.hip): enqueues on g->stream, results in g->result
+int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uint64_t count,
+               zg_outcome *d_out, int block, int blocks_per_cu, unsigned long long *reads = nullptr);
+void finish_sums(const Acc *host_acc, int nch, int K, uint64_t count, zg_sums *out);
+
+}  // namespace zg

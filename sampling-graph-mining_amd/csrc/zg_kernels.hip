@@ -1,0 +1,668 @@
+// zg_kernels.hip -- the estimator loop of ZGraphInstance::calculate
+// (src/ZGraphInstance.cpp:256-297) as one HIP kernel per plan: one estimator
+// per lane, grid-stride over estimator ids, exact integer reduction of the
+// values (32-bit digit sums), wave -> workgroup -> grid, all deterministic.
+#include "zg_internal.h"
+
+namespace zg {
+
+// one estimator's outputs (up to two channels)
+struct Est {
+    U128 v[2];
+    uint8_t st[2];     // 0 fail, 1 success, 3 value overflowed 128 bits
+    uint8_t den[2];    // log2 of the denominator (number of c_i == 2)
+    uint32_t ncond;
+    uint32_t e0;
+};
+
+__device__ __forceinline__ void set_out(Est &o, int ch, const U128 &v, bool ov, int den)
+{
+    o.v[ch] = v;
+    o.st[ch] = ov ? 3 : 1;
+    o.den[ch] = (uint8_t)den;
+}
+
+// pick the r-th valid entry over lists V[0..nv) starting at pos[] (the
+// selection walk of core_conditional_sample_edge, src/ZGraphInstance.cpp:186-208)
+template <int MAXV>
+__device__ __forceinline__ bool cond_sample(const GView &g, Rng &rng, const VI (&V)[MAXV],
+                                            const uint32_t (&pos)[MAXV], int nv, uint64_t &s, SE &e)
+{
+    uint64_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < MAXV; ++i)
+        if (i < nv) sum += V[i].deg - pos[i];
+    s = sum;
+    if (sum == 0) return false;
+    uint64_t r = rng.uniform(sum);
+    uint64_t idx = 0;
+    uint32_t own = 0, p = 0;
+    bool done = false;
+#pragma unroll
+    for (int i = 0; i < MAXV; ++i) {
+        if (i < nv && !done) {
+            const uint64_t cnt = V[i].deg - pos[i];
+            if (r < cnt) {
+                idx = V[i].off + pos[i] + r;
+                own = V[i].v;
+                p = pos[i] + (uint32_t)r;
+                done = true;
+            } else {
+                r -= cnt;
+            }
+        }
+    }
+    const uint64_t ent = g.adj[idx];
+    count_reads(g, 1);
+    e.rank = (uint32_t)(ent >> 32);
+    e.dst = (uint32_t)ent;
+    e.src = own;
+    e.pos = p;
+    return true;
+}
+
+// position of rank r in v's rank-ordered list, for v an endpoint of edge r
+__device__ __forceinline__ uint32_t pos_of(const GView &g, uint32_t r, uint32_t v)
+{
+    const uint4 E = g.edges[r];
+    count_reads(g, 1);
+    return E.x == v ? E.z : E.w;
+}
+
+// canonicality_check (src/SamplerGenerator.cpp:286-303,
+// applications/ChainMining.cpp:23-40) over the sampled list
+template <int N>
+__device__ __forceinline__ bool canonical(const uint32_t (&sr)[N], const uint32_t (&ss)[N],
+                                          const uint32_t (&sd)[N], int ns, const SE &e)
+{
+    if (ns <= 1) return true;
+    bool found = false;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        if (i < ns) {
+            if (!found) {
+                if (e.src == ss[i] || e.src == sd[i] || e.dst == ss[i] || e.dst == sd[i]) found = true;
+            } else if (sr[i] >= e.rank) {
+                return false;
+            }
+        }
+    }
+    return true;
+}
+
+// ---- Triangle / ThreeChain / ThreeMotif ---------------------------------
+// MODE 0: triangle (applications/Triangle.cpp:48-73)
+// MODE 1: 3-chain (applications/ThreeChain.cpp:22-37)
+// MODE 2: 3-motif, channel 0 = 3-chain, channel 1 = triangle on the same
+//         wedge (the cached sub-pattern reuse of applications/ThreeMotif.cpp:48-86)
+template <int MODE>
+__device__ __forceinline__ void plan_wedge(const GView &g, Rng &rng, Est &o)
+{
+    if (g.m == 0) return;
+    const uint32_t r0 = (uint32_t)rng.uniform(g.m);
+    o.e0 = r0;
+    const uint4 E = g.edges[r0];
+    count_reads(g, 1);
+    const VI A = vinfo(g, E.x), B = vinfo(g, E.y);
+    const uint32_t pa = E.z + 1, pb = E.w + 1;   // lower_bound(r0 + 1) in both endpoint lists
+    const uint32_t ca = A.deg - pa, cb = B.deg - pb;
+    const uint64_t s = (uint64_t)ca + cb;
+    o.ncond = 1;
+    if (s == 0) return;
+    const uint64_t r = rng.uniform(s);
+    const bool inA = r < ca;
+    const uint64_t idx = inA ? A.off + pa + r : B.off + pb + (r - ca);
+    const uint64_t ent = g.adj[idx];
+    count_reads(g, 1);
+    const uint32_t r1 = (uint32_t)(ent >> 32), w = (uint32_t)ent, own = inA ? E.x : E.y;
+    const int den = (w == E.x || w == E.y) ? 1 : 0;
+    U128 val{g.m, 0};
+    const bool ov = mul_u128(val, s);
+    if (MODE != 0) set_out(o, 0, val, ov, den);
+    if (MODE == 1) return;
+    // third edge of Triangle(e0, e1) (applications/Triangle.cpp:17-40)
+    uint32_t tx, ty;
+    if (E.x == own) { tx = E.y; ty = w; }
+    else if (E.x == w) { tx = E.y; ty = own; }
+    else if (E.y == own) { tx = E.x; ty = w; }
+    else { tx = E.x; ty = own; }
+    const bool is_e0 = (tx == E.x && ty == E.y) || (tx == E.y && ty == E.x);
+    const bool is_e1 = (tx == own && ty == w) || (tx == w && ty == own);
+    bool closed;
+    if (is_e0 || is_e1) {
+        closed = true;     // nothing left to wait for
+    } else if (tx == ty) {
+        closed = false;
+    } else {
+        const VI X = tx == E.x ? A : (tx == E.y ? B : vinfo(g, tx));
+        const VI Y = ty == E.x ? A : (ty == E.y ? B : vinfo(g, ty));
+        closed = close_edge(g, X, Y, r1 + 1, nullptr);
+    }
+    if (closed) set_out(o, MODE == 2 ? 1 : 0, val, ov, den);
+}
+
+// ---- ChainMining(order) (applications/ChainMining.cpp:45-84) -------------
+__device__ __forceinline__ void plan_chain(const GView &g, const PlanView &P, Rng &rng, Est &o)
+{
+    constexpr int MAXE = 8;
+    if (g.m == 0) return;
+    const uint32_t r0 = (uint32_t)rng.uniform(g.m);
+    o.e0 = r0;
+    const uint4 E = g.edges[r0];
+    count_reads(g, 1);
+    VI ends[2] = {vinfo(g, E.x), vinfo(g, E.y)};
+    uint32_t pos[2] = {E.z + 1, E.w + 1};
+    uint32_t sr[MAXE], ss[MAXE], sd[MAXE], vs[MAXE + 1];
+    sr[0] = r0; ss[0] = E.x; sd[0] = E.y;
+    vs[0] = E.x; vs[1] = E.y;
+    int ns = 1, nv = 2;
+    U128 val{g.m, 0};
+    bool ov = false;
+    int den = 0;
+    for (int i = 0; i < P.order - 2; ++i) {
+        o.ncond++;
+        uint64_t s;
+        SE e;
+        if (!cond_sample<2>(g, rng, ends, pos, 2, s, e)) return;
+        den += (e.dst == ends[0].v || e.dst == ends[1].v);
+        ov |= mul_u128(val, s);
+        if (!canonical<MAXE>(sr, ss, sd, ns, e)) return;
+        if (ns < MAXE) { sr[ns] = e.rank; ss[ns] = e.src; sd[ns] = e.dst; }
+        ++ns;
+        bool in = false;
+#pragma unroll
+        for (int j = 0; j < MAXE + 1; ++j) in |= (j < nv && vs[j] == e.dst);
+        if (!in) { if (nv < MAXE + 1) vs[nv] = e.dst; ++nv; }
+        if (nv != i + 3) return;
+        const int k = ends[0].v == e.src ? 0 : 1;
+        const VI nvi = vinfo(g, e.dst);
+        const uint32_t np = lb_rank(g, nvi, 0, nvi.deg, r0 + 1);
+        if (k == 0) { ends[0] = nvi; pos[0] = np; } else { ends[1] = nvi; pos[1] = np; }
+    }
+    set_out(o, 0, val, ov, den);
+}
+
+// ---- FourChain (applications/FourChain.cpp:24-124) ------------------------
+__device__ __forceinline__ uint32_t fc_far(const uint4 &E, const SE &e1)
+{
+    if (E.x == e1.src) return e1.dst;
+    if (E.x == e1.dst) return e1.src;
+    if (E.y == e1.src) return e1.dst;
+    return e1.src;
+}
+
+__device__ __forceinline__ void plan_four_chain(const GView &g, Rng &rng, Est &o)
+{
+    const uint32_t method = 1 + (uint32_t)rng.uniform(3);   // sample_interger(1, 3)
+    if (g.m == 0) return;
+    const uint32_t r0 = (uint32_t)rng.uniform(g.m);
+    o.e0 = r0;
+    const uint4 E = g.edges[r0];
+    count_reads(g, 1);
+    const VI A = vinfo(g, E.x), B = vinfo(g, E.y);
+    U128 val{g.m, 0};
+    bool ov = mul_u128(val, 3);
+    int den = 0;
+    uint64_t s;
+    SE e1, e2;
+    if (method != 3) {
+        VI V[2] = {A, B};
+        uint32_t pos[2] = {E.z + 1, E.w + 1};
+        o.ncond = 1;
+        if (!cond_sample<2>(g, rng, V, pos, 2, s, e1)) return;
+        den += (e1.dst == E.x || e1.dst == E.y);
+        ov |= mul_u128(val, s);
+        const uint32_t far = fc_far(E, e1);
+        VI F[1] = {vinfo(g, far)};
+        uint32_t fp[1];
+        if (method == 1)   // edges {e0, e1}: lower_bound(r1 + 1) right after e1 in far's list
+            fp[0] = (far == e1.src ? e1.pos : pos_of(g, e1.rank, far)) + 1;
+        else               // edges {e0}: lower_bound(r0 + 1)
+            fp[0] = lb_rank(g, F[0], 0, F[0].deg, r0 + 1);
+        o.ncond = 2;
+        if (!cond_sample<1>(g, rng, F, fp, 1, s, e2)) return;
+        den += (e2.dst == far);
+        ov |= mul_u128(val, s);
+        if (e2.src == E.x || e2.src == E.y || e2.dst == E.x || e2.dst == E.y) return;
+        if (method == 2 && e2.rank >= e1.rank) return;
+    } else {
+        VI V1[1] = {A};
+        uint32_t p1[1] = {E.z + 1};
+        o.ncond = 1;
+        if (!cond_sample<1>(g, rng, V1, p1, 1, s, e1)) return;
+        den += (e1.dst == E.x);
+        ov |= mul_u128(val, s);
+        VI V2[1] = {B};
+        uint32_t p2[1] = {E.w + 1};
+        o.ncond = 2;
+        if (!cond_sample<1>(g, rng, V2, p2, 1, s, e2)) return;
+        den += (e2.dst == E.y);
+        ov |= mul_u128(val, s);
+        if (e2.src == e1.src || e2.src == e1.dst || e2.dst == e1.src || e2.dst == e1.dst) return;
+    }
+    set_out(o, 0, val, ov, den);
+}
+
+// ---- SamplerGenerator<Empty>::estimate (src/SamplerGenerator.cpp:378-464) --
+// The isomorphism loop over unique relabelings is a lookup in a bit table
+// indexed by the mask of relabeled vertex pairs: bit set iff every sampled
+// pair lies in some relabeling of the pattern (Pattern::is_automorphic).
+template <int MAXV>
+__device__ __forceinline__ void plan_pattern(const GView &g, const PlanView &P, Rng &rng, Est &o)
+{
+    constexpr int MAXE = 15;
+    if (g.m == 0) return;
+    const uint32_t r0 = (uint32_t)rng.uniform(g.m);
+    o.e0 = r0;
+    const uint4 E = g.edges[r0];
+    count_reads(g, 1);
+    VI V[MAXV];
+    uint32_t pos[MAXV];
+#pragma unroll
+    for (int i = 0; i < MAXV; ++i) { V[i] = VI{0, 0, 0}; pos[i] = 0; }
+    V[0] = vinfo(g, E.x); V[1] = vinfo(g, E.y);
+    pos[0] = E.z + 1; pos[1] = E.w + 1;
+    int nv = 2;
+    uint32_t sr[MAXE], ss[MAXE], sd[MAXE];
+    sr[0] = r0; ss[0] = E.x; sd[0] = E.y;
+    int ns = 1;
+    U128 val{g.m, 0};
+    bool ov = false;
+    int den = 0;
+    for (int i = 1; i < P.pe; ++i) {
+        o.ncond++;
+        uint64_t s;
+        SE e;
+        if (!cond_sample<MAXV>(g, rng, V, pos, nv, s, e)) return;
+        bool in = false;
+#pragma unroll
+        for (int j = 0; j < MAXV; ++j) in |= (j < nv && V[j].v == e.dst);
+        den += in;
+        ov |= mul_u128(val, s);
+        if (!canonical<MAXE>(sr, ss, sd, ns, e)) return;
+        sr[ns] = e.rank; ss[ns] = e.src; sd[ns] = e.dst;
+        ++ns;
+        if (!in) {
+            if (nv >= P.pk || nv >= MAXV) return;   // |V| would exceed k
+            const VI nvi = vinfo(g, e.dst);
+            const uint32_t np = lb_rank(g, nvi, 0, nvi.deg, r0 + 1);
+#pragma unroll
+            for (int j = 0; j < MAXV; ++j)
+                if (j == nv) { V[j] = nvi; pos[j] = np; }
+            ++nv;
+        }
+    }
+    if (nv != P.pk) return;
+    uint32_t vs[MAXV];
+#pragma unroll
+    for (int i = 0; i < MAXV; ++i) vs[i] = V[i].v;
+    for (int i = 0; i < P.pk; ++i)
+        for (int j = i + 1; j < P.pk; ++j)
+            if (vs[j] < vs[i]) { const uint32_t t = vs[j]; vs[j] = vs[i]; vs[i] = t; }
+    uint32_t mask = 0;
+    for (int i = 0; i < ns; ++i) {
+        int a = 0, b = 0;
+        for (int j = 0; j < P.pk; ++j) { if (vs[j] == ss[i]) a = j; if (vs[j] == sd[i]) b = j; }
+        const int lo = a < b ? a : b, hi = a < b ? b : a;
+        mask |= 1u << (hi * (hi - 1) / 2 + lo);
+    }
+    count_reads(g, 1);
+    if ((P.accept[mask >> 5] >> (mask & 31)) & 1u) set_out(o, 0, val, ov, den);
+}
+
+// ---- 4-clique: three spanning shapes + closing checks --------------------
+// For a clique with ranks r1 < ... < r6: shape A (r1, r2 adjacent, r3 reaches
+// the 4th vertex), shape B (r1, r2, r3 a triangle), shape C (r1, r2 disjoint).
+// Each shape has one draw sequence of probability 1/(m s1 s2), so the sum of
+// the three shape values is unbiased for every rank order.  (The reference's
+// applications/FourClique.cpp:41-76 covers shape A only.)
+__device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
+{
+    if (g.m == 0) return;
+    const uint32_t r0 = (uint32_t)rng.uniform(g.m);
+    o.e0 = r0;
+    const uint4 E = g.edges[r0];
+    count_reads(g, 1);
+    const uint32_t a = E.x, b = E.y;
+    const VI A = vinfo(g, a), B = vinfo(g, b);
+    const uint32_t pa0 = E.z + 1, pb0 = E.w + 1;
+    uint64_t s1;
+    SE e1;
+    {
+        VI V[2] = {A, B};
+        uint32_t pos[2] = {pa0, pb0};
+        o.ncond = 1;
+        if (!cond_sample<2>(g, rng, V, pos, 2, s1, e1)) return;
+    }
+    const uint32_t w = e1.dst, r1 = e1.rank;
+    if (w == a || w == b) return;
+    const bool oa = e1.src == a;
+    const uint32_t ob = oa ? b : a;
+    const VI W = vinfo(g, w);
+    const VI O = oa ? A : B, OB = oa ? B : A;
+    const uint32_t po1 = e1.pos;                 // rank r1 in o's list
+    const uint32_t pw1 = pos_of(g, r1, w);       // rank r1 in w's list
+    const uint32_t pob0 = oa ? pb0 : pa0;        // lower_bound(r0 + 1) in ob's list
+    U128 base{g.m, 0};
+    const bool base_ov = mul_u128(base, s1);
+    U128 total{0, 0};
+    bool ov = false;
+
+    // shape A: e2 above r1 from {a, b, w} to a new vertex d; close the rest above r2
+    {
+        const uint32_t pob = lb_rank(g, OB, pob0, OB.deg, r1 + 1);
+        VI V[3] = {A, B, W};
+        uint32_t pos[3] = {oa ? po1 + 1 : pob, oa ? pob : po1 + 1, pw1 + 1};
+        uint64_t s2;
+        SE e2;
+        o.ncond++;
+        if (cond_sample<3>(g, rng, V, pos, 3, s2, e2)) {
+            const uint32_t d = e2.dst;
+            if (d != a && d != b && d != w) {
+                const VI D = vinfo(g, d);
+                const uint32_t mv = e2.rank + 1;
+                bool ok = close_edge(g, OB, W, mv, nullptr);
+                if (ok && e2.src != a) ok = close_edge(g, A, D, mv, nullptr);
+                if (ok && e2.src != b) ok = close_edge(g, B, D, mv, nullptr);
+                if (ok && e2.src != w) ok = close_edge(g, W, D, mv, nullptr);
+                if (ok) {
+                    U128 v = base;
+                    const bool o2 = mul_u128(v, s2);
+                    add_u128(total, v);
+                    ov |= base_ov | o2;
+                }
+            }
+        }
+    }
+    // shape B: triangle {e0, e1, t}; e2 above rank(t) to a new vertex d; close the rest above r2
+    {
+        uint32_t rt;
+        if (close_edge(g, OB, W, r1 + 1, &rt)) {
+            const uint4 ET = g.edges[rt];
+            count_reads(g, 1);
+            const uint32_t pob_t = (ET.x == ob ? ET.z : ET.w) + 1;
+            const uint32_t pw_t = (ET.x == w ? ET.z : ET.w) + 1;
+            const uint32_t po = lb_rank(g, O, po1 + 1, O.deg, rt + 1);
+            VI V[3] = {A, B, W};
+            uint32_t pos[3] = {oa ? po : pob_t, oa ? pob_t : po, pw_t};
+            uint64_t s3;
+            SE e3;
+            o.ncond++;
+            if (cond_sample<3>(g, rng, V, pos, 3, s3, e3)) {
+                const uint32_t d = e3.dst;
+                if (d != a && d != b && d != w) {
+                    const VI D = vinfo(g, d);
+                    const uint32_t mv = e3.rank + 1;
+                    bool ok = true;
+                    if (e3.src != a) ok = close_edge(g, A, D, mv, nullptr);
+                    if (ok && e3.src != b) ok = close_edge(g, B, D, mv, nullptr);
+                    if (ok && e3.src != w) ok = close_edge(g, W, D, mv, nullptr);
+                    if (ok) {
+                        U128 v = base;
+                        const bool o3 = mul_u128(v, s3);
+                        add_u128(total, v);
+                        ov |= base_ov | o3;
+                    }
+                }
+            }
+        }
+    }
+    // shape C: e2 above r0 from w to a new vertex d, below r1; close the rest above r1
+    {
+        const uint32_t pw0 = lb_rank(g, W, 0, pw1, r0 + 1);
+        VI V[3] = {A, B, W};
+        uint32_t pos[3] = {pa0, pb0, pw0};
+        uint64_t s4;
+        SE e4;
+        o.ncond++;
+        if (cond_sample<3>(g, rng, V, pos, 3, s4, e4)) {
+            const uint32_t d = e4.dst;
+            if (e4.src == w && d != a && d != b && d != w && e4.rank < r1) {
+                const VI D = vinfo(g, d);
+                const uint32_t mv = r1 + 1;
+                if (close_edge(g, OB, W, mv, nullptr) && close_edge(g, A, D, mv, nullptr) &&
+                    close_edge(g, B, D, mv, nullptr)) {
+                    U128 v = base;
+                    const bool o4 = mul_u128(v, s4);
+                    add_u128(total, v);
+                    ov |= base_ov | o4;
+                }
+            }
+        }
+    }
+    if (total.lo | total.hi) set_out(o, 0, total, ov, 0);
+}
+
+// ---- accumulation ---------------------------------------------------------
+struct AccL {
+    uint64_t dig[5];
+    uint64_t succ, ovf;
+    double sq;
+};
+
+__device__ __forceinline__ void acc_add(AccL &a, const U128 &v, int st, int den, int K)
+{
+    if (st == 0) return;
+    if (st != 1) { a.ovf++; return; }
+    a.succ++;
+    const int sh = K - den;
+    const uint64_t w0 = v.lo << sh;
+    const uint64_t w1 = sh ? (v.hi << sh) | (v.lo >> (64 - sh)) : v.hi;
+    const uint64_t w2 = sh ? v.hi >> (64 - sh) : 0;
+    a.dig[0] += (uint32_t)w0;
+    a.dig[1] += w0 >> 32;
+    a.dig[2] += (uint32_t)w1;
+    a.dig[3] += w1 >> 32;
+    a.dig[4] += w2;
+    const double x = ((double)v.hi * 18446744073709551616.0 + (double)v.lo) * ldexp(1.0, -den);
+    a.sq += x * x;
+}
+
+__device__ __forceinline__ void wave_reduce(AccL &a)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) a.dig[i] += __shfl_xor(a.dig[i], off, 64);
+        a.succ += __shfl_xor(a.succ, off, 64);
+        a.ovf += __shfl_xor(a.ovf, off, 64);
+        a.sq += __shfl_xor(a.sq, off, 64);
+    }
+}
+
+template <int NCH>
+__device__ __forceinline__ void block_reduce_store(AccL (&a)[NCH], Acc *__restrict__ partials)
+{
+    __shared__ Acc sm[NCH][16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        wave_reduce(a[c]);
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < 5; ++i) sm[c][wv].dig[i] = a[c].dig[i];
+            sm[c][wv].succ = a[c].succ;
+            sm[c][wv].ovf = a[c].ovf;
+            sm[c][wv].sq = a[c].sq;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            Acc t = sm[c][0];
+            for (int w = 1; w < nw; ++w) {
+#pragma unroll
+                for (int i = 0; i < 5; ++i) t.dig[i] += sm[c][w].dig[i];
+                t.succ += sm[c][w].succ;
+                t.ovf += sm[c][w].ovf;
+                t.sq += sm[c][w].sq;
+            }
+            partials[blockIdx.x * NCH + c] = t;
+        }
+    }
+}
+
+template <int KIND, int NCH>
+__global__ __launch_bounds__(256) void k_run(GView g, PlanView P, uint64_t seed, uint64_t first,
+                                             uint64_t count, zg_outcome *__restrict__ out,
+                                             Acc *__restrict__ partials)
+{
+    AccL acc[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) acc[c].dig[i] = 0;
+        acc[c].succ = acc[c].ovf = 0;
+        acc[c].sq = 0;
+    }
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        Rng rng;
+        rng.init(seed, TAG_ESTIMATOR, first + i);
+        Est o;
+        o.st[0] = o.st[1] = 0;
+        o.den[0] = o.den[1] = 0;
+        o.v[0] = o.v[1] = U128{0, 0};
+        o.ncond = 0;
+        o.e0 = 0;
+        if (KIND == ZG_PLAN_TRIANGLE) plan_wedge<0>(g, rng, o);
+        else if (KIND == ZG_PLAN_THREE_CHAIN) plan_wedge<1>(g, rng, o);
+        else if (KIND == ZG_PLAN_THREE_MOTIF) plan_wedge<2>(g, rng, o);
+        else if (KIND == ZG_PLAN_CHAIN) plan_chain(g, P, rng, o);
+        else if (KIND == ZG_PLAN_FOUR_CHAIN) plan_four_chain(g, rng, o);
+        else if (KIND == ZG_PLAN_K4_CLOSE) plan_k4_close(g, rng, o);
+        else if (KIND == ZG_PLAN_PATTERN) {
+            switch (P.pk) {
+            case 2: plan_pattern<3>(g, P, rng, o); break;
+            case 3: plan_pattern<4>(g, P, rng, o); break;
+            case 4: plan_pattern<5>(g, P, rng, o); break;
+            case 5: plan_pattern<6>(g, P, rng, o); break;
+            default: plan_pattern<7>(g, P, rng, o); break;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            acc_add(acc[c], o.v[c], o.st[c], o.den[c], P.K);
+            if (out) {
+                zg_outcome rec;
+                const bool ok = o.st[c] != 0;
+                rec.num_lo = ok ? o.v[c].lo : 0;
+                rec.num_hi = ok ? o.v[c].hi : 0;
+                rec.e0 = o.e0;
+                rec.status = o.st[c];
+                rec.log2den = ok ? o.den[c] : 0;
+                rec.ncond = (uint8_t)o.ncond;
+                rec.nwords = (uint8_t)(rng.word > 255 ? 255 : rng.word);
+                out[i * NCH + c] = rec;
+            }
+        }
+    }
+    block_reduce_store<NCH>(acc, partials);
+}
+
+// deterministic grid reduction of the block partials (one workgroup)
+__global__ __launch_bounds__(256) void k_final(const Acc *__restrict__ partials, int nblocks, int nch,
+                                               Acc *__restrict__ result)
+{
+    __shared__ Acc sm[256];
+    for (int c = 0; c < nch; ++c) {
+        Acc t;
+        for (int i = 0; i < 5; ++i) t.dig[i] = 0;
+        t.succ = t.ovf = 0;
+        t.sq = 0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+            const Acc &p = partials[b * nch + c];
+            for (int i = 0; i < 5; ++i) t.dig[i] += p.dig[i];
+            t.succ += p.succ;
+            t.ovf += p.ovf;
+            t.sq += p.sq;
+        }
+        sm[threadIdx.x] = t;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+                for (int i = 0; i < 5; ++i) sm[threadIdx.x].dig[i] += sm[threadIdx.x + s].dig[i];
+                sm[threadIdx.x].succ += sm[threadIdx.x + s].succ;
+                sm[threadIdx.x].ovf += sm[threadIdx.x + s].ovf;
+                sm[threadIdx.x].sq += sm[threadIdx.x + s].sq;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) result[c] = sm[0];
+        __syncthreads();
+    }
+}
+
+template <int KIND, int NCH>
+static void launch_kind(const zg_graph *g, const PlanView &P, uint64_t seed, uint64_t first, uint64_t count,
+                        zg_outcome *d_out, int grid, int block, unsigned long long *reads)
+{
+    GView v = g->view();
+    v.reads = reads;
+    hipLaunchKernelGGL((k_run<KIND, NCH>), dim3(grid), dim3(block), 0, g->stream, v, P, seed, first, count, d_out,
+                       g->partials);
+}
+
+int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uint64_t count,
+               zg_outcome *d_out, int block, int blocks_per_cu, unsigned long long *reads)
+{
+    if (block <= 0) block = 256;
+    if (block % 64 || block > 1024 || block > 256)
+        return set_error(ZG_EINVAL, "block_size must be a multiple of 64 and at most 256");
+    if (blocks_per_cu <= 0) blocks_per_cu = 16;
+    const uint64_t want = (count + block - 1) / block;
+    uint64_t grid = (uint64_t)device_cus() * blocks_per_cu;
+    if (want < grid) grid = want;
+    if (grid == 0) grid = 1;
+    const int nch = p->nch;
+    if (grid * nch > g->partials_cap) {
+        if (g->partials) ZG_HIP(hipFree(g->partials));
+        g->partials = nullptr;
+        ZG_HIP(hipMalloc(&g->partials, sizeof(Acc) * grid * nch));
+        g->partials_cap = grid * nch;
+    }
+    const PlanView P = p->view();
+    switch (p->desc.kind) {
+    case ZG_PLAN_TRIANGLE: launch_kind<ZG_PLAN_TRIANGLE, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+    case ZG_PLAN_THREE_CHAIN: launch_kind<ZG_PLAN_THREE_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+    case ZG_PLAN_THREE_MOTIF: launch_kind<ZG_PLAN_THREE_MOTIF, 2>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+    case ZG_PLAN_CHAIN: launch_kind<ZG_PLAN_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+    case ZG_PLAN_FOUR_CHAIN: launch_kind<ZG_PLAN_FOUR_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+    case ZG_PLAN_PATTERN: launch_kind<ZG_PLAN_PATTERN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+    case ZG_PLAN_K4_CLOSE: launch_kind<ZG_PLAN_K4_CLOSE, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+    default: return set_error(ZG_EINVAL, "unknown plan kind");
+    }
+    ZG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, g->stream, g->partials, (int)grid, nch, g->result);
+    ZG_HIP(hipGetLastError());
+    return ZG_OK;
+}
+
+void finish_sums(const Acc *h, int nch, int K, uint64_t count, zg_sums *out)
+{
+    for (int c = 0; c < nch; ++c) {
+        // carry-propagate the 32-bit digit sums into 192 bits
+        uint32_t limb[6] = {0, 0, 0, 0, 0, 0};
+        unsigned __int128 carry = 0;
+        for (int i = 0; i < 6; ++i) {
+            unsigned __int128 t = carry + (i < 5 ? (unsigned __int128)h[c].dig[i] : 0);
+            limb[i] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        zg_sums &s = out[c];
+        s.sum[0] = (uint64_t)limb[0] | ((uint64_t)limb[1] << 32);
+        s.sum[1] = (uint64_t)limb[2] | ((uint64_t)limb[3] << 32);
+        s.sum[2] = (uint64_t)limb[4] | ((uint64_t)limb[5] << 32);
+        s.log2_den = K;
+        s.status = h[c].ovf ? ZG_EOVERFLOW : 0;
+        s.successes = h[c].succ;
+        s.count = count;
+        s.sumsq = h[c].sq;
+        long double v = (long double)s.sum[2] * 0x1p128L + (long double)s.sum[1] * 0x1p64L + (long double)s.sum[0];
+        s.mean = count ? (double)(v / ldexpl(1.0L, K) / (long double)count) : 0.0;
+    }
+}
+
+}  // namespace zg

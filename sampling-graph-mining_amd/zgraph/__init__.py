@@ -1,0 +1,521 @@
+"""zgraph -- Python host mirror of ZGraph's estimator API over the MI355X C ABI.
+
+Names follow the reference (jy-yuan/sampling-graph-mining):
+  ZGraphInstance.calculate        include/ZGraphInstance.hpp:125, src/ZGraphInstance.cpp:256-297
+  Estimation                      include/ZGraphInstance.hpp:18-28, src/ZGraphInstance.cpp:15-35
+  SamplerGenerator / Pattern      include/SamplerGenerator.hpp:9-80
+  ErrorProfile / TimeProfile      include/ErrorLatencyProfile.hpp:13-32
+  load_graph_meta_data            src/Utilities.cpp:24-35
+The estimate() bodies of the reference applications are device plans; each
+application class below selects its plan.  Every call goes to the HIP
+library; nothing here computes estimates on the CPU.
+"""
+import ctypes as C
+import math
+import time
+
+import numpy as np
+
+from . import _capi
+from ._capi import (PLAN_CHAIN, PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_PATTERN, PLAN_THREE_CHAIN,
+                    PLAN_THREE_MOTIF, PLAN_TRIANGLE, ZGraphError, check, lib)
+from .formats import load_biedgelist, load_edgelist, load_graph_meta_data, save_biedgelist
+
+__all__ = [
+    "Graph", "DeviceEdges", "erdos_renyi", "chung_lu", "rmat", "Plan", "Pattern", "Estimation",
+    "ZGraphInstance", "TriangleCoutingInstance", "ThreeChainCountingInstance", "ThreeMotifInstance",
+    "ChainMiningInstance", "FourChainCountingInstance", "FourCliqueCountingInstance",
+    "SamplerGenerator", "AllPredicateZGraphInstance", "ErrorProfile", "TimeProfile",
+    "load_graph_meta_data", "load_biedgelist", "load_edgelist", "save_biedgelist", "ZGraphError",
+    "device_info", "clt_half_width",
+]
+
+OUTCOME_DTYPE = np.dtype([("num_lo", "<u8"), ("num_hi", "<u8"), ("e0", "<u4"), ("status", "u1"),
+                          ("log2den", "u1"), ("ncond", "u1"), ("nwords", "u1")])
+assert OUTCOME_DTYPE.itemsize == C.sizeof(_capi.Outcome)
+
+
+def device_info():
+    cu, lds, ws = C.c_int(), C.c_int(), C.c_int()
+    check(lib().zg_device_info(C.byref(cu), C.byref(lds), C.byref(ws)))
+    return {"num_cus": cu.value, "lds_per_cu": lds.value, "warp_size": ws.value}
+
+
+# ---------------------------------------------------------------------------
+# synthetic inputs (device generators)
+# ---------------------------------------------------------------------------
+class DeviceEdges:
+    """An edge list in HBM produced by a seeded generator (src < dst)."""
+
+    def __init__(self):
+        self.e = _capi.Edges()
+
+    @property
+    def m(self):
+        return self.e.m
+
+    @property
+    def n(self):
+        return self.e.n
+
+    def add_uniform_weights(self, seed):
+        check(lib().zg_edges_add_uniform_weights(C.byref(self.e), seed))
+        return self
+
+    def download(self):
+        m = self.e.m
+        s = np.empty(m, np.uint32)
+        d = np.empty(m, np.uint32)
+        w = np.empty(m, np.float32) if self.e.d_w else None
+        check(lib().zg_edges_download(C.byref(self.e), s.ctypes.data, d.ctypes.data,
+                                      w.ctypes.data if w is not None else None))
+        return s, d, w
+
+    def free(self):
+        if self.e.d_src or self.e.d_dst:
+            lib().zg_edges_free(C.byref(self.e))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def erdos_renyi(n, m, seed):
+    out = DeviceEdges()
+    check(lib().zg_gen_erdos_renyi(n, m, seed, C.byref(out.e)))
+    return out
+
+
+def chung_lu(n, m, beta, max_degree, seed):
+    out = DeviceEdges()
+    check(lib().zg_gen_chung_lu(n, m, beta, max_degree, seed, C.byref(out.e)))
+    return out
+
+
+def rmat(scale, m, a=0.57, b=0.19, c=0.19, seed=1):
+    out = DeviceEdges()
+    check(lib().zg_gen_rmat(scale, m, a, b, c, seed, C.byref(out.e)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# graphs
+# ---------------------------------------------------------------------------
+class Graph:
+    """Device-resident CSR (the reference's Graph<E> at one worker)."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        info = _capi.GraphInfo()
+        check(lib().zg_graph_info_get(self.h, C.byref(info)))
+        self.num_vertices = info.n
+        self.num_local_edges = info.m
+        self.num_edges = info.num_edges
+        self.max_degree = info.max_degree
+        self.device_bytes = info.device_bytes
+        self.has_weights = bool(info.has_weights)
+
+    @classmethod
+    def from_edges(cls, src, dst, num_vertices, weights=None, seed=1):
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        if src.shape != dst.shape:
+            raise ValueError("src/dst length mismatch")
+        wp = None
+        if weights is not None:
+            weights = np.ascontiguousarray(weights, np.float32)
+            if weights.shape != src.shape:
+                raise ValueError("weights length mismatch")
+            wp = weights.ctypes.data
+        h = C.c_void_p()
+        check(lib().zg_graph_create(src.ctypes.data, dst.ctypes.data, wp, len(src), num_vertices, seed,
+                                    C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_device(cls, edges, seed=1):
+        h = C.c_void_p()
+        e = edges.e
+        check(lib().zg_graph_create_device(e.d_src, e.d_dst, e.d_w, e.m, e.n, seed, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load(cls, prefix, weighted=False, seed=1):
+        """`<prefix>.csv` + `<prefix>.biedgelist` (DistGraphLoader::load_graph's files)."""
+        n = load_graph_meta_data(prefix + ".csv")
+        src, dst, w = load_biedgelist(prefix + ".biedgelist", weighted=weighted)
+        return cls.from_edges(src, dst, n, weights=w, seed=seed)
+
+    def export(self):
+        m, n = self.num_local_edges, self.num_vertices
+        out = {"src": np.empty(m, np.uint32), "dst": np.empty(m, np.uint32),
+               "off": np.empty(n + 1, np.uint64), "adj_rank": np.empty(2 * m, np.uint32),
+               "adj_nbr": np.empty(2 * m, np.uint32)}
+        if self.has_weights:
+            out["w"] = np.empty(m, np.float32)
+        check(lib().zg_graph_export(self.h, out["src"].ctypes.data, out["dst"].ctypes.data,
+                                    out["w"].ctypes.data if self.has_weights else None,
+                                    out["off"].ctypes.data, out["adj_rank"].ctypes.data,
+                                    out["adj_nbr"].ctypes.data))
+        return out
+
+    def subsample(self, rate, seed):
+        """GraphSampler::sample_graph (src/Graph.cpp:628-780)."""
+        h = C.c_void_p()
+        check(lib().zg_graph_subsample(self.h, rate, seed, C.byref(h)))
+        return Graph(h)
+
+    def filter(self, threshold, keep_ge=True, seed=1):
+        """GraphFilter::generate_graph (src/GraphFilters.cpp:16-177) on `w >= thr` (or `<`)."""
+        h = C.c_void_p()
+        check(lib().zg_graph_filter(self.h, threshold, 1 if keep_ge else 0, seed, C.byref(h)))
+        return Graph(h)
+
+    def close(self):
+        if self.h:
+            lib().zg_graph_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# plans and patterns
+# ---------------------------------------------------------------------------
+class Pattern:
+    """Pattern(num_vertices, num_edges, edges) (include/SamplerGenerator.hpp:9-47)."""
+
+    def __init__(self, num_vertices, edges):
+        self.num_vertices = num_vertices
+        self.edges = [(int(a), int(b)) for a, b in edges]
+
+    @property
+    def num_edges(self):
+        return len(self.edges)
+
+
+TRIANGLE_PATTERN = Pattern(3, [(0, 1), (0, 2), (1, 2)])                              # Triangle2.cpp:18-24
+FOUR_CLIQUE_PATTERN = Pattern(4, [(1, 0), (2, 0), (2, 1), (3, 0), (3, 1), (3, 2)])    # TestPattern2.cpp:111-122
+FIVE_STAR_PATTERN = Pattern(5, [(0, 1), (0, 2), (0, 3), (0, 4)])                      # FiveStar.cpp:18-26
+FIVE_HOUSE_PATTERN = Pattern(5, [(0, 1), (0, 2), (1, 2), (1, 3), (2, 4), (3, 4)])     # FiveHouse.cpp:18-27
+
+
+class Plan:
+    def __init__(self, kind, order=0, pattern=None):
+        d = _capi.PlanDesc()
+        d.kind = kind
+        d.order = order
+        if pattern is not None:
+            if pattern.num_edges > _capi.MAX_PATTERN_EDGES:
+                raise ValueError("pattern too large")
+            d.pattern_vertices = pattern.num_vertices
+            d.pattern_edges = pattern.num_edges
+            for i, (a, b) in enumerate(pattern.edges):
+                d.pattern_src[i] = a
+                d.pattern_dst[i] = b
+        self.kind = kind
+        self.order = order
+        self.pattern = pattern
+        self.h = C.c_void_p()
+        check(lib().zg_plan_create(C.byref(d), C.byref(self.h)))
+        self.channels = lib().zg_plan_channels(self.h)
+        self.cond_calls = lib().zg_plan_cond_calls(self.h)
+
+    def unique_patterns(self):
+        u, t = C.c_int(), C.c_int()
+        check(lib().zg_plan_unique_patterns(self.h, C.byref(u), C.byref(t)))
+        return u.value, t.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().zg_plan_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+
+def run(graph, plan, seed, first_id, count, block_size=0, blocks_per_cu=0, timed=False):
+    """One launch of the estimator loop; returns the per-channel sums (and kernel ms)."""
+    sums = (_capi.Sums * plan.channels)()
+    launch = _capi.Launch(block_size, blocks_per_cu)
+    ms = C.c_float(0)
+    check(lib().zg_run(graph.h, plan.h, seed, first_id, count, C.byref(launch), sums,
+                       C.byref(ms) if timed else None))
+    out = [sums[i] for i in range(plan.channels)]
+    return (out, ms.value) if timed else out
+
+
+def run_outcomes(graph, plan, seed, first_id, count):
+    """Per-estimator records (count x channels) for parity checks."""
+    out = np.zeros(count * plan.channels, OUTCOME_DTYPE)
+    sums = (_capi.Sums * plan.channels)()
+    check(lib().zg_run_outcomes(graph.h, plan.h, seed, first_id, count, out.ctypes.data, sums))
+    return (out.reshape(count, plan.channels) if plan.channels > 1 else out), [sums[i] for i in
+                                                                               range(plan.channels)]
+
+
+def count_reads(graph, plan, seed, first_id, count):
+    """Dependent global reads (64-byte sectors) issued by these estimators."""
+    r = C.c_uint64()
+    check(lib().zg_count_reads(graph.h, plan.h, seed, first_id, count, C.byref(r)))
+    return r.value
+
+
+def clt_half_width(mean, sumsq, count, confidence=0.95):
+    """CLT half-width of the mean at the given two-sided confidence."""
+    if count < 2:
+        return float("inf")
+    var = max(sumsq / count - mean * mean, 0.0) * count / (count - 1)
+    z = _z(confidence)
+    return z * math.sqrt(var / count)
+
+
+def _z(conf):
+    # inverse normal CDF by bisection on erf (no scipy dependency)
+    p = 0.5 + conf / 2
+    lo, hi = 0.0, 10.0
+    for _ in range(100):
+        mid = (lo + hi) / 2
+        if 0.5 * (1 + math.erf(mid / math.sqrt(2))) < p:
+            lo = mid
+        else:
+            hi = mid
+    return (lo + hi) / 2
+
+
+# ---------------------------------------------------------------------------
+# Estimation / ZGraphInstance
+# ---------------------------------------------------------------------------
+class Estimation:
+    """Running estimator-weighted mean (src/ZGraphInstance.cpp:15-35), plus the
+    exact integer sum and the sum of squares that the device reduction returns."""
+
+    def __init__(self):
+        self.reset()
+
+    def reset(self):
+        self.num_finished_estimators = 0
+        self.current_estimation = 0.0
+        self.exact_sum = 0        # sum of values * 2^log2_den
+        self.log2_den = None
+        self.sumsq = 0.0
+        self.successes = 0
+
+    def update(self, new_num_estimators, new_estimation):
+        n0, n1 = self.num_finished_estimators, new_num_estimators
+        self.current_estimation = (1.0 * n0 / (n0 + n1)) * self.current_estimation + \
+                                  (1.0 * n1 / (n0 + n1)) * new_estimation
+        self.num_finished_estimators += n1
+
+    def get_num_finished_estimators(self):
+        return self.num_finished_estimators
+
+    def get_currnet_estimation(self):   # spelling kept from include/ZGraphInstance.hpp:27
+        return self.current_estimation
+
+    get_current_estimation = get_currnet_estimation
+
+    def absorb(self, sums):
+        if self.log2_den is None:
+            self.log2_den = sums.log2_den
+        self.exact_sum += sums.total()
+        self.sumsq += sums.sumsq
+        self.successes += sums.successes
+
+    def exact_mean(self):
+        if not self.num_finished_estimators:
+            return 0.0
+        return self.exact_sum / (2 ** self.log2_den) / self.num_finished_estimators
+
+    def half_width(self, confidence=0.95):
+        return clt_half_width(self.exact_mean(), self.sumsq, self.num_finished_estimators, confidence)
+
+
+class ZGraphInstance:
+    """Base estimator instance (include/ZGraphInstance.hpp:107-140).
+
+    Estimator ids are global: calculate(N, k, init) runs ids
+    [init.finished, N), so a top-up equals one run of N estimators."""
+
+    plan_kind = None
+
+    def __init__(self, graph, plan=None, seed=1):
+        self.graph = graph
+        self.plan = plan if plan is not None else Plan(self.plan_kind)
+        self.seed = seed
+        self.channel = 0
+
+    def get_graph(self):
+        return self.graph
+
+    def set_graph(self, graph):
+        self.graph = graph
+
+    def calculate(self, num_estimators, num_vertices_in_pattern=None, init_estimation=None):
+        est = init_estimation if init_estimation is not None else Estimation()
+        done = est.get_num_finished_estimators()
+        if done >= num_estimators:
+            return est.get_currnet_estimation()
+        new = num_estimators - done
+        sums = run(self.graph, self.plan, self.seed, done, new)[self.channel]
+        est.absorb(sums)
+        est.update(new, sums.mean)   # W = 1: get_scaling_factor == 1 (src/ZGraphInstance.cpp:117-125)
+        return est.get_currnet_estimation()
+
+    def outcomes(self, first_id, count):
+        return run_outcomes(self.graph, self.plan, self.seed, first_id, count)[0]
+
+    def get_num_sample_edge_calls(self):
+        return 1.0   # src/ZGraphInstance.cpp:452
+
+    def get_num_conditional_sample_edge_calls(self):
+        return self.plan.cond_calls
+
+
+class TriangleCoutingInstance(ZGraphInstance):
+    """applications/Triangle.cpp:42-74 (class name spelled as the reference)."""
+    plan_kind = PLAN_TRIANGLE
+
+
+TriangleCountingInstance = TriangleCoutingInstance
+
+
+class ThreeChainCountingInstance(ZGraphInstance):
+    """applications/ThreeChain.cpp:16-38"""
+    plan_kind = PLAN_THREE_CHAIN
+
+
+class ThreeMotifInstance(ZGraphInstance):
+    """applications/ThreeMotif.cpp: 3-chain (channel 0) and triangle (channel 1)
+    from shared wedges, the device analogue of CachedSubPatterns."""
+    plan_kind = PLAN_THREE_MOTIF
+
+    def calculate_both(self, n_chain, n_triangle):
+        n = max(n_chain, n_triangle)
+        ec, et = Estimation(), Estimation()
+        if n_chain == n_triangle:
+            s = run(self.graph, self.plan, self.seed, 0, n)
+            ec.absorb(s[0]); ec.update(n, s[0].mean)
+            et.absorb(s[1]); et.update(n, s[1].mean)
+        else:
+            lo = min(n_chain, n_triangle)
+            s = run(self.graph, self.plan, self.seed, 0, lo)
+            s2 = run(self.graph, self.plan, self.seed, lo, n - lo)
+            for ch, e, cnt in ((0, ec, n_chain), (1, et, n_triangle)):
+                e.absorb(s[ch]); e.update(lo, s[ch].mean)
+                if cnt > lo:
+                    e.absorb(s2[ch]); e.update(n - lo, s2[ch].mean)
+        return ec, et
+
+
+class ChainMiningInstance(ZGraphInstance):
+    """applications/ChainMining.cpp:18-85 (order = vertices in the path)."""
+
+    def __init__(self, graph, order=4, seed=1):
+        super().__init__(graph, Plan(PLAN_CHAIN, order=order), seed)
+        self.order = order
+
+
+class FourChainCountingInstance(ZGraphInstance):
+    """applications/FourChain.cpp:18-125"""
+    plan_kind = PLAN_FOUR_CHAIN
+
+
+class FourCliqueCountingInstance(ZGraphInstance):
+    """4-cliques by three spanning shapes + closing checks (unbiased; the
+    reference's applications/FourClique.cpp:17-77 misses two of the shapes)."""
+    plan_kind = PLAN_K4_CLOSE
+
+
+class SamplerGenerator(ZGraphInstance):
+    """SamplerGenerator<Empty>(graph, pattern) (src/SamplerGenerator.cpp:311-476)."""
+
+    def __init__(self, graph, pattern, seed=1):
+        super().__init__(graph, Plan(PLAN_PATTERN, pattern=pattern), seed)
+        self.pattern = pattern
+
+    def get_num_conditional_sample_edge_calls(self):
+        return self.pattern.num_edges - 1   # src/SamplerGenerator.cpp:471-474
+
+
+class AllPredicateZGraphInstance(ZGraphInstance):
+    """AllPredicateZGraphInstance (src/ZGraphInstance.cpp:490-525): runs the plan
+    on the sub-graph of edges whose weight satisfies the predicate."""
+
+    def __init__(self, graph, threshold, keep_ge=True, plan_kind=PLAN_TRIANGLE, seed=1, filter_seed=1):
+        self.original_graph = graph
+        filtered = graph.filter(threshold, keep_ge, filter_seed)
+        super().__init__(filtered, Plan(plan_kind), seed)
+
+    def get_graph(self):
+        return self.original_graph
+
+
+# ---------------------------------------------------------------------------
+# error-latency profiles
+# ---------------------------------------------------------------------------
+class ErrorProfile:
+    """ErrorProfile<E> (src/ErrorLatencyProfile.cpp:49-207) on the device."""
+
+    def __init__(self, pilot_scale=0):
+        self.pilot_scale = pilot_scale
+        self.last = None
+
+    @staticmethod
+    def calculate_bound(epsilon, delta, num_sample_edge_calls, num_conditional_sample_edge_calls,
+                        estimation, max_degree, num_edges):
+        return lib().zg_elp_bound(epsilon, delta, num_sample_edge_calls,
+                                  num_conditional_sample_edge_calls, estimation, max_degree, num_edges)
+
+    def get_num_estimators(self, instance, num_vertices_in_pattern, required_error_rate,
+                           required_confidence):
+        r = _capi.ElpResult()
+        check(lib().zg_elp_regression(instance.graph.h, instance.plan.h, required_error_rate,
+                                      required_confidence, instance.seed, self.pilot_scale, C.byref(r)))
+        self.last = r
+        return int(r.num_estimators) if math.isfinite(r.num_estimators) else 0
+
+    def run(self, instance, num_vertices_in_pattern, required_error_rate, required_confidence,
+            init_estimation=None):
+        n = self.get_num_estimators(instance, num_vertices_in_pattern, required_error_rate,
+                                    required_confidence)
+        return instance.calculate(n, num_vertices_in_pattern, init_estimation)
+
+
+class TimeProfile:
+    """TimeProfile<E>::run (src/ErrorLatencyProfile.cpp:24-47): doubling batches
+    while used + 2 * last <= limit, then one batch sized to the rest.  Batches
+    take disjoint estimator-id ranges so they stay independent."""
+
+    def __init__(self, basic_estimators=1 << 16):
+        self.basic = basic_estimators
+
+    def run(self, instance, num_vertices_in_pattern, time_limit_ms):
+        used = 0
+        cur = 0
+        nxt = self.basic
+        first = 0
+        total_n, total_sum = 0, 0.0
+        while used + cur * 2 <= time_limit_ms:
+            t = time.perf_counter()
+            s = run(instance.graph, instance.plan, instance.seed, first, nxt)[instance.channel]
+            cur = int((time.perf_counter() - t) * 1000)
+            used += cur
+            total_sum += s.mean * nxt
+            total_n += nxt
+            first += nxt
+            nxt <<= 1
+        if cur > 0 and time_limit_ms > used:
+            rest = nxt * (time_limit_ms - used) // (cur * 2)
+            if rest > 0:
+                s = run(instance.graph, instance.plan, instance.seed, first, rest)[instance.channel]
+                total_sum += s.mean * rest
+                total_n += rest
+        return total_sum / total_n if total_n else 0.0

@@ -1,0 +1,154 @@
+"""ctypes declarations of the C ABI in include/zgraph_amd.h.
+
+The shared library is loaded from this directory (built in-tree by the
+package Makefile).  There is no CPU fallback: every entry point raises when
+the library or a GPU is missing.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzgraph_amd.so")
+
+ZG_OK = 0
+ZG_EINVAL = -1
+ZG_EHIP = -2
+ZG_ENOMEM = -3
+ZG_EOVERFLOW = -4
+ZG_ESTATE = -5
+
+PLAN_TRIANGLE = 1
+PLAN_THREE_CHAIN = 2
+PLAN_THREE_MOTIF = 3
+PLAN_CHAIN = 4
+PLAN_FOUR_CHAIN = 5
+PLAN_PATTERN = 6
+PLAN_K4_CLOSE = 7
+
+MAX_PATTERN_EDGES = 15
+
+# every symbol include/zgraph_amd.h declares
+EXPORTED = [
+    "zg_last_error", "zg_init", "zg_device_info",
+    "zg_graph_create", "zg_graph_create_device", "zg_graph_destroy", "zg_graph_info_get",
+    "zg_graph_export", "zg_graph_subsample", "zg_graph_filter",
+    "zg_gen_erdos_renyi", "zg_gen_chung_lu", "zg_gen_rmat", "zg_edges_add_uniform_weights",
+    "zg_edges_download", "zg_edges_free",
+    "zg_plan_create", "zg_plan_destroy", "zg_plan_channels", "zg_plan_cond_calls",
+    "zg_plan_unique_patterns",
+    "zg_run", "zg_run_outcomes", "zg_count_reads",
+    "zg_elp_regression", "zg_elp_bound", "zg_regression_k", "zg_elp_from_pilots",
+]
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("m", C.c_uint64), ("num_edges", C.c_uint64),
+                ("max_degree", C.c_uint64), ("device_bytes", C.c_uint64), ("has_weights", C.c_int)]
+
+
+class Edges(C.Structure):
+    _fields_ = [("d_src", C.c_void_p), ("d_dst", C.c_void_p), ("d_w", C.c_void_p),
+                ("m", C.c_uint64), ("n", C.c_uint32)]
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("order", C.c_int), ("pattern_vertices", C.c_int),
+                ("pattern_edges", C.c_int),
+                ("pattern_src", C.c_uint32 * MAX_PATTERN_EDGES),
+                ("pattern_dst", C.c_uint32 * MAX_PATTERN_EDGES)]
+
+
+class Sums(C.Structure):
+    _fields_ = [("sum", C.c_uint64 * 3), ("log2_den", C.c_int32), ("status", C.c_int32),
+                ("successes", C.c_uint64), ("count", C.c_uint64), ("sumsq", C.c_double),
+                ("mean", C.c_double)]
+
+    def total(self):
+        """exact sum of values times 2^log2_den (Python int)"""
+        return self.sum[0] | (self.sum[1] << 64) | (self.sum[2] << 128)
+
+
+class Outcome(C.Structure):
+    _fields_ = [("num_lo", C.c_uint64), ("num_hi", C.c_uint64), ("e0", C.c_uint32),
+                ("status", C.c_uint8), ("log2den", C.c_uint8), ("ncond", C.c_uint8),
+                ("nwords", C.c_uint8)]
+
+
+class Launch(C.Structure):
+    _fields_ = [("block_size", C.c_int), ("blocks_per_cu", C.c_int)]
+
+
+class ElpResult(C.Structure):
+    _fields_ = [("num_estimators", C.c_double), ("K", C.c_double), ("rough_estimate", C.c_double),
+                ("sampled_estimate", C.c_double), ("sampled_m", C.c_uint64),
+                ("sampled_max_degree", C.c_uint64), ("elapsed_ms", C.c_double)]
+
+
+_lib = None
+
+
+class ZGraphError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"zgraph error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libzgraph_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C sampling-graph-mining_amd`")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    L.zg_last_error.argtypes = [C.c_char_p, C.c_size_t]
+    L.zg_init.argtypes = []
+    L.zg_device_info.argtypes = [P(C.c_int), P(C.c_int), P(C.c_int)]
+    L.zg_graph_create.argtypes = [vp, vp, vp, C.c_uint64, C.c_uint32, C.c_uint64, P(vp)]
+    L.zg_graph_create_device.argtypes = [vp, vp, vp, C.c_uint64, C.c_uint32, C.c_uint64, P(vp)]
+    L.zg_graph_destroy.argtypes = [vp]
+    L.zg_graph_info_get.argtypes = [vp, P(GraphInfo)]
+    L.zg_graph_export.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.zg_graph_subsample.argtypes = [vp, C.c_double, C.c_uint64, P(vp)]
+    L.zg_graph_filter.argtypes = [vp, C.c_float, C.c_int, C.c_uint64, P(vp)]
+    L.zg_gen_erdos_renyi.argtypes = [C.c_uint32, C.c_uint64, C.c_uint64, P(Edges)]
+    L.zg_gen_chung_lu.argtypes = [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_uint64, P(Edges)]
+    L.zg_gen_rmat.argtypes = [C.c_int, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                              P(Edges)]
+    L.zg_edges_add_uniform_weights.argtypes = [P(Edges), C.c_uint64]
+    L.zg_edges_download.argtypes = [P(Edges), vp, vp, vp]
+    L.zg_edges_free.argtypes = [P(Edges)]
+    L.zg_plan_create.argtypes = [P(PlanDesc), P(vp)]
+    L.zg_plan_destroy.argtypes = [vp]
+    L.zg_plan_channels.argtypes = [vp]
+    L.zg_plan_cond_calls.argtypes = [vp]
+    L.zg_plan_cond_calls.restype = C.c_double
+    L.zg_plan_unique_patterns.argtypes = [vp, P(C.c_int), P(C.c_int)]
+    L.zg_run.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, P(Launch), P(Sums), P(C.c_float)]
+    L.zg_run_outcomes.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, P(Sums)]
+    L.zg_count_reads.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_uint64)]
+    L.zg_elp_regression.argtypes = [vp, vp, C.c_double, C.c_double, C.c_uint64, C.c_int, P(ElpResult)]
+    L.zg_elp_bound.argtypes = [C.c_double] * 5 + [C.c_uint64, C.c_uint64]
+    L.zg_elp_bound.restype = C.c_double
+    L.zg_regression_k.argtypes = [vp, vp, C.c_uint]
+    L.zg_regression_k.restype = C.c_longdouble
+    L.zg_elp_from_pilots.argtypes = [vp, C.c_uint64, C.c_longdouble, C.c_double, C.c_double,
+                                     C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double,
+                                     C.c_double, P(C.c_longdouble)]
+    L.zg_elp_from_pilots.restype = C.c_double
+    _lib = L
+    return L
+
+
+def last_error():
+    buf = C.create_string_buffer(1024)
+    lib().zg_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc):
+    if rc != ZG_OK:
+        raise ZGraphError(rc, last_error())
+    return rc

@@ -1,0 +1,84 @@
+"""Shared fixtures for the tests: golden graphs, seeded small graphs, and the
+mapping between oracle plans and device plans."""
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+_graphs = None
+
+
+def golden_graph(name):
+    global _graphs
+    if _graphs is None:
+        _graphs = np.load(os.path.join(GOLDEN, "graphs.npz"))
+    n = int(_graphs[name + "/n"])
+    return n, _graphs[name + "/src"], _graphs[name + "/dst"]
+
+
+def golden_weights(name):
+    golden_graph(name)
+    return _graphs[name + "/w"]
+
+
+def random_graph(n, p, seed, flip=True):
+    """G(n, p) edge list with random orientation (simple graph)."""
+    rng = np.random.default_rng(seed)
+    iu = np.triu_indices(n, 1)
+    keep = rng.random(len(iu[0])) < p
+    s, d = iu[0][keep].astype(np.uint32), iu[1][keep].astype(np.uint32)
+    if flip:
+        f = rng.random(len(s)) < 0.5
+        s, d = np.where(f, d, s), np.where(f, s, d)
+    perm = rng.permutation(len(s))
+    return n, s[perm].copy(), d[perm].copy()
+
+
+def complete_graph(n):
+    iu = np.triu_indices(n, 1)
+    return n, iu[0].astype(np.uint32), iu[1].astype(np.uint32)
+
+
+def power_law_graph(n, m, beta, seed):
+    """Small Chung-Lu-like multigraph-free edge list for CPU tests (numpy)."""
+    rng = np.random.default_rng(seed)
+    w = (np.arange(n) + 1.0) ** (-1.0 / (beta - 1.0))
+    w /= w.sum()
+    pairs = set()
+    while len(pairs) < m:
+        u = rng.choice(n, size=2 * m, p=w)
+        v = rng.choice(n, size=2 * m, p=w)
+        for a, b in zip(u, v):
+            if a != b:
+                pairs.add((min(a, b), max(a, b)))
+                if len(pairs) >= m:
+                    break
+    e = np.array(sorted(pairs), dtype=np.uint32)
+    perm = rng.permutation(len(e))
+    return n, e[perm, 0].copy(), e[perm, 1].copy()
+
+
+# (name, oracle plan args, device plan args)
+PLANS = [
+    ("triangle", dict(kind=1), dict(kind=1)),
+    ("three_chain", dict(kind=2), dict(kind=2)),
+    ("three_motif", dict(kind=3), dict(kind=3)),
+    ("chain4", dict(kind=4, order=4), dict(kind=4, order=4)),
+    ("chain5", dict(kind=4, order=5), dict(kind=4, order=5)),
+    ("four_chain", dict(kind=5), dict(kind=5)),
+    ("k4_close", dict(kind=7), dict(kind=7)),
+    ("pattern_triangle", dict(kind=6, pattern="TRIANGLE"), dict(kind=6, pattern="TRIANGLE")),
+    ("pattern_k4", dict(kind=6, pattern="K4"), dict(kind=6, pattern="K4")),
+    ("pattern_5star", dict(kind=6, pattern="FIVE_STAR"), dict(kind=6, pattern="FIVE_STAR")),
+    ("pattern_house", dict(kind=6, pattern="FIVE_HOUSE"), dict(kind=6, pattern="FIVE_HOUSE")),
+]
+
+
+def oracle_plan(O, kind, order=0, pattern=None):
+    return O.make_plan(kind, order=order, pattern=getattr(O, pattern) if pattern else None)
+
+
+def device_plan(Z, kind, order=0, pattern=None):
+    pats = {"TRIANGLE": Z.TRIANGLE_PATTERN, "K4": Z.FOUR_CLIQUE_PATTERN, "FIVE_STAR": Z.FIVE_STAR_PATTERN,
+            "FIVE_HOUSE": Z.FIVE_HOUSE_PATTERN}
+    return Z.Plan(kind, order=order, pattern=pats[pattern] if pattern else None)

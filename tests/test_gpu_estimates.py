@@ -1,0 +1,154 @@
+"""Statistical and API-level checks on the device: estimates fall inside the
+CLT interval of the exact counts, top-ups equal one run, the fused 3-motif
+equals its two halves, predicate triangles follow the filter, and the device
+ELP equals the oracle pipeline run on the same subsampled graph."""
+import ctypes as C
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+import pyoracle as O
+import zgraph as Z
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    say("[cfg1] Erdos-Renyi n=10000 m=100000 seed=1")
+    e = Z.erdos_renyi(10000, 100000, seed=1)
+    s, d, _ = e.download()
+    g = Z.Graph.from_device(e, seed=1)
+    og = O.OGraph(10000, s, d, seed=1)
+    return g, og
+
+
+def within(est, exact, z=4.0):
+    hw = est.half_width(0.95) / 1.959964 * z
+    return abs(est.exact_mean() - exact) <= hw
+
+
+def test_cfg1_triangle_within_interval(cfg1):
+    g, og = cfg1
+    T = og.triangles()
+    inst = Z.TriangleCoutingInstance(g, seed=1)
+    est = Z.Estimation()
+    val = inst.calculate(1_000_000, 3, est)
+    say(f"[cfg1] exact={T} estimate={val:.1f} half-width95={est.half_width():.1f}")
+    assert within(est, T)
+    assert val == pytest.approx(est.exact_mean(), rel=1e-12)
+
+
+def test_top_up_equals_single_run(cfg1):
+    g, _ = cfg1
+    inst = Z.FourCliqueCountingInstance(g, seed=9)
+    a = Z.Estimation()
+    inst.calculate(300_000, 4, a)
+    inst.calculate(1_000_000, 4, a)    # runs ids [300000, 1000000) only
+    b = Z.Estimation()
+    inst.calculate(1_000_000, 4, b)
+    assert a.exact_sum == b.exact_sum and a.num_finished_estimators == b.num_finished_estimators
+    assert a.get_currnet_estimation() == pytest.approx(b.get_currnet_estimation(), rel=1e-12)
+
+
+def test_three_motif_is_both_halves():
+    n, s, d = H.golden_graph("wiki-vote")
+    g = Z.Graph.from_edges(s, d, n, seed=2)
+    m = Z.ThreeMotifInstance(g, seed=4)
+    ec, et = m.calculate_both(500_000, 500_000)
+    c = Z.ThreeChainCountingInstance(g, seed=4)
+    t = Z.TriangleCoutingInstance(g, seed=4)
+    e1, e2 = Z.Estimation(), Z.Estimation()
+    c.calculate(500_000, 3, e1)
+    t.calculate(500_000, 3, e2)
+    assert ec.exact_sum == e1.exact_sum and et.exact_sum == e2.exact_sum
+
+
+@pytest.mark.parametrize("gname,name,exact_fn,n_est", [
+    ("wiki-vote", "three_chain", lambda og: og.wedges(), 400_000),
+    ("wiki-vote", "chain4", lambda og: og.paths3(), 2_000_000),
+    ("wiki-vote", "four_chain", lambda og: og.paths3(), 2_000_000),
+    ("wiki-vote", "k4_close", lambda og: og.k4(), 16_000_000),
+    # SamplerGenerator patterns need E-1 lucky conditional samples (K4: about one
+    # success per 10^9 estimators on wiki-vote), so they are checked on a dense graph
+    ("dense-14", "pattern_k4", lambda og: og.k4(), 64_000_000),
+    ("dense-14", "pattern_house", lambda og: og.house_brute(), 8_000_000),
+])
+def test_estimates_within_interval(gname, name, exact_fn, n_est):
+    n, s, d = H.golden_graph("wiki-vote") if gname == "wiki-vote" else H.random_graph(14, 0.8, 12)
+    g = Z.Graph.from_edges(s, d, n, seed=3)
+    og = O.OGraph(n, s, d, seed=3)
+    oargs = dict(next(p[2] for p in H.PLANS if p[0] == name))
+    plan = H.device_plan(Z, **oargs)
+    inst = Z.ZGraphInstance(g, plan, seed=21)
+    est = Z.Estimation()
+    inst.calculate(n_est, 4, est)
+    exact = exact_fn(og)
+    say(f"[{name}] exact={exact} est={est.exact_mean():.1f} hw95={est.half_width():.1f}")
+    assert within(est, exact)
+
+
+def test_predicate_triangles():
+    # AllPredicateWeightedTriangle / AtLeastOnePredicateWeightedTriangle on
+    # float weights U[0,1) with threshold 0.5
+    n, s, d = H.power_law_graph(3000, 20000, 2.1, 7)
+    w = np.random.default_rng(1).random(len(s)).astype(np.float32)
+    g = Z.Graph.from_edges(s, d, n, weights=w, seed=2)
+    og = O.OGraph(n, s, d, w=w, seed=2)
+    all_inst = Z.AllPredicateZGraphInstance(g, 0.5, keep_ge=True, seed=3, filter_seed=5)
+    fail_inst = Z.AllPredicateZGraphInstance(g, 0.5, keep_ge=False, seed=3, filter_seed=6)
+    tot_inst = Z.TriangleCoutingInstance(g, seed=3)
+    ea, ef, et = Z.Estimation(), Z.Estimation(), Z.Estimation()
+    all_inst.calculate(2_000_000, 3, ea)
+    fail_inst.calculate(2_000_000, 3, ef)
+    tot_inst.calculate(2_000_000, 3, et)
+    exact_all = og.filter(0.5, True, 5).triangles()
+    exact_fail = og.filter(0.5, False, 6).triangles()
+    exact_tot = og.triangles()
+    assert within(ea, exact_all)
+    assert within(ef, exact_fail)
+    at_least_one = et.exact_mean() - ef.exact_mean()
+    hw = 4 / 1.96 * math.hypot(et.half_width(), ef.half_width())
+    assert abs(at_least_one - (exact_tot - exact_fail)) <= hw
+
+
+def _ld(arr):
+    return (C.c_longdouble * len(arr))(*arr)
+
+
+def test_device_elp_equals_oracle_pipeline():
+    n, s, d = H.golden_graph("wiki-vote")
+    g = Z.Graph.from_edges(s, d, n, seed=3)
+    og = O.OGraph(n, s, d, seed=3)
+    inst = Z.TriangleCoutingInstance(g, seed=8)
+    prof = Z.ErrorProfile(pilot_scale=4)
+    N = prof.get_num_estimators(inst, 3, 0.05, 0.95)
+    r = prof.last
+    # oracle pipeline on the same subsample (GraphSampler with the same seed)
+    osg = og.subsample(0.1, 8)
+    assert r.sampled_m == osg.m and r.sampled_max_degree == osg.g.max_degree
+    plan = O.make_plan(O.PLAN_TRIANGLE)
+    npts, per, post = (1 << 14) >> 4, 16, (1 << 23) >> 4
+    oo = O.run_outcomes(osg, plan, 8, 0, npts * per)
+    vals = [(int(x["num_lo"]) | (int(x["num_hi"]) << 64)) << (1 - int(x["log2den"])) if x["status"] == 1 else 0
+            for x in oo]
+    acc, est = 0, []
+    for i in range(npts):
+        acc += sum(vals[i * per:(i + 1) * per])
+        est.append(acc / 2 / ((i + 1) * per))
+    ps = O.run(osg, plan, 8, npts * per, post)[0]
+    F = (acc + ps.total()) / 2 / (npts * per + post)
+    assert r.sampled_estimate == pytest.approx(F, rel=1e-12)
+    want = O.lib().og_elp_num_estimators(_ld(est), npts, F, 1.0, 1.0, osg.g.max_degree, osg.m,
+                                         og.g.max_degree, og.m, 0.05, 0.95, None)
+    say(f"[elp] N={r.num_estimators:.1f} oracle={want:.1f} K={r.K:.4f}")
+    assert r.num_estimators == pytest.approx(want, rel=1e-9)
+    assert N == int(r.num_estimators)

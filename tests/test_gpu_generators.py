@@ -1,0 +1,44 @@
+"""Device generators equal the oracle's restatement element by element, and
+have the properties the workloads need (no loops, no duplicates, degrees)."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+import zgraph as Z
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(dev, ref_s, ref_d):
+    s, d, _ = dev.download()
+    assert (s == ref_s).all() and (d == ref_d).all()
+    assert (s < d).all()
+    key = s.astype(np.uint64) << np.uint64(32) | d
+    assert len(np.unique(key)) == len(key)
+    return s, d
+
+
+def test_erdos_renyi_equals_oracle():
+    ref = O.gen_er(10000, 100000, 1)
+    _check(Z.erdos_renyi(10000, 100000, seed=1), *ref)
+
+
+def test_chung_lu_equals_oracle():
+    ref = O.gen_chung_lu(200000, 2_000_000, 2.1, 4000.0, 3)
+    s, d = _check(Z.chung_lu(200000, 2_000_000, 2.1, 4000.0, 3), *ref)
+    deg = np.bincount(np.concatenate([s, d]))
+    assert 0.7 * 4000 < deg.max() <= 1.1 * 4000
+    assert deg.sum() == 4_000_000
+
+
+def test_rmat_equals_oracle():
+    ref = O.gen_rmat(16, 16 << 16, 0.57, 0.19, 0.19, 5)
+    s, d = _check(Z.rmat(16, 16 << 16, 0.57, 0.19, 0.19, seed=5), *ref)
+    assert d.max() < (1 << 16)
+
+
+def test_weights_uniform():
+    e = Z.erdos_renyi(1000, 20000, seed=2).add_uniform_weights(7)
+    _, _, w = e.download()
+    assert w.dtype == np.float32 and (w >= 0).all() and (w < 1).all()
+    assert abs(w.mean() - 0.5) < 0.01

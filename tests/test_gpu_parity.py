@@ -1,0 +1,130 @@
+"""Device vs oracle parity through the C ABI: the device CSR equals the
+oracle's CSR array for array, and every estimator's outcome (status, exact
+numerator, denominator exponent, first rank, conditional calls, Philox words
+used) is bit-identical to the CPU restatement on the same Philox streams."""
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+import pyoracle as O
+import zgraph as Z
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def both(n, s, d, seed, w=None):
+    say(f"[graph] n={n} m={len(s)} seed={seed}")
+    return Z.Graph.from_edges(s, d, n, weights=w, seed=seed), O.OGraph(n, s, d, w=w, seed=seed)
+
+
+def assert_same_csr(zg, og):
+    a = zg.export()
+    b = og.arrays()
+    assert zg.num_local_edges == og.m and zg.num_vertices == og.n
+    assert zg.max_degree == og.g.max_degree
+    assert (a["src"] == b["src"]).all() and (a["dst"] == b["dst"]).all()
+    assert (a["off"] == b["off"]).all()
+    assert (a["adj_rank"] == b["adj_rank"].astype(np.uint32)).all()
+    assert (a["adj_nbr"] == b["adj_dst"]).all()
+    if "w" in b:
+        assert (a["w"] == b["w"]).all()
+
+
+GRAPHS = {
+    "simple-30": lambda: H.golden_graph("simple-30"),
+    "small-10": lambda: H.golden_graph("small-10"),
+    "wiki-vote": lambda: H.golden_graph("wiki-vote"),
+    "K7": lambda: H.complete_graph(7),
+    "gnp-60": lambda: H.random_graph(60, 0.3, 5),
+    "powerlaw": lambda: H.power_law_graph(3000, 20000, 2.1, 7),
+}
+
+
+@pytest.mark.parametrize("name", sorted(GRAPHS))
+def test_device_csr_equals_oracle(name):
+    n, s, d = GRAPHS[name]()
+    zg, og = both(n, s, d, seed=11)
+    assert_same_csr(zg, og)
+
+
+def test_empty_and_rejected_graphs():
+    g = Z.Graph.from_edges(np.zeros(0, np.uint32), np.zeros(0, np.uint32), 4)
+    assert g.num_local_edges == 0 and g.max_degree == 0
+    sums = Z.run(g, Z.Plan(Z.PLAN_TRIANGLE), 1, 0, 1000)[0]
+    assert sums.count == 1000 and sums.successes == 0 and sums.total() == 0
+    with pytest.raises(Z.ZGraphError):
+        Z.Graph.from_edges(np.array([0, 2], np.uint32), np.array([1, 2], np.uint32), 3)   # self-loop
+    with pytest.raises(Z.ZGraphError):
+        Z.Graph.from_edges(np.array([0], np.uint32), np.array([5], np.uint32), 3)         # id >= n
+
+
+def test_subsample_and_filter_equal_oracle():
+    n, s, d = H.golden_graph("wiki-vote")
+    w = H.golden_weights("wiki-vote").astype(np.float32)
+    zg, og = both(n, s, d, seed=5, w=w)
+    assert_same_csr(zg, og)
+    assert_same_csr(zg.subsample(0.1, 9), og.subsample(0.1, 9))
+    assert_same_csr(zg.filter(50.0, True, 4), og.filter(50.0, True, 4))
+    assert_same_csr(zg.filter(50.0, False, 4), og.filter(50.0, False, 4))
+
+
+def compare_outcomes(zg, og, name, oargs, seed, first, count):
+    zplan = H.device_plan(Z, **oargs)
+    oplan = H.oracle_plan(O, **oargs)
+    say(f"[parity] plan={name} seed={seed} ids=[{first},{first + count})")
+    zo, zs = Z.run_outcomes(zg, zplan, seed, first, count)
+    oo = O.run_outcomes(og, oplan, seed, first, count)
+    zo = zo.reshape(count, -1)
+    oo = oo.reshape(count, -1)
+    for f in ("status", "num_lo", "num_hi", "log2den", "ncond", "nwords", "e0"):
+        bad = np.nonzero(zo[f] != oo[f])
+        assert len(bad[0]) == 0, (f, bad[0][:5], zo[bad][:3], oo[bad][:3])
+    osums = O.run(og, oplan, seed, first, count)
+    for ch in range(zplan.channels):
+        assert zs[ch].total() == osums[ch].total()
+        assert zs[ch].successes == osums[ch].successes
+        assert zs[ch].sumsq == pytest.approx(osums[ch].sumsq, rel=1e-9)
+    return zo
+
+
+@pytest.mark.parametrize("gname", ["wiki-vote", "powerlaw", "gnp-60", "K7", "simple-30"])
+@pytest.mark.parametrize("name,oargs", [(p[0], p[1]) for p in H.PLANS])
+def test_outcomes_bit_exact(gname, name, oargs):
+    n, s, d = GRAPHS[gname]()
+    zg, og = both(n, s, d, seed=3)
+    count = 40000 if gname in ("wiki-vote", "powerlaw") else 20000
+    zo = compare_outcomes(zg, og, name, oargs, seed=17, first=123, count=count)
+    if gname == "K7" and name not in ("pattern_house",):
+        assert (zo["status"] == 1).any()
+
+
+def test_cfg1_triangle_one_million_bit_exact():
+    # Erdos-Renyi 10,000 vertices / 100,000 edges, 10^6 estimators, Philox seed 1
+    e = Z.erdos_renyi(10000, 100000, seed=1)
+    s, d, _ = e.download()
+    assert len(s) == 100000 and (s < d).all()
+    assert len(np.unique(s.astype(np.uint64) << np.uint64(32) | d)) == 100000
+    zg = Z.Graph.from_device(e, seed=1)
+    og = O.OGraph(10000, s, d, seed=1)
+    assert_same_csr(zg, og)
+    zo = compare_outcomes(zg, og, "triangle", dict(kind=1), seed=1, first=0, count=1_000_000)
+    assert (zo["status"] == 1).sum() > 0
+
+
+def test_ids_not_launch_shape_define_outcomes():
+    n, s, d = H.power_law_graph(3000, 20000, 2.1, 7)
+    zg = Z.Graph.from_edges(s, d, n, seed=2)
+    plan = Z.Plan(Z.PLAN_K4_CLOSE)
+    a = Z.run(zg, plan, 5, 0, 300000, block_size=64, blocks_per_cu=1)[0]
+    b = Z.run(zg, plan, 5, 0, 300000, block_size=256, blocks_per_cu=32)[0]
+    c1 = Z.run(zg, plan, 5, 0, 100000)[0]
+    c2 = Z.run(zg, plan, 5, 100000, 200000)[0]
+    assert a.total() == b.total() == c1.total() + c2.total()
+    assert a.successes == b.successes == c1.successes + c2.successes

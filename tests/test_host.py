@@ -1,0 +1,108 @@
+"""CPU-side checks of the product: the C-ABI library loads and exports every
+symbol include/zgraph_amd.h declares, the host-only ELP arithmetic matches the
+oracle, the file-format readers round-trip, and compute entry points fail
+loudly without a GPU."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import zgraph as Z
+from zgraph import _capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    with open(os.path.join(ROOT, "include", "zgraph_amd.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(zg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_symbols_are_exported():
+    names = _declared()
+    assert len(names) >= 25
+    assert sorted(_capi.EXPORTED) == names
+    lib = _capi.lib()
+    for nm in names:
+        assert hasattr(lib, nm), nm
+
+
+def test_library_is_gfx950():
+    with open(_capi.LIB_PATH, "rb") as f:
+        blob = f.read()
+    assert b"gfx950" in blob
+
+
+def _gpu_present():
+    try:
+        import torch
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False
+
+
+@pytest.mark.skipif(_gpu_present(), reason="checks the no-GPU error path")
+def test_no_gpu_fails_loudly():
+    with pytest.raises(Z.ZGraphError) as ei:
+        Z.Graph.from_edges(np.array([0], np.uint32), np.array([1], np.uint32), 2)
+    assert ei.value.code == _capi.ZG_ESTATE
+
+
+def _ld(arr):
+    return (C.c_longdouble * len(arr))(*arr)
+
+
+def test_regression_known_answer_product():
+    x = [v / 100 for v in (0.09, 0.26, 0.15, 0.42, 0.57, 0.10, 2.62)]
+    y = [v / 100 for v in (1, 2, 4, 8, 16, 32, 64)]
+    k = _capi.lib().zg_regression_k(_ld(x), _ld(y), 7)
+    assert f"{k * k:.4f}" == "609.8565"   # tests/TestRegression.cpp
+
+
+def test_elp_arithmetic_matches_oracle():
+    rng = np.random.default_rng(3)
+    for trial in range(20):
+        npts = int(rng.integers(1, 3000))
+        F = float(rng.uniform(1e3, 1e7))
+        est = (F * (1 + rng.normal(0, 0.3, npts) / np.sqrt(np.arange(1, npts + 1)))).tolist()
+        args = (npts, F, 1.0, float(rng.integers(1, 5)), int(rng.integers(10, 5000)),
+                int(rng.integers(1000, 10 ** 7)), int(rng.integers(100, 50000)),
+                int(rng.integers(10 ** 4, 10 ** 9)), 0.05, 0.95)
+        k1, k2 = C.c_longdouble(), C.c_longdouble()
+        a = _capi.lib().zg_elp_from_pilots(_ld(est), *args, C.byref(k1))
+        b = O.lib().og_elp_num_estimators(_ld(est), *args, C.byref(k2))
+        assert a == b and k1.value == k2.value
+    assert _capi.lib().zg_elp_bound(0.05, 0.05, 1, 2, 1e6, 100, 10 ** 6) == \
+        O.lib().og_elp_bound(0.05, 0.05, 1, 2, 1e6, 100, 10 ** 6)
+
+
+def test_biedgelist_and_meta_roundtrip(tmp_path):
+    s = np.array([0, 2, 1, 4], np.uint32)
+    d = np.array([1, 3, 4, 0], np.uint32)
+    w = np.array([3, 75, 50, 0], np.uint32)
+    p = str(tmp_path / "g")
+    Z.save_biedgelist(p + ".biedgelist", s, d)
+    with open(p + ".csv", "w") as f:
+        f.write("vertices, edges\n5, 4\n")       # datasets/simple-*/dataset/*.csv layout
+    assert Z.load_graph_meta_data(p + ".csv") == 5
+    s2, d2, w2 = Z.load_biedgelist(p + ".biedgelist")
+    assert (s2 == s).all() and (d2 == d).all() and w2 is None
+    Z.save_biedgelist(p + ".w.biedgelist", s, d, w)
+    s3, d3, w3 = Z.load_biedgelist(p + ".w.biedgelist", weighted=True)
+    assert (w3 == w.astype(np.float32)).all()
+    with open(p + ".edgelist", "w") as f:
+        f.write("5 4\n" + "".join(f"{a} {b}\n" for a, b in zip(s, d)))
+    n, s4, d4, _ = Z.load_edgelist(p + ".edgelist")
+    assert n == 5 and (s4 == s).all() and (d4 == d).all()
+
+
+def test_clt_half_width():
+    # Bernoulli(1/2) scaled by 2: mean 1, variance 1
+    hw = Z.clt_half_width(1.0, 2.0 * 100, 100, 0.95)
+    assert hw == pytest.approx(1.959964 * math.sqrt(100 / 99) / 10, rel=1e-5)

@@ -1,0 +1,328 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Each check is independent of the formula it checks: published known-answer
+vectors, an independent implementation (rocRAND), closed forms, brute-force
+enumeration, exhaustive expectation over every random choice, the
+reference's own naive counters built from source, and the reference tests'
+printed values.
+"""
+import json
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import helpers as H
+import pyoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ---- Philox -----------------------------------------------------------------
+def test_philox_random123_kat_and_rocrand():
+    with open(os.path.join(H.GOLDEN, "philox_vectors.json")) as f:
+        vecs = json.load(f)["vectors"]
+    assert len(vecs) >= 67
+    for v in vecs:
+        assert O.philox(v["ctr"], v["key"]) == v["out"]
+
+
+def _stream_words(seed, tag, eid, k):
+    out = []
+    for b in range((k + 3) // 4):
+        out += O.philox([b, tag, eid & 0xffffffff, eid >> 32], [seed & 0xffffffff, seed >> 32])
+    return out[:k]
+
+
+# ---- bounded draws ------------------------------------------------------------
+def _draws(n, count, seed=7):
+    """count draws of U[0, n) from the estimator stream of ids 0.. (one per id)"""
+    L = O.lib()
+    import ctypes as C
+    st = (C.c_uint8 * 64)()
+    res, words = [], []
+    L.og_stream_init.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64]
+    L.og_uniform.argtypes = [C.c_void_p, C.c_uint64]
+    L.og_uniform.restype = C.c_uint64
+    for i in range(count):
+        L.og_stream_init(st, seed, 0, i)
+        res.append(L.og_uniform(st, n))
+        words.append(int.from_bytes(bytes(st[24:32]), "little"))   # og_stream.word (after key, tag, id)
+    return np.array(res, dtype=np.uint64), np.array(words)
+
+
+def test_uniform_power_of_two_is_top_bits():
+    # n = 2^k never rejects and returns the top k bits of the first word
+    for k in (1, 5, 17, 32):
+        r, w = _draws(1 << k, 200)
+        assert (w == 1).all()
+        first = np.array([_stream_words(7, 0, i, 1)[0] for i in range(200)], dtype=np.uint64)
+        assert (r == (first >> np.uint64(32 - k))).all()
+
+
+def test_uniform_rejection_rate_and_range():
+    # n = 2^31 + 1 rejects a word with probability (2^32 mod n)/2^32 ~ 1/2
+    n = (1 << 31) + 1
+    r, w = _draws(n, 4000)
+    assert (r < n).all()
+    extra = (w - 1).mean()            # expected words beyond the first: p/(1-p) ~ 1
+    assert 0.85 < extra < 1.15
+    # n > 2^32 uses two words per attempt
+    r, w = _draws((1 << 40) + 3, 500)
+    assert (r < (1 << 40) + 3).all() and (w % 2 == 0).all()
+
+
+@pytest.mark.parametrize("n", [3, 7, 1000])
+def test_uniform_chi_square(n):
+    r, _ = _draws(n, 30000, seed=11)
+    cnt = np.bincount(r.astype(np.int64), minlength=n)
+    exp = len(r) / n
+    chi = ((cnt - exp) ** 2 / exp).sum()
+    dof = n - 1
+    assert chi < dof + 6 * math.sqrt(2 * dof)
+
+
+# ---- CSR construction ----------------------------------------------------------
+@pytest.mark.parametrize("name", ["simple-5", "simple-10", "simple-30", "small-10", "wiki-vote"])
+def test_csr_invariants(name):
+    n, s, d = H.golden_graph(name)
+    g = O.OGraph(n, s, d, seed=3)
+    a = g.arrays()
+    m = len(s)
+    assert a["off"][-1] == 2 * m
+    deg = np.diff(a["off"].astype(np.int64))
+    assert (deg == np.bincount(np.concatenate([s, d]).astype(np.int64), minlength=n)).all()
+    # each rank appears twice, once in each endpoint's list
+    assert (np.bincount(a["adj_rank"].astype(np.int64), minlength=m) == 2).all()
+    for v in range(min(n, 2000)):
+        lo, hi = a["off"][v], a["off"][v + 1]
+        rk = a["adj_rank"][lo:hi]
+        assert (np.diff(rk.astype(np.int64)) > 0).all()      # ascending by rank (src/Graph.cpp:324-334)
+        assert (a["adj_src"][lo:hi] == v).all()
+        for r, nb in zip(rk, a["adj_dst"][lo:hi]):
+            assert {a["src"][r], a["dst"][r]} == {v, nb}
+    # the rank order is a permutation of the input edges
+    inp = sorted(zip(s.tolist(), d.tolist()))
+    assert sorted(zip(a["src"].tolist(), a["dst"].tolist())) == inp
+
+
+def test_csr_rejects_self_loops():
+    with pytest.raises(ValueError):
+        O.OGraph(3, np.array([0, 1]), np.array([1, 1]))
+
+
+def test_subsample_and_filter():
+    n, s, d = H.golden_graph("wiki-vote")
+    w = H.golden_weights("wiki-vote")
+    g = O.OGraph(n, s, d, w=w, seed=5)
+    sg = g.subsample(0.1, 9)
+    frac = sg.m / g.m
+    assert abs(frac - 0.1) < 0.01
+    f = g.filter(50.0, True, 4)
+    assert f.m == int((w >= 50).sum())
+    f2 = g.filter(50.0, False, 4)
+    assert f.m + f2.m == g.m
+
+
+# ---- primitives by brute force ---------------------------------------------------
+def _brute_entries(a, min_valid, verts):
+    out = []
+    for v in verts:
+        for i in range(a["off"][v], a["off"][v + 1]):
+            if a["adj_rank"][i] >= min_valid:
+                out.append((a["adj_rank"][i], a["adj_src"][i], a["adj_dst"][i]))
+    return out
+
+
+def test_conditional_pick_matches_enumeration():
+    import ctypes as C
+    n, s, d = H.random_graph(9, 0.6, 4)
+    # add a hub so that both the linear (deg <= 32) and binary paths run
+    hub_s = np.full(40, 9, np.uint32)
+    hub_d = np.arange(10, 50, dtype=np.uint32)
+    s, d = np.concatenate([s, hub_s]), np.concatenate([d, hub_d])
+    g = O.OGraph(50, s, d, seed=2)
+    a = g.arrays()
+    rng = np.random.default_rng(1)
+    L = O.lib()
+    for trial in range(300):
+        nv = int(rng.integers(1, 4))
+        verts = rng.choice(50, nv, replace=False).astype(np.uint32)
+        if trial % 3 == 0:
+            verts[0] = 9
+        mv = int(rng.integers(0, g.m + 2))
+        want = _brute_entries(a, mv, verts)
+        pos = np.zeros(16, np.uint64)
+        cnt = L.og_cond_valid_count(C.byref(g.g), mv, verts.ctypes.data, nv, pos.ctypes.data)
+        assert cnt == len(want)
+        for r in range(len(want)):
+            rk, sr, ds = C.c_uint64(), C.c_uint32(), C.c_uint32()
+            assert L.og_cond_pick(C.byref(g.g), mv, verts.ctypes.data, nv, r, C.byref(rk), C.byref(sr), C.byref(ds))
+            assert (rk.value, sr.value, ds.value) == tuple(int(x) for x in want[r])
+
+
+def test_close_matches_edge_set():
+    import ctypes as C
+    n, s, d = H.random_graph(12, 0.4, 8)
+    g = O.OGraph(n, s, d, seed=6)
+    a = g.arrays()
+    rank = {}
+    for r in range(g.m):
+        rank[(min(a["src"][r], a["dst"][r]), max(a["src"][r], a["dst"][r]))] = r
+    L = O.lib()
+    rng = np.random.default_rng(3)
+    for _ in range(500):
+        x, y = rng.choice(n, 2, replace=False)
+        mv = int(rng.integers(0, g.m + 1))
+        verts = np.array([x, y, rng.integers(0, n)], np.uint32)
+        ws, wd = np.array([x], np.uint32), np.array([y], np.uint32)
+        got = L.og_close(C.byref(g.g), mv, verts.ctypes.data, 3, ws.ctypes.data, wd.ctypes.data, 1)
+        key = (min(x, y), max(x, y))
+        assert bool(got) == (key in rank and rank[key] >= mv)
+
+
+# ---- estimators: exhaustive expectation equals the exact count ---------------------
+def _exact(g, name):
+    if name in ("triangle", "pattern_triangle"):
+        return g.triangles()
+    if name in ("k4_close", "pattern_k4"):
+        return g.k4()
+    if name == "three_chain":
+        return g.wedges()
+    if name in ("chain4", "four_chain"):
+        return g.paths_brute(4)
+    if name == "chain5":
+        return g.paths_brute(5)
+    if name == "pattern_5star":
+        return g.pattern_brute(*O.FIVE_STAR)
+    if name == "pattern_house":
+        return g.house_brute()
+    raise KeyError(name)
+
+
+TINY = [H.complete_graph(5), H.random_graph(7, 0.6, 1), H.random_graph(7, 0.5, 2),
+        H.random_graph(8, 0.45, 3)]
+
+
+@pytest.mark.parametrize("gi", range(len(TINY)))
+@pytest.mark.parametrize("name,oargs", [(p[0], p[1]) for p in H.PLANS])
+def test_unbiased_by_exhaustive_expectation(gi, name, oargs):
+    n, s, d = TINY[gi]
+    for seed in (1, 2):
+        g = O.OGraph(n, s, d, seed=seed)
+        plan = H.oracle_plan(O, **oargs)
+        if name == "three_motif":
+            assert O.expectation(g, plan, 0) == pytest.approx(g.wedges(), abs=1e-7)
+            assert O.expectation(g, plan, 1) == pytest.approx(g.triangles(), abs=1e-7)
+            continue
+        e = O.expectation(g, plan, 0)
+        assert e >= 0, "too many paths"
+        assert e == pytest.approx(_exact(g, name), rel=1e-9, abs=1e-7)
+
+
+def test_reference_fourclique_shape_alone_is_biased():
+    # Shape A alone is what applications/FourClique.cpp captures: averaged over
+    # all 720 rank orders of K4 it finds 3/5 of the clique (FourClique.cpp:35 TODO).
+    from itertools import permutations
+    n, s, d = H.complete_graph(4)
+    # enumerate rank orders by building the graph with each input order and a
+    # fixed shuffle seed, and count orders whose 3 lowest ranks form a spanning
+    # tree whose 2nd edge touches the 1st
+    good, tot = 0, 0
+    edges = list(zip(s.tolist(), d.tolist()))
+    for perm in permutations(range(6)):
+        ranked = [edges[i] for i in perm]
+        e1, e2, e3 = ranked[0], ranked[1], ranked[2]
+        adj12 = len(set(e1) & set(e2)) == 1
+        span = len(set(e1) | set(e2) | set(e3)) == 4
+        good += adj12 and span
+        tot += 1
+    assert good * 5 == tot * 3
+
+
+# ---- SamplerGenerator pattern machinery (tests/TestPattern2.cpp, TestPattern.cpp) ----
+def test_pattern_unique_relabelings():
+    assert O.pattern_unique(*O.FOUR_CHAIN) == (12, 24)
+    assert O.pattern_unique(*O.K4) == (1, 24)
+    assert O.pattern_unique(*O.FIVE_STAR) == (5, 120)
+    assert O.pattern_unique(*O.FIVE_HOUSE) == (60, 120)   # n!/|Aut| = 120/2
+
+
+def test_pattern_edge_order_classes():
+    assert O.pattern_edge_order_classes(*O.FIVE_CHAIN) == (12, 24)
+    k4 = [(i, j) for i in range(4) for j in range(i)]
+    assert O.pattern_edge_order_classes(4, k4) == (30, 720)   # 720 / |Aut(K4)| = 720/24
+
+
+# ---- exact counters ---------------------------------------------------------------
+def test_snap_wiki_vote_triangles():
+    n, s, d = H.golden_graph("snap-wiki-Vote")
+    g = O.OGraph(n, s, d)
+    assert g.triangles() == 608389            # SNAP's published count for wiki-Vote
+
+
+@pytest.mark.parametrize("gi", range(3))
+def test_exact_counters_agree_with_brute_force(gi):
+    n, s, d = [H.random_graph(12, 0.5, 10), H.random_graph(14, 0.35, 11), H.complete_graph(7)][gi]
+    g = O.OGraph(n, s, d)
+    assert g.triangles() == g.pattern_brute(*O.TRIANGLE)
+    assert g.k4() == g.pattern_brute(*O.K4)
+    assert g.paths3() == g.paths_brute(4)
+    assert g.wedges() == g.paths_brute(3)
+    if gi == 2:   # K7 closed forms
+        assert g.triangles() == math.comb(7, 3) and g.k4() == math.comb(7, 4)
+        assert g.house_brute() == math.comb(7, 5) * 60   # every 5 vertices hold 5!/2 houses
+
+
+NAIVE = {"TriangleCounting": lambda g: g.triangles(), "FourClique": lambda g: g.k4(),
+         "ThreeChain": lambda g: g.wedges(), "FourChain": lambda g: g.paths3(),
+         "FiveChain": lambda g: g.paths_brute(5), "FiveStar": lambda g: g.pattern_brute(*O.FIVE_STAR),
+         "FiveHouse": lambda g: g.house_brute()}
+
+
+@pytest.mark.parametrize("prog", sorted(NAIVE))
+def test_reference_naive_counters(prog, tmp_path):
+    exe = os.path.join(ROOT, "oracle", "_ref", prog)
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref not built (needs the reference sources: make -C oracle ref)")
+    n, s, d = H.golden_graph("simple-30")
+    p = tmp_path / "g.edgelist"
+    with open(p, "w") as f:
+        f.write(f"{n} {len(s)}\n")
+        for a, b in zip(s, d):
+            f.write(f"{a} {b}\n")
+    out = subprocess.run([exe, str(p)], capture_output=True, timeout=120).stdout.decode(errors="replace")
+    got = int([ln for ln in out.replace("\r", "\n").split("\n") if "count:" in ln][-1].split(":")[1])
+    assert got == NAIVE[prog](O.OGraph(n, s, d))
+
+
+# ---- regression ELP ------------------------------------------------------------------
+def _ld(arr):
+    import ctypes as C
+    return (C.c_longdouble * len(arr))(*arr)
+
+
+def test_regression_known_answer():
+    # tests/TestRegression.cpp: k^2 printed as 609.8565
+    x = [v / 100 for v in (0.09, 0.26, 0.15, 0.42, 0.57, 0.10, 2.62)]
+    y = [v / 100 for v in (1, 2, 4, 8, 16, 32, 64)]
+    k = O.lib().og_regression_k(_ld(x), _ld(y), 7)
+    assert f"{k * k:.4f}" == "609.8565"
+
+
+def test_elp_single_point_closed_form():
+    import ctypes as C
+    F, e0 = 1000.0, 1100.0
+    a, b, dmax_s, m_s, dmax, m, eps, conf = 1.0, 1.0, 50, 10000, 400, 100000, 0.05, 0.95
+    err = abs(e0 - F) / F * 5
+    ema = 100 * 0.96 + err * 0.04
+    bound = lambda e, d, est, D, M: (1 / e ** 2) * (D ** b) * (M ** a) / est * math.log(2 / d)
+    X = math.sqrt(bound(ema, 1 - conf, F, dmax_s, m_s) * ema ** 2 / 16)
+    K = ema / X
+    want = bound(eps, 1 - conf, F / 0.1 ** (a + b), dmax, m) * K * K
+    Kout = C.c_longdouble()
+    got = O.lib().og_elp_num_estimators(_ld([e0]), 1, F, a, b, dmax_s, m_s, dmax, m, eps, conf, C.byref(Kout))
+    assert got == pytest.approx(want, rel=1e-12)
+    assert float(Kout.value) == pytest.approx(K, rel=1e-12)
