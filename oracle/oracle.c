@@ -30,6 +30,7 @@
 #include <omp.h>
 
 _Static_assert(sizeof(og_outcome) == 24, "og_outcome must match the 24-byte zg_outcome record");
+_Static_assert(sizeof(og_sums) == 56, "og_sums layout is mirrored by oracle/pyoracle.py");
 
 /* ===================================================================== */
 /* Philox4x32-10 (Salmon et al., SC'11), replacing include/Random.hpp     */
@@ -1109,18 +1110,21 @@ static void acc_add(og_sums *s, const og_outcome *o)
     og_u128 v = ((og_u128)o->num_hi << 64) | o->num_lo;
     og_u128 lo = v << sh;
     uint64_t hi = sh ? (uint64_t)(v >> (128 - sh)) : 0;
-    og_u128 prev = s->sum_lo;
-    s->sum_lo += lo;
-    s->sum_hi += hi + (s->sum_lo < prev);
+    og_u128 cur = ((og_u128)s->sum[1] << 64) | s->sum[0];
+    og_u128 nxt = cur + lo;
+    s->sum[0] = (uint64_t)nxt; s->sum[1] = (uint64_t)(nxt >> 64);
+    s->sum[2] += hi + (nxt < cur);
     long double x = (long double)v / ldexpl(1.0L, o->log2den);
     s->sumsq += (double)(x * x);
 }
 
 static void acc_merge(og_sums *a, const og_sums *b)
 {
-    og_u128 prev = a->sum_lo;
-    a->sum_lo += b->sum_lo;
-    a->sum_hi += b->sum_hi + (a->sum_lo < prev);
+    og_u128 x = ((og_u128)a->sum[1] << 64) | a->sum[0];
+    og_u128 y = ((og_u128)b->sum[1] << 64) | b->sum[0];
+    og_u128 z = x + y;
+    a->sum[0] = (uint64_t)z; a->sum[1] = (uint64_t)(z >> 64);
+    a->sum[2] += b->sum[2] + (z < x);
     a->sumsq += b->sumsq;
     a->successes += b->successes;
     a->count += b->count;
@@ -1164,7 +1168,8 @@ void og_run_outcomes(const og_graph *g, const og_plan *p, uint64_t seed, uint64_
 double og_sums_mean(const og_sums *s)
 {
     if (!s->count) return 0;
-    long double v = (long double)s->sum_hi * ldexpl(1.0L, 128) + (long double)s->sum_lo;
+    long double v = (long double)s->sum[2] * ldexpl(1.0L, 128) + (long double)s->sum[1] * ldexpl(1.0L, 64) +
+                    (long double)s->sum[0];
     return (double)(v / ldexpl(1.0L, s->K) / (long double)s->count);
 }
 

@@ -48,13 +48,15 @@ assert OUTCOME_DTYPE.itemsize == C.sizeof(Outcome) == 24
 
 
 class Sums(C.Structure):
-    _fields_ = [("sum_lo_lo", C.c_uint64), ("sum_lo_hi", C.c_uint64), ("sum_hi", C.c_uint64),
-                ("sumsq", C.c_double), ("successes", C.c_uint64), ("count", C.c_uint64),
-                ("K", C.c_int)]
+    _fields_ = [("sum", C.c_uint64 * 3), ("sumsq", C.c_double), ("successes", C.c_uint64),
+                ("count", C.c_uint64), ("K", C.c_int)]
 
     def total(self):
         """exact sum of values times 2^K, as a Python int"""
-        return self.sum_lo_lo | (self.sum_lo_hi << 64) | (self.sum_hi << 128)
+        return self.sum[0] | (self.sum[1] << 64) | (self.sum[2] << 128)
+
+
+assert C.sizeof(Sums) == 56
 
 
 _lib = None

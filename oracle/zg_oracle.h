@@ -135,8 +135,7 @@ double og_plan_cond_calls(const og_plan *p);
 void og_estimate(const og_graph *g, const og_plan *p, uint64_t seed, uint64_t id, og_outcome *out);
 
 typedef struct {
-    og_u128 sum_lo;            /* sum of num * 2^(K - log2den), low 128 bits */
-    uint64_t sum_hi;           /* bits 128..191 */
+    uint64_t sum[3];           /* sum of num * 2^(K - log2den), 192 bits little-endian */
     double sumsq;              /* sum of value^2 (value = num/2^log2den) */
     uint64_t successes;
     uint64_t count;

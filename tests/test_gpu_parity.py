@@ -84,8 +84,12 @@ def compare_outcomes(zg, og, name, oargs, seed, first, count):
     zo = zo.reshape(count, -1)
     oo = oo.reshape(count, -1)
     for f in ("status", "num_lo", "num_hi", "log2den", "ncond", "nwords", "e0"):
-        bad = np.nonzero(zo[f] != oo[f])
-        assert len(bad[0]) == 0, (f, bad[0][:5], zo[bad][:3], oo[bad][:3])
+        bad = np.argwhere(zo[f] != oo[f])
+        if len(bad):
+            i, c = (int(x) for x in bad[0])
+            say(f"[parity] first mismatch in {f} at id {first + i} channel {c}: "
+                f"device {zo[i, c].tolist()} oracle {oo[i, c].tolist()}")
+        assert len(bad) == 0, f"{len(bad)} estimators differ in {f}"
     osums = O.run(og, oplan, seed, first, count)
     for ch in range(zplan.channels):
         assert zs[ch].total() == osums[ch].total()
