@@ -105,7 +105,7 @@ def test_outcomes_bit_exact(gname, name, oargs):
     zg, og = both(n, s, d, seed=3)
     count = 40000 if gname in ("wiki-vote", "powerlaw") else 20000
     zo = compare_outcomes(zg, og, name, oargs, seed=17, first=123, count=count)
-    if gname == "K7" and name not in ("pattern_house",):
+    if gname == "K7" and name not in ("pattern_house", "pattern_k4", "pattern_5star"):
         assert (zo["status"] == 1).any()
 
 
