@@ -52,7 +52,7 @@ int zg_device_info(int *num_cus, int *lds_per_cu, int *warp_size);
  * ids") are a keyed random permutation of the input order; every vertex's
  * adjacency is ascending by rank, as the reference asserts
  * (src/Graph.cpp:324-334).  Self-loops are rejected (src/Graph.cpp:454).
- * Requires m < 2^32.  Host pointers. */
+ * Requires m < 2^31.  Host pointers. */
 int zg_graph_create(const uint32_t *src, const uint32_t *dst, const float *weights,
                     uint64_t m, uint32_t n, uint64_t shuffle_seed, zg_graph **out);
 /* Same, with the edge list already in device memory (not consumed). */
@@ -185,6 +185,12 @@ int zg_run_outcomes(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first
  * slower (it counts with atomics).  No reference counterpart. */
 int zg_count_reads(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_id, uint64_t count,
                    uint64_t *reads);
+
+/* ---- exact counts (naive_implementation/*.cpp semantics) --------------- *
+ * Non-induced counts on the simple graph underlying the edge list: what = 0
+ * triangles (TriangleCounting.cpp:44-72), 1 4-cliques (FourClique.cpp:44-78),
+ * 2 wedges (ThreeChain.cpp:44-51), 3 three-edge paths (FourChain.cpp:46-66). */
+int zg_exact_counts(zg_graph *g, int what, uint64_t *out);
 
 /* ---- error-latency profile (src/ErrorLatencyProfile.cpp) --------------- */
 /* ErrorProfile<E>::get_num_estimators (:71-207) on the device: subsample at

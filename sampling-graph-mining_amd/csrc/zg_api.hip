@@ -146,11 +146,11 @@ extern "C" int zg_graph_export(const zg_graph *g, uint32_t *src, uint32_t *dst, 
     if (!g) return set_error(ZG_EINVAL, "null graph");
     const uint64_t m = g->m, n2 = 2 * m;
     if (src || dst) {
-        std::vector<uint4> e(m);
-        if (m) ZG_HIP(hipMemcpy(e.data(), g->edges, 16 * m, hipMemcpyDeviceToHost));
+        std::vector<zg::ERec> e(m);
+        if (m) ZG_HIP(hipMemcpy(e.data(), g->edges, sizeof(zg::ERec) * m, hipMemcpyDeviceToHost));
         for (uint64_t r = 0; r < m; ++r) {
-            if (src) src[r] = e[r].x;
-            if (dst) dst[r] = e[r].y;
+            if (src) src[r] = e[r].src;
+            if (dst) dst[r] = e[r].dst;
         }
     }
     if (w) {

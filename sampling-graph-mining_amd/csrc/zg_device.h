@@ -10,26 +10,63 @@
 namespace zg {
 
 // ---------------------------------------------------------------------------
-// HBM layout of one graph (all arrays indexed as in the reference's CSR,
-// include/Graph.hpp:148-166, at one worker):
-//   edges[r] = {src, dst, position of r in src's list, position in dst's list}
-//              16 B per edge, r = rank ("edge id")
-//   off[v]   = start of v's lists (n+1 entries, u64)
+// HBM layout of one graph (the reference's CSR, include/Graph.hpp:148-166, at
+// one worker):
+//   edges[r] = 32-byte record of rank r ("edge id"): its endpoints, and for
+//              each endpoint the list start, the degree and the position of r
+//              in that list -- one 64-byte sector holds everything a sampled
+//              edge needs, so no offset lookups follow a sample
+//   off[v]   = start of v's lists (n+1 entries, u64; export and generic use)
 //   adj[i]   = rank << 32 | neighbour     each list ascending by rank
 //   sadj[i]  = neighbour << 32 | rank     each list ascending by (neighbour, rank)
-// The position index turns the reference's lower_bound on "rank >= e+1" in
+// The stored position turns the reference's lower_bound on "rank >= e+1" in
 // the endpoint lists of a sampled edge e into one load; sadj turns the linear
 // conditional_close scan (src/ZGraphInstance.cpp:371-407) into one binary search.
+// List starts are u32, so a graph holds fewer than 2^31 edges.
 // ---------------------------------------------------------------------------
+struct alignas(32) ERec {
+    uint32_t src, dst;
+    uint32_t s_off, s_deg;   // src's lists
+    uint32_t d_off, d_deg;   // dst's lists
+    uint32_t s_pos, d_pos;   // position of this rank in src's / dst's list
+};
+
+// Edge-set hash for the closing checks: 64-byte buckets of four 16-byte
+// slots {key = min << 32 | max, rank}, at most two edges per bucket on
+// average, linear probing over buckets.  Buckets only fill up, so a lookup
+// may stop at the first bucket that still has an empty slot.
+struct alignas(16) HSlot {
+    uint64_t key;
+    uint32_t rank, pad;
+};
+constexpr uint64_t kHashEmpty = ~0ull;
+
 struct GView {
-    const uint4 *__restrict__ edges;
+    const ERec *__restrict__ edges;
     const uint64_t *__restrict__ off;
     const uint64_t *__restrict__ adj;
     const uint64_t *__restrict__ sadj;
+    const HSlot *__restrict__ htab;
     uint64_t m;
     uint32_t n;
+    uint32_t hnb;                // number of buckets
     unsigned long long *reads;   // non-null only in the read-counting launch
 };
+
+// murmur3 fmix64: spreads (min, max) pairs over the buckets
+__host__ __device__ __forceinline__ uint64_t hmix(uint64_t k)
+{
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+__host__ __device__ __forceinline__ uint32_t hbucket(uint64_t key, uint32_t nb)
+{
+    return (uint32_t)(((hmix(key) >> 32) * (uint64_t)nb) >> 32);
+}
 
 // counts dependent reads (one 64-byte sector each) in the read-counting launch
 __device__ __forceinline__ void count_reads(const GView &g, uint32_t k)
@@ -131,14 +168,34 @@ __device__ __forceinline__ bool add_u128(U128 &a, const U128 &b)
 
 // vertex descriptor: list start and degree
 struct VI {
-    uint32_t v, deg;
-    uint64_t off;
+    uint32_t v, deg, off;
 };
 __device__ __forceinline__ VI vinfo(const GView &g, uint32_t v)
 {
     const uint64_t a = g.off[v], b = g.off[v + 1];
     count_reads(g, ((v & 7u) == 7u) ? 2u : 1u);   // off[v], off[v+1] share a sector unless v%8 == 7
-    return VI{v, (uint32_t)(b - a), a};
+    return VI{v, (uint32_t)(b - a), (uint32_t)a};
+}
+
+// the 32-byte record of rank r (two 16-byte loads of one sector)
+__device__ __forceinline__ ERec load_edge(const GView &g, uint32_t r)
+{
+    const uint4 *p = reinterpret_cast<const uint4 *>(g.edges + r);
+    const uint4 a = p[0], b = p[1];
+    count_reads(g, 1);
+    ERec e;
+    e.src = a.x; e.dst = a.y; e.s_off = a.z; e.s_deg = a.w;
+    e.d_off = b.x; e.d_deg = b.y; e.s_pos = b.z; e.d_pos = b.w;
+    return e;
+}
+__device__ __forceinline__ VI src_vi(const ERec &e) { return VI{e.src, e.s_deg, e.s_off}; }
+__device__ __forceinline__ VI dst_vi(const ERec &e) { return VI{e.dst, e.d_deg, e.d_off}; }
+// endpoint v of e: its lists, and the position of e's rank in them
+__device__ __forceinline__ VI end_vi(const ERec &e, uint32_t v, uint32_t &pos)
+{
+    if (e.src == v) { pos = e.s_pos; return src_vi(e); }
+    pos = e.d_pos;
+    return dst_vi(e);
 }
 
 // first position p in [lo, hi] of v's rank-ordered list with rank >= min_valid
@@ -147,7 +204,7 @@ __device__ __forceinline__ uint32_t lb_rank(const GView &g, const VI &x, uint32_
                                             uint32_t min_valid)
 {
     const uint64_t key = (uint64_t)min_valid << 32;
-    const uint64_t *__restrict__ a = g.adj + x.off;
+    const uint64_t *__restrict__ a = g.adj + (uint64_t)x.off;
     uint32_t probes = 0;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -160,30 +217,38 @@ __device__ __forceinline__ uint32_t lb_rank(const GView &g, const VI &x, uint32_
 
 // conditional_close for one awaited edge {x, y} (src/ZGraphInstance.cpp:371-441):
 // true iff some copy of {x, y} has rank >= min_valid; *rank receives the
-// smallest such rank.  Binary search in the shorter neighbour-sorted list.
+// smallest such rank.  One hash bucket (one 64-byte read) in the common case.
+__device__ __forceinline__ bool close_pair(const GView &g, uint32_t x, uint32_t y, uint32_t min_valid,
+                                           uint32_t *rank)
+{
+    if (x == y) return false;
+    const uint64_t key = x < y ? ((uint64_t)x << 32) | y : ((uint64_t)y << 32) | x;
+    uint32_t b = hbucket(key, g.hnb);
+    uint32_t best = 0xffffffffu;
+    uint32_t buckets = 0;
+    for (;;) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(g.htab + 4ull * b);
+        const uint4 s0 = p[0], s1 = p[1], s2 = p[2], s3 = p[3];
+        ++buckets;
+        const uint64_t k0 = (uint64_t)s0.y << 32 | s0.x, k1 = (uint64_t)s1.y << 32 | s1.x;
+        const uint64_t k2 = (uint64_t)s2.y << 32 | s2.x, k3 = (uint64_t)s3.y << 32 | s3.x;
+        if (k0 == key && s0.z >= min_valid && s0.z < best) best = s0.z;
+        if (k1 == key && s1.z >= min_valid && s1.z < best) best = s1.z;
+        if (k2 == key && s2.z >= min_valid && s2.z < best) best = s2.z;
+        if (k3 == key && s3.z >= min_valid && s3.z < best) best = s3.z;
+        if (k0 == kHashEmpty || k1 == kHashEmpty || k2 == kHashEmpty || k3 == kHashEmpty) break;
+        b = (b + 1 == g.hnb) ? 0 : b + 1;
+    }
+    count_reads(g, buckets);
+    if (best == 0xffffffffu) return false;
+    if (rank) *rank = best;
+    return true;
+}
+
 __device__ __forceinline__ bool close_edge(const GView &g, const VI &x, const VI &y, uint32_t min_valid,
                                            uint32_t *rank)
 {
-    if (x.v == y.v) return false;
-    const bool xs = x.deg <= y.deg;
-    const uint64_t base = xs ? x.off : y.off;
-    uint32_t hi = xs ? x.deg : y.deg;
-    const uint32_t other = xs ? y.v : x.v;
-    const uint64_t key = ((uint64_t)other << 32) | min_valid;
-    const uint64_t *__restrict__ a = g.sadj + base;
-    uint32_t lo = 0, probes = 0;
-    const uint32_t n = hi;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] < key) lo = mid + 1; else hi = mid;
-        ++probes;
-    }
-    count_reads(g, probes + (lo < n));
-    if (lo >= n) return false;
-    const uint64_t e = a[lo];
-    if ((uint32_t)(e >> 32) != other) return false;
-    if (rank) *rank = (uint32_t)e;
-    return true;
+    return close_pair(g, x.v, y.v, min_valid, rank);
 }
 
 // sampled edge: rank, owner (src), neighbour (dst), and its position in the

@@ -1,0 +1,293 @@
+// zg_exact.hip -- exact subgraph counts on the device, the scalable
+// counterpart of naive_implementation/*.cpp (non-induced counts on the
+// simple graph underlying the edge list):
+//   triangles  naive_implementation/TriangleCounting.cpp:44-72
+//   4-cliques  naive_implementation/FourClique.cpp:44-78
+//   wedges     naive_implementation/ThreeChain.cpp:44-51 (sum of C(d, 2))
+//   3-paths    naive_implementation/FourChain.cpp:46-66 (sum over edges (du-1)(dv-1) - 3T)
+// Method: orient every undirected pair from lower to higher (degree, id) and
+// keep the out-lists sorted by neighbour id (taken from the neighbour-ordered
+// copy, parallel edges collapsed).  Triangles: per oriented edge u->v, binary
+// search every out-neighbour of the shorter list in the longer one.  4-cliques:
+// one wavefront per oriented edge builds W = out(u) & out(v) in a scratch
+// slot, then counts pairs w->x inside W.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "zg_internal.h"
+
+namespace zg {
+namespace {
+
+template <class T>
+struct XB {
+    T *p = nullptr;
+    hipError_t alloc(size_t k) { return hipMalloc(&p, sizeof(T) * (k ? k : 1)); }
+    ~XB() { if (p) (void)hipFree(p); }
+};
+#define XALLOC(b, k)                                                                   \
+    do {                                                                               \
+        if ((b).alloc(k) != hipSuccess) return set_error(ZG_ENOMEM, "device allocation failed"); \
+    } while (0)
+#define GRID_STRIDE(i, n) \
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
+
+inline int gridx(uint64_t n)
+{
+    uint64_t g = (n + 255) / 256, cap = (uint64_t)device_cus() * 64;
+    return (int)std::max<uint64_t>(1, std::min(g, cap));
+}
+
+// simple-graph degree of v: distinct neighbours in its neighbour-ordered list
+__global__ void k_simple_deg(const uint64_t *off, const uint64_t *sadj, uint32_t n, uint32_t *sdeg)
+{
+    GRID_STRIDE(v, (uint64_t)n)
+    {
+        uint32_t d = 0;
+        uint32_t prev = 0xffffffffu;
+        for (uint64_t i = off[v]; i < off[v + 1]; ++i) {
+            const uint32_t nb = (uint32_t)(sadj[i] >> 32);
+            d += nb != prev;
+            prev = nb;
+        }
+        sdeg[v] = d;
+    }
+}
+
+__device__ __forceinline__ bool before(const uint32_t *sdeg, uint32_t a, uint32_t b)
+{
+    const uint32_t da = sdeg[a], db = sdeg[b];
+    return da < db || (da == db && a < b);
+}
+
+// out-degree (toward higher order) of every vertex
+__global__ void k_out_deg(const uint64_t *off, const uint64_t *sadj, const uint32_t *sdeg, uint32_t n,
+                          uint64_t *odeg)
+{
+    GRID_STRIDE(v, (uint64_t)n)
+    {
+        uint64_t d = 0;
+        uint32_t prev = 0xffffffffu;
+        for (uint64_t i = off[v]; i < off[v + 1]; ++i) {
+            const uint32_t nb = (uint32_t)(sadj[i] >> 32);
+            if (nb != prev && before(sdeg, (uint32_t)v, nb)) ++d;
+            prev = nb;
+        }
+        odeg[v] = d;
+    }
+}
+
+__global__ void k_out_fill(const uint64_t *off, const uint64_t *sadj, const uint32_t *sdeg, uint32_t n,
+                           const uint64_t *ooff, uint32_t *onb, uint32_t *osrc)
+{
+    GRID_STRIDE(v, (uint64_t)n)
+    {
+        uint64_t w = ooff[v];
+        uint32_t prev = 0xffffffffu;
+        for (uint64_t i = off[v]; i < off[v + 1]; ++i) {
+            const uint32_t nb = (uint32_t)(sadj[i] >> 32);
+            if (nb != prev && before(sdeg, (uint32_t)v, nb)) { onb[w] = nb; osrc[w] = (uint32_t)v; ++w; }
+            prev = nb;
+        }
+    }
+}
+
+__device__ __forceinline__ bool bsearch_u32(const uint32_t *a, uint64_t n, uint32_t x)
+{
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo < n && a[lo] == x;
+}
+
+__device__ __forceinline__ unsigned long long block_sum(unsigned long long v)
+{
+    __shared__ unsigned long long sm[4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    unsigned long long t = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sm[w];
+    return t;
+}
+
+__global__ __launch_bounds__(256) void k_triangles(const uint64_t *ooff, const uint32_t *onb, const uint32_t *osrc,
+                                                   uint64_t ne, unsigned long long *partial)
+{
+    unsigned long long c = 0;
+    GRID_STRIDE(e, ne)
+    {
+        const uint32_t u = osrc[e], v = onb[e];
+        const uint32_t *a = onb + ooff[u], *b = onb + ooff[v];
+        uint64_t na = ooff[u + 1] - ooff[u], nb = ooff[v + 1] - ooff[v];
+        if (na > nb) { const uint32_t *t = a; a = b; b = t; const uint64_t x = na; na = nb; nb = x; }
+        for (uint64_t i = 0; i < na; ++i) c += bsearch_u32(b, nb, a[i]);
+    }
+    const unsigned long long t = block_sum(c);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// one wavefront per oriented edge; W lives in this wave's scratch slot
+__global__ __launch_bounds__(256) void k_k4(const uint64_t *ooff, const uint32_t *onb, const uint32_t *osrc,
+                                            uint64_t ne, uint32_t *scratch, uint64_t slot,
+                                            unsigned long long *partial)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    uint32_t *W = scratch + wave * slot;
+    unsigned long long c = 0;
+    for (uint64_t e = wave; e < ne; e += nwaves) {
+        const uint32_t u = osrc[e], v = onb[e];
+        const uint32_t *a = onb + ooff[u], *b = onb + ooff[v];
+        uint64_t na = ooff[u + 1] - ooff[u], nb = ooff[v + 1] - ooff[v];
+        if (na > nb) { const uint32_t *t = a; a = b; b = t; const uint64_t x = na; na = nb; nb = x; }
+        // W = a & b, in increasing id order (ballot compaction keeps the order)
+        uint64_t nw = 0;
+        for (uint64_t base = 0; base < na; base += 64) {
+            const uint64_t i = base + lane;
+            const bool hit = i < na && bsearch_u32(b, nb, a[i]);
+            const uint64_t mask = __ballot(hit);
+            if (hit) W[nw + __popcll(mask & ((1ull << lane) - 1))] = a[i];
+            nw += __popcll(mask);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        // pairs w -> x with both in W
+        for (uint64_t j = lane; j < nw; j += 64) {
+            const uint32_t w = W[j];
+            const uint32_t *ow = onb + ooff[w];
+            const uint64_t now = ooff[w + 1] - ooff[w];
+            if (now <= nw) {
+                for (uint64_t k = 0; k < now; ++k) c += bsearch_u32(W, nw, ow[k]);
+            } else {
+                for (uint64_t k = 0; k < nw; ++k) c += bsearch_u32(ow, now, W[k]);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    const unsigned long long t = block_sum(c);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void k_wedges_paths(const uint64_t *off, const uint64_t *sadj, const uint32_t *sdeg, uint32_t n,
+                               unsigned long long *wed, unsigned long long *paths)
+{
+    unsigned long long w = 0, p = 0;
+    GRID_STRIDE(v, (uint64_t)n)
+    {
+        const unsigned long long d = sdeg[v];
+        w += d * (d - (d > 0)) / 2;
+        uint32_t prev = 0xffffffffu;
+        for (uint64_t i = off[v]; i < off[v + 1]; ++i) {
+            const uint32_t nb = (uint32_t)(sadj[i] >> 32);
+            if (nb != prev && (uint32_t)v < nb) p += (d - 1) * (unsigned long long)(sdeg[nb] - 1);
+            prev = nb;
+        }
+    }
+    const unsigned long long tw = block_sum(w);
+    __syncthreads();
+    const unsigned long long tp = block_sum(p);
+    if (threadIdx.x == 0) { wed[blockIdx.x] = tw; paths[blockIdx.x] = tp; }
+}
+
+int host_sum(const unsigned long long *d, int n, hipStream_t st, unsigned long long *out)
+{
+    std::vector<unsigned long long> h(n);
+    ZG_HIP(hipMemcpyAsync(h.data(), d, 8ull * n, hipMemcpyDeviceToHost, st));
+    ZG_HIP(hipStreamSynchronize(st));
+    unsigned long long s = 0;
+    for (auto x : h) s += x;
+    *out = s;
+    return ZG_OK;
+}
+
+}  // namespace
+}  // namespace zg
+
+using namespace zg;
+
+extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
+{
+    if (!g || !out) return set_error(ZG_EINVAL, "null argument");
+    if (what < 0 || what > 3) return set_error(ZG_EINVAL, "what must be 0 (triangles), 1 (4-cliques), 2 (wedges), 3 (3-edge paths)");
+    const uint32_t n = g->n;
+    hipStream_t st = g->stream;
+    if (g->m == 0) { *out = 0; return ZG_OK; }
+    XB<uint32_t> sdeg;
+    XALLOC(sdeg, n);
+    hipLaunchKernelGGL(k_simple_deg, dim3(gridx(n)), dim3(256), 0, st, g->off, g->sadj, n, sdeg.p);
+    if (what >= 2) {
+        const int grid = gridx(n);
+        XB<unsigned long long> pw, pp;
+        XALLOC(pw, grid); XALLOC(pp, grid);
+        hipLaunchKernelGGL(k_wedges_paths, dim3(grid), dim3(256), 0, st, g->off, g->sadj, sdeg.p, n, pw.p, pp.p);
+        ZG_HIP(hipGetLastError());
+        unsigned long long w = 0, p = 0;
+        int rc = host_sum(pw.p, grid, st, &w);
+        if (!rc) rc = host_sum(pp.p, grid, st, &p);
+        if (rc) return rc;
+        if (what == 2) { *out = w; return ZG_OK; }
+        uint64_t T = 0;
+        rc = zg_exact_counts(g, 0, &T);
+        if (rc) return rc;
+        *out = p - 3 * T;
+        return ZG_OK;
+    }
+    // oriented out-lists
+    XB<uint64_t> odeg, ooff;
+    XALLOC(odeg, (uint64_t)n + 1); XALLOC(ooff, (uint64_t)n + 1);
+    ZG_HIP(hipMemsetAsync(odeg.p + n, 0, 8, st));
+    hipLaunchKernelGGL(k_out_deg, dim3(gridx(n)), dim3(256), 0, st, g->off, g->sadj, sdeg.p, n, odeg.p);
+    size_t tb = 0;
+    ZG_HIP(rocprim::exclusive_scan(nullptr, tb, odeg.p, ooff.p, (uint64_t)0, (size_t)n + 1, rocprim::plus<uint64_t>(), st));
+    XB<char> tmp;
+    XALLOC(tmp, tb);
+    ZG_HIP(rocprim::exclusive_scan(tmp.p, tb, odeg.p, ooff.p, (uint64_t)0, (size_t)n + 1, rocprim::plus<uint64_t>(), st));
+    uint64_t ne = 0;
+    ZG_HIP(hipMemcpyAsync(&ne, ooff.p + n, 8, hipMemcpyDeviceToHost, st));
+    size_t tr = 0;
+    XB<uint64_t> mx;
+    XALLOC(mx, 1);
+    ZG_HIP(rocprim::reduce(nullptr, tr, odeg.p, mx.p, (uint64_t)0, (size_t)n, rocprim::maximum<uint64_t>(), st));
+    XB<char> tmp2;
+    XALLOC(tmp2, tr);
+    ZG_HIP(rocprim::reduce(tmp2.p, tr, odeg.p, mx.p, (uint64_t)0, (size_t)n, rocprim::maximum<uint64_t>(), st));
+    uint64_t maxout = 0;
+    ZG_HIP(hipMemcpyAsync(&maxout, mx.p, 8, hipMemcpyDeviceToHost, st));
+    ZG_HIP(hipStreamSynchronize(st));
+    XB<uint32_t> onb, osrc;
+    XALLOC(onb, ne); XALLOC(osrc, ne);
+    hipLaunchKernelGGL(k_out_fill, dim3(gridx(n)), dim3(256), 0, st, g->off, g->sadj, sdeg.p, n, ooff.p, onb.p, osrc.p);
+    ZG_HIP(hipGetLastError());
+    const int grid = (int)std::min<uint64_t>((uint64_t)device_cus() * 8, std::max<uint64_t>(1, (ne + 255) / 256));
+    XB<unsigned long long> part;
+    XALLOC(part, grid);
+    if (what == 0) {
+        hipLaunchKernelGGL(k_triangles, dim3(grid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, ne, part.p);
+    } else {
+        const uint64_t slot = maxout ? maxout : 1;
+        const uint64_t nwaves = (uint64_t)grid * 4;
+        XB<uint32_t> scratch;
+        XALLOC(scratch, nwaves * slot);
+        hipLaunchKernelGGL(k_k4, dim3(grid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, ne, scratch.p, slot, part.p);
+        ZG_HIP(hipGetLastError());
+        unsigned long long s = 0;
+        int rc = host_sum(part.p, grid, st, &s);
+        if (rc) return rc;
+        *out = s;
+        return ZG_OK;
+    }
+    ZG_HIP(hipGetLastError());
+    unsigned long long s = 0;
+    int rc = host_sum(part.p, grid, st, &s);
+    if (rc) return rc;
+    *out = s;
+    return ZG_OK;
+}

@@ -86,13 +86,16 @@ __global__ void k_shuffle_keys(uint64_t *keys, uint32_t *idx, uint64_t m, uint64
 
 // gather by rank, count degrees, emit the 2m list entries in rank order
 __global__ void k_gather(const uint32_t *src, const uint32_t *dst, const float *w, const uint32_t *perm, uint64_t m,
-                         uint4 *edges, float *wout, unsigned long long *deg, uint32_t *own, uint64_t *ent)
+                         ERec *edges, float *wout, unsigned long long *deg, uint32_t *own, uint64_t *ent)
 {
     GRID_STRIDE(r, m)
     {
         const uint32_t i = perm[r];
         const uint32_t s = src[i], d = dst[i];
-        edges[r] = make_uint4(s, d, 0u, 0u);
+        ERec e;
+        e.src = s; e.dst = d;
+        e.s_off = e.s_deg = e.d_off = e.d_deg = e.s_pos = e.d_pos = 0;
+        edges[r] = e;
         if (w) wout[r] = w[i];
         atomicAdd(&deg[s], 1ull);
         atomicAdd(&deg[d], 1ull);
@@ -103,8 +106,9 @@ __global__ void k_gather(const uint32_t *src, const uint32_t *dst, const float *
     }
 }
 
-// positions of every rank in both endpoint lists; neighbour-order sort keys
-__global__ void k_positions(const uint32_t *own, const uint64_t *adj, const uint64_t *off, uint64_t n2, uint4 *edges,
+// each rank's endpoint lists (start, degree, position of the rank) into its
+// record; neighbour-order sort keys
+__global__ void k_positions(const uint32_t *own, const uint64_t *adj, const uint64_t *off, uint64_t n2, ERec *edges,
                             uint64_t *skey, uint32_t *sval)
 {
     GRID_STRIDE(i, n2)
@@ -112,8 +116,10 @@ __global__ void k_positions(const uint32_t *own, const uint64_t *adj, const uint
         const uint32_t o = own[i];
         const uint64_t e = adj[i];
         const uint32_t r = (uint32_t)(e >> 32), nb = (uint32_t)e;
-        const uint32_t p = (uint32_t)(i - off[o]);
-        if (edges[r].x == o) edges[r].z = p; else edges[r].w = p;
+        const uint64_t b = off[o];
+        const uint32_t p = (uint32_t)(i - b), dg = (uint32_t)(off[o + 1] - b);
+        if (edges[r].src == o) { edges[r].s_off = (uint32_t)b; edges[r].s_deg = dg; edges[r].s_pos = p; }
+        else { edges[r].d_off = (uint32_t)b; edges[r].d_deg = dg; edges[r].d_pos = p; }
         skey[i] = ((uint64_t)o << 32) | nb;
         sval[i] = r;
     }
@@ -122,6 +128,27 @@ __global__ void k_positions(const uint32_t *own, const uint64_t *adj, const uint
 __global__ void k_sadj(const uint64_t *skey, const uint32_t *sval, uint64_t n2, uint64_t *sadj)
 {
     GRID_STRIDE(i, n2) { sadj[i] = ((skey[i] & 0xffffffffull) << 32) | sval[i]; }
+}
+
+// edge-set hash: one slot per rank, claimed with a CAS on the key
+__global__ void k_hash_insert(const ERec *edges, uint64_t m, HSlot *tab, uint32_t nb)
+{
+    GRID_STRIDE(r, m)
+    {
+        const uint32_t a = edges[r].src, b = edges[r].dst;
+        const uint64_t key = a < b ? ((uint64_t)a << 32) | b : ((uint64_t)b << 32) | a;
+        uint32_t bk = hbucket(key, nb);
+        for (bool done = false; !done;) {
+            for (int s = 0; s < 4 && !done; ++s) {
+                HSlot *slot = tab + 4ull * bk + s;
+                const unsigned long long prev =
+                    atomicCAS(reinterpret_cast<unsigned long long *>(&slot->key), (unsigned long long)kHashEmpty,
+                              (unsigned long long)key);
+                if (prev == kHashEmpty) { slot->rank = (uint32_t)r; done = true; }
+            }
+            bk = (bk + 1 == nb) ? 0 : bk + 1;
+        }
+    }
 }
 
 __global__ void k_deg_to_u64(const unsigned long long *deg, uint64_t *out, uint64_t n1)
@@ -138,6 +165,7 @@ void free_graph(zg_graph *g)
     if (g->off) (void)hipFree(g->off);
     if (g->adj) (void)hipFree(g->adj);
     if (g->sadj) (void)hipFree(g->sadj);
+    if (g->htab) (void)hipFree(g->htab);
     if (g->w) (void)hipFree(g->w);
     if (g->partials) (void)hipFree(g->partials);
     if (g->result) (void)hipFree(g->result);
@@ -151,7 +179,7 @@ void free_graph(zg_graph *g)
 int build_graph(const uint32_t *d_src, const uint32_t *d_dst, const float *d_w, uint64_t m, uint32_t n,
                 uint64_t seed, zg_graph **out)
 {
-    if (m >= (1ull << 32)) return set_error(ZG_EINVAL, "graphs need fewer than 2^32 edges");
+    if (m >= (1ull << 31)) return set_error(ZG_EINVAL, "graphs need fewer than 2^31 edges (u32 list offsets)");
     if (n == 0 || n == 0xffffffffu) return set_error(ZG_EINVAL, "vertex count must be in [1, 2^32-1)");
     zg_graph *g = new zg_graph();
     struct Guard {
@@ -178,12 +206,12 @@ int build_graph(const uint32_t *d_src, const uint32_t *d_dst, const float *d_w, 
         if (hb & 1u) return set_error(ZG_EINVAL, "self-loops are not allowed in the input graph");
     }
 
-    ZG_HIP(hipMalloc(&g->edges, sizeof(uint4) * (m ? m : 1)));
+    ZG_HIP(hipMalloc(&g->edges, sizeof(ERec) * (m ? m : 1)));
     ZG_HIP(hipMalloc(&g->off, sizeof(uint64_t) * ((uint64_t)n + 1)));
     ZG_HIP(hipMalloc(&g->adj, sizeof(uint64_t) * (n2 ? n2 : 1)));
     ZG_HIP(hipMalloc(&g->sadj, sizeof(uint64_t) * (n2 ? n2 : 1)));
     if (d_w) ZG_HIP(hipMalloc(&g->w, sizeof(float) * (m ? m : 1)));
-    g->bytes = sizeof(uint4) * m + 8ull * (n + 1) + 16ull * n2 + (d_w ? 4ull * m : 0);
+    g->bytes = sizeof(ERec) * m + 8ull * (n + 1) + 16ull * n2 + (d_w ? 4ull * m : 0);
 
     if (m == 0) {   // no edges: empty lists, zero offsets
         ZG_HIP(hipMemsetAsync(g->off, 0, 8ull * (n + 1), st));
@@ -286,6 +314,18 @@ int build_graph(const uint32_t *d_src, const uint32_t *d_dst, const float *d_w, 
         ZG_HIP(hipGetLastError());
         ZG_HIP(hipStreamSynchronize(st));
     }
+    // 6. edge-set hash for the closing checks (load <= 0.5: 2 edges per 4-slot bucket)
+    {
+        const uint64_t nb = (m + 1) / 2;
+        if (nb >= (1ull << 32)) return set_error(ZG_EINVAL, "too many edges for the edge hash");
+        g->hnb = (uint32_t)nb;
+        ZG_HIP(hipMalloc(&g->htab, sizeof(HSlot) * 4 * nb));
+        ZG_HIP(hipMemsetAsync(g->htab, 0xff, sizeof(HSlot) * 4 * nb, st));
+        hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(m)), dim3(256), 0, st, g->edges, m, g->htab, g->hnb);
+        ZG_HIP(hipGetLastError());
+        ZG_HIP(hipStreamSynchronize(st));
+        g->bytes += sizeof(HSlot) * 4 * nb;
+    }
     guard.g = nullptr;
     *out = g;
     return ZG_OK;
@@ -309,14 +349,14 @@ __global__ void k_filter_flags(const float *w, uint64_t m, float thr, int keep_g
     GRID_STRIDE(r, m) { flags[r] = keep_ge ? (w[r] >= thr) : (w[r] < thr); }
 }
 
-__global__ void k_gather_kept(const uint4 *edges, const float *w, const uint32_t *sel, uint64_t k, uint32_t *s,
+__global__ void k_gather_kept(const ERec *edges, const float *w, const uint32_t *sel, uint64_t k, uint32_t *s,
                               uint32_t *d, float *wo)
 {
     GRID_STRIDE(i, k)
     {
-        const uint4 e = edges[sel[i]];
-        s[i] = e.x;
-        d[i] = e.y;
+        const ERec &e = edges[sel[i]];
+        s[i] = e.src;
+        d[i] = e.dst;
         if (w) wo[i] = w[sel[i]];
     }
 }

@@ -20,6 +20,7 @@ struct Acc {
     uint64_t ovf;
 };
 static_assert(sizeof(Acc) == 64, "Acc layout");
+static_assert(sizeof(ERec) == 32, "edge record is 32 bytes");
 static_assert(sizeof(zg_outcome) == 24, "zg_outcome is a 24-byte record");
 
 struct PlanView {
@@ -32,9 +33,11 @@ struct PlanView {
 struct zg_graph {
     uint32_t n = 0;
     uint64_t m = 0, num_edges = 0, max_degree = 0;
-    uint4 *edges = nullptr;
+    zg::ERec *edges = nullptr;
     uint64_t *off = nullptr, *adj = nullptr, *sadj = nullptr;
     float *w = nullptr;
+    zg::HSlot *htab = nullptr;
+    uint32_t hnb = 0;
     uint64_t bytes = 0;
     hipStream_t stream = nullptr;
     zg::Acc *partials = nullptr;  // scratch for block partials
@@ -44,7 +47,7 @@ struct zg_graph {
     size_t outbuf_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-    zg::GView view() const { return zg::GView{edges, off, adj, sadj, m, n, nullptr}; }
+    zg::GView view() const { return zg::GView{edges, off, adj, sadj, htab, m, n, hnb, nullptr}; }
 };
 
 struct zg_plan {

@@ -43,7 +43,7 @@ __device__ __forceinline__ bool cond_sample(const GView &g, Rng &rng, const VI (
         if (i < nv && !done) {
             const uint64_t cnt = V[i].deg - pos[i];
             if (r < cnt) {
-                idx = V[i].off + pos[i] + r;
+                idx = (uint64_t)V[i].off + pos[i] + r;
                 own = V[i].v;
                 p = pos[i] + (uint32_t)r;
                 done = true;
@@ -59,14 +59,6 @@ __device__ __forceinline__ bool cond_sample(const GView &g, Rng &rng, const VI (
     e.src = own;
     e.pos = p;
     return true;
-}
-
-// position of rank r in v's rank-ordered list, for v an endpoint of edge r
-__device__ __forceinline__ uint32_t pos_of(const GView &g, uint32_t r, uint32_t v)
-{
-    const uint4 E = g.edges[r];
-    count_reads(g, 1);
-    return E.x == v ? E.z : E.w;
 }
 
 // canonicality_check (src/SamplerGenerator.cpp:286-303,
@@ -101,32 +93,31 @@ __device__ __forceinline__ void plan_wedge(const GView &g, Rng &rng, Est &o)
     if (g.m == 0) return;
     const uint32_t r0 = (uint32_t)rng.uniform(g.m);
     o.e0 = r0;
-    const uint4 E = g.edges[r0];
-    count_reads(g, 1);
-    const VI A = vinfo(g, E.x), B = vinfo(g, E.y);
-    const uint32_t pa = E.z + 1, pb = E.w + 1;   // lower_bound(r0 + 1) in both endpoint lists
+    const ERec E = load_edge(g, r0);
+    const VI A = src_vi(E), B = dst_vi(E);
+    const uint32_t pa = E.s_pos + 1, pb = E.d_pos + 1;   // lower_bound(r0 + 1) in both endpoint lists
     const uint32_t ca = A.deg - pa, cb = B.deg - pb;
     const uint64_t s = (uint64_t)ca + cb;
     o.ncond = 1;
     if (s == 0) return;
     const uint64_t r = rng.uniform(s);
     const bool inA = r < ca;
-    const uint64_t idx = inA ? A.off + pa + r : B.off + pb + (r - ca);
+    const uint64_t idx = inA ? (uint64_t)A.off + pa + r : (uint64_t)B.off + pb + (r - ca);
     const uint64_t ent = g.adj[idx];
     count_reads(g, 1);
-    const uint32_t r1 = (uint32_t)(ent >> 32), w = (uint32_t)ent, own = inA ? E.x : E.y;
-    const int den = (w == E.x || w == E.y) ? 1 : 0;
+    const uint32_t r1 = (uint32_t)(ent >> 32), w = (uint32_t)ent, own = inA ? E.src : E.dst;
+    const int den = (w == E.src || w == E.dst) ? 1 : 0;
     U128 val{g.m, 0};
     const bool ov = mul_u128(val, s);
     if (MODE != 0) set_out(o, 0, val, ov, den);
     if (MODE == 1) return;
     // third edge of Triangle(e0, e1) (applications/Triangle.cpp:17-40)
     uint32_t tx, ty;
-    if (E.x == own) { tx = E.y; ty = w; }
-    else if (E.x == w) { tx = E.y; ty = own; }
-    else if (E.y == own) { tx = E.x; ty = w; }
-    else { tx = E.x; ty = own; }
-    const bool is_e0 = (tx == E.x && ty == E.y) || (tx == E.y && ty == E.x);
+    if (E.src == own) { tx = E.dst; ty = w; }
+    else if (E.src == w) { tx = E.dst; ty = own; }
+    else if (E.dst == own) { tx = E.src; ty = w; }
+    else { tx = E.src; ty = own; }
+    const bool is_e0 = (tx == E.src && ty == E.dst) || (tx == E.dst && ty == E.src);
     const bool is_e1 = (tx == own && ty == w) || (tx == w && ty == own);
     bool closed;
     if (is_e0 || is_e1) {
@@ -134,9 +125,7 @@ __device__ __forceinline__ void plan_wedge(const GView &g, Rng &rng, Est &o)
     } else if (tx == ty) {
         closed = false;
     } else {
-        const VI X = tx == E.x ? A : (tx == E.y ? B : vinfo(g, tx));
-        const VI Y = ty == E.x ? A : (ty == E.y ? B : vinfo(g, ty));
-        closed = close_edge(g, X, Y, r1 + 1, nullptr);
+        closed = close_pair(g, tx, ty, r1 + 1, nullptr);
     }
     if (closed) set_out(o, MODE == 2 ? 1 : 0, val, ov, den);
 }
@@ -148,13 +137,12 @@ __device__ __forceinline__ void plan_chain(const GView &g, const PlanView &P, Rn
     if (g.m == 0) return;
     const uint32_t r0 = (uint32_t)rng.uniform(g.m);
     o.e0 = r0;
-    const uint4 E = g.edges[r0];
-    count_reads(g, 1);
-    VI ends[2] = {vinfo(g, E.x), vinfo(g, E.y)};
-    uint32_t pos[2] = {E.z + 1, E.w + 1};
+    const ERec E = load_edge(g, r0);
+    VI ends[2] = {src_vi(E), dst_vi(E)};
+    uint32_t pos[2] = {E.s_pos + 1, E.d_pos + 1};
     uint32_t sr[MAXE], ss[MAXE], sd[MAXE], vs[MAXE + 1];
-    sr[0] = r0; ss[0] = E.x; sd[0] = E.y;
-    vs[0] = E.x; vs[1] = E.y;
+    sr[0] = r0; ss[0] = E.src; sd[0] = E.dst;
+    vs[0] = E.src; vs[1] = E.dst;
     int ns = 1, nv = 2;
     U128 val{g.m, 0};
     bool ov = false;
@@ -174,20 +162,21 @@ __device__ __forceinline__ void plan_chain(const GView &g, const PlanView &P, Rn
         for (int j = 0; j < MAXE + 1; ++j) in |= (j < nv && vs[j] == e.dst);
         if (!in) { if (nv < MAXE + 1) vs[nv] = e.dst; ++nv; }
         if (nv != i + 3) return;
-        const int k = ends[0].v == e.src ? 0 : 1;
-        const VI nvi = vinfo(g, e.dst);
-        const uint32_t np = lb_rank(g, nvi, 0, nvi.deg, r0 + 1);
-        if (k == 0) { ends[0] = nvi; pos[0] = np; } else { ends[1] = nvi; pos[1] = np; }
+        // the new end's lists and e's position in them; rank(e) > r0 bounds the search
+        uint32_t pe;
+        const VI nvi = end_vi(load_edge(g, e.rank), e.dst, pe);
+        const uint32_t np = lb_rank(g, nvi, 0, pe, r0 + 1);
+        if (ends[0].v == e.src) { ends[0] = nvi; pos[0] = np; } else { ends[1] = nvi; pos[1] = np; }
     }
     set_out(o, 0, val, ov, den);
 }
 
 // ---- FourChain (applications/FourChain.cpp:24-124) ------------------------
-__device__ __forceinline__ uint32_t fc_far(const uint4 &E, const SE &e1)
+__device__ __forceinline__ uint32_t fc_far(const ERec &E, const SE &e1)
 {
-    if (E.x == e1.src) return e1.dst;
-    if (E.x == e1.dst) return e1.src;
-    if (E.y == e1.src) return e1.dst;
+    if (E.src == e1.src) return e1.dst;
+    if (E.src == e1.dst) return e1.src;
+    if (E.dst == e1.src) return e1.dst;
     return e1.src;
 }
 
@@ -197,9 +186,8 @@ __device__ __forceinline__ void plan_four_chain(const GView &g, Rng &rng, Est &o
     if (g.m == 0) return;
     const uint32_t r0 = (uint32_t)rng.uniform(g.m);
     o.e0 = r0;
-    const uint4 E = g.edges[r0];
-    count_reads(g, 1);
-    const VI A = vinfo(g, E.x), B = vinfo(g, E.y);
+    const ERec E = load_edge(g, r0);
+    const VI A = src_vi(E), B = dst_vi(E);
     U128 val{g.m, 0};
     bool ov = mul_u128(val, 3);
     int den = 0;
@@ -207,36 +195,39 @@ __device__ __forceinline__ void plan_four_chain(const GView &g, Rng &rng, Est &o
     SE e1, e2;
     if (method != 3) {
         VI V[2] = {A, B};
-        uint32_t pos[2] = {E.z + 1, E.w + 1};
+        uint32_t pos[2] = {E.s_pos + 1, E.d_pos + 1};
         o.ncond = 1;
         if (!cond_sample<2>(g, rng, V, pos, 2, s, e1)) return;
-        den += (e1.dst == E.x || e1.dst == E.y);
+        den += (e1.dst == E.src || e1.dst == E.dst);
         ov |= mul_u128(val, s);
         const uint32_t far = fc_far(E, e1);
-        VI F[1] = {vinfo(g, far)};
+        VI F[1];
+        uint32_t pf;   // position of rank(e1) in far's list
+        if (far == e1.src) { F[0] = e1.src == E.src ? A : B; pf = e1.pos; }
+        else F[0] = end_vi(load_edge(g, e1.rank), far, pf);
         uint32_t fp[1];
-        if (method == 1)   // edges {e0, e1}: lower_bound(r1 + 1) right after e1 in far's list
-            fp[0] = (far == e1.src ? e1.pos : pos_of(g, e1.rank, far)) + 1;
-        else               // edges {e0}: lower_bound(r0 + 1)
-            fp[0] = lb_rank(g, F[0], 0, F[0].deg, r0 + 1);
+        if (method == 1)   // edges {e0, e1}: lower_bound(r1 + 1) is right after e1
+            fp[0] = pf + 1;
+        else               // edges {e0}: lower_bound(r0 + 1), at most pf
+            fp[0] = lb_rank(g, F[0], 0, pf, r0 + 1);
         o.ncond = 2;
         if (!cond_sample<1>(g, rng, F, fp, 1, s, e2)) return;
         den += (e2.dst == far);
         ov |= mul_u128(val, s);
-        if (e2.src == E.x || e2.src == E.y || e2.dst == E.x || e2.dst == E.y) return;
+        if (e2.src == E.src || e2.src == E.dst || e2.dst == E.src || e2.dst == E.dst) return;
         if (method == 2 && e2.rank >= e1.rank) return;
     } else {
         VI V1[1] = {A};
-        uint32_t p1[1] = {E.z + 1};
+        uint32_t p1[1] = {E.s_pos + 1};
         o.ncond = 1;
         if (!cond_sample<1>(g, rng, V1, p1, 1, s, e1)) return;
-        den += (e1.dst == E.x);
+        den += (e1.dst == E.src);
         ov |= mul_u128(val, s);
         VI V2[1] = {B};
-        uint32_t p2[1] = {E.w + 1};
+        uint32_t p2[1] = {E.d_pos + 1};
         o.ncond = 2;
         if (!cond_sample<1>(g, rng, V2, p2, 1, s, e2)) return;
-        den += (e2.dst == E.y);
+        den += (e2.dst == E.dst);
         ov |= mul_u128(val, s);
         if (e2.src == e1.src || e2.src == e1.dst || e2.dst == e1.src || e2.dst == e1.dst) return;
     }
@@ -254,17 +245,16 @@ __device__ __forceinline__ void plan_pattern(const GView &g, const PlanView &P, 
     if (g.m == 0) return;
     const uint32_t r0 = (uint32_t)rng.uniform(g.m);
     o.e0 = r0;
-    const uint4 E = g.edges[r0];
-    count_reads(g, 1);
+    const ERec E = load_edge(g, r0);
     VI V[MAXV];
     uint32_t pos[MAXV];
 #pragma unroll
     for (int i = 0; i < MAXV; ++i) { V[i] = VI{0, 0, 0}; pos[i] = 0; }
-    V[0] = vinfo(g, E.x); V[1] = vinfo(g, E.y);
-    pos[0] = E.z + 1; pos[1] = E.w + 1;
+    V[0] = src_vi(E); V[1] = dst_vi(E);
+    pos[0] = E.s_pos + 1; pos[1] = E.d_pos + 1;
     int nv = 2;
     uint32_t sr[MAXE], ss[MAXE], sd[MAXE];
-    sr[0] = r0; ss[0] = E.x; sd[0] = E.y;
+    sr[0] = r0; ss[0] = E.src; sd[0] = E.dst;
     int ns = 1;
     U128 val{g.m, 0};
     bool ov = false;
@@ -284,8 +274,9 @@ __device__ __forceinline__ void plan_pattern(const GView &g, const PlanView &P, 
         ++ns;
         if (!in) {
             if (nv >= P.pk || nv >= MAXV) return;   // |V| would exceed k
-            const VI nvi = vinfo(g, e.dst);
-            const uint32_t np = lb_rank(g, nvi, 0, nvi.deg, r0 + 1);
+            uint32_t pe;
+            const VI nvi = end_vi(load_edge(g, e.rank), e.dst, pe);
+            const uint32_t np = lb_rank(g, nvi, 0, pe, r0 + 1);   // rank(e) > r0
 #pragma unroll
             for (int j = 0; j < MAXV; ++j)
                 if (j == nv) { V[j] = nvi; pos[j] = np; }
@@ -321,11 +312,10 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
     if (g.m == 0) return;
     const uint32_t r0 = (uint32_t)rng.uniform(g.m);
     o.e0 = r0;
-    const uint4 E = g.edges[r0];
-    count_reads(g, 1);
-    const uint32_t a = E.x, b = E.y;
-    const VI A = vinfo(g, a), B = vinfo(g, b);
-    const uint32_t pa0 = E.z + 1, pb0 = E.w + 1;
+    const ERec E = load_edge(g, r0);
+    const uint32_t a = E.src, b = E.dst;
+    const VI A = src_vi(E), B = dst_vi(E);
+    const uint32_t pa0 = E.s_pos + 1, pb0 = E.d_pos + 1;
     uint64_t s1;
     SE e1;
     {
@@ -338,10 +328,10 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
     if (w == a || w == b) return;
     const bool oa = e1.src == a;
     const uint32_t ob = oa ? b : a;
-    const VI W = vinfo(g, w);
     const VI O = oa ? A : B, OB = oa ? B : A;
     const uint32_t po1 = e1.pos;                 // rank r1 in o's list
-    const uint32_t pw1 = pos_of(g, r1, w);       // rank r1 in w's list
+    uint32_t pw1;                                // rank r1 in w's list
+    const VI W = end_vi(load_edge(g, r1), w, pw1);
     const uint32_t pob0 = oa ? pb0 : pa0;        // lower_bound(r0 + 1) in ob's list
     U128 base{g.m, 0};
     const bool base_ov = mul_u128(base, s1);
@@ -359,12 +349,11 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
         if (cond_sample<3>(g, rng, V, pos, 3, s2, e2)) {
             const uint32_t d = e2.dst;
             if (d != a && d != b && d != w) {
-                const VI D = vinfo(g, d);
                 const uint32_t mv = e2.rank + 1;
-                bool ok = close_edge(g, OB, W, mv, nullptr);
-                if (ok && e2.src != a) ok = close_edge(g, A, D, mv, nullptr);
-                if (ok && e2.src != b) ok = close_edge(g, B, D, mv, nullptr);
-                if (ok && e2.src != w) ok = close_edge(g, W, D, mv, nullptr);
+                bool ok = close_pair(g, ob, w, mv, nullptr);
+                if (ok && e2.src != a) ok = close_pair(g, a, d, mv, nullptr);
+                if (ok && e2.src != b) ok = close_pair(g, b, d, mv, nullptr);
+                if (ok && e2.src != w) ok = close_pair(g, w, d, mv, nullptr);
                 if (ok) {
                     U128 v = base;
                     const bool o2 = mul_u128(v, s2);
@@ -377,11 +366,10 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
     // shape B: triangle {e0, e1, t}; e2 above rank(t) to a new vertex d; close the rest above r2
     {
         uint32_t rt;
-        if (close_edge(g, OB, W, r1 + 1, &rt)) {
-            const uint4 ET = g.edges[rt];
-            count_reads(g, 1);
-            const uint32_t pob_t = (ET.x == ob ? ET.z : ET.w) + 1;
-            const uint32_t pw_t = (ET.x == w ? ET.z : ET.w) + 1;
+        if (close_pair(g, ob, w, r1 + 1, &rt)) {
+            const ERec ET = load_edge(g, rt);
+            const uint32_t pob_t = (ET.src == ob ? ET.s_pos : ET.d_pos) + 1;
+            const uint32_t pw_t = (ET.src == w ? ET.s_pos : ET.d_pos) + 1;
             const uint32_t po = lb_rank(g, O, po1 + 1, O.deg, rt + 1);
             VI V[3] = {A, B, W};
             uint32_t pos[3] = {oa ? po : pob_t, oa ? pob_t : po, pw_t};
@@ -391,12 +379,11 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
             if (cond_sample<3>(g, rng, V, pos, 3, s3, e3)) {
                 const uint32_t d = e3.dst;
                 if (d != a && d != b && d != w) {
-                    const VI D = vinfo(g, d);
                     const uint32_t mv = e3.rank + 1;
                     bool ok = true;
-                    if (e3.src != a) ok = close_edge(g, A, D, mv, nullptr);
-                    if (ok && e3.src != b) ok = close_edge(g, B, D, mv, nullptr);
-                    if (ok && e3.src != w) ok = close_edge(g, W, D, mv, nullptr);
+                    if (e3.src != a) ok = close_pair(g, a, d, mv, nullptr);
+                    if (ok && e3.src != b) ok = close_pair(g, b, d, mv, nullptr);
+                    if (ok && e3.src != w) ok = close_pair(g, w, d, mv, nullptr);
                     if (ok) {
                         U128 v = base;
                         const bool o3 = mul_u128(v, s3);
@@ -418,10 +405,9 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
         if (cond_sample<3>(g, rng, V, pos, 3, s4, e4)) {
             const uint32_t d = e4.dst;
             if (e4.src == w && d != a && d != b && d != w && e4.rank < r1) {
-                const VI D = vinfo(g, d);
                 const uint32_t mv = r1 + 1;
-                if (close_edge(g, OB, W, mv, nullptr) && close_edge(g, A, D, mv, nullptr) &&
-                    close_edge(g, B, D, mv, nullptr)) {
+                if (close_pair(g, ob, w, mv, nullptr) && close_pair(g, a, d, mv, nullptr) &&
+                    close_pair(g, b, d, mv, nullptr)) {
                     U128 v = base;
                     const bool o4 = mul_u128(v, s4);
                     add_u128(total, v);
@@ -434,9 +420,11 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
 }
 
 // ---- accumulation ---------------------------------------------------------
+// per lane: a 192-bit running sum (three u64 limbs); per wave and beyond: sums
+// of 32-bit digits, which add without carries
 struct AccL {
-    uint64_t dig[5];
-    uint64_t succ, ovf;
+    uint64_t lo, hi, top;
+    uint32_t succ, ovf;
     double sq;
 };
 
@@ -449,17 +437,36 @@ __device__ __forceinline__ void acc_add(AccL &a, const U128 &v, int st, int den,
     const uint64_t w0 = v.lo << sh;
     const uint64_t w1 = sh ? (v.hi << sh) | (v.lo >> (64 - sh)) : v.hi;
     const uint64_t w2 = sh ? v.hi >> (64 - sh) : 0;
-    a.dig[0] += (uint32_t)w0;
-    a.dig[1] += w0 >> 32;
-    a.dig[2] += (uint32_t)w1;
-    a.dig[3] += w1 >> 32;
-    a.dig[4] += w2;
+    const uint64_t lo = a.lo + w0;
+    const uint64_t c0 = lo < a.lo;
+    const uint64_t t = a.hi + w1;
+    uint64_t c1 = t < a.hi;
+    const uint64_t hi = t + c0;
+    c1 += hi < t;
+    a.lo = lo;
+    a.hi = hi;
+    a.top += w2 + c1;
     const double x = ((double)v.hi * 18446744073709551616.0 + (double)v.lo) * ldexp(1.0, -den);
     a.sq += x * x;
 }
 
-__device__ __forceinline__ void wave_reduce(AccL &a)
+struct AccW {
+    uint64_t dig[5];
+    uint64_t succ, ovf;
+    double sq;
+};
+
+__device__ __forceinline__ AccW wave_reduce(const AccL &l)
 {
+    AccW a;
+    a.dig[0] = (uint32_t)l.lo;
+    a.dig[1] = l.lo >> 32;
+    a.dig[2] = (uint32_t)l.hi;
+    a.dig[3] = l.hi >> 32;
+    a.dig[4] = l.top;
+    a.succ = l.succ;
+    a.ovf = l.ovf;
+    a.sq = l.sq;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
@@ -468,6 +475,7 @@ __device__ __forceinline__ void wave_reduce(AccL &a)
         a.ovf += __shfl_xor(a.ovf, off, 64);
         a.sq += __shfl_xor(a.sq, off, 64);
     }
+    return a;
 }
 
 template <int NCH>
@@ -477,13 +485,13 @@ __device__ __forceinline__ void block_reduce_store(AccL (&a)[NCH], Acc *__restri
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        wave_reduce(a[c]);
+        const AccW r = wave_reduce(a[c]);
         if (lane == 0) {
 #pragma unroll
-            for (int i = 0; i < 5; ++i) sm[c][wv].dig[i] = a[c].dig[i];
-            sm[c][wv].succ = a[c].succ;
-            sm[c][wv].ovf = a[c].ovf;
-            sm[c][wv].sq = a[c].sq;
+            for (int i = 0; i < 5; ++i) sm[c][wv].dig[i] = r.dig[i];
+            sm[c][wv].succ = r.succ;
+            sm[c][wv].ovf = r.ovf;
+            sm[c][wv].sq = r.sq;
         }
     }
     __syncthreads();
@@ -504,7 +512,8 @@ __device__ __forceinline__ void block_reduce_store(AccL (&a)[NCH], Acc *__restri
     }
 }
 
-template <int KIND, int NCH>
+// OUT: also write every estimator's record (parity dumps, ELP pilots)
+template <int KIND, int NCH, bool OUT>
 __global__ __launch_bounds__(256) void k_run(GView g, PlanView P, uint64_t seed, uint64_t first,
                                              uint64_t count, zg_outcome *__restrict__ out,
                                              Acc *__restrict__ partials)
@@ -512,8 +521,7 @@ __global__ __launch_bounds__(256) void k_run(GView g, PlanView P, uint64_t seed,
     AccL acc[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-#pragma unroll
-        for (int i = 0; i < 5; ++i) acc[c].dig[i] = 0;
+        acc[c].lo = acc[c].hi = acc[c].top = 0;
         acc[c].succ = acc[c].ovf = 0;
         acc[c].sq = 0;
     }
@@ -545,7 +553,7 @@ __global__ __launch_bounds__(256) void k_run(GView g, PlanView P, uint64_t seed,
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             acc_add(acc[c], o.v[c], o.st[c], o.den[c], P.K);
-            if (out) {
+            if (OUT) {
                 zg_outcome rec;
                 const bool ok = o.st[c] != 0;
                 rec.num_lo = ok ? o.v[c].lo : 0;
@@ -601,8 +609,12 @@ static void launch_kind(const zg_graph *g, const PlanView &P, uint64_t seed, uin
 {
     GView v = g->view();
     v.reads = reads;
-    hipLaunchKernelGGL((k_run<KIND, NCH>), dim3(grid), dim3(block), 0, g->stream, v, P, seed, first, count, d_out,
-                       g->partials);
+    if (d_out)
+        hipLaunchKernelGGL((k_run<KIND, NCH, true>), dim3(grid), dim3(block), 0, g->stream, v, P, seed, first, count,
+                           d_out, g->partials);
+    else
+        hipLaunchKernelGGL((k_run<KIND, NCH, false>), dim3(grid), dim3(block), 0, g->stream, v, P, seed, first,
+                           count, d_out, g->partials);
 }
 
 int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uint64_t count,

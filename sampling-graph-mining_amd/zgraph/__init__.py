@@ -173,6 +173,13 @@ class Graph:
         check(lib().zg_graph_filter(self.h, threshold, 1 if keep_ge else 0, seed, C.byref(h)))
         return Graph(h)
 
+    def exact_count(self, what):
+        """Exact non-induced count: 'triangle', 'k4', 'wedge' or 'path3'."""
+        code = {"triangle": 0, "k4": 1, "wedge": 2, "path3": 3}[what]
+        r = C.c_uint64()
+        check(lib().zg_exact_counts(self.h, code, C.byref(r)))
+        return r.value
+
     def close(self):
         if self.h:
             lib().zg_graph_destroy(self.h)

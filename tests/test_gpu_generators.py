@@ -42,3 +42,22 @@ def test_weights_uniform():
     _, _, w = e.download()
     assert w.dtype == np.float32 and (w >= 0).all() and (w < 1).all()
     assert abs(w.mean() - 0.5) < 0.01
+
+
+@pytest.mark.parametrize("gname", ["wiki-vote", "snap-wiki-Vote", "simple-30", "powerlaw", "K9"])
+def test_device_exact_counts_equal_oracle(gname):
+    import helpers as H
+    if gname == "powerlaw":
+        n, s, d = H.power_law_graph(3000, 20000, 2.1, 7)
+    elif gname == "K9":
+        n, s, d = H.complete_graph(9)
+    else:
+        n, s, d = H.golden_graph(gname)
+    g = Z.Graph.from_edges(s, d, n, seed=4)
+    og = O.OGraph(n, s, d, seed=4)
+    assert g.exact_count("triangle") == og.triangles()
+    assert g.exact_count("k4") == og.k4()
+    assert g.exact_count("wedge") == og.wedges()
+    assert g.exact_count("path3") == og.paths3()
+    if gname == "snap-wiki-Vote":
+        assert g.exact_count("triangle") == 608389
