@@ -218,6 +218,27 @@ double zg_elp_from_pilots(const long double *est, uint64_t npts, long double F, 
                           uint64_t samp_maxdeg, uint64_t samp_m, uint64_t maxdeg, uint64_t m,
                           double eps, double conf, long double *K_out);
 
+/* ---- learned error-latency profile network ----------------------------- *
+ * No reference counterpart: a 16-64-64-64-1 ReLU MLP that predicts the
+ * estimator budget in place of the regression fit of
+ * src/ErrorLatencyProfile.cpp:127-200.  BF16 operands on MFMA, FP32
+ * accumulation, FP32 master weights, Adam.  Host pointers; X is [B][16]
+ * row-major fp32, targets [B].  Parameter order: W1[16][64], b1[64],
+ * W2[64][64], b2, W3[64][64], b3, w4[64], b4 (row-major, input-major). */
+typedef struct zg_mlp zg_mlp;
+int zg_mlp_num_params(void);
+int zg_mlp_input_dim(void);
+int zg_mlp_create(uint64_t seed, zg_mlp **out);
+int zg_mlp_destroy(zg_mlp *m);
+int zg_mlp_get_params(const zg_mlp *m, float *params);
+int zg_mlp_set_params(zg_mlp *m, const float *params);
+int zg_mlp_forward(zg_mlp *m, const float *X, int B, float *y);
+/* gradient of the mean squared error over the B rows (and the loss) */
+int zg_mlp_gradient(zg_mlp *m, const float *X, const float *T, int B, float *grad, float *loss);
+/* `steps` Adam steps on minibatches of `batch` rows drawn from the N rows */
+int zg_mlp_train(zg_mlp *m, const float *X, const float *T, int N, int steps, int batch, float lr, uint64_t seed,
+                 float *final_loss);
+
 #ifdef __cplusplus
 }
 #endif

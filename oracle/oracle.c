@@ -962,6 +962,9 @@ static int est_k4_close(og_ctx *c)
     if (c1 == 2) return 0;
     uint32_t o = e1.src, w = e1.dst, ob = (o == a) ? b : a;
     uint32_t v3[3] = {a, b, w};
+    /* every shape needs the triangle edge {ob, w} above rank(e1): test it first */
+    uint64_t rt;
+    if (!close1(c, e1.rank + 1, v3, 3, ob, w, &rt)) return 0;
     og_u128 base = c->num;                 /* m * s1 */
     int base_ov = c->overflow;
     og_u128 total = 0;
@@ -984,8 +987,7 @@ static int est_k4_close(og_ctx *c)
         if (okc) { total += c->num; ov |= c->overflow; }
     }
     /* shape B: triangle {e0, e1, t}, then e2 above rank(t) to new vertex d */
-    uint64_t rt;
-    if (close1(c, e1.rank + 1, v3, 3, ob, w, &rt)) {
+    {
         og_edge t; t.rank = rt; t.src = ob; t.dst = w;
         edges[1] = e1; edges[2] = t;
         c->num = base; c->overflow = base_ov;

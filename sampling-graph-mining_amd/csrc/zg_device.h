@@ -199,13 +199,33 @@ __device__ __forceinline__ VI end_vi(const ERec &e, uint32_t v, uint32_t &pos)
 }
 
 // first position p in [lo, hi] of v's rank-ordered list with rank >= min_valid
-// (the lower_bound of core_conditional_sample_edge, src/ZGraphInstance.cpp:147-180)
+// (the lower_bound of core_conditional_sample_edge, src/ZGraphInstance.cpp:147-180).
+// Ranks are a uniform random permutation, so an interpolation step on the known
+// key bounds [kl, ku) of the range lands within a few entries; a step that
+// fails to halve the range is followed by a bisection step (O(log n) worst case).
 __device__ __forceinline__ uint32_t lb_rank(const GView &g, const VI &x, uint32_t lo, uint32_t hi,
-                                            uint32_t min_valid)
+                                            uint32_t min_valid, uint64_t kl, uint64_t ku)
 {
-    const uint64_t key = (uint64_t)min_valid << 32;
     const uint64_t *__restrict__ a = g.adj + (uint64_t)x.off;
     uint32_t probes = 0;
+    bool bisect = false;
+    while (hi - lo > 16) {
+        const uint32_t n = hi - lo;
+        uint32_t p;
+        if (bisect || ku <= kl) {
+            p = lo + (n >> 1);
+        } else {
+            const uint64_t t = min_valid > kl ? (uint64_t)min_valid - kl : 0;
+            p = lo + (uint32_t)(((uint64_t)n * t) / (ku - kl));
+            if (p >= hi) p = hi - 1;
+        }
+        const uint32_t r = (uint32_t)(a[p] >> 32);
+        ++probes;
+        if (r < min_valid) { lo = p + 1; kl = (uint64_t)r + 1; }
+        else { hi = p; ku = r; }
+        bisect = (hi - lo) > (n >> 1);
+    }
+    const uint64_t key = (uint64_t)min_valid << 32;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         if (a[mid] < key) lo = mid + 1; else hi = mid;

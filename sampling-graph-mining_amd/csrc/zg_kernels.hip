@@ -165,7 +165,7 @@ __device__ __forceinline__ void plan_chain(const GView &g, const PlanView &P, Rn
         // the new end's lists and e's position in them; rank(e) > r0 bounds the search
         uint32_t pe;
         const VI nvi = end_vi(load_edge(g, e.rank), e.dst, pe);
-        const uint32_t np = lb_rank(g, nvi, 0, pe, r0 + 1);
+        const uint32_t np = lb_rank(g, nvi, 0, pe, r0 + 1, 0, e.rank);
         if (ends[0].v == e.src) { ends[0] = nvi; pos[0] = np; } else { ends[1] = nvi; pos[1] = np; }
     }
     set_out(o, 0, val, ov, den);
@@ -209,7 +209,7 @@ __device__ __forceinline__ void plan_four_chain(const GView &g, Rng &rng, Est &o
         if (method == 1)   // edges {e0, e1}: lower_bound(r1 + 1) is right after e1
             fp[0] = pf + 1;
         else               // edges {e0}: lower_bound(r0 + 1), at most pf
-            fp[0] = lb_rank(g, F[0], 0, pf, r0 + 1);
+            fp[0] = lb_rank(g, F[0], 0, pf, r0 + 1, 0, e1.rank);
         o.ncond = 2;
         if (!cond_sample<1>(g, rng, F, fp, 1, s, e2)) return;
         den += (e2.dst == far);
@@ -276,7 +276,7 @@ __device__ __forceinline__ void plan_pattern(const GView &g, const PlanView &P, 
             if (nv >= P.pk || nv >= MAXV) return;   // |V| would exceed k
             uint32_t pe;
             const VI nvi = end_vi(load_edge(g, e.rank), e.dst, pe);
-            const uint32_t np = lb_rank(g, nvi, 0, pe, r0 + 1);   // rank(e) > r0
+            const uint32_t np = lb_rank(g, nvi, 0, pe, r0 + 1, 0, e.rank);   // rank(e) > r0
 #pragma unroll
             for (int j = 0; j < MAXV; ++j)
                 if (j == nv) { V[j] = nvi; pos[j] = np; }
@@ -328,6 +328,9 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
     if (w == a || w == b) return;
     const bool oa = e1.src == a;
     const uint32_t ob = oa ? b : a;
+    // every shape needs the triangle edge {ob, w} above r1: test it first
+    uint32_t rt;
+    if (!close_pair(g, ob, w, r1 + 1, &rt)) return;
     const VI O = oa ? A : B, OB = oa ? B : A;
     const uint32_t po1 = e1.pos;                 // rank r1 in o's list
     uint32_t pw1;                                // rank r1 in w's list
@@ -340,7 +343,7 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
 
     // shape A: e2 above r1 from {a, b, w} to a new vertex d; close the rest above r2
     {
-        const uint32_t pob = lb_rank(g, OB, pob0, OB.deg, r1 + 1);
+        const uint32_t pob = lb_rank(g, OB, pob0, OB.deg, r1 + 1, (uint64_t)r0 + 1, g.m);
         VI V[3] = {A, B, W};
         uint32_t pos[3] = {oa ? po1 + 1 : pob, oa ? pob : po1 + 1, pw1 + 1};
         uint64_t s2;
@@ -365,12 +368,11 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
     }
     // shape B: triangle {e0, e1, t}; e2 above rank(t) to a new vertex d; close the rest above r2
     {
-        uint32_t rt;
-        if (close_pair(g, ob, w, r1 + 1, &rt)) {
+        {
             const ERec ET = load_edge(g, rt);
             const uint32_t pob_t = (ET.src == ob ? ET.s_pos : ET.d_pos) + 1;
             const uint32_t pw_t = (ET.src == w ? ET.s_pos : ET.d_pos) + 1;
-            const uint32_t po = lb_rank(g, O, po1 + 1, O.deg, rt + 1);
+            const uint32_t po = lb_rank(g, O, po1 + 1, O.deg, rt + 1, (uint64_t)r1 + 1, g.m);
             VI V[3] = {A, B, W};
             uint32_t pos[3] = {oa ? po : pob_t, oa ? pob_t : po, pw_t};
             uint64_t s3;
@@ -396,7 +398,7 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
     }
     // shape C: e2 above r0 from w to a new vertex d, below r1; close the rest above r1
     {
-        const uint32_t pw0 = lb_rank(g, W, 0, pw1, r0 + 1);
+        const uint32_t pw0 = lb_rank(g, W, 0, pw1, r0 + 1, 0, r1);
         VI V[3] = {A, B, W};
         uint32_t pos[3] = {pa0, pb0, pw0};
         uint64_t s4;

@@ -38,6 +38,8 @@ EXPORTED = [
     "zg_plan_unique_patterns",
     "zg_run", "zg_run_outcomes", "zg_count_reads", "zg_exact_counts",
     "zg_elp_regression", "zg_elp_bound", "zg_regression_k", "zg_elp_from_pilots",
+    "zg_mlp_num_params", "zg_mlp_input_dim", "zg_mlp_create", "zg_mlp_destroy", "zg_mlp_get_params",
+    "zg_mlp_set_params", "zg_mlp_forward", "zg_mlp_gradient", "zg_mlp_train",
 ]
 
 
@@ -139,6 +141,13 @@ def lib():
                                      C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double,
                                      C.c_double, P(C.c_longdouble)]
     L.zg_elp_from_pilots.restype = C.c_double
+    L.zg_mlp_create.argtypes = [C.c_uint64, P(vp)]
+    L.zg_mlp_destroy.argtypes = [vp]
+    L.zg_mlp_get_params.argtypes = [vp, vp]
+    L.zg_mlp_set_params.argtypes = [vp, vp]
+    L.zg_mlp_forward.argtypes = [vp, vp, C.c_int, vp]
+    L.zg_mlp_gradient.argtypes = [vp, vp, vp, C.c_int, vp, P(C.c_float)]
+    L.zg_mlp_train.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64, P(C.c_float)]
     _lib = L
     return L
 
