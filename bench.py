@@ -19,6 +19,8 @@ sys.path.insert(0, os.path.join(ROOT, "sampling-graph-mining_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 PEAK_HBM_GBS = 8000.0
+LINE = 128   # bytes per random read: one L2 line (all DRAM reads of these kernels are 128 B)
+PROFILE = os.path.join(ROOT, "profiles", "k_run_triangle_k4_chunglu_n4.8M_m69M_pmc.md")
 
 
 def log(*a):
@@ -96,14 +98,21 @@ def main():
     ms_per_step = wall * 1000 / args.steps
     value = 2 * N / (ms_per_step / 1000)
 
-    # roofline of the dominant kernel: algorithmic bytes = dependent reads x 64 B
+    # roofline of the dominant kernel: algorithmic bytes = dependent reads, one
+    # 128-byte line each (MI355X moves a whole L2 line per random read)
     avg_ms = {k: sum(v) / len(v) for k, v in kms.items()}
     dom = max(avg_ms, key=avg_ms.get)
     sample = 1 << 22
     reads = Z.count_reads(g, plans[dom], args.seed, 0, sample)
-    bytes_per_est = reads * 64 / sample
+    bytes_per_est = reads * LINE / sample
     achieved = bytes_per_est * N / (avg_ms[dom] / 1000) / 1e9
-    log(f"dominant {dom}: {avg_ms[dom]:.2f} ms/launch, {reads / sample:.2f} reads/estimator")
+    log(f"dominant {dom}: {avg_ms[dom]:.2f} ms/launch, {reads / sample:.3f} lines/estimator")
+    gather = {}
+    for ch in (1, 4):
+        gather[ch], _ = Z.random_gather_bandwidth(16 << 30, ch, 1 << 28, args.block, args.blocks_per_cu)
+    achievable = max(gather.values())
+    log(f"random gather: " + ", ".join(f"{c} chain(s)/lane {v:.0f} GB/s" for c, v in gather.items()))
+    traffic = profiled_traffic(dom, args, N)
 
     cpu = None
     if not args.no_cpu:
@@ -129,12 +138,33 @@ def main():
         "per_pattern": {k: {"kernel_ms": avg_ms[k], "estimators_per_s": N / (avg_ms[k] / 1000),
                             "estimate": means[k][-1]} for k in plans},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": dom, "algorithmic_bytes_per_estimator": bytes_per_est},
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "kernel": dom, "algorithmic_bytes_per_estimator": bytes_per_est,
+                     "estimators_per_launch": N,
+                     "achievable_random_gather_gbs": achievable,
+                     "frac_of_achievable": achieved / achievable},
         "cpu_baseline": cpu,
         "device": info,
     }
     print(json.dumps(out), flush=True)
+
+
+def profiled_traffic(kernel, args, N):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's
+    reading for a copy of known size), when it was taken on this workload."""
+    try:
+        with open(PROFILE) as f:
+            text = f.read()
+        rec = json.loads(text.split("```json")[1].split("```")[0])
+    except (OSError, IndexError, ValueError):
+        return None
+    w = rec.get("workload", {})
+    same = (w.get("vertices") == args.vertices and w.get("edges") == args.edges and w.get("beta") == args.beta
+            and w.get("max_degree") == args.max_degree and w.get("seed") == args.seed
+            and w.get("estimators_per_launch") == N)
+    per = rec.get("hbm_bytes_per_launch", {}).get(kernel)
+    return per if same else None
 
 
 def cpu_baseline(Z, g, plans, args, N):

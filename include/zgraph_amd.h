@@ -180,8 +180,9 @@ int zg_run_outcomes(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first
                     uint64_t count, zg_outcome *host_out, zg_sums *sums);
 
 /* Algorithmic read count of a run: the number of dependent global reads the
- * estimators of ids [first_id, first_id+count) issue (each one 64-byte
- * sector), for the roofline's algorithmic bytes.  Same outcomes as zg_run;
+ * estimators of ids [first_id, first_id+count) issue, counted in distinct
+ * 128-byte lines (MI355X's DRAM read granule), for the roofline's
+ * algorithmic bytes.  Same outcomes as zg_run;
  * slower (it counts with atomics).  No reference counterpart. */
 int zg_count_reads(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_id, uint64_t count,
                    uint64_t *reads);
@@ -191,6 +192,16 @@ int zg_count_reads(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_
  * triangles (TriangleCounting.cpp:44-72), 1 4-cliques (FourClique.cpp:44-78),
  * 2 wedges (ThreeChain.cpp:44-51), 3 three-edge paths (FourChain.cpp:46-66). */
 int zg_exact_counts(zg_graph *g, int what, uint64_t *out);
+/* Degree moments sum d^2 and sum d^3 over all vertices (ELP features). */
+int zg_graph_degree_moments(zg_graph *g, double *sum_d2, double *sum_d3);
+
+/* ---- measurement helper (no reference counterpart) ---------------------- *
+ * Achievable random-gather bandwidth: `loads` random 8-byte reads from a
+ * fresh device buffer of `buffer_bytes`, each lane running `chains` (1, 2, 4)
+ * dependent-load chains, launched like zg_run (block_size, blocks_per_cu; 0 =
+ * defaults).  *gbs counts one 128-byte line per read. */
+int zg_random_gather_bw(uint64_t buffer_bytes, int chains, uint64_t loads, int block_size, int blocks_per_cu,
+                        double *gbs, double *ms);
 
 /* ---- error-latency profile (src/ErrorLatencyProfile.cpp) --------------- */
 /* ErrorProfile<E>::get_num_estimators (:71-207) on the device: subsample at

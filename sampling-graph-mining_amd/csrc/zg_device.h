@@ -68,7 +68,8 @@ __host__ __device__ __forceinline__ uint32_t hbucket(uint64_t key, uint32_t nb)
     return (uint32_t)(((hmix(key) >> 32) * (uint64_t)nb) >> 32);
 }
 
-// counts dependent reads (one 64-byte sector each) in the read-counting launch
+// counts dependent reads in the read-counting launch, one per 128-byte L2 line
+// (the DRAM read granule of MI355X: a random 8..32-byte load moves one line)
 __device__ __forceinline__ void count_reads(const GView &g, uint32_t k)
 {
     if (g.reads) atomicAdd(g.reads, (unsigned long long)k);
@@ -173,11 +174,11 @@ struct VI {
 __device__ __forceinline__ VI vinfo(const GView &g, uint32_t v)
 {
     const uint64_t a = g.off[v], b = g.off[v + 1];
-    count_reads(g, ((v & 7u) == 7u) ? 2u : 1u);   // off[v], off[v+1] share a sector unless v%8 == 7
+    count_reads(g, ((v & 15u) == 15u) ? 2u : 1u);   // off[v], off[v+1] share a line unless v%16 == 15
     return VI{v, (uint32_t)(b - a), (uint32_t)a};
 }
 
-// the 32-byte record of rank r (two 16-byte loads of one sector)
+// the 32-byte record of rank r (two 16-byte loads of one line)
 __device__ __forceinline__ ERec load_edge(const GView &g, uint32_t r)
 {
     const uint4 *p = reinterpret_cast<const uint4 *>(g.edges + r);
@@ -208,6 +209,7 @@ __device__ __forceinline__ uint32_t lb_rank(const GView &g, const VI &x, uint32_
 {
     const uint64_t *__restrict__ a = g.adj + (uint64_t)x.off;
     uint32_t probes = 0;
+    uint64_t line = ~0ull;   // last line probed (read counting only)
     bool bisect = false;
     while (hi - lo > 16) {
         const uint32_t n = hi - lo;
@@ -220,7 +222,7 @@ __device__ __forceinline__ uint32_t lb_rank(const GView &g, const VI &x, uint32_
             if (p >= hi) p = hi - 1;
         }
         const uint32_t r = (uint32_t)(a[p] >> 32);
-        ++probes;
+        if (g.reads && ((x.off + p) >> 4) != line) { line = (x.off + p) >> 4; ++probes; }
         if (r < min_valid) { lo = p + 1; kl = (uint64_t)r + 1; }
         else { hi = p; ku = r; }
         bisect = (hi - lo) > (n >> 1);
@@ -229,7 +231,7 @@ __device__ __forceinline__ uint32_t lb_rank(const GView &g, const VI &x, uint32_
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         if (a[mid] < key) lo = mid + 1; else hi = mid;
-        ++probes;
+        if (g.reads && ((x.off + mid) >> 4) != line) { line = (x.off + mid) >> 4; ++probes; }
     }
     count_reads(g, probes);
     return lo;
@@ -237,7 +239,7 @@ __device__ __forceinline__ uint32_t lb_rank(const GView &g, const VI &x, uint32_
 
 // conditional_close for one awaited edge {x, y} (src/ZGraphInstance.cpp:371-441):
 // true iff some copy of {x, y} has rank >= min_valid; *rank receives the
-// smallest such rank.  One hash bucket (one 64-byte read) in the common case.
+// smallest such rank.  One 64-byte hash bucket (one line) in the common case.
 __device__ __forceinline__ bool close_pair(const GView &g, uint32_t x, uint32_t y, uint32_t min_valid,
                                            uint32_t *rank)
 {

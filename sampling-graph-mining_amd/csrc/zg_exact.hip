@@ -197,6 +197,29 @@ __global__ void k_wedges_paths(const uint64_t *off, const uint64_t *sadj, const 
     if (threadIdx.x == 0) { wed[blockIdx.x] = tw; paths[blockIdx.x] = tp; }
 }
 
+// degree moments: sum d^2 and sum d^3 (as doubles) per block
+__global__ void k_deg_moments(const uint64_t *off, uint32_t n, double *p2, double *p3)
+{
+    __shared__ double sm[2][4];
+    double a = 0, b = 0;
+    GRID_STRIDE(v, (uint64_t)n)
+    {
+        const double d = (double)(off[v + 1] - off[v]);
+        a += d * d;
+        b += d * d * d;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { a += __shfl_xor(a, o, 64); b += __shfl_xor(b, o, 64); }
+    if ((threadIdx.x & 63) == 0) { sm[0][threadIdx.x >> 6] = a; sm[1][threadIdx.x >> 6] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0, y = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { x += sm[0][w]; y += sm[1][w]; }
+        p2[blockIdx.x] = x;
+        p3[blockIdx.x] = y;
+    }
+}
+
 int host_sum(const unsigned long long *d, int n, hipStream_t st, unsigned long long *out)
 {
     std::vector<unsigned long long> h(n);
@@ -289,5 +312,24 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
     int rc = host_sum(part.p, grid, st, &s);
     if (rc) return rc;
     *out = s;
+    return ZG_OK;
+}
+
+extern "C" int zg_graph_degree_moments(zg_graph *g, double *sum_d2, double *sum_d3)
+{
+    if (!g || !sum_d2 || !sum_d3) return set_error(ZG_EINVAL, "null argument");
+    const int grid = gridx(g->n);
+    XB<double> p2, p3;
+    XALLOC(p2, grid); XALLOC(p3, grid);
+    hipLaunchKernelGGL(k_deg_moments, dim3(grid), dim3(256), 0, g->stream, g->off, g->n, p2.p, p3.p);
+    ZG_HIP(hipGetLastError());
+    std::vector<double> h2(grid), h3(grid);
+    ZG_HIP(hipMemcpyAsync(h2.data(), p2.p, 8ull * grid, hipMemcpyDeviceToHost, g->stream));
+    ZG_HIP(hipMemcpyAsync(h3.data(), p3.p, 8ull * grid, hipMemcpyDeviceToHost, g->stream));
+    ZG_HIP(hipStreamSynchronize(g->stream));
+    double a = 0, b = 0;
+    for (int i = 0; i < grid; ++i) { a += h2[i]; b += h3[i]; }
+    *sum_d2 = a;
+    *sum_d3 = b;
     return ZG_OK;
 }

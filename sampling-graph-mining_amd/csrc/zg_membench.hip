@@ -1,0 +1,95 @@
+// zg_membench.hip -- random-gather bandwidth microbenchmark.  Measures the
+// rate at which this GPU serves independent random 8-byte reads from a buffer
+// much larger than the last-level cache: the achievable random-access HBM
+// bandwidth that bounds the estimator kernels (each of their reads is a
+// random 8..32-byte access inside one 128-byte L2 line).  Every lane runs
+// `chains` independent dependent-load chains (the next address hashes the
+// loaded word), the shape of an estimator's read sequence.  Measurement
+// helper only; no reference counterpart.
+#include "zg_internal.h"
+
+namespace zg {
+namespace {
+
+__global__ void k_mb_fill(uint64_t *buf, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        buf[i] = hmix(i + 0x51ED27ull);
+}
+
+template <int CH>
+__global__ __launch_bounds__(256) void k_mb_gather(const uint64_t *buf, uint64_t n, uint32_t rounds, uint64_t *sink)
+{
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t x[CH];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) x[j] = hmix(gid * CH + j + 1);
+    for (uint32_t r = 0; r < rounds; ++r) {
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const uint64_t idx = __umul64hi(x[j], n);   // uniform in [0, n)
+            x[j] = hmix(x[j] ^ buf[idx]);
+        }
+    }
+    uint64_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < CH; ++j) acc ^= x[j];
+    if (acc == 0x0123456789ABCDEFull) sink[0] = acc;   // keeps the loads live
+}
+
+}  // namespace
+}  // namespace zg
+
+using namespace zg;
+
+extern "C" int zg_random_gather_bw(uint64_t buffer_bytes, int chains, uint64_t loads, int block_size,
+                                   int blocks_per_cu, double *gbs, double *ms)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!gbs || buffer_bytes < 4096 || loads == 0) return set_error(ZG_EINVAL, "bad gather benchmark arguments");
+    if (chains != 1 && chains != 2 && chains != 4) return set_error(ZG_EINVAL, "chains must be 1, 2 or 4");
+    const int bs = block_size > 0 ? block_size : 256;
+    if (bs > 256 || bs % 64) return set_error(ZG_EINVAL, "block size must be a multiple of 64 up to 256");
+    const int bpc = blocks_per_cu > 0 ? blocks_per_cu : 16;
+    const uint64_t grid = (uint64_t)device_cus() * bpc;
+    const uint64_t lanes = grid * bs;
+    const uint64_t per_lane = (loads + lanes * chains - 1) / (lanes * chains);
+    if (per_lane > 0xffffffffull) return set_error(ZG_EOVERFLOW, "too many loads");
+    const uint32_t rounds = (uint32_t)per_lane;
+    const uint64_t n = buffer_bytes / 8;
+    uint64_t *buf = nullptr, *sink = nullptr;
+    if (hipMalloc(&buf, n * 8) != hipSuccess) return set_error(ZG_ENOMEM, "gather buffer allocation failed");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto run = [&]() -> int {
+        ZG_HIP(hipMalloc(&sink, 8));
+        ZG_HIP(hipEventCreate(&e0));
+        ZG_HIP(hipEventCreate(&e1));
+        hipLaunchKernelGGL(k_mb_fill, dim3(device_cus() * 64), dim3(256), 0, 0, buf, n);
+        ZG_HIP(hipGetLastError());
+        auto launch = [&]() {
+            if (chains == 1) hipLaunchKernelGGL(k_mb_gather<1>, dim3(grid), dim3(bs), 0, 0, buf, n, rounds, sink);
+            else if (chains == 2) hipLaunchKernelGGL(k_mb_gather<2>, dim3(grid), dim3(bs), 0, 0, buf, n, rounds, sink);
+            else hipLaunchKernelGGL(k_mb_gather<4>, dim3(grid), dim3(bs), 0, 0, buf, n, rounds, sink);
+        };
+        launch();   // warm-up
+        ZG_HIP(hipGetLastError());
+        ZG_HIP(hipEventRecord(e0, 0));
+        launch();
+        ZG_HIP(hipGetLastError());
+        ZG_HIP(hipEventRecord(e1, 0));
+        ZG_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        ZG_HIP(hipEventElapsedTime(&t, e0, e1));
+        const double done = (double)lanes * chains * rounds;
+        *gbs = done * 128.0 / (t * 1e-3) / 1e9;
+        if (ms) *ms = t;
+        return ZG_OK;
+    };
+    rc = run();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (sink) (void)hipFree(sink);
+    (void)hipFree(buf);
+    return rc;
+}

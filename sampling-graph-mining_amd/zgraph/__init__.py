@@ -180,6 +180,12 @@ class Graph:
         check(lib().zg_exact_counts(self.h, code, C.byref(r)))
         return r.value
 
+    def degree_moments(self):
+        """(sum d^2, sum d^3) over all vertices"""
+        a, b = C.c_double(), C.c_double()
+        check(lib().zg_graph_degree_moments(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def close(self):
         if self.h:
             lib().zg_graph_destroy(self.h)
@@ -269,10 +275,19 @@ def run_outcomes(graph, plan, seed, first_id, count):
 
 
 def count_reads(graph, plan, seed, first_id, count):
-    """Dependent global reads (64-byte sectors) issued by these estimators."""
+    """Dependent global reads (128-byte lines) issued by these estimators."""
     r = C.c_uint64()
     check(lib().zg_count_reads(graph.h, plan.h, seed, first_id, count, C.byref(r)))
     return r.value
+
+
+def random_gather_bandwidth(buffer_bytes=16 << 30, chains=1, loads=1 << 28, block_size=0, blocks_per_cu=0):
+    """Achievable random-gather rate of this GPU in GB/s (one 128-byte line per
+    random 8-byte read from a buffer far larger than the caches)."""
+    gbs, ms = C.c_double(), C.c_double()
+    check(lib().zg_random_gather_bw(buffer_bytes, chains, loads, block_size, blocks_per_cu, C.byref(gbs),
+                                    C.byref(ms)))
+    return gbs.value, ms.value
 
 
 def clt_half_width(mean, sumsq, count, confidence=0.95):

@@ -37,6 +37,7 @@ EXPORTED = [
     "zg_plan_create", "zg_plan_destroy", "zg_plan_channels", "zg_plan_cond_calls",
     "zg_plan_unique_patterns",
     "zg_run", "zg_run_outcomes", "zg_count_reads", "zg_exact_counts",
+    "zg_graph_degree_moments", "zg_random_gather_bw",
     "zg_elp_regression", "zg_elp_bound", "zg_regression_k", "zg_elp_from_pilots",
     "zg_mlp_num_params", "zg_mlp_input_dim", "zg_mlp_create", "zg_mlp_destroy", "zg_mlp_get_params",
     "zg_mlp_set_params", "zg_mlp_forward", "zg_mlp_gradient", "zg_mlp_train",
@@ -131,6 +132,9 @@ def lib():
     L.zg_run.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, P(Launch), P(Sums), P(C.c_float)]
     L.zg_run_outcomes.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, P(Sums)]
     L.zg_exact_counts.argtypes = [vp, C.c_int, P(C.c_uint64)]
+    L.zg_graph_degree_moments.argtypes = [vp, P(C.c_double), P(C.c_double)]
+    L.zg_random_gather_bw.argtypes = [C.c_uint64, C.c_int, C.c_uint64, C.c_int, C.c_int, P(C.c_double),
+                                      P(C.c_double)]
     L.zg_count_reads.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_uint64)]
     L.zg_elp_regression.argtypes = [vp, vp, C.c_double, C.c_double, C.c_uint64, C.c_int, P(ElpResult)]
     L.zg_elp_bound.argtypes = [C.c_double] * 5 + [C.c_uint64, C.c_uint64]
