@@ -42,6 +42,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--no-elp", action="store_true", help="skip the time-to-error comparison")
+    ap.add_argument("--elp-seeds", type=int, default=5, help="independent queries per profile and pattern")
+    ap.add_argument("--elp-weights", default=None, help="learned-profile weights (default: the package's)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this size (rocprof counter calibration)")
     args = ap.parse_args()
@@ -114,6 +117,10 @@ def main():
     log(f"random gather: " + ", ".join(f"{c} chain(s)/lane {v:.0f} GB/s" for c, v in gather.items()))
     traffic = profiled_traffic(dom, args, N)
 
+    elp = None
+    if not args.no_elp:
+        elp = time_to_error(Z, g, args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(Z, g, plans, args, N)
@@ -144,9 +151,56 @@ def main():
                      "achievable_random_gather_gbs": achievable,
                      "frac_of_achievable": achieved / achievable},
         "cpu_baseline": cpu,
+        "time_to_error": elp,
         "device": info,
     }
     print(json.dumps(out), flush=True)
+
+
+def time_to_error(Z, g, args, eps=0.01, conf=0.95):
+    """Time to an answer within +-eps at `conf` confidence, regression profile
+    (ErrorProfile sizing as the reference runs it, without and with the CLT
+    judge) against the learned profile (pilot + MLP sizing + judge), several
+    independent queries each; achieved error against the device exact count."""
+    from zgraph import elp as L
+    weights = args.elp_weights or L.DEFAULT_WEIGHTS
+    learned = L.LearnedErrorProfile(weights) if os.path.exists(weights) else None
+    out = {"epsilon": eps, "confidence": conf, "queries_per_profile": args.elp_seeds,
+           "learned_weights": os.path.basename(weights) if learned else None}
+    for name, kind, what in (("triangle", Z.PLAN_TRIANGLE, 0), ("4-clique", Z.PLAN_K4_CLOSE, 1)):
+        t = time.perf_counter()
+        truth = g.exact_count(what)
+        t_exact = time.perf_counter() - t
+        log(f"exact {name}: {truth} in {t_exact:.2f}s")
+        res = {"exact": truth, "exact_count_s": t_exact}
+        profiles = [("regression", Z.ErrorProfile(), False), ("regression+judge", Z.ErrorProfile(), True)]
+        if learned is not None:
+            profiles.append(("learned+judge", learned, True))
+        for label, prof, judged in profiles:
+            recs = []
+            for q in range(args.elp_seeds):
+                inst = Z.ZGraphInstance(g, Z.Plan(kind), seed=101 + q)
+                r = L.estimate_to_error(inst, eps, conf, prof, judged=judged)
+                r["error"] = abs(r["estimate"] - truth) / truth if truth else float("nan")
+                recs.append(r)
+                log(f"{name} {label} q{q}: {r['total_s'] * 1e3:.1f} ms N={r['estimators']} "
+                    f"(sized {r['sized_estimators']}, {r['sizing']}, {r['topup_rounds']} top-ups) "
+                    f"err={r['error'] * 100:.3f}%")
+            res[label] = {
+                "mean_time_s": sum(r["total_s"] for r in recs) / len(recs),
+                "mean_sizing_s": sum(r["sizing_s"] for r in recs) / len(recs),
+                "mean_estimators": sum(r["estimators"] for r in recs) / len(recs),
+                "within_eps": sum(r["error"] <= eps for r in recs) / len(recs),
+                "max_error": max(r["error"] for r in recs),
+                "sizing_used": sorted({r["sizing"] for r in recs}),
+                "mean_topup_rounds": sum(r["topup_rounds"] for r in recs) / len(recs),
+            }
+        if learned is not None:
+            res["speedup_learned_vs_regression"] = res["regression"]["mean_time_s"] / res["learned+judge"]["mean_time_s"]
+            res["speedup_learned_vs_regression_judged"] = (res["regression+judge"]["mean_time_s"]
+                                                           / res["learned+judge"]["mean_time_s"])
+        out[name] = res
+    return out
 
 
 def profiled_traffic(kernel, args, N):

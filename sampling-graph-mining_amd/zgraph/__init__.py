@@ -359,6 +359,18 @@ class Estimation:
     def half_width(self, confidence=0.95):
         return clt_half_width(self.exact_mean(), self.sumsq, self.num_finished_estimators, confidence)
 
+    def sums_view(self, channel=0):
+        """The accumulated sums as a zg_sums-like record (count, successes,
+        sumsq, mean)."""
+        return _SumsView(self.num_finished_estimators, self.successes, self.sumsq, self.exact_mean())
+
+
+class _SumsView:
+    __slots__ = ("count", "successes", "sumsq", "mean")
+
+    def __init__(self, count, successes, sumsq, mean):
+        self.count, self.successes, self.sumsq, self.mean = count, successes, sumsq, mean
+
 
 class ZGraphInstance:
     """Base estimator instance (include/ZGraphInstance.hpp:107-140).

@@ -1,0 +1,205 @@
+"""Learned error-latency profile, CLT judge and adaptive top-up.
+
+The reference sizes a query with ErrorProfile::get_num_estimators
+(src/ErrorLatencyProfile.cpp:71-207): pilots on a 10 % edge sample, a
+through-origin regression of pilot error on the Hoeffding-style bound, and the
+bound scaled by K^2.  The learned profile replaces that sizing:
+
+  1. graph features from the device CSR (n, m, degree moments, max degree);
+  2. a pilot: the first `pilot` estimators of the query itself (ids
+     [0, pilot)), whose exact sums give the pilot mean, coefficient of
+     variation (CV) and success fraction;
+  3. the MLP (16-64-64-64-1, BF16 MFMA kernels, fp32 master weights) predicts
+     r = log10(CV_true / CV_pilot), the correction from the pilot's CV to the
+     CV of the estimator's full distribution;
+  4. N = (z * CV_pilot * 10^r / eps)^2 estimators (CLT sizing);
+  5. the judge (the rule of sampling/IEStop.cpp:47-56: stop once
+     z * sd / (sqrt(t) * mean) <= eps) checks the exact running sums and tops
+     the same id sequence up until it holds.
+
+The regression profile stays as the fallback: for plans the network was not
+trained on, for pilots with too few successes, or without a weights file.
+Because estimator ids are global and every run continues the id sequence,
+the final answer equals one run of N_final estimators.
+"""
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import (PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_THREE_CHAIN, PLAN_TRIANGLE, ErrorProfile, Estimation, _z,
+               run)
+
+IN_DIM = 16
+PATTERN_SLOTS = {PLAN_TRIANGLE: 0, PLAN_K4_CLOSE: 1, PLAN_THREE_CHAIN: 2, PLAN_FOUR_CHAIN: 3}
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "elp_weights.npz")
+MIN_PILOT_SUCCESSES = 32
+
+
+def graph_features(graph):
+    """(n, m, sum d^2, sum d^3, max degree) of the graph, cached on it."""
+    f = getattr(graph, "_elp_features", None)
+    if f is None:
+        s2, s3 = graph.degree_moments()
+        f = (float(graph.num_vertices), float(graph.num_local_edges), s2, s3, float(graph.max_degree))
+        graph._elp_features = f
+    return f
+
+
+def pilot_stats(sums):
+    """(mean, cv, success fraction) of one channel's sums."""
+    n = sums.count
+    mean = sums.mean
+    if n == 0 or mean <= 0 or sums.successes == 0:
+        return mean, float("inf"), sums.successes / max(n, 1)
+    var = max(sums.sumsq / n - mean * mean, 0.0)
+    return mean, math.sqrt(var) / mean, sums.successes / n
+
+
+def feature_vector(gf, plan_kind, pilot_n, pilot_cv, pilot_succ):
+    """The 16 network inputs (log10 scales; one-hot pattern slot)."""
+    n, m, s2, s3, dmax = gf
+    x = np.zeros(IN_DIM, np.float32)
+    lg = lambda v: math.log10(max(v, 1e-30))
+    x[0] = lg(n)
+    x[1] = lg(m)
+    x[2] = lg(2 * m / max(n, 1))
+    x[3] = lg(s2 / max(2 * m, 1))
+    x[4] = lg(s3 / max(s2, 1))
+    x[5] = lg(dmax)
+    x[6 + PATTERN_SLOTS[plan_kind]] = 1.0
+    x[10] = lg(pilot_cv)
+    x[11] = lg(pilot_succ)
+    x[12] = lg(pilot_n)
+    x[13] = lg(pilot_cv * pilot_cv * pilot_succ)   # CV^2 of the successes' part
+    x[14] = lg(s2 * s2 / max(m * s3, 1))            # tail-heaviness of the degree sequence
+    return x
+
+
+class LearnedModel:
+    """Weights + input normalisation, evaluated by the device MLP."""
+
+    def __init__(self, path=DEFAULT_WEIGHTS):
+        from .mlp import MLP
+        with np.load(path, allow_pickle=False) as z:
+            self.params = z["params"].astype(np.float32)
+            self.mu = z["mu"].astype(np.float32)
+            self.sd = z["sd"].astype(np.float32)
+            self.meta = json.loads(bytes(z["meta"]).decode()) if "meta" in z else {}
+        self.net = MLP()
+        self.net.set_params(self.params)
+
+    def predict(self, X):
+        X = (np.atleast_2d(np.asarray(X, np.float32)) - self.mu) / self.sd
+        return self.net.forward(X.astype(np.float32))
+
+
+class LearnedErrorProfile:
+    """Drop-in for ErrorProfile with learned sizing and a CLT judge.
+
+    get_num_estimators(instance, k, eps, conf) sizes the query; run(...)
+    sizes, runs, judges and tops up, returning the estimate.  `last` holds the
+    record of the latest call (sizes, rounds, times, which sizing was used).
+    """
+
+    def __init__(self, weights=DEFAULT_WEIGHTS, pilot=1 << 16, max_pilot=1 << 22, max_rounds=8,
+                 fallback=None, min_sample=1 << 12):
+        self.model = None
+        if weights and os.path.exists(weights):
+            self.model = LearnedModel(weights)
+        self.pilot = pilot
+        self.max_pilot = max_pilot
+        self.max_rounds = max_rounds
+        self.fallback = fallback if fallback is not None else ErrorProfile()
+        self.min_sample = min_sample
+        self.last = None
+
+    # -- sizing --------------------------------------------------------------
+    def _size(self, instance, eps, conf, est):
+        """Runs the pilot into `est`; returns (N, method)."""
+        kind = instance.plan.kind
+        if self.model is None or kind not in PATTERN_SLOTS:
+            return None, "regression"
+        n_p = self.pilot
+        while True:
+            instance.calculate(n_p, None, est)
+            s = est.sums_view(instance.channel)
+            if s.successes >= MIN_PILOT_SUCCESSES or n_p >= self.max_pilot:
+                break
+            n_p = min(n_p * 8, self.max_pilot)
+        mean, cv, succ = pilot_stats(s)
+        if s.successes < MIN_PILOT_SUCCESSES or not math.isfinite(cv) or cv <= 0:
+            return None, "regression"
+        x = feature_vector(graph_features(instance.graph), kind, n_p, cv, succ)
+        r = float(self.model.predict(x)[0])
+        cv_hat = cv * 10.0 ** r
+        z = _z(conf)
+        return max(int(math.ceil((z * cv_hat / eps) ** 2)), n_p), "learned"
+
+    def get_num_estimators(self, instance, num_vertices_in_pattern, required_error_rate, required_confidence):
+        est = Estimation()
+        n, how = self._size(instance, required_error_rate, required_confidence, est)
+        if n is None:
+            n = self.fallback.get_num_estimators(instance, num_vertices_in_pattern, required_error_rate,
+                                                 required_confidence)
+        return n
+
+    # -- sizing + judge + top-up ---------------------------------------------
+    def run(self, instance, num_vertices_in_pattern, required_error_rate, required_confidence,
+            init_estimation=None):
+        rec = estimate_to_error(instance, required_error_rate, required_confidence, self,
+                                init_estimation=init_estimation, num_vertices_in_pattern=num_vertices_in_pattern)
+        self.last = rec
+        return rec["estimate"]
+
+
+def judge(est, eps, conf, min_sample=1 << 12):
+    """True when the CLT interval of the running mean is within eps (relative),
+    the stopping rule of sampling/IEStop.cpp:47-56."""
+    t = est.get_num_finished_estimators()
+    mean = est.exact_mean()
+    if t < min_sample or mean <= 0:
+        return False
+    return est.half_width(conf) <= eps * mean
+
+
+def estimate_to_error(instance, eps, conf, profile, init_estimation=None, num_vertices_in_pattern=None,
+                      max_rounds=None, max_estimators=1 << 40, judged=True):
+    """Size the query with `profile` (LearnedErrorProfile or ErrorProfile),
+    run it, and (judged=True) top up until the CLT judge accepts.  Returns a
+    record with the estimate, estimator counts and wall times."""
+    t0 = time.perf_counter()
+    est = init_estimation if init_estimation is not None else Estimation()
+    how = "regression"
+    n = None
+    if isinstance(profile, LearnedErrorProfile):
+        n, how = profile._size(instance, eps, conf, est)
+        if n is None:
+            profile = profile.fallback
+    if n is None:
+        n = profile.get_num_estimators(instance, num_vertices_in_pattern, eps, conf)
+        how = "regression"
+    t_size = time.perf_counter() - t0
+    n0 = n
+    n = min(max(n, 1), max_estimators)
+    instance.calculate(n, num_vertices_in_pattern, est)
+    rounds = 0
+    rmax = max_rounds if max_rounds is not None else getattr(profile, "max_rounds", 8)
+    while judged and not judge(est, eps, conf) and rounds < rmax and n < max_estimators:
+        mean = est.exact_mean()
+        hw = est.half_width(conf)
+        if mean > 0 and math.isfinite(hw) and hw > 0:
+            grow = (hw / (eps * mean)) ** 2 * 1.05
+        else:
+            grow = 4.0
+        n = min(int(math.ceil(n * min(max(grow, 1.25), 64.0))), max_estimators)
+        instance.calculate(n, num_vertices_in_pattern, est)
+        rounds += 1
+    t1 = time.perf_counter()
+    mean = est.exact_mean()
+    hw = est.half_width(conf)
+    return {"estimate": mean, "sizing": how, "sized_estimators": n0, "estimators": est.get_num_finished_estimators(),
+            "topup_rounds": rounds, "half_width": hw, "rel_half_width": hw / mean if mean > 0 else float("inf"),
+            "accepted": judge(est, eps, conf), "sizing_s": t_size, "total_s": t1 - t0}

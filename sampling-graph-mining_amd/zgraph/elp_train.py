@@ -1,0 +1,176 @@
+"""Training data and training of the learned error-latency profile (GPU).
+
+    python -m zgraph.elp_train --graphs 256 --pilots 48 --out DIR
+
+1. Seeded graphs from the device generators (Erdos-Renyi, Chung-Lu with
+   degree exponents 2.05..3, R-MAT), 2e4..3e7 edges, built in HBM.
+2. For every (graph, pattern) with pattern in {triangle, 4-clique, 3-chain,
+   4-chain}: the exact count T from the device exact counters and a large run
+   of 2^26 estimators whose sum of squares gives the estimator's CV,
+   CV_true = sqrt(E[X^2] / T^2 - 1).
+3. `pilots` pilot runs of 2^14..2^18 estimators on disjoint id ranges give
+   one tuple each: the 16 features of zgraph.elp.feature_vector and the target
+   r = log10(CV_true / CV_pilot).
+4. The device MLP is trained on the tuples of 85 % of the graphs (Adam, MSE);
+   the rest are held out.  The weights, input normalisation and the hold-out
+   errors are written to DIR/elp_weights.npz, the tuples to DIR/elp_data.npz.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import (PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_THREE_CHAIN, PLAN_TRIANGLE, Graph, Plan, chung_lu, erdos_renyi,
+               rmat, run)
+from .elp import IN_DIM, MIN_PILOT_SUCCESSES, feature_vector, graph_features, pilot_stats
+from .mlp import MLP
+
+PATTERNS = [("triangle", PLAN_TRIANGLE, 0), ("4-clique", PLAN_K4_CLOSE, 1), ("3-chain", PLAN_THREE_CHAIN, 2),
+            ("4-chain", PLAN_FOUR_CHAIN, 3)]
+
+
+def log(*a):
+    print("[elp_train]", *a, file=sys.stderr, flush=True)
+
+
+def random_graph(rng, i):
+    """One seeded synthetic graph description and its device edges."""
+    fam = ["er", "cl", "cl", "rmat"][i % 4]
+    seed = 1000 + i
+    if fam == "rmat":
+        scale = int(rng.integers(10, 21))
+        ef = float(10 ** rng.uniform(0.5, 1.25))
+        m = int(ef * (1 << scale))
+        a = float(rng.uniform(0.45, 0.6))
+        b = c = float((1 - a) * rng.uniform(0.3, 0.45))
+        desc = {"family": "rmat", "scale": scale, "m": m, "a": a, "b": b, "c": c, "seed": seed}
+        return desc, rmat(scale, m, a, b, c, seed)
+    m = int(10 ** rng.uniform(4.3, 7.0))
+    if i % 37 == 5:
+        m = int(10 ** rng.uniform(7.0, 7.5))
+    avg = 10 ** rng.uniform(0.5, 1.8)
+    n = max(int(2 * m / avg), 64)
+    while n * (n - 1) / 2 < 3 * m:
+        n *= 2
+    if fam == "er":
+        desc = {"family": "er", "n": n, "m": m, "seed": seed}
+        return desc, erdos_renyi(n, m, seed)
+    beta = float(rng.uniform(2.05, 3.0))
+    dmax = float(min(10 ** rng.uniform(1.5, 4.4), n / 4))
+    desc = {"family": "chung_lu", "n": n, "m": m, "beta": beta, "max_degree": dmax, "seed": seed}
+    return desc, chung_lu(n, m, beta, dmax, seed)
+
+
+def collect(num_graphs, pilots, seed, big=1 << 26):
+    rng = np.random.default_rng(seed)
+    plans = {k: Plan(k) for _, k, _ in PATTERNS}
+    X, Y, G, P, NP = [], [], [], [], []
+    graphs = []
+    t0 = time.perf_counter()
+    for i in range(num_graphs):
+        try:
+            desc, edges = random_graph(rng, i)
+        except RuntimeError as e:   # e.g. a generator that could not find m distinct pairs
+            log(f"graph {i}: skipped ({e})")
+            continue
+        try:
+            g = Graph.from_device(edges, seed=desc["seed"])
+        finally:
+            edges.free()
+        gf = graph_features(g)
+        desc.update({"vertices": g.num_vertices, "edges": g.num_local_edges, "max_degree_built": g.max_degree})
+        truths = {"triangle": g.exact_count(0), "4-clique": g.exact_count(1), "3-chain": g.exact_count(2),
+                  "4-chain": g.exact_count(3)}
+        desc["truth"] = truths
+        desc["cv"] = {}
+        for name, kind, slot in PATTERNS:
+            T = truths[name]
+            if T == 0:
+                continue
+            s = run(g, plans[kind], 7, 0, big)[0]
+            ex2 = s.sumsq / big
+            cv2 = ex2 / (float(T) ** 2) - 1.0
+            if cv2 <= 0:
+                continue
+            cv_true = math.sqrt(cv2)
+            desc["cv"][name] = cv_true
+            for j in range(pilots):
+                n_p = 1 << int(rng.integers(14, 19))
+                first = (j + 1) << 32
+                ps = run(g, plans[kind], 7, first, n_p)[0]
+                if ps.successes < MIN_PILOT_SUCCESSES:
+                    continue
+                mean, cv, succ = pilot_stats(ps)
+                if not math.isfinite(cv) or cv <= 0:
+                    continue
+                X.append(feature_vector(gf, kind, n_p, cv, succ))
+                Y.append(math.log10(cv_true) - math.log10(cv))
+                G.append(i)
+                P.append(slot)
+                NP.append(n_p)
+        graphs.append(desc)
+        g.close()
+        log(f"graph {i}: {desc['family']} n={desc['vertices']} m={desc['edges']} dmax={desc['max_degree_built']} "
+            f"tuples={len(Y)} t={time.perf_counter() - t0:.1f}s")
+    return (np.array(X, np.float32), np.array(Y, np.float32), np.array(G, np.int32), np.array(P, np.int32),
+            np.array(NP, np.int64), graphs)
+
+
+def train(X, Y, G, holdout_frac=0.15, steps=40000, batch=256, lr=1e-3, seed=3):
+    graphs = np.unique(G)
+    rng = np.random.default_rng(seed)
+    hold = set(rng.choice(graphs, max(1, int(len(graphs) * holdout_frac)), replace=False).tolist())
+    te = np.array([g in hold for g in G])
+    tr = ~te
+    mu = X[tr].mean(0)
+    sd = X[tr].std(0)
+    sd[sd < 1e-6] = 1.0
+    Xn = ((X - mu) / sd).astype(np.float32)
+    net = MLP(seed)
+    t = time.perf_counter()
+    loss = net.train(Xn[tr], Y[tr], steps, batch, lr, seed)
+    dt = time.perf_counter() - t
+    pred = net.forward(Xn)
+    rmse = lambda a, b: float(np.sqrt(np.mean((a - b) ** 2)))
+    stats = {"train_rmse": rmse(pred[tr], Y[tr]), "holdout_rmse": rmse(pred[te], Y[te]),
+             "holdout_rmse_no_correction": rmse(0 * Y[te], Y[te]),
+             "holdout_rmse_mean_correction": rmse(np.full(te.sum(), Y[tr].mean()), Y[te]),
+             "final_minibatch_loss": loss, "train_seconds": dt, "steps": steps, "batch": batch, "lr": lr,
+             "train_tuples": int(tr.sum()), "holdout_tuples": int(te.sum()), "holdout_graphs": sorted(hold)}
+    return net.params(), mu.astype(np.float32), sd.astype(np.float32), stats, pred
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--graphs", type=int, default=256)
+    ap.add_argument("--pilots", type=int, default=48)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--steps", type=int, default=40000)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--data", default=None, help="train on this saved dataset instead of collecting")
+    a = ap.parse_args()
+    os.makedirs(a.out, exist_ok=True)
+    if a.data:
+        with np.load(a.data, allow_pickle=False) as z:
+            X, Y, G, P, NP = z["X"], z["Y"], z["G"], z["P"], z["NP"]
+            graphs = json.loads(bytes(z["graphs"]).decode())
+    else:
+        X, Y, G, P, NP, graphs = collect(a.graphs, a.pilots, a.seed)
+        np.savez_compressed(os.path.join(a.out, "elp_data.npz"), X=X, Y=Y, G=G, P=P, NP=NP,
+                            graphs=np.frombuffer(json.dumps(graphs).encode(), np.uint8))
+    log(f"{len(Y)} tuples from {len(graphs)} graphs")
+    params, mu, sd, stats, _ = train(X, Y, G, steps=a.steps)
+    stats.update({"tuples": int(len(Y)), "graphs": len(graphs), "in_dim": IN_DIM,
+                  "feature_min": X.min(0).tolist(), "feature_max": X.max(0).tolist()})
+    log(json.dumps({k: v for k, v in stats.items() if not isinstance(v, list)}))
+    np.savez(os.path.join(a.out, "elp_weights.npz"), params=params, mu=mu, sd=sd,
+             meta=np.frombuffer(json.dumps(stats).encode(), np.uint8))
+    print(json.dumps({k: v for k, v in stats.items() if not isinstance(v, list)}), flush=True)
+
+
+if __name__ == "__main__":
+    main()
