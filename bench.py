@@ -167,7 +167,7 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95):
     learned = L.LearnedErrorProfile(weights) if os.path.exists(weights) else None
     out = {"epsilon": eps, "confidence": conf, "queries_per_profile": args.elp_seeds,
            "learned_weights": os.path.basename(weights) if learned else None}
-    for name, kind, what in (("triangle", Z.PLAN_TRIANGLE, 0), ("4-clique", Z.PLAN_K4_CLOSE, 1)):
+    for name, kind, what in (("triangle", Z.PLAN_TRIANGLE, "triangle"), ("4-clique", Z.PLAN_K4_CLOSE, "k4")):
         t = time.perf_counter()
         truth = g.exact_count(what)
         t_exact = time.perf_counter() - t
@@ -180,7 +180,7 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95):
             recs = []
             for q in range(args.elp_seeds):
                 inst = Z.ZGraphInstance(g, Z.Plan(kind), seed=101 + q)
-                r = L.estimate_to_error(inst, eps, conf, prof, judged=judged)
+                r = L.estimate_to_error(inst, eps, conf, prof, judged=judged, max_estimators=1 << 36)
                 r["error"] = abs(r["estimate"] - truth) / truth if truth else float("nan")
                 recs.append(r)
                 log(f"{name} {label} q{q}: {r['total_s'] * 1e3:.1f} ms N={r['estimators']} "
@@ -194,6 +194,8 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95):
                 "max_error": max(r["error"] for r in recs),
                 "sizing_used": sorted({r["sizing"] for r in recs}),
                 "mean_topup_rounds": sum(r["topup_rounds"] for r in recs) / len(recs),
+                "mean_sized_estimators": sum(r["sized_estimators"] for r in recs) / len(recs),
+                "capped_at_2^36": sum(r["capped"] for r in recs),
             }
         if learned is not None:
             res["speedup_learned_vs_regression"] = res["regression"]["mean_time_s"] / res["learned+judge"]["mean_time_s"]

@@ -183,6 +183,7 @@ def estimate_to_error(instance, eps, conf, profile, init_estimation=None, num_ve
         how = "regression"
     t_size = time.perf_counter() - t0
     n0 = n
+    capped = n > max_estimators
     n = min(max(n, 1), max_estimators)
     instance.calculate(n, num_vertices_in_pattern, est)
     rounds = 0
@@ -202,4 +203,4 @@ def estimate_to_error(instance, eps, conf, profile, init_estimation=None, num_ve
     hw = est.half_width(conf)
     return {"estimate": mean, "sizing": how, "sized_estimators": n0, "estimators": est.get_num_finished_estimators(),
             "topup_rounds": rounds, "half_width": hw, "rel_half_width": hw / mean if mean > 0 else float("inf"),
-            "accepted": judge(est, eps, conf), "sizing_s": t_size, "total_s": t1 - t0}
+            "accepted": judge(est, eps, conf), "capped": capped, "sizing_s": t_size, "total_s": t1 - t0}

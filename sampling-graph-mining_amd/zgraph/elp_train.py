@@ -83,8 +83,8 @@ def collect(num_graphs, pilots, seed, big=1 << 26):
             edges.free()
         gf = graph_features(g)
         desc.update({"vertices": g.num_vertices, "edges": g.num_local_edges, "max_degree_built": g.max_degree})
-        truths = {"triangle": g.exact_count(0), "4-clique": g.exact_count(1), "3-chain": g.exact_count(2),
-                  "4-chain": g.exact_count(3)}
+        truths = {"triangle": g.exact_count("triangle"), "4-clique": g.exact_count("k4"),
+                  "3-chain": g.exact_count("wedge"), "4-chain": g.exact_count("path3")}
         desc["truth"] = truths
         desc["cv"] = {}
         for name, kind, slot in PATTERNS:
