@@ -141,6 +141,10 @@ int zg_plan_channels(const zg_plan *p);
 double zg_plan_cond_calls(const zg_plan *p);
 /* SamplerGenerator unique relabelings (src/SamplerGenerator.cpp:336-359). */
 int zg_plan_unique_patterns(const zg_plan *p, int *unique, int *total);
+/* The same count for a pattern description, on the host (no device needed):
+ * the k! relabelings and those left after Pattern::is_automorphic
+ * deduplication (src/SamplerGenerator.cpp:225-242, 311-363). */
+int zg_pattern_unique(const zg_plan_desc *desc, int *unique, int *total);
 
 /* ---- runs (calculate) -------------------------------------------------- */
 typedef struct {
@@ -230,8 +234,9 @@ double zg_elp_from_pilots(const long double *est, uint64_t npts, long double F, 
                           double eps, double conf, long double *K_out);
 
 /* ---- learned error-latency profile network ----------------------------- *
- * No reference counterpart: a 16-64-64-64-1 ReLU MLP that predicts the
- * estimator budget in place of the regression fit of
+ * No reference counterpart: a 16-64-64-64-1 ReLU MLP whose prediction (the
+ * log-ratio of the estimator's coefficient of variation to a pilot's) sizes
+ * the estimator budget in place of the regression fit of
  * src/ErrorLatencyProfile.cpp:127-200.  BF16 operands on MFMA, FP32
  * accumulation, FP32 master weights, Adam.  Host pointers; X is [B][16]
  * row-major fp32, targets [B].  Parameter order: W1[16][64], b1[64],

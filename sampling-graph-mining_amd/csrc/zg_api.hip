@@ -221,7 +221,56 @@ inline int pair_bit(uint32_t a, uint32_t b)
     return (int)(hi * (hi - 1) / 2 + lo);
 }
 
+// unique relabelings of a pattern (SamplerGenerator ctor, src/SamplerGenerator.cpp:311-363):
+// the edge masks of the relabelings that survive Pattern::is_automorphic (:225-242)
+// deduplication, in generate_mapping order.  Host only.
+int pattern_classes(const zg_plan_desc *d, std::vector<uint32_t> &uniq_masks, int *ntotal)
+{
+    const int k = d->pattern_vertices, E = d->pattern_edges;
+    if (k < 2 || k > 6) return set_error(ZG_EINVAL, "pattern vertices must be in [2, 6]");
+    if (E < 1 || E > ZG_MAX_PATTERN_EDGES) return set_error(ZG_EINVAL, "pattern edges must be in [1, 15]");
+    PatEdges base;
+    for (int i = 0; i < E; ++i) {
+        const uint32_t a = d->pattern_src[i], b = d->pattern_dst[i];
+        if ((int)a >= k || (int)b >= k || a == b) return set_error(ZG_EINVAL, "bad pattern edge");
+        base.e.push_back({a, b});
+    }
+    std::vector<PatEdges> all;
+    std::vector<uint32_t> map(k);
+    std::vector<int> used(k, 0);
+    perms(k, 0, map, used, base, all);
+    std::vector<char> dup(all.size(), 0);
+    uniq_masks.clear();
+    for (size_t i = 0; i < all.size(); ++i) {
+        for (size_t j = 0; j < i; ++j) {
+            if (dup[j]) continue;
+            bool same = true;
+            for (auto &x : all[i].e) same = same && pat_has(all[j], x.first, x.second);
+            if (same) { dup[i] = 1; break; }
+        }
+        if (!dup[i]) {
+            uint32_t mk = 0;
+            for (auto &x : all[i].e) mk |= 1u << pair_bit(x.first, x.second);
+            uniq_masks.push_back(mk);
+        }
+    }
+    *ntotal = (int)all.size();
+    return ZG_OK;
+}
+
 }  // namespace
+
+extern "C" int zg_pattern_unique(const zg_plan_desc *d, int *unique, int *total)
+{
+    if (!d) return set_error(ZG_EINVAL, "null argument");
+    std::vector<uint32_t> masks;
+    int nt = 0;
+    int rc = pattern_classes(d, masks, &nt);
+    if (rc) return rc;
+    if (unique) *unique = (int)masks.size();
+    if (total) *total = nt;
+    return ZG_OK;
+}
 
 extern "C" int zg_plan_create(const zg_plan_desc *d, zg_plan **out)
 {
@@ -239,36 +288,10 @@ extern "C" int zg_plan_create(const zg_plan_desc *d, zg_plan **out)
     case ZG_PLAN_K4_CLOSE: p->K = 0; p->cond_calls = 2; break;
     case ZG_PLAN_PATTERN: {
         const int k = d->pattern_vertices, E = d->pattern_edges;
-        if (k < 2 || k > 6) return set_error(ZG_EINVAL, "pattern vertices must be in [2, 6]");
-        if (E < 1 || E > ZG_MAX_PATTERN_EDGES) return set_error(ZG_EINVAL, "pattern edges must be in [1, 15]");
-        PatEdges base;
-        for (int i = 0; i < E; ++i) {
-            const uint32_t a = d->pattern_src[i], b = d->pattern_dst[i];
-            if ((int)a >= k || (int)b >= k || a == b) return set_error(ZG_EINVAL, "bad pattern edge");
-            base.e.push_back({a, b});
-        }
-        // unique relabelings (SamplerGenerator ctor, src/SamplerGenerator.cpp:311-363)
-        std::vector<PatEdges> all;
-        std::vector<uint32_t> map(k);
-        std::vector<int> used(k, 0);
-        perms(k, 0, map, used, base, all);
-        std::vector<char> dup(all.size(), 0);
         std::vector<uint32_t> uniq_masks;
-        for (size_t i = 0; i < all.size(); ++i) {
-            for (size_t j = 0; j < i; ++j) {
-                if (dup[j]) continue;
-                bool same = true;   // Pattern::is_automorphic (:225-242)
-                for (auto &x : all[i].e) same = same && pat_has(all[j], x.first, x.second);
-                if (same) { dup[i] = 1; break; }
-            }
-            if (!dup[i]) {
-                uint32_t mk = 0;
-                for (auto &x : all[i].e) mk |= 1u << pair_bit(x.first, x.second);
-                uniq_masks.push_back(mk);
-            }
-        }
+        int rc = pattern_classes(d, uniq_masks, &p->ntotal);
+        if (rc) return rc;
         p->nuniq = (int)uniq_masks.size();
-        p->ntotal = (int)all.size();
         const int nbits = k * (k - 1) / 2;
         std::vector<uint32_t> table(((1u << nbits) + 31) / 32, 0);
         for (uint32_t P : uniq_masks)
@@ -276,7 +299,7 @@ extern "C" int zg_plan_create(const zg_plan_desc *d, zg_plan **out)
                 table[s >> 5] |= 1u << (s & 31);
                 if (!s) break;
             }
-        int rc = ensure_init();
+        rc = ensure_init();
         if (rc) return rc;
         ZG_HIP(hipMalloc(&p->d_accept, 4 * table.size()));
         ZG_HIP(hipMemcpy(p->d_accept, table.data(), 4 * table.size(), hipMemcpyHostToDevice));

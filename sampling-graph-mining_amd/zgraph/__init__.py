@@ -212,6 +212,25 @@ class Pattern:
     def num_edges(self):
         return len(self.edges)
 
+    def desc(self):
+        d = _capi.PlanDesc()
+        d.kind = PLAN_PATTERN
+        if self.num_edges > _capi.MAX_PATTERN_EDGES:
+            raise ValueError("pattern too large")
+        d.pattern_vertices = self.num_vertices
+        d.pattern_edges = self.num_edges
+        for i, (a, b) in enumerate(self.edges):
+            d.pattern_src[i] = a
+            d.pattern_dst[i] = b
+        return d
+
+    def unique_relabelings(self):
+        """(unique, total) relabelings after is_automorphic deduplication
+        (src/SamplerGenerator.cpp:311-363); host only."""
+        u, t = C.c_int(), C.c_int()
+        check(lib().zg_pattern_unique(C.byref(self.desc()), C.byref(u), C.byref(t)))
+        return u.value, t.value
+
 
 TRIANGLE_PATTERN = Pattern(3, [(0, 1), (0, 2), (1, 2)])                              # Triangle2.cpp:18-24
 FOUR_CLIQUE_PATTERN = Pattern(4, [(1, 0), (2, 0), (2, 1), (3, 0), (3, 1), (3, 2)])    # TestPattern2.cpp:111-122

@@ -35,7 +35,7 @@ EXPORTED = [
     "zg_gen_erdos_renyi", "zg_gen_chung_lu", "zg_gen_rmat", "zg_edges_add_uniform_weights",
     "zg_edges_download", "zg_edges_free",
     "zg_plan_create", "zg_plan_destroy", "zg_plan_channels", "zg_plan_cond_calls",
-    "zg_plan_unique_patterns",
+    "zg_plan_unique_patterns", "zg_pattern_unique",
     "zg_run", "zg_run_outcomes", "zg_count_reads", "zg_exact_counts",
     "zg_graph_degree_moments", "zg_random_gather_bw",
     "zg_elp_regression", "zg_elp_bound", "zg_regression_k", "zg_elp_from_pilots",
@@ -129,6 +129,7 @@ def lib():
     L.zg_plan_cond_calls.argtypes = [vp]
     L.zg_plan_cond_calls.restype = C.c_double
     L.zg_plan_unique_patterns.argtypes = [vp, P(C.c_int), P(C.c_int)]
+    L.zg_pattern_unique.argtypes = [P(PlanDesc), P(C.c_int), P(C.c_int)]
     L.zg_run.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, P(Launch), P(Sums), P(C.c_float)]
     L.zg_run_outcomes.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, P(Sums)]
     L.zg_exact_counts.argtypes = [vp, C.c_int, P(C.c_uint64)]

@@ -106,3 +106,17 @@ def test_clt_half_width():
     # Bernoulli(1/2) scaled by 2: mean 1, variance 1
     hw = Z.clt_half_width(1.0, 2.0 * 100, 100, 0.95)
     assert hw == pytest.approx(1.959964 * math.sqrt(100 / 99) / 10, rel=1e-5)
+
+
+@pytest.mark.parametrize("pattern,expect", [
+    (Z.Pattern(4, [(0, 1), (1, 2), (2, 3)]), (12, 24)),      # tests/TestPattern2.cpp:100-108 (4-chain)
+    (Z.FOUR_CLIQUE_PATTERN, (1, 24)),                           # tests/TestPattern2.cpp:111-122
+    (Z.FIVE_STAR_PATTERN, (5, 120)),                            # tests/TestPattern2.cpp:127-137
+    (Z.FIVE_HOUSE_PATTERN, (60, 120)),
+    (Z.TRIANGLE_PATTERN, (1, 6)),
+])
+def test_pattern_unique_relabelings_product(pattern, expect):
+    """The product's host-side dedupe (zg_pattern_unique) reproduces the
+    reference's TestPattern2 counts and agrees with the oracle."""
+    assert pattern.unique_relabelings() == expect
+    assert O.pattern_unique(pattern.num_vertices, pattern.edges) == expect
