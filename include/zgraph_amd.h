@@ -191,7 +191,7 @@ int zg_run_outcomes(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first
 int zg_count_reads(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_id, uint64_t count,
                    uint64_t *reads);
 
-/* ---- exact counts (naive_implementation/*.cpp semantics) --------------- *
+/* ---- exact counts (semantics of the naive_implementation programs) ----- *
  * Non-induced counts on the simple graph underlying the edge list: what = 0
  * triangles (TriangleCounting.cpp:44-72), 1 4-cliques (FourClique.cpp:44-78),
  * 2 wedges (ThreeChain.cpp:44-51), 3 three-edge paths (FourChain.cpp:46-66). */

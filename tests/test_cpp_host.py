@@ -1,0 +1,129 @@
+"""The C++ host API (sampling-graph-mining_amd/host/ZGraph.hpp): its own
+host-side test program, the application mains failing loudly without a GPU,
+and (GPU) every application main against the Python API on the same graph,
+seed and estimator ids."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import zgraph as Z
+from helpers import golden_graph, golden_weights
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST = os.path.join(ROOT, "sampling-graph-mining_amd", "host")
+BIN = os.path.join(HOST, "_bin")
+
+
+def _bin(name):
+    path = os.path.join(BIN, name)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} is not built (make -C sampling-graph-mining_amd/host)")
+    return path
+
+
+def _write_graph(tmp_path, name, n, src, dst, weights=None):
+    prefix = str(tmp_path / name)
+    with open(prefix + ".csv", "w") as f:
+        f.write("num_vertices,num_edges\n%d,%d\n" % (n, len(src)))
+    Z.save_biedgelist(prefix + ".biedgelist", src, dst, weights)
+    return prefix
+
+
+def _report(out):
+    rows = {}
+    for line in out.splitlines():
+        m = re.match(r"^(\S+) (\S+) (\d+)$", line.strip())
+        if m:
+            rows[m.group(1)] = (float(m.group(2)), int(m.group(3)))
+    return rows
+
+
+def test_host_api_program(tmp_path):
+    r = subprocess.run([_bin("test_host_api"), str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "regression k^2 = 609.8565" in r.stdout
+    assert r.stdout.strip().endswith("OK")
+
+
+def test_apps_fail_loudly_without_gpu(tmp_path):
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    n, s, d = golden_graph("simple-30")
+    prefix = _write_graph(tmp_path, "g", n, s, d)
+    r = subprocess.run([_bin("Triangle"), prefix, "1000"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "error" in r.stderr
+
+
+APPS = [
+    # (binary, extra args, report name, python instance factory)
+    ("Triangle", [], "triangle", lambda g: Z.Plan(Z.PLAN_TRIANGLE)),
+    ("FourClique", [], "4-clique", lambda g: Z.Plan(Z.PLAN_K4_CLOSE)),
+    ("ThreeChain", [], "3-chain", lambda g: Z.Plan(Z.PLAN_THREE_CHAIN)),
+    ("FourChain", [], "4-chain", lambda g: Z.Plan(Z.PLAN_FOUR_CHAIN)),
+    ("ChainMining", ["5"], "chain", lambda g: Z.Plan(Z.PLAN_CHAIN, order=5)),
+    ("FiveStar", [], "5-star", lambda g: Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_STAR_PATTERN)),
+    ("FiveHouse", [], "5-house", lambda g: Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_HOUSE_PATTERN)),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("app,extra,name,plan", APPS, ids=[a[0] for a in APPS])
+def test_app_matches_python_api(tmp_path, app, extra, name, plan):
+    n, s, d = golden_graph("wiki-vote")
+    prefix = _write_graph(tmp_path, "wv", n, s, d)
+    N, seed = 1 << 20, 5
+    print(f"[cpp] {app} N={N} seed={seed}", flush=True)
+    r = subprocess.run([_bin(app), prefix, str(N), str(seed)] + extra, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    est, cnt = _report(r.stdout)[name]
+    assert cnt == N
+    g = Z.Graph.from_edges(s, d, n, seed=1)          # the loader's default shuffle seed
+    ref = Z.run(g, plan(g), seed, 0, N)[0]
+    exact = ref.total() / 2 ** ref.log2_den / N
+    assert est == pytest.approx(exact, rel=1e-13, abs=1e-9)
+
+
+@pytest.mark.gpu
+def test_three_motif_app(tmp_path):
+    n, s, d = golden_graph("wiki-vote")
+    prefix = _write_graph(tmp_path, "wv", n, s, d)
+    N, seed = 1 << 20, 3
+    r = subprocess.run([_bin("ThreeMotif"), prefix, str(N), str(seed)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = _report(r.stdout)
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    ref = Z.run(g, Z.Plan(Z.PLAN_THREE_MOTIF), seed, 0, N)
+    assert rows["3-chain"][0] == pytest.approx(ref[0].mean, rel=1e-12)
+    assert rows["triangle"][0] == pytest.approx(ref[1].mean, rel=1e-12)
+
+
+@pytest.mark.gpu
+def test_predicate_apps(tmp_path):
+    """All-predicate and at-least-one weighted triangles on the reference's
+    weighted wiki-vote (u32 weights in [0, 79)): the C++ mains equal the
+    Python API on the filtered graphs, and at-least-one = all - none."""
+    n, s, d = golden_graph("wiki-vote")
+    w = golden_weights("wiki-vote").astype(np.uint32)
+    prefix = _write_graph(tmp_path, "wvw", n, s, d, w)
+    N, seed = 1 << 20, 2
+    r = subprocess.run([_bin("AllPredicateWeightedTriangle"), prefix, str(N), str(seed)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    est_all = _report(r.stdout)["all-predicate-triangle"][0]
+    g = Z.Graph.from_edges(s, d, n, weights=w.astype(np.float32), seed=1)
+    gf = g.filter(50.0, True, 1)
+    ref = Z.run(gf, Z.Plan(Z.PLAN_TRIANGLE), seed, 0, N)[0]
+    assert est_all == pytest.approx(ref.total() / 2 ** ref.log2_den / N, rel=1e-13)
+    r = subprocess.run([_bin("AtLeastOnePredicateWeightedTriangle"), prefix, str(N), str(seed)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    est_one = _report(r.stdout)["at-least-one-predicate-triangle"][0]
+    ga = Z.run(g, Z.Plan(Z.PLAN_TRIANGLE), seed, 0, N)[0]
+    gn = Z.run(g.filter(50.0, False, 1), Z.Plan(Z.PLAN_TRIANGLE), seed, 0, N)[0]
+    assert est_one == pytest.approx(ga.mean - gn.mean, rel=1e-9, abs=1e-6)
