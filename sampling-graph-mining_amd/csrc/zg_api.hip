@@ -499,12 +499,20 @@ extern "C" int zg_elp_regression(zg_graph *g, const zg_plan *p, double eps, doub
     if (p->nch != 1) return set_error(ZG_EINVAL, "the ELP sizes single-channel plans");
     if (pilot_scale < 0 || pilot_scale > 10) return set_error(ZG_EINVAL, "pilot_scale must be in [0, 10]");
     const auto t0 = std::chrono::steady_clock::now();
-    // get_num_conditional_sample_edge_calls: 1024 estimates, NaN without a success (:470-488)
+    // get_num_conditional_sample_edge_calls (:470-488) runs 1024 estimates and
+    // returns NaN when none succeeds, which makes the reference's budget NaN.
+    // Here b is a constant of the plan and the pilot only guards that case; it
+    // widens (x32, up to 2^20 estimates) before reporting it, since 1024
+    // estimates can all fail on large sparse graphs.
     zg_sums s1;
-    int rc = zg_run(g, p, seed ^ 0x9E3779B97F4A7C15ull, 0, 1024, nullptr, &s1, nullptr);
-    if (rc) return rc;
-    if (s1.successes == 0)
-        return set_error(ZG_ESTATE, "no successful estimator among 1024 pilots (the reference's budget is NaN)");
+    int rc = ZG_OK;
+    for (uint64_t pil = 1024;; pil <<= 5) {
+        rc = zg_run(g, p, seed ^ 0x9E3779B97F4A7C15ull, 0, pil, nullptr, &s1, nullptr);
+        if (rc) return rc;
+        if (s1.successes) break;
+        if (pil >= (1ull << 20))
+            return set_error(ZG_ESTATE, "no successful estimator among 2^20 pilots (the reference's budget is NaN)");
+    }
     const double a = 1.0, b = p->cond_calls;
     zg_graph *sg = nullptr;
     rc = subsample_graph(g, 0.1, seed, &sg);   // sample_rate 0.1 (:83)

@@ -36,6 +36,7 @@ IN_DIM = 16
 PATTERN_SLOTS = {PLAN_TRIANGLE: 0, PLAN_K4_CLOSE: 1, PLAN_THREE_CHAIN: 2, PLAN_FOUR_CHAIN: 3}
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "elp_weights.npz")
 MIN_PILOT_SUCCESSES = 32
+MAX_PILOT = 1 << 22
 
 
 def graph_features(graph):
@@ -104,7 +105,7 @@ class LearnedErrorProfile:
     record of the latest call (sizes, rounds, times, which sizing was used).
     """
 
-    def __init__(self, weights=DEFAULT_WEIGHTS, pilot=1 << 16, max_pilot=1 << 22, max_rounds=8,
+    def __init__(self, weights=DEFAULT_WEIGHTS, pilot=1 << 16, max_pilot=MAX_PILOT, max_rounds=8,
                  fallback=None, min_sample=1 << 12):
         self.model = None
         if weights and os.path.exists(weights):

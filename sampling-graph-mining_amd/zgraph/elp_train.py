@@ -1,6 +1,6 @@
 """Training data and training of the learned error-latency profile (GPU).
 
-    python -m zgraph.elp_train --graphs 256 --pilots 48 --out DIR
+    python -m zgraph.elp_train --graphs 256 --pilots 64 --out DIR
 
 1. Seeded graphs from the device generators (Erdos-Renyi, Chung-Lu with
    degree exponents 2.05..3, R-MAT), 2e4..3e7 edges, built in HBM.
@@ -8,11 +8,13 @@
    4-chain}: the exact count T from the device exact counters and a large run
    of 2^26 estimators whose sum of squares gives the estimator's CV,
    CV_true = sqrt(E[X^2] / T^2 - 1).
-3. `pilots` pilot runs of 2^14..2^18 estimators on disjoint id ranges give
-   one tuple each: the 16 features of zgraph.elp.feature_vector and the target
+3. `pilots` pilot runs on disjoint id ranges, each sized as the learned
+   profile sizes its pilot (2^14..2^18 estimators, widened x8 until 32
+   successes, at most 2^22), give one tuple each: the 16 features of zgraph.elp.feature_vector and the target
    r = log10(CV_true / CV_pilot).
-4. The device MLP is trained on the tuples of 85 % of the graphs (Adam, MSE);
-   the rest are held out.  The weights, input normalisation and the hold-out
+4. The device MLP is trained on the tuples of 70 % of the graphs (Adam,
+   MSE), the parameters kept from the training segment with the lowest error
+   on 15 % of the graphs, and scored on the last 15 %.  The weights, input normalisation and the hold-out
    errors are written to DIR/elp_weights.npz, the tuples to DIR/elp_data.npz.
 """
 import argparse
@@ -26,7 +28,7 @@ import numpy as np
 
 from . import (PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_THREE_CHAIN, PLAN_TRIANGLE, Graph, Plan, chung_lu, erdos_renyi,
                rmat, run)
-from .elp import IN_DIM, MIN_PILOT_SUCCESSES, feature_vector, graph_features, pilot_stats
+from .elp import IN_DIM, MAX_PILOT, MIN_PILOT_SUCCESSES, feature_vector, graph_features, pilot_stats
 from .mlp import MLP
 
 PATTERNS = [("triangle", PLAN_TRIANGLE, 0), ("4-clique", PLAN_K4_CLOSE, 1), ("3-chain", PLAN_THREE_CHAIN, 2),
@@ -99,9 +101,15 @@ def collect(num_graphs, pilots, seed, big=1 << 26):
             cv_true = math.sqrt(cv2)
             desc["cv"][name] = cv_true
             for j in range(pilots):
+                # the pilot procedure of LearnedErrorProfile: start at 2^14..2^18
+                # estimators, widen x8 until 32 successes (at most 2^22)
                 n_p = 1 << int(rng.integers(14, 19))
                 first = (j + 1) << 32
-                ps = run(g, plans[kind], 7, first, n_p)[0]
+                while True:
+                    ps = run(g, plans[kind], 7, first, n_p)[0]
+                    if ps.successes >= MIN_PILOT_SUCCESSES or n_p >= MAX_PILOT:
+                        break
+                    n_p = min(n_p * 8, MAX_PILOT)
                 if ps.successes < MIN_PILOT_SUCCESSES:
                     continue
                 mean, cv, succ = pilot_stats(ps)
@@ -120,34 +128,49 @@ def collect(num_graphs, pilots, seed, big=1 << 26):
             np.array(NP, np.int64), graphs)
 
 
-def train(X, Y, G, holdout_frac=0.15, steps=40000, batch=256, lr=1e-3, seed=3):
+def train(X, Y, G, steps=40000, segment=2000, batch=256, lr=1e-3, seed=3):
+    """Adam on the tuples of 70 % of the graphs; the parameters after the
+    segment with the lowest validation error (15 % of the graphs) are kept;
+    the remaining 15 % of the graphs are the test set."""
     graphs = np.unique(G)
     rng = np.random.default_rng(seed)
-    hold = set(rng.choice(graphs, max(1, int(len(graphs) * holdout_frac)), replace=False).tolist())
-    te = np.array([g in hold for g in G])
-    tr = ~te
+    rng.shuffle(graphs)
+    k = max(1, len(graphs) * 15 // 100)
+    te = np.isin(G, graphs[:k])
+    va = np.isin(G, graphs[k:2 * k])
+    tr = ~(te | va)
     mu = X[tr].mean(0)
     sd = X[tr].std(0)
     sd[sd < 1e-6] = 1.0
     Xn = ((X - mu) / sd).astype(np.float32)
+    rmse = lambda a, b: float(np.sqrt(np.mean((a - b) ** 2))) if len(b) else float("nan")
     net = MLP(seed)
     t = time.perf_counter()
-    loss = net.train(Xn[tr], Y[tr], steps, batch, lr, seed)
+    best = (float("inf"), None, 0)
+    curve = []
+    for s in range(max(1, steps // segment)):
+        net.train(Xn[tr], Y[tr], segment, batch, lr, seed * 1000 + s)
+        v = rmse(net.forward(Xn[va]), Y[va])
+        curve.append(v)
+        if v < best[0]:
+            best = (v, net.params(), (s + 1) * segment)
     dt = time.perf_counter() - t
+    net.set_params(best[1])
     pred = net.forward(Xn)
-    rmse = lambda a, b: float(np.sqrt(np.mean((a - b) ** 2)))
-    stats = {"train_rmse": rmse(pred[tr], Y[tr]), "holdout_rmse": rmse(pred[te], Y[te]),
-             "holdout_rmse_no_correction": rmse(0 * Y[te], Y[te]),
-             "holdout_rmse_mean_correction": rmse(np.full(te.sum(), Y[tr].mean()), Y[te]),
-             "final_minibatch_loss": loss, "train_seconds": dt, "steps": steps, "batch": batch, "lr": lr,
-             "train_tuples": int(tr.sum()), "holdout_tuples": int(te.sum()), "holdout_graphs": sorted(hold)}
-    return net.params(), mu.astype(np.float32), sd.astype(np.float32), stats, pred
+    stats = {"train_rmse": rmse(pred[tr], Y[tr]), "validation_rmse": rmse(pred[va], Y[va]),
+             "test_rmse": rmse(pred[te], Y[te]), "test_rmse_no_correction": rmse(0 * Y[te], Y[te]),
+             "test_rmse_mean_correction": rmse(np.full(te.sum(), Y[tr].mean()), Y[te]),
+             "test_rmse_by_pattern": {}, "chosen_steps": best[2], "validation_curve": curve,
+             "train_seconds": dt, "steps": steps, "batch": batch, "lr": lr, "train_tuples": int(tr.sum()),
+             "validation_tuples": int(va.sum()), "test_tuples": int(te.sum()),
+             "test_graphs": sorted(int(g) for g in graphs[:k])}
+    return net.params(), mu.astype(np.float32), sd.astype(np.float32), stats, pred, te
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--graphs", type=int, default=256)
-    ap.add_argument("--pilots", type=int, default=48)
+    ap.add_argument("--pilots", type=int, default=64)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--steps", type=int, default=40000)
     ap.add_argument("--out", required=True)
@@ -163,7 +186,14 @@ def main():
         np.savez_compressed(os.path.join(a.out, "elp_data.npz"), X=X, Y=Y, G=G, P=P, NP=NP,
                             graphs=np.frombuffer(json.dumps(graphs).encode(), np.uint8))
     log(f"{len(Y)} tuples from {len(graphs)} graphs")
-    params, mu, sd, stats, _ = train(X, Y, G, steps=a.steps)
+    params, mu, sd, stats, pred, te = train(X, Y, G, steps=a.steps)
+    names = ["triangle", "4-clique", "3-chain", "4-chain"]
+    for slot in range(4):
+        sel = te & (P == slot)
+        if sel.any():
+            stats["test_rmse_by_pattern"][names[slot]] = {
+                "tuples": int(sel.sum()), "model": float(np.sqrt(np.mean((pred[sel] - Y[sel]) ** 2))),
+                "no_correction": float(np.sqrt(np.mean(Y[sel] ** 2)))}
     stats.update({"tuples": int(len(Y)), "graphs": len(graphs), "in_dim": IN_DIM,
                   "feature_min": X.min(0).tolist(), "feature_max": X.max(0).tolist()})
     log(json.dumps({k: v for k, v in stats.items() if not isinstance(v, list)}))
