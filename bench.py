@@ -43,6 +43,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--no-elp", action="store_true", help="skip the time-to-error comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-query comparison")
     ap.add_argument("--elp-seeds", type=int, default=5, help="independent queries per profile and pattern")
     ap.add_argument("--elp-weights", default=None, help="learned-profile weights (default: the package's)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -121,6 +122,10 @@ def main():
     if not args.no_elp:
         elp = time_to_error(Z, g, args)
 
+    batch = None
+    if not args.no_batch:
+        batch = batched_queries(Z, args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(Z, g, plans, args, N)
@@ -152,6 +157,7 @@ def main():
                      "frac_of_achievable": achieved / achievable},
         "cpu_baseline": cpu,
         "time_to_error": elp,
+        "batched_queries": batch,
         "device": info,
     }
     print(json.dumps(out), flush=True)
@@ -161,7 +167,9 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95):
     """Time to an answer within +-eps at `conf` confidence, regression profile
     (ErrorProfile sizing as the reference runs it, without and with the CLT
     judge) against the learned profile (pilot + MLP sizing + judge), several
-    independent queries each; achieved error against the device exact count."""
+    independent queries each; achieved error against the device exact count.
+    A profile whose sizing is undefined on this graph is reported as failed."""
+    import math
     from zgraph import elp as L
     weights = args.elp_weights or L.DEFAULT_WEIGHTS
     learned = L.LearnedErrorProfile(weights) if os.path.exists(weights) else None
@@ -173,35 +181,110 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95):
         t_exact = time.perf_counter() - t
         log(f"exact {name}: {truth} in {t_exact:.2f}s")
         res = {"exact": truth, "exact_count_s": t_exact}
-        profiles = [("regression", Z.ErrorProfile(), False), ("regression+judge", Z.ErrorProfile(), True)]
+        profiles = [("regression", Z.ErrorProfile(), False, 0), ("regression+judge", Z.ErrorProfile(), True, 0)]
         if learned is not None:
-            profiles.append(("learned+judge", learned, True))
-        for label, prof, judged in profiles:
+            profiles.append(("learned+judge", learned, True, 1 << 24))
+        for label, prof, judged, fallback in profiles:
             recs = []
             for q in range(args.elp_seeds):
                 inst = Z.ZGraphInstance(g, Z.Plan(kind), seed=101 + q)
-                r = L.estimate_to_error(inst, eps, conf, prof, judged=judged, max_estimators=1 << 36)
-                r["error"] = abs(r["estimate"] - truth) / truth if truth else float("nan")
+                r = L.estimate_to_error(inst, eps, conf, prof, judged=judged, max_estimators=1 << 36,
+                                        fallback_budget=fallback)
+                r["error"] = abs(r["estimate"] - truth) / truth if truth and r["estimators"] else float("nan")
                 recs.append(r)
                 log(f"{name} {label} q{q}: {r['total_s'] * 1e3:.1f} ms N={r['estimators']} "
                     f"(sized {r['sized_estimators']}, {r['sizing']}, {r['topup_rounds']} top-ups) "
-                    f"err={r['error'] * 100:.3f}%")
+                    f"err={r['error'] * 100:.3f}%" + (f" [{r['error_message']}]" if "error_message" in r else ""))
+            ok = [r for r in recs if r["estimators"]]
             res[label] = {
-                "mean_time_s": sum(r["total_s"] for r in recs) / len(recs),
-                "mean_sizing_s": sum(r["sizing_s"] for r in recs) / len(recs),
-                "mean_estimators": sum(r["estimators"] for r in recs) / len(recs),
-                "within_eps": sum(r["error"] <= eps for r in recs) / len(recs),
-                "max_error": max(r["error"] for r in recs),
+                "failed_queries": len(recs) - len(ok),
+                "failure": next((r["error_message"] for r in recs if "error_message" in r), None),
+                "mean_time_s": sum(r["total_s"] for r in ok) / len(ok) if ok else None,
+                "mean_sizing_s": sum(r["sizing_s"] for r in ok) / len(ok) if ok else None,
+                "mean_estimators": sum(r["estimators"] for r in ok) / len(ok) if ok else None,
+                "mean_sized_estimators": sum(r["sized_estimators"] for r in ok) / len(ok) if ok else None,
+                "within_eps": sum(r["error"] <= eps for r in ok) / len(recs),
+                "max_error": max((r["error"] for r in ok), default=None),
                 "sizing_used": sorted({r["sizing"] for r in recs}),
-                "mean_topup_rounds": sum(r["topup_rounds"] for r in recs) / len(recs),
-                "mean_sized_estimators": sum(r["sized_estimators"] for r in recs) / len(recs),
+                "mean_topup_rounds": sum(r["topup_rounds"] for r in ok) / len(ok) if ok else None,
                 "capped_at_2^36": sum(r["capped"] for r in recs),
             }
-        if learned is not None:
-            res["speedup_learned_vs_regression"] = res["regression"]["mean_time_s"] / res["learned+judge"]["mean_time_s"]
-            res["speedup_learned_vs_regression_judged"] = (res["regression+judge"]["mean_time_s"]
-                                                           / res["learned+judge"]["mean_time_s"])
+        lt = res.get("learned+judge", {}).get("mean_time_s")
+        for base in ("regression", "regression+judge"):
+            bt = res[base]["mean_time_s"]
+            res["speedup_learned_vs_" + base.replace("+", "_")] = (bt / lt) if (bt and lt) else None
         out[name] = res
+    return out
+
+
+def batched_queries(Z, args):
+    """256 mixed queries (triangle, 4-clique, 5-star, 5-house, 4-chain) over
+    64 seeded Chung-Lu / R-MAT graphs of 1e4..1e7 edges, budgets 2^16..2^26:
+    one zg_run call per query against one replay of the batched HIP graph
+    (results checked equal, query by query).  Then the same queries at 2^16
+    estimators each, where per-call overhead dominates, with the autotuner."""
+    import numpy as np
+    rng = np.random.default_rng(args.seed + 40)
+    t = time.perf_counter()
+    graphs = []
+    for i in range(64):
+        m = int(10 ** rng.uniform(4, 7))
+        if i % 2 == 0:
+            n = max(m // 8, 64)
+            dmax = float(min(10 ** rng.uniform(2, 4), n / 4))
+            e = Z.chung_lu(n, m, float(rng.uniform(2.1, 2.8)), dmax, 500 + i)
+        else:
+            scale = max(int(np.ceil(np.log2(m / 8))), 8)
+            e = Z.rmat(scale, m, 0.57, 0.19, 0.19, 500 + i)
+        graphs.append(Z.Graph.from_device(e, seed=500 + i))
+        e.free()
+    t_graphs = time.perf_counter() - t
+    plans = [Z.Plan(Z.PLAN_TRIANGLE), Z.Plan(Z.PLAN_K4_CLOSE), Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_STAR_PATTERN),
+             Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_HOUSE_PATTERN), Z.Plan(Z.PLAN_FOUR_CHAIN)]
+    queries = [(graphs[int(rng.integers(64))], plans[i % 5], int(rng.integers(1, 1 << 40)), 0,
+                int(2 ** rng.uniform(16, 26))) for i in range(256)]
+    total = sum(q[4] for q in queries)
+    log(f"batch: 64 graphs in {t_graphs:.1f}s, 256 queries, {total:.3e} estimators")
+
+    def sequential(qs):
+        t0 = time.perf_counter()
+        out = [Z.run(*q) for q in qs]
+        return out, time.perf_counter() - t0
+
+    def batched(b, reps=3):
+        walls, res = [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            res, ms = b.run(timed=True)
+            walls.append(time.perf_counter() - t0)
+        return res, sorted(walls)[len(walls) // 2]
+
+    out = {"graphs": 64, "queries": 256, "graph_build_s": t_graphs}
+    for label, qs, tune in (("mixed_budgets", queries, False),
+                            ("budget_2^16", [(g, p, s, f, 1 << 16) for g, p, s, f, _ in queries], True)):
+        seq, t_seq = sequential(qs)
+        b = Z.Batch(qs)
+        res, t_b = batched(b)
+        same = all(x[0].total() == y[0].total() and x[0].successes == y[0].successes for x, y in zip(seq, res))
+        rec = {"estimators": sum(q[4] for q in qs), "sequential_s": t_seq, "batched_s": t_b,
+               "speedup": t_seq / t_b, "batched_estimators_per_s": sum(q[4] for q in qs) / t_b,
+               "exact_sums_equal": same, "config": b.config()}
+        if tune:
+            t0 = time.perf_counter()
+            cfg = b.autotune(trials=3)
+            rec["autotune_s"] = time.perf_counter() - t0
+            res2, t_tuned = batched(b)
+            rec["autotuned_config"] = cfg
+            rec["autotuned_batched_s"] = t_tuned
+            rec["autotuned_speedup"] = t_seq / t_tuned
+            rec["exact_sums_equal"] = same and all(x[0].total() == y[0].total() for x, y in zip(seq, res2))
+        log(f"batch {label}: sequential {t_seq * 1e3:.1f} ms, batched {t_b * 1e3:.1f} ms "
+            f"({rec['speedup']:.2f}x), equal={rec['exact_sums_equal']}" +
+            (f", autotuned {rec['autotuned_batched_s'] * 1e3:.1f} ms {rec['autotuned_config']}" if tune else ""))
+        out[label] = rec
+        b.close()
+    for g in graphs:
+        g.close()
     return out
 
 

@@ -28,6 +28,14 @@ struct PlanView {
     const uint32_t *accept;   // ZG_PLAN_PATTERN: bit table over relabeled edge masks
 };
 
+// one query of a batched launch (device descriptor)
+struct BQuery {
+    GView g;
+    PlanView P;
+    uint64_t seed, first, count;
+    uint64_t chunk0;   // index of the query's first chunk in its launch
+};
+
 }  // namespace zg
 
 struct zg_graph {
@@ -87,5 +95,8 @@ void free_graph(zg_graph *g);
 int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uint64_t count,
                zg_outcome *d_out, int block, int blocks_per_cu, unsigned long long *reads = nullptr);
 void finish_sums(const Acc *host_acc, int nch, int K, uint64_t count, zg_sums *out);
+// one plan kind's batched launch: counter reset, work-queue kernel, per-query reduction
+int launch_batch(int kind, const BQuery *d_q, int nq, uint64_t nchunks, uint32_t chunk_est,
+                 unsigned long long *counter, Acc *chunk_partials, Acc *result, int grid, int block, hipStream_t st);
 
 }  // namespace zg

@@ -480,8 +480,9 @@ __device__ __forceinline__ AccW wave_reduce(const AccL &l)
     return a;
 }
 
+// reduces the workgroup's accumulators and writes NCH records at dst
 template <int NCH>
-__device__ __forceinline__ void block_reduce_store(AccL (&a)[NCH], Acc *__restrict__ partials)
+__device__ __forceinline__ void block_reduce_store(AccL (&a)[NCH], Acc *__restrict__ dst)
 {
     __shared__ Acc sm[NCH][16];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -509,8 +510,45 @@ __device__ __forceinline__ void block_reduce_store(AccL (&a)[NCH], Acc *__restri
                 t.ovf += sm[c][w].ovf;
                 t.sq += sm[c][w].sq;
             }
-            partials[blockIdx.x * NCH + c] = t;
+            dst[c] = t;
         }
+    }
+}
+
+// one estimator: the plan body for estimator id `id`
+template <int KIND>
+__device__ __forceinline__ void run_one(const GView &g, const PlanView &P, Rng &rng, Est &o)
+{
+    o.st[0] = o.st[1] = 0;
+    o.den[0] = o.den[1] = 0;
+    o.v[0] = o.v[1] = U128{0, 0};
+    o.ncond = 0;
+    o.e0 = 0;
+    if (KIND == ZG_PLAN_TRIANGLE) plan_wedge<0>(g, rng, o);
+    else if (KIND == ZG_PLAN_THREE_CHAIN) plan_wedge<1>(g, rng, o);
+    else if (KIND == ZG_PLAN_THREE_MOTIF) plan_wedge<2>(g, rng, o);
+    else if (KIND == ZG_PLAN_CHAIN) plan_chain(g, P, rng, o);
+    else if (KIND == ZG_PLAN_FOUR_CHAIN) plan_four_chain(g, rng, o);
+    else if (KIND == ZG_PLAN_K4_CLOSE) plan_k4_close(g, rng, o);
+    else if (KIND == ZG_PLAN_PATTERN) {
+        switch (P.pk) {
+        case 2: plan_pattern<3>(g, P, rng, o); break;
+        case 3: plan_pattern<4>(g, P, rng, o); break;
+        case 4: plan_pattern<5>(g, P, rng, o); break;
+        case 5: plan_pattern<6>(g, P, rng, o); break;
+        default: plan_pattern<7>(g, P, rng, o); break;
+        }
+    }
+}
+
+template <int NCH>
+__device__ __forceinline__ void acc_zero(AccL (&acc)[NCH])
+{
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        acc[c].lo = acc[c].hi = acc[c].top = 0;
+        acc[c].succ = acc[c].ovf = 0;
+        acc[c].sq = 0;
     }
 }
 
@@ -521,37 +559,13 @@ __global__ __launch_bounds__(256) void k_run(GView g, PlanView P, uint64_t seed,
                                              Acc *__restrict__ partials)
 {
     AccL acc[NCH];
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        acc[c].lo = acc[c].hi = acc[c].top = 0;
-        acc[c].succ = acc[c].ovf = 0;
-        acc[c].sq = 0;
-    }
+    acc_zero<NCH>(acc);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
         Rng rng;
         rng.init(seed, TAG_ESTIMATOR, first + i);
         Est o;
-        o.st[0] = o.st[1] = 0;
-        o.den[0] = o.den[1] = 0;
-        o.v[0] = o.v[1] = U128{0, 0};
-        o.ncond = 0;
-        o.e0 = 0;
-        if (KIND == ZG_PLAN_TRIANGLE) plan_wedge<0>(g, rng, o);
-        else if (KIND == ZG_PLAN_THREE_CHAIN) plan_wedge<1>(g, rng, o);
-        else if (KIND == ZG_PLAN_THREE_MOTIF) plan_wedge<2>(g, rng, o);
-        else if (KIND == ZG_PLAN_CHAIN) plan_chain(g, P, rng, o);
-        else if (KIND == ZG_PLAN_FOUR_CHAIN) plan_four_chain(g, rng, o);
-        else if (KIND == ZG_PLAN_K4_CLOSE) plan_k4_close(g, rng, o);
-        else if (KIND == ZG_PLAN_PATTERN) {
-            switch (P.pk) {
-            case 2: plan_pattern<3>(g, P, rng, o); break;
-            case 3: plan_pattern<4>(g, P, rng, o); break;
-            case 4: plan_pattern<5>(g, P, rng, o); break;
-            case 5: plan_pattern<6>(g, P, rng, o); break;
-            default: plan_pattern<7>(g, P, rng, o); break;
-            }
-        }
+        run_one<KIND>(g, P, rng, o);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             acc_add(acc[c], o.v[c], o.st[c], o.den[c], P.K);
@@ -569,7 +583,100 @@ __global__ __launch_bounds__(256) void k_run(GView g, PlanView P, uint64_t seed,
             }
         }
     }
-    block_reduce_store<NCH>(acc, partials);
+    block_reduce_store<NCH>(acc, partials + (size_t)blockIdx.x * NCH);
+}
+
+// Batched queries of one plan kind in one persistent launch.  The estimator
+// ids of every query are cut into chunks of `chunk_est`; workgroups claim
+// chunks from a global counter (a work queue) until it passes `nchunks`, and
+// write each chunk's reduced record to its own slot, so the per-query
+// reduction (k_batch_final) sums a fixed sequence: results do not depend on
+// which workgroup ran which chunk.
+template <int KIND, int NCH>
+__global__ __launch_bounds__(256) void k_batch(const BQuery *__restrict__ q, int nq, uint64_t nchunks,
+                                               uint32_t chunk_est, unsigned long long *__restrict__ counter,
+                                               Acc *__restrict__ chunk_partials)
+{
+    __shared__ unsigned long long s_chunk;
+    __shared__ int s_q;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const unsigned long long c = atomicAdd(counter, 1ull);
+            s_chunk = c;
+            if (c < nchunks) {   // the last query whose first chunk is <= c
+                int lo = 0, hi = nq - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (q[mid].chunk0 <= c) lo = mid; else hi = mid - 1;
+                }
+                s_q = lo;
+            }
+        }
+        __syncthreads();
+        const unsigned long long c = s_chunk;
+        if (c >= nchunks) break;   // uniform: every wave of the workgroup leaves here
+        const BQuery &Q = q[s_q];
+        const GView g = Q.g;
+        const PlanView P = Q.P;
+        const uint64_t seed = Q.seed, first = Q.first;
+        const uint64_t base = (c - Q.chunk0) * chunk_est;
+        const uint64_t end = base + chunk_est < Q.count ? base + chunk_est : Q.count;
+        AccL acc[NCH];
+        acc_zero<NCH>(acc);
+        for (uint64_t i = base + threadIdx.x; i < end; i += blockDim.x) {
+            Rng rng;
+            rng.init(seed, TAG_ESTIMATOR, first + i);
+            Est o;
+            run_one<KIND>(g, P, rng, o);
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) acc_add(acc[ch], o.v[ch], o.st[ch], o.den[ch], P.K);
+        }
+        block_reduce_store<NCH>(acc, chunk_partials + c * NCH);
+        __syncthreads();   // s_chunk and the reduction's LDS are reused next round
+    }
+}
+
+// per-query reduction of the chunk records (one workgroup per query)
+__global__ __launch_bounds__(256) void k_batch_final(const BQuery *__restrict__ q, int nch,
+                                                     const Acc *__restrict__ chunk_partials,
+                                                     uint64_t nchunks, Acc *__restrict__ result)
+{
+    __shared__ Acc sm[256];
+    const int qi = blockIdx.x;
+    const uint64_t c0 = q[qi].chunk0;
+    const uint64_t c1 = (qi + 1 < (int)gridDim.x) ? q[qi + 1].chunk0 : nchunks;
+    for (int c = 0; c < nch; ++c) {
+        Acc t;
+        for (int i = 0; i < 5; ++i) t.dig[i] = 0;
+        t.succ = t.ovf = 0;
+        t.sq = 0;
+        for (uint64_t b = c0 + threadIdx.x; b < c1; b += blockDim.x) {
+            const Acc &p = chunk_partials[b * nch + c];
+            for (int i = 0; i < 5; ++i) t.dig[i] += p.dig[i];
+            t.succ += p.succ;
+            t.ovf += p.ovf;
+            t.sq += p.sq;
+        }
+        sm[threadIdx.x] = t;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+                for (int i = 0; i < 5; ++i) sm[threadIdx.x].dig[i] += sm[threadIdx.x + s].dig[i];
+                sm[threadIdx.x].succ += sm[threadIdx.x + s].succ;
+                sm[threadIdx.x].ovf += sm[threadIdx.x + s].ovf;
+                sm[threadIdx.x].sq += sm[threadIdx.x + s].sq;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) result[(size_t)qi * nch + c] = sm[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // deterministic grid reduction of the block partials (one workgroup)
@@ -650,6 +757,34 @@ int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
     }
     ZG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, g->stream, g->partials, (int)grid, nch, g->result);
+    ZG_HIP(hipGetLastError());
+    return ZG_OK;
+}
+
+template <int KIND, int NCH>
+static void launch_batch_kind(const BQuery *d_q, int nq, uint64_t nchunks, uint32_t chunk_est,
+                              unsigned long long *counter, Acc *chunk_partials, Acc *result, int grid, int block,
+                              hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, counter, 1);
+    hipLaunchKernelGGL((k_batch<KIND, NCH>), dim3(grid), dim3(block), 0, st, d_q, nq, nchunks, chunk_est, counter,
+                       chunk_partials);
+    hipLaunchKernelGGL(k_batch_final, dim3(nq), dim3(256), 0, st, d_q, NCH, chunk_partials, nchunks, result);
+}
+
+int launch_batch(int kind, const BQuery *d_q, int nq, uint64_t nchunks, uint32_t chunk_est,
+                 unsigned long long *counter, Acc *chunk_partials, Acc *result, int grid, int block, hipStream_t st)
+{
+    switch (kind) {
+    case ZG_PLAN_TRIANGLE: launch_batch_kind<ZG_PLAN_TRIANGLE, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
+    case ZG_PLAN_THREE_CHAIN: launch_batch_kind<ZG_PLAN_THREE_CHAIN, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
+    case ZG_PLAN_THREE_MOTIF: launch_batch_kind<ZG_PLAN_THREE_MOTIF, 2>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
+    case ZG_PLAN_CHAIN: launch_batch_kind<ZG_PLAN_CHAIN, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
+    case ZG_PLAN_FOUR_CHAIN: launch_batch_kind<ZG_PLAN_FOUR_CHAIN, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
+    case ZG_PLAN_PATTERN: launch_batch_kind<ZG_PLAN_PATTERN, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
+    case ZG_PLAN_K4_CLOSE: launch_batch_kind<ZG_PLAN_K4_CLOSE, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
+    default: return set_error(ZG_EINVAL, "unknown plan kind");
+    }
     ZG_HIP(hipGetLastError());
     return ZG_OK;
 }

@@ -27,7 +27,7 @@ __all__ = [
     "ChainMiningInstance", "FourChainCountingInstance", "FourCliqueCountingInstance",
     "SamplerGenerator", "AllPredicateZGraphInstance", "ErrorProfile", "TimeProfile",
     "load_graph_meta_data", "load_biedgelist", "load_edgelist", "save_biedgelist", "ZGraphError",
-    "device_info", "clt_half_width",
+    "device_info", "clt_half_width", "run", "run_outcomes", "Batch", "random_gather_bandwidth",
 ]
 
 OUTCOME_DTYPE = np.dtype([("num_lo", "<u8"), ("num_hi", "<u8"), ("e0", "<u4"), ("status", "u1"),
@@ -282,6 +282,65 @@ def run(graph, plan, seed, first_id, count, block_size=0, blocks_per_cu=0, timed
                        C.byref(ms) if timed else None))
     out = [sums[i] for i in range(plan.channels)]
     return (out, ms.value) if timed else out
+
+
+class Batch:
+    """Many (graph, plan, seed, first_id, count) queries served by one
+    replayable HIP graph (one persistent work-queue launch per plan kind).
+    run() returns, per query, the list of per-channel sums -- equal to
+    run(graph, plan, seed, first_id, count) for that query."""
+
+    def __init__(self, queries, block_size=0, blocks_per_cu=0, chunk_estimators=0):
+        self._keep = [(q[0], q[1]) for q in queries]   # graphs and plans outlive the batch
+        self.nq = len(queries)
+        arr = (_capi.Query * self.nq)()
+        for i, (g, p, seed, first, count) in enumerate(queries):
+            arr[i] = _capi.Query(g.h, p.h, seed, first, count)
+        self.channels = [q[1].channels for q in queries]
+        cfg = _capi.BatchConfig(block_size, blocks_per_cu, chunk_estimators)
+        self.h = C.c_void_p()
+        check(lib().zg_batch_create(arr, self.nq, C.byref(cfg), C.byref(self.h)))
+        self.nrec = lib().zg_batch_records(self.h)
+
+    def run(self, timed=False):
+        out = (_capi.Sums * self.nrec)()
+        ms = C.c_float(0)
+        check(lib().zg_batch_run(self.h, out, C.byref(ms)))
+        res, k = [], 0
+        for ch in self.channels:
+            res.append([out[k + c] for c in range(ch)])
+            k += ch
+        return (res, ms.value) if timed else res
+
+    def set_ids(self, seeds=None, first_ids=None):
+        s = np.ascontiguousarray(seeds, np.uint64) if seeds is not None else None
+        f = np.ascontiguousarray(first_ids, np.uint64) if first_ids is not None else None
+        check(lib().zg_batch_set_ids(self.h, s.ctypes.data if s is not None else None,
+                                     f.ctypes.data if f is not None else None))
+
+    def autotune(self, trials=3):
+        cfg = _capi.BatchConfig()
+        ms = C.c_float(0)
+        check(lib().zg_batch_autotune(self.h, trials, C.byref(cfg), C.byref(ms)))
+        return {"block_size": cfg.block_size, "blocks_per_cu": cfg.blocks_per_cu,
+                "chunk_estimators": cfg.chunk_estimators, "ms": ms.value}
+
+    def config(self):
+        cfg = _capi.BatchConfig()
+        check(lib().zg_batch_config_get(self.h, C.byref(cfg)))
+        return {"block_size": cfg.block_size, "blocks_per_cu": cfg.blocks_per_cu,
+                "chunk_estimators": cfg.chunk_estimators}
+
+    def close(self):
+        if self.h:
+            lib().zg_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def run_outcomes(graph, plan, seed, first_id, count):

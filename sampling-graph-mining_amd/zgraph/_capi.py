@@ -37,6 +37,8 @@ EXPORTED = [
     "zg_plan_create", "zg_plan_destroy", "zg_plan_channels", "zg_plan_cond_calls",
     "zg_plan_unique_patterns", "zg_pattern_unique",
     "zg_run", "zg_run_outcomes", "zg_count_reads", "zg_exact_counts",
+    "zg_batch_create", "zg_batch_records", "zg_batch_run", "zg_batch_set_ids", "zg_batch_autotune",
+    "zg_batch_config_get", "zg_batch_destroy", "zg_run_batch",
     "zg_graph_degree_moments", "zg_random_gather_bw",
     "zg_elp_regression", "zg_elp_bound", "zg_regression_k", "zg_elp_from_pilots",
     "zg_mlp_num_params", "zg_mlp_input_dim", "zg_mlp_create", "zg_mlp_destroy", "zg_mlp_get_params",
@@ -69,6 +71,15 @@ class Sums(C.Structure):
     def total(self):
         """exact sum of values times 2^log2_den (Python int)"""
         return self.sum[0] | (self.sum[1] << 64) | (self.sum[2] << 128)
+
+
+class Query(C.Structure):
+    _fields_ = [("graph", C.c_void_p), ("plan", C.c_void_p), ("seed", C.c_uint64), ("first_id", C.c_uint64),
+                ("count", C.c_uint64)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("block_size", C.c_int), ("blocks_per_cu", C.c_int), ("chunk_estimators", C.c_uint32)]
 
 
 class Outcome(C.Structure):
@@ -133,6 +144,14 @@ def lib():
     L.zg_run.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, P(Launch), P(Sums), P(C.c_float)]
     L.zg_run_outcomes.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, P(Sums)]
     L.zg_exact_counts.argtypes = [vp, C.c_int, P(C.c_uint64)]
+    L.zg_batch_create.argtypes = [P(Query), C.c_int, P(BatchConfig), P(vp)]
+    L.zg_batch_records.argtypes = [vp]
+    L.zg_batch_run.argtypes = [vp, P(Sums), P(C.c_float)]
+    L.zg_batch_set_ids.argtypes = [vp, vp, vp]
+    L.zg_batch_autotune.argtypes = [vp, C.c_int, P(BatchConfig), P(C.c_float)]
+    L.zg_batch_config_get.argtypes = [vp, P(BatchConfig)]
+    L.zg_batch_destroy.argtypes = [vp]
+    L.zg_run_batch.argtypes = [P(Query), C.c_int, P(BatchConfig), P(Sums), P(C.c_float)]
     L.zg_graph_degree_moments.argtypes = [vp, P(C.c_double), P(C.c_double)]
     L.zg_random_gather_bw.argtypes = [C.c_uint64, C.c_int, C.c_uint64, C.c_int, C.c_int, P(C.c_double),
                                       P(C.c_double)]

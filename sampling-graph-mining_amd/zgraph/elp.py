@@ -29,8 +29,8 @@ import time
 
 import numpy as np
 
-from . import (PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_THREE_CHAIN, PLAN_TRIANGLE, ErrorProfile, Estimation, _z,
-               run)
+from . import (PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_THREE_CHAIN, PLAN_TRIANGLE, ErrorProfile, Estimation,
+               ZGraphError, _z)
 
 IN_DIM = 16
 PATTERN_SLOTS = {PLAN_TRIANGLE: 0, PLAN_K4_CLOSE: 1, PLAN_THREE_CHAIN: 2, PLAN_FOUR_CHAIN: 3}
@@ -89,12 +89,16 @@ class LearnedModel:
             self.mu = z["mu"].astype(np.float32)
             self.sd = z["sd"].astype(np.float32)
             self.meta = json.loads(bytes(z["meta"]).decode()) if "meta" in z else {}
+        # pattern slots where the network beat "no correction" on validation graphs
+        self.use_model = list(self.meta.get("use_model", [True] * len(PATTERN_SLOTS)))
         self.net = MLP()
         self.net.set_params(self.params)
 
     def predict(self, X):
-        X = (np.atleast_2d(np.asarray(X, np.float32)) - self.mu) / self.sd
-        return self.net.forward(X.astype(np.float32))
+        X = np.atleast_2d(np.asarray(X, np.float32))
+        r = self.net.forward(((X - self.mu) / self.sd).astype(np.float32))
+        slot = np.argmax(X[:, 6:10], axis=1)
+        return np.where(np.array(self.use_model)[slot], r, 0.0).astype(np.float32)
 
 
 class LearnedErrorProfile:
@@ -167,10 +171,12 @@ def judge(est, eps, conf, min_sample=1 << 12):
 
 
 def estimate_to_error(instance, eps, conf, profile, init_estimation=None, num_vertices_in_pattern=None,
-                      max_rounds=None, max_estimators=1 << 40, judged=True):
+                      max_rounds=None, max_estimators=1 << 40, judged=True, fallback_budget=1 << 24):
     """Size the query with `profile` (LearnedErrorProfile or ErrorProfile),
-    run it, and (judged=True) top up until the CLT judge accepts.  Returns a
-    record with the estimate, estimator counts and wall times."""
+    run it, and (judged=True) top up until the CLT judge accepts.  When the
+    sizing is undefined, a judged query starts from `fallback_budget`
+    estimators (0: report the failure).  Returns a record with the estimate,
+    estimator counts and wall times."""
     t0 = time.perf_counter()
     est = init_estimation if init_estimation is not None else Estimation()
     how = "regression"
@@ -179,9 +185,20 @@ def estimate_to_error(instance, eps, conf, profile, init_estimation=None, num_ve
         n, how = profile._size(instance, eps, conf, est)
         if n is None:
             profile = profile.fallback
+    error = None
     if n is None:
-        n = profile.get_num_estimators(instance, num_vertices_in_pattern, eps, conf)
-        how = "regression"
+        try:
+            n = profile.get_num_estimators(instance, num_vertices_in_pattern, eps, conf)
+            how = "regression"
+        except ZGraphError as e:
+            # the regression profile is undefined (no success among its pilots,
+            # or a zero estimate on the sub-sampled graph)
+            if not judged or not fallback_budget:
+                return {"estimate": float("nan"), "sizing": "regression", "error_message": str(e),
+                        "sized_estimators": 0, "estimators": 0, "topup_rounds": 0, "half_width": float("inf"),
+                        "rel_half_width": float("inf"), "accepted": False, "capped": False,
+                        "sizing_s": time.perf_counter() - t0, "total_s": time.perf_counter() - t0}
+            n, how, error = fallback_budget, "judge-only", str(e)
     t_size = time.perf_counter() - t0
     n0 = n
     capped = n > max_estimators
@@ -202,6 +219,9 @@ def estimate_to_error(instance, eps, conf, profile, init_estimation=None, num_ve
     t1 = time.perf_counter()
     mean = est.exact_mean()
     hw = est.half_width(conf)
-    return {"estimate": mean, "sizing": how, "sized_estimators": n0, "estimators": est.get_num_finished_estimators(),
+    rec = {"estimate": mean, "sizing": how, "sized_estimators": n0, "estimators": est.get_num_finished_estimators(),
             "topup_rounds": rounds, "half_width": hw, "rel_half_width": hw / mean if mean > 0 else float("inf"),
             "accepted": judge(est, eps, conf), "capped": capped, "sizing_s": t_size, "total_s": t1 - t0}
+    if error:
+        rec["error_message"] = error
+    return rec
