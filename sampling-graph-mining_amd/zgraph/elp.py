@@ -36,7 +36,7 @@ IN_DIM = 16
 PATTERN_SLOTS = {PLAN_TRIANGLE: 0, PLAN_K4_CLOSE: 1, PLAN_THREE_CHAIN: 2, PLAN_FOUR_CHAIN: 3}
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "elp_weights.npz")
 MIN_PILOT_SUCCESSES = 32
-MAX_PILOT = 1 << 22
+MAX_PILOT = 1 << 28
 
 
 def graph_features(graph):

@@ -6,11 +6,11 @@
    degree exponents 2.05..3, R-MAT), 2e4..3e7 edges, built in HBM.
 2. For every (graph, pattern) with pattern in {triangle, 4-clique, 3-chain,
    4-chain}: the exact count T from the device exact counters and a large run
-   of 2^26 estimators whose sum of squares gives the estimator's CV,
-   CV_true = sqrt(E[X^2] / T^2 - 1).
+   (2^26 estimators, extended up to 2^30 until 4096 succeed) whose sum of
+   squares gives the estimator's CV, CV_true = sqrt(E[X^2] / T^2 - 1).
 3. `pilots` pilot runs on disjoint id ranges, each sized as the learned
    profile sizes its pilot (2^14..2^18 estimators, widened x8 until 32
-   successes, at most 2^22), give one tuple each: the 16 features of zgraph.elp.feature_vector and the target
+   successes, at most 2^28), give one tuple each: the 16 features of zgraph.elp.feature_vector and the target
    r = log10(CV_true / CV_pilot).
 4. The device MLP is trained on the tuples of 70 % of the graphs (Adam,
    MSE), the parameters kept from the training segment with the lowest error
@@ -93,8 +93,19 @@ def collect(num_graphs, pilots, seed, big=1 << 26):
             T = truths[name]
             if T == 0:
                 continue
-            s = run(g, plans[kind], 7, 0, big)[0]
-            ex2 = s.sumsq / big
+            # the estimator's second moment from `big` estimators, extended
+            # (x4, at most 2^30) until 4096 of them succeed
+            n_big, sumsq, succ = 0, 0.0, 0
+            step = big
+            while True:
+                s = run(g, plans[kind], 7, n_big, step)[0]
+                n_big += step
+                sumsq += s.sumsq
+                succ += s.successes
+                if succ >= 4096 or n_big >= (1 << 30):
+                    break
+                step = min(3 * n_big, (1 << 30) - n_big)
+            ex2 = sumsq / n_big
             cv2 = ex2 / (float(T) ** 2) - 1.0
             if cv2 <= 0:
                 continue
@@ -102,7 +113,7 @@ def collect(num_graphs, pilots, seed, big=1 << 26):
             desc["cv"][name] = cv_true
             for j in range(pilots):
                 # the pilot procedure of LearnedErrorProfile: start at 2^14..2^18
-                # estimators, widen x8 until 32 successes (at most 2^22)
+                # estimators, widen x8 until 32 successes (at most 2^28)
                 n_p = 1 << int(rng.integers(14, 19))
                 first = (j + 1) << 32
                 while True:

@@ -1,0 +1,70 @@
+"""The learned error-latency profile on the device: the deployed network
+evaluated by the BF16-MFMA forward kernel matches the CPU reference of the
+same network, and judged queries reach the requested error against exact
+counts."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import zgraph as Z
+from zgraph import elp as L
+import helpers as H
+import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def test_deployed_weights_forward_matches_reference():
+    assert os.path.exists(L.DEFAULT_WEIGHTS), "trained weights are part of the package"
+    m = L.LearnedModel()
+    rng = np.random.default_rng(4)
+    gf = (7115.0, 100762.0, 3.0e7, 3.0e10, 1065.0)
+    X = np.stack([L.feature_vector(gf, k, 1 << int(rng.integers(14, 20)), float(10 ** rng.uniform(0, 3)),
+                                   float(10 ** rng.uniform(-4, 0)))
+                  for k in list(L.PATTERN_SLOTS) * 8])
+    got = m.predict(X)
+    Xn = ((X - m.mu) / m.sd).astype(np.float32)
+    want = mlp_ref.forward_backward(m.params, Xn)
+    slot = np.argmax(X[:, 6:10], axis=1)
+    want = np.where(np.array(m.use_model)[slot], want, 0.0)
+    np.testing.assert_allclose(got, want, rtol=2e-3, atol=2e-4)
+
+
+@pytest.mark.parametrize("kind,what", [(Z.PLAN_TRIANGLE, "triangle"), (Z.PLAN_FOUR_CHAIN, "path3"),
+                                       (Z.PLAN_THREE_CHAIN, "wedge")])
+def test_judged_queries_reach_the_error(kind, what):
+    n, s, d = H.golden_graph("wiki-vote")
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    truth = g.exact_count(what)
+    prof = L.LearnedErrorProfile()
+    eps = 0.02
+    errs = []
+    for q in range(6):
+        inst = Z.ZGraphInstance(g, Z.Plan(kind), seed=300 + q)
+        r = L.estimate_to_error(inst, eps, 0.95, prof)
+        assert r["accepted"] and r["sizing"] == "learned"
+        assert r["rel_half_width"] <= eps
+        errs.append(abs(r["estimate"] - truth) / truth)
+        say(f"[elp] kind {kind} q{q}: N={r['estimators']} sized={r['sized_estimators']} "
+            f"top-ups={r['topup_rounds']} err={errs[-1]:.4f}")
+    # each query misses with probability about 5 %; five or more of six hit
+    assert sum(e <= eps for e in errs) >= 5
+
+
+def test_sizing_failure_falls_back_to_the_judge():
+    # 4-cliques on a sparse graph: the pilot stays below 32 successes and the
+    # regression profile is undefined; a judged query still answers
+    e = Z.erdos_renyi(20000, 60000, 3)
+    g = Z.Graph.from_device(e, seed=3)
+    e.free()
+    prof = L.LearnedErrorProfile(max_pilot=1 << 16)
+    inst = Z.ZGraphInstance(g, Z.Plan(Z.PLAN_K4_CLOSE), seed=5)
+    r = L.estimate_to_error(inst, 0.5, 0.95, prof, max_estimators=1 << 26, max_rounds=2)
+    assert r["sizing"] in ("judge-only", "regression")
+    assert r["estimators"] > 0
