@@ -1,0 +1,42 @@
+// A query sized by the learned error-latency profile and judged to
+// +-eps at 95 % confidence: argv[2] pattern (triangle, 4-clique, 3-chain,
+// 4-chain), argv[3] eps (default 0.01), argv[4] estimator seed, argv[5] the
+// weights file (default: elp_weights.bin beside libzgraph_amd.so).
+#include <cstring>
+
+#include "AppCommon.hpp"
+
+int main(int argc, char **argv)
+{
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: %s graph-prefix pattern [eps [seed [weights.bin]]]\n", argv[0]);
+        return 2;
+    }
+    APP_MAIN_GUARD({
+        Graph<Empty> graph;
+        load_app_graph(argv[1], graph);
+        const std::string pat = argv[2];
+        const double eps = argc > 3 ? std::atof(argv[3]) : 0.01;
+        const uint64_t seed = argc > 4 ? std::strtoull(argv[4], nullptr, 10) : 1;
+        std::string weights = argc > 5 ? argv[5] : "";
+        if (weights.empty()) {
+            const std::string self = argv[0];
+            weights = self.substr(0, self.rfind('/') + 1) + "../../zgraph/elp_weights.bin";
+        }
+        std::unique_ptr<ZGraphInstance<Empty>> inst;
+        if (pat == "triangle") inst.reset(new TriangleCoutingInstance<Empty>(&graph));
+        else if (pat == "4-clique") inst.reset(new FourCliqueCountingInstance<Empty>(&graph));
+        else if (pat == "3-chain") inst.reset(new ThreeChainCountingInstance<Empty>(&graph));
+        else if (pat == "4-chain") inst.reset(new FourChainCountingInstance<Empty>(&graph));
+        else throw zgraph::Error(ZG_EINVAL, "unknown pattern " + pat);
+        inst->seed = seed;
+        LearnedErrorProfile<Empty> profile(weights);
+        Estimation est;
+        const double cnt = profile.run(*inst, 0, eps, 0.95, est);
+        Debug::get_instance()->print("estimated number of ", pat, ": ", (unsigned long long)cnt);
+        std::printf("sized %llu\n", profile.last_sized);
+        std::printf("judged %d %.17g\n", zgraph::judge(est, eps, 0.95) ? 1 : 0, est.half_width(0.95) / est.exact_mean());
+        report(pat.c_str(), est.exact_mean(), est.get_num_finished_estimators());
+    })
+    return 0;
+}

@@ -79,6 +79,29 @@ def feature_vector(gf, plan_kind, pilot_n, pilot_cv, pilot_succ):
     return x
 
 
+BIN_MAGIC = b"ZGELP001"
+
+
+def export_binary(npz_path, bin_path):
+    """Write the weights as the flat little-endian file the C++ host API reads:
+    magic "ZGELP001", u32 nparam, u32 in_dim, f32 mu[in_dim], f32 sd[in_dim],
+    u8 use_model[4], u32 0, f32 params[nparam]."""
+    with np.load(npz_path, allow_pickle=False) as z:
+        params = z["params"].astype("<f4")
+        mu = z["mu"].astype("<f4")
+        sd = z["sd"].astype("<f4")
+        meta = json.loads(bytes(z["meta"]).decode()) if "meta" in z else {}
+    use = np.array([1 if u else 0 for u in meta.get("use_model", [True] * 4)], np.uint8)
+    with open(bin_path, "wb") as f:
+        f.write(BIN_MAGIC)
+        f.write(np.array([len(params), len(mu)], "<u4").tobytes())
+        f.write(mu.tobytes())
+        f.write(sd.tobytes())
+        f.write(use.tobytes())
+        f.write(np.zeros(1, "<u4").tobytes())
+        f.write(params.tobytes())
+
+
 class LearnedModel:
     """Weights + input normalisation, evaluated by the device MLP."""
 

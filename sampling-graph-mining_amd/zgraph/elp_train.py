@@ -28,7 +28,8 @@ import numpy as np
 
 from . import (PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_THREE_CHAIN, PLAN_TRIANGLE, Graph, Plan, chung_lu, erdos_renyi,
                rmat, run)
-from .elp import IN_DIM, MAX_PILOT, MIN_PILOT_SUCCESSES, feature_vector, graph_features, pilot_stats
+from .elp import (IN_DIM, MAX_PILOT, MIN_PILOT_SUCCESSES, export_binary, feature_vector, graph_features,
+                  pilot_stats)
 from .mlp import MLP
 
 PATTERNS = [("triangle", PLAN_TRIANGLE, 0), ("4-clique", PLAN_K4_CLOSE, 1), ("3-chain", PLAN_THREE_CHAIN, 2),
@@ -221,6 +222,7 @@ def main():
     log(json.dumps({k: v for k, v in stats.items() if not isinstance(v, list)}))
     np.savez(os.path.join(a.out, "elp_weights.npz"), params=params, mu=mu, sd=sd,
              meta=np.frombuffer(json.dumps(stats).encode(), np.uint8))
+    export_binary(os.path.join(a.out, "elp_weights.npz"), os.path.join(a.out, "elp_weights.bin"))
     print(json.dumps({k: v for k, v in stats.items() if not isinstance(v, list)}), flush=True)
 
 

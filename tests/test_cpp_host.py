@@ -127,3 +127,28 @@ def test_predicate_apps(tmp_path):
     ga = Z.run(g, Z.Plan(Z.PLAN_TRIANGLE), seed, 0, N)[0]
     gn = Z.run(g.filter(50.0, False, 1), Z.Plan(Z.PLAN_TRIANGLE), seed, 0, N)[0]
     assert est_one == pytest.approx(ga.mean - gn.mean, rel=1e-9, abs=1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pattern,kind", [("triangle", Z.PLAN_TRIANGLE), ("4-chain", Z.PLAN_FOUR_CHAIN)])
+def test_learned_profile_cpp_equals_python(tmp_path, pattern, kind):
+    """The C++ LearnedErrorProfile sizes and judges a query as zgraph.elp does."""
+    from zgraph import elp as L
+    n, s, d = golden_graph("wiki-vote")
+    prefix = _write_graph(tmp_path, "wv", n, s, d)
+    eps, seed = 0.02, 11
+    print(f"[cpp] EstimateToError {pattern}", flush=True)
+    r = subprocess.run([_bin("EstimateToError"), prefix, pattern, str(eps), str(seed)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    sized = int(re.search(r"^sized (\d+)$", r.stdout, re.M).group(1))
+    judged = re.search(r"^judged (\d) (\S+)$", r.stdout, re.M)
+    est, cnt = _report(r.stdout)[pattern]
+    assert judged.group(1) == "1" and float(judged.group(2)) <= eps
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    inst = Z.ZGraphInstance(g, Z.Plan(kind), seed=seed)
+    rec = L.estimate_to_error(inst, eps, 0.95, L.LearnedErrorProfile())
+    assert rec["sizing"] == "learned"
+    assert abs(rec["sized_estimators"] - sized) <= 1
+    if rec["estimators"] == cnt:
+        assert est == pytest.approx(rec["estimate"], rel=1e-12)
