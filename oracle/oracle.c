@@ -1364,6 +1364,13 @@ og_u128 og_exact_stars(const og_graph *g, int leaves)
     return S;
 }
 
+void og_exact_stars64(const og_graph *g, int leaves, uint64_t *lo, uint64_t *hi)
+{
+    const og_u128 s = og_exact_stars(g, leaves);
+    *lo = (uint64_t)s;
+    *hi = (uint64_t)(s >> 64);
+}
+
 /* 3-edge simple paths: sum over edges (du-1)(dv-1) - 3 T */
 uint64_t og_exact_paths3(const og_graph *g, int threads)
 {

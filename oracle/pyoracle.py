@@ -104,6 +104,8 @@ def lib():
         L.og_exact_wedges.argtypes = [P(Graph)]
         L.og_exact_paths_brute.restype = C.c_uint64
         L.og_exact_paths_brute.argtypes = [P(Graph), C.c_int]
+        L.og_exact_stars64.restype = None
+        L.og_exact_stars64.argtypes = [P(Graph), C.c_int, P(C.c_uint64), P(C.c_uint64)]
         L.og_exact_house_brute.restype = C.c_uint64
         L.og_exact_house_brute.argtypes = [P(Graph)]
         L.og_exact_pattern_brute.restype = C.c_uint64
@@ -250,6 +252,11 @@ class OGraph:
 
     def paths_brute(self, nverts):
         return lib().og_exact_paths_brute(C.byref(self.g), nverts)
+
+    def stars(self, leaves=4):
+        lo, hi = C.c_uint64(), C.c_uint64()
+        lib().og_exact_stars64(C.byref(self.g), leaves, C.byref(lo), C.byref(hi))
+        return lo.value | (hi.value << 64)
 
     def house_brute(self):
         return lib().og_exact_house_brute(C.byref(self.g))

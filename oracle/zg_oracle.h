@@ -179,6 +179,7 @@ uint64_t og_exact_k4(const og_graph *g, int threads);
 uint64_t og_exact_wedges(const og_graph *g);              /* ThreeChain.cpp:44-51 */
 uint64_t og_exact_paths3(const og_graph *g, int threads);  /* 3-edge simple paths */
 og_u128 og_exact_stars(const og_graph *g, int leaves);     /* FiveStar.cpp:46-60 for leaves=4 */
+void og_exact_stars64(const og_graph *g, int leaves, uint64_t *lo, uint64_t *hi);
 uint64_t og_exact_paths_brute(const og_graph *g, int nverts); /* simple paths with nverts vertices */
 uint64_t og_exact_house_brute(const og_graph *g);          /* 5-house, non-induced */
 uint64_t og_exact_pattern_brute(const og_graph *g, int k, int e, const uint32_t *ps, const uint32_t *pd);

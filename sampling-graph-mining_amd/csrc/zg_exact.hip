@@ -5,6 +5,8 @@
 //   4-cliques  naive_implementation/FourClique.cpp:44-78
 //   wedges     naive_implementation/ThreeChain.cpp:44-51 (sum of C(d, 2))
 //   3-paths    naive_implementation/FourChain.cpp:46-66 (sum over edges (du-1)(dv-1) - 3T)
+//   5-stars    naive_implementation/FiveStar.cpp:51-59 (sum of C(d, 4))
+//   5-houses   naive_implementation/FiveHouse.cpp (per ordered edge, k_house)
 // Method: orient every undirected pair from lower to higher (degree, id) and
 // keep the out-lists sorted by neighbour id (taken from the neighbour-ordered
 // copy, parallel edges collapsed).  Triangles: per oriented edge u->v, binary
@@ -197,6 +199,93 @@ __global__ void k_wedges_paths(const uint64_t *off, const uint64_t *sadj, const 
     if (threadIdx.x == 0) { wed[blockIdx.x] = tw; paths[blockIdx.x] = tp; }
 }
 
+// 5-stars: sum over vertices of C(d, 4) (naive_implementation/FiveStar.cpp:51-59)
+__global__ void k_stars4(const uint32_t *sdeg, uint32_t n, unsigned long long *part)
+{
+    unsigned long long c = 0;
+    GRID_STRIDE(v, (uint64_t)n)
+    {
+        const unsigned __int128 d = sdeg[v];
+        if (d >= 4) c += (unsigned long long)(d * (d - 1) * (d - 2) * (d - 3) / 24);
+    }
+    const unsigned long long t = block_sum(c);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void k_widen(const uint32_t *a, uint32_t n, uint64_t *b)
+{
+    GRID_STRIDE(v, (uint64_t)n) b[v] = a[v];
+}
+
+// full simple adjacency: distinct neighbours ascending by id, and each entry's owner
+__global__ void k_full_fill(const uint64_t *off, const uint64_t *sadj, uint32_t n, const uint64_t *foff,
+                            uint32_t *fnb, uint32_t *fsrc)
+{
+    GRID_STRIDE(v, (uint64_t)n)
+    {
+        uint64_t w = foff[v];
+        uint32_t prev = 0xffffffffu;
+        for (uint64_t i = off[v]; i < off[v + 1]; ++i) {
+            const uint32_t nb = (uint32_t)(sadj[i] >> 32);
+            if (nb != prev) { fnb[w] = nb; fsrc[w] = (uint32_t)v; ++w; }
+            prev = nb;
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t isect_count(const uint32_t *a, uint64_t na, const uint32_t *b, uint64_t nb)
+{
+    if (na > nb) { const uint32_t *t = a; a = b; b = t; const uint64_t x = na; na = nb; nb = x; }
+    uint64_t c = 0;
+    for (uint64_t i = 0; i < na; ++i) c += bsearch_u32(b, nb, a[i]);
+    return c;
+}
+
+// 5-vertex houses (apex x over the edge {a, b} of the square a-c-d-b):
+// one wavefront per ordered edge (a, b) counts the embeddings with a, b as
+// the roof's base, sum over P = {(c, d): c in N(a)\b, d in N(b)\a, c ~ d}
+// of (t_ab - [c in T] - [d in T]) with T = N(a) & N(b), t_ab = |T|:
+//   t_ab * (S - (d_a - 1)) - sum_{c in T}(|N(c) & N(b)| - 1) - sum_{d in T}(|N(d) & N(a)| - 1),
+//   S = sum_{c in N(a)\b} |N(c) & N(b)|.
+// The house count is the embedding total over 2 (its automorphisms).
+__global__ __launch_bounds__(256) void k_house(const uint64_t *foff, const uint32_t *fnb, const uint32_t *fsrc,
+                                               uint64_t ne, unsigned long long *partial)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    unsigned long long acc = 0;
+    for (uint64_t e = wave; e < ne; e += nwaves) {
+        const uint32_t a = fsrc[e], b = fnb[e];
+        const uint32_t *Na = fnb + foff[a], *Nb = fnb + foff[b];
+        const uint64_t da = foff[a + 1] - foff[a], db = foff[b + 1] - foff[b];
+        unsigned long long S = 0, t = 0, X = 0, Y = 0;
+        for (uint64_t i = lane; i < da; i += 64) {
+            const uint32_t c = Na[i];
+            if (c == b) continue;
+            const uint32_t *Nc = fnb + foff[c];
+            const uint64_t dc = foff[c + 1] - foff[c];
+            const uint64_t k = isect_count(Nc, dc, Nb, db);
+            S += k;
+            if (bsearch_u32(Nb, db, c)) {
+                t += 1;
+                X += k - 1;
+                Y += isect_count(Nc, dc, Na, da) - 1;
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            S += __shfl_xor(S, o, 64);
+            t += __shfl_xor(t, o, 64);
+            X += __shfl_xor(X, o, 64);
+            Y += __shfl_xor(Y, o, 64);
+        }
+        if (lane == 0) acc += t * (S - (da - 1)) - X - Y;
+    }
+    const unsigned long long tot = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
 // degree moments: sum d^2 and sum d^3 (as doubles) per block
 __global__ void k_deg_moments(const uint64_t *off, uint32_t n, double *p2, double *p3)
 {
@@ -239,13 +328,57 @@ using namespace zg;
 extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
 {
     if (!g || !out) return set_error(ZG_EINVAL, "null argument");
-    if (what < 0 || what > 3) return set_error(ZG_EINVAL, "what must be 0 (triangles), 1 (4-cliques), 2 (wedges), 3 (3-edge paths)");
+    if (what < 0 || what > 5)
+        return set_error(ZG_EINVAL, "what must be 0 (triangles), 1 (4-cliques), 2 (wedges), 3 (3-edge paths), "
+                                    "4 (5-stars), 5 (5-houses)");
     const uint32_t n = g->n;
     hipStream_t st = g->stream;
     if (g->m == 0) { *out = 0; return ZG_OK; }
     XB<uint32_t> sdeg;
     XALLOC(sdeg, n);
     hipLaunchKernelGGL(k_simple_deg, dim3(gridx(n)), dim3(256), 0, st, g->off, g->sadj, n, sdeg.p);
+    if (what == 4) {
+        const int grid = gridx(n);
+        XB<unsigned long long> part;
+        XALLOC(part, grid);
+        hipLaunchKernelGGL(k_stars4, dim3(grid), dim3(256), 0, st, sdeg.p, n, part.p);
+        ZG_HIP(hipGetLastError());
+        unsigned long long s = 0;
+        int rc = host_sum(part.p, grid, st, &s);
+        if (rc) return rc;
+        *out = s;
+        return ZG_OK;
+    }
+    if (what == 5) {
+        XB<uint64_t> fdeg, foff;
+        XALLOC(fdeg, (uint64_t)n + 1); XALLOC(foff, (uint64_t)n + 1);
+        hipLaunchKernelGGL(k_widen, dim3(gridx(n)), dim3(256), 0, st, sdeg.p, n, fdeg.p);
+        ZG_HIP(hipMemsetAsync(fdeg.p + n, 0, 8, st));
+        size_t tb = 0;
+        ZG_HIP(rocprim::exclusive_scan(nullptr, tb, fdeg.p, foff.p, (uint64_t)0, (size_t)n + 1,
+                                       rocprim::plus<uint64_t>(), st));
+        XB<char> tmp;
+        XALLOC(tmp, tb);
+        ZG_HIP(rocprim::exclusive_scan(tmp.p, tb, fdeg.p, foff.p, (uint64_t)0, (size_t)n + 1,
+                                       rocprim::plus<uint64_t>(), st));
+        uint64_t ne = 0;
+        ZG_HIP(hipMemcpyAsync(&ne, foff.p + n, 8, hipMemcpyDeviceToHost, st));
+        ZG_HIP(hipStreamSynchronize(st));
+        XB<uint32_t> fnb, fsrc;
+        XALLOC(fnb, ne); XALLOC(fsrc, ne);
+        hipLaunchKernelGGL(k_full_fill, dim3(gridx(n)), dim3(256), 0, st, g->off, g->sadj, n, foff.p, fnb.p, fsrc.p);
+        ZG_HIP(hipGetLastError());
+        const int grid = (int)std::min<uint64_t>((uint64_t)device_cus() * 8, std::max<uint64_t>(1, (ne + 3) / 4));
+        XB<unsigned long long> part;
+        XALLOC(part, grid);
+        hipLaunchKernelGGL(k_house, dim3(grid), dim3(256), 0, st, foff.p, fnb.p, fsrc.p, ne, part.p);
+        ZG_HIP(hipGetLastError());
+        unsigned long long s = 0;
+        int rc = host_sum(part.p, grid, st, &s);
+        if (rc) return rc;
+        *out = s / 2;
+        return ZG_OK;
+    }
     if (what >= 2) {
         const int grid = gridx(n);
         XB<unsigned long long> pw, pp;

@@ -59,5 +59,31 @@ def test_device_exact_counts_equal_oracle(gname):
     assert g.exact_count("k4") == og.k4()
     assert g.exact_count("wedge") == og.wedges()
     assert g.exact_count("path3") == og.paths3()
+    assert g.exact_count("star5") == og.stars(4)
+    if n <= 400:
+        assert g.exact_count("house") == og.house_brute()
     if gname == "snap-wiki-Vote":
         assert g.exact_count("triangle") == 608389
+
+
+@pytest.mark.parametrize("n,p,seed", [(40, 0.3, 1), (60, 0.15, 2), (25, 0.6, 3)])
+def test_device_house_count_equals_brute_force(n, p, seed):
+    import helpers as H
+    n, s, d = H.random_graph(n, p, seed)
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    og = O.OGraph(n, s, d, seed=1)
+    assert g.exact_count("house") == og.house_brute()
+
+
+def test_device_house_count_agrees_with_the_house_estimator():
+    """wiki-vote: the exact count (new algorithm) lies in the 99.9 % interval
+    of 2^26 unbiased SamplerGenerator house estimators."""
+    import helpers as H
+    n, s, d = H.golden_graph("wiki-vote")
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    H_exact = g.exact_count("house")
+    N = 1 << 26
+    r = Z.run(g, Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_HOUSE_PATTERN), 3, 0, N)[0]
+    hw = Z.clt_half_width(r.mean, r.sumsq, N, 0.999)
+    print(f"[house] exact {H_exact} estimate {r.mean:.6g} +- {hw:.3g}", flush=True)
+    assert abs(r.mean - H_exact) <= hw

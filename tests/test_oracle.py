@@ -326,3 +326,13 @@ def test_elp_single_point_closed_form():
     got = O.lib().og_elp_num_estimators(_ld([e0]), 1, F, a, b, dmax_s, m_s, dmax, m, eps, conf, C.byref(Kout))
     assert got == pytest.approx(want, rel=1e-12)
     assert float(Kout.value) == pytest.approx(K, rel=1e-12)
+
+
+def test_star_and_house_closed_forms():
+    """K_n: 5-stars n*C(n-1,4); houses n!/(n-5)!/2 (two automorphisms)."""
+    from math import comb, perm
+    for k in (5, 6, 8):
+        n, s, d = H.complete_graph(k)
+        og = O.OGraph(n, s, d, seed=1)
+        assert og.stars(4) == k * comb(k - 1, 4)
+        assert og.house_brute() == perm(k, 5) // 2
