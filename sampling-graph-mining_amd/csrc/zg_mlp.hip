@@ -373,8 +373,10 @@ extern "C" int zg_mlp_create(uint64_t seed, zg_mlp **out)
     ZG_HIP(hipMalloc(&m->loss, sizeof(float)));
     // He-uniform weights from Philox (seed, tag 6, index), zero biases
     std::vector<float> h(NPARAM, 0.f);
-    auto fill = [&](int off, int fan_in, int count) {
-        const float a = sqrtf(6.f / (float)fan_in);
+    // He-uniform hidden layers; the output layer starts at 1 % of that scale so
+    // the untrained network predicts ~0 (the targets are small corrections)
+    auto fill = [&](int off, int fan_in, int count, float scale = 1.f) {
+        const float a = scale * sqrtf(6.f / (float)fan_in);
         for (int i = 0; i < count; ++i) {
             const uint4 r = philox10(make_uint4(0u, 6u, (uint32_t)(off + i), 0u), (uint32_t)seed, (uint32_t)(seed >> 32));
             const float u = (float)(r.x >> 8) * 0x1p-24f;
@@ -384,7 +386,7 @@ extern "C" int zg_mlp_create(uint64_t seed, zg_mlp **out)
     fill(OFF_W1, IN, IN * H);
     fill(OFF_W2, H, H * H);
     fill(OFF_W3, H, H * H);
-    fill(OFF_W4, H, H);
+    fill(OFF_W4, H, H, 0.01f);
     ZG_HIP(hipMemcpy(m->P, h.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice));
     ZG_HIP(hipMemset(m->M, 0, sizeof(float) * NPARAM));
     ZG_HIP(hipMemset(m->V, 0, sizeof(float) * NPARAM));

@@ -846,11 +846,10 @@ class LearnedErrorProfile {
         mu_.resize(hdr[1]);
         sd_.resize(hdr[1]);
         params_.resize(hdr[0]);
-        uint32_t pad;
         f.read((char *)mu_.data(), 4 * hdr[1]);
         f.read((char *)sd_.data(), 4 * hdr[1]);
         f.read((char *)use_, 4);
-        f.read((char *)&pad, 4);
+        f.read((char *)&y_scale_, 4);
         f.read((char *)params_.data(), 4 * (std::streamsize)hdr[0]);
         if (!f) throw Error(ZG_EINVAL, "LearnedErrorProfile: truncated weights file");
         check(zg_mlp_create(1, &net_));
@@ -898,7 +897,7 @@ class LearnedErrorProfile {
         float y = 0;
         for (int i = 0; i < 16; ++i) x[i] = (x[i] - mu_[i]) / sd_[i];
         check(zg_mlp_forward(net_, x, 1, &y));
-        const double r = use_[sl] ? (double)y : 0.0;
+        const double r = use_[sl] ? (double)(y * y_scale_) : 0.0;
         const double z = normal_quantile_two_sided(conf);
         const double q = z * cv * std::pow(10.0, r) / eps;
         return std::max((unsigned long long)std::ceil(q * q), n);
@@ -953,6 +952,7 @@ class LearnedErrorProfile {
     unsigned long long pilot_, max_pilot_;
     std::vector<float> mu_, sd_, params_;
     unsigned char use_[4] = {1, 1, 1, 1};
+    float y_scale_ = 1.f;   // the network predicts r / y_scale
     zg_mlp *net_ = nullptr;
     ErrorProfile<EdgeData> fallback_;
 };

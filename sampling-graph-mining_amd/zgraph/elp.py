@@ -85,7 +85,7 @@ BIN_MAGIC = b"ZGELP001"
 def export_binary(npz_path, bin_path):
     """Write the weights as the flat little-endian file the C++ host API reads:
     magic "ZGELP001", u32 nparam, u32 in_dim, f32 mu[in_dim], f32 sd[in_dim],
-    u8 use_model[4], u32 0, f32 params[nparam]."""
+    u8 use_model[4], f32 y_scale, f32 params[nparam]."""
     with np.load(npz_path, allow_pickle=False) as z:
         params = z["params"].astype("<f4")
         mu = z["mu"].astype("<f4")
@@ -98,7 +98,7 @@ def export_binary(npz_path, bin_path):
         f.write(mu.tobytes())
         f.write(sd.tobytes())
         f.write(use.tobytes())
-        f.write(np.zeros(1, "<u4").tobytes())
+        f.write(np.array([meta.get("y_scale", 1.0)], "<f4").tobytes())
         f.write(params.tobytes())
 
 
@@ -114,12 +114,13 @@ class LearnedModel:
             self.meta = json.loads(bytes(z["meta"]).decode()) if "meta" in z else {}
         # pattern slots where the network beat "no correction" on validation graphs
         self.use_model = list(self.meta.get("use_model", [True] * len(PATTERN_SLOTS)))
+        self.y_scale = float(self.meta.get("y_scale", 1.0))   # the network predicts r / y_scale
         self.net = MLP()
         self.net.set_params(self.params)
 
     def predict(self, X):
         X = np.atleast_2d(np.asarray(X, np.float32))
-        r = self.net.forward(((X - self.mu) / self.sd).astype(np.float32))
+        r = self.net.forward(((X - self.mu) / self.sd).astype(np.float32)) * np.float32(self.y_scale)
         slot = np.argmax(X[:, 6:10], axis=1)
         return np.where(np.array(self.use_model)[slot], r, 0.0).astype(np.float32)
 

@@ -30,7 +30,7 @@ def test_deployed_weights_forward_matches_reference():
                   for k in list(L.PATTERN_SLOTS) * 8])
     got = m.predict(X)
     Xn = ((X - m.mu) / m.sd).astype(np.float32)
-    want = mlp_ref.forward_backward(m.params, Xn)
+    want = mlp_ref.forward_backward(m.params, Xn) * np.float32(m.y_scale)
     slot = np.argmax(X[:, 6:10], axis=1)
     want = np.where(np.array(m.use_model)[slot], want, 0.0)
     np.testing.assert_allclose(got, want, rtol=2e-3, atol=2e-4)
