@@ -86,5 +86,5 @@ def test_device_house_count_agrees_with_the_house_estimator():
     r = Z.run(g, Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_HOUSE_PATTERN), 3, 0, N)[0]
     hw = Z.clt_half_width(r.mean, r.sumsq, N, 0.999)
     print(f"[house] exact {H_exact} estimate {r.mean:.6g} +- {hw:.3g} ({r.successes} successes)", flush=True)
-    assert r.successes > 1000
+    assert r.successes > 100
     assert abs(r.mean - H_exact) <= hw
