@@ -229,10 +229,12 @@ int pattern_classes(const zg_plan_desc *d, std::vector<uint32_t> &uniq_masks, in
     const int k = d->pattern_vertices, E = d->pattern_edges;
     if (k < 2 || k > 6) return set_error(ZG_EINVAL, "pattern vertices must be in [2, 6]");
     if (E < 1 || E > ZG_MAX_PATTERN_EDGES) return set_error(ZG_EINVAL, "pattern edges must be in [1, 15]");
+    if (E > k * (k - 1) / 2) return set_error(ZG_EINVAL, "more pattern edges than vertex pairs");
     PatEdges base;
     for (int i = 0; i < E; ++i) {
         const uint32_t a = d->pattern_src[i], b = d->pattern_dst[i];
         if ((int)a >= k || (int)b >= k || a == b) return set_error(ZG_EINVAL, "bad pattern edge");
+        if (pat_has(base, a, b)) return set_error(ZG_EINVAL, "repeated pattern edge");
         base.e.push_back({a, b});
     }
     std::vector<PatEdges> all;

@@ -166,7 +166,8 @@ extern "C" int zg_batch_create(const zg_query *q, int nq, const zg_batch_config 
         if (!x.graph || !x.plan) return set_error(ZG_EINVAL, "query without graph or plan");
         if (x.count > kMaxQueryCount) return set_error(ZG_EINVAL, "a batched query runs at most 2^31 estimators");
         if (x.graph->m == 0 && x.count) return set_error(ZG_EINVAL, "query on an empty graph");
-        const int kind = x.plan->desc.kind;
+        const int kind = x.plan->desc.kind == ZG_PLAN_PATTERN ? 100 + x.plan->desc.pattern_vertices
+                                                             : x.plan->desc.kind;   // one kernel per pattern size
         auto it = by_kind.find(kind);
         if (it == by_kind.end()) {
             it = by_kind.emplace(kind, b->groups.size()).first;

@@ -95,7 +95,8 @@ void free_graph(zg_graph *g);
 int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uint64_t count,
                zg_outcome *d_out, int block, int blocks_per_cu, unsigned long long *reads = nullptr);
 void finish_sums(const Acc *host_acc, int nch, int K, uint64_t count, zg_sums *out);
-// one plan kind's batched launch: counter reset, work-queue kernel, per-query reduction
+// one kernel kind's batched launch (the plan kind, or 100 + k for k-vertex
+// patterns): counter reset, work-queue kernel, per-query reduction
 int launch_batch(int kind, const BQuery *d_q, int nq, uint64_t nchunks, uint32_t chunk_est,
                  unsigned long long *counter, Acc *chunk_partials, Acc *result, int grid, int block, hipStream_t st);
 

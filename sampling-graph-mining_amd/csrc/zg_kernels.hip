@@ -241,7 +241,7 @@ __device__ __forceinline__ void plan_four_chain(const GView &g, Rng &rng, Est &o
 template <int MAXV>
 __device__ __forceinline__ void plan_pattern(const GView &g, const PlanView &P, Rng &rng, Est &o)
 {
-    constexpr int MAXE = 15;
+    constexpr int MAXE = MAXV * (MAXV - 1) / 2;   // a pattern on MAXV vertices has at most this many edges
     if (g.m == 0) return;
     const uint32_t r0 = (uint32_t)rng.uniform(g.m);
     o.e0 = r0;
@@ -270,7 +270,9 @@ __device__ __forceinline__ void plan_pattern(const GView &g, const PlanView &P, 
         den += in;
         ov |= mul_u128(val, s);
         if (!canonical<MAXE>(sr, ss, sd, ns, e)) return;
-        sr[ns] = e.rank; ss[ns] = e.src; sd[ns] = e.dst;
+#pragma unroll
+        for (int j = 0; j < MAXE; ++j)
+            if (j == ns) { sr[j] = e.rank; ss[j] = e.src; sd[j] = e.dst; }
         ++ns;
         if (!in) {
             if (nv >= P.pk || nv >= MAXV) return;   // |V| would exceed k
@@ -291,11 +293,18 @@ __device__ __forceinline__ void plan_pattern(const GView &g, const PlanView &P, 
         for (int j = i + 1; j < P.pk; ++j)
             if (vs[j] < vs[i]) { const uint32_t t = vs[j]; vs[j] = vs[i]; vs[i] = t; }
     uint32_t mask = 0;
-    for (int i = 0; i < ns; ++i) {
-        int a = 0, b = 0;
-        for (int j = 0; j < P.pk; ++j) { if (vs[j] == ss[i]) a = j; if (vs[j] == sd[i]) b = j; }
-        const int lo = a < b ? a : b, hi = a < b ? b : a;
-        mask |= 1u << (hi * (hi - 1) / 2 + lo);
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+        if (i < ns) {
+            int a = 0, b = 0;
+#pragma unroll
+            for (int j = 0; j < MAXV; ++j) {
+                if (j < P.pk && vs[j] == ss[i]) a = j;
+                if (j < P.pk && vs[j] == sd[i]) b = j;
+            }
+            const int lo = a < b ? a : b, hi = a < b ? b : a;
+            mask |= 1u << (hi * (hi - 1) / 2 + lo);
+        }
     }
     count_reads(g, 1);
     if ((P.accept[mask >> 5] >> (mask & 31)) & 1u) set_out(o, 0, val, ov, den);
@@ -515,6 +524,13 @@ __device__ __forceinline__ void block_reduce_store(AccL (&a)[NCH], Acc *__restri
     }
 }
 
+// kernel kinds: the plan kinds, and one per pattern size k (2..6) for ZG_PLAN_PATTERN
+constexpr int KIND_PATTERN_BASE = 100;
+inline int kernel_kind(const PlanView &P)
+{
+    return P.kind == ZG_PLAN_PATTERN ? KIND_PATTERN_BASE + P.pk : P.kind;
+}
+
 // one estimator: the plan body for estimator id `id`
 template <int KIND>
 __device__ __forceinline__ void run_one(const GView &g, const PlanView &P, Rng &rng, Est &o)
@@ -524,21 +540,13 @@ __device__ __forceinline__ void run_one(const GView &g, const PlanView &P, Rng &
     o.v[0] = o.v[1] = U128{0, 0};
     o.ncond = 0;
     o.e0 = 0;
-    if (KIND == ZG_PLAN_TRIANGLE) plan_wedge<0>(g, rng, o);
-    else if (KIND == ZG_PLAN_THREE_CHAIN) plan_wedge<1>(g, rng, o);
-    else if (KIND == ZG_PLAN_THREE_MOTIF) plan_wedge<2>(g, rng, o);
-    else if (KIND == ZG_PLAN_CHAIN) plan_chain(g, P, rng, o);
-    else if (KIND == ZG_PLAN_FOUR_CHAIN) plan_four_chain(g, rng, o);
-    else if (KIND == ZG_PLAN_K4_CLOSE) plan_k4_close(g, rng, o);
-    else if (KIND == ZG_PLAN_PATTERN) {
-        switch (P.pk) {
-        case 2: plan_pattern<3>(g, P, rng, o); break;
-        case 3: plan_pattern<4>(g, P, rng, o); break;
-        case 4: plan_pattern<5>(g, P, rng, o); break;
-        case 5: plan_pattern<6>(g, P, rng, o); break;
-        default: plan_pattern<7>(g, P, rng, o); break;
-        }
-    }
+    if constexpr (KIND == ZG_PLAN_TRIANGLE) plan_wedge<0>(g, rng, o);
+    else if constexpr (KIND == ZG_PLAN_THREE_CHAIN) plan_wedge<1>(g, rng, o);
+    else if constexpr (KIND == ZG_PLAN_THREE_MOTIF) plan_wedge<2>(g, rng, o);
+    else if constexpr (KIND == ZG_PLAN_CHAIN) plan_chain(g, P, rng, o);
+    else if constexpr (KIND == ZG_PLAN_FOUR_CHAIN) plan_four_chain(g, rng, o);
+    else if constexpr (KIND == ZG_PLAN_K4_CLOSE) plan_k4_close(g, rng, o);
+    else if constexpr (KIND >= KIND_PATTERN_BASE + 2) plan_pattern<KIND - KIND_PATTERN_BASE>(g, P, rng, o);
 }
 
 template <int NCH>
@@ -751,7 +759,16 @@ int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
     case ZG_PLAN_THREE_MOTIF: launch_kind<ZG_PLAN_THREE_MOTIF, 2>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
     case ZG_PLAN_CHAIN: launch_kind<ZG_PLAN_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
     case ZG_PLAN_FOUR_CHAIN: launch_kind<ZG_PLAN_FOUR_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
-    case ZG_PLAN_PATTERN: launch_kind<ZG_PLAN_PATTERN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+    case ZG_PLAN_PATTERN:
+        switch (P.pk) {
+        case 2: launch_kind<KIND_PATTERN_BASE + 2, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+        case 3: launch_kind<KIND_PATTERN_BASE + 3, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+        case 4: launch_kind<KIND_PATTERN_BASE + 4, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+        case 5: launch_kind<KIND_PATTERN_BASE + 5, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+        case 6: launch_kind<KIND_PATTERN_BASE + 6, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+        default: return set_error(ZG_EINVAL, "pattern vertices must be in [2, 6]");
+        }
+        break;
     case ZG_PLAN_K4_CLOSE: launch_kind<ZG_PLAN_K4_CLOSE, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
     default: return set_error(ZG_EINVAL, "unknown plan kind");
     }
@@ -781,7 +798,11 @@ int launch_batch(int kind, const BQuery *d_q, int nq, uint64_t nchunks, uint32_t
     case ZG_PLAN_THREE_MOTIF: launch_batch_kind<ZG_PLAN_THREE_MOTIF, 2>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
     case ZG_PLAN_CHAIN: launch_batch_kind<ZG_PLAN_CHAIN, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
     case ZG_PLAN_FOUR_CHAIN: launch_batch_kind<ZG_PLAN_FOUR_CHAIN, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
-    case ZG_PLAN_PATTERN: launch_batch_kind<ZG_PLAN_PATTERN, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
+    case KIND_PATTERN_BASE + 2: launch_batch_kind<KIND_PATTERN_BASE + 2, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
+    case KIND_PATTERN_BASE + 3: launch_batch_kind<KIND_PATTERN_BASE + 3, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
+    case KIND_PATTERN_BASE + 4: launch_batch_kind<KIND_PATTERN_BASE + 4, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
+    case KIND_PATTERN_BASE + 5: launch_batch_kind<KIND_PATTERN_BASE + 5, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
+    case KIND_PATTERN_BASE + 6: launch_batch_kind<KIND_PATTERN_BASE + 6, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
     case ZG_PLAN_K4_CLOSE: launch_batch_kind<ZG_PLAN_K4_CLOSE, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
     default: return set_error(ZG_EINVAL, "unknown plan kind");
     }
