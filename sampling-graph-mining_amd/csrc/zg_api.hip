@@ -353,10 +353,13 @@ int run_chunks(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
     memset(h, 0, sizeof(h));
     const int block = launch ? launch->block_size : 0;
     const int bpc = launch ? launch->blocks_per_cu : 0;
+    const int order = d_out ? 1 : resolve_order(g, p, count, launch ? launch->order : 0);
+    const uint64_t per_launch = order >= 2 ? kMaxOrderedLaunch : kMaxPerLaunch;
     ZG_HIP(hipEventRecord(g->ev0, g->stream));
     for (uint64_t done = 0; done < count || (count == 0 && done == 0);) {
-        const uint64_t c = std::min(kMaxPerLaunch, count - done);
-        int rc = launch_run(g, p, seed, first + done, c, d_out ? d_out + done * nch : nullptr, block, bpc);
+        const uint64_t c = std::min(per_launch, count - done);
+        int rc = launch_run(g, p, seed, first + done, c, d_out ? d_out + done * nch : nullptr, block, bpc, nullptr,
+                            order);
         if (rc) return rc;
         Acc r[2];
         ZG_HIP(hipMemcpyAsync(r, g->result, sizeof(Acc) * nch, hipMemcpyDeviceToHost, g->stream));

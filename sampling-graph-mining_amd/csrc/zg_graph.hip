@@ -170,6 +170,9 @@ void free_graph(zg_graph *g)
     if (g->partials) (void)hipFree(g->partials);
     if (g->result) (void)hipFree(g->result);
     if (g->outbuf) (void)hipFree(g->outbuf);
+    for (uint32_t *q : g->ord)
+        if (q) (void)hipFree(q);
+    if (g->ord_tmp) (void)hipFree(g->ord_tmp);
     if (g->ev0) (void)hipEventDestroy(g->ev0);
     if (g->ev1) (void)hipEventDestroy(g->ev1);
     if (g->stream) (void)hipStreamDestroy(g->stream);

@@ -53,6 +53,10 @@ struct zg_graph {
     zg::Acc *result = nullptr;    // device result (2 channels)
     zg_outcome *outbuf = nullptr;
     size_t outbuf_cap = 0;
+    uint32_t *ord[4] = {nullptr, nullptr, nullptr, nullptr};   // first-edge ordering: keys, offsets (x2)
+    size_t ord_cap = 0;
+    void *ord_tmp = nullptr;
+    size_t ord_tmp_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     zg::GView view() const { return zg::GView{edges, off, adj, sadj, htab, m, n, hnb, nullptr}; }
@@ -92,8 +96,13 @@ int filter_graph(const zg_graph *in, float thr, int keep_ge, uint64_t seed, zg_g
 void free_graph(zg_graph *g);
 
 // estimator launch (zg_kernels.hip): enqueues on g->stream, results in g->result
+// order: 0 automatic, 1 estimator-id order, 2..16 first-edge order with 2^order buckets
 int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uint64_t count,
-               zg_outcome *d_out, int block, int blocks_per_cu, unsigned long long *reads = nullptr);
+               zg_outcome *d_out, int block, int blocks_per_cu, unsigned long long *reads = nullptr,
+               int order = 1);
+// launches of at most this many estimators when they run in first-edge order
+constexpr uint64_t kMaxOrderedLaunch = 1ull << 28;
+int resolve_order(const zg_graph *g, const zg_plan *p, uint64_t count, int order);
 void finish_sums(const Acc *host_acc, int nch, int K, uint64_t count, zg_sums *out);
 // one kernel kind's batched launch (the plan kind, or 100 + k for k-vertex
 // patterns): counter reset, work-queue kernel, per-query reduction

@@ -2,6 +2,10 @@
 // (src/ZGraphInstance.cpp:256-297) as one HIP kernel per plan: one estimator
 // per lane, grid-stride over estimator ids, exact integer reduction of the
 // values (32-bit digit sums), wave -> workgroup -> grid, all deterministic.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
 #include "zg_internal.h"
 
 namespace zg {
@@ -594,6 +598,46 @@ __global__ __launch_bounds__(256) void k_run(GView g, PlanView P, uint64_t seed,
     block_reduce_store<NCH>(acc, partials + (size_t)blockIdx.x * NCH);
 }
 
+// First-edge order.  Every plan's first random choice is the rank of its first
+// edge (after the method draw for FourChain), uniform over the m ranks, so a
+// launch in estimator-id order reads the 32-byte edge records in random order:
+// about one 128-byte DRAM line per estimator.  Sorting the launch's estimators
+// by the high bits of that rank first makes the in-flight estimators share
+// record lines in the caches.  The set of estimators and each one's outcome are
+// unchanged; the exact sums are identical.
+template <bool METHOD_FIRST>
+__global__ void k_first_edge_keys(uint64_t m, uint64_t seed, uint64_t first, uint64_t count, uint32_t shift,
+                                  uint32_t *__restrict__ keys, uint32_t *__restrict__ offs)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
+        Rng rng;
+        rng.init(seed, TAG_ESTIMATOR, first + i);
+        if (METHOD_FIRST) (void)rng.uniform(3);
+        const uint32_t r0 = (uint32_t)rng.uniform(m);
+        keys[i] = r0 >> shift;
+        offs[i] = (uint32_t)i;
+    }
+}
+
+template <int KIND, int NCH>
+__global__ __launch_bounds__(256) void k_run_ordered(GView g, PlanView P, uint64_t seed, uint64_t first,
+                                                     uint64_t count, const uint32_t *__restrict__ order,
+                                                     Acc *__restrict__ partials)
+{
+    AccL acc[NCH];
+    acc_zero<NCH>(acc);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += stride) {
+        Rng rng;
+        rng.init(seed, TAG_ESTIMATOR, first + order[j]);
+        Est o;
+        run_one<KIND>(g, P, rng, o);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) acc_add(acc[c], o.v[c], o.st[c], o.den[c], P.K);
+    }
+    block_reduce_store<NCH>(acc, partials + (size_t)blockIdx.x * NCH);
+}
+
 // Batched queries of one plan kind in one persistent launch.  The estimator
 // ids of every query are cut into chunks of `chunk_est`; workgroups claim
 // chunks from a global counter (a work queue) until it passes `nchunks`, and
@@ -722,11 +766,15 @@ __global__ __launch_bounds__(256) void k_final(const Acc *__restrict__ partials,
 
 template <int KIND, int NCH>
 static void launch_kind(const zg_graph *g, const PlanView &P, uint64_t seed, uint64_t first, uint64_t count,
-                        zg_outcome *d_out, int grid, int block, unsigned long long *reads)
+                        zg_outcome *d_out, int grid, int block, unsigned long long *reads,
+                        const uint32_t *order = nullptr)
 {
     GView v = g->view();
     v.reads = reads;
-    if (d_out)
+    if (order)
+        hipLaunchKernelGGL((k_run_ordered<KIND, NCH>), dim3(grid), dim3(block), 0, g->stream, v, P, seed, first,
+                           count, order, g->partials);
+    else if (d_out)
         hipLaunchKernelGGL((k_run<KIND, NCH, true>), dim3(grid), dim3(block), 0, g->stream, v, P, seed, first, count,
                            d_out, g->partials);
     else
@@ -734,8 +782,59 @@ static void launch_kind(const zg_graph *g, const PlanView &P, uint64_t seed, uin
                            count, d_out, g->partials);
 }
 
+int resolve_order(const zg_graph *g, const zg_plan *p, uint64_t count, int order)
+{
+    (void)p;
+    if (order == 0)   // automatic: big launches on graphs whose edge records exceed the caches
+        return (count >= (1ull << 22) && g->m * sizeof(ERec) >= (128ull << 20)) ? 8 : 1;
+    return order;
+}
+
+// sorts the launch's estimator offsets by first-edge bucket into g->ord
+static int order_by_first_edge(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uint64_t count,
+                               int bits, const uint32_t **order)
+{
+    if (count > kMaxOrderedLaunch) return set_error(ZG_EINVAL, "ordered launch too large");
+    if (count > g->ord_cap) {
+        for (uint32_t *&q : g->ord) {
+            if (q) ZG_HIP(hipFree(q));
+            q = nullptr;
+        }
+        g->ord_cap = 0;
+        for (uint32_t *&q : g->ord)
+            if (hipMalloc(&q, 4 * count) != hipSuccess) return set_error(ZG_ENOMEM, "ordering scratch: allocation failed");
+        g->ord_cap = count;
+    }
+    unsigned mbits = 1;
+    while (mbits < 32 && (1ull << mbits) < g->m) ++mbits;
+    const unsigned kbits = (unsigned)std::min<int>(bits, (int)mbits);
+    const uint32_t shift = mbits - kbits;
+    const int grid = std::max(1, std::min(device_cus() * 16, (int)((count + 255) / 256)));
+    if (p->desc.kind == ZG_PLAN_FOUR_CHAIN)
+        hipLaunchKernelGGL(k_first_edge_keys<true>, dim3(grid), dim3(256), 0, g->stream, g->m, seed, first, count,
+                           shift, g->ord[0], g->ord[1]);
+    else
+        hipLaunchKernelGGL(k_first_edge_keys<false>, dim3(grid), dim3(256), 0, g->stream, g->m, seed, first, count,
+                           shift, g->ord[0], g->ord[1]);
+    ZG_HIP(hipGetLastError());
+    rocprim::double_buffer<uint32_t> kb(g->ord[0], g->ord[2]);
+    rocprim::double_buffer<uint32_t> vb(g->ord[1], g->ord[3]);
+    size_t tb = 0;
+    ZG_HIP(rocprim::radix_sort_pairs(nullptr, tb, kb, vb, (size_t)count, 0, kbits, g->stream));
+    if (tb > g->ord_tmp_cap) {
+        if (g->ord_tmp) ZG_HIP(hipFree(g->ord_tmp));
+        g->ord_tmp = nullptr;
+        g->ord_tmp_cap = 0;
+        if (hipMalloc(&g->ord_tmp, tb) != hipSuccess) return set_error(ZG_ENOMEM, "ordering scratch: allocation failed");
+        g->ord_tmp_cap = tb;
+    }
+    ZG_HIP(rocprim::radix_sort_pairs(g->ord_tmp, tb, kb, vb, (size_t)count, 0, kbits, g->stream));
+    *order = vb.current();
+    return ZG_OK;
+}
+
 int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uint64_t count,
-               zg_outcome *d_out, int block, int blocks_per_cu, unsigned long long *reads)
+               zg_outcome *d_out, int block, int blocks_per_cu, unsigned long long *reads, int order)
 {
     if (block <= 0) block = 256;
     if (block % 64 || block > 1024 || block > 256)
@@ -753,23 +852,29 @@ int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
         g->partials_cap = grid * nch;
     }
     const PlanView P = p->view();
+    const uint32_t *ord = nullptr;
+    if (order < 0 || order > 16) return set_error(ZG_EINVAL, "order must be 0 (automatic), 1 (id) or 2..16");
+    if (order >= 2 && !d_out && !reads && count > 0 && g->m > 0) {
+        int rc = order_by_first_edge(g, p, seed, first, count, order, &ord);
+        if (rc) return rc;
+    }
     switch (p->desc.kind) {
-    case ZG_PLAN_TRIANGLE: launch_kind<ZG_PLAN_TRIANGLE, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
-    case ZG_PLAN_THREE_CHAIN: launch_kind<ZG_PLAN_THREE_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
-    case ZG_PLAN_THREE_MOTIF: launch_kind<ZG_PLAN_THREE_MOTIF, 2>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
-    case ZG_PLAN_CHAIN: launch_kind<ZG_PLAN_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
-    case ZG_PLAN_FOUR_CHAIN: launch_kind<ZG_PLAN_FOUR_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+    case ZG_PLAN_TRIANGLE: launch_kind<ZG_PLAN_TRIANGLE, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
+    case ZG_PLAN_THREE_CHAIN: launch_kind<ZG_PLAN_THREE_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
+    case ZG_PLAN_THREE_MOTIF: launch_kind<ZG_PLAN_THREE_MOTIF, 2>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
+    case ZG_PLAN_CHAIN: launch_kind<ZG_PLAN_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
+    case ZG_PLAN_FOUR_CHAIN: launch_kind<ZG_PLAN_FOUR_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
     case ZG_PLAN_PATTERN:
         switch (P.pk) {
-        case 2: launch_kind<KIND_PATTERN_BASE + 2, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
-        case 3: launch_kind<KIND_PATTERN_BASE + 3, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
-        case 4: launch_kind<KIND_PATTERN_BASE + 4, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
-        case 5: launch_kind<KIND_PATTERN_BASE + 5, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
-        case 6: launch_kind<KIND_PATTERN_BASE + 6, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+        case 2: launch_kind<KIND_PATTERN_BASE + 2, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
+        case 3: launch_kind<KIND_PATTERN_BASE + 3, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
+        case 4: launch_kind<KIND_PATTERN_BASE + 4, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
+        case 5: launch_kind<KIND_PATTERN_BASE + 5, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
+        case 6: launch_kind<KIND_PATTERN_BASE + 6, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
         default: return set_error(ZG_EINVAL, "pattern vertices must be in [2, 6]");
         }
         break;
-    case ZG_PLAN_K4_CLOSE: launch_kind<ZG_PLAN_K4_CLOSE, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads); break;
+    case ZG_PLAN_K4_CLOSE: launch_kind<ZG_PLAN_K4_CLOSE, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
     default: return set_error(ZG_EINVAL, "unknown plan kind");
     }
     ZG_HIP(hipGetLastError());

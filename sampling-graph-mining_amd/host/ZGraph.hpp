@@ -536,7 +536,7 @@ class ZGraphInstance {
     {
         require_plan();
         std::vector<zg_sums> s((size_t)plan_->channels());
-        zg_launch launch{0, 0};
+        zg_launch launch{0, 0, 0};
         check(zg_run(get_run_graph()->handle(), plan_->handle(), seed, first_id, count, &launch, s.data(),
                      kernel_ms));
         return s[(size_t)channel];
@@ -613,7 +613,7 @@ class ThreeMotifInstance : public ZGraphInstance<EdgeData> {
     {
         this->require_plan();
         zg_sums s[2];
-        zg_launch launch{0, 0};
+        zg_launch launch{0, 0, 0};
         check(zg_run(this->get_run_graph()->handle(), this->plan()->handle(), this->seed, 0, num_estimators, &launch,
                      s, nullptr));
         return {s[0].mean, s[1].mean};

@@ -273,10 +273,11 @@ class Plan:
             pass
 
 
-def run(graph, plan, seed, first_id, count, block_size=0, blocks_per_cu=0, timed=False):
-    """One launch of the estimator loop; returns the per-channel sums (and kernel ms)."""
+def run(graph, plan, seed, first_id, count, block_size=0, blocks_per_cu=0, timed=False, order=0):
+    """One launch of the estimator loop; returns the per-channel sums (and kernel
+    ms).  order: 0 automatic, 1 estimator-id order, 2..16 first-edge order."""
     sums = (_capi.Sums * plan.channels)()
-    launch = _capi.Launch(block_size, blocks_per_cu)
+    launch = _capi.Launch(block_size, blocks_per_cu, order)
     ms = C.c_float(0)
     check(lib().zg_run(graph.h, plan.h, seed, first_id, count, C.byref(launch), sums,
                        C.byref(ms) if timed else None))

@@ -89,7 +89,7 @@ class Outcome(C.Structure):
 
 
 class Launch(C.Structure):
-    _fields_ = [("block_size", C.c_int), ("blocks_per_cu", C.c_int)]
+    _fields_ = [("block_size", C.c_int), ("blocks_per_cu", C.c_int), ("order", C.c_int)]
 
 
 class ElpResult(C.Structure):
