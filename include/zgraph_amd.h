@@ -170,12 +170,11 @@ typedef struct {
 typedef struct {
     int block_size;     /* threads per workgroup (multiple of 64; 0 = default 256) */
     int blocks_per_cu;  /* workgroups per CU in the grid (0 = default) */
-    int order;          /* 0 automatic; 1 estimator-id order; b in 2..16: estimators
-                           sorted by the first edge's rank into 2^b buckets before the
-                           launch (cache locality of the edge records; same outcomes
-                           and exact sums, sumsq up to rounding).  Automatic picks
-                           b = 8 for runs of >= 2^22 estimators on graphs whose edge
-                           records exceed 128 MB. */
+    int order;          /* 0 or 1: estimator-id order; b in 2..16: estimators sorted
+                           by the first edge's rank into 2^b buckets before the launch
+                           (cache locality of the edge records; same outcomes and exact
+                           sums, sumsq up to rounding).  Opt-in: measured slower on
+                           MI355X for the 69 M-edge benchmark graph, sort included. */
 } zg_launch;
 
 /* The calculate() loop: runs ids [first_id, first_id+count) and reduces
