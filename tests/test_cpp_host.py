@@ -152,3 +152,22 @@ def test_learned_profile_cpp_equals_python(tmp_path, pattern, kind):
     assert abs(rec["sized_estimators"] - sized) <= 1
     if rec["estimators"] == cnt:
         assert est == pytest.approx(rec["estimate"], rel=1e-12)
+
+
+@pytest.mark.gpu
+def test_triangle_app_error_profile_equals_python(tmp_path):
+    """Triangle main without a budget sizes by ErrorProfile (the reference's
+    flow, +-5 % at 95 %): the C++ ErrorProfile and the Python one give the
+    same budget and the same estimate on the same graph and seed."""
+    n, s, d = golden_graph("wiki-vote")
+    prefix = _write_graph(tmp_path, "wv", n, s, d)
+    r = subprocess.run([_bin("Triangle"), prefix], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    est, cnt = _report(r.stdout)["triangle"]
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    inst = Z.TriangleCoutingInstance(g, seed=1)
+    N = Z.ErrorProfile().get_num_estimators(inst, 3, 0.05, 0.95)
+    print(f"[cpp] ErrorProfile N={N} (C++ {cnt})", flush=True)
+    assert cnt == N
+    ref = Z.run(g, Z.Plan(Z.PLAN_TRIANGLE), 1, 0, N)[0]
+    assert est == pytest.approx(ref.total() / 2 ** ref.log2_den / N, rel=1e-12)

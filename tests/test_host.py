@@ -120,3 +120,43 @@ def test_pattern_unique_relabelings_product(pattern, expect):
     reference's TestPattern2 counts and agrees with the oracle."""
     assert pattern.unique_relabelings() == expect
     assert O.pattern_unique(pattern.num_vertices, pattern.edges) == expect
+
+
+def test_time_profile_doubling_rule(monkeypatch):
+    """TimeProfile.run follows src/ErrorLatencyProfile.cpp:24-47: doubling
+    batches while used + 2*last <= limit, then one batch of
+    next*(limit-used)/(2*last); batches take consecutive disjoint id ranges.
+    Driven by a fake clock (1 ms per 2^16 estimators) and a fake run."""
+    clock = {"t": 0.0}
+    calls = []
+
+    class FakeTime:
+        @staticmethod
+        def perf_counter():
+            return clock["t"]
+
+    class S:
+        def __init__(self, mean):
+            self.mean = mean
+
+    def fake_run(graph, plan, seed, first, count, *a, **k):
+        calls.append((first, count))
+        clock["t"] += count / 65536 * 1e-3
+        return [S(float(first % 7))]
+
+    monkeypatch.setattr(Z, "time", FakeTime)
+    monkeypatch.setattr(Z, "run", fake_run)
+
+    class Inst:
+        graph = plan = None
+        seed = 1
+        channel = 0
+
+    est = Z.TimeProfile(1 << 16).run(Inst(), 3, 100)
+    sizes = [c for _, c in calls]
+    assert sizes[:6] == [1 << (16 + i) for i in range(6)]          # 1, 2, 4, 8, 16, 32 ms
+    assert sizes[6] == (1 << 22) * (100 - 63) // (32 * 2)          # the remaining-time batch
+    assert len(calls) == 7
+    assert all(a[0] + a[1] == b[0] for a, b in zip(calls, calls[1:])) and calls[0][0] == 0
+    want = sum((f % 7) * c for f, c in calls) / sum(sizes)
+    assert est == pytest.approx(want)
