@@ -23,6 +23,7 @@ class NetworkInterface {
     }
     int get_partition_id() const { return 0; }
     int get_num_partitions() const { return 1; }
+    static void finalize_instance() {}
 };
 
 class SharedMemSys {
@@ -65,13 +66,23 @@ class Timer {
     {
         const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - starts()[name]).count();
         std::cerr << "[timer] " << name << ": " << s << " s" << std::endl;
+        totals()[name] += s;
         return s;
+    }
+    static void report_timers()
+    {
+        for (const auto &kv : totals()) std::cerr << "[timer] total " << kv.first << ": " << kv.second << " s" << std::endl;
     }
 
   private:
     static std::map<std::string, std::chrono::steady_clock::time_point> &starts()
     {
         static std::map<std::string, std::chrono::steady_clock::time_point> m;
+        return m;
+    }
+    static std::map<std::string, double> &totals()
+    {
+        static std::map<std::string, double> m;
         return m;
     }
 };

@@ -171,3 +171,25 @@ def test_triangle_app_error_profile_equals_python(tmp_path):
     assert cnt == N
     ref = Z.run(g, Z.Plan(Z.PLAN_TRIANGLE), 1, 0, N)[0]
     assert est == pytest.approx(ref.total() / 2 ** ref.log2_den / N, rel=1e-12)
+
+
+@pytest.mark.gpu
+def test_triangle2_and_three_motif_without_cache_apps(tmp_path):
+    n, s, d = golden_graph("wiki-vote")
+    prefix = _write_graph(tmp_path, "wv", n, s, d)
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    N, seed = 1 << 20, 6
+    r = subprocess.run([_bin("Triangle2"), prefix, str(N), str(seed)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    ref = Z.run(g, Z.Plan(Z.PLAN_PATTERN, pattern=Z.TRIANGLE_PATTERN), seed, 0, N)[0]
+    assert _report(r.stdout)["triangle-pattern"][0] == pytest.approx(ref.total() / 2 ** ref.log2_den / N,
+                                                                     rel=1e-13)
+    nc, nt = 1 << 19, 3 << 18
+    r = subprocess.run([_bin("ThreeMotifWithoutCache"), prefix, str(nc), str(nt), str(seed)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = _report(r.stdout)
+    rc = Z.run(g, Z.Plan(Z.PLAN_THREE_CHAIN), seed, 0, nc)[0]
+    rt = Z.run(g, Z.Plan(Z.PLAN_TRIANGLE), seed, 0, nt)[0]
+    assert rows["3-chain"] == (pytest.approx(rc.total() / 2 ** rc.log2_den / nc, rel=1e-13), nc)
+    assert rows["triangle"] == (pytest.approx(rt.total() / 2 ** rt.log2_den / nt, rel=1e-13), nt)
