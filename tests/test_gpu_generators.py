@@ -75,6 +75,16 @@ def test_device_house_count_equals_brute_force(n, p, seed):
     assert g.exact_count("house") == og.house_brute()
 
 
+def test_device_house_count_matches_the_reference_ground_truth():
+    """applications/FiveHouse.cpp:68 hard-codes 2,134,981,304 houses for its
+    graph: the simple graph of the reference's wiki-vote fixture has exactly
+    that many."""
+    import helpers as H
+    n, s, d = H.golden_graph("wiki-vote")
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    assert g.exact_count("house") == 2134981304
+
+
 def test_device_house_count_agrees_with_the_house_estimator():
     """G(90, 0.4): the exact count (new algorithm) lies in the 99.9 % interval
     of 2^26 unbiased SamplerGenerator house estimators."""
