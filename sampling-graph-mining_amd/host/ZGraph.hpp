@@ -427,6 +427,72 @@ struct Edge {
     VertexId dst;
 };
 
+// ---- per-estimator state types (include/ZGraphInstance.hpp:30-105) ---------
+// Host value types with the reference's semantics.  The device plans keep
+// this state in registers; these serve host code that describes or checks
+// sampled sub-graphs (and the reference's tests/TestSubgraph.cpp).
+constexpr int MAX_LIST_SIZE = 16;
+
+template <typename Data>
+struct MyList {
+    int list_size = 0;
+    Data list[MAX_LIST_SIZE];
+    void push_back(const Data &d)
+    {
+        if (list_size >= MAX_LIST_SIZE) throw Error(ZG_EINVAL, "MyList: more than 16 entries");
+        list[list_size++] = d;
+    }
+};
+
+template <typename EdgeData>
+struct SubGraph {
+    MyList<Edge<EdgeData>> edges;
+    MyList<VertexId> vertices;
+    EdgeId min_valid_edge_id = 0;       // one past the largest edge id added
+    VertexId min_valid_vertex_id = 0;   // one past the largest vertex added
+
+    bool is_in_vertices_list(VertexId v) const
+    {
+        for (int i = 0; i < vertices.list_size; ++i)
+            if (vertices.list[i] == v) return true;
+        return false;
+    }
+    // appends the edge, then its src and dst (in that order) if new (src/ZGraphInstance.cpp:56-73)
+    void add_edge(const Edge<EdgeData> &e)
+    {
+        edges.push_back(e);
+        if (e.edge_id >= min_valid_edge_id) min_valid_edge_id = e.edge_id + 1;
+        add_vertex(e.src);
+        add_vertex(e.dst);
+    }
+    void add_vertex(VertexId v)
+    {
+        if (is_in_vertices_list(v)) return;
+        vertices.push_back(v);
+        if (v >= min_valid_vertex_id) min_valid_vertex_id = v + 1;
+    }
+    const MyList<Edge<EdgeData>> &get_edges() const { return edges; }
+    const MyList<VertexId> &get_vertices() const { return vertices; }
+    EdgeId get_min_valid_edge_id() const { return min_valid_edge_id; }
+    VertexId get_min_valid_vertex_id() const { return min_valid_vertex_id; }
+    // the edges of this sub-graph whose endpoints (either orientation) are not an edge of `o`
+    // (src/ZGraphInstance.cpp:96-113)
+    SubGraph operator-(const SubGraph &o) const
+    {
+        SubGraph r;
+        for (int i = 0; i < edges.list_size; ++i) {
+            const Edge<EdgeData> &a = edges.list[i];
+            bool drop = false;
+            for (int j = 0; j < o.edges.list_size && !drop; ++j) {
+                const Edge<EdgeData> &b = o.edges.list[j];
+                drop = (a.src == b.src && a.dst == b.dst) || (a.src == b.dst && a.dst == b.src);
+            }
+            if (!drop) r.add_edge(a);
+        }
+        return r;
+    }
+};
+
 class Pattern {
   public:
     Pattern(VertexId num_vertices_, EdgeId num_edges_, const Edge<Empty> *edges_) : nv_(num_vertices_)
@@ -975,6 +1041,8 @@ using zgraph::FourChainCountingInstance;
 using zgraph::FourCliqueCountingInstance;
 using zgraph::get_globally_consistent_seed;
 using zgraph::Graph;
+using zgraph::MyList;
+using zgraph::SubGraph;
 using zgraph::LearnedErrorProfile;
 using zgraph::GraphDestroyer;
 using zgraph::GraphFilter;

@@ -99,6 +99,25 @@ static void test_estimation()
     EXPECT(std::fabs(zgraph::normal_quantile_two_sided(0.95) - 1.959963984540054) < 1e-9);
 }
 
+// tests/TestSubgraph.cpp: {01, 02, 12} - {01, 02} = {12}
+static void test_subgraph()
+{
+    SubGraph<Empty> a, b;
+    a.add_edge(Edge<Empty>{4, 0, 1});
+    a.add_edge(Edge<Empty>{9, 0, 2});
+    a.add_edge(Edge<Empty>{2, 1, 2});
+    b.add_edge(Edge<Empty>{0, 1, 0});   // either orientation matches
+    b.add_edge(Edge<Empty>{1, 0, 2});
+    EXPECT(a.get_min_valid_edge_id() == 10);
+    EXPECT(a.get_vertices().list_size == 3 && a.get_vertices().list[0] == 0 && a.get_vertices().list[1] == 1 &&
+           a.get_vertices().list[2] == 2);
+    const SubGraph<Empty> d = a - b;
+    std::printf("subgraph difference: %d edge(s), first %u-%u\n", d.get_edges().list_size, d.get_edges().list[0].src,
+                d.get_edges().list[0].dst);
+    EXPECT(d.get_edges().list_size == 1 && d.get_edges().list[0].src == 1 && d.get_edges().list[0].dst == 2);
+    EXPECT(d.get_min_valid_edge_id() == 3 && d.get_vertices().list_size == 2);
+}
+
 // src/ErrorLatencyProfile.cpp:51-59
 static void test_bound()
 {
@@ -162,6 +181,7 @@ int main(int argc, char **argv)
     const std::string dir = argc > 1 ? argv[1] : ".";
     test_regression();
     test_patterns();
+    test_subgraph();
     test_estimation();
     test_bound();
     test_files(dir);
