@@ -192,6 +192,10 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95):
         if learned is not None:
             profiles.append(("learned+judge", learned, True, 1 << 24))
         for label, prof, judged, fallback in profiles:
+            # one untimed query first (code-object loads, first allocations), as a serving loop would have
+            warm = Z.ZGraphInstance(g, Z.Plan(kind), seed=100)
+            L.estimate_to_error(warm, eps, conf, prof, judged=judged, max_estimators=1 << 30,
+                                fallback_budget=fallback)
             recs = []
             for q in range(args.elp_seeds):
                 inst = Z.ZGraphInstance(g, Z.Plan(kind), seed=101 + q)

@@ -313,7 +313,9 @@ struct zg_mlp {
     size_t partial_rows = 0;
     int t = 0;
     hipStream_t stream = nullptr;
+    float *io = nullptr;         // inputs and outputs of small forward calls (kSmallRows rows)
 };
+constexpr int kSmallRows = 64;   // forward calls up to this many rows reuse zg_mlp::io
 
 using namespace zg;
 using namespace zg::mlp;
@@ -397,7 +399,7 @@ extern "C" int zg_mlp_create(uint64_t seed, zg_mlp **out)
 extern "C" int zg_mlp_destroy(zg_mlp *m)
 {
     if (!m) return ZG_OK;
-    for (float *p : {m->P, m->M, m->V, m->grad, m->loss, m->partials})
+    for (float *p : {m->P, m->M, m->V, m->grad, m->loss, m->partials, m->io})
         if (p) (void)hipFree(p);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
@@ -425,12 +427,21 @@ extern "C" int zg_mlp_forward(zg_mlp *m, const float *X, int B, float *Y)
 {
     if (!m || !X || !Y || B <= 0) return set_error(ZG_EINVAL, "bad arguments");
     DevF dX, dY;
-    ZG_HIP(hipMalloc(&dX.p, sizeof(float) * IN * (size_t)B));
-    ZG_HIP(hipMalloc(&dY.p, sizeof(float) * (size_t)B));
-    ZG_HIP(hipMemcpyAsync(dX.p, X, sizeof(float) * IN * (size_t)B, hipMemcpyHostToDevice, m->stream));
-    hipLaunchKernelGGL(k_forward, dim3((B + 15) / 16), dim3(64), 0, m->stream, m->P, dX.p, B, dY.p);
+    float *x = nullptr, *y = nullptr;
+    if (B <= kSmallRows) {   // the sizing path: one row per query, no allocation per call
+        if (!m->io) ZG_HIP(hipMalloc(&m->io, sizeof(float) * (IN + 1) * kSmallRows));
+        x = m->io;
+        y = m->io + IN * kSmallRows;
+    } else {
+        ZG_HIP(hipMalloc(&dX.p, sizeof(float) * IN * (size_t)B));
+        ZG_HIP(hipMalloc(&dY.p, sizeof(float) * (size_t)B));
+        x = dX.p;
+        y = dY.p;
+    }
+    ZG_HIP(hipMemcpyAsync(x, X, sizeof(float) * IN * (size_t)B, hipMemcpyHostToDevice, m->stream));
+    hipLaunchKernelGGL(k_forward, dim3((B + 15) / 16), dim3(64), 0, m->stream, m->P, x, B, y);
     ZG_HIP(hipGetLastError());
-    ZG_HIP(hipMemcpyAsync(Y, dY.p, sizeof(float) * (size_t)B, hipMemcpyDeviceToHost, m->stream));
+    ZG_HIP(hipMemcpyAsync(Y, y, sizeof(float) * (size_t)B, hipMemcpyDeviceToHost, m->stream));
     ZG_HIP(hipStreamSynchronize(m->stream));
     return ZG_OK;
 }
