@@ -22,8 +22,8 @@ def say(*a):
 
 
 @pytest.fixture(scope="module")
-def cfg1():
-    say("[cfg1] Erdos-Renyi n=10000 m=100000 seed=1")
+def er_10k_100k():
+    say("[er-10k-100k] Erdos-Renyi n=10000 m=100000 seed=1")
     e = Z.erdos_renyi(10000, 100000, seed=1)
     s, d, _ = e.download()
     g = Z.Graph.from_device(e, seed=1)
@@ -36,19 +36,19 @@ def within(est, exact, z=4.0):
     return abs(est.exact_mean() - exact) <= hw
 
 
-def test_cfg1_triangle_within_interval(cfg1):
-    g, og = cfg1
+def test_er_10k_100k_triangle_within_interval(er_10k_100k):
+    g, og = er_10k_100k
     T = og.triangles()
     inst = Z.TriangleCoutingInstance(g, seed=1)
     est = Z.Estimation()
     val = inst.calculate(1_000_000, 3, est)
-    say(f"[cfg1] exact={T} estimate={val:.1f} half-width95={est.half_width():.1f}")
+    say(f"[er-10k-100k] exact={T} estimate={val:.1f} half-width95={est.half_width():.1f}")
     assert within(est, T)
     assert val == pytest.approx(est.exact_mean(), rel=1e-12)
 
 
-def test_top_up_equals_single_run(cfg1):
-    g, _ = cfg1
+def test_top_up_equals_single_run(er_10k_100k):
+    g, _ = er_10k_100k
     inst = Z.FourCliqueCountingInstance(g, seed=9)
     a = Z.Estimation()
     inst.calculate(300_000, 4, a)

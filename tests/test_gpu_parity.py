@@ -109,7 +109,7 @@ def test_outcomes_bit_exact(gname, name, oargs):
         assert (zo["status"] == 1).any()
 
 
-def test_cfg1_triangle_one_million_bit_exact():
+def test_er_10k_100k_triangle_one_million_bit_exact():
     # Erdos-Renyi 10,000 vertices / 100,000 edges, 10^6 estimators, Philox seed 1
     e = Z.erdos_renyi(10000, 100000, seed=1)
     s, d, _ = e.download()
