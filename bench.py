@@ -410,10 +410,33 @@ def cpu_baseline(Z, g, plans, args, N):
         samples[name] = target
         log(f"cpu {name}: {target} estimators in {spent:.2f}s = {rates[name]:.4g}/s")
     per_step = sum(N / r for r in rates.values())
+    # cost-faithful variant: per-thread MT19937 draws as the reference's Random (timing only)
+    mt = {}
+    O.lib().og_set_rng_mode(1)
+    try:
+        for name, kind in (("triangle", O.PLAN_TRIANGLE), ("4-clique", O.PLAN_K4_CLOSE)):
+            t = time.perf_counter()
+            O.run(og, O.make_plan(kind), args.seed, 0, samples[name], threads)
+            mt[name] = samples[name] / (time.perf_counter() - t)
+            log(f"cpu {name} (mt19937 draws): {mt[name]:.4g}/s")
+    finally:
+        O.lib().og_set_rng_mode(0)
     return {"value": 2 * N / per_step, "unit": "estimators/s", "cores": threads, "kind": "port",
             "sample": "; ".join(f"{k}: {samples[k]} estimators (ids 0..) of the same graph/seed"
                                  for k in samples),
-            "per_pattern": rates}
+            "per_pattern": rates, "cpu_model": _cpu_model(),
+            "mt19937_draws": {"value": 2 * N / sum(N / r for r in mt.values()), "per_pattern": mt}}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 if __name__ == "__main__":

@@ -67,8 +67,59 @@ void og_stream_init(og_stream *s, uint64_t seed, uint32_t tag, uint64_t id)
     s->buf_block = UINT64_MAX;
 }
 
+/* MT19937 (Matsumoto and Nishimura, ACM TOMACS 1998), the engine behind the
+ * reference's Random (include/Random.hpp:6-16: std::mt19937 per thread).  Only
+ * for the cost-faithful CPU baseline: its draws are not the Philox streams, so
+ * runs in this mode are timed, never compared. */
+typedef struct { uint32_t mt[624]; int idx; } og_mt;
+static int og_rng_mode = 0;            /* 0 Philox streams, 1 per-thread MT19937 */
+static __thread og_mt tl_mt;
+static __thread int tl_mt_seeded = 0;
+
+static void mt_seed(og_mt *s, uint32_t seed)
+{
+    s->mt[0] = seed;
+    for (int i = 1; i < 624; ++i) s->mt[i] = 1812433253u * (s->mt[i - 1] ^ (s->mt[i - 1] >> 30)) + (uint32_t)i;
+    s->idx = 624;
+}
+
+static uint32_t mt_next(og_mt *s)
+{
+    if (s->idx >= 624) {
+        for (int i = 0; i < 624; ++i) {
+            const uint32_t y = (s->mt[i] & 0x80000000u) | (s->mt[(i + 1) % 624] & 0x7fffffffu);
+            s->mt[i] = s->mt[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908B0DFu : 0u);
+        }
+        s->idx = 0;
+    }
+    uint32_t y = s->mt[s->idx++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9D2C5680u;
+    y ^= (y << 15) & 0xEFC60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+void og_set_rng_mode(int mode) { og_rng_mode = mode; }
+
+uint32_t og_mt19937_nth(uint32_t seed, uint32_t n)
+{
+    og_mt s;
+    mt_seed(&s, seed);
+    uint32_t y = 0;
+    for (uint32_t i = 0; i < n; ++i) y = mt_next(&s);
+    return y;
+}
+
 uint32_t og_next_u32(og_stream *s)
 {
+    if (og_rng_mode) {
+        if (!tl_mt_seeded) {
+            mt_seed(&tl_mt, 5489u + (uint32_t)omp_get_thread_num());
+            tl_mt_seeded = 1;
+        }
+        return mt_next(&tl_mt);
+    }
     uint64_t blk = s->word >> 2;
     if (blk != s->buf_block) {
         uint32_t ctr[4] = {(uint32_t)blk, s->tag, (uint32_t)s->id, (uint32_t)(s->id >> 32)};

@@ -104,6 +104,10 @@ def lib():
         L.og_exact_wedges.argtypes = [P(Graph)]
         L.og_exact_paths_brute.restype = C.c_uint64
         L.og_exact_paths_brute.argtypes = [P(Graph), C.c_int]
+        L.og_set_rng_mode.restype = None
+        L.og_set_rng_mode.argtypes = [C.c_int]
+        L.og_mt19937_nth.restype = C.c_uint32
+        L.og_mt19937_nth.argtypes = [C.c_uint32, C.c_uint32]
         L.og_exact_stars64.restype = None
         L.og_exact_stars64.argtypes = [P(Graph), C.c_int, P(C.c_uint64), P(C.c_uint64)]
         L.og_exact_house_brute.restype = C.c_uint64

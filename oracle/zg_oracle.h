@@ -48,6 +48,9 @@ typedef struct {
 
 void og_stream_init(og_stream *s, uint64_t seed, uint32_t tag, uint64_t id);
 uint32_t og_next_u32(og_stream *s);
+/* 0: Philox streams (parity); 1: per-thread MT19937 draws (cost-faithful timing only) */
+void og_set_rng_mode(int mode);
+uint32_t og_mt19937_nth(uint32_t seed, uint32_t n);   /* n-th output of MT19937 seeded `seed` */
 /* U[0, n-1], 1 <= n <= 2^64-1 */
 uint64_t og_uniform(og_stream *s, uint64_t n);
 /* first 64 bits of block 0 of (seed, tag, id): the per-edge permutation key */

@@ -336,3 +336,17 @@ def test_star_and_house_closed_forms():
         og = O.OGraph(n, s, d, seed=1)
         assert og.stars(4) == k * comb(k - 1, 4)
         assert og.house_brute() == perm(k, 5) // 2
+
+
+def test_mt19937_engine_matches_the_standard(tmp_path):
+    """The cost-faithful baseline's MT19937: the C++ standard's check value
+    ([rand.predef]: the 10000th output of a default-seeded mt19937 is
+    4123659995) and std::mt19937's first outputs for seed 1 (g++)."""
+    assert O.lib().og_mt19937_nth(5489, 10000) == 4123659995
+    src = tmp_path / "mt.cpp"
+    src.write_text("#include <random>\n#include <cstdio>\nint main(){std::mt19937 g(1u);"
+                   "for(int i=0;i<5;++i)std::printf(\"%u\\n\",(unsigned)g());}\n")
+    exe = tmp_path / "mt"
+    subprocess.check_call(["g++", "-O1", "-o", str(exe), str(src)])
+    want = [int(x) for x in subprocess.check_output([str(exe)]).split()]
+    assert [O.lib().og_mt19937_nth(1, k) for k in range(1, 6)] == want
