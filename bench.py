@@ -384,6 +384,7 @@ def profiled_traffic(kernel, args, N):
 def cpu_baseline(Z, g, plans, args, N):
     """The oracle's OpenMP restatement of ZGraphInstance::calculate on this
     host, on a bounded number of estimators of the same workload."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")   # read when the OpenMP runtime starts
     import pyoracle as O
     threads = min(16, os.cpu_count() or 1)
     log(f"cpu baseline: exporting CSR, {threads} threads")
