@@ -160,3 +160,21 @@ def test_time_profile_doubling_rule(monkeypatch):
     assert all(a[0] + a[1] == b[0] for a, b in zip(calls, calls[1:])) and calls[0][0] == 0
     want = sum((f % 7) * c for f, c in calls) / sum(sizes)
     assert est == pytest.approx(want)
+
+
+def test_abi_argument_errors_without_gpu():
+    """Argument checks of the C ABI that run before any device work return
+    ZG_EINVAL with a message (host logic; no GPU needed)."""
+    L = _capi.lib()
+    assert L.zg_plan_create(None, None) == _capi.ZG_EINVAL
+    assert L.zg_run(None, None, 1, 0, 10, None, None, None) == _capi.ZG_EINVAL
+    assert L.zg_exact_counts(None, 0, None) == _capi.ZG_EINVAL
+    assert L.zg_batch_create(None, 0, None, None) == _capi.ZG_EINVAL
+    assert L.zg_mlp_forward(None, None, 0, None) == _capi.ZG_EINVAL
+    # pattern descriptions are validated on the host
+    for verts, edges in ((3, [(0, 1), (0, 1)]), (3, [(0, 3)]), (2, [(0, 1), (1, 0)]), (7, [(0, 1)]),
+                         (3, [(1, 1)])):
+        P = Z.Pattern(verts, edges)
+        with pytest.raises(Z.ZGraphError):
+            P.unique_relabelings()
+    assert "pattern" in _capi.last_error()
