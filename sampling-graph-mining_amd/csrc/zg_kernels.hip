@@ -246,6 +246,7 @@ template <int MAXV>
 __device__ __forceinline__ void plan_pattern(const GView &g, const PlanView &P, Rng &rng, Est &o)
 {
     constexpr int MAXE = MAXV * (MAXV - 1) / 2;   // a pattern on MAXV vertices has at most this many edges
+    constexpr int NW = (MAXE + 4) / 5;            // 5 edges per word, 6 bits each
     if (g.m == 0) return;
     const uint32_t r0 = (uint32_t)rng.uniform(g.m);
     o.e0 = r0;
@@ -257,8 +258,13 @@ __device__ __forceinline__ void plan_pattern(const GView &g, const PlanView &P, 
     V[0] = src_vi(E); V[1] = dst_vi(E);
     pos[0] = E.s_pos + 1; pos[1] = E.d_pos + 1;
     int nv = 2;
-    uint32_t sr[MAXE], ss[MAXE], sd[MAXE];
-    sr[0] = r0; ss[0] = E.src; sd[0] = E.dst;
+    // sampled edges in sampling order: ranks, and endpoints as indices into V
+    // (3 bits each) -- vertex ids are distinct, so index equality is id equality
+    uint32_t sr[MAXE], ends[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) ends[w] = 0;
+    sr[0] = r0;
+    ends[0] = 0u | (1u << 3);
     int ns = 1;
     U128 val{g.m, 0};
     bool ov = false;
@@ -268,15 +274,36 @@ __device__ __forceinline__ void plan_pattern(const GView &g, const PlanView &P, 
         uint64_t s;
         SE e;
         if (!cond_sample<MAXV>(g, rng, V, pos, nv, s, e)) return;
-        bool in = false;
+        uint32_t si = 0, di = (uint32_t)nv;
 #pragma unroll
-        for (int j = 0; j < MAXV; ++j) in |= (j < nv && V[j].v == e.dst);
+        for (int j = 0; j < MAXV; ++j) {
+            if (j < nv && V[j].v == e.src) si = (uint32_t)j;
+            if (j < nv && V[j].v == e.dst) di = (uint32_t)j;
+        }
+        const bool in = di < (uint32_t)nv;
         den += in;
         ov |= mul_u128(val, s);
-        if (!canonical<MAXE>(sr, ss, sd, ns, e)) return;
+        // canonicality_check (src/SamplerGenerator.cpp:286-303): from the first earlier
+        // edge sharing a vertex with e on, every edge must rank below e
+        if (ns > 1) {
+            bool found = false, ok = true;
 #pragma unroll
-        for (int j = 0; j < MAXE; ++j)
-            if (j == ns) { sr[j] = e.rank; ss[j] = e.src; sd[j] = e.dst; }
+            for (int q = 0; q < MAXE; ++q) {
+                if (q < ns) {
+                    const uint32_t ab = (ends[q / 5] >> (6 * (q % 5))) & 63u;
+                    const uint32_t a = ab & 7u, b = ab >> 3;
+                    if (!found) found = a == si || a == di || b == si || b == di;
+                    else if (sr[q] >= e.rank) ok = false;
+                }
+            }
+            if (!ok) return;
+        }
+#pragma unroll
+        for (int q = 0; q < MAXE; ++q)
+            if (q == ns) {
+                sr[q] = e.rank;
+                ends[q / 5] |= (si | (di << 3)) << (6 * (q % 5));
+            }
         ++ns;
         if (!in) {
             if (nv >= P.pk || nv >= MAXV) return;   // |V| would exceed k
@@ -290,23 +317,27 @@ __device__ __forceinline__ void plan_pattern(const GView &g, const PlanView &P, 
         }
     }
     if (nv != P.pk) return;
-    uint32_t vs[MAXV];
+    // relabel: a vertex's new label is its position among the sorted ids
+    uint32_t rk[MAXV];
 #pragma unroll
-    for (int i = 0; i < MAXV; ++i) vs[i] = V[i].v;
-    for (int i = 0; i < P.pk; ++i)
-        for (int j = i + 1; j < P.pk; ++j)
-            if (vs[j] < vs[i]) { const uint32_t t = vs[j]; vs[j] = vs[i]; vs[i] = t; }
+    for (int j = 0; j < MAXV; ++j) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int l = 0; l < MAXV; ++l) c += V[l].v < V[j].v;
+        rk[j] = c;
+    }
     uint32_t mask = 0;
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i) {
-        if (i < ns) {
-            int a = 0, b = 0;
+    for (int q = 0; q < MAXE; ++q) {
+        if (q < ns) {
+            const uint32_t ab = (ends[q / 5] >> (6 * (q % 5))) & 63u;
+            uint32_t a = 0, b = 0;
 #pragma unroll
             for (int j = 0; j < MAXV; ++j) {
-                if (j < P.pk && vs[j] == ss[i]) a = j;
-                if (j < P.pk && vs[j] == sd[i]) b = j;
+                if ((ab & 7u) == (uint32_t)j) a = rk[j];
+                if ((ab >> 3) == (uint32_t)j) b = rk[j];
             }
-            const int lo = a < b ? a : b, hi = a < b ? b : a;
+            const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
             mask |= 1u << (hi * (hi - 1) / 2 + lo);
         }
     }
