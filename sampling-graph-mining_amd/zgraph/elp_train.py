@@ -3,7 +3,7 @@
     python -m zgraph.elp_train --graphs 256 --pilots 64 --out DIR
 
 1. Seeded graphs from the device generators (Erdos-Renyi, Chung-Lu with
-   degree exponents 2.05..3, R-MAT), 2e4..3e7 edges, built in HBM.
+   degree exponents 2.05..3, R-MAT), 2e4..1.6e8 edges, built in HBM.
 2. For every (graph, pattern) with pattern in {triangle, 4-clique, 3-chain,
    4-chain}: the exact count T from the device exact counters and a large run
    (2^26 estimators, extended up to 2^30 until 4096 succeed) whose sum of
@@ -55,6 +55,8 @@ def random_graph(rng, i):
     m = int(10 ** rng.uniform(4.3, 7.0))
     if i % 37 == 5:
         m = int(10 ** rng.uniform(7.0, 7.5))
+    if i % 16 == 6:   # a few graphs at the scale of the 69 M-edge benchmark graph and above
+        m = int(10 ** rng.uniform(7.5, 8.2))
     avg = 10 ** rng.uniform(0.5, 1.8)
     n = max(int(2 * m / avg), 64)
     while n * (n - 1) / 2 < 3 * m:
