@@ -1,8 +1,8 @@
 // zg_batch.hip -- many estimator queries (graph, plan, seed, id range) in one
 // replayable HIP graph: per plan kind, a counter reset, one persistent
 // work-queue launch over all of that kind's queries (zg_kernels.hip,
-// k_batch) and a per-query reduction, the kinds on parallel branches of the
-// graph; then one copy of every query's record to pinned host memory.  The reference answers one query per
+// k_batch) and a per-query reduction; then one copy of every query's record
+// to pinned host memory.  The reference answers one query per
 // ZGraphInstance::calculate call (src/ZGraphInstance.cpp:256-297); serving
 // many small queries that way is bound by per-call launch and synchronisation
 // cost, which the captured graph pays once per batch.
@@ -49,10 +49,6 @@ struct zg_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    // plan kinds after the first run on their own streams inside the graph
-    // (fork/join by events), so one kind's tail overlaps the others' work
-    std::vector<hipStream_t> side;
-    std::vector<hipEvent_t> fork_join;
 
     ~zg_batch()
     {
@@ -67,8 +63,6 @@ struct zg_batch {
         if (h_result) (void)hipHostFree(h_result);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
-        for (hipEvent_t e : fork_join) (void)hipEventDestroy(e);
-        for (hipStream_t x : side) (void)hipStreamDestroy(x);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -123,21 +117,11 @@ int configure(zg_batch *b)
     const int grid = device_cus() * b->cfg.blocks_per_cu;
     ZG_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int rc = ZG_OK;
-    const size_t ng = b->groups.size();
-    // fork: every side stream waits on the capture stream's start
-    if (ng > 1 && hipEventRecord(b->fork_join[0], b->stream) != hipSuccess) rc = set_error(ZG_EHIP, "batch fork");
-    for (size_t i = 1; i < ng && !rc; ++i)
-        if (hipStreamWaitEvent(b->side[i - 1], b->fork_join[0], 0) != hipSuccess) rc = set_error(ZG_EHIP, "batch fork");
-    for (size_t i = 0; i < ng && !rc; ++i) {
-        auto &g = b->groups[i];
+    for (auto &g : b->groups) {
         rc = launch_batch(g.kind, g.dq, (int)g.hq.size(), g.nchunks, (uint32_t)ch, g.counter, g.partials,
-                          b->d_result + g.result_off, grid, b->cfg.block_size, i ? b->side[i - 1] : b->stream);
+                          b->d_result + g.result_off, grid, b->cfg.block_size, b->stream);
+        if (rc) break;
     }
-    // join: the capture stream waits on every side stream's end
-    for (size_t i = 1; i < ng && !rc; ++i)
-        if (hipEventRecord(b->fork_join[i], b->side[i - 1]) != hipSuccess ||
-            hipStreamWaitEvent(b->stream, b->fork_join[i], 0) != hipSuccess)
-            rc = set_error(ZG_EHIP, "batch join");
     if (!rc && hipMemcpyAsync(b->h_result, b->d_result, sizeof(Acc) * b->nres, hipMemcpyDeviceToHost, b->stream) !=
                    hipSuccess)
         rc = set_error(ZG_EHIP, "batch: result copy could not be captured");
@@ -219,16 +203,6 @@ extern "C" int zg_batch_create(const zg_query *q, int nq, const zg_batch_config 
     ZG_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     ZG_HIP(hipEventCreate(&b->e0));
     ZG_HIP(hipEventCreate(&b->e1));
-    for (size_t i = 0; i < b->groups.size(); ++i) {   // default priority, like every stream here
-        hipEvent_t e = nullptr;
-        ZG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        b->fork_join.push_back(e);
-        if (i) {
-            hipStream_t x = nullptr;
-            ZG_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
-            b->side.push_back(x);
-        }
-    }
     rc = configure(b.get());
     if (rc) return rc;
     *out = b.release();
