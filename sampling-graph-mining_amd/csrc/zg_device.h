@@ -31,14 +31,20 @@ struct alignas(32) ERec {
     uint32_t s_pos, d_pos;   // position of this rank in src's / dst's list
 };
 
-// Edge-set hash for the closing checks: 64-byte buckets of four 16-byte
-// slots {key = min << 32 | max, rank}, at most two edges per bucket on
-// average, linear probing over buckets.  Buckets only fill up, so a lookup
-// may stop at the first bucket that still has an empty slot.
-struct alignas(16) HSlot {
-    uint64_t key;
-    uint32_t rank, pad;
+// Edge-set hash for the closing checks: 128-byte buckets -- one L2 line, the
+// DRAM read granule -- holding ten slots {key = min << 32 | max, rank} as a
+// key array and a rank array, four edges per bucket on average (a full bucket
+// spills into the next one for < 1 % of the pairs), linear probing over
+// buckets.  Buckets only fill up, so a lookup may stop at the first bucket
+// that still has an empty slot.
+constexpr int kBucketSlots = 10;
+constexpr uint64_t kEdgesPerBucket = 4;
+struct alignas(128) HBucket {
+    uint64_t key[kBucketSlots];
+    uint32_t rank[kBucketSlots];
+    uint32_t pad[2];
 };
+static_assert(sizeof(HBucket) == 128, "a hash bucket is one 128-byte line");
 constexpr uint64_t kHashEmpty = ~0ull;
 
 struct GView {
@@ -46,7 +52,7 @@ struct GView {
     const uint64_t *__restrict__ off;
     const uint64_t *__restrict__ adj;
     const uint64_t *__restrict__ sadj;
-    const HSlot *__restrict__ htab;
+    const HBucket *__restrict__ htab;
     uint64_t m;
     uint32_t n;
     uint32_t hnb;                // number of buckets
@@ -239,7 +245,7 @@ __device__ __forceinline__ uint32_t lb_rank(const GView &g, const VI &x, uint32_
 
 // conditional_close for one awaited edge {x, y} (src/ZGraphInstance.cpp:371-441):
 // true iff some copy of {x, y} has rank >= min_valid; *rank receives the
-// smallest such rank.  One 64-byte hash bucket (one line) in the common case.
+// smallest such rank.  One 128-byte hash bucket (one line) in the common case.
 __device__ __forceinline__ bool close_pair(const GView &g, uint32_t x, uint32_t y, uint32_t min_valid,
                                            uint32_t *rank)
 {
@@ -249,16 +255,30 @@ __device__ __forceinline__ bool close_pair(const GView &g, uint32_t x, uint32_t 
     uint32_t best = 0xffffffffu;
     uint32_t buckets = 0;
     for (;;) {
-        const uint4 *p = reinterpret_cast<const uint4 *>(g.htab + 4ull * b);
-        const uint4 s0 = p[0], s1 = p[1], s2 = p[2], s3 = p[3];
+        const uint4 *p = reinterpret_cast<const uint4 *>(g.htab + b);
+        uint4 kw[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) kw[i] = p[i];   // the ten keys (the line's first 80 bytes)
         ++buckets;
-        const uint64_t k0 = (uint64_t)s0.y << 32 | s0.x, k1 = (uint64_t)s1.y << 32 | s1.x;
-        const uint64_t k2 = (uint64_t)s2.y << 32 | s2.x, k3 = (uint64_t)s3.y << 32 | s3.x;
-        if (k0 == key && s0.z >= min_valid && s0.z < best) best = s0.z;
-        if (k1 == key && s1.z >= min_valid && s1.z < best) best = s1.z;
-        if (k2 == key && s2.z >= min_valid && s2.z < best) best = s2.z;
-        if (k3 == key && s3.z >= min_valid && s3.z < best) best = s3.z;
-        if (k0 == kHashEmpty || k1 == kHashEmpty || k2 == kHashEmpty || k3 == kHashEmpty) break;
+        bool open = false;
+        uint32_t hit = 0;
+#pragma unroll
+        for (int s = 0; s < kBucketSlots; ++s) {
+            const uint4 w = kw[s >> 1];
+            const uint64_t k = (s & 1) ? ((uint64_t)w.w << 32 | w.z) : ((uint64_t)w.y << 32 | w.x);
+            hit |= (uint32_t)(k == key) << s;
+            open |= k == kHashEmpty;
+        }
+        if (hit) {   // ranks from the same line, now in the caches
+            const uint32_t *rk = g.htab[b].rank;
+#pragma unroll
+            for (int s = 0; s < kBucketSlots; ++s)
+                if ((hit >> s) & 1u) {
+                    const uint32_t r = rk[s];
+                    if (r >= min_valid && r < best) best = r;
+                }
+        }
+        if (open) break;
         b = (b + 1 == g.hnb) ? 0 : b + 1;
     }
     count_reads(g, buckets);

@@ -131,7 +131,9 @@ __global__ void k_sadj(const uint64_t *skey, const uint32_t *sval, uint64_t n2, 
 }
 
 // edge-set hash: one slot per rank, claimed with a CAS on the key
-__global__ void k_hash_insert(const ERec *edges, uint64_t m, HSlot *tab, uint32_t nb)
+// claims the first empty slot of the pair's bucket (or of the buckets after
+// it) with a 64-bit compare-and-swap on the key, then writes the rank
+__global__ void k_hash_insert(const ERec *edges, uint64_t m, HBucket *tab, uint32_t nb)
 {
     GRID_STRIDE(r, m)
     {
@@ -139,14 +141,13 @@ __global__ void k_hash_insert(const ERec *edges, uint64_t m, HSlot *tab, uint32_
         const uint64_t key = a < b ? ((uint64_t)a << 32) | b : ((uint64_t)b << 32) | a;
         uint32_t bk = hbucket(key, nb);
         for (bool done = false; !done;) {
-            for (int s = 0; s < 4 && !done; ++s) {
-                HSlot *slot = tab + 4ull * bk + s;
+            for (int s = 0; s < kBucketSlots && !done; ++s) {
                 const unsigned long long prev =
-                    atomicCAS(reinterpret_cast<unsigned long long *>(&slot->key), (unsigned long long)kHashEmpty,
+                    atomicCAS(reinterpret_cast<unsigned long long *>(&tab[bk].key[s]), (unsigned long long)kHashEmpty,
                               (unsigned long long)key);
-                if (prev == kHashEmpty) { slot->rank = (uint32_t)r; done = true; }
+                if (prev == kHashEmpty) { tab[bk].rank[s] = (uint32_t)r; done = true; }
             }
-            bk = (bk + 1 == nb) ? 0 : bk + 1;
+            if (!done) bk = (bk + 1 == nb) ? 0 : bk + 1;
         }
     }
 }
@@ -317,17 +318,17 @@ int build_graph(const uint32_t *d_src, const uint32_t *d_dst, const float *d_w, 
         ZG_HIP(hipGetLastError());
         ZG_HIP(hipStreamSynchronize(st));
     }
-    // 6. edge-set hash for the closing checks (load <= 0.5: 2 edges per 4-slot bucket)
+    // 6. edge-set hash for the closing checks (load 0.4: 4 edges per 10-slot bucket)
     {
-        const uint64_t nb = (m + 1) / 2;
+        const uint64_t nb = (m + kEdgesPerBucket - 1) / kEdgesPerBucket;
         if (nb >= (1ull << 32)) return set_error(ZG_EINVAL, "too many edges for the edge hash");
         g->hnb = (uint32_t)nb;
-        ZG_HIP(hipMalloc(&g->htab, sizeof(HSlot) * 4 * nb));
-        ZG_HIP(hipMemsetAsync(g->htab, 0xff, sizeof(HSlot) * 4 * nb, st));
+        ZG_HIP(hipMalloc(&g->htab, sizeof(HBucket) * nb));
+        ZG_HIP(hipMemsetAsync(g->htab, 0xff, sizeof(HBucket) * nb, st));
         hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(m)), dim3(256), 0, st, g->edges, m, g->htab, g->hnb);
         ZG_HIP(hipGetLastError());
         ZG_HIP(hipStreamSynchronize(st));
-        g->bytes += sizeof(HSlot) * 4 * nb;
+        g->bytes += sizeof(HBucket) * nb;
     }
     guard.g = nullptr;
     *out = g;

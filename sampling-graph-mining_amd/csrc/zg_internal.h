@@ -44,7 +44,7 @@ struct zg_graph {
     zg::ERec *edges = nullptr;
     uint64_t *off = nullptr, *adj = nullptr, *sadj = nullptr;
     float *w = nullptr;
-    zg::HSlot *htab = nullptr;
+    zg::HBucket *htab = nullptr;
     uint32_t hnb = 0;
     uint64_t bytes = 0;
     hipStream_t stream = nullptr;
