@@ -5,6 +5,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace zg {
@@ -31,21 +32,31 @@ struct alignas(32) ERec {
     uint32_t s_pos, d_pos;   // position of this rank in src's / dst's list
 };
 
-// Edge-set hash for the closing checks: 128-byte buckets -- one L2 line, the
-// DRAM read granule -- holding ten slots {key = min << 32 | max, rank} as a
-// key array and a rank array, four edges per bucket on average (a full bucket
-// spills into the next one for < 1 % of the pairs), linear probing over
-// buckets.  Buckets only fill up, so a lookup may stop at the first bucket
-// that still has an empty slot.
-constexpr int kBucketSlots = 10;
+// Edge-set hash for the closing checks.  A 128-byte bucket (one L2 line, the
+// DRAM read granule) holds eight slots {key = min << 32 | max, rank} behind a
+// 16-byte header of eight 16-bit tags: bits 0..14 are a fingerprint of the
+// slot's key (0 = empty slot), and bit 15 of tag 0 marks a bucket that some
+// key passed over because it was full (linear probing over buckets).  A
+// lookup loads the header alone -- one 16-byte load -- and reads a slot only
+// where its tag matches; it moves to the next bucket only past a marked one.
+// Each load instruction of a lane pays its own address translation, which
+// dominates once the table outgrows the translation caches (several GiB), so
+// the common lookup is kept to one load.  Four edges per bucket on average:
+// about 2 % of the buckets overflow.
+constexpr int kBucketSlots = 8;
 constexpr uint64_t kEdgesPerBucket = 4;
+constexpr uint32_t kBucketPassed = 0x8000u;   // in tag word 0
+struct HSlot {
+    uint32_t key_lo, key_hi, rank;
+};
 struct alignas(128) HBucket {
-    uint64_t key[kBucketSlots];
-    uint32_t rank[kBucketSlots];
-    uint32_t pad[2];
+    uint32_t tag[kBucketSlots / 2];   // two 16-bit tags per word, slot s in word s / 2
+    HSlot slot[kBucketSlots];
+    uint32_t pad[3];
+    uint32_t fill;                    // slots claimed while building (may pass 8)
 };
 static_assert(sizeof(HBucket) == 128, "a hash bucket is one 128-byte line");
-constexpr uint64_t kHashEmpty = ~0ull;
+static_assert(offsetof(HBucket, slot) == 16, "the tag header is the bucket's first 16 bytes");
 
 struct GView {
     const ERec *__restrict__ edges;
@@ -69,9 +80,15 @@ __host__ __device__ __forceinline__ uint64_t hmix(uint64_t k)
     k ^= k >> 33;
     return k;
 }
-__host__ __device__ __forceinline__ uint32_t hbucket(uint64_t key, uint32_t nb)
+__host__ __device__ __forceinline__ uint32_t hbucket_of(uint64_t h, uint32_t nb)
 {
-    return (uint32_t)(((hmix(key) >> 32) * (uint64_t)nb) >> 32);
+    return (uint32_t)(((h >> 32) * (uint64_t)nb) >> 32);
+}
+// 15-bit nonzero fingerprint from the low bits of the mixed key
+__host__ __device__ __forceinline__ uint32_t htag_of(uint64_t h)
+{
+    const uint32_t t = (uint32_t)h & 0x7fffu;
+    return t ? t : 1u;
 }
 
 // counts dependent reads in the read-counting launch, one per 128-byte L2 line
@@ -245,40 +262,35 @@ __device__ __forceinline__ uint32_t lb_rank(const GView &g, const VI &x, uint32_
 
 // conditional_close for one awaited edge {x, y} (src/ZGraphInstance.cpp:371-441):
 // true iff some copy of {x, y} has rank >= min_valid; *rank receives the
-// smallest such rank.  One 128-byte hash bucket (one line) in the common case.
+// smallest such rank.  One 128-byte hash bucket (one line) in the common case:
+// a 16-byte header load, plus one 12-byte slot load per matching tag.
 __device__ __forceinline__ bool close_pair(const GView &g, uint32_t x, uint32_t y, uint32_t min_valid,
                                            uint32_t *rank)
 {
     if (x == y) return false;
     const uint64_t key = x < y ? ((uint64_t)x << 32) | y : ((uint64_t)y << 32) | x;
-    uint32_t b = hbucket(key, g.hnb);
+    const uint64_t h = hmix(key);
+    const uint32_t t = htag_of(h);
+    uint32_t b = hbucket_of(h, g.hnb);
     uint32_t best = 0xffffffffu;
     uint32_t buckets = 0;
     for (;;) {
-        const uint4 *p = reinterpret_cast<const uint4 *>(g.htab + b);
-        uint4 kw[5];
-#pragma unroll
-        for (int i = 0; i < 5; ++i) kw[i] = p[i];   // the ten keys (the line's first 80 bytes)
+        const uint4 hd = *reinterpret_cast<const uint4 *>(g.htab[b].tag);
         ++buckets;
-        bool open = false;
-        uint32_t hit = 0;
+        const uint32_t w[4] = {hd.x, hd.y, hd.z, hd.w};
+        uint32_t match = 0;
 #pragma unroll
-        for (int s = 0; s < kBucketSlots; ++s) {
-            const uint4 w = kw[s >> 1];
-            const uint64_t k = (s & 1) ? ((uint64_t)w.w << 32 | w.z) : ((uint64_t)w.y << 32 | w.x);
-            hit |= (uint32_t)(k == key) << s;
-            open |= k == kHashEmpty;
+        for (int s = 0; s < kBucketSlots; ++s)
+            match |= (uint32_t)(((w[s >> 1] >> (16 * (s & 1))) & 0x7fffu) == t) << s;
+        while (match) {   // a slot in the same line, read only on a tag match
+            const int s = __builtin_ctz(match);
+            match &= match - 1;
+            const HSlot sl = g.htab[b].slot[s];
+            if (sl.key_lo == (uint32_t)key && sl.key_hi == (uint32_t)(key >> 32) && sl.rank >= min_valid &&
+                sl.rank < best)
+                best = sl.rank;
         }
-        if (hit) {   // ranks from the same line, now in the caches
-            const uint32_t *rk = g.htab[b].rank;
-#pragma unroll
-            for (int s = 0; s < kBucketSlots; ++s)
-                if ((hit >> s) & 1u) {
-                    const uint32_t r = rk[s];
-                    if (r >= min_valid && r < best) best = r;
-                }
-        }
-        if (open) break;
+        if (!(hd.x & kBucketPassed)) break;
         b = (b + 1 == g.hnb) ? 0 : b + 1;
     }
     count_reads(g, buckets);

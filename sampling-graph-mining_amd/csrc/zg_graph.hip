@@ -130,24 +130,30 @@ __global__ void k_sadj(const uint64_t *skey, const uint32_t *sval, uint64_t n2, 
     GRID_STRIDE(i, n2) { sadj[i] = ((skey[i] & 0xffffffffull) << 32) | sval[i]; }
 }
 
-// edge-set hash: one slot per rank, claimed with a CAS on the key
-// claims the first empty slot of the pair's bucket (or of the buckets after
-// it) with a 64-bit compare-and-swap on the key, then writes the rank
+// edge-set hash: one slot per rank.  A bucket's slots are claimed with an
+// atomic add on its fill count; the claimant writes the slot and then ORs its
+// tag into the header.  A pair whose bucket is full marks it as passed over
+// and tries the next one.  Which slot a copy lands in depends on the order of
+// the atomics; a lookup reads every slot whose tag matches, so its answer
+// does not.
 __global__ void k_hash_insert(const ERec *edges, uint64_t m, HBucket *tab, uint32_t nb)
 {
     GRID_STRIDE(r, m)
     {
         const uint32_t a = edges[r].src, b = edges[r].dst;
         const uint64_t key = a < b ? ((uint64_t)a << 32) | b : ((uint64_t)b << 32) | a;
-        uint32_t bk = hbucket(key, nb);
-        for (bool done = false; !done;) {
-            for (int s = 0; s < kBucketSlots && !done; ++s) {
-                const unsigned long long prev =
-                    atomicCAS(reinterpret_cast<unsigned long long *>(&tab[bk].key[s]), (unsigned long long)kHashEmpty,
-                              (unsigned long long)key);
-                if (prev == kHashEmpty) { tab[bk].rank[s] = (uint32_t)r; done = true; }
+        const uint64_t h = hmix(key);
+        const uint32_t t = htag_of(h);
+        uint32_t bk = hbucket_of(h, nb);
+        for (;;) {
+            const uint32_t s = atomicAdd(&tab[bk].fill, 1u);
+            if (s < (uint32_t)kBucketSlots) {
+                tab[bk].slot[s] = HSlot{(uint32_t)key, (uint32_t)(key >> 32), (uint32_t)r};
+                atomicOr(&tab[bk].tag[s >> 1], t << (16 * (s & 1)));
+                break;
             }
-            if (!done) bk = (bk + 1 == nb) ? 0 : bk + 1;
+            atomicOr(&tab[bk].tag[0], kBucketPassed);
+            bk = (bk + 1 == nb) ? 0 : bk + 1;
         }
     }
 }
@@ -318,13 +324,13 @@ int build_graph(const uint32_t *d_src, const uint32_t *d_dst, const float *d_w, 
         ZG_HIP(hipGetLastError());
         ZG_HIP(hipStreamSynchronize(st));
     }
-    // 6. edge-set hash for the closing checks (load 0.4: 4 edges per 10-slot bucket)
+    // 6. edge-set hash for the closing checks (4 edges per 8-slot bucket on average)
     {
         const uint64_t nb = (m + kEdgesPerBucket - 1) / kEdgesPerBucket;
         if (nb >= (1ull << 32)) return set_error(ZG_EINVAL, "too many edges for the edge hash");
         g->hnb = (uint32_t)nb;
         ZG_HIP(hipMalloc(&g->htab, sizeof(HBucket) * nb));
-        ZG_HIP(hipMemsetAsync(g->htab, 0xff, sizeof(HBucket) * nb, st));
+        ZG_HIP(hipMemsetAsync(g->htab, 0, sizeof(HBucket) * nb, st));
         hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(m)), dim3(256), 0, st, g->edges, m, g->htab, g->hnb);
         ZG_HIP(hipGetLastError());
         ZG_HIP(hipStreamSynchronize(st));
