@@ -46,11 +46,16 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-query comparison")
     ap.add_argument("--elp-seeds", type=int, default=5, help="independent queries per profile and pattern")
     ap.add_argument("--elp-weights", default=None, help="learned-profile weights (default: the package's)")
-    ap.add_argument("--workload", default="chunglu69m", choices=["chunglu69m", "rmat26"],
+    ap.add_argument("--workload", default="chunglu69m", choices=["chunglu69m", "rmat26", "sizes"],
                     help="chunglu69m: the headline triangle + 4-clique workload; rmat26: R-MAT scale 26, "
-                         "edge factor 16 with the 3-motif, 3-chain, 4-chain and predicate-triangle plans")
+                         "edge factor 16 with the 3-motif, 3-chain, 4-chain and predicate-triangle plans; "
+                         "sizes: 3-chain, triangle and 4-clique on R-MAT 22/24/26 and Chung-Lu 69M/500M "
+                         "(how the rate holds as the graph outgrows the translation caches)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this size (rocprof counter calibration)")
+    ap.add_argument("--line-sweep", action="store_true",
+                    help="only measure the random 128-byte line rate by load shape (1, 2, 5, 8 loads of 16 bytes "
+                         "per lane, or 8 lanes per line) on 2, 8 and 32 GiB buffers")
     args = ap.parse_args()
 
     if args.calibrate_copy_mib:
@@ -64,6 +69,19 @@ def main():
         print(json.dumps({"calibration": "copy", "bytes": n, "launches": 3}), flush=True)
         return
 
+    if args.line_sweep:
+        import zgraph as Z
+        rows = []
+        for gib in (2, 8, 32):
+            for shape in (1, 2, 5, 8, "coop"):
+                coop = shape == "coop"
+                gbs, ms = Z.random_line_bandwidth(gib << 30, 8 if coop else shape, coop, 1 << 28)
+                rows.append({"buffer_gib": gib, "shape": "8 lanes x 16 B" if coop else f"1 lane x {shape} x 16 B",
+                             "gbs": round(gbs, 1), "ms": round(ms, 3)})
+                log(f"line sweep: {gib} GiB, {rows[-1]['shape']}: {gbs:.0f} GB/s")
+        print(json.dumps({"line_sweep": rows, "lines_per_point": 1 << 28}), flush=True)
+        return
+
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
                                    "(estimator ids would shard across GPUs with one all-reduce of the sums)"}))
@@ -73,6 +91,9 @@ def main():
 
     if args.workload == "rmat26":
         rmat26(Z, args)
+        return
+    if args.workload == "sizes":
+        size_sweep(Z, args)
         return
 
     info = Z.device_info()
@@ -297,6 +318,40 @@ def batched_queries(Z, args):
     for g in graphs:
         g.close()
     return out
+
+
+def size_sweep(Z, args):
+    """3-chain (no closing check), triangle and 4-clique estimators, 2^28 per
+    launch, on seeded graphs from 67 M to 1.07 B edges: R-MAT scales 22, 24, 26
+    (edge factor 16) and Chung-Lu with 69 M and 500 M edges.  Per plan: mean
+    kernel time over `--steps` launches after `--warmup`, lines per estimator
+    and the line rate."""
+    N = 1 << args.log2_estimators
+    graphs = [("rmat22", lambda: Z.rmat(22, 16 << 22, 0.57, 0.19, 0.19, args.seed)),
+              ("rmat24", lambda: Z.rmat(24, 16 << 24, 0.57, 0.19, 0.19, args.seed)),
+              ("rmat26", lambda: Z.rmat(26, 16 << 26, 0.57, 0.19, 0.19, args.seed)),
+              ("chunglu69m", lambda: Z.chung_lu(4_800_000, 69_000_000, 2.1, 20000.0, args.seed)),
+              ("chunglu500m", lambda: Z.chung_lu(30_000_000, 500_000_000, 2.1, 20000.0, args.seed))]
+    plans = [("3-chain", Z.PLAN_THREE_CHAIN), ("triangle", Z.PLAN_TRIANGLE), ("4-clique", Z.PLAN_K4_CLOSE)]
+    rows = []
+    for gname, make in graphs:
+        e = make()
+        g = Z.Graph.from_device(e, seed=args.seed)
+        e.free()
+        log(f"sizes {gname}: m={g.num_local_edges} maxdeg={g.max_degree} hbm={g.device_bytes / 2**30:.1f}GiB")
+        for pname, kind in plans:
+            plan = Z.Plan(kind)
+            for w in range(args.warmup):
+                Z.run(g, plan, args.seed, (1 << 40) + w * N, N)
+            ms = [Z.run(g, plan, args.seed, i * N, N, timed=True)[1] for i in range(args.steps)]
+            k = sum(ms) / len(ms)
+            lines = Z.count_reads(g, plan, args.seed, 0, 1 << 22) / (1 << 22)
+            rows.append({"graph": gname, "edges": g.num_local_edges, "hbm_gib": round(g.device_bytes / 2**30, 2),
+                         "plan": pname, "kernel_ms": round(k, 3), "lines_per_estimator": round(lines, 4),
+                         "line_rate_gbs": round(lines * LINE * N / (k / 1e3) / 1e9, 1)})
+            log(f"sizes {gname} {pname}: {k:.2f} ms, {lines:.3f} lines, {rows[-1]['line_rate_gbs']:.0f} GB/s")
+        g.close()
+    print(json.dumps({"size_sweep": rows, "estimators_per_launch": N}), flush=True)
 
 
 def rmat26(Z, args):

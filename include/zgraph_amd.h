@@ -249,6 +249,17 @@ int zg_graph_degree_moments(zg_graph *g, double *sum_d2, double *sum_d3);
  * defaults).  *gbs counts one 128-byte line per read. */
 int zg_random_gather_bw(uint64_t buffer_bytes, int chains, uint64_t loads, int block_size, int blocks_per_cu,
                         double *gbs, double *ms);
+/* Random-line read rate by load shape: `lines` random 128-byte lines of a
+ * fresh buffer of `buffer_bytes`, one dependent chain per reader.  With
+ * cooperative = 0 every lane reads `loads_per_line` (1..8) consecutive 16-byte
+ * pieces of its own line, one load instruction each; with cooperative = 1
+ * groups of 8 lanes read one line, one 16-byte piece per lane, in a single
+ * instruction (loads_per_line is then ignored).  *gbs counts 128 bytes per
+ * line.  Shows what each extra load instruction per random line costs (each
+ * pays its own address translation) -- the measurement behind the edge-set
+ * hash's one-load header. */
+int zg_random_line_bw(uint64_t buffer_bytes, int loads_per_line, int cooperative, uint64_t lines, int block_size,
+                      int blocks_per_cu, double *gbs, double *ms);
 
 /* ---- error-latency profile (src/ErrorLatencyProfile.cpp) --------------- */
 /* ErrorProfile<E>::get_num_estimators (:71-207) on the device: subsample at

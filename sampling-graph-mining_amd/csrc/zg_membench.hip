@@ -37,6 +37,42 @@ __global__ __launch_bounds__(256) void k_mb_gather(const uint64_t *buf, uint64_t
     if (acc == 0x0123456789ABCDEFull) sink[0] = acc;   // keeps the loads live
 }
 
+// L loads of 16 bytes per lane from its own random 128-byte line
+template <int L>
+__global__ __launch_bounds__(256) void k_mb_lines(const uint4 *buf, uint64_t nlines, uint32_t rounds, uint64_t *sink)
+{
+    uint64_t x = hmix((uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1);
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint4 *p = buf + __umul64hi(x, nlines) * 8;
+        uint4 v[L];
+#pragma unroll
+        for (int i = 0; i < L; ++i) v[i] = p[i];
+        uint64_t a = 0;
+#pragma unroll
+        for (int i = 0; i < L; ++i) a ^= ((uint64_t)v[i].x << 32 | v[i].y) ^ v[i].z ^ ((uint64_t)v[i].w << 17);
+        x = hmix(x ^ a);
+    }
+    if (x == 0x0123456789ABCDEFull) sink[0] = x;
+}
+
+// groups of 8 lanes read one random line, one 16-byte piece each; the group
+// combines the pieces so the next line depends on the whole line
+__global__ __launch_bounds__(256) void k_mb_lines_coop(const uint4 *buf, uint64_t nlines, uint32_t rounds,
+                                                       uint64_t *sink)
+{
+    const uint32_t piece = threadIdx.x & 7;
+    uint64_t x = hmix(((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 8 + 1);
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint4 v = buf[__umul64hi(x, nlines) * 8 + piece];
+        uint64_t a = ((uint64_t)v.x << 32 | v.y) ^ v.z ^ ((uint64_t)v.w << 17);
+        a ^= __shfl_xor(a, 1);
+        a ^= __shfl_xor(a, 2);
+        a ^= __shfl_xor(a, 4);
+        x = hmix(x ^ a);
+    }
+    if (x == 0x0123456789ABCDEFull) sink[0] = x;
+}
+
 }  // namespace
 }  // namespace zg
 
@@ -82,6 +118,72 @@ extern "C" int zg_random_gather_bw(uint64_t buffer_bytes, int chains, uint64_t l
         float t = 0;
         ZG_HIP(hipEventElapsedTime(&t, e0, e1));
         const double done = (double)lanes * chains * rounds;
+        *gbs = done * 128.0 / (t * 1e-3) / 1e9;
+        if (ms) *ms = t;
+        return ZG_OK;
+    };
+    rc = run();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (sink) (void)hipFree(sink);
+    (void)hipFree(buf);
+    return rc;
+}
+
+extern "C" int zg_random_line_bw(uint64_t buffer_bytes, int loads_per_line, int cooperative, uint64_t lines,
+                                 int block_size, int blocks_per_cu, double *gbs, double *ms)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!gbs || buffer_bytes < 4096 || lines == 0) return set_error(ZG_EINVAL, "bad line benchmark arguments");
+    if (!cooperative && (loads_per_line < 1 || loads_per_line > 8))
+        return set_error(ZG_EINVAL, "loads_per_line must be in 1..8");
+    const int bs = block_size > 0 ? block_size : 256;
+    if (bs > 256 || bs % 64) return set_error(ZG_EINVAL, "block size must be a multiple of 64 up to 256");
+    const int bpc = blocks_per_cu > 0 ? blocks_per_cu : 16;
+    const uint64_t grid = (uint64_t)device_cus() * bpc;
+    const uint64_t readers = cooperative ? grid * bs / 8 : grid * bs;
+    const uint64_t per_reader = (lines + readers - 1) / readers;
+    if (per_reader > 0xffffffffull) return set_error(ZG_EOVERFLOW, "too many lines");
+    const uint32_t rounds = (uint32_t)per_reader;
+    const uint64_t nlines = buffer_bytes / 128;
+    uint4 *buf = nullptr;
+    uint64_t *sink = nullptr;
+    if (hipMalloc(&buf, nlines * 128) != hipSuccess) return set_error(ZG_ENOMEM, "line buffer allocation failed");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto run = [&]() -> int {
+        ZG_HIP(hipMalloc(&sink, 8));
+        ZG_HIP(hipEventCreate(&e0));
+        ZG_HIP(hipEventCreate(&e1));
+        hipLaunchKernelGGL(k_mb_fill, dim3(device_cus() * 64), dim3(256), 0, 0, reinterpret_cast<uint64_t *>(buf),
+                           nlines * 16);
+        ZG_HIP(hipGetLastError());
+        auto launch = [&]() {
+            if (cooperative) {
+                hipLaunchKernelGGL(k_mb_lines_coop, dim3(grid), dim3(bs), 0, 0, buf, nlines, rounds, sink);
+                return;
+            }
+            switch (loads_per_line) {
+            case 1: hipLaunchKernelGGL(k_mb_lines<1>, dim3(grid), dim3(bs), 0, 0, buf, nlines, rounds, sink); break;
+            case 2: hipLaunchKernelGGL(k_mb_lines<2>, dim3(grid), dim3(bs), 0, 0, buf, nlines, rounds, sink); break;
+            case 3: hipLaunchKernelGGL(k_mb_lines<3>, dim3(grid), dim3(bs), 0, 0, buf, nlines, rounds, sink); break;
+            case 4: hipLaunchKernelGGL(k_mb_lines<4>, dim3(grid), dim3(bs), 0, 0, buf, nlines, rounds, sink); break;
+            case 5: hipLaunchKernelGGL(k_mb_lines<5>, dim3(grid), dim3(bs), 0, 0, buf, nlines, rounds, sink); break;
+            case 6: hipLaunchKernelGGL(k_mb_lines<6>, dim3(grid), dim3(bs), 0, 0, buf, nlines, rounds, sink); break;
+            case 7: hipLaunchKernelGGL(k_mb_lines<7>, dim3(grid), dim3(bs), 0, 0, buf, nlines, rounds, sink); break;
+            default: hipLaunchKernelGGL(k_mb_lines<8>, dim3(grid), dim3(bs), 0, 0, buf, nlines, rounds, sink); break;
+            }
+        };
+        launch();   // warm-up
+        ZG_HIP(hipGetLastError());
+        ZG_HIP(hipEventRecord(e0, 0));
+        launch();
+        ZG_HIP(hipGetLastError());
+        ZG_HIP(hipEventRecord(e1, 0));
+        ZG_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        ZG_HIP(hipEventElapsedTime(&t, e0, e1));
+        const double done = (double)readers * rounds;
         *gbs = done * 128.0 / (t * 1e-3) / 1e9;
         if (ms) *ms = t;
         return ZG_OK;

@@ -28,6 +28,7 @@ __all__ = [
     "SamplerGenerator", "AllPredicateZGraphInstance", "ErrorProfile", "TimeProfile",
     "load_graph_meta_data", "load_biedgelist", "load_edgelist", "save_biedgelist", "ZGraphError",
     "device_info", "clt_half_width", "run", "run_outcomes", "Batch", "random_gather_bandwidth",
+    "random_line_bandwidth",
 ]
 
 OUTCOME_DTYPE = np.dtype([("num_lo", "<u8"), ("num_hi", "<u8"), ("e0", "<u4"), ("status", "u1"),
@@ -366,6 +367,17 @@ def random_gather_bandwidth(buffer_bytes=16 << 30, chains=1, loads=1 << 28, bloc
     gbs, ms = C.c_double(), C.c_double()
     check(lib().zg_random_gather_bw(buffer_bytes, chains, loads, block_size, blocks_per_cu, C.byref(gbs),
                                     C.byref(ms)))
+    return gbs.value, ms.value
+
+
+def random_line_bandwidth(buffer_bytes=16 << 30, loads_per_line=1, cooperative=False, lines=1 << 28,
+                          block_size=0, blocks_per_cu=0):
+    """Random 128-byte line rate in GB/s when each line is read with
+    `loads_per_line` 16-byte loads by one lane, or (cooperative) by 8 lanes
+    with one 16-byte load each in a single instruction."""
+    gbs, ms = C.c_double(), C.c_double()
+    check(lib().zg_random_line_bw(buffer_bytes, loads_per_line, int(bool(cooperative)), lines, block_size,
+                                  blocks_per_cu, C.byref(gbs), C.byref(ms)))
     return gbs.value, ms.value
 
 
