@@ -10,7 +10,8 @@
 // Method: orient every undirected pair from lower to higher (degree, id) and
 // keep the out-lists sorted by neighbour id (taken from the neighbour-ordered
 // copy, parallel edges collapsed).  Triangles: per oriented edge u->v, binary
-// search every out-neighbour of the shorter list in the longer one.  4-cliques:
+// search every out-neighbour of the shorter list in the longer one, the work
+// spread evenly over lanes through a prefix sum of the shorter lengths.  4-cliques:
 // one wavefront per oriented edge builds W = out(u) & out(v) in a scratch
 // slot, then counts pairs w->x inside W.
 #include <cstring>
@@ -41,19 +42,41 @@ inline int gridx(uint64_t n)
     return (int)std::max<uint64_t>(1, std::min(g, cap));
 }
 
+// One wavefront per vertex: the lanes read 64 consecutive entries of the
+// vertex's neighbour-ordered list at a time (coalesced), so a hub's list is
+// shared by 64 lanes instead of walked by one thread.
+#define WAVE_STRIDE(v, n)                                                                    \
+    for (uint64_t v = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; v < (n);       \
+         v += ((uint64_t)gridDim.x * blockDim.x) >> 6)
+
+// neighbour of entry i of a list [b, e), or a value no vertex id takes
+__device__ __forceinline__ uint32_t nbr_or_none(const uint64_t *sadj, uint64_t i, uint64_t e)
+{
+    return i < e ? (uint32_t)(sadj[i] >> 32) : 0xffffffffu;
+}
+
+// first entry of a run of equal neighbours (parallel edges collapse to one)
+__device__ __forceinline__ bool first_of_run(uint32_t nb, uint32_t &carry)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t prev = __shfl_up(nb, 1, 64);
+    if (lane == 0) prev = carry;
+    carry = __shfl(nb, 63, 64);
+    return nb != 0xffffffffu && nb != prev;
+}
+
 // simple-graph degree of v: distinct neighbours in its neighbour-ordered list
 __global__ void k_simple_deg(const uint64_t *off, const uint64_t *sadj, uint32_t n, uint32_t *sdeg)
 {
-    GRID_STRIDE(v, (uint64_t)n)
+    WAVE_STRIDE(v, (uint64_t)n)
     {
-        uint32_t d = 0;
-        uint32_t prev = 0xffffffffu;
-        for (uint64_t i = off[v]; i < off[v + 1]; ++i) {
-            const uint32_t nb = (uint32_t)(sadj[i] >> 32);
-            d += nb != prev;
-            prev = nb;
+        const uint64_t b = off[v], e = off[v + 1];
+        uint32_t d = 0, carry = 0xffffffffu;
+        for (uint64_t base = b; base < e; base += 64) {
+            const uint32_t nb = nbr_or_none(sadj, base + (threadIdx.x & 63), e);
+            d += (uint32_t)__popcll(__ballot(first_of_run(nb, carry)));
         }
-        sdeg[v] = d;
+        if ((threadIdx.x & 63) == 0) sdeg[v] = d;
     }
 }
 
@@ -67,30 +90,40 @@ __device__ __forceinline__ bool before(const uint32_t *sdeg, uint32_t a, uint32_
 __global__ void k_out_deg(const uint64_t *off, const uint64_t *sadj, const uint32_t *sdeg, uint32_t n,
                           uint64_t *odeg)
 {
-    GRID_STRIDE(v, (uint64_t)n)
+    WAVE_STRIDE(v, (uint64_t)n)
     {
+        const uint64_t b = off[v], e = off[v + 1];
         uint64_t d = 0;
-        uint32_t prev = 0xffffffffu;
-        for (uint64_t i = off[v]; i < off[v + 1]; ++i) {
-            const uint32_t nb = (uint32_t)(sadj[i] >> 32);
-            if (nb != prev && before(sdeg, (uint32_t)v, nb)) ++d;
-            prev = nb;
+        uint32_t carry = 0xffffffffu;
+        for (uint64_t base = b; base < e; base += 64) {
+            const uint32_t nb = nbr_or_none(sadj, base + (threadIdx.x & 63), e);
+            const bool q = first_of_run(nb, carry) && before(sdeg, (uint32_t)v, nb);
+            d += (uint64_t)__popcll(__ballot(q));
         }
-        odeg[v] = d;
+        if ((threadIdx.x & 63) == 0) odeg[v] = d;
     }
 }
 
+// out-lists in neighbour order (ballot compaction keeps the order)
 __global__ void k_out_fill(const uint64_t *off, const uint64_t *sadj, const uint32_t *sdeg, uint32_t n,
                            const uint64_t *ooff, uint32_t *onb, uint32_t *osrc)
 {
-    GRID_STRIDE(v, (uint64_t)n)
+    const uint32_t lane = threadIdx.x & 63;
+    WAVE_STRIDE(v, (uint64_t)n)
     {
+        const uint64_t b = off[v], e = off[v + 1];
         uint64_t w = ooff[v];
-        uint32_t prev = 0xffffffffu;
-        for (uint64_t i = off[v]; i < off[v + 1]; ++i) {
-            const uint32_t nb = (uint32_t)(sadj[i] >> 32);
-            if (nb != prev && before(sdeg, (uint32_t)v, nb)) { onb[w] = nb; osrc[w] = (uint32_t)v; ++w; }
-            prev = nb;
+        uint32_t carry = 0xffffffffu;
+        for (uint64_t base = b; base < e; base += 64) {
+            const uint32_t nb = nbr_or_none(sadj, base + lane, e);
+            const bool q = first_of_run(nb, carry) && before(sdeg, (uint32_t)v, nb);
+            const uint64_t mask = __ballot(q);
+            if (q) {
+                const uint64_t at = w + (uint64_t)__popcll(mask & ((1ull << lane) - 1));
+                onb[at] = nb;
+                osrc[at] = (uint32_t)v;
+            }
+            w += (uint64_t)__popcll(mask);
         }
     }
 }
@@ -118,17 +151,71 @@ __device__ __forceinline__ unsigned long long block_sum(unsigned long long v)
     return t;
 }
 
-__global__ __launch_bounds__(256) void k_triangles(const uint64_t *ooff, const uint32_t *onb, const uint32_t *osrc,
-                                                   uint64_t ne, unsigned long long *partial)
+// Load-balanced triangle count.  Work item j of oriented edge e (P[e] <= j <
+// P[e+1], P the exclusive prefix of the shorter out-list lengths) is element
+// j - P[e] of the edge's shorter out-list, searched in the longer one.  Waves
+// take contiguous chunks of items and lanes interleave inside a chunk, so the
+// items of one edge are read together and every lane does the same amount of
+// work however skewed the degrees are.
+__global__ void k_tri_work(const uint64_t *ooff, const uint32_t *onb, const uint32_t *osrc, uint64_t ne,
+                           uint64_t *x)
 {
-    unsigned long long c = 0;
     GRID_STRIDE(e, ne)
     {
         const uint32_t u = osrc[e], v = onb[e];
-        const uint32_t *a = onb + ooff[u], *b = onb + ooff[v];
-        uint64_t na = ooff[u + 1] - ooff[u], nb = ooff[v + 1] - ooff[v];
-        if (na > nb) { const uint32_t *t = a; a = b; b = t; const uint64_t x = na; na = nb; nb = x; }
-        for (uint64_t i = 0; i < na; ++i) c += bsearch_u32(b, nb, a[i]);
+        const uint64_t na = ooff[u + 1] - ooff[u], nb = ooff[v + 1] - ooff[v];
+        x[e] = na < nb ? na : nb;
+    }
+}
+
+// largest e in [lo, ne) with P[e] <= j, given P[lo] <= j < P[ne]: gallop, then bisect
+__device__ __forceinline__ uint64_t tri_edge_of(const uint64_t *P, uint64_t ne, uint64_t lo, uint64_t j)
+{
+    uint64_t step = 1, hi = lo;
+    while (true) {   // find hi with P[hi] > j
+        const uint64_t t = lo + step;
+        if (t >= ne) { hi = ne; break; }
+        if (P[t] > j) { hi = t; break; }
+        lo = t;
+        step <<= 1;
+    }
+    while (hi - lo > 1) {   // P[lo] <= j < P[hi]
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (P[mid] <= j) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+constexpr uint64_t kTriChunk = 64 * 64;   // items per wave task
+
+__global__ __launch_bounds__(256) void k_tri_flat(const uint64_t *ooff, const uint32_t *onb, const uint32_t *osrc,
+                                                  const uint64_t *P, uint64_t ne, uint64_t total,
+                                                  unsigned long long *partial)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t nchunks = (total + kTriChunk - 1) / kTriChunk;
+    unsigned long long c = 0;
+    for (uint64_t ch = wave; ch < nchunks; ch += nwaves) {
+        const uint64_t start = ch * kTriChunk;
+        const uint64_t end = start + kTriChunk < total ? start + kTriChunk : total;
+        uint64_t e = tri_edge_of(P, ne, 0, start);
+        uint64_t pe = 0, pn = 0, nb = 0;
+        const uint32_t *a = nullptr, *b = nullptr;
+        for (uint64_t j = start + lane; j < end; j += 64) {
+            if (a == nullptr || j >= pn) {
+                e = tri_edge_of(P, ne, e, j);
+                pe = P[e];
+                pn = P[e + 1];
+                const uint32_t u = osrc[e], v = onb[e];
+                const uint64_t ou = ooff[u], ov = ooff[v];
+                const uint64_t nu = ooff[u + 1] - ou, nv = ooff[v + 1] - ov;
+                if (nu <= nv) { a = onb + ou; b = onb + ov; nb = nv; }
+                else { a = onb + ov; b = onb + ou; nb = nu; }
+            }
+            c += bsearch_u32(b, nb, a[j - pe]);
+        }
     }
     const unsigned long long t = block_sum(c);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
@@ -182,15 +269,15 @@ __global__ void k_wedges_paths(const uint64_t *off, const uint64_t *sadj, const 
                                unsigned long long *wed, unsigned long long *paths)
 {
     unsigned long long w = 0, p = 0;
-    GRID_STRIDE(v, (uint64_t)n)
+    WAVE_STRIDE(v, (uint64_t)n)
     {
         const unsigned long long d = sdeg[v];
-        w += d * (d - (d > 0)) / 2;
-        uint32_t prev = 0xffffffffu;
-        for (uint64_t i = off[v]; i < off[v + 1]; ++i) {
-            const uint32_t nb = (uint32_t)(sadj[i] >> 32);
-            if (nb != prev && (uint32_t)v < nb) p += (d - 1) * (unsigned long long)(sdeg[nb] - 1);
-            prev = nb;
+        if ((threadIdx.x & 63) == 0) w += d * (d - (d > 0)) / 2;
+        const uint64_t b = off[v], e = off[v + 1];
+        uint32_t carry = 0xffffffffu;
+        for (uint64_t base = b; base < e; base += 64) {
+            const uint32_t nb = nbr_or_none(sadj, base + (threadIdx.x & 63), e);
+            if (first_of_run(nb, carry) && (uint32_t)v < nb) p += (d - 1) * (unsigned long long)(sdeg[nb] - 1);
         }
     }
     const unsigned long long tw = block_sum(w);
@@ -422,24 +509,44 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
     XALLOC(onb, ne); XALLOC(osrc, ne);
     hipLaunchKernelGGL(k_out_fill, dim3(gridx(n)), dim3(256), 0, st, g->off, g->sadj, sdeg.p, n, ooff.p, onb.p, osrc.p);
     ZG_HIP(hipGetLastError());
-    const int grid = (int)std::min<uint64_t>((uint64_t)device_cus() * 8, std::max<uint64_t>(1, (ne + 255) / 256));
-    XB<unsigned long long> part;
-    XALLOC(part, grid);
     if (what == 0) {
-        hipLaunchKernelGGL(k_triangles, dim3(grid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, ne, part.p);
-    } else {
-        const uint64_t slot = maxout ? maxout : 1;
-        const uint64_t nwaves = (uint64_t)grid * 4;
-        XB<uint32_t> scratch;
-        XALLOC(scratch, nwaves * slot);
-        hipLaunchKernelGGL(k_k4, dim3(grid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, ne, scratch.p, slot, part.p);
+        XB<uint64_t> work, pre;
+        XALLOC(work, ne + 1); XALLOC(pre, ne + 1);
+        ZG_HIP(hipMemsetAsync(work.p + ne, 0, 8, st));
+        hipLaunchKernelGGL(k_tri_work, dim3(gridx(ne)), dim3(256), 0, st, ooff.p, onb.p, osrc.p, ne, work.p);
+        ZG_HIP(hipGetLastError());
+        size_t tw = 0;
+        ZG_HIP(rocprim::exclusive_scan(nullptr, tw, work.p, pre.p, (uint64_t)0, (size_t)ne + 1,
+                                       rocprim::plus<uint64_t>(), st));
+        XB<char> tmpw;
+        XALLOC(tmpw, tw);
+        ZG_HIP(rocprim::exclusive_scan(tmpw.p, tw, work.p, pre.p, (uint64_t)0, (size_t)ne + 1,
+                                       rocprim::plus<uint64_t>(), st));
+        uint64_t total = 0;
+        ZG_HIP(hipMemcpyAsync(&total, pre.p + ne, 8, hipMemcpyDeviceToHost, st));
+        ZG_HIP(hipStreamSynchronize(st));
+        if (total == 0) { *out = 0; return ZG_OK; }
+        const uint64_t nchunks = (total + kTriChunk - 1) / kTriChunk;
+        const int tgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 8, (nchunks + 3) / 4);
+        XB<unsigned long long> tpart;
+        XALLOC(tpart, tgrid);
+        hipLaunchKernelGGL(k_tri_flat, dim3(tgrid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, pre.p, ne, total,
+                           tpart.p);
         ZG_HIP(hipGetLastError());
         unsigned long long s = 0;
-        int rc = host_sum(part.p, grid, st, &s);
+        int rc = host_sum(tpart.p, tgrid, st, &s);
         if (rc) return rc;
         *out = s;
         return ZG_OK;
     }
+    const int grid = (int)std::min<uint64_t>((uint64_t)device_cus() * 8, std::max<uint64_t>(1, (ne + 255) / 256));
+    XB<unsigned long long> part;
+    XALLOC(part, grid);
+    const uint64_t slot = maxout ? maxout : 1;
+    const uint64_t nwaves = (uint64_t)grid * 4;
+    XB<uint32_t> scratch;
+    XALLOC(scratch, nwaves * slot);
+    hipLaunchKernelGGL(k_k4, dim3(grid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, ne, scratch.p, slot, part.p);
     ZG_HIP(hipGetLastError());
     unsigned long long s = 0;
     int rc = host_sum(part.p, grid, st, &s);

@@ -1,7 +1,8 @@
 """Size-independent properties at R-MAT scale (edge factor 16, a=.57,
 b=c=.19, float weights U[0,1)): the fused 3-motif plan's two channels equal
 the separate 3-chain and triangle plans exactly (same draws per id); the
-3-chain estimate lies in the 99.9 % interval of the exact wedge count; a top-up
+3-chain and triangle estimates lie in the 99.9 % intervals of the exact wedge
+and triangle counts; a top-up
 equals one run; the two complementary weight filters partition the edges.
 Scale 20 (16.8 M edges) always runs; scale 26 (1.07 B edges, about 150 GB of
 HBM while building) runs when ZG_FULL_SCALE=1."""
@@ -35,6 +36,10 @@ def _check(scale, N):
     hw = Z.clt_half_width(chain.mean, chain.sumsq, N, 0.999)
     say(f"[scale] wedges exact {W} estimate {chain.mean:.6g} +- {hw:.3g}")
     assert abs(chain.mean - W) <= hw
+    T = g.exact_count("triangle")
+    hwt = Z.clt_half_width(tri.mean, tri.sumsq, N, 0.999)
+    say(f"[scale] triangles exact {T} estimate {tri.mean:.6g} +- {hwt:.3g}")
+    assert abs(tri.mean - T) <= hwt
     half = Z.run(g, Z.Plan(Z.PLAN_FOUR_CHAIN), 11, 0, N // 2)[0]
     rest = Z.run(g, Z.Plan(Z.PLAN_FOUR_CHAIN), 11, N // 2, N - N // 2)[0]
     whole = Z.run(g, Z.Plan(Z.PLAN_FOUR_CHAIN), 11, 0, N)[0]
