@@ -58,6 +58,21 @@ def power_law_graph(n, m, beta, seed):
     return n, e[perm, 0].copy(), e[perm, 1].copy()
 
 
+def multigraph(n, p, seed, max_copies=12, heavy_copies=40):
+    """G(n, p) with every pair repeated 1..max_copies times (each copy randomly
+    oriented) and one pair repeated heavy_copies times: parallel edges, which
+    the reference's loader accepts, beyond what one hash bucket holds."""
+    rng = np.random.default_rng(seed)
+    _, s, d = random_graph(n, p, seed)
+    reps = rng.integers(1, max_copies + 1, size=len(s))
+    reps[0] = heavy_copies
+    s, d = np.repeat(s, reps), np.repeat(d, reps)
+    f = rng.random(len(s)) < 0.5
+    s, d = np.where(f, d, s), np.where(f, s, d)
+    perm = rng.permutation(len(s))
+    return n, s[perm].astype(np.uint32), d[perm].astype(np.uint32)
+
+
 # (name, oracle plan args, device plan args)
 PLANS = [
     ("triangle", dict(kind=1), dict(kind=1)),

@@ -44,11 +44,13 @@ def test_weights_uniform():
     assert abs(w.mean() - 0.5) < 0.01
 
 
-@pytest.mark.parametrize("gname", ["wiki-vote", "snap-wiki-Vote", "simple-30", "powerlaw", "K9"])
+@pytest.mark.parametrize("gname", ["wiki-vote", "snap-wiki-Vote", "simple-30", "powerlaw", "K9", "multi"])
 def test_device_exact_counts_equal_oracle(gname):
     import helpers as H
     if gname == "powerlaw":
         n, s, d = H.power_law_graph(3000, 20000, 2.1, 7)
+    elif gname == "multi":
+        n, s, d = H.multigraph(24, 0.4, 5)
     elif gname == "K9":
         n, s, d = H.complete_graph(9)
     else:

@@ -44,6 +44,7 @@ GRAPHS = {
     "K7": lambda: H.complete_graph(7),
     "gnp-60": lambda: H.random_graph(60, 0.3, 5),
     "powerlaw": lambda: H.power_law_graph(3000, 20000, 2.1, 7),
+    "multi": lambda: H.multigraph(24, 0.4, 5),
 }
 
 
@@ -98,7 +99,7 @@ def compare_outcomes(zg, og, name, oargs, seed, first, count):
     return zo
 
 
-@pytest.mark.parametrize("gname", ["wiki-vote", "powerlaw", "gnp-60", "K7", "simple-30"])
+@pytest.mark.parametrize("gname", ["wiki-vote", "powerlaw", "gnp-60", "K7", "simple-30", "multi"])
 @pytest.mark.parametrize("name,oargs", [(p[0], p[1]) for p in H.PLANS])
 def test_outcomes_bit_exact(gname, name, oargs):
     n, s, d = GRAPHS[gname]()

@@ -183,6 +183,43 @@ def test_close_matches_edge_set():
         assert bool(got) == (key in rank and rank[key] >= mv)
 
 
+def test_close_on_parallel_edges_matches_any_copy():
+    # with parallel edges the awaited edge is closed when some copy has rank >= min_valid
+    import ctypes as C
+    n, s, d = H.multigraph(10, 0.5, 4, max_copies=5, heavy_copies=12)
+    g = O.OGraph(n, s, d, seed=2)
+    a = g.arrays()
+    ranks = {}
+    for r in range(g.m):
+        ranks.setdefault((min(a["src"][r], a["dst"][r]), max(a["src"][r], a["dst"][r])), []).append(r)
+    assert max(len(v) for v in ranks.values()) == 12
+    L = O.lib()
+    rng = np.random.default_rng(5)
+    for _ in range(800):
+        x, y = rng.choice(n, 2, replace=False)
+        mv = int(rng.integers(0, g.m + 1))
+        verts = np.array([x, y], np.uint32)
+        ws, wd = np.array([x], np.uint32), np.array([y], np.uint32)
+        got = L.og_close(C.byref(g.g), mv, verts.ctypes.data, 2, ws.ctypes.data, wd.ctypes.data, 1)
+        key = (min(x, y), max(x, y))
+        assert bool(got) == any(r >= mv for r in ranks.get(key, []))
+
+
+def test_exact_counters_collapse_parallel_edges():
+    # the naive_implementation counts are over the simple graph under the edge list
+    n, s, d = H.multigraph(12, 0.5, 6)
+    multi = O.OGraph(n, s, d)
+    pairs = sorted(set((min(x, y), max(x, y)) for x, y in zip(s.tolist(), d.tolist())))
+    ss = np.array([p[0] for p in pairs], np.uint32)
+    sd = np.array([p[1] for p in pairs], np.uint32)
+    simple = O.OGraph(n, ss, sd)
+    assert multi.m == len(s) > simple.m == len(pairs)
+    assert multi.triangles() == simple.pattern_brute(*O.TRIANGLE)
+    assert multi.k4() == simple.pattern_brute(*O.K4)
+    assert multi.wedges() == simple.paths_brute(3)
+    assert multi.paths3() == simple.paths_brute(4)
+
+
 # ---- estimators: exhaustive expectation equals the exact count ---------------------
 def _exact(g, name):
     if name in ("triangle", "pattern_triangle"):
