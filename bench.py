@@ -46,6 +46,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-query comparison")
     ap.add_argument("--elp-seeds", type=int, default=5, help="independent queries per profile and pattern")
     ap.add_argument("--elp-weights", default=None, help="learned-profile weights (default: the package's)")
+    ap.add_argument("--elp-cap-log2", type=int, default=37,
+                    help="largest budget of one time-to-error query, log2 estimators (2^37: above the "
+                         "~1.1e11 a +-1%% 4-clique answer needs on the bench graph)")
     ap.add_argument("--workload", default="chunglu69m", choices=["chunglu69m", "rmat26", "sizes"],
                     help="chunglu69m: the headline triangle + 4-clique workload; rmat26: R-MAT scale 26, "
                          "edge factor 16 with the 3-motif, 3-chain, 4-chain and predicate-triangle plans; "
@@ -202,6 +205,7 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95):
     weights = args.elp_weights or L.DEFAULT_WEIGHTS
     learned = L.LearnedErrorProfile(weights) if os.path.exists(weights) else None
     out = {"epsilon": eps, "confidence": conf, "queries_per_profile": args.elp_seeds,
+           "budget_cap": 1 << args.elp_cap_log2,
            "learned_weights": os.path.basename(weights) if learned else None}
     for name, kind, what in (("triangle", Z.PLAN_TRIANGLE, "triangle"), ("4-clique", Z.PLAN_K4_CLOSE, "k4")):
         t = time.perf_counter()
@@ -220,8 +224,8 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95):
             recs = []
             for q in range(args.elp_seeds):
                 inst = Z.ZGraphInstance(g, Z.Plan(kind), seed=101 + q)
-                r = L.estimate_to_error(inst, eps, conf, prof, judged=judged, max_estimators=1 << 36,
-                                        fallback_budget=fallback)
+                r = L.estimate_to_error(inst, eps, conf, prof, judged=judged,
+                                        max_estimators=1 << args.elp_cap_log2, fallback_budget=fallback)
                 r["error"] = abs(r["estimate"] - truth) / truth if truth and r["estimators"] else float("nan")
                 recs.append(r)
                 log(f"{name} {label} q{q}: {r['total_s'] * 1e3:.1f} ms N={r['estimators']} "
@@ -239,7 +243,9 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95):
                 "max_error": max((r["error"] for r in ok), default=None),
                 "sizing_used": sorted({r["sizing"] for r in recs}),
                 "mean_topup_rounds": sum(r["topup_rounds"] for r in ok) / len(ok) if ok else None,
-                "capped_at_2^36": sum(r["capped"] for r in recs),
+                "judge_accepted": sum(bool(r["accepted"]) for r in ok) / len(recs),
+                "mean_rel_half_width": sum(r["rel_half_width"] for r in ok) / len(ok) if ok else None,
+                "capped": sum(r["capped"] for r in recs),
             }
         lt = res.get("learned+judge", {}).get("mean_time_s")
         for base in ("regression", "regression+judge"):
