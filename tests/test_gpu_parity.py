@@ -76,6 +76,18 @@ def test_subsample_and_filter_equal_oracle():
     assert_same_csr(zg.filter(50.0, False, 4), og.filter(50.0, False, 4))
 
 
+def test_subsample_and_filter_on_parallel_edges_equal_oracle():
+    n, s, d = H.multigraph(40, 0.3, 9)
+    w = np.random.default_rng(9).integers(0, 79, size=len(s)).astype(np.float32)
+    zg, og = both(n, s, d, seed=6, w=w)
+    assert_same_csr(zg, og)
+    assert_same_csr(zg.subsample(0.3, 2), og.subsample(0.3, 2))
+    assert_same_csr(zg.filter(50.0, True, 3), og.filter(50.0, True, 3))
+    zf, of = zg.filter(50.0, False, 3), og.filter(50.0, False, 3)
+    assert_same_csr(zf, of)
+    compare_outcomes(zf, of, "triangle", dict(kind=1), seed=4, first=0, count=20000)
+
+
 def compare_outcomes(zg, og, name, oargs, seed, first, count):
     zplan = H.device_plan(Z, **oargs)
     oplan = H.oracle_plan(O, **oargs)
