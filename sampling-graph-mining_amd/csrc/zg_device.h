@@ -15,14 +15,15 @@ namespace zg {
 // one worker):
 //   edges[r] = 32-byte record of rank r ("edge id"): its endpoints, and for
 //              each endpoint the list start, the degree and the position of r
-//              in that list -- one 64-byte sector holds everything a sampled
+//              in that list -- one 128-byte line holds everything a sampled
 //              edge needs, so no offset lookups follow a sample
 //   off[v]   = start of v's lists (n+1 entries, u64; export and generic use)
 //   adj[i]   = rank << 32 | neighbour     each list ascending by rank
 //   sadj[i]  = neighbour << 32 | rank     each list ascending by (neighbour, rank)
 // The stored position turns the reference's lower_bound on "rank >= e+1" in
-// the endpoint lists of a sampled edge e into one load; sadj turns the linear
-// conditional_close scan (src/ZGraphInstance.cpp:371-407) into one binary search.
+// the endpoint lists of a sampled edge e into one load.  The linear
+// conditional_close scan (src/ZGraphInstance.cpp:371-407) becomes one lookup in
+// the edge-set hash below; sadj serves the exact counters and the export.
 // List starts are u32, so a graph holds fewer than 2^31 edges.
 // ---------------------------------------------------------------------------
 struct alignas(32) ERec {
