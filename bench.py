@@ -159,7 +159,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(Z, g, plans, args, N)
+        cpu = cpu_baseline(Z, g, plans, args, N, elp)
 
     out = {
         "metric": "estimators/s; time to ±1% err @95% conf, learned vs regression ELP",
@@ -246,6 +246,8 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95):
                 "judge_accepted": sum(bool(r["accepted"]) for r in ok) / len(recs),
                 "mean_rel_half_width": sum(r["rel_half_width"] for r in ok) / len(ok) if ok else None,
                 "capped": sum(r["capped"] for r in recs),
+                "queries": [{"seed": 101 + q, "estimators": r["estimators"], "estimate": r["estimate"],
+                             "total_s": r["total_s"]} for q, r in enumerate(recs)],
             }
         lt = res.get("learned+judge", {}).get("mean_time_s")
         for base in ("regression", "regression+judge"):
@@ -458,9 +460,11 @@ def profiled_traffic(kernel, args, N):
     return per if same else None
 
 
-def cpu_baseline(Z, g, plans, args, N):
+def cpu_baseline(Z, g, plans, args, N, elp=None):
     """The oracle's OpenMP restatement of ZGraphInstance::calculate on this
-    host, on a bounded number of estimators of the same workload."""
+    host, on a bounded number of estimators of the same workload.  With the
+    time-to-error results, also the CPU time of the first learned triangle
+    query's estimators (same seed and ids: the same answer, bit for bit)."""
     os.environ.setdefault("OMP_PROC_BIND", "close")   # read when the OpenMP runtime starts
     import pyoracle as O
     threads = min(16, os.cpu_count() or 1)
@@ -499,7 +503,21 @@ def cpu_baseline(Z, g, plans, args, N):
             log(f"cpu {name} (mt19937 draws): {mt[name]:.4g}/s")
     finally:
         O.lib().og_set_rng_mode(0)
+    tte = None
+    q = (((elp or {}).get("triangle") or {}).get("learned+judge") or {}).get("queries") or []
+    if q and q[0]["estimators"]:
+        nq = q[0]["estimators"]
+        t = time.perf_counter()
+        sums = O.run(og, O.make_plan(O.PLAN_TRIANGLE), q[0]["seed"], 0, nq, threads)
+        spent = time.perf_counter() - t
+        est = O.mean(sums[0])
+        tte = {"pattern": "triangle", "estimators": nq, "cpu_s": spent, "gpu_s": q[0]["total_s"],
+               "speedup": spent / q[0]["total_s"], "cpu_estimate": est, "gpu_estimate": q[0]["estimate"],
+               "same_answer": abs(est - q[0]["estimate"]) <= 1e-12 * abs(q[0]["estimate"])}
+        log(f"cpu time to +-1% (triangle, {nq} estimators of the learned query): {spent:.1f}s "
+            f"against {q[0]['total_s'] * 1e3:.1f} ms on the GPU, same answer {tte['same_answer']}")
     return {"value": 2 * N / per_step, "unit": "estimators/s", "cores": threads, "kind": "port",
+            "time_to_error": tte,
             "sample": "; ".join(f"{k}: {samples[k]} estimators (ids 0..) of the same graph/seed"
                                  for k in samples),
             "per_pattern": rates, "cpu_model": _cpu_model(),
