@@ -117,8 +117,13 @@ enum {
     ZG_PLAN_CHAIN = 4,        /* ChainMiningInstance(order) (applications/ChainMining.cpp:45-84) */
     ZG_PLAN_FOUR_CHAIN = 5,   /* FourChainCountingInstance (applications/FourChain.cpp:24-124) */
     ZG_PLAN_PATTERN = 6,      /* SamplerGenerator<Empty>(Pattern) (src/SamplerGenerator.cpp:378-464) */
-    ZG_PLAN_K4_CLOSE = 7      /* 4-clique: three spanning shapes + closing checks (unbiased;
+    ZG_PLAN_K4_CLOSE = 7,     /* 4-clique: three spanning shapes + closing checks (unbiased;
                                  replaces the biased applications/FourClique.cpp:41-76) */
+    ZG_PLAN_TRIANGLE_K4 = 8   /* fused: channel 0 = triangle (ZG_PLAN_TRIANGLE), channel 1 =
+                                 4-clique (ZG_PLAN_K4_CLOSE) from one first edge, conditional
+                                 sample and closing check per id -- the cached sub-pattern
+                                 sharing of ThreeMotif (src/ZGraphInstance.cpp:596-833) applied
+                                 to triangle -> 4-clique; each channel equals its own plan's run */
 };
 
 #define ZG_MAX_PATTERN_EDGES 15
@@ -133,7 +138,7 @@ typedef struct {
 
 int zg_plan_create(const zg_plan_desc *desc, zg_plan **out);
 int zg_plan_destroy(zg_plan *p);
-/* outputs per estimator (2 for ZG_PLAN_THREE_MOTIF, else 1) */
+/* outputs per estimator (2 for ZG_PLAN_THREE_MOTIF and ZG_PLAN_TRIANGLE_K4, else 1) */
 int zg_plan_channels(const zg_plan *p);
 /* conditional_sample_edge calls per successful estimate
  * (get_num_conditional_sample_edge_calls, src/ZGraphInstance.cpp:470-488;
@@ -197,7 +202,7 @@ int zg_run_outcomes(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first
  * (graph, plan, seed, first_id, count) exactly (sumsq up to rounding).  The
  * graphs and plans must outlive the batch; all queries run on device 0.
  * Output: one zg_sums per query and channel, in query order (a
- * ZG_PLAN_THREE_MOTIF query takes two records); zg_batch_records() gives the
+ * two-channel query takes two records); zg_batch_records() gives the
  * total.  count <= 2^31 per query. */
 typedef struct {
     zg_graph *graph;

@@ -288,6 +288,8 @@ extern "C" int zg_plan_create(const zg_plan_desc *d, zg_plan **out)
         break;
     case ZG_PLAN_FOUR_CHAIN: p->K = 2; p->cond_calls = 2; break;
     case ZG_PLAN_K4_CLOSE: p->K = 0; p->cond_calls = 2; break;
+    // each channel keeps its own plan's exponent, so its sums equal that plan's word for word
+    case ZG_PLAN_TRIANGLE_K4: p->K = 1; p->K1 = 0; p->cond_calls = 2; p->nch = 2; break;
     case ZG_PLAN_PATTERN: {
         const int k = d->pattern_vertices, E = d->pattern_edges;
         std::vector<uint32_t> uniq_masks;
@@ -393,7 +395,8 @@ int run_chunks(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
         a.succ = h[ch].succ;
         a.ovf = h[ch].ovf;
         a.sq = h[ch].sq;
-        finish_sums(&a, 1, p->K, count, &sums[ch]);
+        const int k = p->channel_k(ch);
+        finish_sums(&a, 1, k, k, count, &sums[ch]);
     }
     return ZG_OK;
 }

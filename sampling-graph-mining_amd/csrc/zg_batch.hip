@@ -219,7 +219,8 @@ extern "C" int zg_batch_run(zg_batch *b, zg_sums *out, float *ms)
     for (auto &g : b->groups)
         for (size_t j = 0; j < g.hq.size(); ++j) {
             const int qi = g.qidx[j];
-            finish_sums(b->h_result + g.result_off + j * g.nch, g.nch, b->queries[qi].plan->K, g.hq[j].count,
+            const zg_plan *pl = b->queries[qi].plan;
+            finish_sums(b->h_result + g.result_off + j * g.nch, g.nch, pl->channel_k(0), pl->channel_k(1), g.hq[j].count,
                         out + b->rec_off[qi]);
         }
     for (size_t r = 0; r < b->nrec; ++r)

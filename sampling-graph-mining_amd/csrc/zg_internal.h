@@ -26,6 +26,7 @@ static_assert(sizeof(zg_outcome) == 24, "zg_outcome is a 24-byte record");
 struct PlanView {
     int kind, order, pk, pe, K, nch;
     const uint32_t *accept;   // ZG_PLAN_PATTERN: bit table over relabeled edge masks
+    int K1;                   // channel 1's denominator exponent (two-channel plans)
 };
 
 // one query of a batched launch (device descriptor)
@@ -65,13 +66,16 @@ struct zg_graph {
 struct zg_plan {
     zg_plan_desc desc;
     int nch = 1, K = 0;
+    int K1 = -1;              // channel 1's exponent when it differs from channel 0's
     double cond_calls = 0;
     int nuniq = 0, ntotal = 0;
     uint32_t *d_accept = nullptr;
     zg::PlanView view() const
     {
-        return zg::PlanView{desc.kind, desc.order, desc.pattern_vertices, desc.pattern_edges, K, nch, d_accept};
+        return zg::PlanView{desc.kind, desc.order, desc.pattern_vertices, desc.pattern_edges, K, nch, d_accept,
+                            K1 < 0 ? K : K1};
     }
+    int channel_k(int ch) const { return ch == 1 && K1 >= 0 ? K1 : K; }
 };
 
 namespace zg {
@@ -103,7 +107,7 @@ int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
 // launches of at most this many estimators when they run in first-edge order
 constexpr uint64_t kMaxOrderedLaunch = 1ull << 28;
 int resolve_order(const zg_graph *g, const zg_plan *p, uint64_t count, int order);
-void finish_sums(const Acc *host_acc, int nch, int K, uint64_t count, zg_sums *out);
+void finish_sums(const Acc *host_acc, int nch, int K0, int K1, uint64_t count, zg_sums *out);
 // one kernel kind's batched launch (the plan kind, or 100 + k for k-vertex
 // patterns): counter reset, work-queue kernel, per-query reduction
 int launch_batch(int kind, const BQuery *d_q, int nq, uint64_t nchunks, uint32_t chunk_est,

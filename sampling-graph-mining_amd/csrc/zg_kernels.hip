@@ -351,6 +351,12 @@ __device__ __forceinline__ void plan_pattern(const GView &g, const PlanView &P, 
 // Each shape has one draw sequence of probability 1/(m s1 s2), so the sum of
 // the three shape values is unbiased for every rank order.  (The reference's
 // applications/FourClique.cpp:41-76 covers shape A only.)
+//
+// TRI: also channel 0 = the triangle plan's outcome for the same id.  Both
+// plans draw the first edge and the conditional sample identically, and for
+// w outside {a, b} the triangle plan's closing check is this plan's first one,
+// {ob, w} above r1 (for w in {a, b} neither succeeds), with value m * s1.
+template <bool TRI>
 __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
 {
     if (g.m == 0) return;
@@ -375,6 +381,11 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
     // every shape needs the triangle edge {ob, w} above r1: test it first
     uint32_t rt;
     if (!close_pair(g, ob, w, r1 + 1, &rt)) return;
+    if constexpr (TRI) {
+        U128 tv{g.m, 0};
+        const bool tov = mul_u128(tv, s1);
+        set_out(o, 0, tv, tov, 0);
+    }
     const VI O = oa ? A : B, OB = oa ? B : A;
     const uint32_t po1 = e1.pos;                 // rank r1 in o's list
     uint32_t pw1;                                // rank r1 in w's list
@@ -462,7 +473,7 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
             }
         }
     }
-    if (total.lo | total.hi) set_out(o, 0, total, ov, 0);
+    if (total.lo | total.hi) set_out(o, TRI ? 1 : 0, total, ov, 0);
 }
 
 // ---- accumulation ---------------------------------------------------------
@@ -580,7 +591,8 @@ __device__ __forceinline__ void run_one(const GView &g, const PlanView &P, Rng &
     else if constexpr (KIND == ZG_PLAN_THREE_MOTIF) plan_wedge<2>(g, rng, o);
     else if constexpr (KIND == ZG_PLAN_CHAIN) plan_chain(g, P, rng, o);
     else if constexpr (KIND == ZG_PLAN_FOUR_CHAIN) plan_four_chain(g, rng, o);
-    else if constexpr (KIND == ZG_PLAN_K4_CLOSE) plan_k4_close(g, rng, o);
+    else if constexpr (KIND == ZG_PLAN_K4_CLOSE) plan_k4_close<false>(g, rng, o);
+    else if constexpr (KIND == ZG_PLAN_TRIANGLE_K4) plan_k4_close<true>(g, rng, o);
     else if constexpr (KIND >= KIND_PATTERN_BASE + 2) plan_pattern<KIND - KIND_PATTERN_BASE>(g, P, rng, o);
 }
 
@@ -611,7 +623,7 @@ __global__ __launch_bounds__(256) void k_run(GView g, PlanView P, uint64_t seed,
         run_one<KIND>(g, P, rng, o);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
-            acc_add(acc[c], o.v[c], o.st[c], o.den[c], P.K);
+            acc_add(acc[c], o.v[c], o.st[c], o.den[c], c == 0 ? P.K : P.K1);
             if (OUT) {
                 zg_outcome rec;
                 const bool ok = o.st[c] != 0;
@@ -664,7 +676,7 @@ __global__ __launch_bounds__(256) void k_run_ordered(GView g, PlanView P, uint64
         Est o;
         run_one<KIND>(g, P, rng, o);
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) acc_add(acc[c], o.v[c], o.st[c], o.den[c], P.K);
+        for (int c = 0; c < NCH; ++c) acc_add(acc[c], o.v[c], o.st[c], o.den[c], c == 0 ? P.K : P.K1);
     }
     block_reduce_store<NCH>(acc, partials + (size_t)blockIdx.x * NCH);
 }
@@ -712,7 +724,7 @@ __global__ __launch_bounds__(256) void k_batch(const BQuery *__restrict__ q, int
             Est o;
             run_one<KIND>(g, P, rng, o);
 #pragma unroll
-            for (int ch = 0; ch < NCH; ++ch) acc_add(acc[ch], o.v[ch], o.st[ch], o.den[ch], P.K);
+            for (int ch = 0; ch < NCH; ++ch) acc_add(acc[ch], o.v[ch], o.st[ch], o.den[ch], ch == 0 ? P.K : P.K1);
         }
         block_reduce_store<NCH>(acc, chunk_partials + c * NCH);
         __syncthreads();   // s_chunk and the reduction's LDS are reused next round
@@ -909,6 +921,7 @@ int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
         }
         break;
     case ZG_PLAN_K4_CLOSE: launch_kind<ZG_PLAN_K4_CLOSE, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
+    case ZG_PLAN_TRIANGLE_K4: launch_kind<ZG_PLAN_TRIANGLE_K4, 2>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
     default: return set_error(ZG_EINVAL, "unknown plan kind");
     }
     ZG_HIP(hipGetLastError());
@@ -943,15 +956,17 @@ int launch_batch(int kind, const BQuery *d_q, int nq, uint64_t nchunks, uint32_t
     case KIND_PATTERN_BASE + 5: launch_batch_kind<KIND_PATTERN_BASE + 5, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
     case KIND_PATTERN_BASE + 6: launch_batch_kind<KIND_PATTERN_BASE + 6, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
     case ZG_PLAN_K4_CLOSE: launch_batch_kind<ZG_PLAN_K4_CLOSE, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
+    case ZG_PLAN_TRIANGLE_K4: launch_batch_kind<ZG_PLAN_TRIANGLE_K4, 2>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
     default: return set_error(ZG_EINVAL, "unknown plan kind");
     }
     ZG_HIP(hipGetLastError());
     return ZG_OK;
 }
 
-void finish_sums(const Acc *h, int nch, int K, uint64_t count, zg_sums *out)
+void finish_sums(const Acc *h, int nch, int K0, int K1, uint64_t count, zg_sums *out)
 {
     for (int c = 0; c < nch; ++c) {
+        const int K = c == 0 ? K0 : K1;
         // carry-propagate the 32-bit digit sums into 192 bits
         uint32_t limb[6] = {0, 0, 0, 0, 0, 0};
         unsigned __int128 carry = 0;

@@ -686,6 +686,31 @@ class ThreeMotifInstance : public ZGraphInstance<EdgeData> {
     }
 };
 
+// Triangles (channel 0, TriangleCoutingInstance's estimator) and 4-cliques
+// (channel 1, the three-shape plan) from one first edge, conditional sample and
+// closing check per id: ThreeMotif's cached sub-pattern sharing
+// (src/ZGraphInstance.cpp:596-833) applied to triangle -> 4-clique.  Each
+// channel equals its own instance's calculate on the same ids.
+template <typename EdgeData>
+class TriangleFourCliqueInstance : public ZGraphInstance<EdgeData> {
+  public:
+    explicit TriangleFourCliqueInstance(Graph<EdgeData> *g, int channel_ = 0) : ZGraphInstance<EdgeData>(g)
+    {
+        this->set_plan(Plan::of(ZG_PLAN_TRIANGLE_K4));
+        this->channel = channel_;
+    }
+    // both counts from one pass over ids [0, N)
+    std::pair<double, double> calculate_both(unsigned long long num_estimators)
+    {
+        this->require_plan();
+        zg_sums s[2];
+        zg_launch launch{0, 0, 0};
+        check(zg_run(this->get_run_graph()->handle(), this->plan()->handle(), this->seed, 0, num_estimators, &launch,
+                     s, nullptr));
+        return {s[0].mean, s[1].mean};
+    }
+};
+
 // applications/ChainMining.cpp:18-85 (order = vertices of the path)
 template <typename EdgeData>
 class ChainMiningInstance : public ZGraphInstance<EdgeData> {
@@ -1054,6 +1079,7 @@ using zgraph::RandomMapping;
 using zgraph::SamplerGenerator;
 using zgraph::ThreeChainCountingInstance;
 using zgraph::ThreeMotifInstance;
+using zgraph::TriangleFourCliqueInstance;
 using zgraph::TimeProfile;
 using zgraph::TriangleCoutingInstance;
 using zgraph::UniformGraphPartitioner;

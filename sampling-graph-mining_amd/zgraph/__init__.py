@@ -18,12 +18,13 @@ import numpy as np
 
 from . import _capi
 from ._capi import (PLAN_CHAIN, PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_PATTERN, PLAN_THREE_CHAIN,
-                    PLAN_THREE_MOTIF, PLAN_TRIANGLE, ZGraphError, check, lib)
+                    PLAN_THREE_MOTIF, PLAN_TRIANGLE, PLAN_TRIANGLE_K4, ZGraphError, check, lib)
 from .formats import load_biedgelist, load_edgelist, load_graph_meta_data, save_biedgelist
 
 __all__ = [
     "Graph", "DeviceEdges", "erdos_renyi", "chung_lu", "rmat", "Plan", "Pattern", "Estimation",
     "ZGraphInstance", "TriangleCoutingInstance", "ThreeChainCountingInstance", "ThreeMotifInstance",
+    "TriangleFourCliqueInstance",
     "ChainMiningInstance", "FourChainCountingInstance", "FourCliqueCountingInstance",
     "SamplerGenerator", "AllPredicateZGraphInstance", "ErrorProfile", "TimeProfile",
     "load_graph_meta_data", "load_biedgelist", "load_edgelist", "save_biedgelist", "ZGraphError",
@@ -517,27 +518,42 @@ class ThreeChainCountingInstance(ZGraphInstance):
     plan_kind = PLAN_THREE_CHAIN
 
 
-class ThreeMotifInstance(ZGraphInstance):
+class _TwoChannelInstance(ZGraphInstance):
+    """A plan whose every estimator id yields two patterns' outcomes."""
+
+    def calculate_both(self, n0, n1):
+        """Channel 0 over ids [0, n0) and channel 1 over ids [0, n1), from
+        max(n0, n1) ids: two Estimations."""
+        n = max(n0, n1)
+        e0, e1 = Estimation(), Estimation()
+        if n0 == n1:
+            s = run(self.graph, self.plan, self.seed, 0, n)
+            e0.absorb(s[0]); e0.update(n, s[0].mean)
+            e1.absorb(s[1]); e1.update(n, s[1].mean)
+        else:
+            lo = min(n0, n1)
+            s = run(self.graph, self.plan, self.seed, 0, lo)
+            s2 = run(self.graph, self.plan, self.seed, lo, n - lo)
+            for ch, e, cnt in ((0, e0, n0), (1, e1, n1)):
+                e.absorb(s[ch]); e.update(lo, s[ch].mean)
+                if cnt > lo:
+                    e.absorb(s2[ch]); e.update(n - lo, s2[ch].mean)
+        return e0, e1
+
+
+class ThreeMotifInstance(_TwoChannelInstance):
     """applications/ThreeMotif.cpp: 3-chain (channel 0) and triangle (channel 1)
     from shared wedges, the device analogue of CachedSubPatterns."""
     plan_kind = PLAN_THREE_MOTIF
 
-    def calculate_both(self, n_chain, n_triangle):
-        n = max(n_chain, n_triangle)
-        ec, et = Estimation(), Estimation()
-        if n_chain == n_triangle:
-            s = run(self.graph, self.plan, self.seed, 0, n)
-            ec.absorb(s[0]); ec.update(n, s[0].mean)
-            et.absorb(s[1]); et.update(n, s[1].mean)
-        else:
-            lo = min(n_chain, n_triangle)
-            s = run(self.graph, self.plan, self.seed, 0, lo)
-            s2 = run(self.graph, self.plan, self.seed, lo, n - lo)
-            for ch, e, cnt in ((0, ec, n_chain), (1, et, n_triangle)):
-                e.absorb(s[ch]); e.update(lo, s[ch].mean)
-                if cnt > lo:
-                    e.absorb(s2[ch]); e.update(n - lo, s2[ch].mean)
-        return ec, et
+
+class TriangleFourCliqueInstance(_TwoChannelInstance):
+    """Triangles (channel 0, TriangleCoutingInstance's estimator) and 4-cliques
+    (channel 1, the three-shape plan) from one first edge, conditional sample
+    and closing check per id: the CachedSubPatterns sharing of ThreeMotif
+    (src/ZGraphInstance.cpp:596-833) applied to triangle -> 4-clique.  Each
+    channel equals its own plan's run on the same ids."""
+    plan_kind = PLAN_TRIANGLE_K4
 
 
 class ChainMiningInstance(ZGraphInstance):

@@ -24,6 +24,7 @@ PLAN_CHAIN = 4
 PLAN_FOUR_CHAIN = 5
 PLAN_PATTERN = 6
 PLAN_K4_CLOSE = 7
+PLAN_TRIANGLE_K4 = 8
 
 MAX_PATTERN_EDGES = 15
 

@@ -104,6 +104,22 @@ def test_three_motif_app(tmp_path):
 
 
 @pytest.mark.gpu
+def test_triangle_four_clique_app(tmp_path):
+    n, s, d = golden_graph("wiki-vote")
+    prefix = _write_graph(tmp_path, "wv", n, s, d)
+    N, seed = 1 << 20, 5
+    r = subprocess.run([_bin("TriangleFourClique"), prefix, str(N), str(seed)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = _report(r.stdout)
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    tri = Z.run(g, Z.Plan(Z.PLAN_TRIANGLE), seed, 0, N)[0]
+    k4 = Z.run(g, Z.Plan(Z.PLAN_K4_CLOSE), seed, 0, N)[0]
+    assert rows["triangle"][0] == pytest.approx(tri.mean, rel=1e-12)
+    assert rows["4-clique"][0] == pytest.approx(k4.mean, rel=1e-12)
+
+
+@pytest.mark.gpu
 def test_predicate_apps(tmp_path):
     """All-predicate and at-least-one weighted triangles on the reference's
     weighted wiki-vote (u32 weights in [0, 79)): the C++ mains equal the
