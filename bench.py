@@ -49,11 +49,13 @@ def main():
     ap.add_argument("--elp-cap-log2", type=int, default=37,
                     help="largest budget of one time-to-error query, log2 estimators (2^37: above the "
                          "~1.1e11 a +-1%% 4-clique answer needs on the bench graph)")
-    ap.add_argument("--workload", default="chunglu69m", choices=["chunglu69m", "rmat26", "sizes"],
+    ap.add_argument("--workload", default="chunglu69m", choices=["chunglu69m", "rmat26", "sizes", "elp"],
                     help="chunglu69m: the headline triangle + 4-clique workload; rmat26: R-MAT scale 26, "
                          "edge factor 16 with the 3-motif, 3-chain, 4-chain and predicate-triangle plans; "
                          "sizes: 3-chain, triangle and 4-clique on R-MAT 22/24/26 and Chung-Lu 69M/500M "
-                         "(how the rate holds as the graph outgrows the translation caches)")
+                         "(how the rate holds as the graph outgrows the translation caches); elp: time to "
+                         "+-1%% for triangle, 4-clique, 3-chain and 4-chain on four graphs, learned against "
+                         "regression sizing")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this size (rocprof counter calibration)")
     ap.add_argument("--line-sweep", action="store_true",
@@ -97,6 +99,9 @@ def main():
         return
     if args.workload == "sizes":
         size_sweep(Z, args)
+        return
+    if args.workload == "elp":
+        elp_sweep(Z, args)
         return
 
     info = Z.device_info()
@@ -221,7 +226,7 @@ def fused_triangle_k4(Z, g, args, N, first_sums):
             "equal_to_separate_launches": equal}
 
 
-def time_to_error(Z, g, args, eps=0.01, conf=0.95):
+def time_to_error(Z, g, args, eps=0.01, conf=0.95, patterns=None, cap_log2=None, queries=None):
     """Time to an answer within +-eps at `conf` confidence, regression profile
     (ErrorProfile sizing as the reference runs it, without and with the CLT
     judge) against the learned profile (pilot + MLP sizing + judge), several
@@ -231,10 +236,13 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95):
     from zgraph import elp as L
     weights = args.elp_weights or L.DEFAULT_WEIGHTS
     learned = L.LearnedErrorProfile(weights) if os.path.exists(weights) else None
-    out = {"epsilon": eps, "confidence": conf, "queries_per_profile": args.elp_seeds,
-           "budget_cap": 1 << args.elp_cap_log2,
+    cap_log2 = cap_log2 or args.elp_cap_log2
+    queries = queries or args.elp_seeds
+    patterns = patterns or (("triangle", Z.PLAN_TRIANGLE, "triangle"), ("4-clique", Z.PLAN_K4_CLOSE, "k4"))
+    out = {"epsilon": eps, "confidence": conf, "queries_per_profile": queries,
+           "budget_cap": 1 << cap_log2,
            "learned_weights": os.path.basename(weights) if learned else None}
-    for name, kind, what in (("triangle", Z.PLAN_TRIANGLE, "triangle"), ("4-clique", Z.PLAN_K4_CLOSE, "k4")):
+    for name, kind, what in patterns:
         t = time.perf_counter()
         truth = g.exact_count(what)
         t_exact = time.perf_counter() - t
@@ -249,10 +257,10 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95):
             L.estimate_to_error(warm, eps, conf, prof, judged=judged, max_estimators=1 << 30,
                                 fallback_budget=fallback)
             recs = []
-            for q in range(args.elp_seeds):
+            for q in range(queries):
                 inst = Z.ZGraphInstance(g, Z.Plan(kind), seed=101 + q)
                 r = L.estimate_to_error(inst, eps, conf, prof, judged=judged,
-                                        max_estimators=1 << args.elp_cap_log2, fallback_budget=fallback)
+                                        max_estimators=1 << cap_log2, fallback_budget=fallback)
                 r["error"] = abs(r["estimate"] - truth) / truth if truth and r["estimators"] else float("nan")
                 recs.append(r)
                 log(f"{name} {label} q{q}: {r['total_s'] * 1e3:.1f} ms N={r['estimators']} "
@@ -352,6 +360,55 @@ def batched_queries(Z, args):
         b.close()
     for g in graphs:
         g.close()
+    return out
+
+
+def elp_sweep(Z, args):
+    """Time to +-1 % at 95 % for triangle, 4-clique, 3-chain and 4-chain on four
+    seeded graphs (the 69 M-edge bench graph, R-MAT scale 22, Erdos-Renyi with
+    10 M vertices / 100 M edges, Chung-Lu with 2 M vertices / 20 M edges and
+    beta 2.5): the reference's regression sizing against pilot + network +
+    judge, 3 queries each, budgets capped at 2^34, errors against the device
+    exact counts."""
+    graphs = [("chunglu 4.8M/69M b2.1", lambda: Z.chung_lu(4_800_000, 69_000_000, 2.1, 20000.0, args.seed)),
+              ("rmat22", lambda: Z.rmat(22, 16 << 22, 0.57, 0.19, 0.19, args.seed)),
+              ("erdos-renyi 10M/100M", lambda: Z.erdos_renyi(10_000_000, 100_000_000, args.seed)),
+              ("chunglu 2M/20M b2.5", lambda: Z.chung_lu(2_000_000, 20_000_000, 2.5, 5000.0, args.seed))]
+    patterns = (("triangle", Z.PLAN_TRIANGLE, "triangle"), ("4-clique", Z.PLAN_K4_CLOSE, "k4"),
+                ("3-chain", Z.PLAN_THREE_CHAIN, "wedge"), ("4-chain", Z.PLAN_FOUR_CHAIN, "path3"))
+    rows = {}
+    for gname, make in graphs:
+        e = make()
+        g = Z.Graph.from_device(e, seed=args.seed)
+        e.free()
+        log(f"elp sweep {gname}: m={g.num_local_edges} maxdeg={g.max_degree}")
+        rows[gname] = time_to_error(Z, g, args, patterns=patterns, cap_log2=34, queries=3)
+        if gname in ("chunglu 4.8M/69M b2.1", "rmat22"):
+            rows[gname]["coverage"] = elp_coverage(Z, g, args, patterns[0:1] + patterns[2:])
+        g.close()
+    print(json.dumps({"elp_sweep": rows}), flush=True)
+
+
+def elp_coverage(Z, g, args, patterns, queries=40, eps=0.01, conf=0.95):
+    """Fraction of learned + judge answers within +-eps of the exact count over
+    `queries` independent queries (seeds 1000..): the judge's stopping rule
+    should hold the nominal confidence."""
+    from zgraph import elp as L
+    prof = L.LearnedErrorProfile(args.elp_weights or L.DEFAULT_WEIGHTS)
+    out = {}
+    for name, kind, what in patterns:
+        truth = g.exact_count(what)
+        errs, accepted, ns = [], 0, []
+        for q in range(queries):
+            inst = Z.ZGraphInstance(g, Z.Plan(kind), seed=1000 + q)
+            r = L.estimate_to_error(inst, eps, conf, prof, max_estimators=1 << 34, fallback_budget=1 << 24)
+            errs.append(abs(r["estimate"] - truth) / truth)
+            accepted += bool(r["accepted"])
+            ns.append(r["estimators"])
+        within = sum(x <= eps for x in errs)
+        out[name] = {"queries": queries, "within_eps": within, "judge_accepted": accepted,
+                     "median_estimators": sorted(ns)[queries // 2], "max_error": max(errs)}
+        log(f"coverage {name}: {within}/{queries} within +-{eps * 100:g}%, max error {max(errs) * 100:.2f}%")
     return out
 
 
