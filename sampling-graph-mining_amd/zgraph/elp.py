@@ -29,13 +29,16 @@ import time
 
 import numpy as np
 
-from . import (PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_THREE_CHAIN, PLAN_TRIANGLE, ErrorProfile, Estimation,
-               ZGraphError, _z)
+from . import (PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_THREE_CHAIN, PLAN_THREE_MOTIF, PLAN_TRIANGLE,
+               PLAN_TRIANGLE_K4, ErrorProfile, Estimation, ZGraphError, _z, run)
 
 IN_DIM = 16
 PATTERN_SLOTS = {PLAN_TRIANGLE: 0, PLAN_K4_CLOSE: 1, PLAN_THREE_CHAIN: 2, PLAN_FOUR_CHAIN: 3}
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "elp_weights.npz")
 MIN_PILOT_SUCCESSES = 32
+# the single plan whose estimator each channel of a two-channel plan carries
+CHANNEL_KINDS = {PLAN_THREE_MOTIF: (PLAN_THREE_CHAIN, PLAN_TRIANGLE),
+                 PLAN_TRIANGLE_K4: (PLAN_TRIANGLE, PLAN_K4_CLOSE)}
 MAX_PILOT = 1 << 28
 
 
@@ -249,3 +252,71 @@ def estimate_to_error(instance, eps, conf, profile, init_estimation=None, num_ve
     if error:
         rec["error_message"] = error
     return rec
+
+
+def estimate_both_to_error(instance, eps, conf, profile, max_estimators=1 << 40, max_rounds=8,
+                           fallback_budget=1 << 24):
+    """Both channels of a two-channel plan (3-motif; fused triangle + 4-clique)
+    to +-eps at `conf`: one pilot gives both channels' statistics, each channel
+    is sized by the network with its own pattern slot (judge-only start where it
+    cannot be), the query runs to the larger size over one id range, and the
+    judge tops up until both channels' intervals hold.  Returns one record per
+    channel (estimate, estimators, half-width, accepted) and the shared times."""
+    kinds = CHANNEL_KINDS.get(instance.plan.kind)
+    if kinds is None:
+        raise ValueError("not a two-channel plan")
+    t0 = time.perf_counter()
+    g, plan, seed = instance.graph, instance.plan, instance.seed
+    ests = [Estimation(), Estimation()]
+    done = 0
+
+    def extend(n):
+        nonlocal done
+        if n > done:
+            sums = run(g, plan, seed, done, n - done)
+            for c in (0, 1):
+                ests[c].absorb(sums[c])
+                ests[c].update(n - done, sums[c].mean)
+            done = n
+
+    n_p = profile.pilot
+    while True:
+        extend(n_p)
+        if min(e.successes for e in ests) >= MIN_PILOT_SUCCESSES or n_p >= profile.max_pilot:
+            break
+        n_p = min(n_p * 8, profile.max_pilot)
+    gf = graph_features(g)
+    z = _z(conf)
+    sizes, how = [], []
+    for c in (0, 1):
+        mean, cv, succ = pilot_stats(ests[c].sums_view())
+        if (profile.model is not None and ests[c].successes >= MIN_PILOT_SUCCESSES and math.isfinite(cv)
+                and cv > 0):
+            r = float(profile.model.predict(feature_vector(gf, kinds[c], n_p, cv, succ))[0])
+            sizes.append(max(int(math.ceil((z * cv * 10.0 ** r / eps) ** 2)), n_p))
+            how.append("learned")
+        else:
+            sizes.append(max(fallback_budget, n_p))
+            how.append("judge-only")
+    t_size = time.perf_counter() - t0
+    n = min(max(sizes), max_estimators)
+    extend(n)
+    rounds = 0
+    while not all(judge(e, eps, conf) for e in ests) and rounds < max_rounds and n < max_estimators:
+        grow = 1.25
+        for e in ests:
+            mean, hw = e.exact_mean(), e.half_width(conf)
+            g_c = (hw / (eps * mean)) ** 2 * 1.05 if mean > 0 and math.isfinite(hw) and hw > 0 else 4.0
+            grow = max(grow, g_c)
+        n = min(int(math.ceil(n * min(grow, 64.0))), max_estimators)
+        extend(n)
+        rounds += 1
+    t1 = time.perf_counter()
+    chans = []
+    for c in (0, 1):
+        mean, hw = ests[c].exact_mean(), ests[c].half_width(conf)
+        chans.append({"estimate": mean, "sizing": how[c], "sized_estimators": sizes[c],
+                      "half_width": hw, "rel_half_width": hw / mean if mean > 0 else float("inf"),
+                      "accepted": judge(ests[c], eps, conf)})
+    return {"channels": chans, "estimators": done, "topup_rounds": rounds, "capped": max(sizes) > max_estimators,
+            "sizing_s": t_size, "total_s": t1 - t0}

@@ -68,3 +68,24 @@ def test_sizing_failure_falls_back_to_the_judge():
     r = L.estimate_to_error(inst, 0.5, 0.95, prof, max_estimators=1 << 26, max_rounds=2)
     assert r["sizing"] in ("judge-only", "regression")
     assert r["estimators"] > 0
+
+
+@pytest.mark.parametrize("inst_cls,whats", [(Z.ThreeMotifInstance, ("wedge", "triangle")),
+                                            (Z.TriangleFourCliqueInstance, ("triangle", "k4"))])
+def test_two_channel_query_reaches_the_error_on_both(inst_cls, whats):
+    # one id range serves both channels; each channel's answer equals its own plan over those ids
+    n, s, d = H.golden_graph("wiki-vote")
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    truths = [g.exact_count(w) for w in whats]
+    prof = L.LearnedErrorProfile()
+    eps = 0.05
+    inst = inst_cls(g, seed=41)
+    r = L.estimate_both_to_error(inst, eps, 0.95, prof, max_estimators=1 << 32)
+    say(f"[elp] {inst_cls.__name__}: N={r['estimators']} rounds={r['topup_rounds']} "
+        + ", ".join(f"{c['sizing']} {c['estimate']:.6g} +- {c['half_width']:.3g}" for c in r["channels"]))
+    kinds = L.CHANNEL_KINDS[inst.plan.kind]
+    for c, (chan, truth) in enumerate(zip(r["channels"], truths)):
+        assert chan["accepted"] and chan["rel_half_width"] <= eps
+        assert abs(chan["estimate"] - truth) <= 2 * chan["half_width"]
+        single = Z.run(g, Z.Plan(kinds[c]), 41, 0, r["estimators"])[0]
+        assert chan["estimate"] == pytest.approx(single.mean, rel=1e-12)
