@@ -978,20 +978,60 @@ class LearnedErrorProfile {
             if (est.get_successes() >= 32 || n >= max_pilot_) break;
             n = std::min(n * 8, max_pilot_);
         }
-        const double t = (double)est.get_num_finished_estimators(), mean = est.exact_mean();
-        if (est.get_successes() < 32 || !(mean > 0)) return 0;
-        const double var = std::max(est.get_sumsq() / t - mean * mean, 0.0);
-        const double cv = std::sqrt(var) / mean, succ = (double)est.get_successes() / t;
-        if (!(cv > 0) || !std::isfinite(cv)) return 0;
-        float x[16];
-        features(inst, sl, (double)n, cv, succ, x);
-        float y = 0;
-        for (int i = 0; i < 16; ++i) x[i] = (x[i] - mu_[i]) / sd_[i];
-        check(zg_mlp_forward(net_, x, 1, &y));
-        const double r = use_[sl] ? (double)(y * y_scale_) : 0.0;
-        const double z = normal_quantile_two_sided(conf);
-        const double q = z * cv * std::pow(10.0, r) / eps;
-        return std::max((unsigned long long)std::ceil(q * q), n);
+        return budget(inst.get_run_graph(), sl, n, est, eps, conf);
+    }
+
+    // Both channels of a two-channel plan (3-motif; fused triangle + 4-clique)
+    // to +-eps at `conf` over one id range: one pilot, each channel sized with
+    // its own pattern slot (a 2^24 start where the network cannot size it), and
+    // the judge on both (zgraph/elp.py estimate_both_to_error).  Returns the ids run.
+    unsigned long long run_both(ZGraphInstance<EdgeData> &inst, double eps, double conf, Estimation &e0,
+                                Estimation &e1, unsigned long long max_estimators = 1ull << 40, int max_rounds = 8)
+    {
+        if (!inst.plan()) throw Error(ZG_EINVAL, "LearnedErrorProfile: instance has no device plan");
+        int k0, k1;
+        switch (plan_kind(inst)) {
+        case ZG_PLAN_THREE_MOTIF: k0 = ZG_PLAN_THREE_CHAIN; k1 = ZG_PLAN_TRIANGLE; break;
+        case ZG_PLAN_TRIANGLE_K4: k0 = ZG_PLAN_TRIANGLE; k1 = ZG_PLAN_K4_CLOSE; break;
+        default: throw Error(ZG_EINVAL, "LearnedErrorProfile::run_both: not a two-channel plan");
+        }
+        Graph<EdgeData> *g = inst.get_run_graph();
+        unsigned long long done = 0;
+        auto extend = [&](unsigned long long n) {
+            if (n <= done) return;
+            zg_sums s[2];
+            zg_launch launch{0, 0, 0};
+            check(zg_run(g->handle(), inst.plan()->handle(), inst.seed, done, n - done, &launch, s, nullptr));
+            e0.absorb(s[0]);
+            e0.update(n - done, s[0].mean);
+            e1.absorb(s[1]);
+            e1.update(n - done, s[1].mean);
+            done = n;
+        };
+        unsigned long long n = pilot_;
+        for (;;) {
+            extend(n);
+            if (std::min(e0.get_successes(), e1.get_successes()) >= 32 || n >= max_pilot_) break;
+            n = std::min(n * 8, max_pilot_);
+        }
+        unsigned long long b[2] = {budget(g, slot(k0), n, e0, eps, conf), budget(g, slot(k1), n, e1, eps, conf)};
+        for (auto &x : b)
+            if (!x) x = std::max(1ull << 24, n);
+        unsigned long long N = std::min(std::max(b[0], b[1]), max_estimators);
+        extend(N);
+        for (int r = 0; r < max_rounds && N < max_estimators && !(judge(e0, eps, conf) && judge(e1, eps, conf));
+             ++r) {
+            double grow = 1.25;
+            for (const Estimation *e : {&e0, &e1}) {
+                const double mu = e->exact_mean(), hw = e->half_width(conf);
+                grow = std::max(grow, (mu > 0 && std::isfinite(hw) && hw > 0) ? (hw / (eps * mu)) * (hw / (eps * mu)) * 1.05
+                                                                             : 4.0);
+            }
+            N = std::min((unsigned long long)std::ceil((double)N * std::min(grow, 64.0)), max_estimators);
+            extend(N);
+        }
+        last_sized = std::max(b[0], b[1]);
+        return done;
     }
 
     unsigned long long get_num_estimators(ZGraphInstance<EdgeData> &inst, VertexId k, double eps, double conf)
@@ -1019,9 +1059,28 @@ class LearnedErrorProfile {
 
   private:
     static int plan_kind(ZGraphInstance<EdgeData> &inst) { return inst.plan()->kind(); }
-    void features(ZGraphInstance<EdgeData> &inst, int sl, double pilot_n, double cv, double succ, float *x)
+    // the learned budget of a pattern slot from the pilot's statistics, or 0 when undefined
+    unsigned long long budget(Graph<EdgeData> *g, int sl, unsigned long long n, const Estimation &est, double eps,
+                              double conf)
     {
-        Graph<EdgeData> *g = inst.get_run_graph();
+        if (sl < 0) return 0;
+        const double t = (double)est.get_num_finished_estimators(), mean = est.exact_mean();
+        if (est.get_successes() < 32 || !(mean > 0)) return 0;
+        const double var = std::max(est.get_sumsq() / t - mean * mean, 0.0);
+        const double cv = std::sqrt(var) / mean, succ = (double)est.get_successes() / t;
+        if (!(cv > 0) || !std::isfinite(cv)) return 0;
+        float x[16];
+        features(g, sl, (double)n, cv, succ, x);
+        float y = 0;
+        for (int i = 0; i < 16; ++i) x[i] = (x[i] - mu_[i]) / sd_[i];
+        check(zg_mlp_forward(net_, x, 1, &y));
+        const double r = use_[sl] ? (double)(y * y_scale_) : 0.0;
+        const double z = normal_quantile_two_sided(conf);
+        const double q = z * cv * std::pow(10.0, r) / eps;
+        return std::max((unsigned long long)std::ceil(q * q), n);
+    }
+    void features(Graph<EdgeData> *g, int sl, double pilot_n, double cv, double succ, float *x)
+    {
         double s2 = 0, s3 = 0;
         check(zg_graph_degree_moments(g->handle(), &s2, &s3));
         const double n = (double)g->num_vertices, m = (double)g->num_local_edges, dmax = (double)g->max_degree;

@@ -1,7 +1,8 @@
 // A query sized by the learned error-latency profile and judged to
 // +-eps at 95 % confidence: argv[2] pattern (triangle, 4-clique, 3-chain,
-// 4-chain), argv[3] eps (default 0.01), argv[4] estimator seed, argv[5] the
-// weights file (default: elp_weights.bin beside libzgraph_amd.so).
+// 4-chain; or the two-channel 3-motif and triangle+4-clique, both channels to
+// +-eps over one id range), argv[3] eps (default 0.01), argv[4] estimator seed,
+// argv[5] the weights file (default: elp_weights.bin beside libzgraph_amd.so).
 #include <cstring>
 
 #include "AppCommon.hpp"
@@ -22,6 +23,24 @@ int main(int argc, char **argv)
         if (weights.empty()) {
             const std::string self = argv[0];
             weights = self.substr(0, self.rfind('/') + 1) + "../../zgraph/elp_weights.bin";
+        }
+        if (pat == "3-motif" || pat == "triangle+4-clique") {
+            std::unique_ptr<ZGraphInstance<Empty>> both;
+            if (pat == "3-motif") both.reset(new ThreeMotifInstance<Empty>(&graph));
+            else both.reset(new TriangleFourCliqueInstance<Empty>(&graph));
+            both->seed = seed;
+            LearnedErrorProfile<Empty> profile(weights);
+            Estimation e0, e1;
+            const unsigned long long n = profile.run_both(*both, eps, 0.95, e0, e1);
+            const char *names[2] = {pat == "3-motif" ? "3-chain" : "triangle", pat == "3-motif" ? "triangle" : "4-clique"};
+            std::printf("sized %llu\n", profile.last_sized);
+            const Estimation *es[2] = {&e0, &e1};
+            for (int c = 0; c < 2; ++c) {
+                std::printf("judged %s %d %.17g\n", names[c], zgraph::judge(*es[c], eps, 0.95) ? 1 : 0,
+                            es[c]->half_width(0.95) / es[c]->exact_mean());
+                report(names[c], es[c]->exact_mean(), n);
+            }
+            return 0;
         }
         std::unique_ptr<ZGraphInstance<Empty>> inst;
         if (pat == "triangle") inst.reset(new TriangleCoutingInstance<Empty>(&graph));

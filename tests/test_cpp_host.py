@@ -171,6 +171,33 @@ def test_learned_profile_cpp_equals_python(tmp_path, pattern, kind):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("pattern,cls,names", [("triangle+4-clique", "TriangleFourCliqueInstance", ("triangle", "4-clique")),
+                                               ("3-motif", "ThreeMotifInstance", ("3-chain", "triangle"))])
+def test_learned_two_channel_cpp_equals_python(tmp_path, pattern, cls, names):
+    """The C++ LearnedErrorProfile::run_both sizes, runs and judges a
+    two-channel query as zgraph.elp.estimate_both_to_error does."""
+    from zgraph import elp as L
+    n, s, d = golden_graph("wiki-vote")
+    prefix = _write_graph(tmp_path, "wv", n, s, d)
+    eps, seed = 0.05, 41
+    print(f"[cpp] EstimateToError {pattern}", flush=True)
+    r = subprocess.run([_bin("EstimateToError"), prefix, pattern, str(eps), str(seed)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    sized = int(re.search(r"^sized (\d+)$", r.stdout, re.M).group(1))
+    rows = _report(r.stdout)
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    rec = L.estimate_both_to_error(getattr(Z, cls)(g, seed=seed), eps, 0.95, L.LearnedErrorProfile())
+    assert abs(max(c["sized_estimators"] for c in rec["channels"]) - sized) <= 1
+    for c, name in enumerate(names):
+        judged = re.search(r"^judged " + re.escape(name) + r" (\d) (\S+)$", r.stdout, re.M)
+        assert judged.group(1) == "1" and float(judged.group(2)) <= eps
+        est, cnt = rows[name]
+        if rec["estimators"] == cnt:
+            assert est == pytest.approx(rec["channels"][c]["estimate"], rel=1e-12)
+
+
+@pytest.mark.gpu
 def test_triangle_app_error_profile_equals_python(tmp_path):
     """Triangle main without a budget sizes by ErrorProfile (the reference's
     flow, +-5 % at 95 %): the C++ ErrorProfile and the Python one give the
