@@ -15,7 +15,7 @@ def say(*a):
 def _plans():
     return [Z.Plan(Z.PLAN_TRIANGLE), Z.Plan(Z.PLAN_THREE_CHAIN), Z.Plan(Z.PLAN_THREE_MOTIF),
             Z.Plan(Z.PLAN_CHAIN, order=5), Z.Plan(Z.PLAN_FOUR_CHAIN), Z.Plan(Z.PLAN_K4_CLOSE),
-            Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_STAR_PATTERN)]
+            Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_STAR_PATTERN), Z.Plan(Z.PLAN_TRIANGLE_K4)]
 
 
 @pytest.mark.parametrize("order", [2, 8, 16])
