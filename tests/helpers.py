@@ -58,6 +58,45 @@ def power_law_graph(n, m, beta, seed):
     return n, e[perm, 0].copy(), e[perm, 1].copy()
 
 
+def hub_graph(leaves, extra, seed):
+    """Vertex 0 joined to every leaf 1..leaves, plus `extra` random leaf-leaf
+    edges: one very long list (the long-list search paths) and triangles that
+    all pass through the hub."""
+    rng = np.random.default_rng(seed)
+    pairs = {(0, i) for i in range(1, leaves + 1)}
+    while len(pairs) < leaves + extra:
+        a, b = (int(x) for x in rng.integers(1, leaves + 1, size=2))
+        if a != b:
+            pairs.add((min(a, b), max(a, b)))
+    e = np.array(sorted(pairs), dtype=np.uint32)
+    e = e[rng.permutation(len(e))]
+    return leaves + 1, e[:, 0].copy(), e[:, 1].copy()
+
+
+def bipartite_graph(a, b, p, seed):
+    """Random bipartite graph between {0..a-1} and {a..a+b-1}: no odd cycles."""
+    rng = np.random.default_rng(seed)
+    u, v = np.meshgrid(np.arange(a), np.arange(a, a + b), indexing="ij")
+    keep = rng.random(u.shape) < p
+    s, d = u[keep].astype(np.uint32), v[keep].astype(np.uint32)
+    perm = rng.permutation(len(s))
+    return a + b, s[perm].copy(), d[perm].copy()
+
+
+def sparse_ids_graph(n, m, seed):
+    """m random edges among a few hundred vertices of an id space of size n
+    (most vertices isolated)."""
+    rng = np.random.default_rng(seed)
+    ids = rng.choice(n, size=300, replace=False).astype(np.uint32)
+    pairs = set()
+    while len(pairs) < m:
+        a, b = rng.choice(ids, size=2, replace=False)
+        pairs.add((min(a, b), max(a, b)))
+    e = np.array(sorted(pairs), dtype=np.uint32)
+    e = e[rng.permutation(len(e))]
+    return n, e[:, 0].copy(), e[:, 1].copy()
+
+
 def multigraph(n, p, seed, max_copies=12, heavy_copies=40):
     """G(n, p) with every pair repeated 1..max_copies times (each copy randomly
     oriented) and one pair repeated heavy_copies times: parallel edges, which

@@ -45,6 +45,9 @@ GRAPHS = {
     "gnp-60": lambda: H.random_graph(60, 0.3, 5),
     "powerlaw": lambda: H.power_law_graph(3000, 20000, 2.1, 7),
     "multi": lambda: H.multigraph(24, 0.4, 5),
+    "hub": lambda: H.hub_graph(6000, 3000, 2),
+    "bipartite": lambda: H.bipartite_graph(40, 60, 0.3, 3),
+    "sparse-ids": lambda: H.sparse_ids_graph(1_000_000, 3000, 4),
 }
 
 
@@ -111,7 +114,8 @@ def compare_outcomes(zg, og, name, oargs, seed, first, count):
     return zo
 
 
-@pytest.mark.parametrize("gname", ["wiki-vote", "powerlaw", "gnp-60", "K7", "simple-30", "multi"])
+@pytest.mark.parametrize("gname", ["wiki-vote", "powerlaw", "gnp-60", "K7", "simple-30", "multi", "hub", "bipartite",
+                                   "sparse-ids"])
 @pytest.mark.parametrize("name,oargs", [(p[0], p[1]) for p in H.PLANS])
 def test_outcomes_bit_exact(gname, name, oargs):
     n, s, d = GRAPHS[gname]()
@@ -120,6 +124,8 @@ def test_outcomes_bit_exact(gname, name, oargs):
     zo = compare_outcomes(zg, og, name, oargs, seed=17, first=123, count=count)
     if gname == "K7" and name not in ("pattern_house", "pattern_k4", "pattern_5star"):
         assert (zo["status"] == 1).any()
+    if gname == "bipartite" and name in ("triangle", "k4_close", "pattern_triangle", "pattern_k4"):
+        assert not (zo["status"] == 1).any()   # no odd cycles
 
 
 @pytest.mark.parametrize("gname", ["wiki-vote", "powerlaw", "gnp-60", "K7", "multi"])
