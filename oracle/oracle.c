@@ -6,12 +6,13 @@
  *
  * Parity status (what pins each function; see tests/):
  *   og_philox4x32_10   Random123 known-answer vectors + rocRAND's Philox
- *                      (tests/golden/philox_kat.json)                pinned
+ *                      (tests/golden/philox_vectors.json)            pinned
  *   og_uniform         exhaustive small-range rejection checks + chi^2   pinned
  *   og_graph_build     CSR invariants (rank-ascending lists, sum deg = 2m,
  *                      every rank twice) on the reference fixtures       pinned
  *   core cond. sample  brute-force enumeration of every r on tiny graphs pinned
- *   og_close           brute-force edge-set membership on tiny graphs    pinned
+ *   og_close           brute-force edge-set membership on tiny graphs,
+ *                      parallel edges ("some copy above min_valid")     pinned
  *   estimators         exhaustive expectation E[X|graph,ranks] == exact
  *                      count (closed forms / brute force)              pinned
  *   pattern dedupe     tests/TestPattern2.cpp values (12/24, 1/24, 5/120,
@@ -21,6 +22,9 @@
  *                      binaries built from source into oracle/_ref       pinned
  *   og_regression_k    tests/TestRegression.cpp known answer k^2=609.8565 pinned
  *   og_elp_*           hand-computed closed form on 1-3 points           pinned
+ *   generators         restated from the device generators' definition
+ *                      (not the reference's); the device output is
+ *                      compared element by element                     pinned
  */
 #include "zg_oracle.h"
 
