@@ -178,3 +178,20 @@ def test_abi_argument_errors_without_gpu():
         with pytest.raises(Z.ZGraphError):
             P.unique_relabelings()
     assert "pattern" in _capi.last_error()
+
+
+def test_plan_kinds_and_channels_without_gpu():
+    """Plan creation is host logic (pattern plans aside, which upload their
+    acceptance table): every kind has its channel count and conditional-call
+    constant; an unknown kind is rejected."""
+    want = {Z.PLAN_TRIANGLE: (1, 1), Z.PLAN_THREE_CHAIN: (1, 1), Z.PLAN_THREE_MOTIF: (2, 1),
+            Z.PLAN_FOUR_CHAIN: (1, 2), Z.PLAN_K4_CLOSE: (1, 2), Z.PLAN_TRIANGLE_K4: (2, 2)}
+    for kind, (nch, calls) in want.items():
+        p = Z.Plan(kind)
+        assert (p.channels, p.cond_calls) == (nch, calls)
+    assert Z.Plan(Z.PLAN_CHAIN, order=6).cond_calls == 4
+    for bad in (0, 9, 100):
+        with pytest.raises(Z.ZGraphError):
+            Z.Plan(bad)
+    with pytest.raises(Z.ZGraphError):
+        Z.Plan(Z.PLAN_CHAIN, order=2)
