@@ -155,6 +155,10 @@ def main():
         gather[ch], _ = Z.random_gather_bandwidth(16 << 30, ch, 1 << 28, args.block, args.blocks_per_cu)
     achievable = max(gather.values())
     log(f"random gather: " + ", ".join(f"{c} chain(s)/lane {v:.0f} GB/s" for c, v in gather.items()))
+    # the same measurement on a buffer the size of the graph's resident arrays
+    same_size = max(Z.random_gather_bandwidth(max(g.device_bytes, 1 << 30), ch, 1 << 28, args.block,
+                                              args.blocks_per_cu)[0] for ch in (1, 4))
+    log(f"random gather on {g.device_bytes / 2**30:.1f} GiB: {same_size:.0f} GB/s")
     traffic = profiled_traffic(dom, args, N)
     fused = fused_triangle_k4(Z, g, args, N, first_sums)
 
@@ -194,7 +198,9 @@ def main():
                      "kernel": dom, "algorithmic_bytes_per_estimator": bytes_per_est,
                      "estimators_per_launch": N,
                      "achievable_random_gather_gbs": achievable,
-                     "frac_of_achievable": achieved / achievable},
+                     "frac_of_achievable": achieved / achievable,
+                     "achievable_random_gather_gbs_graph_size": same_size,
+                     "frac_of_achievable_graph_size": achieved / same_size},
         "fused_triangle_4clique": fused,
         "cpu_baseline": cpu,
         "time_to_error": elp,
