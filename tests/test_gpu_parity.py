@@ -171,3 +171,34 @@ def test_ids_not_launch_shape_define_outcomes():
     c2 = Z.run(zg, plan, 5, 100000, 200000)[0]
     assert a.total() == b.total() == c1.total() + c2.total()
     assert a.successes == b.successes == c1.successes + c2.successes
+
+
+def _random_case(k):
+    """Seeded random graph k: G(n, p) with n in [5, 200] and p in [0.03, 0.9],
+    sometimes with parallel edges, sometimes spread over a large id space."""
+    rng = np.random.default_rng(9000 + k)
+    n = int(rng.integers(5, 201))
+    p = float(rng.uniform(0.03, 0.9))
+    kind = k % 3
+    if kind == 1:
+        n, s, d = H.multigraph(n, p, 9000 + k, max_copies=int(rng.integers(2, 13)),
+                               heavy_copies=int(rng.integers(9, 30)))
+    else:
+        n, s, d = H.random_graph(n, p, 9000 + k)
+    if kind == 2 and len(s):   # spread the vertices over an id space 1000x larger
+        ids = np.sort(rng.choice(n * 1000, size=n, replace=False)).astype(np.uint32)
+        s, d, n = ids[s], ids[d], n * 1000
+    first = int(rng.integers(0, 1 << 44))
+    seed = int(rng.integers(1, 1 << 62))
+    return n, s, d, seed, first
+
+
+@pytest.mark.parametrize("k", range(12))
+def test_random_graphs_all_plans_bit_exact(k):
+    n, s, d, seed, first = _random_case(k)
+    if len(s) == 0:
+        pytest.skip("empty draw")
+    zg, og = both(n, s, d, seed=seed % 1000)
+    assert_same_csr(zg, og)
+    for name, oargs in [(p[0], p[1]) for p in H.PLANS]:
+        compare_outcomes(zg, og, name, oargs, seed=seed, first=first, count=4000)
