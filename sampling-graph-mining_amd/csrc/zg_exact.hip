@@ -12,8 +12,8 @@
 // copy, parallel edges collapsed).  Triangles: per oriented edge u->v, binary
 // search every out-neighbour of the shorter list in the longer one, the work
 // spread evenly over lanes through a prefix sum of the shorter lengths.  4-cliques:
-// one wavefront per oriented edge builds W = out(u) & out(v) in a scratch
-// slot, then counts pairs w->x inside W.
+// one wavefront per oriented edge builds W = out(u) & out(v) in LDS, then
+// counts pairs w->x inside W with the lanes sharing each intersection.
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -221,21 +221,31 @@ __global__ __launch_bounds__(256) void k_tri_flat(const uint64_t *ooff, const ui
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-// one wavefront per oriented edge; W lives in this wave's scratch slot
+// One wavefront per oriented edge u -> v: W = out(u) & out(v), built by ballot
+// compaction in the wave's LDS slot (in its global scratch slot when the
+// shorter list exceeds the LDS slot), then for each w in W the pairs w -> x
+// with x in W, the lanes sharing the shorter side of each intersection.  An
+// edge whose shorter list has fewer than two entries closes no 4-clique.
+constexpr int kK4LdsSlot = 2048;   // u32 entries of W per wave (8 KB; 32 KB per 256-thread block)
+
 __global__ __launch_bounds__(256) void k_k4(const uint64_t *ooff, const uint32_t *onb, const uint32_t *osrc,
                                             uint64_t ne, uint32_t *scratch, uint64_t slot,
                                             unsigned long long *partial)
 {
+    __shared__ uint32_t lds[4 * kK4LdsSlot];
     const int lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    uint32_t *W = scratch + wave * slot;
+    uint32_t *const Wl = lds + (threadIdx.x >> 6) * kK4LdsSlot;
+    uint32_t *const Wg = scratch + wave * slot;
     unsigned long long c = 0;
     for (uint64_t e = wave; e < ne; e += nwaves) {
         const uint32_t u = osrc[e], v = onb[e];
         const uint32_t *a = onb + ooff[u], *b = onb + ooff[v];
         uint64_t na = ooff[u + 1] - ooff[u], nb = ooff[v + 1] - ooff[v];
         if (na > nb) { const uint32_t *t = a; a = b; b = t; const uint64_t x = na; na = nb; nb = x; }
+        if (na < 2) continue;
+        uint32_t *W = na <= (uint64_t)kK4LdsSlot ? Wl : Wg;
         // W = a & b, in increasing id order (ballot compaction keeps the order)
         uint64_t nw = 0;
         for (uint64_t base = 0; base < na; base += 64) {
@@ -248,18 +258,21 @@ __global__ __launch_bounds__(256) void k_k4(const uint64_t *ooff, const uint32_t
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        // pairs w -> x with both in W
-        for (uint64_t j = lane; j < nw; j += 64) {
-            const uint32_t w = W[j];
-            const uint32_t *ow = onb + ooff[w];
-            const uint64_t now = ooff[w + 1] - ooff[w];
-            if (now <= nw) {
-                for (uint64_t k = 0; k < now; ++k) c += bsearch_u32(W, nw, ow[k]);
-            } else {
-                for (uint64_t k = 0; k < nw; ++k) c += bsearch_u32(ow, now, W[k]);
+        if (nw >= 2) {
+            for (uint64_t j = 0; j < nw; ++j) {   // wave-uniform: w = W[j]
+                const uint32_t w = W[j];
+                const uint32_t *ow = onb + ooff[w];
+                const uint64_t now = ooff[w + 1] - ooff[w];
+                if (now <= nw) {
+                    for (uint64_t k = lane; k < now; k += 64) c += bsearch_u32(W, nw, ow[k]);
+                } else {
+                    for (uint64_t k = lane; k < nw; k += 64) c += bsearch_u32(ow, now, W[k]);
+                }
             }
         }
-        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();   // W is rewritten by the next edge
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
     const unsigned long long t = block_sum(c);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
