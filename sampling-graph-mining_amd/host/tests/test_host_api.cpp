@@ -176,6 +176,23 @@ static void test_no_plan()
     EXPECT(code == ZG_EINVAL);
 }
 
+// plan descriptors are host logic: channel counts and conditional-call
+// constants of the plans; an unknown kind is rejected
+static void test_plans()
+{
+    zgraph::Plan tri(zgraph::Plan::of(ZG_PLAN_TRIANGLE)), motif(zgraph::Plan::of(ZG_PLAN_THREE_MOTIF)),
+        fused(zgraph::Plan::of(ZG_PLAN_TRIANGLE_K4));
+    EXPECT(tri.channels() == 1 && motif.channels() == 2 && fused.channels() == 2);
+    EXPECT(fused.cond_calls() == 2.0 && tri.cond_calls() == 1.0);
+    int code = 0;
+    try {
+        zgraph::Plan bad(zgraph::Plan::of(9));
+    } catch (const zgraph::Error &err) {
+        code = err.code();
+    }
+    EXPECT(code == ZG_EINVAL);
+}
+
 int main(int argc, char **argv)
 {
     const std::string dir = argc > 1 ? argv[1] : ".";
@@ -186,6 +203,7 @@ int main(int argc, char **argv)
     test_bound();
     test_files(dir);
     test_no_plan();
+    test_plans();
     std::printf("%s\n", failures ? "FAILED" : "OK");
     return failures ? 1 : 0;
 }
