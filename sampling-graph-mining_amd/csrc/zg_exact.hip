@@ -12,8 +12,10 @@
 // copy, parallel edges collapsed).  Triangles: per oriented edge u->v, binary
 // search every out-neighbour of the shorter list in the longer one, the work
 // spread evenly over lanes through a prefix sum of the shorter lengths.  4-cliques:
-// one wavefront per oriented edge builds W = out(u) & out(v) in LDS, then
-// counts pairs w->x inside W with the lanes sharing each intersection.
+// per vertex u, bit rows over out(u) (row i: which of out(u) are in
+// out(out(u)[i])) and the sum of popcount(row_i & row_j) over the set bits j
+// of each row.
+#include <cstdlib>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -221,58 +223,119 @@ __global__ __launch_bounds__(256) void k_tri_flat(const uint64_t *ooff, const ui
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-// One wavefront per oriented edge u -> v: W = out(u) & out(v), built by ballot
-// compaction in the wave's LDS slot (in its global scratch slot when the
-// shorter list exceeds the LDS slot), then for each w in W the pairs w -> x
-// with x in W, the lanes sharing the shorter side of each intersection.  An
-// edge whose shorter list has fewer than two entries closes no 4-clique.
-constexpr int kK4LdsSlot = 2048;   // u32 entries of W per wave (8 KB; 32 KB per 256-thread block)
+// 4-cliques by bit rows: the binary-encoded induced subgraphs of Almasri et
+// al., "Parallel K-clique Counting on GPUs" (ICS 2022).  For every vertex u
+// whose out-list L = out(u) has d >= 3 entries, row i (the row of oriented
+// edge u -> L[i]) has bit p set iff L[p] is in out(L[i]), i.e. it is the set
+// W = out(u) & out(L[i]) over positions of L.  A 4-clique whose lowest vertex
+// is u, next L[i], then L[j], then L[p] is bit p of row_i & row_j, so the
+// count at u is the sum over rows i and set bits j of popcount(row_i & row_j).
+// Vertex u's rows take d * ceil(d / 64) words from boff[u]; a vertex whose
+// out-list is longer than sqrt(2m) cannot exist, so one vertex needs at most
+// 2^16 rows of 2^10 words.
+constexpr int kK4RowLds = 256;   // words of one row staged in LDS (out-lists up to 16384)
 
-__global__ __launch_bounds__(256) void k_k4(const uint64_t *ooff, const uint32_t *onb, const uint32_t *osrc,
-                                            uint64_t ne, uint32_t *scratch, uint64_t slot,
-                                            unsigned long long *partial)
+__device__ __forceinline__ uint64_t clog2_u64(uint64_t x) { return x <= 1 ? 0 : 64 - __clzll(x - 1); }
+
+__device__ __forceinline__ uint64_t lower_u32(const uint32_t *a, uint64_t n, uint32_t x)
 {
-    __shared__ uint32_t lds[4 * kK4LdsSlot];
-    const int lane = threadIdx.x & 63;
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// words of each vertex's rows (0 below three out-neighbours)
+__global__ void k_k4_words(const uint64_t *ooff, uint32_t n, uint64_t *words)
+{
+    GRID_STRIDE(u, (uint64_t)n + 1)
+    {
+        const uint64_t d = u < n ? ooff[u + 1] - ooff[u] : 0;
+        words[u] = d >= 3 ? d * ((d + 63) >> 6) : 0;
+    }
+}
+
+// One wavefront per oriented edge of [e0, e1) writes its row, whichever way
+// probes less: ballots over 64 positions of L at a time, each lane searching
+// its entry of L in out(v); or, for a short out(v), each lane searching one
+// entry of out(v) in L and setting its bit in an LDS copy of the row.
+__global__ __launch_bounds__(256) void k_k4_rows(const uint64_t *ooff, const uint32_t *onb, const uint32_t *osrc,
+                                                 const uint64_t *boff, uint64_t bbase, uint64_t e0, uint64_t e1,
+                                                 uint64_t *bits)
+{
+    __shared__ unsigned long long srows[4 * kK4RowLds];
+    const uint32_t lane = threadIdx.x & 63;
+    unsigned long long *const srow = srows + (threadIdx.x >> 6) * kK4RowLds;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    uint32_t *const Wl = lds + (threadIdx.x >> 6) * kK4LdsSlot;
-    uint32_t *const Wg = scratch + wave * slot;
-    unsigned long long c = 0;
-    for (uint64_t e = wave; e < ne; e += nwaves) {
+    for (uint64_t e = e0 + wave; e < e1; e += nwaves) {
         const uint32_t u = osrc[e], v = onb[e];
-        const uint32_t *a = onb + ooff[u], *b = onb + ooff[v];
-        uint64_t na = ooff[u + 1] - ooff[u], nb = ooff[v + 1] - ooff[v];
-        if (na > nb) { const uint32_t *t = a; a = b; b = t; const uint64_t x = na; na = nb; nb = x; }
-        if (na < 2) continue;
-        uint32_t *W = na <= (uint64_t)kK4LdsSlot ? Wl : Wg;
-        // W = a & b, in increasing id order (ballot compaction keeps the order)
-        uint64_t nw = 0;
-        for (uint64_t base = 0; base < na; base += 64) {
-            const uint64_t i = base + lane;
-            const bool hit = i < na && bsearch_u32(b, nb, a[i]);
-            const uint64_t mask = __ballot(hit);
-            if (hit) W[nw + __popcll(mask & ((1ull << lane) - 1))] = a[i];
-            nw += __popcll(mask);
+        const uint64_t lb = ooff[u], d = ooff[u + 1] - lb;
+        if (d < 3) continue;
+        const uint64_t R = (d + 63) >> 6;
+        uint64_t *const row = bits + (boff[u] - bbase) + (e - lb) * R;
+        const uint32_t *const L = onb + lb;
+        const uint64_t vb = ooff[v], dv = ooff[v + 1] - vb;
+        const uint32_t *const ov = onb + vb;
+        const bool staged = R <= (uint64_t)kK4RowLds && ((dv + 63) >> 6) * clog2_u64(d + 1) < R * clog2_u64(dv + 1);
+        if (!staged) {
+            for (uint64_t kb = 0; kb < R; ++kb) {
+                const uint64_t j = kb * 64 + lane;
+                const bool hit = j < d && bsearch_u32(ov, dv, L[j]);
+                const uint64_t w = __ballot(hit);
+                if (lane == 0) row[kb] = w;
+            }
+            continue;
+        }
+        for (uint64_t t = lane; t < R; t += 64) srow[t] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (uint64_t xb = 0; xb < dv; xb += 64) {
+            const uint64_t t = xb + lane;
+            if (t < dv) {
+                const uint32_t x = ov[t];
+                const uint64_t j = lower_u32(L, d, x);
+                if (j < d && L[j] == x) atomicOr(&srow[j >> 6], 1ull << (j & 63));
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        if (nw >= 2) {
-            for (uint64_t j = 0; j < nw; ++j) {   // wave-uniform: w = W[j]
-                const uint32_t w = W[j];
-                const uint32_t *ow = onb + ooff[w];
-                const uint64_t now = ooff[w + 1] - ooff[w];
-                if (now <= nw) {
-                    for (uint64_t k = lane; k < now; k += 64) c += bsearch_u32(W, nw, ow[k]);
-                } else {
-                    for (uint64_t k = lane; k < nw; k += 64) c += bsearch_u32(ow, now, W[k]);
-                }
+        for (uint64_t t = lane; t < R; t += 64) row[t] = srow[t];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();   // the LDS row is rewritten by the next edge
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+}
+
+// One thread per row of [e0, e1): the sum over its set bits j of
+// popcount(row_i & row_j).
+__global__ void k_k4_count(const uint64_t *ooff, const uint32_t *osrc, const uint64_t *boff, uint64_t bbase,
+                           uint64_t e0, uint64_t e1, const uint64_t *bits, unsigned long long *partial)
+{
+    unsigned long long c = 0;
+    for (uint64_t e = e0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < e1;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = osrc[e];
+        const uint64_t lb = ooff[u], d = ooff[u + 1] - lb;
+        if (d < 3) continue;
+        const uint64_t R = (d + 63) >> 6;
+        const uint64_t *const rb = bits + (boff[u] - bbase);
+        const uint64_t *const ri = rb + (e - lb) * R;
+        if (R == 1) {
+            const uint64_t w0 = ri[0];
+            for (uint64_t wb = w0; wb; wb &= wb - 1) c += __popcll(w0 & rb[__builtin_ctzll(wb)]);
+            continue;
+        }
+        for (uint64_t kb = 0; kb < R; ++kb) {
+            for (uint64_t wb = ri[kb]; wb; wb &= wb - 1) {
+                const uint64_t *const rj = rb + (kb * 64 + __builtin_ctzll(wb)) * R;
+                for (uint64_t t = 0; t < R; ++t) c += __popcll(ri[t] & rj[t]);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();   // W is rewritten by the next edge
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
     const unsigned long long t = block_sum(c);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
@@ -552,18 +615,74 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         *out = s;
         return ZG_OK;
     }
-    const int grid = (int)std::min<uint64_t>((uint64_t)device_cus() * 8, std::max<uint64_t>(1, (ne + 255) / 256));
-    XB<unsigned long long> part;
-    XALLOC(part, grid);
-    const uint64_t slot = maxout ? maxout : 1;
-    const uint64_t nwaves = (uint64_t)grid * 4;
-    XB<uint32_t> scratch;
-    XALLOC(scratch, nwaves * slot);
-    hipLaunchKernelGGL(k_k4, dim3(grid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, ne, scratch.p, slot, part.p);
+    // 4-cliques: bit rows in vertex ranges whose rows fit the scratch budget
+    XB<uint64_t> words, boff;
+    XALLOC(words, (uint64_t)n + 1); XALLOC(boff, (uint64_t)n + 1);
+    hipLaunchKernelGGL(k_k4_words, dim3(gridx((uint64_t)n + 1)), dim3(256), 0, st, ooff.p, n, words.p);
     ZG_HIP(hipGetLastError());
-    unsigned long long s = 0;
-    int rc = host_sum(part.p, grid, st, &s);
+    size_t tk = 0;
+    ZG_HIP(rocprim::exclusive_scan(nullptr, tk, words.p, boff.p, (uint64_t)0, (size_t)n + 1,
+                                   rocprim::plus<uint64_t>(), st));
+    XB<char> tmpk;
+    XALLOC(tmpk, tk);
+    ZG_HIP(rocprim::exclusive_scan(tmpk.p, tk, words.p, boff.p, (uint64_t)0, (size_t)n + 1,
+                                   rocprim::plus<uint64_t>(), st));
+    auto read_u64 = [&](const uint64_t *p, uint64_t *v) -> int {
+        ZG_HIP(hipMemcpyAsync(v, p, 8, hipMemcpyDeviceToHost, st));
+        ZG_HIP(hipStreamSynchronize(st));
+        return ZG_OK;
+    };
+    uint64_t total_words = 0;
+    int rc = read_u64(boff.p + n, &total_words);
     if (rc) return rc;
+    if (total_words == 0) { *out = 0; return ZG_OK; }
+    size_t free_b = 0, total_b = 0;
+    ZG_HIP(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t one_vertex = maxout * ((maxout + 63) >> 6);
+    uint64_t budget = std::min<uint64_t>(16ull << 30, std::max<uint64_t>(1ull << 30, free_b / 4)) / 8;
+    if (const char *ev = getenv("ZGRAPH_K4_SCRATCH_WORDS")) budget = strtoull(ev, nullptr, 10);   // smaller passes
+    budget = std::max(budget, one_vertex);
+    XB<uint64_t> bits;
+    XALLOC(bits, std::min(total_words, budget));
+    const int rgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16, std::max<uint64_t>(1, (ne + 3) / 4));
+    const int cgrid = gridx(ne);
+    XB<unsigned long long> part;
+    XALLOC(part, cgrid);
+    unsigned long long s = 0;
+    for (uint32_t u0 = 0; u0 < n;) {
+        uint64_t b0 = 0;
+        rc = read_u64(boff.p + u0, &b0);
+        if (rc) return rc;
+        uint32_t u1 = n;
+        if (total_words - b0 > budget) {   // largest u1 with boff[u1] - boff[u0] <= budget
+            uint32_t lo = u0 + 1, hi = n;
+            while (lo < hi) {
+                const uint32_t mid = lo + (hi - lo + 1) / 2;
+                uint64_t bm = 0;
+                rc = read_u64(boff.p + mid, &bm);
+                if (rc) return rc;
+                if (bm - b0 <= budget) lo = mid; else hi = mid - 1;
+            }
+            u1 = lo;
+        }
+        uint64_t e0 = 0, e1 = 0;
+        rc = read_u64(ooff.p + u0, &e0);
+        if (!rc) rc = read_u64(ooff.p + u1, &e1);
+        if (rc) return rc;
+        if (e1 > e0) {
+            hipLaunchKernelGGL(k_k4_rows, dim3(rgrid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, boff.p, b0, e0, e1,
+                               bits.p);
+            ZG_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_k4_count, dim3(cgrid), dim3(256), 0, st, ooff.p, osrc.p, boff.p, b0, e0, e1, bits.p,
+                               part.p);
+            ZG_HIP(hipGetLastError());
+            unsigned long long ps = 0;
+            rc = host_sum(part.p, cgrid, st, &ps);
+            if (rc) return rc;
+            s += ps;
+        }
+        u0 = u1;
+    }
     *out = s;
     return ZG_OK;
 }

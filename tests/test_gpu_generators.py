@@ -100,3 +100,51 @@ def test_device_house_count_agrees_with_the_house_estimator():
     print(f"[house] exact {H_exact} estimate {r.mean:.6g} +- {hw:.3g} ({r.successes} successes)", flush=True)
     assert r.successes > 100
     assert abs(r.mean - H_exact) <= hw
+
+
+@pytest.mark.parametrize("n", [66, 130, 200])
+def test_device_clique_counts_of_complete_graphs(n):
+    """K_n: C(n, 3) triangles and C(n, 4) 4-cliques.  Ids break the degree
+    ties, so the common out-list of edge (0, 1) holds n - 2 members: more
+    than one group of 64 for the 4-clique counter's per-member work."""
+    from math import comb
+    import helpers as H
+    n, s, d = H.complete_graph(n)
+    g = Z.Graph.from_edges(s, d, n, seed=2)
+    assert g.exact_count("triangle") == comb(n, 3)
+    assert g.exact_count("k4") == comb(n, 4)
+
+
+def test_device_k4_count_with_long_out_lists():
+    """Two vertices u, v share 2 200 common out-neighbours H, so u's bit rows
+    are 35 words long and its rows are written both by ballots and through
+    the LDS copy.  H holds a matching of 100 pairs, and every H vertex also
+    links to all of 2 300 low-degree vertices L, which keeps H above u and v
+    in degree order.
+    Closed forms: 4-cliques {u, v, h1, h2} per matched pair = 100; triangles
+    {u, v, h} 2 200, {u | v, h1, h2} 200, {h1, h2, l} 100 * 2 300."""
+    nh, nl, pairs = 2200, 2300, 100
+    u, v = 0, 1
+    H0, L0 = 2, 2 + nh
+    hs = np.arange(H0, H0 + nh, dtype=np.uint32)
+    ls = np.arange(L0, L0 + nl, dtype=np.uint32)
+    src = [np.array([u], np.uint32), np.full(nh, u, np.uint32), np.full(nh, v, np.uint32),
+           hs[0:2 * pairs:2], np.repeat(hs, nl)]
+    dst = [np.array([v], np.uint32), hs, hs, hs[1:2 * pairs:2], np.tile(ls, nh)]
+    s, d = np.concatenate(src), np.concatenate(dst)
+    g = Z.Graph.from_edges(s, d, L0 + nl, seed=3)
+    assert g.exact_count("k4") == pairs
+    assert g.exact_count("triangle") == nh + 2 * pairs + pairs * nl
+
+
+@pytest.mark.parametrize("scratch_words", [None, "20000"])
+def test_device_k4_count_rmat16_in_one_and_in_many_passes(monkeypatch, scratch_words):
+    """R-MAT scale 16 (1 M edges): the 4-clique count equals the oracle's,
+    with the default scratch and with a scratch of 20 000 words, which
+    splits the vertices into many ranges counted one after another."""
+    if scratch_words:
+        monkeypatch.setenv("ZGRAPH_K4_SCRATCH_WORDS", scratch_words)
+    s, d = O.gen_rmat(16, 16 << 16, 0.57, 0.19, 0.19, 1)
+    og = O.OGraph(1 << 16, s, d, seed=1)
+    g = Z.Graph.from_edges(s, d, 1 << 16, seed=1)
+    assert g.exact_count("k4") == og.k4() == 456481728
