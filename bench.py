@@ -456,9 +456,9 @@ def rmat26(Z, args):
     """R-MAT scale 26, edge factor 16 (a=.57, b=c=.19): 1,073,741,824 distinct
     undirected edges with float weights U[0,1), built in HBM.  One step = 2^28
     estimators of each of: 3-motif (3-chains and triangles from one wedge),
-    3-chain, 4-chain, triangles whose edges all have weight >= 0.5 (on the
-    filtered graph) and triangles with at least one such edge (all triangles
-    minus those on the complementary filtered graph)."""
+    3-chain, 4-chain, triangle, 4-clique, triangles whose edges all have
+    weight >= 0.5 (on the filtered graph) and triangles with at least one such
+    edge (all triangles minus those on the complementary filtered graph)."""
     scale, m = 26, 16 << 26
     N = 1 << args.log2_estimators
     t0 = time.perf_counter()
@@ -470,7 +470,8 @@ def rmat26(Z, args):
     log(f"rmat26: n={g.num_vertices} m={g.num_local_edges} maxdeg={g.max_degree} gen={t1 - t0:.1f}s "
         f"csr={t2 - t1:.1f}s hbm={g.device_bytes / 2**30:.1f}GiB")
     runs = [("3-motif", g, Z.Plan(Z.PLAN_THREE_MOTIF)), ("3-chain", g, Z.Plan(Z.PLAN_THREE_CHAIN)),
-            ("4-chain", g, Z.Plan(Z.PLAN_FOUR_CHAIN)), ("triangle", g, Z.Plan(Z.PLAN_TRIANGLE))]
+            ("4-chain", g, Z.Plan(Z.PLAN_FOUR_CHAIN)), ("triangle", g, Z.Plan(Z.PLAN_TRIANGLE)),
+            ("4-clique", g, Z.Plan(Z.PLAN_K4_CLOSE))]
     res, kms = {}, {}
 
     def timed(name, graph, plan):
@@ -483,7 +484,8 @@ def rmat26(Z, args):
         kms[name] = sum(ms) / len(ms)
         reads = Z.count_reads(graph, plan, args.seed, 0, 1 << 22)
         res[name] = {"kernel_ms": kms[name], "estimators_per_s": N / (kms[name] / 1e3),
-                     "estimate": [x.mean for x in sums], "lines_per_estimator": reads / (1 << 22)}
+                     "estimate": [x.mean for x in sums], "lines_per_estimator": reads / (1 << 22),
+                     "half_width_95": [Z.clt_half_width(x.mean, x.sumsq, N, 0.95) for x in sums]}
         log(f"rmat26 {name}: {kms[name]:.2f} ms per 2^{args.log2_estimators}, {reads / (1 << 22):.3f} lines, "
             f"estimate {[f'{x.mean:.5g}' for x in sums]}")
 
@@ -508,12 +510,21 @@ def rmat26(Z, args):
     wedges, tris, paths3 = g.exact_count("wedge"), g.exact_count("triangle"), g.exact_count("path3")
     exact_s = time.perf_counter() - te
     log(f"rmat26 exact: wedges {wedges}, triangles {tris}, 3-edge paths {paths3} in {exact_s:.1f}s")
+    te = time.perf_counter()
+    k4 = g.exact_count("k4")
+    exact_k4_s = time.perf_counter() - te
+    log(f"rmat26 exact: 4-cliques {k4} in {exact_k4_s:.1f}s")
     exact.update({"3-motif": [wedges, tris], "3-chain": [wedges], "4-chain": [paths3], "triangle": [tris],
+                  "4-clique": [k4],
                   "triangle >=1 edge w>=0.5": [tris - exact["triangle w<0.5 (all edges)"][0]]})
     for k, v in res.items():
         v["exact"] = exact[k]
         v["rel_error"] = [abs(e - x) / x if x else None for e, x in zip(v["estimate"], exact[k])]
-        log(f"rmat26 {k}: relative error " + ", ".join(f"{r:.2e}" for r in v["rel_error"] if r is not None))
+        hw = ""
+        if "half_width_95" in v:
+            hw = " (95 % CLT half-width " + ", ".join(f"{h / x:.2e}" for h, x in zip(v["half_width_95"], exact[k])
+                                                      if x) + " of the exact count)"
+        log(f"rmat26 {k}: relative error " + ", ".join(f"{r:.2e}" for r in v["rel_error"] if r is not None) + hw)
     total_ms = sum(kms.values())
     launches = len(kms)
     out = {
@@ -522,12 +533,13 @@ def rmat26(Z, args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
         "data": "synthetic (seeded R-MAT generated in HBM)",
         "config": {"workload": "R-MAT scale 26 edge factor 16 (a=.57 b=.19 c=.19), 3-motif / 3-chain / "
-                               "4-chain / predicate triangles, 2^28 estimators per plan per launch",
+                               "4-chain / triangle / 4-clique / predicate triangles, 2^28 estimators per "
+                               "plan per launch",
                    "vertices": g.num_vertices, "edges": g.num_local_edges, "max_degree": g.max_degree,
                    "estimators_per_launch": N, "parallelism": "single GPU"},
         "per_plan": res, "wall_s_including_filters": wall,
         "graph_build_s": {"generate": t1 - t0, "csr": t2 - t1}, "hbm_gib": g.device_bytes / 2**30,
-        "exact_count_s": exact_s,
+        "exact_count_s": exact_s, "exact_4clique_s": exact_k4_s,
     }
     print(json.dumps(out), flush=True)
 
