@@ -281,11 +281,24 @@ __global__ __launch_bounds__(256) void k_k4_rows(const uint64_t *ooff, const uin
         const uint32_t *const ov = onb + vb;
         const bool staged = R <= (uint64_t)kK4RowLds && ((dv + 63) >> 6) * clog2_u64(d + 1) < R * clog2_u64(dv + 1);
         if (!staged) {
-            for (uint64_t kb = 0; kb < R; ++kb) {
-                const uint64_t j = kb * 64 + lane;
-                const bool hit = j < d && bsearch_u32(ov, dv, L[j]);
-                const uint64_t w = __ballot(hit);
-                if (lane == 0) row[kb] = w;
+            // two positions of L per lane and pass, their searches interleaved
+            for (uint64_t kb = 0; kb < R; kb += 2) {
+                const uint64_t j0 = kb * 64 + lane, j1 = j0 + 64;
+                const uint32_t x0 = j0 < d ? L[j0] : 0, x1 = j1 < d ? L[j1] : 0;
+                uint64_t lo0 = 0, hi0 = j0 < d ? dv : 0, lo1 = 0, hi1 = j1 < d ? dv : 0;
+                while (lo0 < hi0 || lo1 < hi1) {
+                    const uint64_t m0 = (lo0 + hi0) >> 1, m1 = (lo1 + hi1) >> 1;
+                    const uint32_t y0 = lo0 < hi0 ? ov[m0] : 0, y1 = lo1 < hi1 ? ov[m1] : 0;
+                    if (lo0 < hi0) { if (y0 < x0) lo0 = m0 + 1; else hi0 = m0; }
+                    if (lo1 < hi1) { if (y1 < x1) lo1 = m1 + 1; else hi1 = m1; }
+                }
+                const bool h0 = j0 < d && lo0 < dv && ov[lo0] == x0;
+                const bool h1 = j1 < d && lo1 < dv && ov[lo1] == x1;
+                const uint64_t w0 = __ballot(h0), w1 = __ballot(h1);
+                if (lane == 0) {
+                    row[kb] = w0;
+                    if (kb + 1 < R) row[kb + 1] = w1;
+                }
             }
             continue;
         }
@@ -311,7 +324,11 @@ __global__ __launch_bounds__(256) void k_k4_rows(const uint64_t *ooff, const uin
     }
 }
 
-// One thread per row of [e0, e1): the sum over its set bits j of
+// Rows of at most this many words are counted one thread per row, longer
+// ones one wavefront per row.
+constexpr uint64_t kK4NarrowWords = 8;
+
+// One thread per narrow row of [e0, e1): the sum over its set bits j of
 // popcount(row_i & row_j).
 __global__ void k_k4_count(const uint64_t *ooff, const uint32_t *osrc, const uint64_t *boff, uint64_t bbase,
                            uint64_t e0, uint64_t e1, const uint64_t *bits, unsigned long long *partial)
@@ -323,6 +340,7 @@ __global__ void k_k4_count(const uint64_t *ooff, const uint32_t *osrc, const uin
         const uint64_t lb = ooff[u], d = ooff[u + 1] - lb;
         if (d < 3) continue;
         const uint64_t R = (d + 63) >> 6;
+        if (R > kK4NarrowWords) continue;
         const uint64_t *const rb = bits + (boff[u] - bbase);
         const uint64_t *const ri = rb + (e - lb) * R;
         if (R == 1) {
@@ -334,6 +352,64 @@ __global__ void k_k4_count(const uint64_t *ooff, const uint32_t *osrc, const uin
             for (uint64_t wb = ri[kb]; wb; wb &= wb - 1) {
                 const uint64_t *const rj = rb + (kb * 64 + __builtin_ctzll(wb)) * R;
                 for (uint64_t t = 0; t < R; ++t) c += __popcll(ri[t] & rj[t]);
+            }
+        }
+    }
+    const unsigned long long t = block_sum(c);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// One wavefront per wide row of [e0, e1), the lanes sharing its words (held
+// in registers up to 64 words).  The wave takes 64 consecutive rows at a time
+// and walks the wide ones among them, so a hub's rows spread over many waves.
+__global__ __launch_bounds__(256) void k_k4_count_wide(const uint64_t *ooff, const uint32_t *osrc,
+                                                       const uint64_t *boff, uint64_t bbase, uint64_t e0,
+                                                       uint64_t e1, const uint64_t *bits,
+                                                       unsigned long long *partial)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    unsigned long long c = 0;
+    for (uint64_t b = e0 + wave * 64; b < e1; b += nwaves * 64) {
+        uint64_t d = 0;
+        if (b + lane < e1) {
+            const uint32_t u = osrc[b + lane];
+            d = ooff[u + 1] - ooff[u];
+        }
+        for (uint64_t m = __ballot(((d + 63) >> 6) > kK4NarrowWords); m; m &= m - 1) {
+            const uint64_t e = b + __builtin_ctzll(m);
+            const uint32_t u = osrc[e];
+            const uint64_t lb = ooff[u], R = (ooff[u + 1] - lb + 63) >> 6;
+            const uint64_t *const rb = bits + (boff[u] - bbase);
+            const uint64_t *const ri = rb + (e - lb) * R;
+            if (R <= 64) {
+                // four set bits per pass, so four row loads are in flight
+                const uint64_t mine = lane < R ? ri[lane] : 0;
+                for (uint64_t kb = 0; kb < R; ++kb) {
+                    for (uint64_t wb = ri[kb]; wb;) {
+                        uint64_t j[4];
+                        bool ok[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            ok[q] = wb != 0;
+                            j[q] = kb * 64 + (ok[q] ? (uint64_t)__builtin_ctzll(wb) : 0);
+                            wb &= wb - 1;
+                        }
+                        uint64_t x[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) x[q] = lane < R ? rb[(ok[q] ? j[q] : j[0]) * R + lane] : 0;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) c += ok[q] ? __popcll(mine & x[q]) : 0;
+                    }
+                }
+                continue;
+            }
+            for (uint64_t kb = 0; kb < R; ++kb) {
+                for (uint64_t wb = ri[kb]; wb; wb &= wb - 1) {
+                    const uint64_t *const rj = rb + (kb * 64 + __builtin_ctzll(wb)) * R;
+                    for (uint64_t t = lane; t < R; t += 64) c += __popcll(ri[t] & rj[t]);
+                }
             }
         }
     }
@@ -646,8 +722,9 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
     XALLOC(bits, std::min(total_words, budget));
     const int rgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16, std::max<uint64_t>(1, (ne + 3) / 4));
     const int cgrid = gridx(ne);
-    XB<unsigned long long> part;
-    XALLOC(part, cgrid);
+    const int wgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16, std::max<uint64_t>(1, (ne + 255) / 256));
+    XB<unsigned long long> part, wpart;
+    XALLOC(part, cgrid); XALLOC(wpart, wgrid);
     unsigned long long s = 0;
     for (uint32_t u0 = 0; u0 < n;) {
         uint64_t b0 = 0;
@@ -676,10 +753,14 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
             hipLaunchKernelGGL(k_k4_count, dim3(cgrid), dim3(256), 0, st, ooff.p, osrc.p, boff.p, b0, e0, e1, bits.p,
                                part.p);
             ZG_HIP(hipGetLastError());
-            unsigned long long ps = 0;
+            hipLaunchKernelGGL(k_k4_count_wide, dim3(wgrid), dim3(256), 0, st, ooff.p, osrc.p, boff.p, b0, e0, e1,
+                               bits.p, wpart.p);
+            ZG_HIP(hipGetLastError());
+            unsigned long long ps = 0, pw = 0;
             rc = host_sum(part.p, cgrid, st, &ps);
+            if (!rc) rc = host_sum(wpart.p, wgrid, st, &pw);
             if (rc) return rc;
-            s += ps;
+            s += ps + pw;
         }
         u0 = u1;
     }

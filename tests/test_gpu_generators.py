@@ -115,15 +115,17 @@ def test_device_clique_counts_of_complete_graphs(n):
     assert g.exact_count("k4") == comb(n, 4)
 
 
-def test_device_k4_count_with_long_out_lists():
-    """Two vertices u, v share 2 200 common out-neighbours H, so u's bit rows
-    are 35 words long and its rows are written both by ballots and through
-    the LDS copy.  H holds a matching of 100 pairs, and every H vertex also
-    links to all of 2 300 low-degree vertices L, which keeps H above u and v
-    in degree order.
+@pytest.mark.parametrize("nh,nl", [(2200, 2300), (4200, 4300)])
+def test_device_k4_count_with_long_out_lists(nh, nl):
+    """Two vertices u, v share nh common out-neighbours H, so u's bit rows are
+    ceil((nh + 1) / 64) words long (35: the wavefront path with one word per
+    lane; 66: more words than lanes) and its rows are written both by
+    ballots and through the LDS copy.  H holds a matching of 100 pairs, and
+    every H vertex also links to all of nl >= nh low-degree vertices L, which
+    keeps H above u and v in degree order.
     Closed forms: 4-cliques {u, v, h1, h2} per matched pair = 100; triangles
-    {u, v, h} 2 200, {u | v, h1, h2} 200, {h1, h2, l} 100 * 2 300."""
-    nh, nl, pairs = 2200, 2300, 100
+    {u, v, h} nh, {u | v, h1, h2} 200, {h1, h2, l} 100 * nl."""
+    pairs = 100
     u, v = 0, 1
     H0, L0 = 2, 2 + nh
     hs = np.arange(H0, H0 + nh, dtype=np.uint32)
