@@ -9,9 +9,9 @@
 //   5-houses   naive_implementation/FiveHouse.cpp (per ordered edge, k_house)
 // Method: orient every undirected pair from lower to higher (degree, id) and
 // keep the out-lists sorted by neighbour id (taken from the neighbour-ordered
-// copy, parallel edges collapsed).  Triangles: per oriented edge u->v, binary
-// search every out-neighbour of the shorter list in the longer one, the work
-// spread evenly over lanes through a prefix sum of the shorter lengths.  4-cliques:
+// copy, parallel edges collapsed).  Triangles: one wavefront per oriented edge
+// u->v binary-searches the entries of one out-list in the other, whichever
+// takes fewer passes times search depth.  4-cliques:
 // per vertex u, bit rows over out(u) (row i: which of out(u) are in
 // out(out(u)[i])) and the sum of popcount(row_i & row_j) over the set bits j
 // of each row.
@@ -140,6 +140,9 @@ __device__ __forceinline__ bool bsearch_u32(const uint32_t *a, uint64_t n, uint3
     return lo < n && a[lo] == x;
 }
 
+// ceil(log2(x)): the depth of a binary search over x - 1 entries
+__device__ __forceinline__ uint64_t clog2_u64(uint64_t x) { return x <= 1 ? 0 : 64 - __clzll(x - 1); }
+
 __device__ __forceinline__ unsigned long long block_sum(unsigned long long v)
 {
     __shared__ unsigned long long sm[4];
@@ -153,71 +156,46 @@ __device__ __forceinline__ unsigned long long block_sum(unsigned long long v)
     return t;
 }
 
-// Load-balanced triangle count.  Work item j of oriented edge e (P[e] <= j <
-// P[e+1], P the exclusive prefix of the shorter out-list lengths) is element
-// j - P[e] of the edge's shorter out-list, searched in the longer one.  Waves
-// take contiguous chunks of items and lanes interleave inside a chunk, so the
-// items of one edge are read together and every lane does the same amount of
-// work however skewed the degrees are.
-__global__ void k_tri_work(const uint64_t *ooff, const uint32_t *onb, const uint32_t *osrc, uint64_t ne,
-                           uint64_t *x)
+// Entries of a[0, na) found in b[0, nb), counted per lane: each lane takes two
+// entries of a per pass and runs their binary searches interleaved, so two
+// dependent chains of loads are in flight.
+__device__ __forceinline__ unsigned long long count_found(const uint32_t *a, uint64_t na, const uint32_t *b,
+                                                          uint64_t nb, uint32_t lane)
 {
-    GRID_STRIDE(e, ne)
-    {
-        const uint32_t u = osrc[e], v = onb[e];
-        const uint64_t na = ooff[u + 1] - ooff[u], nb = ooff[v + 1] - ooff[v];
-        x[e] = na < nb ? na : nb;
+    unsigned long long c = 0;
+    for (uint64_t base = 0; base < na; base += 128) {
+        const uint64_t i0 = base + lane, i1 = i0 + 64;
+        const uint32_t x0 = i0 < na ? a[i0] : 0, x1 = i1 < na ? a[i1] : 0;
+        uint64_t lo0 = 0, hi0 = i0 < na ? nb : 0, lo1 = 0, hi1 = i1 < na ? nb : 0;
+        while (lo0 < hi0 || lo1 < hi1) {
+            const uint64_t m0 = (lo0 + hi0) >> 1, m1 = (lo1 + hi1) >> 1;
+            const uint32_t y0 = lo0 < hi0 ? b[m0] : 0, y1 = lo1 < hi1 ? b[m1] : 0;
+            if (lo0 < hi0) { if (y0 < x0) lo0 = m0 + 1; else hi0 = m0; }
+            if (lo1 < hi1) { if (y1 < x1) lo1 = m1 + 1; else hi1 = m1; }
+        }
+        c += (i0 < na && lo0 < nb && b[lo0] == x0) ? 1 : 0;
+        c += (i1 < na && lo1 < nb && b[lo1] == x1) ? 1 : 0;
     }
+    return c;
 }
 
-// largest e in [lo, ne) with P[e] <= j, given P[lo] <= j < P[ne]: gallop, then bisect
-__device__ __forceinline__ uint64_t tri_edge_of(const uint64_t *P, uint64_t ne, uint64_t lo, uint64_t j)
-{
-    uint64_t step = 1, hi = lo;
-    while (true) {   // find hi with P[hi] > j
-        const uint64_t t = lo + step;
-        if (t >= ne) { hi = ne; break; }
-        if (P[t] > j) { hi = t; break; }
-        lo = t;
-        step <<= 1;
-    }
-    while (hi - lo > 1) {   // P[lo] <= j < P[hi]
-        const uint64_t mid = lo + (hi - lo) / 2;
-        if (P[mid] <= j) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
-constexpr uint64_t kTriChunk = 64 * 64;   // items per wave task
-
-__global__ __launch_bounds__(256) void k_tri_flat(const uint64_t *ooff, const uint32_t *onb, const uint32_t *osrc,
-                                                  const uint64_t *P, uint64_t ne, uint64_t total,
-                                                  unsigned long long *partial)
+// One wavefront per oriented edge u -> v counts |out(u) & out(v)|, the lanes
+// taking the entries of whichever list costs fewer passes of 128 entries
+// times search depth in the other: a short list is searched in a long one,
+// but 64 entries of a long list searched in a list of 3 cost one pass of
+// depth 2 instead of a pass of depth 7.
+__global__ __launch_bounds__(256) void k_tri_wave(const uint64_t *ooff, const uint32_t *onb, const uint32_t *osrc,
+                                                  uint64_t ne, unsigned long long *partial)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    const uint64_t nchunks = (total + kTriChunk - 1) / kTriChunk;
     unsigned long long c = 0;
-    for (uint64_t ch = wave; ch < nchunks; ch += nwaves) {
-        const uint64_t start = ch * kTriChunk;
-        const uint64_t end = start + kTriChunk < total ? start + kTriChunk : total;
-        uint64_t e = tri_edge_of(P, ne, 0, start);
-        uint64_t pe = 0, pn = 0, nb = 0;
-        const uint32_t *a = nullptr, *b = nullptr;
-        for (uint64_t j = start + lane; j < end; j += 64) {
-            if (a == nullptr || j >= pn) {
-                e = tri_edge_of(P, ne, e, j);
-                pe = P[e];
-                pn = P[e + 1];
-                const uint32_t u = osrc[e], v = onb[e];
-                const uint64_t ou = ooff[u], ov = ooff[v];
-                const uint64_t nu = ooff[u + 1] - ou, nv = ooff[v + 1] - ov;
-                if (nu <= nv) { a = onb + ou; b = onb + ov; nb = nv; }
-                else { a = onb + ov; b = onb + ou; nb = nu; }
-            }
-            c += bsearch_u32(b, nb, a[j - pe]);
-        }
+    for (uint64_t e = wave; e < ne; e += nwaves) {
+        const uint32_t u = osrc[e], v = onb[e];
+        const uint64_t ub = ooff[u], du = ooff[u + 1] - ub, vb = ooff[v], dv = ooff[v + 1] - vb;
+        const bool from_u = ((du + 127) >> 7) * clog2_u64(dv + 1) <= ((dv + 127) >> 7) * clog2_u64(du + 1);
+        c += from_u ? count_found(onb + ub, du, onb + vb, dv, lane) : count_found(onb + vb, dv, onb + ub, du, lane);
     }
     const unsigned long long t = block_sum(c);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
@@ -234,8 +212,6 @@ __global__ __launch_bounds__(256) void k_tri_flat(const uint64_t *ooff, const ui
 // out-list is longer than sqrt(2m) cannot exist, so one vertex needs at most
 // 2^16 rows of 2^10 words.
 constexpr int kK4RowLds = 256;   // words of one row staged in LDS (out-lists up to 16384)
-
-__device__ __forceinline__ uint64_t clog2_u64(uint64_t x) { return x <= 1 ? 0 : 64 - __clzll(x - 1); }
 
 __device__ __forceinline__ uint64_t lower_u32(const uint32_t *a, uint64_t n, uint32_t x)
 {
@@ -662,28 +638,10 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
     hipLaunchKernelGGL(k_out_fill, dim3(gridx(n)), dim3(256), 0, st, g->off, g->sadj, sdeg.p, n, ooff.p, onb.p, osrc.p);
     ZG_HIP(hipGetLastError());
     if (what == 0) {
-        XB<uint64_t> work, pre;
-        XALLOC(work, ne + 1); XALLOC(pre, ne + 1);
-        ZG_HIP(hipMemsetAsync(work.p + ne, 0, 8, st));
-        hipLaunchKernelGGL(k_tri_work, dim3(gridx(ne)), dim3(256), 0, st, ooff.p, onb.p, osrc.p, ne, work.p);
-        ZG_HIP(hipGetLastError());
-        size_t tw = 0;
-        ZG_HIP(rocprim::exclusive_scan(nullptr, tw, work.p, pre.p, (uint64_t)0, (size_t)ne + 1,
-                                       rocprim::plus<uint64_t>(), st));
-        XB<char> tmpw;
-        XALLOC(tmpw, tw);
-        ZG_HIP(rocprim::exclusive_scan(tmpw.p, tw, work.p, pre.p, (uint64_t)0, (size_t)ne + 1,
-                                       rocprim::plus<uint64_t>(), st));
-        uint64_t total = 0;
-        ZG_HIP(hipMemcpyAsync(&total, pre.p + ne, 8, hipMemcpyDeviceToHost, st));
-        ZG_HIP(hipStreamSynchronize(st));
-        if (total == 0) { *out = 0; return ZG_OK; }
-        const uint64_t nchunks = (total + kTriChunk - 1) / kTriChunk;
-        const int tgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 8, (nchunks + 3) / 4);
+        const int tgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16, std::max<uint64_t>(1, (ne + 3) / 4));
         XB<unsigned long long> tpart;
         XALLOC(tpart, tgrid);
-        hipLaunchKernelGGL(k_tri_flat, dim3(tgrid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, pre.p, ne, total,
-                           tpart.p);
+        hipLaunchKernelGGL(k_tri_wave, dim3(tgrid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, ne, tpart.p);
         ZG_HIP(hipGetLastError());
         unsigned long long s = 0;
         int rc = host_sum(tpart.p, tgrid, st, &s);
