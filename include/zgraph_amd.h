@@ -242,7 +242,11 @@ int zg_count_reads(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_
  * triangles (TriangleCounting.cpp:44-72), 1 4-cliques (FourClique.cpp:44-78),
  * 2 wedges (ThreeChain.cpp:44-51), 3 three-edge paths (FourChain.cpp:46-66),
  * 4 5-stars (FiveStar.cpp:51-59), 5 5-houses (FiveHouse.cpp; cost grows as
- * the sum over edges of d_a * d_c: meant for graphs up to a few million edges). */
+ * the sum over edges of d_a * d_c: meant for graphs up to a few million edges),
+ * 6 5-vertex chains (FiveChain.cpp:45-72, simple paths of four edges), 7
+ * 4-cycles (no reference program; the 5-chain count needs them).  Counts
+ * that exceed 64 bits fail with ZG_EOVERFLOW (5-chains) or wrap (4-cycles,
+ * not reached below about 10^18). */
 int zg_exact_counts(zg_graph *g, int what, uint64_t *out);
 /* Degree moments sum d^2 and sum d^3 over all vertices (ELP features). */
 int zg_graph_degree_moments(zg_graph *g, double *sum_d2, double *sum_d3);

@@ -1440,6 +1440,67 @@ uint64_t og_exact_paths3(const og_graph *g, int threads)
     return P - 3 * og_exact_triangles(g, threads);
 }
 
+/* 4-cycles by their definition: every 4-cycle has two diagonals, and a pair
+ * of vertices {x, y} with c common neighbours is the diagonal of C(c, 2)
+ * 4-cycles, so C4 = (1/2) sum over pairs x < y of C(c_xy, 2).  For each x the
+ * common-neighbour counts of all y > x come from walking x's 2-hop paths into
+ * a dense counter: O(sum of d^2). */
+uint64_t og_exact_c4(const og_graph *g)
+{
+    og_simple s; simple_build(g, &s);
+    uint32_t *cnt = (uint32_t *)calloc((size_t)s.n + 1, 4);
+    uint32_t *seen = (uint32_t *)malloc(4 * ((size_t)s.n + 1));
+    uint64_t pairs = 0;
+    for (uint32_t x = 0; x < s.n; ++x) {
+        uint64_t ns = 0;
+        for (uint64_t i = s.off[x]; i < s.off[x + 1]; ++i) {
+            uint32_t v = s.nb[i];
+            for (uint64_t j = s.off[v]; j < s.off[v + 1]; ++j) {
+                uint32_t y = s.nb[j];
+                if (y <= x) continue;
+                if (cnt[y]++ == 0) seen[ns++] = y;
+            }
+        }
+        for (uint64_t k = 0; k < ns; ++k) {
+            uint64_t c = cnt[seen[k]];
+            pairs += c * (c - 1) / 2;
+            cnt[seen[k]] = 0;
+        }
+    }
+    free(cnt); free(seen); simple_free(&s);
+    return pairs / 2;
+}
+
+/* 5-vertex chains (simple paths of four edges, FiveChain.cpp's count) by a
+ * closed form over the path's middle vertex v and its neighbours x, y on the
+ * path: the ordered choices (x, v, y) with an end beyond x and one beyond y
+ * number (d_x - 1 - a)(d_y - 1 - a) - (c_xy - 1), a = [x ~ y], c_xy the
+ * common neighbours of x and y.  Summed over ordered wedges:
+ *   2P = sum_v [(sum_{x~v} (d_x - 1))^2 - sum_{x~v} (d_x - 1)^2]
+ *        - 4 sum_v (d_v - 1) t_v + 6 T - 8 C4,
+ * t_v the triangles at v.  Pinned against og_exact_paths_brute(5). */
+uint64_t og_exact_paths4(const og_graph *g)
+{
+    og_simple s; simple_build(g, &s);
+    __int128 two_p = 0, T = 0, TW = 0;
+    for (uint32_t v = 0; v < s.n; ++v) {
+        unsigned __int128 s1 = 0, s2 = 0;
+        uint64_t tv = 0;
+        for (uint64_t i = s.off[v]; i < s.off[v + 1]; ++i) {
+            uint32_t x = s.nb[i];
+            uint64_t a = sdeg(&s, x) - 1;
+            s1 += a; s2 += (unsigned __int128)a * a;
+            for (uint64_t j = i + 1; j < s.off[v + 1]; ++j) tv += s_has(&s, x, s.nb[j]);
+        }
+        two_p += (__int128)(s1 * s1 - s2);
+        T += tv;
+        TW += (__int128)(sdeg(&s, v) - (sdeg(&s, v) > 0)) * tv;
+    }
+    simple_free(&s);
+    two_p += -4 * TW + 6 * (T / 3) - 8 * (__int128)og_exact_c4(g);
+    return (uint64_t)(two_p / 2);
+}
+
 /* brute force: injective maps of pattern vertices with every pattern edge
  * present, divided by the automorphism count */
 static uint64_t bf_rec(const og_simple *s, int k, int e, const uint32_t *ps, const uint32_t *pd, int idx, uint32_t *map)

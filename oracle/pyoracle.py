@@ -102,6 +102,9 @@ def lib():
             getattr(L, nm).argtypes = [P(Graph), C.c_int]
         L.og_exact_wedges.restype = C.c_uint64
         L.og_exact_wedges.argtypes = [P(Graph)]
+        for nm in ("og_exact_c4", "og_exact_paths4"):
+            getattr(L, nm).restype = C.c_uint64
+            getattr(L, nm).argtypes = [P(Graph)]
         L.og_exact_paths_brute.restype = C.c_uint64
         L.og_exact_paths_brute.argtypes = [P(Graph), C.c_int]
         L.og_set_rng_mode.restype = None
@@ -254,6 +257,12 @@ class OGraph:
     def paths3(self, threads=0):
         return lib().og_exact_paths3(C.byref(self.g), threads)
 
+    def c4(self):
+        return lib().og_exact_c4(C.byref(self.g))
+
+    def paths4(self):
+        return lib().og_exact_paths4(C.byref(self.g))
+
     def paths_brute(self, nverts):
         return lib().og_exact_paths_brute(C.byref(self.g), nverts)
 
@@ -332,5 +341,6 @@ TRIANGLE = (3, [(0, 1), (0, 2), (1, 2)])                          # Triangle2.cp
 K4 = (4, [(1, 0), (2, 0), (2, 1), (3, 0), (3, 1), (3, 2)])          # FourClique2 / TestPattern2
 FIVE_STAR = (5, [(0, 1), (0, 2), (0, 3), (0, 4)])                  # FiveStar.cpp:18-26
 FIVE_HOUSE = (5, [(0, 1), (0, 2), (1, 2), (1, 3), (2, 4), (3, 4)])  # FiveHouse.cpp:18-27
+C4 = (4, [(0, 1), (1, 2), (2, 3), (3, 0)])                         # 4-cycle
 FOUR_CHAIN = (4, [(0, 1), (1, 2), (2, 3)])                         # TestPattern2.cpp:101-108
 FIVE_CHAIN = (5, [(0, 1), (1, 2), (2, 3), (3, 4)])                 # TestPattern.cpp:92-99

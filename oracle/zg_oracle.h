@@ -181,6 +181,8 @@ uint64_t og_exact_triangles(const og_graph *g, int threads);
 uint64_t og_exact_k4(const og_graph *g, int threads);
 uint64_t og_exact_wedges(const og_graph *g);              /* ThreeChain.cpp:44-51 */
 uint64_t og_exact_paths3(const og_graph *g, int threads);  /* 3-edge simple paths */
+uint64_t og_exact_c4(const og_graph *g);                  /* 4-cycles, by pair counting */
+uint64_t og_exact_paths4(const og_graph *g);              /* 4-edge simple paths (FiveChain.cpp), closed form */
 og_u128 og_exact_stars(const og_graph *g, int leaves);     /* FiveStar.cpp:46-60 for leaves=4 */
 void og_exact_stars64(const og_graph *g, int leaves, uint64_t *lo, uint64_t *hi);
 uint64_t og_exact_paths_brute(const og_graph *g, int nverts); /* simple paths with nverts vertices */

@@ -158,9 +158,13 @@ __device__ __forceinline__ unsigned long long block_sum(unsigned long long v)
 
 // Entries of a[0, na) found in b[0, nb), counted per lane: each lane takes two
 // entries of a per pass and runs their binary searches interleaved, so two
-// dependent chains of loads are in flight.
+// dependent chains of loads are in flight.  With WEIGHT, *wsum also gets
+// sdeg[x] - 1 for every entry x found.
+template <bool WEIGHT = false>
 __device__ __forceinline__ unsigned long long count_found(const uint32_t *a, uint64_t na, const uint32_t *b,
-                                                          uint64_t nb, uint32_t lane)
+                                                          uint64_t nb, uint32_t lane,
+                                                          const uint32_t *sdeg = nullptr,
+                                                          unsigned long long *wsum = nullptr)
 {
     unsigned long long c = 0;
     for (uint64_t base = 0; base < na; base += 128) {
@@ -173,8 +177,13 @@ __device__ __forceinline__ unsigned long long count_found(const uint32_t *a, uin
             if (lo0 < hi0) { if (y0 < x0) lo0 = m0 + 1; else hi0 = m0; }
             if (lo1 < hi1) { if (y1 < x1) lo1 = m1 + 1; else hi1 = m1; }
         }
-        c += (i0 < na && lo0 < nb && b[lo0] == x0) ? 1 : 0;
-        c += (i1 < na && lo1 < nb && b[lo1] == x1) ? 1 : 0;
+        const bool f0 = i0 < na && lo0 < nb && b[lo0] == x0;
+        const bool f1 = i1 < na && lo1 < nb && b[lo1] == x1;
+        c += (f0 ? 1 : 0) + (f1 ? 1 : 0);
+        if (WEIGHT) {
+            if (f0) *wsum += sdeg[x0] - 1;
+            if (f1) *wsum += sdeg[x1] - 1;
+        }
     }
     return c;
 }
@@ -184,21 +193,35 @@ __device__ __forceinline__ unsigned long long count_found(const uint32_t *a, uin
 // times search depth in the other: a short list is searched in a long one,
 // but 64 entries of a long list searched in a list of 3 cost one pass of
 // depth 2 instead of a pass of depth 7.
+// With WEIGHT the kernel also sums, over the triangles it finds, the
+// simple degrees minus one of their three vertices (sum_v (d_v - 1) t_v, with
+// t_v the triangles at v) into wpartial.
+template <bool WEIGHT>
 __global__ __launch_bounds__(256) void k_tri_wave(const uint64_t *ooff, const uint32_t *onb, const uint32_t *osrc,
-                                                  uint64_t ne, unsigned long long *partial)
+                                                  uint64_t ne, const uint32_t *sdeg, unsigned long long *partial,
+                                                  unsigned long long *wpartial)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    unsigned long long c = 0;
+    unsigned long long c = 0, wsum = 0;
     for (uint64_t e = wave; e < ne; e += nwaves) {
         const uint32_t u = osrc[e], v = onb[e];
         const uint64_t ub = ooff[u], du = ooff[u + 1] - ub, vb = ooff[v], dv = ooff[v + 1] - vb;
         const bool from_u = ((du + 127) >> 7) * clog2_u64(dv + 1) <= ((dv + 127) >> 7) * clog2_u64(du + 1);
-        c += from_u ? count_found(onb + ub, du, onb + vb, dv, lane) : count_found(onb + vb, dv, onb + ub, du, lane);
+        const unsigned long long f =
+            from_u ? count_found<WEIGHT>(onb + ub, du, onb + vb, dv, lane, sdeg, &wsum)
+                   : count_found<WEIGHT>(onb + vb, dv, onb + ub, du, lane, sdeg, &wsum);
+        c += f;
+        if (WEIGHT) wsum += f * ((unsigned long long)sdeg[u] + sdeg[v] - 2);
     }
     const unsigned long long t = block_sum(c);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    if (WEIGHT) {
+        __syncthreads();
+        const unsigned long long tw = block_sum(wsum);
+        if (threadIdx.x == 0) wpartial[blockIdx.x] = tw;
+    }
 }
 
 // 4-cliques by bit rows: the binary-encoded induced subgraphs of Almasri et
@@ -524,6 +547,219 @@ __global__ void k_deg_moments(const uint64_t *off, uint32_t n, double *p2, doubl
     }
 }
 
+// ---- 4-cycles and 5-vertex chains -------------------------------------------
+// The full simple adjacency with every entry as an order key
+// (simple degree << 32 | id) and each list sorted by key, so that the
+// neighbours ranked below a vertex form a prefix of its list.
+__global__ void k_keyify(const uint32_t *fnb, const uint32_t *sdeg, uint64_t ne, uint64_t *key)
+{
+    GRID_STRIDE(i, ne)
+    {
+        const uint32_t x = fnb[i];
+        key[i] = ((uint64_t)sdeg[x] << 32) | x;
+    }
+}
+
+__device__ __forceinline__ uint64_t lower_u64(const uint64_t *a, uint64_t n, uint64_t x)
+{
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// 4-cycles at their highest vertex u (the degree ordering of Chiba and
+// Nishizeki, "Arboricity and subgraph listing algorithms", SIAM J. Comput.
+// 14(1), 1985): for every neighbour v below u and every neighbour w of v
+// below u, cnt[w] += 1; the cycles whose top is u and whose vertex opposite u
+// is w number C(cnt[w], 2).  The work is at most the sum over edges of the
+// smaller degree.  cnt lives in a per-wavefront LDS hash; a vertex that fills
+// it past three quarters goes to a list for k_c4_global.
+constexpr int kC4Bits = 10;
+constexpr uint32_t kC4Slots = 1u << kC4Bits;
+constexpr uint32_t kC4Empty = 0xffffffffu;
+
+__device__ __forceinline__ uint32_t c4_hash(uint32_t w, uint32_t bits) { return (w * 0x9E3779B1u) >> (32 - bits); }
+
+__global__ __launch_bounds__(256) void k_c4_lds(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg,
+                                                uint32_t n, unsigned long long *partial, uint32_t *ovf_list,
+                                                unsigned int *ovf_count)
+{
+    __shared__ uint32_t keys[4 * kC4Slots], vals[4 * kC4Slots];
+    __shared__ uint32_t fill[4], full[4];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t *const K = keys + wv * kC4Slots;
+    uint32_t *const V = vals + wv * kC4Slots;
+    for (uint32_t t = lane; t < kC4Slots; t += 64) { K[t] = kC4Empty; V[t] = 0; }
+    if (lane == 0) { fill[wv] = 0; full[wv] = 0; }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    unsigned long long c = 0;
+    WAVE_STRIDE(u, (uint64_t)n)
+    {
+        const uint64_t ub = foff[u], ku = ((uint64_t)sdeg[u] << 32) | u;
+        const uint64_t pu = lower_u64(fkey + ub, foff[u + 1] - ub, ku);   // neighbours below u
+        if (pu < 2) continue;
+        for (uint64_t jb = 0; jb < pu; jb += 64) {
+            if (jb + lane < pu) {
+                const uint32_t v = (uint32_t)fkey[ub + jb + lane];
+                const uint64_t vb = foff[v];
+                const uint64_t qv = lower_u64(fkey + vb, foff[v + 1] - vb, ku);   // neighbours of v below u
+                for (uint64_t k = 0; k < qv; ++k) {
+                    if (*(volatile uint32_t *)&full[wv]) break;
+                    const uint32_t w = (uint32_t)fkey[vb + k];
+                    uint32_t h = c4_hash(w, kC4Bits);
+                    bool done = false;
+                    for (uint32_t probe = 0; probe < kC4Slots && !done; ++probe) {
+                        const uint32_t old = atomicCAS(&K[h], kC4Empty, w);
+                        if (old == kC4Empty || old == w) {
+                            atomicAdd(&V[h], 1u);
+                            if (old == kC4Empty && atomicAdd(&fill[wv], 1u) + 1 >= kC4Slots * 3 / 4)
+                                atomicOr(&full[wv], 1u);
+                            done = true;
+                        } else {
+                            h = (h + 1) & (kC4Slots - 1);
+                        }
+                    }
+                    if (!done) atomicOr(&full[wv], 1u);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            if (full[wv]) break;
+        }
+        if (!full[wv]) {
+            for (uint32_t t = lane; t < kC4Slots; t += 64) {
+                const unsigned long long x = V[t];
+                c += x * (x - (x > 0)) / 2;
+            }
+        } else if (lane == 0) {
+            ovf_list[atomicAdd(ovf_count, 1u)] = (uint32_t)u;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (uint32_t t = lane; t < kC4Slots; t += 64) { K[t] = kC4Empty; V[t] = 0; }
+        if (lane == 0) { fill[wv] = 0; full[wv] = 0; }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    const unsigned long long t = block_sum(c);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// upper bound on the distinct w of each listed vertex: the summed degrees of
+// its neighbours below it (read from their keys), capped at n
+__global__ void k_c4_bound(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg, uint32_t n,
+                           const uint32_t *list, uint32_t nlist, uint64_t *bound)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    WAVE_STRIDE(i, (uint64_t)nlist)
+    {
+        const uint32_t u = list[i];
+        const uint64_t ub = foff[u], ku = ((uint64_t)sdeg[u] << 32) | u;
+        const uint64_t pu = lower_u64(fkey + ub, foff[u + 1] - ub, ku);
+        unsigned long long b = 0;
+        for (uint64_t j = lane; j < pu; j += 64) b += fkey[ub + j] >> 32;
+        for (int off = 32; off > 0; off >>= 1) b += __shfl_xor(b, off, 64);
+        if (lane == 0) bound[i] = b < n ? b : n;
+    }
+}
+
+// The listed vertices, one workgroup each, with cnt in a global hash: each
+// workgroup owns 2^max_bits slots and uses the smallest power of two at
+// least twice the vertex's bound (so every insert finds a slot, and only
+// that many slots are cleared and summed).
+__global__ __launch_bounds__(256) void k_c4_global(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg,
+                                                   const uint32_t *list, const uint64_t *bound, uint32_t nlist,
+                                                   uint32_t *tab, uint32_t max_bits, unsigned long long *partial)
+{
+    uint32_t *const K = tab + (uint64_t)blockIdx.x * (2ull << max_bits);
+    unsigned long long c = 0;
+    for (uint32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
+        const uint32_t u = list[i];
+        uint32_t bits = 1;
+        while (bits < max_bits && (1ull << bits) < 2 * bound[i] + 2) ++bits;
+        const uint64_t size = 1ull << bits;
+        uint32_t *const V = K + size;
+        for (uint64_t t = threadIdx.x; t < size; t += blockDim.x) { K[t] = kC4Empty; V[t] = 0; }
+        __syncthreads();
+        const uint64_t ub = foff[u], ku = ((uint64_t)sdeg[u] << 32) | u;
+        const uint64_t pu = lower_u64(fkey + ub, foff[u + 1] - ub, ku);
+        for (uint64_t j = threadIdx.x; j < pu; j += blockDim.x) {
+            const uint32_t v = (uint32_t)fkey[ub + j];
+            const uint64_t vb = foff[v];
+            const uint64_t qv = lower_u64(fkey + vb, foff[v + 1] - vb, ku);
+            for (uint64_t k = 0; k < qv; ++k) {
+                const uint32_t w = (uint32_t)fkey[vb + k];
+                uint64_t h = c4_hash(w, bits);
+                for (uint64_t probe = 0; probe < size; ++probe) {
+                    const uint32_t old = atomicCAS(&K[h], kC4Empty, w);
+                    if (old == kC4Empty || old == w) { atomicAdd(&V[h], 1u); break; }
+                    h = (h + 1) & (size - 1);
+                }
+            }
+        }
+        __syncthreads();
+        for (uint64_t t = threadIdx.x; t < size; t += blockDim.x) {
+            const unsigned long long x = V[t];
+            c += x * (x - (x > 0)) / 2;
+        }
+        __syncthreads();
+    }
+    const unsigned long long t = block_sum(c);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// sum over vertices v of S1^2 - S2, S1 = sum of (d_x - 1) and S2 = sum of
+// (d_x - 1)^2 over the neighbours x of v (degrees read from the keys): the
+// ordered pairs of distinct neighbours of v weighted by (d_x - 1)(d_y - 1).
+// One wavefront per vertex; per-wave 128-bit partials as (lo, hi) pairs.
+__device__ __forceinline__ void add128(unsigned long long &lo, unsigned long long &hi, unsigned long long blo,
+                                       unsigned long long bhi)
+{
+    const unsigned long long l = lo + blo;
+    hi += bhi + (l < lo ? 1 : 0);
+    lo = l;
+}
+
+__global__ __launch_bounds__(256) void k_c5_pairs(const uint64_t *foff, const uint64_t *fkey, uint32_t n,
+                                                  unsigned long long *partial)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    unsigned long long lo = 0, hi = 0;
+    WAVE_STRIDE(v, (uint64_t)n)
+    {
+        const uint64_t vb = foff[v], dv = foff[v + 1] - vb;
+        unsigned long long s1 = 0, s2lo = 0, s2hi = 0;
+        for (uint64_t j = lane; j < dv; j += 64) {
+            const unsigned long long a = (fkey[vb + j] >> 32) - 1;
+            s1 += a;
+            const unsigned __int128 q = (unsigned __int128)a * a;
+            add128(s2lo, s2hi, (unsigned long long)q, (unsigned long long)(q >> 64));
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            s1 += __shfl_xor(s1, off, 64);
+            const unsigned long long ol = __shfl_xor(s2lo, off, 64), oh = __shfl_xor(s2hi, off, 64);
+            add128(s2lo, s2hi, ol, oh);
+        }
+        if (lane == 0) {
+            const unsigned __int128 w = (unsigned __int128)s1 * s1 - (((unsigned __int128)s2hi << 64) | s2lo);
+            add128(lo, hi, (unsigned long long)w, (unsigned long long)(w >> 64));
+        }
+    }
+    if (lane == 0) {
+        const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+        partial[2 * wave] = lo;
+        partial[2 * wave + 1] = hi;
+    }
+}
+
 int host_sum(const unsigned long long *d, int n, hipStream_t st, unsigned long long *out)
 {
     std::vector<unsigned long long> h(n);
@@ -543,9 +779,9 @@ using namespace zg;
 extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
 {
     if (!g || !out) return set_error(ZG_EINVAL, "null argument");
-    if (what < 0 || what > 5)
+    if (what < 0 || what > 7)
         return set_error(ZG_EINVAL, "what must be 0 (triangles), 1 (4-cliques), 2 (wedges), 3 (3-edge paths), "
-                                    "4 (5-stars), 5 (5-houses)");
+                                    "4 (5-stars), 5 (5-houses), 6 (5-vertex chains), 7 (4-cycles)");
     const uint32_t n = g->n;
     hipStream_t st = g->stream;
     if (g->m == 0) { *out = 0; return ZG_OK; }
@@ -594,7 +830,7 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         *out = s / 2;
         return ZG_OK;
     }
-    if (what >= 2) {
+    if (what == 2 || what == 3) {
         const int grid = gridx(n);
         XB<unsigned long long> pw, pp;
         XALLOC(pw, grid); XALLOC(pp, grid);
@@ -641,12 +877,126 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         const int tgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16, std::max<uint64_t>(1, (ne + 3) / 4));
         XB<unsigned long long> tpart;
         XALLOC(tpart, tgrid);
-        hipLaunchKernelGGL(k_tri_wave, dim3(tgrid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, ne, tpart.p);
+        hipLaunchKernelGGL(k_tri_wave<false>, dim3(tgrid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, ne, sdeg.p,
+                           tpart.p, nullptr);
         ZG_HIP(hipGetLastError());
         unsigned long long s = 0;
         int rc = host_sum(tpart.p, tgrid, st, &s);
         if (rc) return rc;
         *out = s;
+        return ZG_OK;
+    }
+    if (what == 6 || what == 7) {
+        // full simple adjacency as order keys, every list sorted by key
+        XB<uint64_t> fdeg, foff;
+        XALLOC(fdeg, (uint64_t)n + 1); XALLOC(foff, (uint64_t)n + 1);
+        hipLaunchKernelGGL(k_widen, dim3(gridx(n)), dim3(256), 0, st, sdeg.p, n, fdeg.p);
+        ZG_HIP(hipMemsetAsync(fdeg.p + n, 0, 8, st));
+        size_t tf = 0;
+        ZG_HIP(rocprim::exclusive_scan(nullptr, tf, fdeg.p, foff.p, (uint64_t)0, (size_t)n + 1,
+                                       rocprim::plus<uint64_t>(), st));
+        XB<char> tmpf;
+        XALLOC(tmpf, tf);
+        ZG_HIP(rocprim::exclusive_scan(tmpf.p, tf, fdeg.p, foff.p, (uint64_t)0, (size_t)n + 1,
+                                       rocprim::plus<uint64_t>(), st));
+        uint64_t nf = 0;
+        ZG_HIP(hipMemcpyAsync(&nf, foff.p + n, 8, hipMemcpyDeviceToHost, st));
+        ZG_HIP(hipStreamSynchronize(st));
+        if (nf >= (1ull << 32)) return set_error(ZG_EINVAL, "4-cycle and 5-chain counts need 2m < 2^32");
+        XB<uint64_t> fkey;
+        XALLOC(fkey, nf);
+        {
+            XB<uint32_t> fnb, fsrc;
+            XB<uint64_t> kin;
+            XALLOC(fnb, nf); XALLOC(fsrc, nf); XALLOC(kin, nf);
+            hipLaunchKernelGGL(k_full_fill, dim3(gridx(n)), dim3(256), 0, st, g->off, g->sadj, n, foff.p, fnb.p,
+                               fsrc.p);
+            ZG_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_keyify, dim3(gridx(nf)), dim3(256), 0, st, fnb.p, sdeg.p, nf, kin.p);
+            ZG_HIP(hipGetLastError());
+            size_t ts = 0;
+            ZG_HIP(rocprim::segmented_radix_sort_keys(nullptr, ts, kin.p, fkey.p, (unsigned int)nf, n, foff.p,
+                                                      foff.p + 1, 0, 64, st));
+            XB<char> tmps;
+            XALLOC(tmps, ts);
+            ZG_HIP(rocprim::segmented_radix_sort_keys(tmps.p, ts, kin.p, fkey.p, (unsigned int)nf, n, foff.p,
+                                                      foff.p + 1, 0, 64, st));
+            ZG_HIP(hipStreamSynchronize(st));   // before the temporaries are freed
+        }
+        // 4-cycles: LDS hashes, then the vertices that overflowed them with global hashes
+        const int cgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 8, std::max<uint64_t>(1, ((uint64_t)n + 3) / 4));
+        XB<unsigned long long> cpart;
+        XB<uint32_t> ovf;
+        XB<unsigned int> novf_d;
+        XALLOC(cpart, cgrid); XALLOC(ovf, n); XALLOC(novf_d, 1);
+        ZG_HIP(hipMemsetAsync(novf_d.p, 0, sizeof(unsigned int), st));
+        hipLaunchKernelGGL(k_c4_lds, dim3(cgrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p, n, cpart.p, ovf.p,
+                           novf_d.p);
+        ZG_HIP(hipGetLastError());
+        unsigned long long c4 = 0;
+        int rc = host_sum(cpart.p, cgrid, st, &c4);
+        if (rc) return rc;
+        unsigned int novf = 0;
+        ZG_HIP(hipMemcpyAsync(&novf, novf_d.p, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        ZG_HIP(hipStreamSynchronize(st));
+        if (novf) {
+            XB<uint64_t> bnd;
+            XALLOC(bnd, novf);
+            hipLaunchKernelGGL(k_c4_bound, dim3(gridx((uint64_t)novf * 64)), dim3(256), 0, st, foff.p, fkey.p, sdeg.p,
+                               n, ovf.p, novf, bnd.p);
+            ZG_HIP(hipGetLastError());
+            std::vector<uint64_t> hb(novf);
+            ZG_HIP(hipMemcpyAsync(hb.data(), bnd.p, 8ull * novf, hipMemcpyDeviceToHost, st));
+            ZG_HIP(hipStreamSynchronize(st));
+            uint64_t mb = 0;
+            for (uint64_t b : hb) mb = std::max(mb, b);
+            uint32_t bits = 1;
+            while ((1ull << bits) < 2 * mb + 2) ++bits;
+            if (bits > 31) return set_error(ZG_EINVAL, "4-cycle count: too many vertices for the counter's hash");
+            size_t free_b = 0, total_b = 0;
+            ZG_HIP(hipMemGetInfo(&free_b, &total_b));
+            const uint64_t tbytes = 8ull << bits;   // keys and counts, u32 each
+            const uint64_t budget = std::min<uint64_t>(16ull << 30, std::max<uint64_t>(1ull << 30, free_b / 4));
+            const int ggrid = (int)std::max<uint64_t>(
+                1, std::min<uint64_t>(std::min<uint64_t>(novf, budget / tbytes), (uint64_t)device_cus() * 4));
+            XB<uint32_t> tab;
+            XB<unsigned long long> gpart;
+            XALLOC(tab, ((uint64_t)ggrid * 2) << bits); XALLOC(gpart, ggrid);
+            hipLaunchKernelGGL(k_c4_global, dim3(ggrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p, ovf.p, bnd.p, novf,
+                               tab.p, bits, gpart.p);
+            ZG_HIP(hipGetLastError());
+            unsigned long long g4 = 0;
+            rc = host_sum(gpart.p, ggrid, st, &g4);
+            if (rc) return rc;
+            c4 += g4;
+        }
+        if (what == 7) { *out = c4; return ZG_OK; }
+        // 5-vertex chains (FiveChain.cpp semantics): 2P = sum_v (S1^2 - S2)
+        // - 4 sum_v (d_v - 1) t_v + 6 T - 8 C4
+        const int tgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16, std::max<uint64_t>(1, (ne + 3) / 4));
+        XB<unsigned long long> tpart, wpart;
+        XALLOC(tpart, tgrid); XALLOC(wpart, tgrid);
+        hipLaunchKernelGGL(k_tri_wave<true>, dim3(tgrid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, ne, sdeg.p,
+                           tpart.p, wpart.p);
+        ZG_HIP(hipGetLastError());
+        unsigned long long T = 0, TW = 0;
+        rc = host_sum(tpart.p, tgrid, st, &T);
+        if (!rc) rc = host_sum(wpart.p, tgrid, st, &TW);
+        if (rc) return rc;
+        const int pgrid = gridx((uint64_t)n * 64);
+        XB<unsigned long long> ppart;
+        XALLOC(ppart, (uint64_t)pgrid * 8);   // (lo, hi) per wave, 4 waves per block
+        hipLaunchKernelGGL(k_c5_pairs, dim3(pgrid), dim3(256), 0, st, foff.p, fkey.p, n, ppart.p);
+        ZG_HIP(hipGetLastError());
+        std::vector<unsigned long long> hp((size_t)pgrid * 8);
+        ZG_HIP(hipMemcpyAsync(hp.data(), ppart.p, 8ull * hp.size(), hipMemcpyDeviceToHost, st));
+        ZG_HIP(hipStreamSynchronize(st));
+        unsigned __int128 W = 0;
+        for (size_t i = 0; i < hp.size(); i += 2) W += ((unsigned __int128)hp[i + 1] << 64) | hp[i];
+        const __int128 two_p = (__int128)W - 4 * (__int128)TW + 6 * (__int128)T - 8 * (__int128)c4;
+        if (two_p < 0 || (two_p >> 65) != 0)
+            return set_error(ZG_EOVERFLOW, "5-vertex chain count exceeds 64 bits");
+        *out = (uint64_t)(two_p / 2);
         return ZG_OK;
     }
     // 4-cliques: bit rows in vertex ranges whose rows fit the scratch budget

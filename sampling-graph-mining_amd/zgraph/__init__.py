@@ -176,8 +176,9 @@ class Graph:
         return Graph(h)
 
     def exact_count(self, what):
-        """Exact non-induced count: 'triangle', 'k4', 'wedge', 'path3', 'star5' or 'house'."""
-        code = {"triangle": 0, "k4": 1, "wedge": 2, "path3": 3, "star5": 4, "house": 5}[what]
+        """Exact non-induced count: 'triangle', 'k4', 'wedge', 'path3', 'star5', 'house',
+        'path4' (5-vertex chains) or 'c4' (4-cycles)."""
+        code = {"triangle": 0, "k4": 1, "wedge": 2, "path3": 3, "star5": 4, "house": 5, "path4": 6, "c4": 7}[what]
         r = C.c_uint64()
         check(lib().zg_exact_counts(self.h, code, C.byref(r)))
         return r.value

@@ -64,6 +64,10 @@ def test_device_exact_counts_equal_oracle(gname):
     assert g.exact_count("star5") == og.stars(4)
     if n <= 400:
         assert g.exact_count("house") == og.house_brute()
+    assert g.exact_count("c4") == og.c4()
+    assert g.exact_count("path4") == og.paths4()
+    if n <= 400:
+        assert g.exact_count("path4") == og.paths_brute(5)
     if gname == "snap-wiki-Vote":
         assert g.exact_count("triangle") == 608389
 
@@ -150,3 +154,26 @@ def test_device_k4_count_rmat16_in_one_and_in_many_passes(monkeypatch, scratch_w
     og = O.OGraph(1 << 16, s, d, seed=1)
     g = Z.Graph.from_edges(s, d, 1 << 16, seed=1)
     assert g.exact_count("k4") == og.k4() == 456481728
+
+
+@pytest.mark.parametrize("n", [9, 70, 1000])
+def test_device_c4_and_five_chains_of_complete_graphs(n):
+    """K_n: 3 C(n, 4) 4-cycles and n!/(n-5)!/2 5-vertex chains.  In K_1000
+    the top vertices see up to 998 distinct opposite vertices, more than the
+    LDS hash holds, so they are counted again with global hashes."""
+    from math import comb, perm
+    import helpers as H
+    n, s, d = H.complete_graph(n)
+    g = Z.Graph.from_edges(s, d, n, seed=2)
+    assert g.exact_count("c4") == 3 * comb(n, 4)
+    assert g.exact_count("path4") == perm(n, 5) // 2
+
+
+def test_device_c4_and_five_chains_rmat14():
+    """R-MAT scale 14 (262 k edges, hubs of a few thousand): device counts
+    equal the oracle's pair-counting 4-cycles and closed-form chains."""
+    s, d = O.gen_rmat(14, 16 << 14, 0.57, 0.19, 0.19, 3)
+    og = O.OGraph(1 << 14, s, d, seed=1)
+    g = Z.Graph.from_edges(s, d, 1 << 14, seed=1)
+    assert g.exact_count("c4") == og.c4()
+    assert g.exact_count("path4") == og.paths4()

@@ -313,6 +313,30 @@ def test_exact_counters_agree_with_brute_force(gi):
         assert g.house_brute() == math.comb(7, 5) * 60   # every 5 vertices hold 5!/2 houses
 
 
+@pytest.mark.parametrize("gi", range(6))
+def test_c4_and_five_chain_counts_agree_with_brute_force(gi):
+    """4-cycles by pair counting equal the brute-force count of the 4-cycle
+    pattern, and the closed form for 5-vertex chains equals the enumeration of
+    simple 4-edge paths (FiveChain.cpp's count), on graphs with triangles,
+    hubs and parallel edges."""
+    n, s, d = [H.random_graph(12, 0.5, 10), H.random_graph(14, 0.35, 11), H.complete_graph(7),
+               H.random_graph(30, 0.2, 12), H.multigraph(20, 0.4, 13), H.hub_graph(40, 3, 14)][gi]
+    g = O.OGraph(n, s, d)
+    assert g.c4() == g.pattern_brute(*O.C4)
+    assert g.paths4() == g.paths_brute(5)
+    if gi == 2:   # K7: 3 C(7, 4) 4-cycles, 7!/2!/2 chains of 5 vertices
+        assert g.c4() == 3 * math.comb(7, 4) and g.paths4() == math.perm(7, 5) // 2
+
+
+def test_five_chain_closed_form_misses_a_dropped_term():
+    """Each term of the closed form matters: on G(30, 0.2) leaving out the
+    4-cycle term, or the triangle terms, changes the count."""
+    n, s, d = H.random_graph(30, 0.2, 12)
+    g = O.OGraph(n, s, d)
+    assert g.c4() > 0 and g.triangles() > 0
+    assert g.paths4() + 4 * g.c4() != g.paths_brute(5)
+
+
 NAIVE = {"TriangleCounting": lambda g: g.triangles(), "FourClique": lambda g: g.k4(),
          "ThreeChain": lambda g: g.wedges(), "FourChain": lambda g: g.paths3(),
          "FiveChain": lambda g: g.paths_brute(5), "FiveStar": lambda g: g.pattern_brute(*O.FIVE_STAR),
