@@ -177,3 +177,19 @@ def test_device_c4_and_five_chains_rmat14():
     g = Z.Graph.from_edges(s, d, 1 << 14, seed=1)
     assert g.exact_count("c4") == og.c4()
     assert g.exact_count("path4") == og.paths4()
+
+
+@pytest.mark.parametrize("order,what", [(4, "path3"), (5, "path4")])
+def test_chain_mining_estimate_covers_the_exact_chain_count(order, what):
+    """ChainMining(order) (applications/ChainMining.cpp) on R-MAT scale 14:
+    the exact count of order-vertex chains lies in the 99.9 % CLT interval of
+    2^26 estimators."""
+    s, d = O.gen_rmat(14, 16 << 14, 0.57, 0.19, 0.19, 3)
+    g = Z.Graph.from_edges(s, d, 1 << 14, seed=1)
+    exact = g.exact_count(what)
+    N = 1 << 26
+    r = Z.run(g, Z.Plan(Z.PLAN_CHAIN, order=order), 5, 0, N)[0]
+    hw = Z.clt_half_width(r.mean, r.sumsq, N, 0.999)
+    print(f"[chain{order}] exact {exact} estimate {r.mean:.6g} +- {hw:.3g} ({r.successes} successes)", flush=True)
+    assert r.successes > 1000
+    assert abs(r.mean - exact) <= hw
