@@ -236,3 +236,42 @@ def test_triangle2_and_three_motif_without_cache_apps(tmp_path):
     rt = Z.run(g, Z.Plan(Z.PLAN_TRIANGLE), seed, 0, nt)[0]
     assert rows["3-chain"] == (pytest.approx(rc.total() / 2 ** rc.log2_den / nc, rel=1e-13), nc)
     assert rows["triangle"] == (pytest.approx(rt.total() / 2 ** rt.log2_den / nt, rel=1e-13), nt)
+
+
+EXACT = {"TriangleCounting": "triangle", "FourClique": "k4", "ThreeChain": "wedge", "FourChain": "path3",
+         "FiveChain": "path4", "FiveStar": "star5", "FiveHouse": "house", "FourCycle": "c4"}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("gname", ["simple-30", "wiki-vote"])
+def test_exact_count_app_matches_the_naive_programs_output(tmp_path, gname):
+    """host/apps/ExactCount reads the naive_implementation programs' text
+    edge list and prints their "count: N" line; every program's count equals
+    the Python API's and the oracle's (which the CPU tests pin against the
+    reference's programs built from source)."""
+    import pyoracle as O
+    n, s, d = golden_graph(gname)
+    p = tmp_path / "g.txt"
+    with open(p, "w") as f:
+        f.write(f"{n} {len(s)}\n")
+        for a, b in zip(s, d):
+            f.write(f"{a} {b}\n")
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    og = O.OGraph(n, s, d)
+    oracle = {"triangle": og.triangles, "k4": og.k4, "wedge": og.wedges, "path3": og.paths3, "path4": og.paths4,
+              "star5": lambda: og.stars(4), "house": og.house_brute if n <= 400 else None, "c4": og.c4}
+    for prog, what in EXACT.items():
+        print(f"[cpp] ExactCount {gname} {prog}", flush=True)
+        r = subprocess.run([_bin("ExactCount"), str(p), prog], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        got = int(r.stdout.strip().split("count:")[1])
+        assert got == g.exact_count(what)
+        if oracle[what] is not None:
+            assert got == oracle[what]()
+
+
+def test_exact_count_app_rejects_unknown_programs(tmp_path):
+    p = tmp_path / "g.txt"
+    p.write_text("3 2\n0 1\n1 2\n")
+    r = subprocess.run([_bin("ExactCount"), str(p), "SixClique"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "unknown program" in r.stderr
