@@ -655,8 +655,8 @@ __global__ __launch_bounds__(256) void k_c4_lds(const uint64_t *foff, const uint
 }
 
 // For each listed vertex: the number of its neighbours below it (pu) and an
-// upper bound on its distinct w, the summed degrees of those neighbours (read
-// from their keys), capped at n.
+// upper bound on its distinct w, its wedges u - v - w with v and w below u
+// (each v's count found by the same search the inserts make), capped at n.
 __global__ void k_c4_bound(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg, uint32_t n,
                            const uint32_t *list, uint32_t nlist, uint64_t *below, uint64_t *bound)
 {
@@ -667,7 +667,11 @@ __global__ void k_c4_bound(const uint64_t *foff, const uint64_t *fkey, const uin
         const uint64_t ub = foff[u], ku = ((uint64_t)sdeg[u] << 32) | u;
         const uint64_t pu = lower_u64(fkey + ub, foff[u + 1] - ub, ku);
         unsigned long long b = 0;
-        for (uint64_t j = lane; j < pu; j += 64) b += fkey[ub + j] >> 32;
+        for (uint64_t j = lane; j < pu; j += 64) {
+            const uint32_t v = (uint32_t)fkey[ub + j];
+            const uint64_t vb = foff[v];
+            b += lower_u64(fkey + vb, foff[v + 1] - vb, ku);
+        }
         for (int off = 32; off > 0; off >>= 1) b += __shfl_xor(b, off, 64);
         if (lane == 0) {
             below[i] = pu;

@@ -193,3 +193,29 @@ def test_chain_mining_estimate_covers_the_exact_chain_count(order, what):
     print(f"[chain{order}] exact {exact} estimate {r.mean:.6g} +- {hw:.3g} ({r.successes} successes)", flush=True)
     assert r.successes > 1000
     assert abs(r.mean - exact) <= hw
+
+
+def _ring(n, first=0):
+    return [(first + i, first + (i + 1) % n) for i in range(n)]
+
+
+@pytest.mark.parametrize("name,n,edges,c4,chains", [
+    ("path P10", 10, [(i, i + 1) for i in range(9)], 0, 6),
+    ("cycle C10", 10, _ring(10), 0, 10),
+    ("cycle C5", 5, _ring(5), 0, 5),
+    ("cycle C4", 4, _ring(4), 1, 0),
+    ("two C4 and a K4", 12, _ring(4) + _ring(4, 4) + [(8, 9), (8, 10), (8, 11), (9, 10), (9, 11), (10, 11)], 5, 0),
+    ("star S20", 21, [(0, i) for i in range(1, 21)], 0, 0),
+    ("isolated vertices only", 7, [], 0, 0),
+])
+def test_device_c4_and_five_chains_of_small_structures(name, n, edges, c4, chains):
+    """Closed forms: a path of 10 vertices holds 6 chains of 5, a cycle C_n
+    (n >= 5) n of them and no 4-cycle, C4 one 4-cycle, K4 three 4-cycles and
+    4!/2 = 12 chains of 4 vertices but none of 5, a star neither."""
+    s = np.array([a for a, _ in edges], np.uint32)
+    d = np.array([b for _, b in edges], np.uint32)
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    assert g.exact_count("c4") == c4, name
+    assert g.exact_count("path4") == chains, name
+    og = O.OGraph(n, s, d)
+    assert og.c4() == c4 and og.paths4() == chains and og.paths_brute(5) == chains
