@@ -49,13 +49,14 @@ def main():
     ap.add_argument("--elp-cap-log2", type=int, default=37,
                     help="largest budget of one time-to-error query, log2 estimators (2^37: above the "
                          "~1.1e11 a +-1%% 4-clique answer needs on the bench graph)")
-    ap.add_argument("--workload", default="chunglu69m", choices=["chunglu69m", "rmat26", "sizes", "elp"],
+    ap.add_argument("--workload", default="chunglu69m", choices=["chunglu69m", "rmat26", "sizes", "elp", "exact"],
                     help="chunglu69m: the headline triangle + 4-clique workload; rmat26: R-MAT scale 26, "
                          "edge factor 16 with the 3-motif, 3-chain, 4-chain and predicate-triangle plans; "
                          "sizes: 3-chain, triangle and 4-clique on R-MAT 22/24/26 and Chung-Lu 69M/500M "
                          "(how the rate holds as the graph outgrows the translation caches); elp: time to "
                          "+-1%% for triangle, 4-clique, 3-chain and 4-chain on four graphs, learned against "
-                         "regression sizing")
+                         "regression sizing; exact: the device exact counters on R-MAT 20 and 22, with the "
+                         "CPU oracle's triangle and 4-clique counts beside them")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this size (rocprof counter calibration)")
     ap.add_argument("--line-sweep", action="store_true",
@@ -102,6 +103,9 @@ def main():
         return
     if args.workload == "elp":
         elp_sweep(Z, args)
+        return
+    if args.workload == "exact":
+        exact_sweep(Z, args)
         return
 
     info = Z.device_info()
@@ -450,6 +454,45 @@ def size_sweep(Z, args):
             log(f"sizes {gname} {pname}: {k:.2f} ms, {lines:.3f} lines, {rows[-1]['line_rate_gbs']:.0f} GB/s")
         g.close()
     print(json.dumps({"size_sweep": rows, "estimators_per_launch": N}), flush=True)
+
+
+def exact_sweep(Z, args):
+    """The device exact counters (the naive_implementation programs'
+    semantics) on seeded R-MAT graphs of scale 20 and 22 (edge factor 16):
+    wall time of each count, orientation and list building included, and the
+    oracle's OpenMP triangle and 4-clique counts on this host's cores beside
+    the device's on the scale-20 graph (same counts)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    import pyoracle as O
+    threads = min(16, os.cpu_count() or 1)
+    kinds = ["triangle", "k4", "wedge", "path3", "c4", "path4", "star5"]
+    rows = []
+    for scale in (20, 22):
+        e = Z.rmat(scale, 16 << scale, 0.57, 0.19, 0.19, args.seed)
+        g = Z.Graph.from_device(e, seed=args.seed)
+        e.free()
+        log(f"exact rmat{scale}: m={g.num_local_edges} maxdeg={g.max_degree}")
+        row = {"graph": f"rmat{scale}", "edges": g.num_local_edges, "max_degree": g.max_degree, "counts": {}, "s": {}}
+        for what in kinds:
+            t = time.perf_counter()
+            c = g.exact_count(what)
+            row["counts"][what], row["s"][what] = c, time.perf_counter() - t
+            log(f"exact rmat{scale} {what}: {c} in {row['s'][what]:.2f}s")
+        if scale == 20:
+            a = g.export()
+            og = O.OGraph.from_csr(g.num_vertices, a["src"], a["dst"], a["off"], a["adj_rank"], a["adj_nbr"])
+            del a
+            row["cpu"] = {"threads": threads}
+            for what, fn in (("triangle", og.triangles), ("k4", og.k4)):
+                t = time.perf_counter()
+                c = fn(threads)
+                dt = time.perf_counter() - t
+                row["cpu"][what] = {"s": dt, "equal": c == row["counts"][what]}
+                log(f"exact rmat{scale} {what} on the CPU ({threads} threads): {c} in {dt:.2f}s, "
+                    f"equal {c == row['counts'][what]}")
+        rows.append(row)
+        g.close()
+    print(json.dumps({"exact_counts": rows}), flush=True)
 
 
 def rmat26(Z, args):
