@@ -576,8 +576,8 @@ __device__ __forceinline__ uint64_t lower_u64(const uint64_t *a, uint64_t n, uin
 // below u, cnt[w] += 1; the cycles whose top is u and whose vertex opposite u
 // is w number C(cnt[w], 2).  The work is at most the sum over edges of the
 // smaller degree.  cnt lives in a per-wavefront LDS hash; a vertex that fills
-// it past three quarters goes to a list counted with global hashes
-// (k_c4g_clear, k_c4g_insert, k_c4g_sum).
+// it past three quarters goes to a list counted with dense global counters
+// (k_c4d_clear, k_c4d_insert, k_c4d_sum).
 constexpr int kC4Bits = 10;
 constexpr uint32_t kC4Slots = 1u << kC4Bits;
 constexpr uint32_t kC4Empty = 0xffffffffu;
@@ -654,45 +654,31 @@ __global__ __launch_bounds__(256) void k_c4_lds(const uint64_t *foff, const uint
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-// For each listed vertex: the number of its neighbours below it (pu) and an
-// upper bound on its distinct w, its wedges u - v - w with v and w below u
-// (each v's count found by the same search the inserts make), capped at n.
-__global__ void k_c4_bound(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg, uint32_t n,
-                           const uint32_t *list, uint32_t nlist, uint64_t *below, uint64_t *bound)
+// For each listed vertex: the number of its neighbours below it.
+__global__ void k_c4_below(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg, const uint32_t *list,
+                           uint32_t nlist, uint64_t *below)
 {
-    const uint32_t lane = threadIdx.x & 63;
-    WAVE_STRIDE(i, (uint64_t)nlist)
+    GRID_STRIDE(i, (uint64_t)nlist)
     {
         const uint32_t u = list[i];
         const uint64_t ub = foff[u], ku = ((uint64_t)sdeg[u] << 32) | u;
-        const uint64_t pu = lower_u64(fkey + ub, foff[u + 1] - ub, ku);
-        unsigned long long b = 0;
-        for (uint64_t j = lane; j < pu; j += 64) {
-            const uint32_t v = (uint32_t)fkey[ub + j];
-            const uint64_t vb = foff[v];
-            b += lower_u64(fkey + vb, foff[v + 1] - vb, ku);
-        }
-        for (int off = 32; off > 0; off >>= 1) b += __shfl_xor(b, off, 64);
-        if (lane == 0) {
-            below[i] = pu;
-            bound[i] = b < n ? b : n;
-        }
+        below[i] = lower_u64(fkey + ub, foff[u + 1] - ub, ku);
     }
 }
 
-__global__ void k_c4g_clear(uint32_t *K, uint32_t *V, uint64_t total)
+__global__ void k_c4d_clear(uint32_t *cnt, uint64_t total)
 {
-    GRID_STRIDE(t, total) { K[t] = kC4Empty; V[t] = 0; }
+    GRID_STRIDE(t, total) cnt[t] = 0;
 }
 
-// The listed vertices (hubs) of one batch: one wavefront per (vertex u,
-// neighbour v below u), the lanes inserting v's neighbours below u into u's
-// own global hash (2^rbits[i] slots from roff[i], at least twice u's bound,
-// so every insert finds a slot).  ipre is the exclusive prefix of the
-// vertices' neighbour counts below them.
-__global__ __launch_bounds__(256) void k_c4g_insert(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg,
-                                                    const uint32_t *list, const uint64_t *ipre, const uint64_t *roff,
-                                                    const uint32_t *rbits, uint32_t nb, uint32_t *K, uint32_t *V)
+// The listed vertices (hubs) of one batch, each with a dense counter over all
+// n vertices (cnt + i * n): one wavefront per (vertex u, neighbour v below u),
+// the lanes adding 1 at every neighbour of v below u (atomics whose results
+// are not read).  ipre is the exclusive prefix of the vertices' neighbour
+// counts below them.
+__global__ __launch_bounds__(256) void k_c4d_insert(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg,
+                                                    const uint32_t *list, const uint64_t *ipre, uint32_t nb,
+                                                    uint32_t n, uint32_t *cnt)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -709,28 +695,17 @@ __global__ __launch_bounds__(256) void k_c4g_insert(const uint64_t *foff, const 
         const uint32_t v = (uint32_t)fkey[ub + (t - ipre[lo])];
         const uint64_t vb = foff[v];
         const uint64_t qv = lower_u64(fkey + vb, foff[v + 1] - vb, ku);
-        const uint32_t bits = rbits[lo];
-        const uint64_t size = 1ull << bits;
-        uint32_t *const Ku = K + roff[lo];
-        uint32_t *const Vu = V + roff[lo];
-        for (uint64_t k = lane; k < qv; k += 64) {
-            const uint32_t w = (uint32_t)fkey[vb + k];
-            uint64_t h = c4_hash(w, bits);
-            for (uint64_t probe = 0; probe < size; ++probe) {
-                const uint32_t old = atomicCAS(&Ku[h], kC4Empty, w);
-                if (old == kC4Empty || old == w) { atomicAdd(&Vu[h], 1u); break; }
-                h = (h + 1) & (size - 1);
-            }
-        }
+        uint32_t *const cu = cnt + (uint64_t)lo * n;
+        for (uint64_t k = lane; k < qv; k += 64) atomicAdd(&cu[(uint32_t)fkey[vb + k]], 1u);
     }
 }
 
-__global__ void k_c4g_sum(const uint32_t *V, uint64_t total, unsigned long long *partial)
+__global__ void k_c4d_sum(const uint32_t *cnt, uint64_t total, unsigned long long *partial)
 {
     unsigned long long c = 0;
     GRID_STRIDE(t, total)
     {
-        const unsigned long long x = V[t];
+        const unsigned long long x = cnt[t];
         c += x * (x - (x > 0)) / 2;
     }
     const unsigned long long t = block_sum(c);
@@ -961,68 +936,45 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         ZG_HIP(hipMemcpyAsync(&novf, novf_d.p, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         ZG_HIP(hipStreamSynchronize(st));
         if (novf) {
-            XB<uint64_t> below, bnd;
-            XALLOC(below, novf); XALLOC(bnd, novf);
-            hipLaunchKernelGGL(k_c4_bound, dim3(gridx((uint64_t)novf * 64)), dim3(256), 0, st, foff.p, fkey.p, sdeg.p,
-                               n, ovf.p, novf, below.p, bnd.p);
+            XB<uint64_t> below;
+            XALLOC(below, novf);
+            hipLaunchKernelGGL(k_c4_below, dim3(gridx(novf)), dim3(256), 0, st, foff.p, fkey.p, sdeg.p, ovf.p, novf,
+                               below.p);
             ZG_HIP(hipGetLastError());
-            std::vector<uint64_t> hbelow(novf), hb(novf);
+            std::vector<uint64_t> hbelow(novf);
             ZG_HIP(hipMemcpyAsync(hbelow.data(), below.p, 8ull * novf, hipMemcpyDeviceToHost, st));
-            ZG_HIP(hipMemcpyAsync(hb.data(), bnd.p, 8ull * novf, hipMemcpyDeviceToHost, st));
             ZG_HIP(hipStreamSynchronize(st));
-            std::vector<uint32_t> bits(novf);
-            uint64_t mb = 0;
-            for (uint32_t i = 0; i < novf; ++i) {
-                uint32_t b = 1;
-                while ((1ull << b) < 2 * hb[i] + 2) ++b;
-                if (b > 31) return set_error(ZG_EINVAL, "4-cycle count: too many vertices for the counter's hash");
-                bits[i] = b;
-                mb = std::max<uint64_t>(mb, 1ull << b);
-            }
             size_t free_b = 0, total_b = 0;
             ZG_HIP(hipMemGetInfo(&free_b, &total_b));
-            const uint64_t budget =
-                std::max<uint64_t>(mb, std::min<uint64_t>(16ull << 30, std::max<uint64_t>(1ull << 30, free_b / 4)) / 8);
-            XB<uint32_t> Kt, Vt;
-            XB<uint64_t> d_ipre, d_roff;
-            XB<uint32_t> d_bits;
-            XALLOC(Kt, budget); XALLOC(Vt, budget);
-            XALLOC(d_ipre, (uint64_t)novf + 1); XALLOC(d_roff, novf); XALLOC(d_bits, novf);
-            const int sgrid = gridx(budget);
+            const uint64_t budget = std::min<uint64_t>(16ull << 30, std::max<uint64_t>(1ull << 30, free_b / 4));
+            const uint32_t per = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(novf, budget / (4ull * n)));
+            XB<uint32_t> cnt;
+            XB<uint64_t> d_ipre;
+            XALLOC(cnt, (uint64_t)per * n); XALLOC(d_ipre, (uint64_t)per + 1);
+            const int sgrid = gridx((uint64_t)per * n);
             XB<unsigned long long> gpart;
             XALLOC(gpart, sgrid);
-            // batches of consecutive listed vertices whose hashes fit the budget (in slots)
-            for (uint32_t i0 = 0; i0 < novf;) {
-                std::vector<uint64_t> ipre(1, 0), roff;
-                std::vector<uint32_t> rb;
-                uint64_t slots = 0;
-                uint32_t i1 = i0;
-                while (i1 < novf && slots + (1ull << bits[i1]) <= budget) {
-                    roff.push_back(slots);
-                    rb.push_back(bits[i1]);
-                    slots += 1ull << bits[i1];
-                    ipre.push_back(ipre.back() + hbelow[i1]);
-                    ++i1;
-                }
-                const uint32_t nb = i1 - i0;
+            // batches of `per` listed vertices, each with a dense counter over the n vertices
+            for (uint32_t i0 = 0; i0 < novf; i0 += per) {
+                const uint32_t nb = std::min<uint32_t>(per, novf - i0);
+                std::vector<uint64_t> ipre(nb + 1, 0);
+                for (uint32_t i = 0; i < nb; ++i) ipre[i + 1] = ipre[i] + hbelow[i0 + i];
+                const uint64_t total = (uint64_t)nb * n;
                 ZG_HIP(hipMemcpyAsync(d_ipre.p, ipre.data(), 8ull * (nb + 1), hipMemcpyHostToDevice, st));
-                ZG_HIP(hipMemcpyAsync(d_roff.p, roff.data(), 8ull * nb, hipMemcpyHostToDevice, st));
-                ZG_HIP(hipMemcpyAsync(d_bits.p, rb.data(), 4ull * nb, hipMemcpyHostToDevice, st));
-                hipLaunchKernelGGL(k_c4g_clear, dim3(gridx(slots)), dim3(256), 0, st, Kt.p, Vt.p, slots);
+                hipLaunchKernelGGL(k_c4d_clear, dim3(gridx(total)), dim3(256), 0, st, cnt.p, total);
                 ZG_HIP(hipGetLastError());
                 const int igrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16,
-                                                          std::max<uint64_t>(1, (ipre.back() + 3) / 4));
-                hipLaunchKernelGGL(k_c4g_insert, dim3(igrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p, ovf.p + i0,
-                                   d_ipre.p, d_roff.p, d_bits.p, nb, Kt.p, Vt.p);
+                                                          std::max<uint64_t>(1, (ipre[nb] + 3) / 4));
+                hipLaunchKernelGGL(k_c4d_insert, dim3(igrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p, ovf.p + i0,
+                                   d_ipre.p, nb, n, cnt.p);
                 ZG_HIP(hipGetLastError());
-                const int g2 = gridx(slots);
-                hipLaunchKernelGGL(k_c4g_sum, dim3(g2), dim3(256), 0, st, Vt.p, slots, gpart.p);
+                const int g2 = gridx(total);
+                hipLaunchKernelGGL(k_c4d_sum, dim3(g2), dim3(256), 0, st, cnt.p, total, gpart.p);
                 ZG_HIP(hipGetLastError());
                 unsigned long long g4 = 0;
-                rc = host_sum(gpart.p, g2, st, &g4);   // synchronizes before the host arrays change
+                rc = host_sum(gpart.p, g2, st, &g4);   // synchronizes before ipre changes
                 if (rc) return rc;
                 c4 += g4;
-                i0 = i1;
             }
         }
         if (what == 7) { *out = c4; return ZG_OK; }
