@@ -7,6 +7,9 @@
 //   3-paths    naive_implementation/FourChain.cpp:46-66 (sum over edges (du-1)(dv-1) - 3T)
 //   5-stars    naive_implementation/FiveStar.cpp:51-59 (sum of C(d, 4))
 //   5-houses   naive_implementation/FiveHouse.cpp (per ordered edge, k_house)
+//   5-chains   naive_implementation/FiveChain.cpp:45-72 (closed form over degrees,
+//              per-vertex triangles and 4-cycles)
+//   4-cycles   no reference program (the 5-chain closed form needs them)
 // Method: orient every undirected pair from lower to higher (degree, id) and
 // keep the out-lists sorted by neighbour id (taken from the neighbour-ordered
 // copy, parallel edges collapsed).  Triangles: one wavefront per oriented edge
@@ -14,7 +17,8 @@
 // takes fewer passes times search depth.  4-cliques:
 // per vertex u, bit rows over out(u) (row i: which of out(u) are in
 // out(out(u)[i])) and the sum of popcount(row_i & row_j) over the set bits j
-// of each row.
+// of each row.  4-cycles: at their highest vertex, counting the wedges below
+// it per opposite vertex (LDS hashes, dense counters for hubs).
 #include <cstdlib>
 #include <cstring>
 
