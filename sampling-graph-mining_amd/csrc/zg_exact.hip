@@ -22,6 +22,8 @@
 #include <cstdlib>
 #include <cstring>
 
+#include <utility>
+
 #include <rocprim/rocprim.hpp>
 
 #include "zg_internal.h"
@@ -580,8 +582,10 @@ __device__ __forceinline__ uint64_t lower_u64(const uint64_t *a, uint64_t n, uin
 // below u, cnt[w] += 1; the cycles whose top is u and whose vertex opposite u
 // is w number C(cnt[w], 2).  The work is at most the sum over edges of the
 // smaller degree.  cnt lives in a per-wavefront LDS hash; a vertex that fills
-// it past three quarters goes to a list counted with dense global counters
-// (k_c4d_clear, k_c4d_insert, k_c4d_sum).
+// it past three quarters goes to a list counted by sorting its wedges
+// (k_c4s_fill, radix sort, run-length encoding, k_c4s_pairs), or, past the
+// sort batch's size, with a dense counter (k_c4d_clear, k_c4d_insert,
+// k_c4d_sum).
 constexpr int kC4Bits = 10;
 constexpr uint32_t kC4Slots = 1u << kC4Bits;
 constexpr uint32_t kC4Empty = 0xffffffffu;
@@ -658,16 +662,74 @@ __global__ __launch_bounds__(256) void k_c4_lds(const uint64_t *foff, const uint
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-// For each listed vertex: the number of its neighbours below it.
-__global__ void k_c4_below(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg, const uint32_t *list,
-                           uint32_t nlist, uint64_t *below)
+// For each listed vertex: the number of its neighbours below it, and its
+// wedges u - v - w with v and w below u (each v's share found by the same
+// search the counting kernels make).
+__global__ void k_c4_wedges(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg, const uint32_t *list,
+                            uint32_t nlist, uint64_t *below, uint64_t *wedges)
 {
-    GRID_STRIDE(i, (uint64_t)nlist)
+    const uint32_t lane = threadIdx.x & 63;
+    WAVE_STRIDE(i, (uint64_t)nlist)
     {
         const uint32_t u = list[i];
         const uint64_t ub = foff[u], ku = ((uint64_t)sdeg[u] << 32) | u;
-        below[i] = lower_u64(fkey + ub, foff[u + 1] - ub, ku);
+        const uint64_t pu = lower_u64(fkey + ub, foff[u + 1] - ub, ku);
+        unsigned long long w = 0;
+        for (uint64_t j = lane; j < pu; j += 64) {
+            const uint32_t v = (uint32_t)fkey[ub + j];
+            const uint64_t vb = foff[v];
+            w += lower_u64(fkey + vb, foff[v + 1] - vb, ku);
+        }
+        for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off, 64);
+        if (lane == 0) {
+            below[i] = pu;
+            wedges[i] = w;
+        }
     }
+}
+
+// Sorted counting of a batch of listed vertices: one wavefront per (vertex
+// u = list[i], neighbour v below u) writes the keys (i << 32 | w) of v's
+// neighbours w below u at places reserved from u's cursor; after a radix sort
+// and a run-length encoding, every run of equal keys is one cnt[w] and adds
+// C(run, 2).
+__global__ __launch_bounds__(256) void k_c4s_fill(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg,
+                                                  const uint32_t *list, const uint64_t *ipre, uint32_t nb,
+                                                  unsigned long long *cursor, uint64_t *keys)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t items = ipre[nb];
+    for (uint64_t t = wave; t < items; t += nwaves) {
+        uint32_t lo = 0, hi = nb;   // ipre[lo] <= t < ipre[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (ipre[mid] <= t) lo = mid; else hi = mid;
+        }
+        const uint32_t u = list[lo];
+        const uint64_t ub = foff[u], ku = ((uint64_t)sdeg[u] << 32) | u;
+        const uint32_t v = (uint32_t)fkey[ub + (t - ipre[lo])];
+        const uint64_t vb = foff[v];
+        const uint64_t qv = lower_u64(fkey + vb, foff[v + 1] - vb, ku);
+        unsigned long long base = 0;
+        if (lane == 0 && qv) base = atomicAdd(&cursor[lo], (unsigned long long)qv);
+        base = __shfl(base, 0, 64);
+        for (uint64_t k = lane; k < qv; k += 64) keys[base + k] = ((uint64_t)lo << 32) | (uint32_t)fkey[vb + k];
+    }
+}
+
+__global__ void k_c4s_pairs(const unsigned int *counts, const unsigned int *nruns, unsigned long long *partial)
+{
+    const uint64_t runs = *nruns;
+    unsigned long long c = 0;
+    GRID_STRIDE(t, runs)
+    {
+        const unsigned long long x = counts[t];
+        c += x * (x - 1) / 2;
+    }
+    const unsigned long long t = block_sum(c);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
 __global__ void k_c4d_clear(uint32_t *cnt, uint64_t total)
@@ -940,45 +1002,126 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         ZG_HIP(hipMemcpyAsync(&novf, novf_d.p, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         ZG_HIP(hipStreamSynchronize(st));
         if (novf) {
-            XB<uint64_t> below;
-            XALLOC(below, novf);
-            hipLaunchKernelGGL(k_c4_below, dim3(gridx(novf)), dim3(256), 0, st, foff.p, fkey.p, sdeg.p, ovf.p, novf,
-                               below.p);
+            XB<uint64_t> below, wedges;
+            XALLOC(below, novf); XALLOC(wedges, novf);
+            hipLaunchKernelGGL(k_c4_wedges, dim3(gridx((uint64_t)novf * 64)), dim3(256), 0, st, foff.p, fkey.p,
+                               sdeg.p, ovf.p, novf, below.p, wedges.p);
             ZG_HIP(hipGetLastError());
-            std::vector<uint64_t> hbelow(novf);
+            std::vector<uint64_t> hbelow(novf), hwedge(novf);
+            std::vector<uint32_t> hlist(novf);
             ZG_HIP(hipMemcpyAsync(hbelow.data(), below.p, 8ull * novf, hipMemcpyDeviceToHost, st));
+            ZG_HIP(hipMemcpyAsync(hwedge.data(), wedges.p, 8ull * novf, hipMemcpyDeviceToHost, st));
+            ZG_HIP(hipMemcpyAsync(hlist.data(), ovf.p, 4ull * novf, hipMemcpyDeviceToHost, st));
             ZG_HIP(hipStreamSynchronize(st));
             size_t free_b = 0, total_b = 0;
             ZG_HIP(hipMemGetInfo(&free_b, &total_b));
             const uint64_t budget = std::min<uint64_t>(16ull << 30, std::max<uint64_t>(1ull << 30, free_b / 4));
-            const uint32_t per = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(novf, budget / (4ull * n)));
-            XB<uint32_t> cnt;
-            XB<uint64_t> d_ipre;
-            XALLOC(cnt, (uint64_t)per * n); XALLOC(d_ipre, (uint64_t)per + 1);
-            const int sgrid = gridx((uint64_t)per * n);
-            XB<unsigned long long> gpart;
-            XALLOC(gpart, sgrid);
-            // batches of `per` listed vertices, each with a dense counter over the n vertices
-            for (uint32_t i0 = 0; i0 < novf; i0 += per) {
-                const uint32_t nb = std::min<uint32_t>(per, novf - i0);
-                std::vector<uint64_t> ipre(nb + 1, 0);
-                for (uint32_t i = 0; i < nb; ++i) ipre[i + 1] = ipre[i] + hbelow[i0 + i];
-                const uint64_t total = (uint64_t)nb * n;
-                ZG_HIP(hipMemcpyAsync(d_ipre.p, ipre.data(), 8ull * (nb + 1), hipMemcpyHostToDevice, st));
-                hipLaunchKernelGGL(k_c4d_clear, dim3(gridx(total)), dim3(256), 0, st, cnt.p, total);
-                ZG_HIP(hipGetLastError());
-                const int igrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16,
-                                                          std::max<uint64_t>(1, (ipre[nb] + 3) / 4));
-                hipLaunchKernelGGL(k_c4d_insert, dim3(igrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p, ovf.p + i0,
-                                   d_ipre.p, nb, n, cnt.p);
-                ZG_HIP(hipGetLastError());
-                const int g2 = gridx(total);
-                hipLaunchKernelGGL(k_c4d_sum, dim3(g2), dim3(256), 0, st, cnt.p, total, gpart.p);
-                ZG_HIP(hipGetLastError());
-                unsigned long long g4 = 0;
-                rc = host_sum(gpart.p, g2, st, &g4);   // synchronizes before ipre changes
-                if (rc) return rc;
-                c4 += g4;
+            // sorted batches of consecutive vertices, up to `cap` wedges each (about 24 bytes per
+            // wedge); a vertex with more wedges than that goes to the dense counters instead
+            uint64_t cap = std::min<uint64_t>(budget / 24, 0xF0000000ull);
+            if (const char *ev = getenv("ZGRAPH_C4_SORT_WEDGES")) cap = std::min<uint64_t>(cap, strtoull(ev, nullptr, 10));
+            std::vector<std::pair<uint32_t, uint32_t>> batches;   // [first, end) in the list
+            std::vector<uint32_t> dense;
+            uint64_t tmax = 0;
+            uint32_t nbmax = 0;
+            for (uint32_t i = 0; i < novf;) {
+                if (hwedge[i] > cap) { dense.push_back(hlist[i]); ++i; continue; }
+                uint64_t tot = 0;
+                uint32_t j = i;
+                while (j < novf && hwedge[j] <= cap && tot + hwedge[j] <= cap) tot += hwedge[j++];
+                batches.push_back({i, j});
+                tmax = std::max(tmax, tot);
+                nbmax = std::max(nbmax, j - i);
+                i = j;
+            }
+            if (!batches.empty() && tmax > 0) {
+                XB<uint64_t> ka, kb, d_ipre;
+                XB<unsigned int> counts, nruns;
+                XB<unsigned long long> cursor;
+                XALLOC(ka, tmax); XALLOC(kb, tmax); XALLOC(counts, tmax); XALLOC(nruns, 1);
+                XALLOC(d_ipre, (uint64_t)nbmax + 1); XALLOC(cursor, nbmax);
+                size_t ts = 0, tr = 0;
+                ZG_HIP(rocprim::radix_sort_keys(nullptr, ts, ka.p, kb.p, (size_t)tmax, 0, 64, st));
+                ZG_HIP(rocprim::run_length_encode(nullptr, tr, kb.p, (unsigned int)tmax, ka.p, counts.p, nruns.p,
+                                                  st));
+                XB<char> tmps;
+                XALLOC(tmps, std::max(ts, tr));
+                const int pgrid = gridx(tmax);
+                XB<unsigned long long> spart;
+                XALLOC(spart, pgrid);
+                for (const auto &b : batches) {
+                    const uint32_t nb = b.second - b.first;
+                    std::vector<uint64_t> ipre(nb + 1, 0), cur(nb, 0);
+                    uint64_t tot = 0;
+                    for (uint32_t i = 0; i < nb; ++i) {
+                        ipre[i + 1] = ipre[i] + hbelow[b.first + i];
+                        cur[i] = tot;
+                        tot += hwedge[b.first + i];
+                    }
+                    if (tot == 0) continue;
+                    uint32_t kbits = 1;
+                    while ((1ull << kbits) < nb) ++kbits;
+                    ZG_HIP(hipMemcpyAsync(d_ipre.p, ipre.data(), 8ull * (nb + 1), hipMemcpyHostToDevice, st));
+                    ZG_HIP(hipMemcpyAsync(cursor.p, cur.data(), 8ull * nb, hipMemcpyHostToDevice, st));
+                    const int fgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16,
+                                                              std::max<uint64_t>(1, (ipre[nb] + 3) / 4));
+                    hipLaunchKernelGGL(k_c4s_fill, dim3(fgrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p,
+                                       ovf.p + b.first, d_ipre.p, nb, cursor.p, ka.p);
+                    ZG_HIP(hipGetLastError());
+                    size_t t1 = 0, t2 = 0;
+                    ZG_HIP(rocprim::radix_sort_keys(nullptr, t1, ka.p, kb.p, (size_t)tot, 0, 32 + kbits, st));
+                    ZG_HIP(rocprim::run_length_encode(nullptr, t2, kb.p, (unsigned int)tot, ka.p, counts.p, nruns.p,
+                                                      st));
+                    if (std::max(t1, t2) > std::max(ts, tr))
+                        return set_error(ZG_ENOMEM, "4-cycle count: sort scratch larger than sized");
+                    t1 = t2 = std::max(ts, tr);
+                    ZG_HIP(rocprim::radix_sort_keys(tmps.p, t1, ka.p, kb.p, (size_t)tot, 0, 32 + kbits, st));
+                    ZG_HIP(rocprim::run_length_encode(tmps.p, t2, kb.p, (unsigned int)tot, ka.p, counts.p, nruns.p,
+                                                      st));
+                    const int g2 = gridx(tot);
+                    hipLaunchKernelGGL(k_c4s_pairs, dim3(g2), dim3(256), 0, st, counts.p, nruns.p, spart.p);
+                    ZG_HIP(hipGetLastError());
+                    unsigned long long g4 = 0;
+                    rc = host_sum(spart.p, g2, st, &g4);   // synchronizes before ipre and cur change
+                    if (rc) return rc;
+                    c4 += g4;
+                }
+            }
+            if (!dense.empty()) {
+                // one dense counter over the n vertices per vertex, `per` vertices at a time
+                const uint32_t nd = (uint32_t)dense.size();
+                const uint32_t per = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nd, budget / (4ull * n)));
+                XB<uint32_t> cnt, dlist;
+                XB<uint64_t> d_ipre;
+                XALLOC(cnt, (uint64_t)per * n); XALLOC(dlist, nd); XALLOC(d_ipre, (uint64_t)per + 1);
+                ZG_HIP(hipMemcpyAsync(dlist.p, dense.data(), 4ull * nd, hipMemcpyHostToDevice, st));
+                std::vector<uint64_t> dbelow(nd);
+                for (uint32_t i = 0, k = 0; i < novf; ++i)
+                    if (hwedge[i] > cap) dbelow[k++] = hbelow[i];
+                const int sgrid = gridx((uint64_t)per * n);
+                XB<unsigned long long> gpart;
+                XALLOC(gpart, sgrid);
+                for (uint32_t i0 = 0; i0 < nd; i0 += per) {
+                    const uint32_t nb = std::min<uint32_t>(per, nd - i0);
+                    std::vector<uint64_t> ipre(nb + 1, 0);
+                    for (uint32_t i = 0; i < nb; ++i) ipre[i + 1] = ipre[i] + dbelow[i0 + i];
+                    const uint64_t total = (uint64_t)nb * n;
+                    ZG_HIP(hipMemcpyAsync(d_ipre.p, ipre.data(), 8ull * (nb + 1), hipMemcpyHostToDevice, st));
+                    hipLaunchKernelGGL(k_c4d_clear, dim3(gridx(total)), dim3(256), 0, st, cnt.p, total);
+                    ZG_HIP(hipGetLastError());
+                    const int igrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16,
+                                                              std::max<uint64_t>(1, (ipre[nb] + 3) / 4));
+                    hipLaunchKernelGGL(k_c4d_insert, dim3(igrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p,
+                                       dlist.p + i0, d_ipre.p, nb, n, cnt.p);
+                    ZG_HIP(hipGetLastError());
+                    const int g2 = gridx(total);
+                    hipLaunchKernelGGL(k_c4d_sum, dim3(g2), dim3(256), 0, st, cnt.p, total, gpart.p);
+                    ZG_HIP(hipGetLastError());
+                    unsigned long long g4 = 0;
+                    rc = host_sum(gpart.p, g2, st, &g4);   // synchronizes before ipre changes
+                    if (rc) return rc;
+                    c4 += g4;
+                }
             }
         }
         if (what == 7) { *out = c4; return ZG_OK; }

@@ -156,13 +156,17 @@ def test_device_k4_count_rmat16_in_one_and_in_many_passes(monkeypatch, scratch_w
     assert g.exact_count("k4") == og.k4() == 456481728
 
 
-@pytest.mark.parametrize("n", [9, 70, 1000])
-def test_device_c4_and_five_chains_of_complete_graphs(n):
+@pytest.mark.parametrize("n,sort_wedges", [(9, None), (70, None), (1000, None), (1000, "5000"), (1000, "1")])
+def test_device_c4_and_five_chains_of_complete_graphs(monkeypatch, n, sort_wedges):
     """K_n: 3 C(n, 4) 4-cycles and n!/(n-5)!/2 5-vertex chains.  In K_1000
     the top vertices see up to 998 distinct opposite vertices, more than the
-    LDS hash holds, so they are counted again with global hashes."""
+    LDS hash holds, so they are counted again by sorting their wedges; with
+    the sort batches limited to 5 000 wedges the larger of them take the dense
+    counters, and with 1 every one of them does."""
     from math import comb, perm
     import helpers as H
+    if sort_wedges:
+        monkeypatch.setenv("ZGRAPH_C4_SORT_WEDGES", sort_wedges)
     n, s, d = H.complete_graph(n)
     g = Z.Graph.from_edges(s, d, n, seed=2)
     assert g.exact_count("c4") == 3 * comb(n, 4)
