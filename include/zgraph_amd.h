@@ -183,7 +183,9 @@ typedef struct {
 } zg_launch;
 
 /* The calculate() loop: runs ids [first_id, first_id+count) and reduces
- * exactly.  sums has zg_plan_channels() entries.  kernel_ms (may be NULL)
+ * exactly (first_id + count must not exceed 2^64 - 1, else ZG_EINVAL; the
+ * same holds for zg_run_outcomes, zg_count_reads and batched queries).
+ * sums has zg_plan_channels() entries.  kernel_ms (may be NULL)
  * receives the estimator kernel's duration from HIP events on the handle's
  * stream.  Blocks until done. */
 int zg_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_id, uint64_t count,

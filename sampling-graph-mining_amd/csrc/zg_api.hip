@@ -350,6 +350,7 @@ struct HostAcc {
 int run_chunks(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uint64_t count, zg_outcome *d_out,
                const zg_launch *launch, zg_sums *sums, float *kernel_ms)
 {
+    if (count > ~0ull - first) return set_error(ZG_EINVAL, "estimator ids: first_id + count must not exceed 2^64 - 1");
     const int nch = p->nch;
     HostAcc h[2];
     memset(h, 0, sizeof(h));
@@ -438,6 +439,7 @@ extern "C" int zg_count_reads(zg_graph *g, const zg_plan *p, uint64_t seed, uint
 {
     if (!g || !p || !reads) return set_error(ZG_EINVAL, "null argument");
     if (count > kMaxPerLaunch) return set_error(ZG_EINVAL, "count too large for one counting launch");
+    if (count > ~0ull - first_id) return set_error(ZG_EINVAL, "estimator ids: first_id + count must not exceed 2^64 - 1");
     unsigned long long *d = nullptr;
     ZG_HIP(hipMalloc(&d, 8));
     int rc = ZG_OK;

@@ -165,6 +165,7 @@ extern "C" int zg_batch_create(const zg_query *q, int nq, const zg_batch_config 
         const zg_query &x = q[i];
         if (!x.graph || !x.plan) return set_error(ZG_EINVAL, "query without graph or plan");
         if (x.count > kMaxQueryCount) return set_error(ZG_EINVAL, "a batched query runs at most 2^31 estimators");
+        if (x.count > ~0ull - x.first_id) return set_error(ZG_EINVAL, "estimator ids: first_id + count must not exceed 2^64 - 1");
         if (x.graph->m == 0 && x.count) return set_error(ZG_EINVAL, "query on an empty graph");
         const int kind = x.plan->desc.kind == ZG_PLAN_PATTERN ? 100 + x.plan->desc.pattern_vertices
                                                              : x.plan->desc.kind;   // one kernel per pattern size
@@ -231,6 +232,10 @@ extern "C" int zg_batch_run(zg_batch *b, zg_sums *out, float *ms)
 extern "C" int zg_batch_set_ids(zg_batch *b, const uint64_t *seeds, const uint64_t *first_ids)
 {
     if (!b) return set_error(ZG_EINVAL, "null argument");
+    if (first_ids)
+        for (size_t qi = 0; qi < b->queries.size(); ++qi)
+            if (b->queries[qi].count > ~0ull - first_ids[qi])
+                return set_error(ZG_EINVAL, "estimator ids: first_id + count must not exceed 2^64 - 1");
     for (auto &g : b->groups) {
         for (size_t j = 0; j < g.hq.size(); ++j) {
             const int qi = g.qidx[j];

@@ -69,6 +69,19 @@ def test_empty_and_rejected_graphs():
         Z.Graph.from_edges(np.array([0], np.uint32), np.array([5], np.uint32), 3)         # id >= n
 
 
+def test_estimator_id_ranges_past_2_64_are_rejected():
+    """Ids are 64-bit: a run whose first_id + count exceeds 2^64 - 1 fails
+    with an error instead of reusing ids from 0; the last range within it
+    still runs."""
+    n, s, d = H.golden_graph("simple-30")
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    plan = Z.Plan(Z.PLAN_TRIANGLE)
+    top = (1 << 64) - 1 - 1000
+    assert Z.run(g, plan, 1, top, 1000)[0].count == 1000
+    with pytest.raises(Z.ZGraphError):
+        Z.run(g, plan, 1, top, 1001)
+
+
 def test_subsample_and_filter_equal_oracle():
     n, s, d = H.golden_graph("wiki-vote")
     w = H.golden_weights("wiki-vote").astype(np.float32)
