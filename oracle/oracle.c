@@ -20,6 +20,9 @@
  *   exact counters     closed forms, brute force, SNAP's published
  *                      wiki-Vote triangle count, naive_implementation
  *                      binaries built from source into oracle/_ref       pinned
+ *   og_exact_c4        brute force of the 4-cycle pattern; K_n, C_n      pinned
+ *   og_exact_paths4    og_exact_paths_brute(5), itself equal to the
+ *                      reference's FiveChain built from source            pinned
  *   og_regression_k    tests/TestRegression.cpp known answer k^2=609.8565 pinned
  *   og_elp_*           hand-computed closed form on 1-3 points           pinned
  *   generators         restated from the device generators' definition
