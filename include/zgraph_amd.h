@@ -248,7 +248,12 @@ int zg_count_reads(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_
  * 6 5-vertex chains (FiveChain.cpp:45-72, simple paths of four edges), 7
  * 4-cycles (no reference program; the 5-chain count needs them).  Counts
  * that exceed 64 bits fail with ZG_EOVERFLOW (5-chains) or wrap (4-cycles,
- * not reached below about 10^18). */
+ * not reached below about 10^18).  Scratch: 4-cliques work in vertex ranges
+ * whose bit rows fit min(16 GiB, max(1 GiB, free / 4)) (the environment
+ * variable ZGRAPH_K4_SCRATCH_WORDS lowers it, in 8-byte words); 4-cycles and
+ * 5-chains sort the hubs' wedges in batches of up to budget / 24 wedges
+ * (ZGRAPH_C4_SORT_WEDGES lowers it; a hub with more wedges is counted with
+ * a dense counter of n entries instead). */
 int zg_exact_counts(zg_graph *g, int what, uint64_t *out);
 /* Degree moments sum d^2 and sum d^3 over all vertices (ELP features). */
 int zg_graph_degree_moments(zg_graph *g, double *sum_d2, double *sum_d3);
