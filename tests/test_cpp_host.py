@@ -275,3 +275,15 @@ def test_exact_count_app_rejects_unknown_programs(tmp_path):
     p.write_text("3 2\n0 1\n1 2\n")
     r = subprocess.run([_bin("ExactCount"), str(p), "SixClique"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "unknown program" in r.stderr
+
+
+def test_exact_count_app_rejects_bad_input(tmp_path):
+    """A missing file, a header without counts and an edge naming a vertex
+    outside [0, n) are reported before any device work."""
+    exe = _bin("ExactCount")
+    r = subprocess.run([exe, str(tmp_path / "missing.txt"), "FiveChain"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "cannot read the header" in r.stderr
+    p = tmp_path / "bad.txt"
+    p.write_text("3 2\n0 1\n1 7\n")
+    r = subprocess.run([exe, str(p), "TriangleCounting"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "bad edge 1" in r.stderr
