@@ -689,13 +689,14 @@ __global__ void k_c4_wedges(const uint64_t *foff, const uint64_t *fkey, const ui
 }
 
 // Sorted counting of a batch of listed vertices: one wavefront per (vertex
-// u = list[i], neighbour v below u) writes the keys (i << 32 | w) of v's
+// u = list[i], neighbour v below u) writes the keys (i << wbits | w) of v's
 // neighbours w below u at places reserved from u's cursor; after a radix sort
 // and a run-length encoding, every run of equal keys is one cnt[w] and adds
 // C(run, 2).
+template <class Key>
 __global__ __launch_bounds__(256) void k_c4s_fill(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg,
                                                   const uint32_t *list, const uint64_t *ipre, uint32_t nb,
-                                                  unsigned long long *cursor, uint64_t *keys)
+                                                  uint32_t wbits, unsigned long long *cursor, Key *keys)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -715,7 +716,7 @@ __global__ __launch_bounds__(256) void k_c4s_fill(const uint64_t *foff, const ui
         unsigned long long base = 0;
         if (lane == 0 && qv) base = atomicAdd(&cursor[lo], (unsigned long long)qv);
         base = __shfl(base, 0, 64);
-        for (uint64_t k = lane; k < qv; k += 64) keys[base + k] = ((uint64_t)lo << 32) | (uint32_t)fkey[vb + k];
+        for (uint64_t k = lane; k < qv; k += 64) keys[base + k] = ((Key)lo << wbits) | (Key)(uint32_t)fkey[vb + k];
     }
 }
 
@@ -1020,6 +1021,12 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
             // wedge); a vertex with more wedges than that goes to the dense counters instead
             uint64_t cap = std::min<uint64_t>(budget / 24, 0xF0000000ull);
             if (const char *ev = getenv("ZGRAPH_C4_SORT_WEDGES")) cap = std::min<uint64_t>(cap, strtoull(ev, nullptr, 10));
+            // keys (batch index << wbits | w): 32-bit while ceil(log2 n) + the batch index's bits fit,
+            // which caps a batch at 2^(32 - wbits) vertices; 64-bit keys past n = 2^30
+            uint32_t wbits = 1;
+            while ((1ull << wbits) < n) ++wbits;
+            const bool narrow = wbits <= 30;
+            const uint32_t nbcap = narrow ? (1u << (32 - wbits)) : 0xffffffffu;
             std::vector<std::pair<uint32_t, uint32_t>> batches;   // [first, end) in the list
             std::vector<uint32_t> dense;
             uint64_t tmax = 0;
@@ -1028,7 +1035,7 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                 if (hwedge[i] > cap) { dense.push_back(hlist[i]); ++i; continue; }
                 uint64_t tot = 0;
                 uint32_t j = i;
-                while (j < novf && hwedge[j] <= cap && tot + hwedge[j] <= cap) tot += hwedge[j++];
+                while (j < novf && j - i < nbcap && hwedge[j] <= cap && tot + hwedge[j] <= cap) tot += hwedge[j++];
                 batches.push_back({i, j});
                 tmax = std::max(tmax, tot);
                 nbmax = std::max(nbmax, j - i);
@@ -1040,10 +1047,18 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                 XB<unsigned long long> cursor;
                 XALLOC(ka, tmax); XALLOC(kb, tmax); XALLOC(counts, tmax); XALLOC(nruns, 1);
                 XALLOC(d_ipre, (uint64_t)nbmax + 1); XALLOC(cursor, nbmax);
+                uint32_t *const ka32 = reinterpret_cast<uint32_t *>(ka.p);
+                uint32_t *const kb32 = reinterpret_cast<uint32_t *>(kb.p);
                 size_t ts = 0, tr = 0;
-                ZG_HIP(rocprim::radix_sort_keys(nullptr, ts, ka.p, kb.p, (size_t)tmax, 0, 64, st));
-                ZG_HIP(rocprim::run_length_encode(nullptr, tr, kb.p, (unsigned int)tmax, ka.p, counts.p, nruns.p,
-                                                  st));
+                if (narrow) {
+                    ZG_HIP(rocprim::radix_sort_keys(nullptr, ts, ka32, kb32, (size_t)tmax, 0, 32, st));
+                    ZG_HIP(rocprim::run_length_encode(nullptr, tr, kb32, (unsigned int)tmax, ka32, counts.p,
+                                                      nruns.p, st));
+                } else {
+                    ZG_HIP(rocprim::radix_sort_keys(nullptr, ts, ka.p, kb.p, (size_t)tmax, 0, 64, st));
+                    ZG_HIP(rocprim::run_length_encode(nullptr, tr, kb.p, (unsigned int)tmax, ka.p, counts.p,
+                                                      nruns.p, st));
+                }
                 XB<char> tmps;
                 XALLOC(tmps, std::max(ts, tr));
                 const int pgrid = gridx(tmax);
@@ -1065,19 +1080,35 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                     ZG_HIP(hipMemcpyAsync(cursor.p, cur.data(), 8ull * nb, hipMemcpyHostToDevice, st));
                     const int fgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16,
                                                               std::max<uint64_t>(1, (ipre[nb] + 3) / 4));
-                    hipLaunchKernelGGL(k_c4s_fill, dim3(fgrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p,
-                                       ovf.p + b.first, d_ipre.p, nb, cursor.p, ka.p);
-                    ZG_HIP(hipGetLastError());
+                    const uint32_t ebits = narrow ? wbits + kbits : 32 + kbits;   // <= 32 when narrow
                     size_t t1 = 0, t2 = 0;
-                    ZG_HIP(rocprim::radix_sort_keys(nullptr, t1, ka.p, kb.p, (size_t)tot, 0, 32 + kbits, st));
-                    ZG_HIP(rocprim::run_length_encode(nullptr, t2, kb.p, (unsigned int)tot, ka.p, counts.p, nruns.p,
-                                                      st));
+                    if (narrow) {
+                        hipLaunchKernelGGL(k_c4s_fill<uint32_t>, dim3(fgrid), dim3(256), 0, st, foff.p, fkey.p,
+                                           sdeg.p, ovf.p + b.first, d_ipre.p, nb, wbits, cursor.p, ka32);
+                        ZG_HIP(hipGetLastError());
+                        ZG_HIP(rocprim::radix_sort_keys(nullptr, t1, ka32, kb32, (size_t)tot, 0, ebits, st));
+                        ZG_HIP(rocprim::run_length_encode(nullptr, t2, kb32, (unsigned int)tot, ka32, counts.p,
+                                                          nruns.p, st));
+                    } else {
+                        hipLaunchKernelGGL(k_c4s_fill<uint64_t>, dim3(fgrid), dim3(256), 0, st, foff.p, fkey.p,
+                                           sdeg.p, ovf.p + b.first, d_ipre.p, nb, 32u, cursor.p, ka.p);
+                        ZG_HIP(hipGetLastError());
+                        ZG_HIP(rocprim::radix_sort_keys(nullptr, t1, ka.p, kb.p, (size_t)tot, 0, ebits, st));
+                        ZG_HIP(rocprim::run_length_encode(nullptr, t2, kb.p, (unsigned int)tot, ka.p, counts.p,
+                                                          nruns.p, st));
+                    }
                     if (std::max(t1, t2) > std::max(ts, tr))
                         return set_error(ZG_ENOMEM, "4-cycle count: sort scratch larger than sized");
                     t1 = t2 = std::max(ts, tr);
-                    ZG_HIP(rocprim::radix_sort_keys(tmps.p, t1, ka.p, kb.p, (size_t)tot, 0, 32 + kbits, st));
-                    ZG_HIP(rocprim::run_length_encode(tmps.p, t2, kb.p, (unsigned int)tot, ka.p, counts.p, nruns.p,
-                                                      st));
+                    if (narrow) {
+                        ZG_HIP(rocprim::radix_sort_keys(tmps.p, t1, ka32, kb32, (size_t)tot, 0, ebits, st));
+                        ZG_HIP(rocprim::run_length_encode(tmps.p, t2, kb32, (unsigned int)tot, ka32, counts.p,
+                                                          nruns.p, st));
+                    } else {
+                        ZG_HIP(rocprim::radix_sort_keys(tmps.p, t1, ka.p, kb.p, (size_t)tot, 0, ebits, st));
+                        ZG_HIP(rocprim::run_length_encode(tmps.p, t2, kb.p, (unsigned int)tot, ka.p, counts.p,
+                                                          nruns.p, st));
+                    }
                     const int g2 = gridx(tot);
                     hipLaunchKernelGGL(k_c4s_pairs, dim3(g2), dim3(256), 0, st, counts.p, nruns.p, spart.p);
                     ZG_HIP(hipGetLastError());
