@@ -247,8 +247,8 @@ int zg_count_reads(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_
  * the sum over edges of d_a * d_c: meant for graphs up to a few million edges),
  * 6 5-vertex chains (FiveChain.cpp:45-72, simple paths of four edges), 7
  * 4-cycles (no reference program; the 5-chain count needs them).  Counts
- * that exceed 64 bits fail with ZG_EOVERFLOW (5-chains) or wrap (4-cycles,
- * not reached below about 10^18).  Scratch: 4-cliques work in vertex ranges
+ * that exceed 64 bits fail with ZG_EOVERFLOW (5-stars, 5-chains) or wrap
+ * (4-cycles, not reached below about 10^18).  Scratch: 4-cliques work in vertex ranges
  * whose bit rows fit min(16 GiB, max(1 GiB, free / 4)) (the environment
  * variable ZGRAPH_K4_SCRATCH_WORDS lowers it, in 8-byte words); 4-cycles and
  * 5-chains sort the hubs' wedges in batches of up to budget / 24 wedges

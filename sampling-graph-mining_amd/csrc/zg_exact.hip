@@ -149,6 +149,15 @@ __device__ __forceinline__ bool bsearch_u32(const uint32_t *a, uint64_t n, uint3
 // ceil(log2(x)): the depth of a binary search over x - 1 entries
 __device__ __forceinline__ uint64_t clog2_u64(uint64_t x) { return x <= 1 ? 0 : 64 - __clzll(x - 1); }
 
+// (lo, hi) += (blo, bhi) in 128 bits
+__device__ __forceinline__ void add128(unsigned long long &lo, unsigned long long &hi, unsigned long long blo,
+                                       unsigned long long bhi)
+{
+    const unsigned long long l = lo + blo;
+    hi += bhi + (l < lo ? 1 : 0);
+    lo = l;
+}
+
 __device__ __forceinline__ unsigned long long block_sum(unsigned long long v)
 {
     __shared__ unsigned long long sm[4];
@@ -443,17 +452,29 @@ __global__ void k_wedges_paths(const uint64_t *off, const uint64_t *sadj, const 
     if (threadIdx.x == 0) { wed[blockIdx.x] = tw; paths[blockIdx.x] = tp; }
 }
 
-// 5-stars: sum over vertices of C(d, 4) (naive_implementation/FiveStar.cpp:51-59)
-__global__ void k_stars4(const uint32_t *sdeg, uint32_t n, unsigned long long *part)
+// 5-stars: sum over vertices of C(d, 4) (naive_implementation/FiveStar.cpp:51-59),
+// in 128 bits (one hub of 150 000 neighbours already passes 2^64); per-wave
+// (lo, hi) partials
+__global__ __launch_bounds__(256) void k_stars4(const uint32_t *sdeg, uint32_t n, unsigned long long *part)
 {
-    unsigned long long c = 0;
+    unsigned long long lo = 0, hi = 0;
     GRID_STRIDE(v, (uint64_t)n)
     {
         const unsigned __int128 d = sdeg[v];
-        if (d >= 4) c += (unsigned long long)(d * (d - 1) * (d - 2) * (d - 3) / 24);
+        if (d >= 4) {
+            const unsigned __int128 c = d * (d - 1) * (d - 2) * (d - 3) / 24;
+            add128(lo, hi, (unsigned long long)c, (unsigned long long)(c >> 64));
+        }
     }
-    const unsigned long long t = block_sum(c);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long ol = __shfl_xor(lo, off, 64), oh = __shfl_xor(hi, off, 64);
+        add128(lo, hi, ol, oh);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+        part[2 * wave] = lo;
+        part[2 * wave + 1] = hi;
+    }
 }
 
 __global__ void k_widen(const uint32_t *a, uint32_t n, uint64_t *b)
@@ -783,14 +804,6 @@ __global__ void k_c4d_sum(const uint32_t *cnt, uint64_t total, unsigned long lon
 // (d_x - 1)^2 over the neighbours x of v (degrees read from the keys): the
 // ordered pairs of distinct neighbours of v weighted by (d_x - 1)(d_y - 1).
 // One wavefront per vertex; per-wave 128-bit partials as (lo, hi) pairs.
-__device__ __forceinline__ void add128(unsigned long long &lo, unsigned long long &hi, unsigned long long blo,
-                                       unsigned long long bhi)
-{
-    const unsigned long long l = lo + blo;
-    hi += bhi + (l < lo ? 1 : 0);
-    lo = l;
-}
-
 __global__ __launch_bounds__(256) void k_c5_pairs(const uint64_t *foff, const uint64_t *fkey, uint32_t n,
                                                   unsigned long long *partial)
 {
@@ -854,13 +867,16 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
     if (what == 4) {
         const int grid = gridx(n);
         XB<unsigned long long> part;
-        XALLOC(part, grid);
+        XALLOC(part, (uint64_t)grid * 8);   // (lo, hi) per wave, 4 waves per block
         hipLaunchKernelGGL(k_stars4, dim3(grid), dim3(256), 0, st, sdeg.p, n, part.p);
         ZG_HIP(hipGetLastError());
-        unsigned long long s = 0;
-        int rc = host_sum(part.p, grid, st, &s);
-        if (rc) return rc;
-        *out = s;
+        std::vector<unsigned long long> hp((size_t)grid * 8);
+        ZG_HIP(hipMemcpyAsync(hp.data(), part.p, 8ull * hp.size(), hipMemcpyDeviceToHost, st));
+        ZG_HIP(hipStreamSynchronize(st));
+        unsigned __int128 total = 0;
+        for (size_t i = 0; i < hp.size(); i += 2) total += ((unsigned __int128)hp[i + 1] << 64) | hp[i];
+        if (total >> 64) return set_error(ZG_EOVERFLOW, "5-star count exceeds 64 bits");
+        *out = (uint64_t)total;
         return ZG_OK;
     }
     if (what == 5) {

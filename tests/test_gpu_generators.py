@@ -223,3 +223,19 @@ def test_device_c4_and_five_chains_of_small_structures(name, n, edges, c4, chain
     assert g.exact_count("path4") == chains, name
     og = O.OGraph(n, s, d)
     assert og.c4() == c4 and og.paths4() == chains and og.paths_brute(5) == chains
+
+
+def test_device_five_star_count_near_and_past_64_bits():
+    """A star of 100 000 leaves holds C(100000, 4) ~ 4.2e18 5-stars (exact,
+    near 2^64); one of 150 000 leaves holds ~2.1e19, past 2^64, which the call
+    reports as an overflow instead of wrapping."""
+    from math import comb
+    for leaves, ok in ((100_000, True), (150_000, False)):
+        s = np.zeros(leaves, np.uint32)
+        d = np.arange(1, leaves + 1, dtype=np.uint32)
+        g = Z.Graph.from_edges(s, d, leaves + 1, seed=1)
+        if ok:
+            assert g.exact_count("star5") == comb(leaves, 4)
+        else:
+            with pytest.raises(Z.ZGraphError):
+                g.exact_count("star5")
