@@ -190,6 +190,22 @@ typedef struct {
  * stream.  Blocks until done. */
 int zg_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_id, uint64_t count,
            const zg_launch *launch, zg_sums *sums, float *kernel_ms);
+/* zg_run that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and sums as zg_run (estimator-id order only: a launch
+ * order of 2..16 fails with ZG_EINVAL).  The block partials and the result
+ * record live in a fixed device array of the library (16384 partial records
+ * of 64 bytes plus the result, about 1 MiB, held by each GPU until the
+ * process ends); a launch whose grid needs more partial records than that
+ * (workgroups x channels > 16384, e.g. blocks_per_cu > 32 for two-channel
+ * plans on 256 CUs) fails with ZG_EINVAL before anything runs.  Every call on
+ * one GPU shares that array: calls that overlap in time on one GPU (from two
+ * threads, or on two graph handles' streams) race and must be serialised by
+ * the caller.  Nothing needs clearing between calls (each launch writes every
+ * record it reads). */
+int zg_run_noalloc(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_id, uint64_t count,
+                   const zg_launch *launch, zg_sums *sums, float *kernel_ms);
 /* Same, and copies every estimator's outcome(s) to host memory
  * (count * channels records, channel-minor). */
 int zg_run_outcomes(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_id,

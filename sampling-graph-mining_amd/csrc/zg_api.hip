@@ -19,6 +19,18 @@ static thread_local std::string g_err;
 static std::mutex g_init_mu;
 static int g_inited = 0, g_cus = 0, g_lds = 0, g_warp = 0;
 
+// Device scratch of zg_run_noalloc: kNoAllocPartials block partial records
+// followed by the two-channel result record, in a file-scope device array
+// (each GPU holds its own copy until the process ends; about 1 MiB).  Host
+// code must not pass `scratch` itself to a kernel: its device address is
+// looked up once per device with hipGetSymbolAddress (in ensure_init, with
+// the device current) and kept in g_scratch_dev[device].
+constexpr size_t kNoAllocPartials = 16384;
+constexpr size_t kNoAllocRecords = kNoAllocPartials + 2;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) Acc scratch[kNoAllocRecords];
+static Acc *g_scratch_dev[kMaxDevices];
+
 int set_error(int code, const std::string &msg)
 {
     g_err = msg;
@@ -44,6 +56,11 @@ int ensure_init()
     g_lds = (int)prop.maxSharedMemoryPerMultiProcessor;
     g_warp = prop.warpSize;
     if (g_warp != 64) return set_error(ZG_ESTATE, "expected 64-wide wavefronts (CDNA)");
+    // the library's handles all live on device 0, the current device here
+    void *sp = nullptr;
+    ZG_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+    if (!sp) return set_error(ZG_ESTATE, "no device address for the no-allocation scratch");
+    g_scratch_dev[0] = static_cast<Acc *>(sp);
     g_inited = 1;
     return ZG_OK;
 }
@@ -347,8 +364,10 @@ struct HostAcc {
     double sq;
 };
 
+// ext (may be null): device scratch of kNoAllocRecords records that replaces
+// the handle's partials and result buffers, so that nothing is allocated
 int run_chunks(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uint64_t count, zg_outcome *d_out,
-               const zg_launch *launch, zg_sums *sums, float *kernel_ms)
+               const zg_launch *launch, zg_sums *sums, float *kernel_ms, Acc *ext = nullptr)
 {
     if (count > ~0ull - first) return set_error(ZG_EINVAL, "estimator ids: first_id + count must not exceed 2^64 - 1");
     const int nch = p->nch;
@@ -362,10 +381,11 @@ int run_chunks(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
     for (uint64_t done = 0; done < count || (count == 0 && done == 0);) {
         const uint64_t c = std::min(per_launch, count - done);
         int rc = launch_run(g, p, seed, first + done, c, d_out ? d_out + done * nch : nullptr, block, bpc, nullptr,
-                            order);
+                            order, ext, ext ? kNoAllocPartials : 0, ext ? ext + kNoAllocPartials : nullptr);
         if (rc) return rc;
         Acc r[2];
-        ZG_HIP(hipMemcpyAsync(r, g->result, sizeof(Acc) * nch, hipMemcpyDeviceToHost, g->stream));
+        ZG_HIP(hipMemcpyAsync(r, ext ? ext + kNoAllocPartials : g->result, sizeof(Acc) * nch, hipMemcpyDeviceToHost,
+                              g->stream));
         ZG_HIP(hipStreamSynchronize(g->stream));
         for (int ch = 0; ch < nch; ++ch) {
             for (int i = 0; i < 5; ++i) h[ch].dig[i] += r[ch].dig[i];
@@ -409,6 +429,23 @@ extern "C" int zg_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t fir
 {
     if (!g || !p || !sums) return set_error(ZG_EINVAL, "null argument");
     int rc = run_chunks(g, p, seed, first_id, count, nullptr, launch, sums, kernel_ms);
+    if (rc) return rc;
+    for (int c = 0; c < p->nch; ++c)
+        if (sums[c].status) return set_error(ZG_EOVERFLOW, "an estimator value exceeded 128 bits");
+    return ZG_OK;
+}
+
+extern "C" int zg_run_noalloc(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_id, uint64_t count,
+                              const zg_launch *launch, zg_sums *sums, float *kernel_ms)
+{
+    if (!g || !p || !sums) return set_error(ZG_EINVAL, "null argument");
+    if (launch && launch->order >= 2)
+        return set_error(ZG_EINVAL, "zg_run_noalloc runs in estimator-id order only (order 0 or 1)");
+    int dev = 0;
+    ZG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices || !g_scratch_dev[dev])
+        return set_error(ZG_ESTATE, "zg_run_noalloc: no scratch on the current device (the library runs on device 0)");
+    int rc = run_chunks(g, p, seed, first_id, count, nullptr, launch, sums, kernel_ms, g_scratch_dev[dev]);
     if (rc) return rc;
     for (int c = 0; c < p->nch; ++c)
         if (sums[c].status) return set_error(ZG_EOVERFLOW, "an estimator value exceeded 128 bits");

@@ -101,9 +101,12 @@ void free_graph(zg_graph *g);
 
 // estimator launch (zg_kernels.hip): enqueues on g->stream, results in g->result
 // order: 0 automatic, 1 estimator-id order, 2..16 first-edge order with 2^order buckets
+// ext_partials / ext_result (both or neither): caller-provided block partials
+// (ext_cap records) and result record instead of the handle's own buffers; the
+// launch then allocates nothing and fails if its grid needs more than ext_cap
 int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uint64_t count,
                zg_outcome *d_out, int block, int blocks_per_cu, unsigned long long *reads = nullptr,
-               int order = 1);
+               int order = 1, Acc *ext_partials = nullptr, size_t ext_cap = 0, Acc *ext_result = nullptr);
 // launches of at most this many estimators when they run in first-edge order
 constexpr uint64_t kMaxOrderedLaunch = 1ull << 28;
 int resolve_order(const zg_graph *g, const zg_plan *p, uint64_t count, int order);

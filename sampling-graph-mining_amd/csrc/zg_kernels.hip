@@ -809,20 +809,20 @@ __global__ __launch_bounds__(256) void k_final(const Acc *__restrict__ partials,
 
 template <int KIND, int NCH>
 static void launch_kind(const zg_graph *g, const PlanView &P, uint64_t seed, uint64_t first, uint64_t count,
-                        zg_outcome *d_out, int grid, int block, unsigned long long *reads,
-                        const uint32_t *order = nullptr)
+                        zg_outcome *d_out, int grid, int block, unsigned long long *reads, const uint32_t *order,
+                        Acc *partials)
 {
     GView v = g->view();
     v.reads = reads;
     if (order)
         hipLaunchKernelGGL((k_run_ordered<KIND, NCH>), dim3(grid), dim3(block), 0, g->stream, v, P, seed, first,
-                           count, order, g->partials);
+                           count, order, partials);
     else if (d_out)
         hipLaunchKernelGGL((k_run<KIND, NCH, true>), dim3(grid), dim3(block), 0, g->stream, v, P, seed, first, count,
-                           d_out, g->partials);
+                           d_out, partials);
     else
         hipLaunchKernelGGL((k_run<KIND, NCH, false>), dim3(grid), dim3(block), 0, g->stream, v, P, seed, first,
-                           count, d_out, g->partials);
+                           count, d_out, partials);
 }
 
 // Automatic = estimator-id order: on MI355X the ordered launch measured slower
@@ -880,7 +880,8 @@ static int order_by_first_edge(zg_graph *g, const zg_plan *p, uint64_t seed, uin
 }
 
 int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uint64_t count,
-               zg_outcome *d_out, int block, int blocks_per_cu, unsigned long long *reads, int order)
+               zg_outcome *d_out, int block, int blocks_per_cu, unsigned long long *reads, int order,
+               Acc *ext_partials, size_t ext_cap, Acc *ext_result)
 {
     if (block <= 0) block = 256;
     if (block % 64 || block > 1024 || block > 256)
@@ -891,41 +892,51 @@ int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
     if (want < grid) grid = want;
     if (grid == 0) grid = 1;
     const int nch = p->nch;
-    if (grid * nch > g->partials_cap) {
+    if (order < 0 || order > 16) return set_error(ZG_EINVAL, "order must be 0 (automatic), 1 (id) or 2..16");
+    Acc *partials = g->partials, *result = g->result;
+    if (ext_partials) {
+        if (order >= 2) return set_error(ZG_EINVAL, "a run into caller scratch takes estimator-id order only");
+        if (grid * nch > ext_cap)
+            return set_error(ZG_EINVAL, "the launch needs " + std::to_string(grid * nch) +
+                                            " block partial records but the scratch holds " +
+                                            std::to_string(ext_cap) + "; lower blocks_per_cu");
+        partials = ext_partials;
+        result = ext_result;
+    } else if (grid * nch > g->partials_cap) {
         if (g->partials) ZG_HIP(hipFree(g->partials));
         g->partials = nullptr;
         ZG_HIP(hipMalloc(&g->partials, sizeof(Acc) * grid * nch));
         g->partials_cap = grid * nch;
+        partials = g->partials;
     }
     const PlanView P = p->view();
     const uint32_t *ord = nullptr;
-    if (order < 0 || order > 16) return set_error(ZG_EINVAL, "order must be 0 (automatic), 1 (id) or 2..16");
     if (order >= 2 && !d_out && !reads && count > 0 && g->m > 0) {
         int rc = order_by_first_edge(g, p, seed, first, count, order, &ord);
         if (rc) return rc;
     }
     switch (p->desc.kind) {
-    case ZG_PLAN_TRIANGLE: launch_kind<ZG_PLAN_TRIANGLE, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
-    case ZG_PLAN_THREE_CHAIN: launch_kind<ZG_PLAN_THREE_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
-    case ZG_PLAN_THREE_MOTIF: launch_kind<ZG_PLAN_THREE_MOTIF, 2>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
-    case ZG_PLAN_CHAIN: launch_kind<ZG_PLAN_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
-    case ZG_PLAN_FOUR_CHAIN: launch_kind<ZG_PLAN_FOUR_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
+    case ZG_PLAN_TRIANGLE: launch_kind<ZG_PLAN_TRIANGLE, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
+    case ZG_PLAN_THREE_CHAIN: launch_kind<ZG_PLAN_THREE_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
+    case ZG_PLAN_THREE_MOTIF: launch_kind<ZG_PLAN_THREE_MOTIF, 2>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
+    case ZG_PLAN_CHAIN: launch_kind<ZG_PLAN_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
+    case ZG_PLAN_FOUR_CHAIN: launch_kind<ZG_PLAN_FOUR_CHAIN, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
     case ZG_PLAN_PATTERN:
         switch (P.pk) {
-        case 2: launch_kind<KIND_PATTERN_BASE + 2, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
-        case 3: launch_kind<KIND_PATTERN_BASE + 3, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
-        case 4: launch_kind<KIND_PATTERN_BASE + 4, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
-        case 5: launch_kind<KIND_PATTERN_BASE + 5, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
-        case 6: launch_kind<KIND_PATTERN_BASE + 6, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
+        case 2: launch_kind<KIND_PATTERN_BASE + 2, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
+        case 3: launch_kind<KIND_PATTERN_BASE + 3, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
+        case 4: launch_kind<KIND_PATTERN_BASE + 4, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
+        case 5: launch_kind<KIND_PATTERN_BASE + 5, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
+        case 6: launch_kind<KIND_PATTERN_BASE + 6, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
         default: return set_error(ZG_EINVAL, "pattern vertices must be in [2, 6]");
         }
         break;
-    case ZG_PLAN_K4_CLOSE: launch_kind<ZG_PLAN_K4_CLOSE, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
-    case ZG_PLAN_TRIANGLE_K4: launch_kind<ZG_PLAN_TRIANGLE_K4, 2>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord); break;
+    case ZG_PLAN_K4_CLOSE: launch_kind<ZG_PLAN_K4_CLOSE, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
+    case ZG_PLAN_TRIANGLE_K4: launch_kind<ZG_PLAN_TRIANGLE_K4, 2>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
     default: return set_error(ZG_EINVAL, "unknown plan kind");
     }
     ZG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, g->stream, g->partials, (int)grid, nch, g->result);
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, g->stream, partials, (int)grid, nch, result);
     ZG_HIP(hipGetLastError());
     return ZG_OK;
 }

@@ -37,7 +37,7 @@ EXPORTED = [
     "zg_edges_download", "zg_edges_free",
     "zg_plan_create", "zg_plan_destroy", "zg_plan_channels", "zg_plan_cond_calls",
     "zg_plan_unique_patterns", "zg_pattern_unique",
-    "zg_run", "zg_run_outcomes", "zg_count_reads", "zg_exact_counts",
+    "zg_run", "zg_run_noalloc", "zg_run_outcomes", "zg_count_reads", "zg_exact_counts",
     "zg_batch_create", "zg_batch_records", "zg_batch_run", "zg_batch_set_ids", "zg_batch_autotune",
     "zg_batch_config_get", "zg_batch_destroy", "zg_run_batch",
     "zg_graph_degree_moments", "zg_random_gather_bw", "zg_random_line_bw",
@@ -143,6 +143,7 @@ def lib():
     L.zg_plan_unique_patterns.argtypes = [vp, P(C.c_int), P(C.c_int)]
     L.zg_pattern_unique.argtypes = [P(PlanDesc), P(C.c_int), P(C.c_int)]
     L.zg_run.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, P(Launch), P(Sums), P(C.c_float)]
+    L.zg_run_noalloc.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, P(Launch), P(Sums), P(C.c_float)]
     L.zg_run_outcomes.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, P(Sums)]
     L.zg_exact_counts.argtypes = [vp, C.c_int, P(C.c_uint64)]
     L.zg_batch_create.argtypes = [P(Query), C.c_int, P(BatchConfig), P(vp)]
