@@ -612,8 +612,9 @@ def cpu_baseline(Z, g, plans, args, N, elp=None):
     query's estimators (same seed and ids: the same answer, bit for bit)."""
     os.environ.setdefault("OMP_PROC_BIND", "close")   # read when the OpenMP runtime starts
     import pyoracle as O
-    threads = min(16, os.cpu_count() or 1)
-    log(f"cpu baseline: exporting CSR, {threads} threads")
+    host = _host_cpus()
+    threads = host["threads_used"]
+    log(f"cpu baseline: exporting CSR, {threads} threads ({host})")
     a = g.export()
     og = O.OGraph.from_csr(g.num_vertices, a["src"], a["dst"], a["off"], a["adj_rank"], a["adj_nbr"])
     del a
@@ -665,19 +666,49 @@ def cpu_baseline(Z, g, plans, args, N, elp=None):
             "time_to_error": tte,
             "sample": "; ".join(f"{k}: {samples[k]} estimators (ids 0..) of the same graph/seed"
                                  for k in samples),
-            "per_pattern": rates, "cpu_model": _cpu_model(),
+            "per_pattern": rates, "cpu_model": host["cpu_model"], "host": host,
+            "per_thread": 2 * N / per_step / threads,
             "mt19937_draws": {"value": 2 * N / sum(N / r for r in mt.values()), "per_pattern": mt}}
 
 
+def _host_cpus():
+    """The host's CPUs as this process sees them, and the threads the CPU
+    baseline uses: every CPU of the affinity mask (the reference runs one
+    OpenMP thread per configured CPU, src/SharedMemSys.cpp:17-19), capped by
+    OMP_NUM_THREADS when the environment sets it (a shared host's CPU share)."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    try:
+        cap = int(omp) if omp else 0
+    except ValueError:
+        cap = 0
+    threads = min(aff, cap) if cap > 0 else aff
+    try:
+        numa = len([d for d in os.listdir("/sys/devices/system/node") if d.startswith("node") and d[4:].isdigit()])
+    except OSError:
+        numa = None
+    return {"nproc": os.cpu_count(), "affinity": aff, "omp_num_threads": omp, "threads_used": threads,
+            "numa_nodes": numa, "cpu_model": _cpu_model(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
+
+
 def _cpu_model():
+    """model name, plus family/model/stepping (a VM may shorten the name)"""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" in line:
+                    k, v = (x.strip() for x in line.split(":", 1))
+                    if k in ("model name", "cpu family", "model", "stepping") and k not in info:
+                        info[k] = v
+                if not line.strip() and info:
+                    break
     except OSError:
-        pass
-    return None
+        return None
+    if not info:
+        return None
+    return f"{info.get('model name', '?')} (family {info.get('cpu family', '?')}, model {info.get('model', '?')}, " \
+           f"stepping {info.get('stepping', '?')})"
 
 
 if __name__ == "__main__":
