@@ -302,14 +302,14 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95, patterns=None, cap_log2=None,
     return out
 
 
-def batched_queries(Z, args):
-    """256 mixed queries (triangle, 4-clique, 5-star, 5-house, 4-chain) over
-    64 seeded Chung-Lu / R-MAT graphs of 1e4..1e7 edges, budgets 2^16..2^26:
-    one zg_run call per query against one replay of the batched HIP graph
-    (results checked equal, query by query).  Then the same queries at 2^16
-    estimators each, where per-call overhead dominates, with the autotuner."""
+def batched_workload(Z, seed=1):
+    """The batched-query workload: 64 seeded Chung-Lu / R-MAT graphs of
+    1e4..1e7 edges (log-uniform) and 256 queries cycling through triangle,
+    4-clique, 5-star, 5-house and 4-chain plans on random graphs, budgets
+    log-uniform in 2^16..2^26.  Returns (graphs, plans, queries, build_s);
+    a query is (graph, plan, seed, first_id, count)."""
     import numpy as np
-    rng = np.random.default_rng(args.seed + 40)
+    rng = np.random.default_rng(seed + 40)
     t = time.perf_counter()
     graphs = []
     for i in range(64):
@@ -328,6 +328,16 @@ def batched_queries(Z, args):
              Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_HOUSE_PATTERN), Z.Plan(Z.PLAN_FOUR_CHAIN)]
     queries = [(graphs[int(rng.integers(64))], plans[i % 5], int(rng.integers(1, 1 << 40)), 0,
                 int(2 ** rng.uniform(16, 26))) for i in range(256)]
+    return graphs, plans, queries, t_graphs
+
+
+def batched_queries(Z, args):
+    """The batched-query workload (batched_workload): one zg_run call per
+    query against one replay of the batched HIP graph (results checked equal,
+    query by query), the batch's random lines per estimator and its rate
+    against the HBM peak; then the same queries at 2^16 estimators each,
+    where per-call overhead dominates, with the autotuner."""
+    graphs, plans, queries, t_graphs = batched_workload(Z, args.seed)
     total = sum(q[4] for q in queries)
     log(f"batch: 64 graphs in {t_graphs:.1f}s, 256 queries, {total:.3e} estimators")
 
@@ -354,6 +364,29 @@ def batched_queries(Z, args):
         rec = {"estimators": sum(q[4] for q in qs), "sequential_s": t_seq, "batched_s": t_b,
                "speedup": t_seq / t_b, "batched_estimators_per_s": sum(q[4] for q in qs) / t_b,
                "exact_sums_equal": same, "config": b.config()}
+        if label == "mixed_budgets":
+            # algorithmic bytes: each query's lines per estimator from a read-counting
+            # launch over its first (up to) 2^14 ids, scaled to its budget
+            # graphs up to the 256 MB memory-side cache (MALL) serve many lines from
+            # it, so only the lines of larger graphs are HBM lines for certain
+            lines = big = 0.0
+            for g, p, sd, f, cnt in qs:
+                k = min(cnt, 1 << 14)
+                x = Z.count_reads(g, p, sd, f, k) * cnt / k
+                lines += x
+                if g.device_bytes > (256 << 20):
+                    big += x
+            _, ms_dev = b.run(timed=True)
+            rec["lines_per_estimator"] = lines / rec["estimators"]
+            rec["replay_device_ms"] = ms_dev
+            rec["line_rate_gbs"] = lines * LINE / (ms_dev / 1e3) / 1e9
+            rec["hbm_lines_gbs"] = big * LINE / (ms_dev / 1e3) / 1e9
+            rec["hbm_lines_frac_of_peak"] = rec["hbm_lines_gbs"] / PEAK_HBM_GBS
+            rec["lines_on_graphs_over_256MB"] = big / lines if lines else 0.0
+            log(f"batch {label}: {rec['lines_per_estimator']:.2f} lines/estimator, device replay {ms_dev:.1f} ms, "
+                f"all lines {rec['line_rate_gbs']:.0f} GB/s, lines of graphs > 256 MB "
+                f"{rec['hbm_lines_gbs']:.0f} GB/s ({rec['hbm_lines_frac_of_peak']:.2f} of the HBM peak, "
+                f"{rec['lines_on_graphs_over_256MB']:.2f} of the lines)")
         if tune:
             t0 = time.perf_counter()
             cfg = b.autotune(trials=3)

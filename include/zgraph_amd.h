@@ -214,9 +214,10 @@ int zg_run_outcomes(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first
 /* ---- batched queries (one replayable HIP graph) ------------------------ *
  * No single reference counterpart: many ZGraphInstance::calculate calls
  * (src/ZGraphInstance.cpp:256-297), each a (graph, plan, seed, id range),
- * served together.  Per plan kind one persistent work-queue launch runs all
- * of that kind's queries; the whole batch is captured once in a HIP graph and
- * replayed by zg_batch_run.  Each query's sums equal zg_run's for the same
+ * served together.  One persistent work-queue launch runs the chunks of every
+ * query, whatever its plan kind: wavefronts claim chunks of estimator ids
+ * from one counter, queries laid out by decreasing cost per estimator; the
+ * whole batch is captured once in a HIP graph and replayed by zg_batch_run.  Each query's sums equal zg_run's for the same
  * (graph, plan, seed, first_id, count) exactly (sumsq up to rounding).  The
  * graphs and plans must outlive the batch; all queries run on device 0.
  * Output: one zg_sums per query and channel, in query order (a
@@ -230,7 +231,9 @@ typedef struct {
 typedef struct {
     int block_size;             /* threads per workgroup (64..256, multiple of 64; 0 = 256) */
     int blocks_per_cu;          /* persistent workgroups per CU (0 = 8) */
-    uint32_t chunk_estimators;  /* estimators per claimed chunk (256..2^24; 0 = 16384) */
+    uint32_t chunk_estimators;  /* estimator ids one wavefront claims at a time (a multiple of
+                                   64 in 64..2^24; 0 = 1024); a query with more than 16384 such
+                                   chunks takes proportionally longer ones */
 } zg_batch_config;
 typedef struct zg_batch zg_batch;
 int zg_batch_create(const zg_query *queries, int nq, const zg_batch_config *config, zg_batch **out);
@@ -239,8 +242,9 @@ int zg_batch_records(const zg_batch *b);
 int zg_batch_run(zg_batch *b, zg_sums *out, float *ms);
 /* new seeds and/or first ids (arrays of nq, either may be NULL) for later runs */
 int zg_batch_set_ids(zg_batch *b, const uint64_t *seeds, const uint64_t *first_ids);
-/* times replays over block size x workgroups per CU x chunk size (coordinate
- * descent, median of `trials`), keeps the fastest configuration */
+/* times replays over block size x workgroups per CU x estimators per
+ * wavefront claim {256, 1024, 4096, 16384} (coordinate descent, median of
+ * `trials`), keeps the fastest configuration */
 int zg_batch_autotune(zg_batch *b, int trials, zg_batch_config *chosen, float *best_ms);
 int zg_batch_config_get(const zg_batch *b, zg_batch_config *out);
 int zg_batch_destroy(zg_batch *b);

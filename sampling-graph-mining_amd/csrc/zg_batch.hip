@@ -1,21 +1,25 @@
 // zg_batch.hip -- many estimator queries (graph, plan, seed, id range) in one
-// replayable HIP graph: per plan kind, a counter reset, one persistent
-// work-queue launch over all of that kind's queries (zg_kernels.hip,
-// k_batch) and a per-query reduction; then one copy of every query's record
-// to pinned host memory.  The reference answers one query per
-// ZGraphInstance::calculate call (src/ZGraphInstance.cpp:256-297); serving
-// many small queries that way is bound by per-call launch and synchronisation
-// cost, which the captured graph pays once per batch.
+// replayable HIP graph: a counter reset, ONE persistent work-queue launch over
+// the chunks of every query of every plan kind (zg_kernels.hip, k_batch), a
+// per-query reduction, and one copy of every query's record to pinned host
+// memory.  The reference answers one query per ZGraphInstance::calculate call
+// (src/ZGraphInstance.cpp:256-297); serving many small queries that way is
+// bound by per-call launch and synchronisation cost, which the captured graph
+// pays once per batch, and a launch per plan kind would idle the GPU on each
+// kind's tail, which the single queue does not.
 //
+// Queries are laid out in the queue by decreasing estimated cost per
+// estimator (random lines each estimator reads, larger graphs first on ties),
+// so the long chunks are claimed first and the queue drains on short ones.
 // The autotuner picks the launch shape (threads per workgroup, persistent
-// workgroups per CU, estimators per claimed chunk) by timing graph replays;
-// every shape gives the same exact sums (only the floating-point sum of
-// squares can differ in its last bits, as the chunking changes its order).
+// workgroups per CU, estimators per wavefront per claim) by timing graph
+// replays; every shape gives the same exact sums (only the floating-point sum
+// of squares can differ in its last bits, as the chunking changes its order).
 #include <algorithm>
 #include <chrono>
 #include <cstring>
-#include <map>
 #include <memory>
+#include <numeric>
 #include <vector>
 
 #include "zg_internal.h"
@@ -23,29 +27,25 @@
 using namespace zg;
 
 namespace {
-constexpr uint64_t kMaxQueryCount = 1ull << 31;   // digit sums of one query stay below 2^63
+constexpr uint64_t kMaxQueryCount = 1ull << 31;      // digit sums of one query stay below 2^63
+constexpr uint64_t kMaxChunksPerQuery = 16384;       // a larger query takes longer chunks
+constexpr uint32_t kDefaultChunk = 1024;             // 16 estimators per lane per claim
 }
 
 struct zg_batch {
-    struct Group {
-        int kind = 0, nch = 1;
-        std::vector<int> qidx;          // positions in the caller's query array
-        std::vector<BQuery> hq;         // host copies of the descriptors
-        BQuery *dq = nullptr;
-        uint64_t nchunks = 0;
-        unsigned long long *counter = nullptr;
-        Acc *partials = nullptr;
-        size_t partials_cap = 0;        // in Acc records
-        size_t result_off = 0;          // first Acc record of this group in the result buffer
-    };
-    std::vector<Group> groups;
-    std::vector<zg_query> queries;
-    std::vector<size_t> rec_off;        // first output record of each query
+    std::vector<zg_query> queries;      // the caller's order
+    std::vector<int> order;             // queue position -> caller's index
+    std::vector<BQuery> hq;             // descriptors in queue order
+    std::vector<size_t> rec_off;        // first output record of each query (caller's order)
     size_t nrec = 0;
+    uint64_t nchunks = 0;
     zg_batch_config cfg{};
     hipStream_t stream = nullptr;
+    BQuery *dq = nullptr;
+    unsigned long long *counter = nullptr;
+    Acc *recs = nullptr;                // chunk records
+    size_t recs_cap = 0;
     Acc *d_result = nullptr, *h_result = nullptr;   // h_result is pinned
-    size_t nres = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -54,11 +54,9 @@ struct zg_batch {
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (auto &g : groups) {
-            if (g.dq) (void)hipFree(g.dq);
-            if (g.counter) (void)hipFree(g.counter);
-            if (g.partials) (void)hipFree(g.partials);
-        }
+        if (dq) (void)hipFree(dq);
+        if (counter) (void)hipFree(counter);
+        if (recs) (void)hipFree(recs);
         if (d_result) (void)hipFree(d_result);
         if (h_result) (void)hipHostFree(h_result);
         if (e0) (void)hipEventDestroy(e0);
@@ -74,14 +72,14 @@ int check_config(const zg_batch_config &c)
     if (c.block_size % 64 || c.block_size < 64 || c.block_size > 256)
         return set_error(ZG_EINVAL, "block_size must be 64, 128, 192 or 256");
     if (c.blocks_per_cu < 1 || c.blocks_per_cu > 64) return set_error(ZG_EINVAL, "blocks_per_cu must be in [1, 64]");
-    if (c.chunk_estimators < 256 || c.chunk_estimators > (1u << 24))
-        return set_error(ZG_EINVAL, "chunk_estimators must be in [256, 2^24]");
+    if (c.chunk_estimators < 64 || c.chunk_estimators > (1u << 24) || c.chunk_estimators % 64)
+        return set_error(ZG_EINVAL, "chunk_estimators must be a multiple of 64 in [64, 2^24]");
     return ZG_OK;
 }
 
 zg_batch_config defaults(const zg_batch_config *c)
 {
-    zg_batch_config d{256, 8, 16384};
+    zg_batch_config d{256, 8, kDefaultChunk};
     if (c) {
         if (c->block_size) d.block_size = c->block_size;
         if (c->blocks_per_cu) d.blocks_per_cu = c->blocks_per_cu;
@@ -90,39 +88,52 @@ zg_batch_config defaults(const zg_batch_config *c)
     return d;
 }
 
+// random 128-byte lines one estimator of the kernel kind reads (measured on
+// power-law graphs; only the queue order depends on them)
+double lines_per_estimator(const zg_plan *p)
+{
+    switch (p->desc.kind) {
+    case ZG_PLAN_THREE_CHAIN: return 2.0;
+    case ZG_PLAN_TRIANGLE: case ZG_PLAN_THREE_MOTIF: case ZG_PLAN_K4_CLOSE: case ZG_PLAN_TRIANGLE_K4: return 3.0;
+    case ZG_PLAN_FOUR_CHAIN: return 4.5;
+    case ZG_PLAN_CHAIN: return 1.0 + 2.3 * (p->desc.order - 2);
+    case ZG_PLAN_PATTERN: return 1.0 + 2.1 * (p->desc.pattern_edges - 1);
+    default: return 3.0;
+    }
+}
+
 // chunking, descriptor upload and graph capture for the batch's current config
 int configure(zg_batch *b)
 {
     if (b->exec) { ZG_HIP(hipGraphExecDestroy(b->exec)); b->exec = nullptr; }
     if (b->graph) { ZG_HIP(hipGraphDestroy(b->graph)); b->graph = nullptr; }
-    const uint64_t ch = b->cfg.chunk_estimators;
-    for (auto &g : b->groups) {
-        uint64_t c = 0;
-        for (auto &q : g.hq) {
-            q.chunk0 = c;
-            c += (q.count + ch - 1) / ch;
-        }
-        g.nchunks = c;
-        const size_t need = std::max<size_t>(1, (size_t)c * g.nch);
-        if (need > g.partials_cap) {
-            if (g.partials) ZG_HIP(hipFree(g.partials));
-            g.partials = nullptr;
-            if (hipMalloc(&g.partials, sizeof(Acc) * need) != hipSuccess)
-                return set_error(ZG_ENOMEM, "batch chunk records: device allocation failed");
-            g.partials_cap = need;
-        }
-        ZG_HIP(hipMemcpyAsync(g.dq, g.hq.data(), sizeof(BQuery) * g.hq.size(), hipMemcpyHostToDevice, b->stream));
+    uint64_t c = 0, r = 0;
+    for (auto &q : b->hq) {
+        const uint64_t need = (q.count + kMaxChunksPerQuery - 1) / kMaxChunksPerQuery;
+        q.chunk = std::max<uint64_t>(b->cfg.chunk_estimators, (need + 63) / 64 * 64);
+        const uint64_t nc = (q.count + q.chunk - 1) / q.chunk;
+        q.chunk0 = c;
+        q.rec0 = r;
+        c += nc;
+        r += nc * (uint64_t)q.P.nch;
     }
+    b->nchunks = c;
+    const size_t need = std::max<size_t>(1, (size_t)r);
+    if (need > b->recs_cap) {
+        if (b->recs) ZG_HIP(hipFree(b->recs));
+        b->recs = nullptr;
+        b->recs_cap = 0;
+        if (hipMalloc(&b->recs, sizeof(Acc) * need) != hipSuccess)
+            return set_error(ZG_ENOMEM, "batch chunk records: device allocation failed");
+        b->recs_cap = need;
+    }
+    ZG_HIP(hipMemcpyAsync(b->dq, b->hq.data(), sizeof(BQuery) * b->hq.size(), hipMemcpyHostToDevice, b->stream));
     ZG_HIP(hipStreamSynchronize(b->stream));
     const int grid = device_cus() * b->cfg.blocks_per_cu;
     ZG_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = ZG_OK;
-    for (auto &g : b->groups) {
-        rc = launch_batch(g.kind, g.dq, (int)g.hq.size(), g.nchunks, (uint32_t)ch, g.counter, g.partials,
-                          b->d_result + g.result_off, grid, b->cfg.block_size, b->stream);
-        if (rc) break;
-    }
-    if (!rc && hipMemcpyAsync(b->h_result, b->d_result, sizeof(Acc) * b->nres, hipMemcpyDeviceToHost, b->stream) !=
+    int rc = launch_batch(b->dq, (int)b->hq.size(), b->nchunks, b->counter, b->recs, b->d_result, grid,
+                          b->cfg.block_size, b->stream);
+    if (!rc && hipMemcpyAsync(b->h_result, b->d_result, sizeof(Acc) * b->nrec, hipMemcpyDeviceToHost, b->stream) !=
                    hipSuccess)
         rc = set_error(ZG_EHIP, "batch: result copy could not be captured");
     hipGraph_t graph = nullptr;
@@ -160,23 +171,26 @@ extern "C" int zg_batch_create(const zg_query *q, int nq, const zg_batch_config 
     std::unique_ptr<zg_batch> b(new zg_batch());
     b->cfg = cfg;
     b->queries.assign(q, q + nq);
-    std::map<int, size_t> by_kind;
+    std::vector<double> cost(nq);
     for (int i = 0; i < nq; ++i) {
         const zg_query &x = q[i];
         if (!x.graph || !x.plan) return set_error(ZG_EINVAL, "query without graph or plan");
         if (x.count > kMaxQueryCount) return set_error(ZG_EINVAL, "a batched query runs at most 2^31 estimators");
         if (x.count > ~0ull - x.first_id) return set_error(ZG_EINVAL, "estimator ids: first_id + count must not exceed 2^64 - 1");
         if (x.graph->m == 0 && x.count) return set_error(ZG_EINVAL, "query on an empty graph");
-        const int kind = x.plan->desc.kind == ZG_PLAN_PATTERN ? 100 + x.plan->desc.pattern_vertices
-                                                             : x.plan->desc.kind;   // one kernel per pattern size
-        auto it = by_kind.find(kind);
-        if (it == by_kind.end()) {
-            it = by_kind.emplace(kind, b->groups.size()).first;
-            b->groups.emplace_back();
-            b->groups.back().kind = kind;
-            b->groups.back().nch = x.plan->nch;
-        }
-        auto &g = b->groups[it->second];
+        cost[i] = lines_per_estimator(x.plan);
+        b->rec_off.push_back(b->nrec);
+        b->nrec += x.plan->nch;
+    }
+    b->order.resize(nq);
+    std::iota(b->order.begin(), b->order.end(), 0);
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) {
+        if (cost[a] != cost[c]) return cost[a] > cost[c];
+        return q[a].graph->m > q[c].graph->m;
+    });
+    uint32_t res = 0;
+    for (int i : b->order) {
+        const zg_query &x = q[i];
         BQuery d;
         std::memset(&d, 0, sizeof d);
         d.g = x.graph->view();
@@ -184,23 +198,16 @@ extern "C" int zg_batch_create(const zg_query *q, int nq, const zg_batch_config 
         d.seed = x.seed;
         d.first = x.first_id;
         d.count = x.count;
-        g.qidx.push_back(i);
-        g.hq.push_back(d);
-        b->rec_off.push_back(b->nrec);
-        b->nrec += x.plan->nch;
+        d.res0 = res;
+        d.kk = x.plan->desc.kind == ZG_PLAN_PATTERN ? 100 + x.plan->desc.pattern_vertices : x.plan->desc.kind;
+        res += (uint32_t)x.plan->nch;
+        b->hq.push_back(d);
     }
-    size_t off = 0;
-    for (auto &g : b->groups) {
-        g.result_off = off;
-        off += g.hq.size() * g.nch;
-        if (hipMalloc(&g.dq, sizeof(BQuery) * g.hq.size()) != hipSuccess ||
-            hipMalloc(&g.counter, sizeof(unsigned long long)) != hipSuccess)
-            return set_error(ZG_ENOMEM, "batch descriptors: device allocation failed");
-    }
-    b->nres = off;
-    if (hipMalloc(&b->d_result, sizeof(Acc) * off) != hipSuccess)
-        return set_error(ZG_ENOMEM, "batch results: device allocation failed");
-    ZG_HIP(hipHostMalloc(&b->h_result, sizeof(Acc) * off, hipHostMallocDefault));
+    if (hipMalloc(&b->dq, sizeof(BQuery) * nq) != hipSuccess ||
+        hipMalloc(&b->counter, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&b->d_result, sizeof(Acc) * b->nrec) != hipSuccess)
+        return set_error(ZG_ENOMEM, "batch descriptors: device allocation failed");
+    ZG_HIP(hipHostMalloc(&b->h_result, sizeof(Acc) * b->nrec, hipHostMallocDefault));
     ZG_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     ZG_HIP(hipEventCreate(&b->e0));
     ZG_HIP(hipEventCreate(&b->e1));
@@ -217,13 +224,12 @@ extern "C" int zg_batch_run(zg_batch *b, zg_sums *out, float *ms)
     if (!b || !out) return set_error(ZG_EINVAL, "null argument");
     int rc = replay(b, ms);
     if (rc) return rc;
-    for (auto &g : b->groups)
-        for (size_t j = 0; j < g.hq.size(); ++j) {
-            const int qi = g.qidx[j];
-            const zg_plan *pl = b->queries[qi].plan;
-            finish_sums(b->h_result + g.result_off + j * g.nch, g.nch, pl->channel_k(0), pl->channel_k(1), g.hq[j].count,
-                        out + b->rec_off[qi]);
-        }
+    for (size_t j = 0; j < b->hq.size(); ++j) {
+        const int qi = b->order[j];
+        const zg_plan *pl = b->queries[qi].plan;
+        finish_sums(b->h_result + b->hq[j].res0, pl->nch, pl->channel_k(0), pl->channel_k(1), b->hq[j].count,
+                    out + b->rec_off[qi]);
+    }
     for (size_t r = 0; r < b->nrec; ++r)
         if (out[r].status) return set_error(ZG_EOVERFLOW, "an estimator value exceeded 128 bits");
     return ZG_OK;
@@ -236,14 +242,12 @@ extern "C" int zg_batch_set_ids(zg_batch *b, const uint64_t *seeds, const uint64
         for (size_t qi = 0; qi < b->queries.size(); ++qi)
             if (b->queries[qi].count > ~0ull - first_ids[qi])
                 return set_error(ZG_EINVAL, "estimator ids: first_id + count must not exceed 2^64 - 1");
-    for (auto &g : b->groups) {
-        for (size_t j = 0; j < g.hq.size(); ++j) {
-            const int qi = g.qidx[j];
-            if (seeds) g.hq[j].seed = b->queries[qi].seed = seeds[qi];
-            if (first_ids) g.hq[j].first = b->queries[qi].first_id = first_ids[qi];
-        }
-        ZG_HIP(hipMemcpyAsync(g.dq, g.hq.data(), sizeof(BQuery) * g.hq.size(), hipMemcpyHostToDevice, b->stream));
+    for (size_t j = 0; j < b->hq.size(); ++j) {
+        const int qi = b->order[j];
+        if (seeds) b->hq[j].seed = b->queries[qi].seed = seeds[qi];
+        if (first_ids) b->hq[j].first = b->queries[qi].first_id = first_ids[qi];
     }
+    ZG_HIP(hipMemcpyAsync(b->dq, b->hq.data(), sizeof(BQuery) * b->hq.size(), hipMemcpyHostToDevice, b->stream));
     ZG_HIP(hipStreamSynchronize(b->stream));
     return ZG_OK;
 }
@@ -254,7 +258,7 @@ extern "C" int zg_batch_autotune(zg_batch *b, int trials, zg_batch_config *chose
     if (trials < 1) trials = 3;
     const int blocks[] = {64, 128, 256};
     const int bpcs[] = {2, 4, 8, 16};
-    const uint32_t chunks[] = {4096, 16384, 65536};
+    const uint32_t chunks[] = {256, 1024, 4096, 16384};   // estimators per wavefront per claim
     auto measure = [&](const zg_batch_config &c, float *t) -> int {
         b->cfg = c;
         int rc = configure(b);
@@ -274,10 +278,10 @@ extern "C" int zg_batch_autotune(zg_batch *b, int trials, zg_batch_config *chose
     float best_t = 0;
     int rc = measure(best, &best_t);
     if (rc) return rc;
-    // coordinate descent: chunk size, then workgroups per CU, then block size, twice
+    // coordinate descent: estimators per claim, then workgroups per CU, then block size, twice
     for (int pass = 0; pass < 2; ++pass) {
         for (int dim = 0; dim < 3; ++dim) {
-            const int n = dim == 0 ? 3 : dim == 1 ? 4 : 3;
+            const int n = dim == 0 ? 4 : dim == 1 ? 4 : 3;
             for (int k = 0; k < n; ++k) {
                 zg_batch_config c = best;
                 if (dim == 0) c.chunk_estimators = chunks[k];

@@ -34,7 +34,11 @@ struct BQuery {
     GView g;
     PlanView P;
     uint64_t seed, first, count;
-    uint64_t chunk0;   // index of the query's first chunk in its launch
+    uint64_t chunk0;   // index of the query's first chunk in the launch
+    uint64_t chunk;    // estimators per chunk (a multiple of 64)
+    uint64_t rec0;     // first chunk record (P.nch records per chunk)
+    uint32_t res0;     // first result record (P.nch records)
+    int kk;            // kernel kind: the plan kind, or 100 + k for k-vertex patterns
 };
 
 }  // namespace zg
@@ -111,9 +115,9 @@ int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
 constexpr uint64_t kMaxOrderedLaunch = 1ull << 28;
 int resolve_order(const zg_graph *g, const zg_plan *p, uint64_t count, int order);
 void finish_sums(const Acc *host_acc, int nch, int K0, int K1, uint64_t count, zg_sums *out);
-// one kernel kind's batched launch (the plan kind, or 100 + k for k-vertex
-// patterns): counter reset, work-queue kernel, per-query reduction
-int launch_batch(int kind, const BQuery *d_q, int nq, uint64_t nchunks, uint32_t chunk_est,
-                 unsigned long long *counter, Acc *chunk_partials, Acc *result, int grid, int block, hipStream_t st);
+// a batch's launches (queries in chunk order): counter reset, the work-queue
+// kernel over every query of every plan kind, the per-query reduction
+int launch_batch(const BQuery *d_q, int nq, uint64_t nchunks, unsigned long long *counter, Acc *chunk_recs,
+                 Acc *result, int grid, int block, hipStream_t st);
 
 }  // namespace zg

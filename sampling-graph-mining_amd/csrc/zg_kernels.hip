@@ -681,72 +681,144 @@ __global__ __launch_bounds__(256) void k_run_ordered(GView g, PlanView P, uint64
     block_reduce_store<NCH>(acc, partials + (size_t)blockIdx.x * NCH);
 }
 
-// Batched queries of one plan kind in one persistent launch.  The estimator
-// ids of every query are cut into chunks of `chunk_est`; workgroups claim
-// chunks from a global counter (a work queue) until it passes `nchunks`, and
-// write each chunk's reduced record to its own slot, so the per-query
-// reduction (k_batch_final) sums a fixed sequence: results do not depend on
-// which workgroup ran which chunk.
-template <int KIND, int NCH>
-__global__ __launch_bounds__(256) void k_batch(const BQuery *__restrict__ q, int nq, uint64_t nchunks,
-                                               uint32_t chunk_est, unsigned long long *__restrict__ counter,
-                                               Acc *__restrict__ chunk_partials)
+// Batched queries of every plan kind in one persistent launch.  Each query's
+// estimator ids are cut into chunks of Q.chunk ids; every wavefront claims
+// chunks from one global counter (a work queue) until it passes `nchunks`,
+// runs the chunk's estimators (Q.chunk / 64 per lane) with the kernel kind of
+// the chunk's query, and writes the chunk's reduced record to its own slot.
+// The per-query reduction (k_batch_final) sums each query's records in chunk
+// order, so results do not depend on which wavefront ran which chunk.  Queries
+// are laid out by decreasing cost per estimator, so the long chunks are
+// claimed first and the queue drains on short ones.  No workgroup barrier:
+// the waves of a workgroup run different chunks, of different kinds.
+__device__ __forceinline__ uint32_t uni32(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uni64(uint64_t x)
 {
-    __shared__ unsigned long long s_chunk;
-    __shared__ int s_q;
-    for (;;) {
-        if (threadIdx.x == 0) {
-            const unsigned long long c = atomicAdd(counter, 1ull);
-            s_chunk = c;
-            if (c < nchunks) {   // the last query whose first chunk is <= c
-                int lo = 0, hi = nq - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (q[mid].chunk0 <= c) lo = mid; else hi = mid - 1;
-                }
-                s_q = lo;
-            }
-        }
-        __syncthreads();
-        const unsigned long long c = s_chunk;
-        if (c >= nchunks) break;   // uniform: every wave of the workgroup leaves here
-        const BQuery &Q = q[s_q];
-        const GView g = Q.g;
-        const PlanView P = Q.P;
-        const uint64_t seed = Q.seed, first = Q.first;
-        const uint64_t base = (c - Q.chunk0) * chunk_est;
-        const uint64_t end = base + chunk_est < Q.count ? base + chunk_est : Q.count;
-        AccL acc[NCH];
-        acc_zero<NCH>(acc);
-        for (uint64_t i = base + threadIdx.x; i < end; i += blockDim.x) {
-            Rng rng;
-            rng.init(seed, TAG_ESTIMATOR, first + i);
-            Est o;
-            run_one<KIND>(g, P, rng, o);
+    return ((uint64_t)uni32((uint32_t)(x >> 32)) << 32) | uni32((uint32_t)x);
+}
+template <class T>
+__device__ __forceinline__ T *uniptr(T *p)
+{
+    return reinterpret_cast<T *>(uni64(reinterpret_cast<uint64_t>(p)));
+}
+
+// Not inlined: the kernel's register footprint is then the largest plan's,
+// not the union of every plan's (measured: 256 VGPRs inlined).  The query's
+// fields are read once and made wave-uniform (scalar registers), as a
+// kernel's arguments are; read as per-lane values they take ~30 VGPRs.
+template <int KIND, int NCH>
+__device__ __attribute__((noinline)) void batch_chunk(const BQuery *__restrict__ qp, uint64_t base, uint64_t end,
+                                                      Acc *__restrict__ rec)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    GView g;
+    g.edges = uniptr(qp->g.edges);
+    g.off = uniptr(qp->g.off);
+    g.adj = uniptr(qp->g.adj);
+    g.sadj = uniptr(qp->g.sadj);
+    g.htab = uniptr(qp->g.htab);
+    g.m = uni64(qp->g.m);
+    g.n = uni32(qp->g.n);
+    g.hnb = uni32(qp->g.hnb);
+    g.reads = nullptr;
+    PlanView P;
+    P.kind = (int)uni32((uint32_t)qp->P.kind);
+    P.order = (int)uni32((uint32_t)qp->P.order);
+    P.pk = (int)uni32((uint32_t)qp->P.pk);
+    P.pe = (int)uni32((uint32_t)qp->P.pe);
+    P.K = (int)uni32((uint32_t)qp->P.K);
+    P.nch = (int)uni32((uint32_t)qp->P.nch);
+    P.accept = uniptr(qp->P.accept);
+    P.K1 = (int)uni32((uint32_t)qp->P.K1);
+    const uint64_t seed = uni64(qp->seed), first = uni64(qp->first);
+    base = uni64(base);
+    end = uni64(end);
+    rec = uniptr(rec);
+    AccL acc[NCH];
+    acc_zero<NCH>(acc);
+    for (uint64_t i = base + lane; i < end; i += 64) {
+        Rng rng;
+        rng.init(seed, TAG_ESTIMATOR, first + i);
+        Est o;
+        run_one<KIND>(g, P, rng, o);
 #pragma unroll
-            for (int ch = 0; ch < NCH; ++ch) acc_add(acc[ch], o.v[ch], o.st[ch], o.den[ch], ch == 0 ? P.K : P.K1);
+        for (int ch = 0; ch < NCH; ++ch) acc_add(acc[ch], o.v[ch], o.st[ch], o.den[ch], ch == 0 ? P.K : P.K1);
+    }
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+        const AccW r = wave_reduce(acc[ch]);
+        if (lane == 0) {
+            Acc t;
+#pragma unroll
+            for (int d = 0; d < 5; ++d) t.dig[d] = r.dig[d];
+            t.succ = r.succ;
+            t.ovf = r.ovf;
+            t.sq = r.sq;
+            rec[ch] = t;
         }
-        block_reduce_store<NCH>(acc, chunk_partials + c * NCH);
-        __syncthreads();   // s_chunk and the reduction's LDS are reused next round
     }
 }
 
-// per-query reduction of the chunk records (one workgroup per query)
-__global__ __launch_bounds__(256) void k_batch_final(const BQuery *__restrict__ q, int nch,
-                                                     const Acc *__restrict__ chunk_partials,
+__device__ __forceinline__ uint64_t wave_uniform(uint64_t x)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_batch(
+    const BQuery *__restrict__ q, int nq, uint64_t nchunks,
+                                               unsigned long long *__restrict__ counter, Acc *__restrict__ recs)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    for (;;) {
+        unsigned long long c0 = 0;
+        if (lane == 0) c0 = atomicAdd(counter, 1ull);
+        const uint64_t c = wave_uniform(__shfl(c0, 0, 64));
+        if (c >= nchunks) break;   // uniform: the whole wavefront leaves, every wave reaches this
+        int lo = 0, hi = nq - 1;   // the last query whose first chunk is <= c
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (q[mid].chunk0 <= c) lo = mid; else hi = mid - 1;
+        }
+        const BQuery &Q = q[lo];
+        const uint64_t j = c - Q.chunk0;
+        const uint64_t base = j * Q.chunk;
+        const uint64_t end = base + Q.chunk < Q.count ? base + Q.chunk : Q.count;
+        Acc *const rec = recs + Q.rec0 + j * (uint64_t)Q.P.nch;
+        switch (Q.kk) {
+        case ZG_PLAN_TRIANGLE: batch_chunk<ZG_PLAN_TRIANGLE, 1>(&Q, base, end, rec); break;
+        case ZG_PLAN_THREE_CHAIN: batch_chunk<ZG_PLAN_THREE_CHAIN, 1>(&Q, base, end, rec); break;
+        case ZG_PLAN_THREE_MOTIF: batch_chunk<ZG_PLAN_THREE_MOTIF, 2>(&Q, base, end, rec); break;
+        case ZG_PLAN_CHAIN: batch_chunk<ZG_PLAN_CHAIN, 1>(&Q, base, end, rec); break;
+        case ZG_PLAN_FOUR_CHAIN: batch_chunk<ZG_PLAN_FOUR_CHAIN, 1>(&Q, base, end, rec); break;
+        case ZG_PLAN_K4_CLOSE: batch_chunk<ZG_PLAN_K4_CLOSE, 1>(&Q, base, end, rec); break;
+        case ZG_PLAN_TRIANGLE_K4: batch_chunk<ZG_PLAN_TRIANGLE_K4, 2>(&Q, base, end, rec); break;
+        case KIND_PATTERN_BASE + 2: batch_chunk<KIND_PATTERN_BASE + 2, 1>(&Q, base, end, rec); break;
+        case KIND_PATTERN_BASE + 3: batch_chunk<KIND_PATTERN_BASE + 3, 1>(&Q, base, end, rec); break;
+        case KIND_PATTERN_BASE + 4: batch_chunk<KIND_PATTERN_BASE + 4, 1>(&Q, base, end, rec); break;
+        case KIND_PATTERN_BASE + 5: batch_chunk<KIND_PATTERN_BASE + 5, 1>(&Q, base, end, rec); break;
+        default: batch_chunk<KIND_PATTERN_BASE + 6, 1>(&Q, base, end, rec); break;
+        }
+    }
+}
+
+// per-query reduction of the chunk records (one workgroup per query), in chunk order
+__global__ __launch_bounds__(256) void k_batch_final(const BQuery *__restrict__ q, const Acc *__restrict__ recs,
                                                      uint64_t nchunks, Acc *__restrict__ result)
 {
     __shared__ Acc sm[256];
     const int qi = blockIdx.x;
-    const uint64_t c0 = q[qi].chunk0;
-    const uint64_t c1 = (qi + 1 < (int)gridDim.x) ? q[qi + 1].chunk0 : nchunks;
+    const BQuery &Q = q[qi];
+    const int nch = Q.P.nch;
+    const uint64_t nc = ((qi + 1 < (int)gridDim.x) ? q[qi + 1].chunk0 : nchunks) - Q.chunk0;
     for (int c = 0; c < nch; ++c) {
         Acc t;
         for (int i = 0; i < 5; ++i) t.dig[i] = 0;
         t.succ = t.ovf = 0;
         t.sq = 0;
-        for (uint64_t b = c0 + threadIdx.x; b < c1; b += blockDim.x) {
-            const Acc &p = chunk_partials[b * nch + c];
+        for (uint64_t b = threadIdx.x; b < nc; b += blockDim.x) {
+            const Acc &p = recs[Q.rec0 + b * nch + c];
             for (int i = 0; i < 5; ++i) t.dig[i] += p.dig[i];
             t.succ += p.succ;
             t.ovf += p.ovf;
@@ -763,7 +835,7 @@ __global__ __launch_bounds__(256) void k_batch_final(const BQuery *__restrict__ 
             }
             __syncthreads();
         }
-        if (threadIdx.x == 0) result[(size_t)qi * nch + c] = sm[0];
+        if (threadIdx.x == 0) result[Q.res0 + c] = sm[0];
         __syncthreads();
     }
 }
@@ -941,35 +1013,12 @@ int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
     return ZG_OK;
 }
 
-template <int KIND, int NCH>
-static void launch_batch_kind(const BQuery *d_q, int nq, uint64_t nchunks, uint32_t chunk_est,
-                              unsigned long long *counter, Acc *chunk_partials, Acc *result, int grid, int block,
-                              hipStream_t st)
+int launch_batch(const BQuery *d_q, int nq, uint64_t nchunks, unsigned long long *counter, Acc *chunk_recs,
+                 Acc *result, int grid, int block, hipStream_t st)
 {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, counter, 1);
-    hipLaunchKernelGGL((k_batch<KIND, NCH>), dim3(grid), dim3(block), 0, st, d_q, nq, nchunks, chunk_est, counter,
-                       chunk_partials);
-    hipLaunchKernelGGL(k_batch_final, dim3(nq), dim3(256), 0, st, d_q, NCH, chunk_partials, nchunks, result);
-}
-
-int launch_batch(int kind, const BQuery *d_q, int nq, uint64_t nchunks, uint32_t chunk_est,
-                 unsigned long long *counter, Acc *chunk_partials, Acc *result, int grid, int block, hipStream_t st)
-{
-    switch (kind) {
-    case ZG_PLAN_TRIANGLE: launch_batch_kind<ZG_PLAN_TRIANGLE, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
-    case ZG_PLAN_THREE_CHAIN: launch_batch_kind<ZG_PLAN_THREE_CHAIN, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
-    case ZG_PLAN_THREE_MOTIF: launch_batch_kind<ZG_PLAN_THREE_MOTIF, 2>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
-    case ZG_PLAN_CHAIN: launch_batch_kind<ZG_PLAN_CHAIN, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
-    case ZG_PLAN_FOUR_CHAIN: launch_batch_kind<ZG_PLAN_FOUR_CHAIN, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
-    case KIND_PATTERN_BASE + 2: launch_batch_kind<KIND_PATTERN_BASE + 2, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
-    case KIND_PATTERN_BASE + 3: launch_batch_kind<KIND_PATTERN_BASE + 3, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
-    case KIND_PATTERN_BASE + 4: launch_batch_kind<KIND_PATTERN_BASE + 4, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
-    case KIND_PATTERN_BASE + 5: launch_batch_kind<KIND_PATTERN_BASE + 5, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
-    case KIND_PATTERN_BASE + 6: launch_batch_kind<KIND_PATTERN_BASE + 6, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
-    case ZG_PLAN_K4_CLOSE: launch_batch_kind<ZG_PLAN_K4_CLOSE, 1>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
-    case ZG_PLAN_TRIANGLE_K4: launch_batch_kind<ZG_PLAN_TRIANGLE_K4, 2>(d_q, nq, nchunks, chunk_est, counter, chunk_partials, result, grid, block, st); break;
-    default: return set_error(ZG_EINVAL, "unknown plan kind");
-    }
+    hipLaunchKernelGGL(k_batch, dim3(grid), dim3(block), 0, st, d_q, nq, nchunks, counter, chunk_recs);
+    hipLaunchKernelGGL(k_batch_final, dim3(nq), dim3(256), 0, st, d_q, chunk_recs, nchunks, result);
     ZG_HIP(hipGetLastError());
     return ZG_OK;
 }

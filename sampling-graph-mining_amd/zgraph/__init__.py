@@ -290,7 +290,9 @@ def run(graph, plan, seed, first_id, count, block_size=0, blocks_per_cu=0, timed
 
 class Batch:
     """Many (graph, plan, seed, first_id, count) queries served by one
-    replayable HIP graph (one persistent work-queue launch per plan kind).
+    replayable HIP graph: one persistent work-queue launch over the chunks of
+    every query, whatever its plan.  chunk_estimators is the number of
+    estimator ids one wavefront claims at a time (a multiple of 64).
     run() returns, per query, the list of per-channel sums -- equal to
     run(graph, plan, seed, first_id, count) for that query."""
 
@@ -317,8 +319,13 @@ class Batch:
         return (res, ms.value) if timed else res
 
     def set_ids(self, seeds=None, first_ids=None):
+        """New seeds and/or first ids for later runs: 1-D sequences of one
+        entry per query (the library reads exactly nq entries)."""
         s = np.ascontiguousarray(seeds, np.uint64) if seeds is not None else None
         f = np.ascontiguousarray(first_ids, np.uint64) if first_ids is not None else None
+        for name, a in (("seeds", s), ("first_ids", f)):
+            if a is not None and (a.ndim != 1 or a.shape[0] != self.nq):
+                raise ValueError(f"{name} must be a 1-D array of {self.nq} entries, got shape {a.shape}")
         check(lib().zg_batch_set_ids(self.h, s.ctypes.data if s is not None else None,
                                      f.ctypes.data if f is not None else None))
 

@@ -1,6 +1,7 @@
-"""Batched queries (zg_batch_*): one replayable HIP graph of persistent
-work-queue launches must give every query exactly the sums of its own
-zg_run, for any launch shape, across replays and seed updates."""
+"""Batched queries (zg_batch_*): one replayable HIP graph around one
+persistent work-queue launch over every query's chunks must give every query
+exactly the sums of its own zg_run, for any launch shape, across replays and
+seed updates."""
 import numpy as np
 import pytest
 
@@ -77,7 +78,8 @@ def test_batch_equals_single_runs(graphs, plans):
 
 @pytest.mark.parametrize("cfg", [dict(block_size=64, blocks_per_cu=2, chunk_estimators=256),
                                  dict(block_size=128, blocks_per_cu=16, chunk_estimators=65536),
-                                 dict(block_size=256, blocks_per_cu=1, chunk_estimators=4096)])
+                                 dict(block_size=256, blocks_per_cu=1, chunk_estimators=4096),
+                                 dict(block_size=192, blocks_per_cu=3, chunk_estimators=64)])
 def test_batch_launch_shapes_agree(graphs, plans, cfg):
     qs = _queries(graphs, plans)[:16]
     say(f"[batch] config {cfg}")
@@ -102,7 +104,7 @@ def test_batch_set_ids_and_autotune(graphs, plans):
     cfg = b.autotune(trials=1)
     say(f"[batch] autotuned {cfg}")
     assert cfg["block_size"] in (64, 128, 256) and cfg["blocks_per_cu"] in (2, 4, 8, 16)
-    assert cfg["chunk_estimators"] in (4096, 16384, 65536)
+    assert cfg["chunk_estimators"] in (256, 1024, 4096, 16384)
     assert b.config() == {k: cfg[k] for k in ("block_size", "blocks_per_cu", "chunk_estimators")}
     res2 = b.run()
     for r, a in zip(res, res2):
@@ -115,3 +117,10 @@ def test_batch_rejects_bad_queries(graphs, plans):
         Z.Batch([(graphs[0], plans["tri"], 1, 0, (1 << 31) + 1)])
     with pytest.raises(Z.ZGraphError):
         Z.Batch([(graphs[0], plans["tri"], 1, 0, 10)], block_size=96)
+    with pytest.raises(Z.ZGraphError):   # estimators per claim: a multiple of 64
+        Z.Batch([(graphs[0], plans["tri"], 1, 0, 10)], chunk_estimators=1000)
+    b = Z.Batch([(graphs[0], plans["tri"], 1, 0, 10), (graphs[1], plans["k4"], 2, 0, 10)])
+    with pytest.raises(ValueError):
+        b.set_ids(seeds=[1])
+    with pytest.raises(ValueError):
+        b.set_ids(first_ids=5)
