@@ -42,6 +42,9 @@ int zg_last_error(char *buf, size_t len);
 int zg_init(void);
 /* Device properties used by the launch heuristics: CU count, LDS per CU. */
 int zg_device_info(int *num_cus, int *lds_per_cu, int *warp_size);
+/* Free and total device memory of the library's device, in bytes (no
+ * reference counterpart: callers size graphs to HBM with it). */
+int zg_device_memory(uint64_t *free_bytes, uint64_t *total_bytes);
 
 /* ---- graphs ------------------------------------------------------------ *
  * Replaces DistGraphLoader<E>::load_graph + CSRConstructor<E>::
@@ -67,6 +70,7 @@ typedef struct {
     uint64_t max_degree;   /* graph.max_degree */
     uint64_t device_bytes; /* HBM held by the handle */
     int has_weights;
+    int has_parallel_edges; /* some vertex pair is joined by more than one edge */
 } zg_graph_info;
 int zg_graph_info_get(const zg_graph *g, zg_graph_info *out);
 
@@ -267,8 +271,9 @@ int zg_count_reads(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_
  * the sum over edges of d_a * d_c: meant for graphs up to a few million edges),
  * 6 5-vertex chains (FiveChain.cpp:45-72, simple paths of four edges), 7
  * 4-cycles (no reference program; the 5-chain count needs them).  Counts
- * that exceed 64 bits fail with ZG_EOVERFLOW (5-stars, 5-chains) or wrap
- * (4-cycles, not reached below about 10^18).  Scratch: 4-cliques work in vertex ranges
+ * that exceed 64 bits fail with ZG_EOVERFLOW (5-stars, 5-chains; 4-cycles are
+ * summed in 128 bits and checked too, though at most C(m, 2) < 2^61 of them
+ * exist for m < 2^31).  Scratch: 4-cliques work in vertex ranges
  * whose bit rows fit min(16 GiB, max(1 GiB, free / 4)) (the environment
  * variable ZGRAPH_K4_SCRATCH_WORDS lowers it, in 8-byte words); 4-cycles and
  * 5-chains sort the hubs' wedges in batches of up to budget / 24 wedges
@@ -285,6 +290,15 @@ int zg_graph_degree_moments(zg_graph *g, double *sum_d2, double *sum_d3);
  * defaults).  *gbs counts one 128-byte line per read. */
 int zg_random_gather_bw(uint64_t buffer_bytes, int chains, uint64_t loads, int block_size, int blocks_per_cu,
                         double *gbs, double *ms);
+/* Achievable rate of the triangle estimator's access chain: `estimators`
+ * chains of three dependent random reads -- a 32-byte record (two 16-byte
+ * loads) of an m-record array, an 8-byte entry of a 2m-entry array at an
+ * index taken from the record, a 16-byte header of one of m/4 128-byte
+ * buckets at an index taken from the entry -- over fresh buffers of the
+ * graph's array sizes, no estimator logic.  *gbs counts 3 lines of 128 bytes
+ * per chain: the denominator matched to an estimator kernel's line traffic. */
+int zg_random_chain_bw(uint64_t m, uint64_t estimators, int block_size, int blocks_per_cu, double *gbs,
+                       double *ms);
 /* Random-line read rate by load shape: `lines` random 128-byte lines of a
  * fresh buffer of `buffer_bytes`, one dependent chain per reader.  With
  * cooperative = 0 every lane reads `loads_per_line` (1..8) consecutive 16-byte

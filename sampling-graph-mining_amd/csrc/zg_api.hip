@@ -93,6 +93,17 @@ extern "C" int zg_device_info(int *num_cus, int *lds_per_cu, int *warp_size)
     return ZG_OK;
 }
 
+extern "C" int zg_device_memory(uint64_t *free_bytes, uint64_t *total_bytes)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    size_t f = 0, t = 0;
+    ZG_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return ZG_OK;
+}
+
 // ---- graphs ----------------------------------------------------------------
 extern "C" int zg_graph_create(const uint32_t *src, const uint32_t *dst, const float *weights, uint64_t m,
                                uint32_t n, uint64_t shuffle_seed, zg_graph **out)
@@ -154,6 +165,7 @@ extern "C" int zg_graph_info_get(const zg_graph *g, zg_graph_info *out)
     out->max_degree = g->max_degree;
     out->device_bytes = g->bytes;
     out->has_weights = g->w != nullptr;
+    out->has_parallel_edges = g->simple ? 0 : 1;
     return ZG_OK;
 }
 

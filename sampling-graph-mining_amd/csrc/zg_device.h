@@ -68,6 +68,8 @@ struct GView {
     uint64_t m;
     uint32_t n;
     uint32_t hnb;                // number of buckets
+    uint32_t simple;             // 1: no pair of vertices is joined twice (no parallel edges)
+    uint32_t pad_;
     unsigned long long *reads;   // non-null only in the read-counting launch
 };
 

@@ -171,6 +171,27 @@ __device__ __forceinline__ unsigned long long block_sum(unsigned long long v)
     return t;
 }
 
+// (lo, hi) summed over the workgroup in 128 bits; thread 0 writes the pair to
+// part[2 * blockIdx.x], part[2 * blockIdx.x + 1]
+__device__ __forceinline__ void block_sum128_store(unsigned long long lo, unsigned long long hi,
+                                                   unsigned long long *part)
+{
+    __shared__ unsigned long long sl[4], sh[4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long ol = __shfl_xor(lo, off, 64), oh = __shfl_xor(hi, off, 64);
+        add128(lo, hi, ol, oh);
+    }
+    if ((threadIdx.x & 63) == 0) { sl[threadIdx.x >> 6] = lo; sh[threadIdx.x >> 6] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long L = 0, H = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) add128(L, H, sl[w], sh[w]);
+        part[2 * blockIdx.x] = L;
+        part[2 * blockIdx.x + 1] = H;
+    }
+}
+
 // Entries of a[0, na) found in b[0, nb), counted per lane: each lane takes two
 // entries of a per pass and runs their binary searches interleaved, so two
 // dependent chains of loads are in flight.  With WEIGHT, *wsum also gets
@@ -627,7 +648,7 @@ __global__ __launch_bounds__(256) void k_c4_lds(const uint64_t *foff, const uint
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    unsigned long long c = 0;
+    unsigned long long clo = 0, chi = 0;
     WAVE_STRIDE(u, (uint64_t)n)
     {
         const uint64_t ub = foff[u], ku = ((uint64_t)sdeg[u] << 32) | u;
@@ -665,7 +686,7 @@ __global__ __launch_bounds__(256) void k_c4_lds(const uint64_t *foff, const uint
         if (!full[wv]) {
             for (uint32_t t = lane; t < kC4Slots; t += 64) {
                 const unsigned long long x = V[t];
-                c += x * (x - (x > 0)) / 2;
+                add128(clo, chi, x * (x - (x > 0)) / 2, 0);
             }
         } else if (lane == 0) {
             ovf_list[atomicAdd(ovf_count, 1u)] = (uint32_t)u;
@@ -679,8 +700,7 @@ __global__ __launch_bounds__(256) void k_c4_lds(const uint64_t *foff, const uint
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
-    const unsigned long long t = block_sum(c);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    block_sum128_store(clo, chi, partial);
 }
 
 // For each listed vertex: the number of its neighbours below it, and its
@@ -744,14 +764,13 @@ __global__ __launch_bounds__(256) void k_c4s_fill(const uint64_t *foff, const ui
 __global__ void k_c4s_pairs(const unsigned int *counts, const unsigned int *nruns, unsigned long long *partial)
 {
     const uint64_t runs = *nruns;
-    unsigned long long c = 0;
+    unsigned long long lo = 0, hi = 0;
     GRID_STRIDE(t, runs)
     {
         const unsigned long long x = counts[t];
-        c += x * (x - 1) / 2;
+        add128(lo, hi, x * (x - 1) / 2, 0);
     }
-    const unsigned long long t = block_sum(c);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    block_sum128_store(lo, hi, partial);
 }
 
 __global__ void k_c4d_clear(uint32_t *cnt, uint64_t total)
@@ -790,14 +809,13 @@ __global__ __launch_bounds__(256) void k_c4d_insert(const uint64_t *foff, const 
 
 __global__ void k_c4d_sum(const uint32_t *cnt, uint64_t total, unsigned long long *partial)
 {
-    unsigned long long c = 0;
+    unsigned long long lo = 0, hi = 0;
     GRID_STRIDE(t, total)
     {
         const unsigned long long x = cnt[t];
-        c += x * (x - (x > 0)) / 2;
+        add128(lo, hi, x * (x - (x > 0)) / 2, 0);
     }
-    const unsigned long long t = block_sum(c);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    block_sum128_store(lo, hi, partial);
 }
 
 // sum over vertices v of S1^2 - S2, S1 = sum of (d_x - 1) and S2 = sum of
@@ -834,6 +852,18 @@ __global__ __launch_bounds__(256) void k_c5_pairs(const uint64_t *foff, const ui
         partial[2 * wave] = lo;
         partial[2 * wave + 1] = hi;
     }
+}
+
+// sum of n (lo, hi) pairs written by block_sum128_store
+int host_sum128(const unsigned long long *d, int n, hipStream_t st, unsigned __int128 *out)
+{
+    std::vector<unsigned long long> h(2 * (size_t)n);
+    ZG_HIP(hipMemcpyAsync(h.data(), d, 16ull * n, hipMemcpyDeviceToHost, st));
+    ZG_HIP(hipStreamSynchronize(st));
+    unsigned __int128 s = 0;
+    for (int i = 0; i < n; ++i) s += ((unsigned __int128)h[2 * i + 1] << 64) | h[2 * i];
+    *out = s;
+    return ZG_OK;
 }
 
 int host_sum(const unsigned long long *d, int n, hipStream_t st, unsigned long long *out)
@@ -1007,13 +1037,13 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         XB<unsigned long long> cpart;
         XB<uint32_t> ovf;
         XB<unsigned int> novf_d;
-        XALLOC(cpart, cgrid); XALLOC(ovf, n); XALLOC(novf_d, 1);
+        XALLOC(cpart, 2 * (uint64_t)cgrid); XALLOC(ovf, n); XALLOC(novf_d, 1);
         ZG_HIP(hipMemsetAsync(novf_d.p, 0, sizeof(unsigned int), st));
         hipLaunchKernelGGL(k_c4_lds, dim3(cgrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p, n, cpart.p, ovf.p,
                            novf_d.p);
         ZG_HIP(hipGetLastError());
-        unsigned long long c4 = 0;
-        int rc = host_sum(cpart.p, cgrid, st, &c4);
+        unsigned __int128 c4 = 0;   // 4-cycles, summed in 128 bits (see the bound at the end)
+        int rc = host_sum128(cpart.p, cgrid, st, &c4);
         if (rc) return rc;
         unsigned int novf = 0;
         ZG_HIP(hipMemcpyAsync(&novf, novf_d.p, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
@@ -1079,7 +1109,7 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                 XALLOC(tmps, std::max(ts, tr));
                 const int pgrid = gridx(tmax);
                 XB<unsigned long long> spart;
-                XALLOC(spart, pgrid);
+                XALLOC(spart, 2 * (uint64_t)pgrid);
                 for (const auto &b : batches) {
                     const uint32_t nb = b.second - b.first;
                     std::vector<uint64_t> ipre(nb + 1, 0), cur(nb, 0);
@@ -1128,8 +1158,8 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                     const int g2 = gridx(tot);
                     hipLaunchKernelGGL(k_c4s_pairs, dim3(g2), dim3(256), 0, st, counts.p, nruns.p, spart.p);
                     ZG_HIP(hipGetLastError());
-                    unsigned long long g4 = 0;
-                    rc = host_sum(spart.p, g2, st, &g4);   // synchronizes before ipre and cur change
+                    unsigned __int128 g4 = 0;
+                    rc = host_sum128(spart.p, g2, st, &g4);   // synchronizes before ipre and cur change
                     if (rc) return rc;
                     c4 += g4;
                 }
@@ -1147,7 +1177,7 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                     if (hwedge[i] > cap) dbelow[k++] = hbelow[i];
                 const int sgrid = gridx((uint64_t)per * n);
                 XB<unsigned long long> gpart;
-                XALLOC(gpart, sgrid);
+                XALLOC(gpart, 2 * (uint64_t)sgrid);
                 for (uint32_t i0 = 0; i0 < nd; i0 += per) {
                     const uint32_t nb = std::min<uint32_t>(per, nd - i0);
                     std::vector<uint64_t> ipre(nb + 1, 0);
@@ -1164,14 +1194,18 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                     const int g2 = gridx(total);
                     hipLaunchKernelGGL(k_c4d_sum, dim3(g2), dim3(256), 0, st, cnt.p, total, gpart.p);
                     ZG_HIP(hipGetLastError());
-                    unsigned long long g4 = 0;
-                    rc = host_sum(gpart.p, g2, st, &g4);   // synchronizes before ipre changes
+                    unsigned __int128 g4 = 0;
+                    rc = host_sum128(gpart.p, g2, st, &g4);   // synchronizes before ipre changes
                     if (rc) return rc;
                     c4 += g4;
                 }
             }
         }
-        if (what == 7) { *out = c4; return ZG_OK; }
+        // A simple graph with m edges has at most C(m, 2) 4-cycles (each holds two pairs of
+        // disjoint edges, and each such pair closes at most two 4-cycles), so with 2m < 2^32
+        // the count stays below 2^61; the check guards the summation itself.
+        if (c4 >> 64) return set_error(ZG_EOVERFLOW, "4-cycle count exceeds 64 bits");
+        if (what == 7) { *out = (uint64_t)c4; return ZG_OK; }
         // 5-vertex chains (FiveChain.cpp semantics): 2P = sum_v (S1^2 - S2)
         // - 4 sum_v (d_v - 1) t_v + 6 T - 8 C4
         const int tgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16, std::max<uint64_t>(1, (ne + 3) / 4));

@@ -130,6 +130,15 @@ __global__ void k_sadj(const uint64_t *skey, const uint32_t *sval, uint64_t n2, 
     GRID_STRIDE(i, n2) { sadj[i] = ((skey[i] & 0xffffffffull) << 32) | sval[i]; }
 }
 
+// any two equal consecutive (owner, neighbour) keys: a pair joined by parallel edges
+__global__ void k_parallel_flag(const uint64_t *skey, uint64_t n2, uint32_t *flag)
+{
+    GRID_STRIDE(i, n2 - 1)
+    {
+        if (skey[i] == skey[i + 1]) *flag = 1u;
+    }
+}
+
 // edge-set hash: one slot per rank.  A bucket's slots are claimed with an
 // atomic add on its fill count; the claimant writes the slot and then ORs its
 // tag into the header.  A pair whose bucket is full marks it as passed over
@@ -322,7 +331,16 @@ int build_graph(const uint32_t *d_src, const uint32_t *d_dst, const float *d_w, 
         ZG_HIP(rocprim::radix_sort_pairs(tmp.p, tb, kb, vb, n2, 0, eb, st));
         if (n2) hipLaunchKernelGGL(k_sadj, dim3(grid_for(n2)), dim3(256), 0, st, kb.current(), vb.current(), n2, g->sadj);
         ZG_HIP(hipGetLastError());
+        DevBuf<uint32_t> par;
+        ZG_ALLOC(par, 1);
+        ZG_HIP(hipMemsetAsync(par.p, 0, 4, st));
+        if (n2 > 1)
+            hipLaunchKernelGGL(k_parallel_flag, dim3(grid_for(n2 - 1)), dim3(256), 0, st, kb.current(), n2, par.p);
+        ZG_HIP(hipGetLastError());
+        uint32_t hpar = 0;
+        ZG_HIP(hipMemcpyAsync(&hpar, par.p, 4, hipMemcpyDeviceToHost, st));
         ZG_HIP(hipStreamSynchronize(st));
+        g->simple = hpar == 0;
     }
     // 6. edge-set hash for the closing checks (4 edges per 8-slot bucket on average)
     {

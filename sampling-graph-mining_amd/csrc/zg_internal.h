@@ -51,6 +51,7 @@ struct zg_graph {
     float *w = nullptr;
     zg::HBucket *htab = nullptr;
     uint32_t hnb = 0;
+    bool simple = true;           // no parallel edges (set by the build)
     uint64_t bytes = 0;
     hipStream_t stream = nullptr;
     zg::Acc *partials = nullptr;  // scratch for block partials
@@ -64,7 +65,7 @@ struct zg_graph {
     size_t ord_tmp_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-    zg::GView view() const { return zg::GView{edges, off, adj, sadj, htab, m, n, hnb, nullptr}; }
+    zg::GView view() const { return zg::GView{edges, off, adj, sadj, htab, m, n, hnb, simple ? 1u : 0u, 0u, nullptr}; }
 };
 
 struct zg_plan {

@@ -105,14 +105,21 @@ __device__ __forceinline__ void plan_wedge(const GView &g, Rng &rng, Est &o)
     o.ncond = 1;
     if (s == 0) return;
     const uint64_t r = rng.uniform(s);
+    U128 val{g.m, 0};
+    const bool ov = mul_u128(val, s);
+    // 3-chain on a graph without parallel edges: the sampled entry's neighbour is
+    // neither endpoint of e0 (that would be a second copy of e0 or a self-loop),
+    // so the denominator is 1 and the entry need not be read
+    if (MODE == 1 && g.simple) {
+        set_out(o, 0, val, ov, 0);
+        return;
+    }
     const bool inA = r < ca;
     const uint64_t idx = inA ? (uint64_t)A.off + pa + r : (uint64_t)B.off + pb + (r - ca);
     const uint64_t ent = g.adj[idx];
     count_reads(g, 1);
     const uint32_t r1 = (uint32_t)(ent >> 32), w = (uint32_t)ent, own = inA ? E.src : E.dst;
     const int den = (w == E.src || w == E.dst) ? 1 : 0;
-    U128 val{g.m, 0};
-    const bool ov = mul_u128(val, s);
     if (MODE != 0) set_out(o, 0, val, ov, den);
     if (MODE == 1) return;
     // third edge of Triangle(e0, e1) (applications/Triangle.cpp:17-40)
@@ -720,6 +727,8 @@ __device__ __attribute__((noinline)) void batch_chunk(const BQuery *__restrict__
     g.m = uni64(qp->g.m);
     g.n = uni32(qp->g.n);
     g.hnb = uni32(qp->g.hnb);
+    g.simple = uni32(qp->g.simple);
+    g.pad_ = 0;
     g.reads = nullptr;
     PlanView P;
     P.kind = (int)uni32((uint32_t)qp->P.kind);

@@ -6,6 +6,8 @@
 // `chains` independent dependent-load chains (the next address hashes the
 // loaded word), the shape of an estimator's read sequence.  Measurement
 // helper only; no reference counterpart.
+#include <algorithm>
+
 #include "zg_internal.h"
 
 namespace zg {
@@ -73,10 +75,95 @@ __global__ __launch_bounds__(256) void k_mb_lines_coop(const uint4 *buf, uint64_
     if (x == 0x0123456789ABCDEFull) sink[0] = x;
 }
 
+// One estimator's access chain without the estimator: a random 32-byte record
+// (two 16-byte loads), then an 8-byte entry of a second array at an index
+// taken from the record, then the 16-byte header of a 128-byte bucket of a
+// third array at an index taken from the entry -- the three dependent random
+// reads of a triangle estimator (edge record, sampled list entry, hash
+// bucket), over arrays of the graph's sizes.  Grid-stride over estimator
+// indices, as the estimator kernels are.
+__global__ __launch_bounds__(256) void k_mb_chain(const uint4 *__restrict__ rec, uint64_t nrec,
+                                                  const uint64_t *__restrict__ ent, uint64_t nent,
+                                                  const uint4 *__restrict__ bkt, uint64_t nbkt, uint64_t count,
+                                                  uint64_t *sink)
+{
+    uint64_t acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t x = hmix(i + 1);
+        const uint4 *r = rec + 2 * __umul64hi(x, nrec);
+        const uint4 a = r[0], b = r[1];
+        const uint64_t e = ent[__umul64hi(hmix(x ^ a.x ^ ((uint64_t)b.w << 32)), nent)];
+        const uint4 h = bkt[8 * __umul64hi(hmix(x ^ e), nbkt)];
+        // a full 64-bit fold: the sentinel compare below must stay possible, or
+        // the compiler proves it false and drops every load
+        // (every field is used, so the record and header are read with 16-byte loads, as the
+        // estimator kernels read them)
+        acc += (((uint64_t)h.w << 32) | h.x) ^ (((uint64_t)h.y << 32) | h.z) ^ e ^
+               (((uint64_t)a.w << 32) | a.z) ^ (((uint64_t)b.x << 32) | b.y) ^ (((uint64_t)b.w << 32) | a.y) ^ b.z;
+    }
+    if (acc == 0x0123456789ABCDEFull) sink[0] = acc;   // keeps the loads live
+}
+
 }  // namespace
 }  // namespace zg
 
 using namespace zg;
+
+extern "C" int zg_random_chain_bw(uint64_t m, uint64_t estimators, int block_size, int blocks_per_cu, double *gbs,
+                                  double *ms)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!gbs || m < 64 || estimators == 0) return set_error(ZG_EINVAL, "bad chain benchmark arguments");
+    const int bs = block_size > 0 ? block_size : 256;
+    if (bs > 256 || bs % 64) return set_error(ZG_EINVAL, "block size must be a multiple of 64 up to 256");
+    const int bpc = blocks_per_cu > 0 ? blocks_per_cu : 16;
+    const uint64_t grid = std::min<uint64_t>((uint64_t)device_cus() * bpc, (estimators + bs - 1) / bs);
+    // the graph's arrays: m 32-byte records, 2m 8-byte entries, m/4 128-byte buckets
+    const uint64_t nrec = m, nent = 2 * m, nbkt = (m + 3) / 4;
+    uint4 *rec = nullptr, *bkt = nullptr;
+    uint64_t *ent = nullptr, *sink = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto run = [&]() -> int {
+        if (hipMalloc(&rec, 32 * nrec) != hipSuccess || hipMalloc(&ent, 8 * nent) != hipSuccess ||
+            hipMalloc(&bkt, 128 * nbkt) != hipSuccess)
+            return set_error(ZG_ENOMEM, "chain benchmark buffers: allocation failed");
+        ZG_HIP(hipMalloc(&sink, 8));
+        ZG_HIP(hipEventCreate(&e0));
+        ZG_HIP(hipEventCreate(&e1));
+        hipLaunchKernelGGL(k_mb_fill, dim3(device_cus() * 64), dim3(256), 0, 0, reinterpret_cast<uint64_t *>(rec),
+                           4 * nrec);
+        hipLaunchKernelGGL(k_mb_fill, dim3(device_cus() * 64), dim3(256), 0, 0, ent, nent);
+        hipLaunchKernelGGL(k_mb_fill, dim3(device_cus() * 64), dim3(256), 0, 0, reinterpret_cast<uint64_t *>(bkt),
+                           16 * nbkt);
+        ZG_HIP(hipGetLastError());
+        auto launch = [&]() {
+            hipLaunchKernelGGL(k_mb_chain, dim3(grid), dim3(bs), 0, 0, rec, nrec, ent, nent, bkt, nbkt, estimators,
+                               sink);
+        };
+        launch();   // warm-up
+        ZG_HIP(hipGetLastError());
+        ZG_HIP(hipEventRecord(e0, 0));
+        launch();
+        ZG_HIP(hipGetLastError());
+        ZG_HIP(hipEventRecord(e1, 0));
+        ZG_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        ZG_HIP(hipEventElapsedTime(&t, e0, e1));
+        *gbs = (double)estimators * 3 * 128.0 / (t * 1e-3) / 1e9;
+        if (ms) *ms = t;
+        return ZG_OK;
+    };
+    rc = run();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (sink) (void)hipFree(sink);
+    if (rec) (void)hipFree(rec);
+    if (ent) (void)hipFree(ent);
+    if (bkt) (void)hipFree(bkt);
+    return rc;
+}
 
 extern "C" int zg_random_gather_bw(uint64_t buffer_bytes, int chains, uint64_t loads, int block_size,
                                    int blocks_per_cu, double *gbs, double *ms)

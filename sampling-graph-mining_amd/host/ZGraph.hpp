@@ -828,6 +828,7 @@ class TimeProfile {
     {
         (void)num_vertices_in_pattern;
         Estimation acc;
+        batches_.clear();
         unsigned long long used = 0, cur = 0, next = basic_, first = 0;
         while (used + cur * 2 <= time_limit) {
             const auto t0 = std::chrono::steady_clock::now();
@@ -835,7 +836,9 @@ class TimeProfile {
             cur = (unsigned long long)std::chrono::duration_cast<std::chrono::milliseconds>(
                       std::chrono::steady_clock::now() - t0).count();
             used += cur;
+            acc.absorb(s);
             acc.update(next, s.mean);
+            batches_.push_back({first, next});
             first += next;
             next <<= 1;
         }
@@ -843,13 +846,21 @@ class TimeProfile {
             const unsigned long long rest = next * (time_limit - used) / (cur * 2);
             if (rest) {
                 const zg_sums s = inst.run_ids(first, rest);
+                acc.absorb(s);
                 acc.update(rest, s.mean);
+                batches_.push_back({first, rest});
             }
         }
+        last_ = acc;
         return acc.get_currnet_estimation();
     }
+    // the last run: its Estimation (running mean, exact sums) and its batches (first id, count)
+    const Estimation &last_estimation() const { return last_; }
+    const std::vector<std::pair<unsigned long long, unsigned long long>> &batches() const { return batches_; }
 
   private:
+    Estimation last_;
+    std::vector<std::pair<unsigned long long, unsigned long long>> batches_;
     unsigned long long basic_;
 };
 

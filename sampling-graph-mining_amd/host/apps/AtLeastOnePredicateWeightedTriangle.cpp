@@ -27,7 +27,16 @@ int main(int argc, char **argv)
             report("at-least-one-predicate-triangle", cnt, n);
         } else {
             TimeProfile<VertexId> tp;
-            cnt = tp.run(all, 3, 500) - tp.run(none, 3, 500);
+            const double a = tp.run(all, 3, 500);
+            const unsigned long long na = tp.last_estimation().get_num_finished_estimators();
+            const size_t ba = tp.batches().size();
+            const double b = tp.run(none, 3, 500);
+            const unsigned long long nb = tp.last_estimation().get_num_finished_estimators();
+            cnt = a - b;
+            // each profile's estimators: ids [0, n) in tp.batches().size() batches
+            report("time-profile-all", a, na);
+            report("time-profile-none", b, nb);
+            std::printf("time-profile-batches %zu %zu\n", ba, tp.batches().size());
             report("at-least-one-predicate-triangle", cnt, 0);
         }
         Debug::get_instance()->print("estimated number of triangles: ", (long long)cnt);

@@ -28,8 +28,8 @@ __all__ = [
     "ChainMiningInstance", "FourChainCountingInstance", "FourCliqueCountingInstance",
     "SamplerGenerator", "AllPredicateZGraphInstance", "ErrorProfile", "TimeProfile",
     "load_graph_meta_data", "load_biedgelist", "load_edgelist", "save_biedgelist", "ZGraphError",
-    "device_info", "clt_half_width", "run", "run_outcomes", "Batch", "random_gather_bandwidth",
-    "random_line_bandwidth",
+    "device_info", "device_memory", "clt_half_width", "run", "run_outcomes", "Batch", "random_gather_bandwidth",
+    "random_line_bandwidth", "random_chain_bandwidth",
 ]
 
 OUTCOME_DTYPE = np.dtype([("num_lo", "<u8"), ("num_hi", "<u8"), ("e0", "<u4"), ("status", "u1"),
@@ -41,6 +41,13 @@ def device_info():
     cu, lds, ws = C.c_int(), C.c_int(), C.c_int()
     check(lib().zg_device_info(C.byref(cu), C.byref(lds), C.byref(ws)))
     return {"num_cus": cu.value, "lds_per_cu": lds.value, "warp_size": ws.value}
+
+
+def device_memory():
+    """(free, total) bytes of device memory"""
+    f, t = C.c_uint64(), C.c_uint64()
+    check(lib().zg_device_memory(C.byref(f), C.byref(t)))
+    return f.value, t.value
 
 
 # ---------------------------------------------------------------------------
@@ -118,6 +125,7 @@ class Graph:
         self.max_degree = info.max_degree
         self.device_bytes = info.device_bytes
         self.has_weights = bool(info.has_weights)
+        self.has_parallel_edges = bool(info.has_parallel_edges)
 
     @classmethod
     def from_edges(cls, src, dst, num_vertices, weights=None, seed=1):
@@ -390,6 +398,15 @@ def random_line_bandwidth(buffer_bytes=16 << 30, loads_per_line=1, cooperative=F
     return gbs.value, ms.value
 
 
+def random_chain_bandwidth(m, estimators=1 << 28, block_size=0, blocks_per_cu=0):
+    """Line rate in GB/s of `estimators` triangle-estimator access chains (a
+    random 32-byte record, an 8-byte entry, a 16-byte bucket header, each
+    address taken from the previous read) over arrays sized for m edges."""
+    gbs, ms = C.c_double(), C.c_double()
+    check(lib().zg_random_chain_bw(m, estimators, block_size, blocks_per_cu, C.byref(gbs), C.byref(ms)))
+    return gbs.value, ms.value
+
+
 def clt_half_width(mean, sumsq, count, confidence=0.95):
     """CLT half-width of the mean at the given two-sided confidence."""
     if count < 2:
@@ -645,26 +662,34 @@ class TimeProfile:
 
     def __init__(self, basic_estimators=1 << 16):
         self.basic = basic_estimators
+        self.batches = []          # (first id, count, whole ms) of the last run's batches
+        self.remainder = False     # whether the last run ended with a batch sized to the time left
+        self.estimation = None     # the last run's Estimation (running mean + exact sums)
 
     def run(self, instance, num_vertices_in_pattern, time_limit_ms):
         used = 0
         cur = 0
         nxt = self.basic
         first = 0
-        total_n, total_sum = 0, 0.0
+        est = Estimation()
+        self.batches, self.remainder = [], False
         while used + cur * 2 <= time_limit_ms:
             t = time.perf_counter()
             s = run(instance.graph, instance.plan, instance.seed, first, nxt)[instance.channel]
             cur = int((time.perf_counter() - t) * 1000)
             used += cur
-            total_sum += s.mean * nxt
-            total_n += nxt
+            est.absorb(s)
+            est.update(nxt, s.mean)
+            self.batches.append((first, nxt, cur))
             first += nxt
             nxt <<= 1
         if cur > 0 and time_limit_ms > used:
             rest = nxt * (time_limit_ms - used) // (cur * 2)
             if rest > 0:
                 s = run(instance.graph, instance.plan, instance.seed, first, rest)[instance.channel]
-                total_sum += s.mean * rest
-                total_n += rest
-        return total_sum / total_n if total_n else 0.0
+                est.absorb(s)
+                est.update(rest, s.mean)
+                self.batches.append((first, rest, None))
+                self.remainder = True
+        self.estimation = est
+        return est.get_currnet_estimation()

@@ -30,7 +30,7 @@ MAX_PATTERN_EDGES = 15
 
 # every symbol include/zgraph_amd.h declares
 EXPORTED = [
-    "zg_last_error", "zg_init", "zg_device_info",
+    "zg_last_error", "zg_init", "zg_device_info", "zg_device_memory",
     "zg_graph_create", "zg_graph_create_device", "zg_graph_destroy", "zg_graph_info_get",
     "zg_graph_export", "zg_graph_subsample", "zg_graph_filter",
     "zg_gen_erdos_renyi", "zg_gen_chung_lu", "zg_gen_rmat", "zg_edges_add_uniform_weights",
@@ -40,7 +40,7 @@ EXPORTED = [
     "zg_run", "zg_run_noalloc", "zg_run_outcomes", "zg_count_reads", "zg_exact_counts",
     "zg_batch_create", "zg_batch_records", "zg_batch_run", "zg_batch_set_ids", "zg_batch_autotune",
     "zg_batch_config_get", "zg_batch_destroy", "zg_run_batch",
-    "zg_graph_degree_moments", "zg_random_gather_bw", "zg_random_line_bw",
+    "zg_graph_degree_moments", "zg_random_gather_bw", "zg_random_line_bw", "zg_random_chain_bw",
     "zg_elp_regression", "zg_elp_bound", "zg_regression_k", "zg_elp_from_pilots",
     "zg_mlp_num_params", "zg_mlp_input_dim", "zg_mlp_create", "zg_mlp_destroy", "zg_mlp_get_params",
     "zg_mlp_set_params", "zg_mlp_forward", "zg_mlp_gradient", "zg_mlp_train",
@@ -49,7 +49,8 @@ EXPORTED = [
 
 class GraphInfo(C.Structure):
     _fields_ = [("n", C.c_uint32), ("m", C.c_uint64), ("num_edges", C.c_uint64),
-                ("max_degree", C.c_uint64), ("device_bytes", C.c_uint64), ("has_weights", C.c_int)]
+                ("max_degree", C.c_uint64), ("device_bytes", C.c_uint64), ("has_weights", C.c_int),
+                ("has_parallel_edges", C.c_int)]
 
 
 class Edges(C.Structure):
@@ -121,6 +122,7 @@ def lib():
     L.zg_last_error.argtypes = [C.c_char_p, C.c_size_t]
     L.zg_init.argtypes = []
     L.zg_device_info.argtypes = [P(C.c_int), P(C.c_int), P(C.c_int)]
+    L.zg_device_memory.argtypes = [P(C.c_uint64), P(C.c_uint64)]
     L.zg_graph_create.argtypes = [vp, vp, vp, C.c_uint64, C.c_uint32, C.c_uint64, P(vp)]
     L.zg_graph_create_device.argtypes = [vp, vp, vp, C.c_uint64, C.c_uint32, C.c_uint64, P(vp)]
     L.zg_graph_destroy.argtypes = [vp]
@@ -159,6 +161,7 @@ def lib():
                                       P(C.c_double)]
     L.zg_random_line_bw.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, P(C.c_double),
                                     P(C.c_double)]
+    L.zg_random_chain_bw.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, P(C.c_double), P(C.c_double)]
     L.zg_count_reads.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_uint64)]
     L.zg_elp_regression.argtypes = [vp, vp, C.c_double, C.c_double, C.c_uint64, C.c_int, P(ElpResult)]
     L.zg_elp_bound.argtypes = [C.c_double] * 5 + [C.c_uint64, C.c_uint64]

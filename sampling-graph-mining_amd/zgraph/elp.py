@@ -130,6 +130,8 @@ class LearnedModel:
 
 class LearnedErrorProfile:
     """Drop-in for ErrorProfile with learned sizing and a CLT judge.
+    With correction=False the network is left out (r = 0): the pilot's own CV
+    sizes the query -- the baseline that shows what the network adds.
 
     get_num_estimators(instance, k, eps, conf) sizes the query; run(...)
     sizes, runs, judges and tops up, returning the estimate.  `last` holds the
@@ -137,9 +139,10 @@ class LearnedErrorProfile:
     """
 
     def __init__(self, weights=DEFAULT_WEIGHTS, pilot=1 << 16, max_pilot=MAX_PILOT, max_rounds=8,
-                 fallback=None, min_sample=1 << 12):
+                 fallback=None, min_sample=1 << 12, correction=True):
         self.model = None
-        if weights and os.path.exists(weights):
+        self.correction = correction   # False: size from the pilot's CV alone (r = 0), no network
+        if correction and weights and os.path.exists(weights):
             self.model = LearnedModel(weights)
         self.pilot = pilot
         self.max_pilot = max_pilot
@@ -152,7 +155,7 @@ class LearnedErrorProfile:
     def _size(self, instance, eps, conf, est):
         """Runs the pilot into `est`; returns (N, method)."""
         kind = instance.plan.kind
-        if self.model is None or kind not in PATTERN_SLOTS:
+        if self.correction and (self.model is None or kind not in PATTERN_SLOTS):
             return None, "regression"
         n_p = self.pilot
         while True:
@@ -164,11 +167,14 @@ class LearnedErrorProfile:
         mean, cv, succ = pilot_stats(s)
         if s.successes < MIN_PILOT_SUCCESSES or not math.isfinite(cv) or cv <= 0:
             return None, "regression"
-        x = feature_vector(graph_features(instance.graph), kind, n_p, cv, succ)
-        r = float(self.model.predict(x)[0])
+        if self.correction:
+            x = feature_vector(graph_features(instance.graph), kind, n_p, cv, succ)
+            r = float(self.model.predict(x)[0])
+        else:
+            r = 0.0
         cv_hat = cv * 10.0 ** r
         z = _z(conf)
-        return max(int(math.ceil((z * cv_hat / eps) ** 2)), n_p), "learned"
+        return max(int(math.ceil((z * cv_hat / eps) ** 2)), n_p), "learned" if self.correction else "pilot"
 
     def get_num_estimators(self, instance, num_vertices_in_pattern, required_error_rate, required_confidence):
         est = Estimation()

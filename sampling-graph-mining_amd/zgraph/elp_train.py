@@ -1,9 +1,14 @@
 """Training data and training of the learned error-latency profile (GPU).
 
     python -m zgraph.elp_train --graphs 256 --pilots 64 --out DIR
+    python -m zgraph.elp_train --first-graph 0 --num-graphs 128 --collect-only --out DIR/a
+    python -m zgraph.elp_train --merge DIR/a/elp_data.npz DIR/b/elp_data.npz --out DIR
 
 1. Seeded graphs from the device generators (Erdos-Renyi, Chung-Lu with
-   degree exponents 2.05..3, R-MAT), 2e4..1.6e8 edges, built in HBM.
+   degree exponents 2.05..3, R-MAT), 2e4..1e9 edges, built in HBM; graph i is
+   drawn from its own seeded stream, so index ranges can be collected
+   separately and merged.  Every 16th graph is large: R-MAT scale 24..26
+   (edge factor 8..16, up to 1.07e9 edges) or Chung-Lu with 1.5e8..6e8 edges.
 2. For every (graph, pattern) with pattern in {triangle, 4-clique, 3-chain,
    4-chain}: the exact count T from the device exact counters and a large run
    (2^26 estimators, extended up to 2^30 until 4096 succeed) whose sum of
@@ -44,6 +49,21 @@ def random_graph(rng, i):
     """One seeded synthetic graph description and its device edges."""
     fam = ["er", "cl", "cl", "rmat"][i % 4]
     seed = 1000 + i
+    if i % 16 == 11:   # the large graphs: R-MAT 24..26 or Chung-Lu up to 6e8 edges
+        if rng.random() < 0.5:
+            scale = int(rng.integers(24, 27))
+            ef = float(rng.uniform(8, 16))
+            m = int(ef * (1 << scale))
+            a = float(rng.uniform(0.5, 0.6))
+            b = c = float((1 - a) * rng.uniform(0.35, 0.45))
+            desc = {"family": "rmat", "scale": scale, "m": m, "a": a, "b": b, "c": c, "seed": seed}
+            return desc, rmat(scale, m, a, b, c, seed)
+        m = int(10 ** rng.uniform(8.2, 8.78))
+        n = int(2 * m / 10 ** rng.uniform(1.0, 1.6))
+        beta = float(rng.uniform(2.05, 2.8))
+        dmax = float(10 ** rng.uniform(3.5, 4.6))
+        desc = {"family": "chung_lu", "n": n, "m": m, "beta": beta, "max_degree": dmax, "seed": seed}
+        return desc, chung_lu(n, m, beta, dmax, seed)
     if fam == "rmat":
         scale = int(rng.integers(10, 21))
         ef = float(10 ** rng.uniform(0.5, 1.25))
@@ -70,13 +90,13 @@ def random_graph(rng, i):
     return desc, chung_lu(n, m, beta, dmax, seed)
 
 
-def collect(num_graphs, pilots, seed, big=1 << 26):
-    rng = np.random.default_rng(seed)
+def collect(first_graph, num_graphs, pilots, seed, big=1 << 26):
     plans = {k: Plan(k) for _, k, _ in PATTERNS}
     X, Y, G, P, NP = [], [], [], [], []
     graphs = []
     t0 = time.perf_counter()
-    for i in range(num_graphs):
+    for i in range(first_graph, first_graph + num_graphs):
+        rng = np.random.default_rng([seed, i])
         try:
             desc, edges = random_graph(rng, i)
         except RuntimeError as e:   # e.g. a generator that could not find m distinct pairs
@@ -134,6 +154,7 @@ def collect(num_graphs, pilots, seed, big=1 << 26):
                 G.append(i)
                 P.append(slot)
                 NP.append(n_p)
+        desc["index"] = i
         graphs.append(desc)
         g.close()
         log(f"graph {i}: {desc['family']} n={desc['vertices']} m={desc['edges']} dmax={desc['max_degree_built']} "
@@ -193,22 +214,35 @@ def train(X, Y, G, steps=20000, segment=500, batch=256, lr=1e-3, seed=3):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--graphs", type=int, default=256)
+    ap.add_argument("--graphs", type=int, default=256, help="graphs 0..graphs-1 (without --num-graphs)")
+    ap.add_argument("--first-graph", type=int, default=0)
+    ap.add_argument("--num-graphs", type=int, default=0, help="collect graphs first-graph .. +num-graphs-1")
+    ap.add_argument("--collect-only", action="store_true", help="write the tuples, do not train")
     ap.add_argument("--pilots", type=int, default=64)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--out", required=True)
     ap.add_argument("--data", default=None, help="train on this saved dataset instead of collecting")
+    ap.add_argument("--merge", nargs="+", default=None, help="train on the union of these saved datasets")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
-    if a.data:
-        with np.load(a.data, allow_pickle=False) as z:
-            X, Y, G, P, NP = z["X"], z["Y"], z["G"], z["P"], z["NP"]
-            graphs = json.loads(bytes(z["graphs"]).decode())
+    if a.data or a.merge:
+        parts = []
+        for path in ([a.data] if a.data else a.merge):
+            with np.load(path, allow_pickle=False) as z:
+                parts.append((z["X"], z["Y"], z["G"], z["P"], z["NP"], json.loads(bytes(z["graphs"]).decode())))
+        X, Y, G, P, NP = (np.concatenate([p[k] for p in parts]) for k in range(5))
+        graphs = [d for p in parts for d in p[5]]
+        if len({d.get("index", -1) for d in graphs}) < len(graphs) and a.merge:
+            raise SystemExit("merged datasets hold the same graph twice")
     else:
-        X, Y, G, P, NP, graphs = collect(a.graphs, a.pilots, a.seed)
+        n = a.num_graphs or a.graphs
+        X, Y, G, P, NP, graphs = collect(a.first_graph, n, a.pilots, a.seed)
         np.savez_compressed(os.path.join(a.out, "elp_data.npz"), X=X, Y=Y, G=G, P=P, NP=NP,
                             graphs=np.frombuffer(json.dumps(graphs).encode(), np.uint8))
+        if a.collect_only:
+            log(f"{len(Y)} tuples from {len(graphs)} graphs written")
+            return
     log(f"{len(Y)} tuples from {len(graphs)} graphs")
     params, mu, sd, stats, pred, te = train(X, Y, G, steps=a.steps)
     names = ["triangle", "4-clique", "3-chain", "4-chain"]

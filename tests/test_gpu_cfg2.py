@@ -1,0 +1,118 @@
+"""The headline workload at full size: seeded Chung-Lu graph, 4.8 M vertices,
+69 M undirected edges, degree exponent 2.1, expected maximum degree 20 000,
+seed 1, built in HBM (the graph bench.py times).
+
+* The device CSR satisfies the reference's invariants (src/Graph.cpp:215-377):
+  every list ascends by rank, the degrees sum to 2m, each rank appears in
+  exactly two lists, and each list entry is its edge seen from the owner.
+* The exact triangle and 4-clique counts are the recorded ones.
+* 2^28 triangle and 4-clique estimators (one launch each, as benchmarked) lie
+  in their 99.9 % CLT intervals around the exact counts -- the statistical
+  parity with the reference (applications/Triangle.cpp:48-73 and the
+  unbiased 4-clique plan) that the reference's own non-reproducible RNG allows.
+* The first 2^20 estimators of both plans are bit-exact against the CPU
+  oracle (the restatement of ZGraphInstance::calculate,
+  src/ZGraphInstance.cpp:256-297, with linear-scan closing checks) run on the
+  exported CSR, estimator by estimator."""
+import numpy as np
+import pytest
+
+import zgraph as Z
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+N = 1 << 28
+N_ORACLE = 1 << 20
+EXACT_TRIANGLES = 403_600_055
+EXACT_K4 = 10_220_447_520
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def graph():
+    say("[cfg2] Chung-Lu n=4.8M m=69M beta=2.1 maxdeg~20k seed=1: generating")
+    e = Z.chung_lu(4_800_000, 69_000_000, 2.1, 20000.0, 1)
+    g = Z.Graph.from_device(e, seed=1)
+    e.free()
+    say(f"[cfg2] built: m={g.num_local_edges} maxdeg={g.max_degree} {g.device_bytes / 2**30:.2f} GiB")
+    yield g
+    g.close()
+
+
+@pytest.fixture(scope="module")
+def exported(graph):
+    a = graph.export()
+    say("[cfg2] CSR exported")
+    return a
+
+
+def test_cfg2_csr_invariants(graph, exported):
+    a = exported
+    n, m = graph.num_vertices, graph.num_local_edges
+    assert n == 4_800_000 and m == 69_000_000
+    off = a["off"]
+    deg = np.diff(off)
+    assert off[0] == 0 and off[-1] == 2 * m
+    assert int(deg.max()) == graph.max_degree
+    rank, nbr = a["adj_rank"], a["adj_nbr"]
+    # ascending ranks inside every list (a descent may only happen at a list start)
+    desc = np.flatnonzero(rank[1:] <= rank[:-1]) + 1
+    starts = np.zeros(2 * m + 1, bool)
+    starts[off[:-1][deg > 0]] = True
+    assert starts[desc].all()
+    # each rank in exactly two lists, and each entry is its edge seen from the owner
+    assert (np.bincount(rank, minlength=m) == 2).all()
+    own = np.repeat(np.arange(n, dtype=np.uint32), deg.astype(np.int64))
+    s, d = a["src"][rank], a["dst"][rank]
+    ok = ((s == own) & (d == nbr)) | ((d == own) & (s == nbr))
+    assert ok.all()
+    assert (a["src"] != a["dst"]).all()
+
+
+def test_cfg2_exact_counts(graph):
+    assert graph.exact_count("triangle") == EXACT_TRIANGLES
+    assert graph.exact_count("k4") == EXACT_K4
+
+
+# 99.9 % half-widths at 2^28 estimators: triangles ~0.5 % of the count; 4-cliques are
+# rare (a success per ~1.4e6 estimators, per-estimator CV ~1 660), ~33 %
+@pytest.mark.parametrize("kind,exact,max_rel_hw", [(Z.PLAN_TRIANGLE, EXACT_TRIANGLES, 0.02),
+                                                   (Z.PLAN_K4_CLOSE, EXACT_K4, 0.5)])
+def test_cfg2_estimates_in_clt_interval(graph, kind, exact, max_rel_hw):
+    s, ms = Z.run(graph, Z.Plan(kind), 1, 0, N, timed=True)
+    s = s[0]
+    hw = Z.clt_half_width(s.mean, s.sumsq, N, 0.999)
+    say(f"[cfg2] plan {kind}: {N} estimators in {ms:.2f} ms, estimate {s.mean:.6g} +- {hw:.3g} "
+        f"(99.9 %), exact {exact}")
+    assert s.count == N and s.status == 0
+    assert abs(s.mean - exact) <= hw
+    assert hw < max_rel_hw * exact
+    assert s.successes > 100
+
+
+@pytest.mark.parametrize("kind", [Z.PLAN_TRIANGLE, Z.PLAN_K4_CLOSE])
+def test_cfg2_oracle_parity_first_ids(graph, exported, kind):
+    a = exported
+    og = O.OGraph.from_csr(graph.num_vertices, a["src"], a["dst"], a["off"], a["adj_rank"], a["adj_nbr"])
+    zo, zs = Z.run_outcomes(graph, Z.Plan(kind), 1, 0, N_ORACLE)
+    oo = O.run_outcomes(og, O.make_plan(kind), 1, 0, N_ORACLE)
+    for f in ("status", "num_lo", "num_hi", "log2den", "e0", "ncond", "nwords"):
+        assert (zo[f] == oo[f]).all(), f
+    osum = O.run(og, O.make_plan(kind), 1, 0, N_ORACLE)[0]
+    assert zs[0].total() == osum.total() and zs[0].successes == osum.successes
+    say(f"[cfg2] plan {kind}: {N_ORACLE} estimators bit-exact, {int((zo['status'] == 1).sum())} successes")
+    if kind == Z.PLAN_K4_CLOSE:
+        # 4-cliques succeed about once per 1.4e6 estimators: compare every success among
+        # the first 2^25 ids with the oracle's outcome for that id
+        zo, _ = Z.run_outcomes(graph, Z.Plan(kind), 1, 0, 1 << 25)
+        ids = np.flatnonzero(zo["status"] != 0)
+        assert len(ids) >= 5
+        for i in ids:
+            o = O.run_outcomes(og, O.make_plan(kind), 1, int(i), 1)
+            for f in ("status", "num_lo", "num_hi", "log2den", "e0", "ncond", "nwords"):
+                assert zo[f][i] == o[f][0], (int(i), f)
+        say(f"[cfg2] 4-clique: all {len(ids)} successes among 2^25 ids bit-exact")

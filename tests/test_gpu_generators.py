@@ -173,6 +173,20 @@ def test_device_c4_and_five_chains_of_complete_graphs(monkeypatch, n, sort_wedge
     assert g.exact_count("path4") == perm(n, 5) // 2
 
 
+@pytest.mark.parametrize("a,b,sort_wedges", [(2, 60000, None), (300, 300, None), (300, 300, "5000")])
+def test_device_c4_complete_bipartite(monkeypatch, a, b, sort_wedges):
+    """K_{a,b}: C(a, 2) C(b, 2) 4-cycles.  K_{2,60000} puts 1.8e9 of them on
+    one opposite pair (one LDS slot); in K_{300,300} the top vertices overflow
+    the LDS hash into the sorted batches (and, with batches limited to 5 000
+    wedges, into the dense counters), each summed in 128 bits."""
+    from math import comb
+    if sort_wedges:
+        monkeypatch.setenv("ZGRAPH_C4_SORT_WEDGES", sort_wedges)
+    u, v = np.meshgrid(np.arange(a), np.arange(a, a + b), indexing="ij")
+    g = Z.Graph.from_edges(u.ravel().astype(np.uint32), v.ravel().astype(np.uint32), a + b, seed=3)
+    assert g.exact_count("c4") == comb(a, 2) * comb(b, 2)
+
+
 def test_device_c4_and_five_chains_rmat14():
     """R-MAT scale 14 (262 k edges, hubs of a few thousand): device counts
     equal the oracle's pair-counting 4-cycles and closed-form chains."""

@@ -52,6 +52,25 @@ GRAPHS = {
 
 
 @pytest.mark.parametrize("name", sorted(GRAPHS))
+def test_parallel_edge_flag_and_three_chain_reads(name):
+    """The build flags a graph with parallel edges exactly when some pair
+    repeats; on a graph without them the 3-chain plan reads one line per
+    estimator (the sampled edge's record: the sampled list entry can only set
+    the parallel-copy denominator, which is then 0), with outcomes unchanged
+    (the parity tests cover both cases)."""
+    n, s, d = GRAPHS[name]()
+    g = Z.Graph.from_edges(s, d, n, seed=11)
+    pairs = np.minimum(s, d).astype(np.uint64) << np.uint64(32) | np.maximum(s, d).astype(np.uint64)
+    assert g.has_parallel_edges == (len(np.unique(pairs)) < len(pairs))
+    k = 1 << 14
+    reads = Z.count_reads(g, Z.Plan(Z.PLAN_THREE_CHAIN), 3, 0, k)
+    if g.has_parallel_edges:
+        assert reads > k
+    else:
+        assert reads == k
+
+
+@pytest.mark.parametrize("name", sorted(GRAPHS))
 def test_device_csr_equals_oracle(name):
     n, s, d = GRAPHS[name]()
     zg, og = both(n, s, d, seed=11)

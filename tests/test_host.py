@@ -135,14 +135,18 @@ def test_time_profile_doubling_rule(monkeypatch):
         def perf_counter():
             return clock["t"]
 
-    class S:
-        def __init__(self, mean):
-            self.mean = mean
+    class S:   # a zg_sums stand-in: integer values first % 7, so the exact sum is known
+        def __init__(self, v, count):
+            self.mean, self.log2_den, self.sumsq, self.successes = float(v), 0, float(v * v * count), count
+            self._total = v * count
+
+        def total(self):
+            return self._total
 
     def fake_run(graph, plan, seed, first, count, *a, **k):
         calls.append((first, count))
         clock["t"] += count / 65536 * 1e-3
-        return [S(float(first % 7))]
+        return [S(first % 7, count)]
 
     monkeypatch.setattr(Z, "time", FakeTime)
     monkeypatch.setattr(Z, "run", fake_run)
@@ -152,8 +156,12 @@ def test_time_profile_doubling_rule(monkeypatch):
         seed = 1
         channel = 0
 
-    est = Z.TimeProfile(1 << 16).run(Inst(), 3, 100)
+    tp = Z.TimeProfile(1 << 16)
+    est = tp.run(Inst(), 3, 100)
     sizes = [c for _, c in calls]
+    assert [(f, c) for f, c, _ in tp.batches] == calls and tp.remainder
+    assert [ms for _, _, ms in tp.batches[:6]] == [1, 2, 4, 8, 16, 32]
+    assert tp.estimation.exact_sum == sum((f % 7) * c for f, c in calls)
     assert sizes[:6] == [1 << (16 + i) for i in range(6)]          # 1, 2, 4, 8, 16, 32 ms
     assert sizes[6] == (1 << 22) * (100 - 63) // (32 * 2)          # the remaining-time batch
     assert len(calls) == 7

@@ -328,6 +328,33 @@ def test_c4_and_five_chain_counts_agree_with_brute_force(gi):
         assert g.c4() == 3 * math.comb(7, 4) and g.paths4() == math.perm(7, 5) // 2
 
 
+def _complete_bipartite(a, b):
+    u, v = np.meshgrid(np.arange(a), np.arange(a, a + b), indexing="ij")
+    return a + b, u.ravel().astype(np.uint32), v.ravel().astype(np.uint32)
+
+
+@pytest.mark.parametrize("a,b", [(2, 2), (2, 9), (3, 5), (6, 7), (1, 12)])
+def test_c4_complete_bipartite_closed_form_and_bound(a, b):
+    """K_{a,b} holds C(a, 2) C(b, 2) 4-cycles (two vertices on each side), and
+    every simple graph with m edges at most C(m, 2): each 4-cycle holds two
+    pairs of disjoint edges and each such pair closes at most two 4-cycles --
+    the bound that keeps the device's 4-cycle count below 2^61 for m < 2^31."""
+    n, s, d = _complete_bipartite(a, b)
+    g = O.OGraph(n, s, d)
+    assert g.c4() == math.comb(a, 2) * math.comb(b, 2)
+    assert g.c4() == g.pattern_brute(*O.C4)
+    m = a * b
+    assert g.c4() <= math.comb(m, 2)
+
+
+@pytest.mark.parametrize("gi", range(4))
+def test_c4_bound_on_random_graphs(gi):
+    n, s, d = [H.random_graph(16, 0.9, 20), H.random_graph(40, 0.3, 21), H.complete_graph(9),
+               H.power_law_graph(200, 900, 2.1, 22)][gi]
+    g = O.OGraph(n, s, d)
+    assert 0 < g.c4() <= math.comb(len(s), 2)
+
+
 def test_five_chain_closed_form_misses_a_dropped_term():
     """Each term of the closed form matters: on G(30, 0.2) leaving out the
     4-cycle term, or the triangle terms, changes the count."""
