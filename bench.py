@@ -422,16 +422,18 @@ def batched_queries(Z, args):
 
 
 def elp_sweep(Z, args):
-    """Time to +-1 % at 95 % for triangle, 4-clique, 3-chain and 4-chain on four
+    """Time to +-1 % at 95 % for triangle, 4-clique, 3-chain and 4-chain on five
     seeded graphs (the 69 M-edge bench graph, R-MAT scale 22, Erdos-Renyi with
     10 M vertices / 100 M edges, Chung-Lu with 2 M vertices / 20 M edges and
-    beta 2.5): the reference's regression sizing against pilot + network +
-    judge, 3 queries each, budgets capped at 2^34, errors against the device
-    exact counts."""
+    beta 2.5, R-MAT scale 26 with 1.07 B edges): the reference's regression
+    sizing against pilot-only and pilot + network sizing with the judge, 3
+    queries each, budgets capped at 2^34 (2^36 on R-MAT 26), errors against the
+    device exact counts."""
     graphs = [("chunglu 4.8M/69M b2.1", lambda: Z.chung_lu(4_800_000, 69_000_000, 2.1, 20000.0, args.seed)),
               ("rmat22", lambda: Z.rmat(22, 16 << 22, 0.57, 0.19, 0.19, args.seed)),
               ("erdos-renyi 10M/100M", lambda: Z.erdos_renyi(10_000_000, 100_000_000, args.seed)),
-              ("chunglu 2M/20M b2.5", lambda: Z.chung_lu(2_000_000, 20_000_000, 2.5, 5000.0, args.seed))]
+              ("chunglu 2M/20M b2.5", lambda: Z.chung_lu(2_000_000, 20_000_000, 2.5, 5000.0, args.seed)),
+              ("rmat26", lambda: Z.rmat(26, 16 << 26, 0.57, 0.19, 0.19, args.seed))]
     patterns = (("triangle", Z.PLAN_TRIANGLE, "triangle"), ("4-clique", Z.PLAN_K4_CLOSE, "k4"),
                 ("3-chain", Z.PLAN_THREE_CHAIN, "wedge"), ("4-chain", Z.PLAN_FOUR_CHAIN, "path3"))
     rows = {}
@@ -440,7 +442,8 @@ def elp_sweep(Z, args):
         g = Z.Graph.from_device(e, seed=args.seed)
         e.free()
         log(f"elp sweep {gname}: m={g.num_local_edges} maxdeg={g.max_degree}")
-        rows[gname] = time_to_error(Z, g, args, patterns=patterns, cap_log2=34, queries=3)
+        rows[gname] = time_to_error(Z, g, args, patterns=patterns, cap_log2=36 if gname == "rmat26" else 34,
+                                    queries=3)
         if gname in ("chunglu 4.8M/69M b2.1", "rmat22"):
             rows[gname]["coverage"] = elp_coverage(Z, g, args, patterns[0:1] + patterns[2:])
         g.close()

@@ -77,6 +77,8 @@ def lib():
         L.og_graph_free.argtypes = [P(Graph)]
         L.og_graph_from_csr.argtypes = [P(Graph), C.c_uint32, C.c_uint64] + [C.c_void_p] * 5
         L.og_graph_subsample.argtypes = [P(Graph), P(Graph), C.c_double, C.c_uint64]
+        L.og_device_lines.argtypes = [P(Graph), C.c_int, C.c_uint64, C.c_uint64, C.c_uint64]
+        L.og_device_lines.restype = C.c_uint64
         L.og_graph_filter.argtypes = [P(Graph), P(Graph), C.c_float, C.c_int, C.c_uint64]
         L.og_estimate.argtypes = [P(Graph), P(Plan), C.c_uint64, C.c_uint64, P(Outcome)]
         L.og_run.argtypes = [P(Graph), P(Plan), C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, P(Sums)]
@@ -310,6 +312,15 @@ def run(g, plan, seed, first, count, threads=0):
     sums = (Sums * nch)()
     lib().og_run(C.byref(g.g), C.byref(plan), seed, first, count, threads, sums)
     return list(sums)
+
+
+def device_lines(g, kind, seed, first, count):
+    """128-byte lines the device kernel of `kind` reads for these estimators
+    (lines.c: the device layout's costs along the oracle's draws)."""
+    r = lib().og_device_lines(C.byref(g.g), kind, seed, first, count)
+    if r == (1 << 64) - 1:
+        raise ValueError("no line model for this plan kind")
+    return r
 
 
 def mean(s):

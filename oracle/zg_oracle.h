@@ -176,6 +176,12 @@ int og_cond_pick(const og_graph *g, uint64_t min_valid, const uint32_t *verts, i
 int og_close(const og_graph *g, uint64_t min_valid, const uint32_t *verts, int nv,
              const uint32_t *wsrc, const uint32_t *wdst, int nw);
 
+/* ---- algorithmic line count of the device kernels (lines.c) ------------ */
+/* 128-byte lines that the device estimator kernel of `kind` (triangle, 3-chain,
+ * 3-motif, 4-clique) reads for estimators [first, first+count) under the
+ * device layout; UINT64_MAX for other kinds. */
+uint64_t og_device_lines(const og_graph *g, int kind, uint64_t seed, uint64_t first, uint64_t count);
+
 /* ---- exact counts (naive_implementation semantics) --------------------- */
 uint64_t og_exact_triangles(const og_graph *g, int threads);
 uint64_t og_exact_k4(const og_graph *g, int threads);

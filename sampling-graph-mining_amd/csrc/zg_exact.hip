@@ -1071,8 +1071,13 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
             // which caps a batch at 2^(32 - wbits) vertices; 64-bit keys past n = 2^30
             uint32_t wbits = 1;
             while ((1ull << wbits) < n) ++wbits;
-            const bool narrow = wbits <= 30;
-            const uint32_t nbcap = narrow ? (1u << (32 - wbits)) : 0xffffffffu;
+            // ZGRAPH_C4_WIDE_KEYS=1 forces the 64-bit keys and ZGRAPH_C4_BATCH_VERTICES caps the
+            // vertices per batch (tests reach those branches on small graphs with them)
+            const char *ew = getenv("ZGRAPH_C4_WIDE_KEYS");
+            const bool narrow = wbits <= 30 && !(ew && ew[0] == '1');
+            uint32_t nbcap = narrow ? (1u << (32 - wbits)) : 0xffffffffu;
+            if (const char *ev = getenv("ZGRAPH_C4_BATCH_VERTICES"))
+                nbcap = std::max<uint32_t>(1, std::min<uint32_t>(nbcap, (uint32_t)strtoul(ev, nullptr, 10)));
             std::vector<std::pair<uint32_t, uint32_t>> batches;   // [first, end) in the list
             std::vector<uint32_t> dense;
             uint64_t tmax = 0;

@@ -3,11 +3,15 @@
 // No reference counterpart: it stands in for the regression fit of
 // ErrorProfile::get_num_estimators (src/ErrorLatencyProfile.cpp:127-200).
 //
-// One wavefront owns a 16-row tile.  Every matrix product is a sequence of
-// v_mfma_f32_16x16x16_bf16 ops (A 16x16, B 16x16, D 16x16 fp32):
-//   A operand: lane l holds A[l % 16][4 (l / 16) + j], j = 0..3
-//   B operand: lane l holds B[4 (l / 16) + j][l % 16]
-//   D result : lane l holds D[4 (l / 16) + j][l % 16]
+// One wavefront owns a 16-row tile.  Products with K = 64 (the hidden layers
+// forward, dH = dZ W^T backward) use gfx950's v_mfma_f32_16x16x32_bf16 (A
+// 16x32, B 32x16), two per 16x16 output tile; products with K = 16 (the
+// first layer, the per-tile weight gradients) use v_mfma_f32_16x16x16_bf16:
+//   16x16x16 A operand: lane l holds A[l % 16][4 (l / 16) + j], j = 0..3
+//            B operand: lane l holds B[4 (l / 16) + j][l % 16]
+//   16x16x32 A operand: lane l holds A[l % 16][8 (l / 16) + j], j = 0..7
+//            B operand: lane l holds B[8 (l / 16) + j][l % 16]
+//   D result (both)   : lane l holds D[4 (l / 16) + j][l % 16]
 // Forward: X(16x16) W1 -> relu -> (16x64) W2 -> relu -> W3 -> relu -> . w4.
 // Backward (MSE loss mean over the batch) per tile: dZ3 = dy w4^T * relu',
 // gW3 = H2^T dZ3, dH2 = dZ3 W3^T, ... gW1 = X^T dZ1; each workgroup writes
@@ -26,6 +30,8 @@ namespace zg {
 namespace mlp {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short i16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int IN = 16, H = 64;
@@ -101,6 +107,26 @@ __device__ __forceinline__ f32x4 mma(bf16x4 a, bf16x4 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// K = 32 operands: 8 consecutive bf16 of a row (A) or of an n-major column (B)
+__device__ __forceinline__ bf16x8 ld8(const uint16_t *p)
+{
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8 *>(p));
+}
+// A operand from row-major S[16][ld], K chunk kc of 32
+__device__ __forceinline__ bf16x8 opA32(const uint16_t *S, int ld, int kc, int l)
+{
+    return ld8(S + (l & 15) * ld + kc * 32 + 8 * (l >> 4));
+}
+// B operand from n-major T[n][ldk], K chunk kc of 32, N tile nt
+__device__ __forceinline__ bf16x8 opB32(const uint16_t *T, int ldk, int kc, int nt, int l)
+{
+    return ld8(T + (nt * 16 + (l & 15)) * ldk + kc * 32 + 8 * (l >> 4));
+}
+__device__ __forceinline__ f32x4 mma32(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
 __device__ void load_weights(Smem &s, const float *__restrict__ P, int l)
 {
     for (int i = l; i < IN * H; i += 64) {
@@ -127,8 +153,13 @@ __device__ void dense_relu(const uint16_t *in, const uint16_t *wt, const float *
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        if constexpr (K % 32 == 0) {
 #pragma unroll
-        for (int kc = 0; kc < K / 16; ++kc) acc = mma(opA(in, K, kc, l), opB(wt, K, kc, nt, l), acc);
+            for (int kc = 0; kc < K / 32; ++kc) acc = mma32(opA32(in, K, kc, l), opB32(wt, K, kc, nt, l), acc);
+        } else {
+#pragma unroll
+            for (int kc = 0; kc < K / 16; ++kc) acc = mma(opA(in, K, kc, l), opB(wt, K, kc, nt, l), acc);
+        }
         const int n = nt * 16 + (l & 15);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -190,7 +221,8 @@ __device__ void backprop_hidden(Smem &s, const uint16_t *hb /* this layer's acti
     for (int nt = 0; nt < 4; ++nt) {
         acc[nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kc = 0; kc < 4; ++kc) acc[nt] = mma(opA(&s.dz[0][0], H, kc, l), opB(wt_next_row, H, kc, nt, l), acc[nt]);
+        for (int kc = 0; kc < H / 32; ++kc)
+            acc[nt] = mma32(opA32(&s.dz[0][0], H, kc, l), opB32(wt_next_row, H, kc, nt, l), acc[nt]);
     }
     __syncthreads();   // everyone has read dz before it is overwritten
 #pragma unroll

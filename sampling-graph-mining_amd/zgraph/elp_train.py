@@ -163,11 +163,13 @@ def collect(first_graph, num_graphs, pilots, seed, big=1 << 26):
             np.array(NP, np.int64), graphs)
 
 
-def train(X, Y, G, steps=20000, segment=500, batch=256, lr=1e-3, seed=3):
+def train(X, Y, G, steps=20000, segment=500, batch=256, lr=1e-3, seed=3, init_seed=None):
     """Adam on the tuples of 70 % of the graphs, targets scaled to unit
     variance; the parameters after the segment with the lowest validation
     error (15 % of the graphs) are kept; the remaining 15 % of the graphs are
-    the test set."""
+    the test set.  `seed` fixes the split, `init_seed` (default: seed) the
+    initial weights and minibatches."""
+    init_seed = seed if init_seed is None else init_seed
     graphs = np.unique(G)
     rng = np.random.default_rng(seed)
     rng.shuffle(graphs)
@@ -182,12 +184,12 @@ def train(X, Y, G, steps=20000, segment=500, batch=256, lr=1e-3, seed=3):
     y_scale = float(Y[tr].std()) or 1.0
     Ys = (Y / y_scale).astype(np.float32)
     rmse = lambda a, b: float(np.sqrt(np.mean((a - b) ** 2))) if len(b) else float("nan")
-    net = MLP(seed)
+    net = MLP(init_seed)
     t = time.perf_counter()
     best = (float("inf"), None, 0)
     curve = []
     for s in range(max(1, steps // segment)):
-        net.train(Xn[tr], Ys[tr], segment, batch, lr, seed * 1000 + s)
+        net.train(Xn[tr], Ys[tr], segment, batch, lr, init_seed * 1000 + s)
         v = rmse(net.forward(Xn[va]) * y_scale, Y[va])
         curve.append(v)
         if v < best[0]:
@@ -206,7 +208,8 @@ def train(X, Y, G, steps=20000, segment=500, batch=256, lr=1e-3, seed=3):
              "test_rmse_mean_correction": rmse(np.full(te.sum(), Y[tr].mean()), Y[te]),
              "test_rmse_by_pattern": {}, "chosen_steps": best[2], "validation_curve": curve,
              "use_model": use_model, "y_scale": y_scale,
-             "train_seconds": dt, "steps": steps, "batch": batch, "lr": lr, "train_tuples": int(tr.sum()),
+             "train_seconds": dt, "steps": steps, "batch": batch, "lr": lr, "init_seed": init_seed,
+             "train_tuples": int(tr.sum()),
              "validation_tuples": int(va.sum()), "test_tuples": int(te.sum()),
              "test_graphs": sorted(int(g) for g in graphs[:k])}
     return net.params(), mu.astype(np.float32), sd.astype(np.float32), stats, pred, te
@@ -224,6 +227,8 @@ def main():
     ap.add_argument("--out", required=True)
     ap.add_argument("--data", default=None, help="train on this saved dataset instead of collecting")
     ap.add_argument("--merge", nargs="+", default=None, help="train on the union of these saved datasets")
+    ap.add_argument("--search", action="store_true",
+                    help="train over learning rate x batch x initial seed and keep the lowest validation error")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
     if a.data or a.merge:
@@ -244,7 +249,22 @@ def main():
             log(f"{len(Y)} tuples from {len(graphs)} graphs written")
             return
     log(f"{len(Y)} tuples from {len(graphs)} graphs")
-    params, mu, sd, stats, pred, te = train(X, Y, G, steps=a.steps)
+    if a.search:
+        # model selection on the validation graphs only; the test graphs stay unseen
+        grid, best = [], None
+        for lr in (3e-4, 1e-3, 3e-3):
+            for batch in (256, 1024):
+                for init in (3, 4, 5):
+                    r = train(X, Y, G, steps=a.steps, lr=lr, batch=batch, init_seed=init)
+                    v = r[3]["validation_rmse"]
+                    grid.append({"lr": lr, "batch": batch, "init_seed": init, "validation_rmse": v})
+                    log(f"search lr={lr:g} batch={batch} init={init}: validation RMSE {v:.5f}")
+                    if best is None or v < best[3]["validation_rmse"]:
+                        best = r
+        params, mu, sd, stats, pred, te = best
+        stats["search"] = grid
+    else:
+        params, mu, sd, stats, pred, te = train(X, Y, G, steps=a.steps)
     names = ["triangle", "4-clique", "3-chain", "4-chain"]
     for slot in range(4):
         sel = te & (P == slot)

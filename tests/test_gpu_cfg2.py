@@ -105,6 +105,9 @@ def test_cfg2_oracle_parity_first_ids(graph, exported, kind):
     osum = O.run(og, O.make_plan(kind), 1, 0, N_ORACLE)[0]
     assert zs[0].total() == osum.total() and zs[0].successes == osum.successes
     say(f"[cfg2] plan {kind}: {N_ORACLE} estimators bit-exact, {int((zo['status'] == 1).sum())} successes")
+    # the roofline's algorithmic bytes: the device read count equals the CPU line model's
+    k = 1 << 18
+    assert Z.count_reads(graph, Z.Plan(kind), 1, 0, k) == O.device_lines(og, kind, 1, 0, k)
     if kind == Z.PLAN_K4_CLOSE:
         # 4-cliques succeed about once per 1.4e6 estimators: compare every success among
         # the first 2^25 ids with the oracle's outcome for that id

@@ -156,8 +156,10 @@ def test_device_k4_count_rmat16_in_one_and_in_many_passes(monkeypatch, scratch_w
     assert g.exact_count("k4") == og.k4() == 456481728
 
 
-@pytest.mark.parametrize("n,sort_wedges", [(9, None), (70, None), (1000, None), (1000, "5000"), (1000, "1")])
-def test_device_c4_and_five_chains_of_complete_graphs(monkeypatch, n, sort_wedges):
+@pytest.mark.parametrize("n,sort_wedges,wide", [(9, None, False), (70, None, False), (1000, None, False),
+                                                (1000, "5000", False), (1000, "1", False), (1000, None, True),
+                                                (1000, "5000", True)])
+def test_device_c4_and_five_chains_of_complete_graphs(monkeypatch, n, sort_wedges, wide):
     """K_n: 3 C(n, 4) 4-cycles and n!/(n-5)!/2 5-vertex chains.  In K_1000
     the top vertices see up to 998 distinct opposite vertices, more than the
     LDS hash holds, so they are counted again by sorting their wedges; with
@@ -167,6 +169,9 @@ def test_device_c4_and_five_chains_of_complete_graphs(monkeypatch, n, sort_wedge
     import helpers as H
     if sort_wedges:
         monkeypatch.setenv("ZGRAPH_C4_SORT_WEDGES", sort_wedges)
+    if wide:   # 64-bit sort keys, batches of at most 2 vertices
+        monkeypatch.setenv("ZGRAPH_C4_WIDE_KEYS", "1")
+        monkeypatch.setenv("ZGRAPH_C4_BATCH_VERTICES", "2")
     n, s, d = H.complete_graph(n)
     g = Z.Graph.from_edges(s, d, n, seed=2)
     assert g.exact_count("c4") == 3 * comb(n, 4)
@@ -187,9 +192,17 @@ def test_device_c4_complete_bipartite(monkeypatch, a, b, sort_wedges):
     assert g.exact_count("c4") == comb(a, 2) * comb(b, 2)
 
 
-def test_device_c4_and_five_chains_rmat14():
+@pytest.mark.parametrize("env", [{}, {"ZGRAPH_C4_WIDE_KEYS": "1"}, {"ZGRAPH_C4_BATCH_VERTICES": "3"},
+                                 {"ZGRAPH_C4_WIDE_KEYS": "1", "ZGRAPH_C4_BATCH_VERTICES": "2"}],
+                         ids=["default", "wide-keys", "3-vertex-batches", "wide-keys-2-vertex-batches"])
+def test_device_c4_and_five_chains_rmat14(monkeypatch, env):
     """R-MAT scale 14 (262 k edges, hubs of a few thousand): device counts
-    equal the oracle's pair-counting 4-cycles and closed-form chains."""
+    equal the oracle's pair-counting 4-cycles and closed-form chains -- also
+    with the hubs' wedges sorted under 64-bit keys (the path of graphs past
+    2^30 vertices) and with batches split at a few vertices (the split that
+    32-bit keys force at 2^(32 - ceil(log2 n)) vertices)."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
     s, d = O.gen_rmat(14, 16 << 14, 0.57, 0.19, 0.19, 3)
     og = O.OGraph(1 << 14, s, d, seed=1)
     g = Z.Graph.from_edges(s, d, 1 << 14, seed=1)

@@ -71,6 +71,23 @@ def test_parallel_edge_flag_and_three_chain_reads(name):
 
 
 @pytest.mark.parametrize("name", sorted(GRAPHS))
+def test_read_count_equals_the_cpu_line_model(name):
+    """The algorithmic bytes of the roofline come from the device's
+    read-counting launch (zg_count_reads); the CPU line model (oracle/lines.c:
+    the device layout's cost of each step along the oracle's draws -- records,
+    entries, search probes, hash buckets walked) gives the same count for the
+    triangle, 3-chain, 3-motif and 4-clique plans, estimator set by estimator
+    set."""
+    n, s, d = GRAPHS[name]()
+    zg, og = both(n, s, d, seed=11)
+    k = 1 << 15
+    for kind in (Z.PLAN_TRIANGLE, Z.PLAN_THREE_CHAIN, Z.PLAN_THREE_MOTIF, Z.PLAN_K4_CLOSE):
+        dev = Z.count_reads(zg, Z.Plan(kind), 9, 1 << 20, k)
+        cpu = O.device_lines(og, kind, 9, 1 << 20, k)
+        assert dev == cpu, (name, kind, dev, cpu)
+
+
+@pytest.mark.parametrize("name", sorted(GRAPHS))
 def test_device_csr_equals_oracle(name):
     n, s, d = GRAPHS[name]()
     zg, og = both(n, s, d, seed=11)

@@ -438,3 +438,25 @@ def test_mt19937_engine_matches_the_standard(tmp_path):
     subprocess.check_call(["g++", "-O1", "-o", str(exe), str(src)])
     want = [int(x) for x in subprocess.check_output([str(exe)]).split()]
     assert [O.lib().og_mt19937_nth(1, k) for k in range(1, 6)] == want
+
+
+@pytest.mark.parametrize("gi", range(3))
+def test_line_model_basic_costs(gi):
+    """The CPU line model of the device kernels: a 3-chain estimator on a graph
+    without parallel edges reads only its first edge's record (one line); a
+    triangle estimator reads at most record + entry + the buckets of one
+    closing check; parallel edges add the 3-chain's entry read."""
+    n, s, d = [H.golden_graph("wiki-vote"), H.power_law_graph(2000, 12000, 2.2, 3), H.multigraph(24, 0.4, 5)][gi]
+    g = O.OGraph(n, s, d, seed=4)
+    k = 1 << 14
+    chain = O.device_lines(g, O.PLAN_THREE_CHAIN, 1, 0, k)
+    tri = O.device_lines(g, O.PLAN_TRIANGLE, 1, 0, k)
+    motif = O.device_lines(g, O.PLAN_THREE_MOTIF, 1, 0, k)
+    if gi < 2:
+        assert chain == k
+    else:
+        assert k < chain <= 2 * k
+    assert tri == motif          # the fused plan reads exactly the triangle plan's lines
+    assert k < tri < (4 if gi < 2 else 8) * k   # the multigraph's copies overflow buckets: longer walks
+    with pytest.raises(ValueError):
+        O.device_lines(g, O.PLAN_FOUR_CHAIN, 1, 0, 16)
