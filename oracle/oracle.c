@@ -25,6 +25,9 @@
  *                      reference's FiveChain built from source            pinned
  *   og_regression_k    tests/TestRegression.cpp known answer k^2=609.8565 pinned
  *   og_elp_*           hand-computed closed form on 1-3 points           pinned
+ *   og_device_lines    (lines.c) closed costs on CPU (3-chain = 1 line on a
+ *                      simple graph, 3-motif = triangle); equals the device's
+ *                      own read count on eleven graphs (GPU tests)         pinned
  *   generators         restated from the device generators' definition
  *                      (not the reference's); the device output is
  *                      compared element by element                     pinned

@@ -941,8 +941,10 @@ class LearnedErrorProfile {
         std::ifstream f(weights_bin, std::ios::binary);
         char magic[8];
         uint32_t hdr[2];
-        if (!f || !f.read(magic, 8) || std::memcmp(magic, "ZGELP001", 8) || !f.read((char *)hdr, 8))
+        if (!f || !f.read(magic, 8) || (std::memcmp(magic, "ZGELP001", 8) && std::memcmp(magic, "ZGELP002", 8)) ||
+            !f.read((char *)hdr, 8))
             throw Error(ZG_EINVAL, "LearnedErrorProfile: not a weights file: " + weights_bin);
+        const bool v2 = !std::memcmp(magic, "ZGELP002", 8);
         if ((int)hdr[0] != zg_mlp_num_params() || (int)hdr[1] != zg_mlp_input_dim())
             throw Error(ZG_EINVAL, "LearnedErrorProfile: weights do not match the network");
         mu_.resize(hdr[1]);
@@ -952,6 +954,7 @@ class LearnedErrorProfile {
         f.read((char *)sd_.data(), 4 * hdr[1]);
         f.read((char *)use_, 4);
         f.read((char *)&y_scale_, 4);
+        if (v2) f.read((char *)val_rmse_, 16);
         f.read((char *)params_.data(), 4 * (std::streamsize)hdr[0]);
         if (!f) throw Error(ZG_EINVAL, "LearnedErrorProfile: truncated weights file");
         check(zg_mlp_create(1, &net_));
@@ -1085,7 +1088,9 @@ class LearnedErrorProfile {
         float y = 0;
         for (int i = 0; i < 16; ++i) x[i] = (x[i] - mu_[i]) / sd_[i];
         check(zg_mlp_forward(net_, x, 1, &y));
-        const double r = use_[sl] ? (double)(y * y_scale_) : 0.0;
+        // aim one validation RMSE of the correction below the predicted CV: a short run costs
+        // one top-up of the id sequence, a long one its excess
+        const double r = (use_[sl] ? (double)(y * y_scale_) : 0.0) - aim_sigmas_ * (double)val_rmse_[sl];
         const double z = normal_quantile_two_sided(conf);
         const double q = z * cv * std::pow(10.0, r) / eps;
         return std::max((unsigned long long)std::ceil(q * q), n);
@@ -1114,6 +1119,8 @@ class LearnedErrorProfile {
     std::vector<float> mu_, sd_, params_;
     unsigned char use_[4] = {1, 1, 1, 1};
     float y_scale_ = 1.f;   // the network predicts r / y_scale
+    float val_rmse_[4] = {0.f, 0.f, 0.f, 0.f};   // the correction's RMSE on validation graphs, per slot
+    double aim_sigmas_ = 1.0;
     zg_mlp *net_ = nullptr;
     ErrorProfile<EdgeData> fallback_;
 };

@@ -12,7 +12,9 @@ bound scaled by K^2.  The learned profile replaces that sizing:
   3. the MLP (16-64-64-64-1, BF16 MFMA kernels, fp32 master weights) predicts
      r = log10(CV_true / CV_pilot), the correction from the pilot's CV to the
      CV of the estimator's full distribution;
-  4. N = (z * CV_pilot * 10^r / eps)^2 estimators (CLT sizing);
+  4. N = (z * CV_pilot * 10^(r - s) / eps)^2 estimators (CLT sizing), s the
+     correction's RMSE on validation graphs: the first run aims one typical
+     error low, since a short run costs one top-up and a long one its excess;
   5. the judge (the rule of sampling/IEStop.cpp:47-56: stop once
      z * sd / (sqrt(t) * mean) <= eps) checks the exact running sums and tops
      the same id sequence up until it holds.
@@ -82,13 +84,14 @@ def feature_vector(gf, plan_kind, pilot_n, pilot_cv, pilot_succ):
     return x
 
 
-BIN_MAGIC = b"ZGELP001"
+BIN_MAGIC = b"ZGELP002"
 
 
 def export_binary(npz_path, bin_path):
     """Write the weights as the flat little-endian file the C++ host API reads:
-    magic "ZGELP001", u32 nparam, u32 in_dim, f32 mu[in_dim], f32 sd[in_dim],
-    u8 use_model[4], f32 y_scale, f32 params[nparam]."""
+    magic "ZGELP002", u32 nparam, u32 in_dim, f32 mu[in_dim], f32 sd[in_dim],
+    u8 use_model[4], f32 y_scale, f32 validation RMSE per pattern slot [4],
+    f32 params[nparam]."""
     with np.load(npz_path, allow_pickle=False) as z:
         params = z["params"].astype("<f4")
         mu = z["mu"].astype("<f4")
@@ -102,6 +105,7 @@ def export_binary(npz_path, bin_path):
         f.write(sd.tobytes())
         f.write(use.tobytes())
         f.write(np.array([meta.get("y_scale", 1.0)], "<f4").tobytes())
+        f.write(np.array(meta.get("validation_rmse_by_slot", [0.0] * 4), "<f4").tobytes())
         f.write(params.tobytes())
 
 
@@ -117,6 +121,10 @@ class LearnedModel:
             self.meta = json.loads(bytes(z["meta"]).decode()) if "meta" in z else {}
         # pattern slots where the network beat "no correction" on validation graphs
         self.use_model = list(self.meta.get("use_model", [True] * len(PATTERN_SLOTS)))
+        # the deployed correction's RMSE on the validation graphs (log10 CV), per slot
+        # (rounded to fp32, as the C++ API reads them from the binary weights)
+        self.val_rmse = [float(np.float32(v)) for v in
+                         self.meta.get("validation_rmse_by_slot", [0.0] * len(PATTERN_SLOTS))]
         self.y_scale = float(self.meta.get("y_scale", 1.0))   # the network predicts r / y_scale
         self.net = MLP()
         self.net.set_params(self.params)
@@ -139,9 +147,12 @@ class LearnedErrorProfile:
     """
 
     def __init__(self, weights=DEFAULT_WEIGHTS, pilot=1 << 16, max_pilot=MAX_PILOT, max_rounds=8,
-                 fallback=None, min_sample=1 << 12, correction=True):
+                 fallback=None, min_sample=1 << 12, correction=True, aim_sigmas=1.0):
         self.model = None
         self.correction = correction   # False: size from the pilot's CV alone (r = 0), no network
+        # the first run aims `aim_sigmas` validation RMSEs below the predicted CV: a run that
+        # falls short costs one top-up of the id sequence, one that overshoots costs its excess
+        self.aim_sigmas = aim_sigmas
         if correction and weights and os.path.exists(weights):
             self.model = LearnedModel(weights)
         self.pilot = pilot
@@ -169,7 +180,7 @@ class LearnedErrorProfile:
             return None, "regression"
         if self.correction:
             x = feature_vector(graph_features(instance.graph), kind, n_p, cv, succ)
-            r = float(self.model.predict(x)[0])
+            r = float(self.model.predict(x)[0]) - self.aim_sigmas * self.model.val_rmse[PATTERN_SLOTS[kind]]
         else:
             r = 0.0
         cv_hat = cv * 10.0 ** r
@@ -298,7 +309,8 @@ def estimate_both_to_error(instance, eps, conf, profile, max_estimators=1 << 40,
         mean, cv, succ = pilot_stats(ests[c].sums_view())
         if (profile.model is not None and ests[c].successes >= MIN_PILOT_SUCCESSES and math.isfinite(cv)
                 and cv > 0):
-            r = float(profile.model.predict(feature_vector(gf, kinds[c], n_p, cv, succ))[0])
+            r = float(profile.model.predict(feature_vector(gf, kinds[c], n_p, cv, succ))[0]) - \
+                profile.aim_sigmas * profile.model.val_rmse[PATTERN_SLOTS[kinds[c]]]
             sizes.append(max(int(math.ceil((z * cv * 10.0 ** r / eps) ** 2)), n_p))
             how.append("learned")
         else:

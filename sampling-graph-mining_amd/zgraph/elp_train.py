@@ -203,11 +203,18 @@ def train(X, Y, G, steps=20000, segment=500, batch=256, lr=1e-3, seed=3, init_se
     for slot in range(4):
         sel = va & (X[:, 6 + slot] > 0.5)
         use_model.append(bool(sel.any() and rmse(pred[sel], Y[sel]) < rmse(0 * Y[sel], Y[sel])))
+    # the deployed correction's error on the validation graphs, per pattern slot (log10 CV):
+    # the learned profile aims its first run this far below the predicted budget
+    val_rmse_slot = []
+    for slot in range(4):
+        sel = va & (X[:, 6 + slot] > 0.5)
+        used = pred[sel] if use_model[slot] else 0 * Y[sel]
+        val_rmse_slot.append(rmse(used, Y[sel]) if sel.any() else 0.0)
     stats = {"train_rmse": rmse(pred[tr], Y[tr]), "validation_rmse": rmse(pred[va], Y[va]),
              "test_rmse": rmse(pred[te], Y[te]), "test_rmse_no_correction": rmse(0 * Y[te], Y[te]),
              "test_rmse_mean_correction": rmse(np.full(te.sum(), Y[tr].mean()), Y[te]),
              "test_rmse_by_pattern": {}, "chosen_steps": best[2], "validation_curve": curve,
-             "use_model": use_model, "y_scale": y_scale,
+             "use_model": use_model, "y_scale": y_scale, "validation_rmse_by_slot": val_rmse_slot,
              "train_seconds": dt, "steps": steps, "batch": batch, "lr": lr, "init_seed": init_seed,
              "train_tuples": int(tr.sum()),
              "validation_tuples": int(va.sum()), "test_tuples": int(te.sum()),

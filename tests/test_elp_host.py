@@ -69,8 +69,9 @@ class FixedProfile:
 
 
 class FakeModel:
-    def __init__(self, r):
+    def __init__(self, r, val_rmse=(0.0, 0.0, 0.0, 0.0)):
         self.r = r
+        self.val_rmse = list(val_rmse)
 
     def predict(self, X):
         return np.full(np.atleast_2d(X).shape[0], self.r, np.float32)
@@ -132,16 +133,20 @@ def test_unjudged_run_uses_the_sized_count():
     assert rec["estimators"] == 3000 and rec["topup_rounds"] == 0
 
 
-def test_learned_sizing_is_the_clt_formula():
-    prof = L.LearnedErrorProfile(weights=None, pilot=1 << 14, fallback=FixedProfile(777))
-    prof.model = FakeModel(0.1)
+@pytest.mark.parametrize("aim", [0.0, 1.0, 2.0])
+def test_learned_sizing_is_the_clt_formula(aim):
+    """N = (z * CV_pilot * 10^(r - aim * s) / eps)^2 with r the network's
+    correction and s its validation RMSE for the query's pattern slot."""
+    prof = L.LearnedErrorProfile(weights=None, pilot=1 << 14, fallback=FixedProfile(777), aim_sigmas=aim)
+    prof.model = FakeModel(0.1, val_rmse=(0.03, 0.05, 0.0, 0.02))
     inst = FakeInstance()
     est = Z.Estimation()
     n, how = prof._size(inst, 0.01, 0.95, est)
     assert how == "learned"
     v = inst.values(0, 1 << 14).astype(np.float64)
     cv = math.sqrt(max((v ** 2).mean() - v.mean() ** 2, 0)) / v.mean()
-    expect = math.ceil((1.959963984540054 * cv * 10 ** 0.1 / 0.01) ** 2)
+    s = (0.03, 0.05, 0.0, 0.02)[L.PATTERN_SLOTS[inst.plan.kind]]
+    expect = math.ceil((1.959963984540054 * cv * 10 ** (0.1 - aim * s) / 0.01) ** 2)
     assert n == expect
     assert est.get_num_finished_estimators() == 1 << 14    # the pilot is part of the query
 
