@@ -26,11 +26,16 @@ namespace zg {
 // the edge-set hash below; sadj serves the exact counters and the export.
 // List starts are u32, so a graph holds fewer than 2^31 edges.
 // ---------------------------------------------------------------------------
+// Field order: the first 16 bytes hold the endpoints' degrees and the rank's
+// positions -- all that a plan needs to size its first conditional sample
+// without reading the sampled entry (the 3-chain plan on a graph without
+// parallel edges), so that plan issues one 16-byte load per record; the
+// second 16 bytes hold the endpoints and their list starts.
 struct alignas(32) ERec {
+    uint32_t s_deg, s_pos;   // src's degree, position of this rank in src's list
+    uint32_t d_deg, d_pos;   // the same for dst
     uint32_t src, dst;
-    uint32_t s_off, s_deg;   // src's lists
-    uint32_t d_off, d_deg;   // dst's lists
-    uint32_t s_pos, d_pos;   // position of this rank in src's / dst's list
+    uint32_t s_off, d_off;   // list starts
 };
 
 // Edge-set hash for the closing checks.  A 128-byte bucket (one L2 line, the
@@ -211,9 +216,16 @@ __device__ __forceinline__ ERec load_edge(const GView &g, uint32_t r)
     const uint4 a = p[0], b = p[1];
     count_reads(g, 1);
     ERec e;
-    e.src = a.x; e.dst = a.y; e.s_off = a.z; e.s_deg = a.w;
-    e.d_off = b.x; e.d_deg = b.y; e.s_pos = b.z; e.d_pos = b.w;
+    e.s_deg = a.x; e.s_pos = a.y; e.d_deg = a.z; e.d_pos = a.w;
+    e.src = b.x; e.dst = b.y; e.s_off = b.z; e.d_off = b.w;
     return e;
+}
+// the record's first half only (one 16-byte load): {s_deg, s_pos, d_deg, d_pos}
+__device__ __forceinline__ uint4 load_edge_degpos(const GView &g, uint32_t r)
+{
+    const uint4 a = reinterpret_cast<const uint4 *>(g.edges + r)[0];
+    count_reads(g, 1);
+    return a;
 }
 __device__ __forceinline__ VI src_vi(const ERec &e) { return VI{e.src, e.s_deg, e.s_off}; }
 __device__ __forceinline__ VI dst_vi(const ERec &e) { return VI{e.dst, e.d_deg, e.d_off}; }

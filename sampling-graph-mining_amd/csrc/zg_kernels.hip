@@ -97,6 +97,21 @@ __device__ __forceinline__ void plan_wedge(const GView &g, Rng &rng, Est &o)
     if (g.m == 0) return;
     const uint32_t r0 = (uint32_t)rng.uniform(g.m);
     o.e0 = r0;
+    if (MODE == 1 && g.simple) {
+        // 3-chain on a graph without parallel edges: the sampled entry's neighbour is
+        // neither endpoint of e0 (that would be a second copy of e0 or a self-loop), so
+        // the denominator is 1 and only the sizes s = ca + cb are needed -- the record's
+        // first half (one 16-byte load); the index draw keeps the stream's words
+        const uint4 dp = load_edge_degpos(g, r0);   // s_deg, s_pos, d_deg, d_pos
+        const uint64_t s = (uint64_t)(dp.x - dp.y - 1) + (dp.z - dp.w - 1);
+        o.ncond = 1;
+        if (s == 0) return;
+        (void)rng.uniform(s);
+        U128 val{g.m, 0};
+        const bool ov = mul_u128(val, s);
+        set_out(o, 0, val, ov, 0);
+        return;
+    }
     const ERec E = load_edge(g, r0);
     const VI A = src_vi(E), B = dst_vi(E);
     const uint32_t pa = E.s_pos + 1, pb = E.d_pos + 1;   // lower_bound(r0 + 1) in both endpoint lists
@@ -107,13 +122,6 @@ __device__ __forceinline__ void plan_wedge(const GView &g, Rng &rng, Est &o)
     const uint64_t r = rng.uniform(s);
     U128 val{g.m, 0};
     const bool ov = mul_u128(val, s);
-    // 3-chain on a graph without parallel edges: the sampled entry's neighbour is
-    // neither endpoint of e0 (that would be a second copy of e0 or a self-loop),
-    // so the denominator is 1 and the entry need not be read
-    if (MODE == 1 && g.simple) {
-        set_out(o, 0, val, ov, 0);
-        return;
-    }
     const bool inA = r < ca;
     const uint64_t idx = inA ? (uint64_t)A.off + pa + r : (uint64_t)B.off + pb + (r - ca);
     const uint64_t ent = g.adj[idx];
