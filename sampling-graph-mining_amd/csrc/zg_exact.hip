@@ -35,7 +35,12 @@ template <class T>
 struct XB {
     T *p = nullptr;
     hipError_t alloc(size_t k) { return hipMalloc(&p, sizeof(T) * (k ? k : 1)); }
-    ~XB() { if (p) (void)hipFree(p); }
+    void reset()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
+    ~XB() { reset(); }
 };
 #define XALLOC(b, k)                                                                   \
     do {                                                                               \
@@ -57,35 +62,19 @@ inline int gridx(uint64_t n)
     for (uint64_t v = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; v < (n);       \
          v += ((uint64_t)gridDim.x * blockDim.x) >> 6)
 
-// neighbour of entry i of a list [b, e), or a value no vertex id takes
-__device__ __forceinline__ uint32_t nbr_or_none(const uint64_t *sadj, uint64_t i, uint64_t e)
+constexpr uint32_t kNone = 0xffffffffu;   // a value no vertex id takes
+
+__device__ __forceinline__ void wave_sync()
 {
-    return i < e ? (uint32_t)(sadj[i] >> 32) : 0xffffffffu;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// first entry of a run of equal neighbours (parallel edges collapse to one)
-__device__ __forceinline__ bool first_of_run(uint32_t nb, uint32_t &carry)
+__device__ __forceinline__ uint64_t uni64(uint64_t x)
 {
-    const uint32_t lane = threadIdx.x & 63;
-    uint32_t prev = __shfl_up(nb, 1, 64);
-    if (lane == 0) prev = carry;
-    carry = __shfl(nb, 63, 64);
-    return nb != 0xffffffffu && nb != prev;
-}
-
-// simple-graph degree of v: distinct neighbours in its neighbour-ordered list
-__global__ void k_simple_deg(const uint64_t *off, const uint64_t *sadj, uint32_t n, uint32_t *sdeg)
-{
-    WAVE_STRIDE(v, (uint64_t)n)
-    {
-        const uint64_t b = off[v], e = off[v + 1];
-        uint32_t d = 0, carry = 0xffffffffu;
-        for (uint64_t base = b; base < e; base += 64) {
-            const uint32_t nb = nbr_or_none(sadj, base + (threadIdx.x & 63), e);
-            d += (uint32_t)__popcll(__ballot(first_of_run(nb, carry)));
-        }
-        if ((threadIdx.x & 63) == 0) sdeg[v] = d;
-    }
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
+           __builtin_amdgcn_readfirstlane((uint32_t)x);
 }
 
 __device__ __forceinline__ bool before(const uint32_t *sdeg, uint32_t a, uint32_t b)
@@ -94,46 +83,184 @@ __device__ __forceinline__ bool before(const uint32_t *sdeg, uint32_t a, uint32_
     return da < db || (da == db && a < b);
 }
 
-// out-degree (toward higher order) of every vertex
-__global__ void k_out_deg(const uint64_t *off, const uint64_t *sadj, const uint32_t *sdeg, uint32_t n,
-                          uint64_t *odeg)
+// ---- entry walk -----------------------------------------------------------------
+// The neighbour-ordered lists taken as one array of 2m entries and walked 64
+// entries (a group) per wavefront step, each wavefront taking a tile of kEwTile
+// consecutive groups.  The owner of a lane's entry comes from the offsets of
+// the next 64 vertices (one coalesced load, mostly cache hits since
+// consecutive groups see the same offsets), so a hub's list is spread over as
+// many wavefronts as its length asks for and a pass streams the adjacency once,
+// where one wavefront per vertex would walk a hub's million entries alone.
+constexpr uint64_t kEwTile = 64;   // groups per tile (4096 entries)
+
+// largest v in [0, n] with off[v] <= x (off[0] = 0 <= x): a 64-ary search
+__device__ uint32_t ew_find(const uint64_t *off, uint32_t n, uint64_t x)
 {
-    WAVE_STRIDE(v, (uint64_t)n)
+    const uint64_t lane = threadIdx.x & 63;
+    uint64_t lo = 0, hi = (uint64_t)n + 1;   // off[lo] <= x, the answer lies in [lo, hi)
+    while (hi - lo > 1) {
+        const uint64_t step = (hi - lo + 63) >> 6, p = lo + lane * step;
+        const uint64_t bal = __ballot(p < hi && off[p] <= x);   // a prefix of the lanes, lane 0 included
+        lo += (uint64_t)(63 - __clzll(bal)) * step;
+        hi = hi < lo + step ? hi : lo + step;
+    }
+    return (uint32_t)lo;
+}
+
+struct EntryWalk {
+    const uint64_t *off, *sadj;
+    uint64_t m2, base;   // 2m entries; first entry of the next group
+    uint32_t n, v;       // v: a vertex with off[v] <= base
+    uint32_t cu, cnb;    // owner and neighbour of entry base - 1 (kNone at a list start)
+    // after next(), for entry base + lane of the group just walked:
+    uint32_t u, nb;      // owner and neighbour
+    bool valid, first;   // entry exists; first of its run of equal neighbours in its list
+
+    __device__ EntryWalk(const uint64_t *off_, const uint64_t *sadj_, uint32_t n_, uint64_t m2_, uint64_t b)
+        : off(off_), sadj(sadj_), m2(m2_), base(b), n(n_)
     {
-        const uint64_t b = off[v], e = off[v + 1];
-        uint64_t d = 0;
-        uint32_t carry = 0xffffffffu;
-        for (uint64_t base = b; base < e; base += 64) {
-            const uint32_t nb = nbr_or_none(sadj, base + (threadIdx.x & 63), e);
-            const bool q = first_of_run(nb, carry) && before(sdeg, (uint32_t)v, nb);
-            d += (uint64_t)__popcll(__ballot(q));
+        v = ew_find(off, n, b);
+        const bool mid = off[v] < b;
+        cu = mid ? v : kNone;
+        cnb = mid ? (uint32_t)(sadj[b - 1] >> 32) : kNone;
+    }
+
+    __device__ void next()
+    {
+        const uint32_t lane = threadIdx.x & 63;
+        const uint64_t i = base + lane;
+        valid = i < m2;
+        nb = valid ? (uint32_t)(sadj[i] >> 32) : kNone;
+        // owner(i) = v + #{k >= 1 : off[v + k] <= i}, 64 offsets a round: r is where
+        // vertex w starts relative to the group (0 before it, 64 past it), ascending
+        // over the lanes, and a binary search over the lanes counts the r <= lane
+        uint32_t cnt = 0;
+        for (uint64_t w0 = (uint64_t)v + 1;; w0 += 64) {
+            const uint64_t w = w0 + lane;
+            const uint64_t o = w <= n ? off[w] : ~0ull;
+            const uint32_t r = o <= base ? 0u : (o - base >= 64 ? 64u : (uint32_t)(o - base));
+            uint32_t pos = 0;
+            for (uint32_t s = 32; s; s >>= 1) pos += __shfl(r, (int)(pos + s - 1), 64) <= lane ? s : 0;
+            const uint32_t r63 = __shfl(r, 63, 64);
+            pos += (pos == 63 && r63 <= lane) ? 1 : 0;
+            cnt += pos;
+            if (r63 >= 64 || w0 + 64 > n) break;   // no later vertex starts inside the group
         }
-        if ((threadIdx.x & 63) == 0) odeg[v] = d;
+        u = v + cnt;
+        uint32_t pu = __shfl_up(u, 1, 64), pn = __shfl_up(nb, 1, 64);
+        if (lane == 0) { pu = cu; pn = cnb; }
+        first = valid && (u != pu || nb != pn);
+        cu = __shfl(u, 63, 64);
+        cnb = __shfl(nb, 63, 64);
+        v = cu;   // off[cu] <= base + 63
+        base += 64;
+    }
+};
+
+// every wavefront's tiles; g0..g1 are the tile's groups
+#define EW_TILES(m2)                                                                                \
+    const uint64_t ew_groups = ((m2) + 63) >> 6;                                                    \
+    for (uint64_t ew_t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;                     \
+         ew_t * kEwTile < ew_groups; ew_t += ((uint64_t)gridDim.x * blockDim.x) >> 6)
+
+// bit j of fmask[g]: entry 64g + j is the first of its run of equal neighbours
+// (the parallel copies of an edge collapse onto the first)
+__global__ __launch_bounds__(256) void k_ew_first(const uint64_t *off, const uint64_t *sadj, uint32_t n, uint64_t m2,
+                                                  uint64_t *fmask)
+{
+    EW_TILES(m2)
+    {
+        const uint64_t g0 = ew_t * kEwTile, g1 = g0 + kEwTile < ew_groups ? g0 + kEwTile : ew_groups;
+        EntryWalk w(off, sadj, n, m2, g0 << 6);
+        for (uint64_t g = g0; g < g1; ++g) {
+            w.next();
+            const uint64_t bal = __ballot(w.first);
+            if ((threadIdx.x & 63) == 0) fmask[g] = bal;
+        }
     }
 }
 
-// out-lists in neighbour order (ballot compaction keeps the order)
-__global__ void k_out_fill(const uint64_t *off, const uint64_t *sadj, const uint32_t *sdeg, uint32_t n,
-                           const uint64_t *ooff, uint32_t *onb, uint32_t *osrc)
+// bit j of mask[g]: entry 64g + j is the first of its run and its neighbour
+// comes after (DIR 0: out-lists) or before (DIR 1: in-lists) its owner in the
+// (simple degree, id) order
+template <int DIR>
+__global__ __launch_bounds__(256) void k_ew_orient(const uint64_t *off, const uint64_t *sadj, uint32_t n, uint64_t m2,
+                                                   const uint32_t *sdeg, uint64_t *mask)
 {
-    const uint32_t lane = threadIdx.x & 63;
-    WAVE_STRIDE(v, (uint64_t)n)
+    EW_TILES(m2)
     {
-        const uint64_t b = off[v], e = off[v + 1];
-        uint64_t w = ooff[v];
-        uint32_t carry = 0xffffffffu;
-        for (uint64_t base = b; base < e; base += 64) {
-            const uint32_t nb = nbr_or_none(sadj, base + lane, e);
-            const bool q = first_of_run(nb, carry) && before(sdeg, (uint32_t)v, nb);
-            const uint64_t mask = __ballot(q);
-            if (q) {
-                const uint64_t at = w + (uint64_t)__popcll(mask & ((1ull << lane) - 1));
-                onb[at] = nb;
-                osrc[at] = (uint32_t)v;
-            }
-            w += (uint64_t)__popcll(mask);
+        const uint64_t g0 = ew_t * kEwTile, g1 = g0 + kEwTile < ew_groups ? g0 + kEwTile : ew_groups;
+        EntryWalk w(off, sadj, n, m2, g0 << 6);
+        for (uint64_t g = g0; g < g1; ++g) {
+            w.next();
+            const bool q = w.first && (DIR == 0 ? before(sdeg, w.u, w.nb) : before(sdeg, w.nb, w.u));
+            const uint64_t bal = __ballot(q);
+            if ((threadIdx.x & 63) == 0) mask[g] = bal;
         }
     }
+}
+
+// The entries flagged in mask, in entry order (lists stay in neighbour order):
+// the k-th flagged entry's neighbour to nbr[k] and its owner to own[k] (own may
+// be null); pre[g] = flagged entries before group g.
+__global__ __launch_bounds__(256) void k_ew_compact(const uint64_t *off, const uint64_t *sadj, uint32_t n, uint64_t m2,
+                                                    const uint64_t *mask, const uint64_t *pre, uint32_t *nbr,
+                                                    uint32_t *own)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    EW_TILES(m2)
+    {
+        const uint64_t g0 = ew_t * kEwTile, g1 = g0 + kEwTile < ew_groups ? g0 + kEwTile : ew_groups;
+        EntryWalk w(off, sadj, n, m2, g0 << 6);
+        for (uint64_t g = g0; g < g1; ++g) {
+            w.next();
+            const uint64_t gu = uni64(g), mk = mask[gu];
+            if ((mk >> lane) & 1) {
+                const uint64_t at = pre[gu] + (uint64_t)__popcll(mk & ((1ull << lane) - 1));
+                nbr[at] = w.nb;
+                if (own) own[at] = w.u;
+            }
+        }
+    }
+}
+
+// List offsets of the flagged entries: out[w] = flagged entries before off[w]
+// (off[n] = m2 gives the total, as pre[groups] holds it and mask[groups] = 0).
+__global__ void k_ew_offsets(const uint64_t *off, uint32_t n, const uint64_t *mask, const uint64_t *pre,
+                             uint64_t *out)
+{
+    GRID_STRIDE(w, (uint64_t)n + 1)
+    {
+        const uint64_t i = off[w], g = i >> 6;
+        out[w] = pre[g] + (uint64_t)__popcll(mask[g] & ((1ull << (i & 63)) - 1));
+    }
+}
+
+struct Popc {
+    __device__ __host__ uint64_t operator()(uint64_t x) const { return (uint64_t)__builtin_popcountll(x); }
+};
+
+// simple degree from list offsets (first-of-run counts, or the lists themselves
+// on a graph without parallel edges)
+__global__ void k_deg_from_offsets(const uint64_t *o, uint32_t n, uint32_t *sdeg)
+{
+    GRID_STRIDE(v, (uint64_t)n) sdeg[v] = (uint32_t)(o[v + 1] - o[v]);
+}
+
+// largest gap o[v + 1] - o[v] (the longest list)
+__global__ void k_max_gap(const uint64_t *o, uint32_t n, unsigned long long *mx)
+{
+    unsigned long long b = 0;
+    GRID_STRIDE(v, (uint64_t)n)
+    {
+        const unsigned long long d = o[v + 1] - o[v];
+        b = d > b ? d : b;
+    }
+    for (int s = 32; s > 0; s >>= 1) {
+        const unsigned long long x = __shfl_xor(b, s, 64);
+        b = x > b ? x : b;
+    }
+    if ((threadIdx.x & 63) == 0 && b) atomicMax(mx, b);
 }
 
 __device__ __forceinline__ bool bsearch_u32(const uint32_t *a, uint64_t n, uint32_t x)
@@ -160,7 +287,7 @@ __device__ __forceinline__ void add128(unsigned long long &lo, unsigned long lon
 
 __device__ __forceinline__ unsigned long long block_sum(unsigned long long v)
 {
-    __shared__ unsigned long long sm[4];
+    __shared__ unsigned long long sm[16];   // blocks of up to 1024 threads
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
@@ -258,6 +385,233 @@ __global__ __launch_bounds__(256) void k_tri_wave(const uint64_t *ooff, const ui
         const unsigned long long tw = block_sum(wsum);
         if (threadIdx.x == 0) wpartial[blockIdx.x] = tw;
     }
+}
+
+// Triangles by head: every triangle a -> b -> c (a -> c) is found once, at
+// its middle vertex b, as an entry c of out(a) that is also in out(b).  A work
+// item is a head v and up to kTriChunk entries of its in-list: out(v) goes
+// into an LDS hash table, then the out-lists of the item's tails u stream
+// through it, 64 tails at a time.  The lanes take 64 consecutive entries of
+// the tails' concatenated lists per load (a wavefront-uniform cursor hands
+// them out, so a long list is read in full lines and a short one costs one
+// scalar step), kTriUnroll loads in flight per lane.  The stream reads sum over
+// tails of dout(u)^2 list entries, each list once per edge out of its tail.
+// Tables are at most a quarter full: heads with up to kTriLight
+// out-neighbours take one wavefront and 2048 slots, larger ones a block of
+// 16 wavefronts sharing 16384 slots, and past 4096 out-neighbours a binary
+// search of out(v) replaces the table.
+constexpr uint32_t kTriChunk = 1024;            // in-list entries per wavefront work item
+constexpr uint32_t kTriChunkHeavy = 8192;       // in-list entries per block work item
+constexpr uint32_t kTriHeavyThreads = 1024;
+constexpr int kTriWaveBits = 11, kTriBlockBits = 14;
+constexpr uint64_t kTriLight = 1u << (kTriWaveBits - 2);   // out-degree limit of a wavefront item
+constexpr int kTriUnroll = 8;
+
+__device__ __forceinline__ uint32_t id_hash(uint32_t x, uint32_t bits) { return (x * 0x9E3779B1u) >> (32 - bits); }
+
+// work items of head v: ceil(din / kTriChunk) when v has out-neighbours in
+// the class (HEAVY: more than kTriLight of them)
+template <bool HEAVY>
+__global__ void k_tri_items_count(const uint64_t *ooff, const uint64_t *iin, uint32_t n, uint64_t *cnt)
+{
+    GRID_STRIDE(v, (uint64_t)n + 1)
+    {
+        uint64_t c = 0;
+        if (v < n) {
+            const uint64_t dv = ooff[v + 1] - ooff[v];
+            const uint64_t chunk = HEAVY ? kTriChunkHeavy : kTriChunk;
+            if (dv > 0 && (dv > kTriLight) == HEAVY) c = (iin[v + 1] - iin[v] + chunk - 1) / chunk;
+        }
+        cnt[v] = c;
+    }
+}
+
+__global__ void k_tri_items_fill(const uint64_t *ipre, uint32_t n, uint64_t *items)
+{
+    GRID_STRIDE(v, (uint64_t)n)
+    {
+        for (uint64_t k = ipre[v]; k < ipre[v + 1]; ++k) items[k] = (v << 32) | (k - ipre[v]);
+    }
+}
+
+// insert out(v) into T (2^bits slots, cleared), by `lanes` threads from index t
+__device__ __forceinline__ void tri_insert(uint32_t *T, uint32_t bits, const uint32_t *list, uint64_t dv, uint32_t t,
+                                           uint32_t lanes)
+{
+    const uint32_t hm = (1u << bits) - 1;
+    for (uint64_t k = t; k < dv; k += lanes) {
+        const uint32_t x = list[k];
+        uint32_t h = id_hash(x, bits);
+        while (atomicCAS(&T[h], kNone, x) != kNone) h = (h + 1) & hm;
+    }
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, uint32_t l)
+{
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(x >> 32), (int)l) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, (int)l);
+}
+
+// Tails sb + lane (< ie) of the item: entries of their out-lists found in the
+// head's table T (2^bits slots) or, when T is null, in the sorted list vl of
+// dv entries.
+__device__ unsigned long long tri_stream(const uint64_t *ooff, const uint32_t *onb, const uint32_t *inb, uint64_t sb,
+                                         uint64_t ie, const uint32_t *T, uint32_t bits, const uint32_t *vl,
+                                         uint64_t dv)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t si = sb + lane;
+    uint64_t b = 0;
+    uint32_t d = 0;
+    if (si < ie) {
+        const uint32_t u = inb[si];
+        b = ooff[u];
+        d = (uint32_t)(ooff[u + 1] - b);
+    }
+    const uint64_t nz = __ballot(d > 0);   // tails with a non-empty list
+    if (!nz) return 0;
+    const uint32_t hm = (1u << bits) - 1;
+    // cursor (uniform): tail cs, position cp in its list of cd entries at cb
+    uint32_t cs = (uint32_t)__builtin_ctzll(nz), cp = 0;
+    uint32_t cd = (uint32_t)__builtin_amdgcn_readlane((int)d, (int)cs);
+    uint64_t cb = readlane64(b, cs);
+    unsigned long long c = 0;
+    while (cs < 64) {
+        uint32_t x[kTriUnroll];
+#pragma unroll
+        for (int k = 0; k < kTriUnroll; ++k) {
+            // the next 64 entries: lane l takes entry `filled + l` of the window
+            uint64_t at = 0;
+            bool ok = false;
+            uint32_t filled = 0;
+            if (cs < 64 && cd - cp > 64) {   // inside one list, short of its end
+                at = cb + cp + lane;
+                ok = true;
+                cp += 64;
+                filled = 64;
+            }
+            while (cs < 64 && filled < 64) {
+                const uint32_t take = cd - cp < 64 - filled ? cd - cp : 64 - filled;
+                if (lane >= filled && lane < filled + take) {
+                    at = cb + cp + (lane - filled);
+                    ok = true;
+                }
+                filled += take;
+                cp += take;
+                if (cp == cd) {
+                    const uint64_t rest = cs < 63 ? nz >> (cs + 1) : 0;
+                    cs = rest ? cs + 1 + (uint32_t)__builtin_ctzll(rest) : 64;
+                    cp = 0;
+                    if (cs < 64) {
+                        cd = (uint32_t)__builtin_amdgcn_readlane((int)d, (int)cs);
+                        cb = readlane64(b, cs);
+                    }
+                }
+            }
+            x[k] = ok ? onb[at] : kNone;
+        }
+        if (T) {
+            // first probes of all kTriUnroll entries together, then the ones
+            // that met another key step on together (linear probing)
+            uint32_t h[kTriUnroll], y[kTriUnroll], need = 0;
+#pragma unroll
+            for (int k = 0; k < kTriUnroll; ++k) h[k] = id_hash(x[k], bits);
+#pragma unroll
+            for (int k = 0; k < kTriUnroll; ++k) y[k] = T[h[k]];
+#pragma unroll
+            for (int k = 0; k < kTriUnroll; ++k) {
+                if (x[k] == kNone) continue;   // past the lists (an empty slot reads kNone too)
+                if (y[k] == x[k]) ++c;
+                else if (y[k] != kNone) need |= 1u << k;
+            }
+            while (need) {
+#pragma unroll
+                for (int k = 0; k < kTriUnroll; ++k) {
+                    h[k] = (h[k] + ((need >> k) & 1)) & hm;
+                    y[k] = T[h[k]];
+                }
+#pragma unroll
+                for (int k = 0; k < kTriUnroll; ++k) {
+                    if (!((need >> k) & 1)) continue;
+                    if (y[k] == x[k]) { ++c; need &= ~(1u << k); }
+                    else if (y[k] == kNone) need &= ~(1u << k);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kTriUnroll; ++k)
+                if (x[k] != kNone) c += bsearch_u32(vl, dv, x[k]) ? 1 : 0;
+        }
+    }
+    return c;
+}
+
+// one wavefront per work item (heads with up to kTriLight out-neighbours)
+__global__ __launch_bounds__(256) void k_tri_light(const uint64_t *ooff, const uint32_t *onb, const uint64_t *iin,
+                                                   const uint32_t *inb, const uint64_t *items, uint64_t nitems,
+                                                   unsigned long long *partial)
+{
+    __shared__ uint32_t tab[4][1u << kTriWaveBits];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t *const T = tab[wv];
+    unsigned long long c = 0;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; it < nitems; it += nwaves) {
+        const uint64_t item = uni64(items[uni64(it)]);
+        const uint32_t v = (uint32_t)(item >> 32);
+        const uint64_t vb = ooff[v], dv = ooff[v + 1] - vb;
+        const uint64_t ib = iin[v] + (item & 0xffffffffull) * kTriChunk, iend = iin[v + 1];
+        const uint64_t ie = ib + kTriChunk < iend ? ib + kTriChunk : iend;
+        const uint32_t bits = kTriWaveBits;   // the whole table: fewer collisions than a quarter-full one
+        for (uint32_t t = lane; t < (1u << bits); t += 64) T[t] = kNone;
+        wave_sync();
+        tri_insert(T, bits, onb + vb, dv, lane, 64);
+        wave_sync();
+        for (uint64_t sb = ib; sb < ie; sb += 64) c += tri_stream(ooff, onb, inb, sb, ie, T, bits, nullptr, 0);
+        wave_sync();   // before the table is cleared
+    }
+    const unsigned long long t = block_sum(c);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// One block of 16 wavefronts per work item of a head with more than kTriLight
+// out-neighbours, sharing one table; the wavefronts claim the item's batches
+// of 64 tails from an LDS counter.
+__global__ __launch_bounds__(kTriHeavyThreads) void k_tri_heavy(const uint64_t *ooff, const uint32_t *onb,
+                                                                const uint64_t *iin, const uint32_t *inb,
+                                                                const uint64_t *items, uint64_t nitems,
+                                                                unsigned long long *partial)
+{
+    __shared__ uint32_t T[1u << kTriBlockBits];
+    __shared__ uint32_t next_batch;
+    const uint32_t lane = threadIdx.x & 63;
+    unsigned long long c = 0;
+    for (uint64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const uint64_t item = items[it];
+        const uint32_t v = (uint32_t)(item >> 32);
+        const uint64_t vb = ooff[v], dv = ooff[v + 1] - vb;
+        const uint64_t ib = iin[v] + (item & 0xffffffffull) * kTriChunkHeavy, iend = iin[v + 1];
+        const uint64_t ie = ib + kTriChunkHeavy < iend ? ib + kTriChunkHeavy : iend;
+        const bool hashed = 4 * dv <= (1u << kTriBlockBits);
+        const uint32_t bits = hashed ? kTriBlockBits : 0;
+        if (threadIdx.x == 0) next_batch = 0;
+        if (hashed)
+            for (uint32_t t = threadIdx.x; t < (1u << bits); t += blockDim.x) T[t] = kNone;
+        __syncthreads();
+        if (hashed) tri_insert(T, bits, onb + vb, dv, threadIdx.x, blockDim.x);
+        __syncthreads();
+        const uint32_t batches = (uint32_t)((ie - ib + 63) >> 6);
+        for (;;) {
+            uint32_t bi = 0;
+            if (lane == 0) bi = atomicAdd(&next_batch, 1u);
+            bi = __builtin_amdgcn_readfirstlane(bi);
+            if (bi >= batches) break;
+            c += tri_stream(ooff, onb, inb, ib + 64ull * bi, ie, hashed ? T : nullptr, bits, onb + vb, dv);
+        }
+        __syncthreads();   // before the table and the counter are reset
+    }
+    const unsigned long long t = block_sum(c);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
 // 4-cliques by bit rows: the binary-encoded induced subgraphs of Almasri et
@@ -452,25 +806,36 @@ __global__ __launch_bounds__(256) void k_k4_count_wide(const uint64_t *ooff, con
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-__global__ void k_wedges_paths(const uint64_t *off, const uint64_t *sadj, const uint32_t *sdeg, uint32_t n,
-                               unsigned long long *wed, unsigned long long *paths)
+// wedges: sum over vertices of C(d, 2), one per-block partial
+__global__ void k_wedges(const uint32_t *sdeg, uint32_t n, unsigned long long *partial)
 {
-    unsigned long long w = 0, p = 0;
-    WAVE_STRIDE(v, (uint64_t)n)
+    unsigned long long w = 0;
+    GRID_STRIDE(v, (uint64_t)n)
     {
         const unsigned long long d = sdeg[v];
-        if ((threadIdx.x & 63) == 0) w += d * (d - (d > 0)) / 2;
-        const uint64_t b = off[v], e = off[v + 1];
-        uint32_t carry = 0xffffffffu;
-        for (uint64_t base = b; base < e; base += 64) {
-            const uint32_t nb = nbr_or_none(sadj, base + (threadIdx.x & 63), e);
-            if (first_of_run(nb, carry) && (uint32_t)v < nb) p += (d - 1) * (unsigned long long)(sdeg[nb] - 1);
+        w += d * (d - (d > 0)) / 2;
+    }
+    const unsigned long long t = block_sum(w);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// sum over simple edges {u, v} of (d_u - 1)(d_v - 1), each edge taken at its
+// lower id's entry
+__global__ __launch_bounds__(256) void k_ew_paths(const uint64_t *off, const uint64_t *sadj, uint32_t n, uint64_t m2,
+                                                  const uint32_t *sdeg, unsigned long long *partial)
+{
+    unsigned long long p = 0;
+    EW_TILES(m2)
+    {
+        const uint64_t g0 = ew_t * kEwTile, g1 = g0 + kEwTile < ew_groups ? g0 + kEwTile : ew_groups;
+        EntryWalk w(off, sadj, n, m2, g0 << 6);
+        for (uint64_t g = g0; g < g1; ++g) {
+            w.next();
+            if (w.first && w.u < w.nb) p += (unsigned long long)(sdeg[w.u] - 1) * (sdeg[w.nb] - 1);
         }
     }
-    const unsigned long long tw = block_sum(w);
-    __syncthreads();
-    const unsigned long long tp = block_sum(p);
-    if (threadIdx.x == 0) { wed[blockIdx.x] = tw; paths[blockIdx.x] = tp; }
+    const unsigned long long t = block_sum(p);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
 // 5-stars: sum over vertices of C(d, 4) (naive_implementation/FiveStar.cpp:51-59),
@@ -495,27 +860,6 @@ __global__ __launch_bounds__(256) void k_stars4(const uint32_t *sdeg, uint32_t n
         const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
         part[2 * wave] = lo;
         part[2 * wave + 1] = hi;
-    }
-}
-
-__global__ void k_widen(const uint32_t *a, uint32_t n, uint64_t *b)
-{
-    GRID_STRIDE(v, (uint64_t)n) b[v] = a[v];
-}
-
-// full simple adjacency: distinct neighbours ascending by id, and each entry's owner
-__global__ void k_full_fill(const uint64_t *off, const uint64_t *sadj, uint32_t n, const uint64_t *foff,
-                            uint32_t *fnb, uint32_t *fsrc)
-{
-    GRID_STRIDE(v, (uint64_t)n)
-    {
-        uint64_t w = foff[v];
-        uint32_t prev = 0xffffffffu;
-        for (uint64_t i = off[v]; i < off[v + 1]; ++i) {
-            const uint32_t nb = (uint32_t)(sadj[i] >> 32);
-            if (nb != prev) { fnb[w] = nb; fsrc[w] = (uint32_t)v; ++w; }
-            prev = nb;
-        }
     }
 }
 
@@ -877,6 +1221,70 @@ int host_sum(const unsigned long long *d, int n, hipStream_t st, unsigned long l
     return ZG_OK;
 }
 
+inline int ew_grid(uint64_t groups)
+{
+    const uint64_t waves = (groups + kEwTile - 1) / kEwTile, cap = (uint64_t)device_cus() * 64;
+    return (int)std::max<uint64_t>(1, std::min((waves + 3) / 4, cap));
+}
+
+// Flags over the 2m list entries, one bit each (mask[groups] = 0), and their
+// exclusive prefix per group (pre[groups] = the flagged total).  kind -1: first
+// of its run; 0: out-list entry (neighbour after the owner in the order);
+// 1: in-list entry (neighbour before the owner).
+struct EwMask {
+    XB<uint64_t> mask, pre;
+};
+
+int ew_mask(zg_graph *g, const uint32_t *sdeg, int kind, EwMask &e, uint64_t *total)
+{
+    const uint64_t m2 = 2 * (uint64_t)g->m, groups = (m2 + 63) >> 6;
+    hipStream_t st = g->stream;
+    XALLOC(e.mask, groups + 1);
+    XALLOC(e.pre, groups + 1);
+    ZG_HIP(hipMemsetAsync(e.mask.p + groups, 0, 8, st));
+    const int grid = ew_grid(groups);
+    if (kind < 0)
+        hipLaunchKernelGGL(k_ew_first, dim3(grid), dim3(256), 0, st, g->off, g->sadj, g->n, m2, e.mask.p);
+    else if (kind == 0)
+        hipLaunchKernelGGL(k_ew_orient<0>, dim3(grid), dim3(256), 0, st, g->off, g->sadj, g->n, m2, sdeg, e.mask.p);
+    else
+        hipLaunchKernelGGL(k_ew_orient<1>, dim3(grid), dim3(256), 0, st, g->off, g->sadj, g->n, m2, sdeg, e.mask.p);
+    ZG_HIP(hipGetLastError());
+    const auto it = rocprim::make_transform_iterator(e.mask.p, Popc());
+    size_t tb = 0;
+    ZG_HIP(rocprim::exclusive_scan(nullptr, tb, it, e.pre.p, (uint64_t)0, (size_t)groups + 1,
+                                   rocprim::plus<uint64_t>(), st));
+    XB<char> tmp;
+    XALLOC(tmp, tb);
+    ZG_HIP(rocprim::exclusive_scan(tmp.p, tb, it, e.pre.p, (uint64_t)0, (size_t)groups + 1,
+                                   rocprim::plus<uint64_t>(), st));
+    ZG_HIP(hipMemcpyAsync(total, e.pre.p + groups, 8, hipMemcpyDeviceToHost, st));
+    ZG_HIP(hipStreamSynchronize(st));
+    return ZG_OK;
+}
+
+// The lists of the flagged entries (kind as for ew_mask), neighbours ascending
+// by id: offsets (n + 1 entries), neighbours and, when own is given, owners.
+int ew_lists(zg_graph *g, const uint32_t *sdeg, int kind, XB<uint64_t> &offs, XB<uint32_t> &nbr,
+             XB<uint32_t> *own, uint64_t *total)
+{
+    EwMask e;
+    int rc = ew_mask(g, sdeg, kind, e, total);
+    if (rc) return rc;
+    const uint64_t m2 = 2 * (uint64_t)g->m, groups = (m2 + 63) >> 6;
+    hipStream_t st = g->stream;
+    XALLOC(offs, (uint64_t)g->n + 1);
+    XALLOC(nbr, *total);
+    if (own) XALLOC(*own, *total);
+    hipLaunchKernelGGL(k_ew_offsets, dim3(gridx((uint64_t)g->n + 1)), dim3(256), 0, st, g->off, g->n, e.mask.p,
+                       e.pre.p, offs.p);
+    hipLaunchKernelGGL(k_ew_compact, dim3(ew_grid(groups)), dim3(256), 0, st, g->off, g->sadj, g->n, m2, e.mask.p,
+                       e.pre.p, nbr.p, own ? own->p : nullptr);
+    ZG_HIP(hipGetLastError());
+    ZG_HIP(hipStreamSynchronize(st));   // before the mask is freed
+    return ZG_OK;
+}
+
 }  // namespace
 }  // namespace zg
 
@@ -891,9 +1299,33 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
     const uint32_t n = g->n;
     hipStream_t st = g->stream;
     if (g->m == 0) { *out = 0; return ZG_OK; }
+    auto read_u64 = [&](const void *p, uint64_t *v) -> int {
+        ZG_HIP(hipMemcpyAsync(v, p, 8, hipMemcpyDeviceToHost, st));
+        ZG_HIP(hipStreamSynchronize(st));
+        return ZG_OK;
+    };
+    // Simple degrees: the first-of-run entries of each list, which the full
+    // simple lists (5-houses, 5-chains, 4-cycles) hold; without parallel edges
+    // the list lengths themselves.
     XB<uint32_t> sdeg;
+    XB<uint64_t> foff;          // full simple lists: offsets, neighbours, owners
+    XB<uint32_t> fnb, fsrc;
+    uint64_t nf = 2 * (uint64_t)g->m;
     XALLOC(sdeg, n);
-    hipLaunchKernelGGL(k_simple_deg, dim3(gridx(n)), dim3(256), 0, st, g->off, g->sadj, n, sdeg.p);
+    if (what >= 5) {
+        const int rc = ew_lists(g, nullptr, -1, foff, fnb, &fsrc, &nf);
+        if (rc) return rc;
+    } else if (!g->simple) {
+        EwMask fm;
+        const int rc = ew_mask(g, nullptr, -1, fm, &nf);
+        if (rc) return rc;
+        XALLOC(foff, (uint64_t)n + 1);
+        hipLaunchKernelGGL(k_ew_offsets, dim3(gridx((uint64_t)n + 1)), dim3(256), 0, st, g->off, n, fm.mask.p,
+                           fm.pre.p, foff.p);
+        ZG_HIP(hipStreamSynchronize(st));   // before the mask is freed
+    }
+    hipLaunchKernelGGL(k_deg_from_offsets, dim3(gridx(n)), dim3(256), 0, st, foff.p ? foff.p : g->off, n, sdeg.p);
+    ZG_HIP(hipGetLastError());
     if (what == 4) {
         const int grid = gridx(n);
         XB<unsigned long long> part;
@@ -910,24 +1342,7 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         return ZG_OK;
     }
     if (what == 5) {
-        XB<uint64_t> fdeg, foff;
-        XALLOC(fdeg, (uint64_t)n + 1); XALLOC(foff, (uint64_t)n + 1);
-        hipLaunchKernelGGL(k_widen, dim3(gridx(n)), dim3(256), 0, st, sdeg.p, n, fdeg.p);
-        ZG_HIP(hipMemsetAsync(fdeg.p + n, 0, 8, st));
-        size_t tb = 0;
-        ZG_HIP(rocprim::exclusive_scan(nullptr, tb, fdeg.p, foff.p, (uint64_t)0, (size_t)n + 1,
-                                       rocprim::plus<uint64_t>(), st));
-        XB<char> tmp;
-        XALLOC(tmp, tb);
-        ZG_HIP(rocprim::exclusive_scan(tmp.p, tb, fdeg.p, foff.p, (uint64_t)0, (size_t)n + 1,
-                                       rocprim::plus<uint64_t>(), st));
-        uint64_t ne = 0;
-        ZG_HIP(hipMemcpyAsync(&ne, foff.p + n, 8, hipMemcpyDeviceToHost, st));
-        ZG_HIP(hipStreamSynchronize(st));
-        XB<uint32_t> fnb, fsrc;
-        XALLOC(fnb, ne); XALLOC(fsrc, ne);
-        hipLaunchKernelGGL(k_full_fill, dim3(gridx(n)), dim3(256), 0, st, g->off, g->sadj, n, foff.p, fnb.p, fsrc.p);
-        ZG_HIP(hipGetLastError());
+        const uint64_t ne = nf;
         const int grid = (int)std::min<uint64_t>((uint64_t)device_cus() * 8, std::max<uint64_t>(1, (ne + 3) / 4));
         XB<unsigned long long> part;
         XALLOC(part, grid);
@@ -940,14 +1355,16 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         return ZG_OK;
     }
     if (what == 2 || what == 3) {
-        const int grid = gridx(n);
+        const int grid = gridx(n), pgrid = ew_grid((2 * (uint64_t)g->m + 63) >> 6);
         XB<unsigned long long> pw, pp;
-        XALLOC(pw, grid); XALLOC(pp, grid);
-        hipLaunchKernelGGL(k_wedges_paths, dim3(grid), dim3(256), 0, st, g->off, g->sadj, sdeg.p, n, pw.p, pp.p);
+        XALLOC(pw, grid); XALLOC(pp, pgrid);
+        hipLaunchKernelGGL(k_wedges, dim3(grid), dim3(256), 0, st, sdeg.p, n, pw.p);
+        hipLaunchKernelGGL(k_ew_paths, dim3(pgrid), dim3(256), 0, st, g->off, g->sadj, n, 2 * (uint64_t)g->m, sdeg.p,
+                           pp.p);
         ZG_HIP(hipGetLastError());
         unsigned long long w = 0, p = 0;
         int rc = host_sum(pw.p, grid, st, &w);
-        if (!rc) rc = host_sum(pp.p, grid, st, &p);
+        if (!rc) rc = host_sum(pp.p, pgrid, st, &p);
         if (rc) return rc;
         if (what == 2) { *out = w; return ZG_OK; }
         uint64_t T = 0;
@@ -956,71 +1373,79 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         *out = p - 3 * T;
         return ZG_OK;
     }
-    // oriented out-lists
-    XB<uint64_t> odeg, ooff;
-    XALLOC(odeg, (uint64_t)n + 1); XALLOC(ooff, (uint64_t)n + 1);
-    ZG_HIP(hipMemsetAsync(odeg.p + n, 0, 8, st));
-    hipLaunchKernelGGL(k_out_deg, dim3(gridx(n)), dim3(256), 0, st, g->off, g->sadj, sdeg.p, n, odeg.p);
-    size_t tb = 0;
-    ZG_HIP(rocprim::exclusive_scan(nullptr, tb, odeg.p, ooff.p, (uint64_t)0, (size_t)n + 1, rocprim::plus<uint64_t>(), st));
-    XB<char> tmp;
-    XALLOC(tmp, tb);
-    ZG_HIP(rocprim::exclusive_scan(tmp.p, tb, odeg.p, ooff.p, (uint64_t)0, (size_t)n + 1, rocprim::plus<uint64_t>(), st));
-    uint64_t ne = 0;
-    ZG_HIP(hipMemcpyAsync(&ne, ooff.p + n, 8, hipMemcpyDeviceToHost, st));
-    size_t tr = 0;
-    XB<uint64_t> mx;
-    XALLOC(mx, 1);
-    ZG_HIP(rocprim::reduce(nullptr, tr, odeg.p, mx.p, (uint64_t)0, (size_t)n, rocprim::maximum<uint64_t>(), st));
-    XB<char> tmp2;
-    XALLOC(tmp2, tr);
-    ZG_HIP(rocprim::reduce(tmp2.p, tr, odeg.p, mx.p, (uint64_t)0, (size_t)n, rocprim::maximum<uint64_t>(), st));
-    uint64_t maxout = 0;
-    ZG_HIP(hipMemcpyAsync(&maxout, mx.p, 8, hipMemcpyDeviceToHost, st));
-    ZG_HIP(hipStreamSynchronize(st));
+    // oriented out-lists (owners for the 4-clique rows and the weighted triangles)
+    XB<uint64_t> ooff;
     XB<uint32_t> onb, osrc;
-    XALLOC(onb, ne); XALLOC(osrc, ne);
-    hipLaunchKernelGGL(k_out_fill, dim3(gridx(n)), dim3(256), 0, st, g->off, g->sadj, sdeg.p, n, ooff.p, onb.p, osrc.p);
-    ZG_HIP(hipGetLastError());
-    if (what == 0) {
-        const int tgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16, std::max<uint64_t>(1, (ne + 3) / 4));
-        XB<unsigned long long> tpart;
-        XALLOC(tpart, tgrid);
-        hipLaunchKernelGGL(k_tri_wave<false>, dim3(tgrid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, ne, sdeg.p,
-                           tpart.p, nullptr);
-        ZG_HIP(hipGetLastError());
-        unsigned long long s = 0;
-        int rc = host_sum(tpart.p, tgrid, st, &s);
+    uint64_t ne = 0, maxout = 0;
+    if (what != 7) {
+        int rc = ew_lists(g, sdeg.p, 0, ooff, onb, what == 0 ? nullptr : &osrc, &ne);
         if (rc) return rc;
-        *out = s;
+        XB<unsigned long long> mx;
+        XALLOC(mx, 1);
+        ZG_HIP(hipMemsetAsync(mx.p, 0, 8, st));
+        hipLaunchKernelGGL(k_max_gap, dim3(gridx(n)), dim3(256), 0, st, ooff.p, n, mx.p);
+        ZG_HIP(hipGetLastError());
+        rc = read_u64(mx.p, &maxout);
+        if (rc) return rc;
+    }
+    if (what == 0) {
+        // in-lists: for every head v the tails u -> v, then work items of up to
+        // kTriChunk in-list entries of one head
+        XB<uint64_t> iin, icnt, ipre, items;
+        XB<uint32_t> inb;
+        uint64_t nin = 0, nitems = 0;
+        int rc = ew_lists(g, sdeg.p, 1, iin, inb, nullptr, &nin);
+        if (rc) return rc;
+        XALLOC(icnt, (uint64_t)n + 1);
+        XALLOC(ipre, (uint64_t)n + 1);
+        unsigned long long total = 0;
+        for (int heavy = 0; heavy < 2; ++heavy) {
+            if (heavy)
+                hipLaunchKernelGGL(k_tri_items_count<true>, dim3(gridx((uint64_t)n + 1)), dim3(256), 0, st, ooff.p,
+                                   iin.p, n, icnt.p);
+            else
+                hipLaunchKernelGGL(k_tri_items_count<false>, dim3(gridx((uint64_t)n + 1)), dim3(256), 0, st,
+                                   ooff.p, iin.p, n, icnt.p);
+            size_t tb = 0;
+            ZG_HIP(rocprim::exclusive_scan(nullptr, tb, icnt.p, ipre.p, (uint64_t)0, (size_t)n + 1,
+                                           rocprim::plus<uint64_t>(), st));
+            XB<char> tmp;
+            XALLOC(tmp, tb);
+            ZG_HIP(rocprim::exclusive_scan(tmp.p, tb, icnt.p, ipre.p, (uint64_t)0, (size_t)n + 1,
+                                           rocprim::plus<uint64_t>(), st));
+            rc = read_u64(ipre.p + n, &nitems);
+            if (rc) return rc;
+            if (nitems == 0) continue;
+            items.reset();
+            XALLOC(items, nitems);
+            hipLaunchKernelGGL(k_tri_items_fill, dim3(gridx(n)), dim3(256), 0, st, ipre.p, n, items.p);
+            const uint64_t per_block = heavy ? 1 : 4, cap = (uint64_t)device_cus() * (heavy ? 2 : 8);
+            const int tgrid = (int)std::min<uint64_t>(cap, std::max<uint64_t>(1, (nitems + per_block - 1) / per_block));
+            XB<unsigned long long> tpart;
+            XALLOC(tpart, tgrid);
+            if (heavy)
+                hipLaunchKernelGGL(k_tri_heavy, dim3(tgrid), dim3(kTriHeavyThreads), 0, st, ooff.p, onb.p, iin.p,
+                                   inb.p, items.p, nitems, tpart.p);
+            else
+                hipLaunchKernelGGL(k_tri_light, dim3(tgrid), dim3(256), 0, st, ooff.p, onb.p, iin.p, inb.p, items.p,
+                                   nitems, tpart.p);
+            ZG_HIP(hipGetLastError());
+            unsigned long long s = 0;
+            rc = host_sum(tpart.p, tgrid, st, &s);
+            if (rc) return rc;
+            total += s;
+        }
+        *out = total;
         return ZG_OK;
     }
     if (what == 6 || what == 7) {
         // full simple adjacency as order keys, every list sorted by key
-        XB<uint64_t> fdeg, foff;
-        XALLOC(fdeg, (uint64_t)n + 1); XALLOC(foff, (uint64_t)n + 1);
-        hipLaunchKernelGGL(k_widen, dim3(gridx(n)), dim3(256), 0, st, sdeg.p, n, fdeg.p);
-        ZG_HIP(hipMemsetAsync(fdeg.p + n, 0, 8, st));
-        size_t tf = 0;
-        ZG_HIP(rocprim::exclusive_scan(nullptr, tf, fdeg.p, foff.p, (uint64_t)0, (size_t)n + 1,
-                                       rocprim::plus<uint64_t>(), st));
-        XB<char> tmpf;
-        XALLOC(tmpf, tf);
-        ZG_HIP(rocprim::exclusive_scan(tmpf.p, tf, fdeg.p, foff.p, (uint64_t)0, (size_t)n + 1,
-                                       rocprim::plus<uint64_t>(), st));
-        uint64_t nf = 0;
-        ZG_HIP(hipMemcpyAsync(&nf, foff.p + n, 8, hipMemcpyDeviceToHost, st));
-        ZG_HIP(hipStreamSynchronize(st));
         if (nf >= (1ull << 32)) return set_error(ZG_EINVAL, "4-cycle and 5-chain counts need 2m < 2^32");
         XB<uint64_t> fkey;
         XALLOC(fkey, nf);
         {
-            XB<uint32_t> fnb, fsrc;
             XB<uint64_t> kin;
-            XALLOC(fnb, nf); XALLOC(fsrc, nf); XALLOC(kin, nf);
-            hipLaunchKernelGGL(k_full_fill, dim3(gridx(n)), dim3(256), 0, st, g->off, g->sadj, n, foff.p, fnb.p,
-                               fsrc.p);
-            ZG_HIP(hipGetLastError());
+            XALLOC(kin, nf);
             hipLaunchKernelGGL(k_keyify, dim3(gridx(nf)), dim3(256), 0, st, fnb.p, sdeg.p, nf, kin.p);
             ZG_HIP(hipGetLastError());
             size_t ts = 0;
@@ -1032,6 +1457,8 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                                                       foff.p + 1, 0, 64, st));
             ZG_HIP(hipStreamSynchronize(st));   // before the temporaries are freed
         }
+        fnb.reset();
+        fsrc.reset();
         // 4-cycles: LDS hashes, then the vertices that overflowed them with global hashes
         const int cgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 8, std::max<uint64_t>(1, ((uint64_t)n + 3) / 4));
         XB<unsigned long long> cpart;
@@ -1251,11 +1678,6 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
     XALLOC(tmpk, tk);
     ZG_HIP(rocprim::exclusive_scan(tmpk.p, tk, words.p, boff.p, (uint64_t)0, (size_t)n + 1,
                                    rocprim::plus<uint64_t>(), st));
-    auto read_u64 = [&](const uint64_t *p, uint64_t *v) -> int {
-        ZG_HIP(hipMemcpyAsync(v, p, 8, hipMemcpyDeviceToHost, st));
-        ZG_HIP(hipStreamSynchronize(st));
-        return ZG_OK;
-    };
     uint64_t total_words = 0;
     int rc = read_u64(boff.p + n, &total_words);
     if (rc) return rc;
