@@ -398,14 +398,14 @@ __global__ __launch_bounds__(256) void k_tri_wave(const uint64_t *ooff, const ui
 // tails of dout(u)^2 list entries, each list once per edge out of its tail.
 // Tables are at most a quarter full: heads with up to kTriLight
 // out-neighbours take one wavefront and 2048 slots, larger ones a block of
-// 16 wavefronts sharing 16384 slots, and past 4096 out-neighbours a binary
-// search of out(v) replaces the table.
+// 16 wavefronts sharing 16384 slots; past 4096 out-neighbours (or should a
+// table's build fail) a binary search of out(v) replaces the table.
 constexpr uint32_t kTriChunk = 1024;            // in-list entries per wavefront work item
 constexpr uint32_t kTriChunkHeavy = 8192;       // in-list entries per block work item
 constexpr uint32_t kTriHeavyThreads = 1024;
 constexpr int kTriWaveBits = 11, kTriBlockBits = 14;
 constexpr uint64_t kTriLight = 1u << (kTriWaveBits - 2);   // out-degree limit of a wavefront item
-constexpr int kTriUnroll = 8;
+constexpr int kTriUnroll = 8, kTriUnrollHeavy = 8;   // loads per lane per step (two steps in flight)
 
 __device__ __forceinline__ uint32_t id_hash(uint32_t x, uint32_t bits) { return (x * 0x9E3779B1u) >> (32 - bits); }
 
@@ -434,16 +434,38 @@ __global__ void k_tri_items_fill(const uint64_t *ipre, uint32_t n, uint64_t *ite
     }
 }
 
-// insert out(v) into T (2^bits slots, cleared), by `lanes` threads from index t
-__device__ __forceinline__ void tri_insert(uint32_t *T, uint32_t bits, const uint32_t *list, uint64_t dv, uint32_t t,
+// The head's table: 2^bits slots, a quarter full at most, cuckoo hashing
+// (Pagh and Rodler): a key sits in slot h1 or slot h2 of its own, so a lookup
+// is two LDS reads and no probe sequence.  Insertion evicts with atomic
+// exchanges, each evicted key moving to its other slot (Alcantara et al.'s
+// GPU scheme); a key still homeless after kCuckooMoves moves fails the build.
+constexpr int kCuckooMoves = 256;
+
+__device__ __forceinline__ uint32_t cuckoo_h1(uint32_t x, uint32_t bits) { return (x * 0x9E3779B1u) >> (32 - bits); }
+
+// h2 != h1: an odd step from h1 in a table of a power-of-two size
+__device__ __forceinline__ uint32_t cuckoo_h2(uint32_t x, uint32_t bits)
+{
+    return (cuckoo_h1(x, bits) + (((x * 0x85EBCA77u) >> (32 - bits)) | 1u)) & ((1u << bits) - 1);
+}
+
+// insert list[t], list[t + lanes], ...; false if one of them found no slot
+__device__ __forceinline__ bool tri_insert(uint32_t *T, uint32_t bits, const uint32_t *list, uint64_t dv, uint32_t t,
                                            uint32_t lanes)
 {
-    const uint32_t hm = (1u << bits) - 1;
+    bool placed = true;
     for (uint64_t k = t; k < dv; k += lanes) {
-        const uint32_t x = list[k];
-        uint32_t h = id_hash(x, bits);
-        while (atomicCAS(&T[h], kNone, x) != kNone) h = (h + 1) & hm;
+        uint32_t cur = list[k], pos = cuckoo_h1(cur, bits);
+        int moves = 0;
+        for (; moves < kCuckooMoves; ++moves) {
+            cur = atomicExch(&T[pos], cur);
+            if (cur == kNone) break;
+            const uint32_t p1 = cuckoo_h1(cur, bits);
+            pos = pos == p1 ? cuckoo_h2(cur, bits) : p1;
+        }
+        if (moves == kCuckooMoves) placed = false;
     }
+    return placed;
 }
 
 __device__ __forceinline__ uint64_t readlane64(uint64_t x, uint32_t l)
@@ -452,35 +474,33 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t x, uint32_t l)
            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, (int)l);
 }
 
-// Tails sb + lane (< ie) of the item: entries of their out-lists found in the
-// head's table T (2^bits slots) or, when T is null, in the sorted list vl of
-// dv entries.
-__device__ unsigned long long tri_stream(const uint64_t *ooff, const uint32_t *onb, const uint32_t *inb, uint64_t sb,
-                                         uint64_t ie, const uint32_t *T, uint32_t bits, const uint32_t *vl,
-                                         uint64_t dv)
-{
-    const uint32_t lane = threadIdx.x & 63;
-    const uint64_t si = sb + lane;
-    uint64_t b = 0;
-    uint32_t d = 0;
-    if (si < ie) {
-        const uint32_t u = inb[si];
-        b = ooff[u];
-        d = (uint32_t)(ooff[u + 1] - b);
+// Wavefront-uniform cursor over the concatenated out-lists of 64 tails (lane
+// s holds tail s's list: d entries at b): tail cs, position cp in its list of
+// cd entries at cb; fill() hands out the next U windows of 64 entries.
+struct TriCursor {
+    uint64_t nz, b, cb;   // nz: tails with a non-empty list
+    uint32_t d, cs, cp, cd;
+
+    __device__ TriCursor(uint64_t b_, uint32_t d_) : b(b_), d(d_)
+    {
+        nz = __ballot(d > 0);
+        cs = nz ? (uint32_t)__builtin_ctzll(nz) : 64;
+        cp = 0;
+        cd = cs < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)d, (int)cs) : 0;
+        cb = cs < 64 ? readlane64(b, cs) : 0;
     }
-    const uint64_t nz = __ballot(d > 0);   // tails with a non-empty list
-    if (!nz) return 0;
-    const uint32_t hm = (1u << bits) - 1;
-    // cursor (uniform): tail cs, position cp in its list of cd entries at cb
-    uint32_t cs = (uint32_t)__builtin_ctzll(nz), cp = 0;
-    uint32_t cd = (uint32_t)__builtin_amdgcn_readlane((int)d, (int)cs);
-    uint64_t cb = readlane64(b, cs);
-    unsigned long long c = 0;
-    while (cs < 64) {
-        uint32_t x[kTriUnroll];
+    __device__ bool more() const { return cs < 64; }
+
+    // bit k of the result: the lane's entry of window k exists (x[k] is loaded
+    // either way, so no select waits for the loads)
+    template <int U>
+    __device__ __forceinline__ uint32_t fill(const uint32_t *onb, uint32_t (&x)[U])
+    {
+        const uint32_t lane = threadIdx.x & 63;
+        uint32_t okbits = 0;
 #pragma unroll
-        for (int k = 0; k < kTriUnroll; ++k) {
-            // the next 64 entries: lane l takes entry `filled + l` of the window
+        for (int k = 0; k < U; ++k) {
+            // lane l takes entry `filled + l` of the window
             uint64_t at = 0;
             bool ok = false;
             uint32_t filled = 0;
@@ -508,40 +528,71 @@ __device__ unsigned long long tri_stream(const uint64_t *ooff, const uint32_t *o
                     }
                 }
             }
-            x[k] = ok ? onb[at] : kNone;
+            x[k] = onb[ok ? at : 0];   // every lane loads, so the loads are not branched over
+            okbits |= (ok ? 1u : 0u) << k;
         }
-        if (T) {
-            // first probes of all kTriUnroll entries together, then the ones
-            // that met another key step on together (linear probing)
-            uint32_t h[kTriUnroll], y[kTriUnroll], need = 0;
+        return okbits;
+    }
+};
+
+// entries x[k] with bit k of ok set found in the table T (2^bits slots) or,
+// when T is null, in the sorted list vl of dv entries
+template <int U>
+__device__ __forceinline__ unsigned long long tri_probe(const uint32_t (&x)[U], uint32_t ok, const uint32_t *T,
+                                                        uint32_t bits, const uint32_t *vl, uint64_t dv)
+{
+    unsigned long long c = 0;
+    if (T) {
+        // two reads per entry; a lane with nothing to look up reads slot 0,
+        // which all such lanes share (a broadcast, no bank conflict)
+        uint32_t y1[U], y2[U];
 #pragma unroll
-            for (int k = 0; k < kTriUnroll; ++k) h[k] = id_hash(x[k], bits);
-#pragma unroll
-            for (int k = 0; k < kTriUnroll; ++k) y[k] = T[h[k]];
-#pragma unroll
-            for (int k = 0; k < kTriUnroll; ++k) {
-                if (x[k] == kNone) continue;   // past the lists (an empty slot reads kNone too)
-                if (y[k] == x[k]) ++c;
-                else if (y[k] != kNone) need |= 1u << k;
-            }
-            while (need) {
-#pragma unroll
-                for (int k = 0; k < kTriUnroll; ++k) {
-                    h[k] = (h[k] + ((need >> k) & 1)) & hm;
-                    y[k] = T[h[k]];
-                }
-#pragma unroll
-                for (int k = 0; k < kTriUnroll; ++k) {
-                    if (!((need >> k) & 1)) continue;
-                    if (y[k] == x[k]) { ++c; need &= ~(1u << k); }
-                    else if (y[k] == kNone) need &= ~(1u << k);
-                }
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < kTriUnroll; ++k)
-                if (x[k] != kNone) c += bsearch_u32(vl, dv, x[k]) ? 1 : 0;
+        for (int k = 0; k < U; ++k) {
+            const bool v = (ok >> k) & 1;
+            y1[k] = T[v ? cuckoo_h1(x[k], bits) : 0];
+            y2[k] = T[v ? cuckoo_h2(x[k], bits) : 0];
         }
+#pragma unroll
+        for (int k = 0; k < U; ++k) c += (((ok >> k) & 1) && (y1[k] == x[k] || y2[k] == x[k])) ? 1 : 0;
+    } else {
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            if ((ok >> k) & 1) c += bsearch_u32(vl, dv, x[k]) ? 1 : 0;
+    }
+    return c;
+}
+
+// Tails sb + lane (< ie) of the item: entries of their out-lists found in the
+// head's table.  Software-pipelined: the next U windows' loads are in flight
+// while the current ones probe the table.
+template <int U>
+__device__ __forceinline__ unsigned long long tri_stream(const uint64_t *ooff, const uint32_t *onb, const uint32_t *inb, uint64_t sb,
+                                         uint64_t ie, const uint32_t *T, uint32_t bits, const uint32_t *vl,
+                                         uint64_t dv)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t si = sb + lane;
+    uint64_t b = 0;
+    uint32_t d = 0;
+    if (si < ie) {
+        const uint32_t u = inb[si];
+        b = ooff[u];
+        d = (uint32_t)(ooff[u + 1] - b);
+    }
+    TriCursor cur(b, d);
+    if (!cur.more()) return 0;
+    unsigned long long c = 0;
+    uint32_t xa[U], xb[U];
+    uint32_t oka = cur.fill<U>(onb, xa), okb;
+    for (;;) {   // two buffers in turn, no copies (a copy would wait for the loads)
+        bool more = cur.more();
+        okb = cur.fill<U>(onb, xb);
+        c += tri_probe<U>(xa, oka, T, bits, vl, dv);
+        if (!more) break;
+        more = cur.more();
+        oka = cur.fill<U>(onb, xa);
+        c += tri_probe<U>(xb, okb, T, bits, vl, dv);
+        if (!more) break;
     }
     return c;
 }
@@ -562,12 +613,13 @@ __global__ __launch_bounds__(256) void k_tri_light(const uint64_t *ooff, const u
         const uint64_t vb = ooff[v], dv = ooff[v + 1] - vb;
         const uint64_t ib = iin[v] + (item & 0xffffffffull) * kTriChunk, iend = iin[v + 1];
         const uint64_t ie = ib + kTriChunk < iend ? ib + kTriChunk : iend;
-        const uint32_t bits = kTriWaveBits;   // the whole table: fewer collisions than a quarter-full one
+        const uint32_t bits = kTriWaveBits;
         for (uint32_t t = lane; t < (1u << bits); t += 64) T[t] = kNone;
         wave_sync();
-        tri_insert(T, bits, onb + vb, dv, lane, 64);
+        const bool built = __ballot(!tri_insert(T, bits, onb + vb, dv, lane, 64)) == 0;
         wave_sync();
-        for (uint64_t sb = ib; sb < ie; sb += 64) c += tri_stream(ooff, onb, inb, sb, ie, T, bits, nullptr, 0);
+        for (uint64_t sb = ib; sb < ie; sb += 64)
+            c += tri_stream<kTriUnroll>(ooff, onb, inb, sb, ie, built ? T : nullptr, bits, onb + vb, dv);
         wave_sync();   // before the table is cleared
     }
     const unsigned long long t = block_sum(c);
@@ -577,13 +629,13 @@ __global__ __launch_bounds__(256) void k_tri_light(const uint64_t *ooff, const u
 // One block of 16 wavefronts per work item of a head with more than kTriLight
 // out-neighbours, sharing one table; the wavefronts claim the item's batches
 // of 64 tails from an LDS counter.
-__global__ __launch_bounds__(kTriHeavyThreads) void k_tri_heavy(const uint64_t *ooff, const uint32_t *onb,
+__global__ __launch_bounds__(kTriHeavyThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tri_heavy(const uint64_t *ooff, const uint32_t *onb,
                                                                 const uint64_t *iin, const uint32_t *inb,
                                                                 const uint64_t *items, uint64_t nitems,
                                                                 unsigned long long *partial)
 {
     __shared__ uint32_t T[1u << kTriBlockBits];
-    __shared__ uint32_t next_batch;
+    __shared__ uint32_t next_batch, failed;
     const uint32_t lane = threadIdx.x & 63;
     unsigned long long c = 0;
     for (uint64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
@@ -592,21 +644,22 @@ __global__ __launch_bounds__(kTriHeavyThreads) void k_tri_heavy(const uint64_t *
         const uint64_t vb = ooff[v], dv = ooff[v + 1] - vb;
         const uint64_t ib = iin[v] + (item & 0xffffffffull) * kTriChunkHeavy, iend = iin[v + 1];
         const uint64_t ie = ib + kTriChunkHeavy < iend ? ib + kTriChunkHeavy : iend;
-        const bool hashed = 4 * dv <= (1u << kTriBlockBits);
-        const uint32_t bits = hashed ? kTriBlockBits : 0;
-        if (threadIdx.x == 0) next_batch = 0;
-        if (hashed)
+        const bool fits = 4 * dv <= (1u << kTriBlockBits);
+        const uint32_t bits = kTriBlockBits;
+        if (threadIdx.x == 0) { next_batch = 0; failed = 0; }
+        if (fits)
             for (uint32_t t = threadIdx.x; t < (1u << bits); t += blockDim.x) T[t] = kNone;
         __syncthreads();
-        if (hashed) tri_insert(T, bits, onb + vb, dv, threadIdx.x, blockDim.x);
+        if (fits && !tri_insert(T, bits, onb + vb, dv, threadIdx.x, blockDim.x)) failed = 1;
         __syncthreads();
+        const bool hashed = fits && !failed;
         const uint32_t batches = (uint32_t)((ie - ib + 63) >> 6);
         for (;;) {
             uint32_t bi = 0;
             if (lane == 0) bi = atomicAdd(&next_batch, 1u);
             bi = __builtin_amdgcn_readfirstlane(bi);
             if (bi >= batches) break;
-            c += tri_stream(ooff, onb, inb, ib + 64ull * bi, ie, hashed ? T : nullptr, bits, onb + vb, dv);
+            c += tri_stream<kTriUnrollHeavy>(ooff, onb, inb, ib + 64ull * bi, ie, hashed ? T : nullptr, bits, onb + vb, dv);
         }
         __syncthreads();   // before the table and the counter are reset
     }
