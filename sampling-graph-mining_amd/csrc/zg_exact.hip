@@ -77,10 +77,20 @@ __device__ __forceinline__ uint64_t uni64(uint64_t x)
            __builtin_amdgcn_readfirstlane((uint32_t)x);
 }
 
-__device__ __forceinline__ bool before(const uint32_t *sdeg, uint32_t a, uint32_t b)
+// The orientation's total order: (simple degree, id), with degrees capped at
+// 65535 so the key array is 2 bytes a vertex.  A vertex of degree d below the
+// cap still points only to vertices of degree >= d, so its out-list stays
+// within sqrt(2m); one at the cap points to at most 2m / 65535 others, also
+// within sqrt(2m) while m <= 2^31.
+__device__ __forceinline__ bool before(const uint16_t *okey, uint32_t a, uint32_t b)
 {
-    const uint32_t da = sdeg[a], db = sdeg[b];
+    const uint32_t da = okey[a], db = okey[b];
     return da < db || (da == db && a < b);
+}
+
+__global__ void k_order_key(const uint32_t *sdeg, uint32_t n, uint16_t *okey)
+{
+    GRID_STRIDE(v, (uint64_t)n) okey[v] = (uint16_t)(sdeg[v] < 65535u ? sdeg[v] : 65535u);
 }
 
 // ---- entry walk -----------------------------------------------------------------
@@ -180,12 +190,12 @@ __global__ __launch_bounds__(256) void k_ew_first(const uint64_t *off, const uin
     }
 }
 
-// bit j of mask[g]: entry 64g + j is the first of its run and its neighbour
-// comes after (DIR 0: out-lists) or before (DIR 1: in-lists) its owner in the
-// (simple degree, id) order
-template <int DIR>
+// bit j of omask[g]: entry 64g + j is the first of its run and its neighbour
+// comes after its owner in the order (an out-list entry); with IN, bit j of
+// imask[g] the same for a neighbour before its owner (an in-list entry).
+template <bool IN>
 __global__ __launch_bounds__(256) void k_ew_orient(const uint64_t *off, const uint64_t *sadj, uint32_t n, uint64_t m2,
-                                                   const uint32_t *sdeg, uint64_t *mask)
+                                                   const uint16_t *okey, uint64_t *omask, uint64_t *imask)
 {
     EW_TILES(m2)
     {
@@ -193,9 +203,13 @@ __global__ __launch_bounds__(256) void k_ew_orient(const uint64_t *off, const ui
         EntryWalk w(off, sadj, n, m2, g0 << 6);
         for (uint64_t g = g0; g < g1; ++g) {
             w.next();
-            const bool q = w.first && (DIR == 0 ? before(sdeg, w.u, w.nb) : before(sdeg, w.nb, w.u));
-            const uint64_t bal = __ballot(q);
-            if ((threadIdx.x & 63) == 0) mask[g] = bal;
+            const bool fwd = w.first && before(okey, w.u, w.nb);
+            const uint64_t bo = __ballot(fwd);
+            const uint64_t bi = IN ? __ballot(w.first && !fwd) : 0;   // no self-loops: first and not forward is backward
+            if ((threadIdx.x & 63) == 0) {
+                omask[g] = bo;
+                if (IN) imask[g] = bi;
+            }
         }
     }
 }
@@ -407,8 +421,6 @@ constexpr int kTriWaveBits = 11, kTriBlockBits = 14;
 constexpr uint64_t kTriLight = 1u << (kTriWaveBits - 2);   // out-degree limit of a wavefront item
 constexpr int kTriUnroll = 8, kTriUnrollHeavy = 8;   // loads per lane per step (two steps in flight)
 
-__device__ __forceinline__ uint32_t id_hash(uint32_t x, uint32_t bits) { return (x * 0x9E3779B1u) >> (32 - bits); }
-
 // work items of head v: ceil(din / kTriChunk) when v has out-neighbours in
 // the class (HEAVY: more than kTriLight of them)
 template <bool HEAVY>
@@ -441,12 +453,14 @@ __global__ void k_tri_items_fill(const uint64_t *ipre, uint32_t n, uint64_t *ite
 // GPU scheme); a key still homeless after kCuckooMoves moves fails the build.
 constexpr int kCuckooMoves = 256;
 
-__device__ __forceinline__ uint32_t cuckoo_h1(uint32_t x, uint32_t bits) { return (x * 0x9E3779B1u) >> (32 - bits); }
-
-// h2 != h1: an odd step from h1 in a table of a power-of-two size
-__device__ __forceinline__ uint32_t cuckoo_h2(uint32_t x, uint32_t bits)
+// Both slots from one multiplicative hash p = x * 0x9E3779B1 (bits <= 16): h1
+// is its top `bits` bits and h2 = h1 plus an odd step taken from the next
+// `bits` bits, so h2 != h1 in a table of a power-of-two size.
+__device__ __forceinline__ void cuckoo_slots(uint32_t x, uint32_t bits, uint32_t &h1, uint32_t &h2)
 {
-    return (cuckoo_h1(x, bits) + (((x * 0x85EBCA77u) >> (32 - bits)) | 1u)) & ((1u << bits) - 1);
+    const uint32_t p = x * 0x9E3779B1u, hm = (1u << bits) - 1;
+    h1 = p >> (32 - bits);
+    h2 = (h1 + (((p >> (32 - 2 * bits)) & hm) | 1u)) & hm;
 }
 
 // insert list[t], list[t + lanes], ...; false if one of them found no slot
@@ -455,13 +469,15 @@ __device__ __forceinline__ bool tri_insert(uint32_t *T, uint32_t bits, const uin
 {
     bool placed = true;
     for (uint64_t k = t; k < dv; k += lanes) {
-        uint32_t cur = list[k], pos = cuckoo_h1(cur, bits);
+        uint32_t cur = list[k], pos, other;
+        cuckoo_slots(cur, bits, pos, other);
         int moves = 0;
         for (; moves < kCuckooMoves; ++moves) {
             cur = atomicExch(&T[pos], cur);
             if (cur == kNone) break;
-            const uint32_t p1 = cuckoo_h1(cur, bits);
-            pos = pos == p1 ? cuckoo_h2(cur, bits) : p1;
+            uint32_t p1, p2;
+            cuckoo_slots(cur, bits, p1, p2);
+            pos = pos == p1 ? p2 : p1;
         }
         if (moves == kCuckooMoves) placed = false;
     }
@@ -497,6 +513,13 @@ struct TriCursor {
     __device__ __forceinline__ uint32_t fill(const uint32_t *onb, uint32_t (&x)[U])
     {
         const uint32_t lane = threadIdx.x & 63;
+        if (cs < 64 && cd - cp > 64u * U) {   // all U windows inside the current list, short of its end
+            const uint32_t *p = onb + cb + cp + lane;
+#pragma unroll
+            for (int k = 0; k < U; ++k) x[k] = p[64 * k];
+            cp += 64 * U;
+            return (1u << U) - 1;
+        }
         uint32_t okbits = 0;
 #pragma unroll
         for (int k = 0; k < U; ++k) {
@@ -549,8 +572,10 @@ __device__ __forceinline__ unsigned long long tri_probe(const uint32_t (&x)[U], 
 #pragma unroll
         for (int k = 0; k < U; ++k) {
             const bool v = (ok >> k) & 1;
-            y1[k] = T[v ? cuckoo_h1(x[k], bits) : 0];
-            y2[k] = T[v ? cuckoo_h2(x[k], bits) : 0];
+            uint32_t h1, h2;
+            cuckoo_slots(x[k], bits, h1, h2);
+            y1[k] = T[v ? h1 : 0];
+            y2[k] = T[v ? h2 : 0];
         }
 #pragma unroll
         for (int k = 0; k < U; ++k) c += (((ok >> k) & 1) && (y1[k] == x[k] || y2[k] == x[k])) ? 1 : 0;
@@ -1281,28 +1306,25 @@ inline int ew_grid(uint64_t groups)
 }
 
 // Flags over the 2m list entries, one bit each (mask[groups] = 0), and their
-// exclusive prefix per group (pre[groups] = the flagged total).  kind -1: first
-// of its run; 0: out-list entry (neighbour after the owner in the order);
-// 1: in-list entry (neighbour before the owner).
+// exclusive prefix per group (pre[groups] = the flagged total).
 struct EwMask {
     XB<uint64_t> mask, pre;
+    uint64_t total = 0;
 };
 
-int ew_mask(zg_graph *g, const uint32_t *sdeg, int kind, EwMask &e, uint64_t *total)
+int ew_alloc(zg_graph *g, EwMask &e)
 {
-    const uint64_t m2 = 2 * (uint64_t)g->m, groups = (m2 + 63) >> 6;
-    hipStream_t st = g->stream;
+    const uint64_t groups = (2 * (uint64_t)g->m + 63) >> 6;
     XALLOC(e.mask, groups + 1);
     XALLOC(e.pre, groups + 1);
-    ZG_HIP(hipMemsetAsync(e.mask.p + groups, 0, 8, st));
-    const int grid = ew_grid(groups);
-    if (kind < 0)
-        hipLaunchKernelGGL(k_ew_first, dim3(grid), dim3(256), 0, st, g->off, g->sadj, g->n, m2, e.mask.p);
-    else if (kind == 0)
-        hipLaunchKernelGGL(k_ew_orient<0>, dim3(grid), dim3(256), 0, st, g->off, g->sadj, g->n, m2, sdeg, e.mask.p);
-    else
-        hipLaunchKernelGGL(k_ew_orient<1>, dim3(grid), dim3(256), 0, st, g->off, g->sadj, g->n, m2, sdeg, e.mask.p);
-    ZG_HIP(hipGetLastError());
+    ZG_HIP(hipMemsetAsync(e.mask.p + groups, 0, 8, g->stream));
+    return ZG_OK;
+}
+
+int ew_scan(zg_graph *g, EwMask &e)
+{
+    const uint64_t groups = (2 * (uint64_t)g->m + 63) >> 6;
+    hipStream_t st = g->stream;
     const auto it = rocprim::make_transform_iterator(e.mask.p, Popc());
     size_t tb = 0;
     ZG_HIP(rocprim::exclusive_scan(nullptr, tb, it, e.pre.p, (uint64_t)0, (size_t)groups + 1,
@@ -1311,30 +1333,57 @@ int ew_mask(zg_graph *g, const uint32_t *sdeg, int kind, EwMask &e, uint64_t *to
     XALLOC(tmp, tb);
     ZG_HIP(rocprim::exclusive_scan(tmp.p, tb, it, e.pre.p, (uint64_t)0, (size_t)groups + 1,
                                    rocprim::plus<uint64_t>(), st));
-    ZG_HIP(hipMemcpyAsync(total, e.pre.p + groups, 8, hipMemcpyDeviceToHost, st));
+    ZG_HIP(hipMemcpyAsync(&e.total, e.pre.p + groups, 8, hipMemcpyDeviceToHost, st));
     ZG_HIP(hipStreamSynchronize(st));
     return ZG_OK;
 }
 
-// The lists of the flagged entries (kind as for ew_mask), neighbours ascending
-// by id: offsets (n + 1 entries), neighbours and, when own is given, owners.
-int ew_lists(zg_graph *g, const uint32_t *sdeg, int kind, XB<uint64_t> &offs, XB<uint32_t> &nbr,
-             XB<uint32_t> *own, uint64_t *total)
+// first-of-run flags (the full simple lists)
+int ew_first(zg_graph *g, EwMask &e)
 {
-    EwMask e;
-    int rc = ew_mask(g, sdeg, kind, e, total);
+    const uint64_t m2 = 2 * (uint64_t)g->m, groups = (m2 + 63) >> 6;
+    int rc = ew_alloc(g, e);
     if (rc) return rc;
+    hipLaunchKernelGGL(k_ew_first, dim3(ew_grid(groups)), dim3(256), 0, g->stream, g->off, g->sadj, g->n, m2,
+                       e.mask.p);
+    ZG_HIP(hipGetLastError());
+    return ew_scan(g, e);
+}
+
+// out-list flags and, when in is given, in-list flags, from one pass
+int ew_orient(zg_graph *g, const uint16_t *okey, EwMask &out, EwMask *in)
+{
+    const uint64_t m2 = 2 * (uint64_t)g->m, groups = (m2 + 63) >> 6;
+    int rc = ew_alloc(g, out);
+    if (!rc && in) rc = ew_alloc(g, *in);
+    if (rc) return rc;
+    if (in)
+        hipLaunchKernelGGL(k_ew_orient<true>, dim3(ew_grid(groups)), dim3(256), 0, g->stream, g->off, g->sadj, g->n,
+                           m2, okey, out.mask.p, in->mask.p);
+    else
+        hipLaunchKernelGGL(k_ew_orient<false>, dim3(ew_grid(groups)), dim3(256), 0, g->stream, g->off, g->sadj,
+                           g->n, m2, okey, out.mask.p, nullptr);
+    ZG_HIP(hipGetLastError());
+    rc = ew_scan(g, out);
+    if (!rc && in) rc = ew_scan(g, *in);
+    return rc;
+}
+
+// The lists of the flagged entries, neighbours ascending by id: offsets (n + 1
+// entries), neighbours and, when own is given, owners.
+int ew_lists(zg_graph *g, const EwMask &e, XB<uint64_t> &offs, XB<uint32_t> &nbr, XB<uint32_t> *own)
+{
     const uint64_t m2 = 2 * (uint64_t)g->m, groups = (m2 + 63) >> 6;
     hipStream_t st = g->stream;
     XALLOC(offs, (uint64_t)g->n + 1);
-    XALLOC(nbr, *total);
-    if (own) XALLOC(*own, *total);
+    XALLOC(nbr, e.total);
+    if (own) XALLOC(*own, e.total);
     hipLaunchKernelGGL(k_ew_offsets, dim3(gridx((uint64_t)g->n + 1)), dim3(256), 0, st, g->off, g->n, e.mask.p,
                        e.pre.p, offs.p);
     hipLaunchKernelGGL(k_ew_compact, dim3(ew_grid(groups)), dim3(256), 0, st, g->off, g->sadj, g->n, m2, e.mask.p,
                        e.pre.p, nbr.p, own ? own->p : nullptr);
     ZG_HIP(hipGetLastError());
-    ZG_HIP(hipStreamSynchronize(st));   // before the mask is freed
+    ZG_HIP(hipStreamSynchronize(st));   // before a mask is freed
     return ZG_OK;
 }
 
@@ -1366,12 +1415,16 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
     uint64_t nf = 2 * (uint64_t)g->m;
     XALLOC(sdeg, n);
     if (what >= 5) {
-        const int rc = ew_lists(g, nullptr, -1, foff, fnb, &fsrc, &nf);
+        EwMask fm;
+        int rc = ew_first(g, fm);
+        if (!rc) rc = ew_lists(g, fm, foff, fnb, &fsrc);
         if (rc) return rc;
+        nf = fm.total;
     } else if (!g->simple) {
         EwMask fm;
-        const int rc = ew_mask(g, nullptr, -1, fm, &nf);
+        const int rc = ew_first(g, fm);
         if (rc) return rc;
+        nf = fm.total;
         XALLOC(foff, (uint64_t)n + 1);
         hipLaunchKernelGGL(k_ew_offsets, dim3(gridx((uint64_t)n + 1)), dim3(256), 0, st, g->off, n, fm.mask.p,
                            fm.pre.p, foff.p);
@@ -1430,9 +1483,16 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
     XB<uint64_t> ooff;
     XB<uint32_t> onb, osrc;
     uint64_t ne = 0, maxout = 0;
+    EwMask imask;   // in-list flags (triangles)
     if (what != 7) {
-        int rc = ew_lists(g, sdeg.p, 0, ooff, onb, what == 0 ? nullptr : &osrc, &ne);
+        XB<uint16_t> okey;
+        XALLOC(okey, n);
+        hipLaunchKernelGGL(k_order_key, dim3(gridx(n)), dim3(256), 0, st, sdeg.p, n, okey.p);
+        EwMask omask;
+        int rc = ew_orient(g, okey.p, omask, what == 0 ? &imask : nullptr);
+        if (!rc) rc = ew_lists(g, omask, ooff, onb, what == 0 ? nullptr : &osrc);
         if (rc) return rc;
+        ne = omask.total;
         XB<unsigned long long> mx;
         XALLOC(mx, 1);
         ZG_HIP(hipMemsetAsync(mx.p, 0, 8, st));
@@ -1446,9 +1506,11 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         // kTriChunk in-list entries of one head
         XB<uint64_t> iin, icnt, ipre, items;
         XB<uint32_t> inb;
-        uint64_t nin = 0, nitems = 0;
-        int rc = ew_lists(g, sdeg.p, 1, iin, inb, nullptr, &nin);
+        uint64_t nitems = 0;
+        int rc = ew_lists(g, imask, iin, inb, nullptr);
         if (rc) return rc;
+        imask.mask.reset();
+        imask.pre.reset();
         XALLOC(icnt, (uint64_t)n + 1);
         XALLOC(ipre, (uint64_t)n + 1);
         unsigned long long total = 0;
