@@ -123,7 +123,7 @@ struct EntryWalk {
     uint32_t n, v;       // v: a vertex with off[v] <= base
     uint32_t cu, cnb;    // owner and neighbour of entry base - 1 (kNone at a list start)
     // after next(), for entry base + lane of the group just walked:
-    uint32_t u, nb;      // owner and neighbour
+    uint32_t u, nb, rk;  // owner, neighbour and edge rank
     bool valid, first;   // entry exists; first of its run of equal neighbours in its list
 
     __device__ EntryWalk(const uint64_t *off_, const uint64_t *sadj_, uint32_t n_, uint64_t m2_, uint64_t b)
@@ -140,7 +140,9 @@ struct EntryWalk {
         const uint32_t lane = threadIdx.x & 63;
         const uint64_t i = base + lane;
         valid = i < m2;
-        nb = valid ? (uint32_t)(sadj[i] >> 32) : kNone;
+        const uint64_t ent = valid ? sadj[i] : ~0ull;
+        nb = (uint32_t)(ent >> 32);
+        rk = (uint32_t)ent;
         // owner(i) = v + #{k >= 1 : off[v + k] <= i}, 64 offsets a round: r is where
         // vertex w starts relative to the group (0 before it, 64 past it), ascending
         // over the lanes, and a binary search over the lanes counts the r <= lane
@@ -216,10 +218,15 @@ __global__ __launch_bounds__(256) void k_ew_orient(const uint64_t *off, const ui
 
 // The entries flagged in mask, in entry order (lists stay in neighbour order):
 // the k-th flagged entry's neighbour to nbr[k] and its owner to own[k] (own may
-// be null); pre[g] = flagged entries before group g.
+// be null); pre[g] = flagged entries before group g.  With pos_of_rank, also
+// pos_of_rank[edge rank] = k; with edge_of_rank, nbr[k] = edge_of_rank[edge
+// rank] instead of the neighbour (an in-list entry becomes the index of its
+// twin in the out-lists: both copies of an edge carry its rank, and a run of
+// parallel copies keeps its lowest rank first in both lists).
 __global__ __launch_bounds__(256) void k_ew_compact(const uint64_t *off, const uint64_t *sadj, uint32_t n, uint64_t m2,
                                                     const uint64_t *mask, const uint64_t *pre, uint32_t *nbr,
-                                                    uint32_t *own)
+                                                    uint32_t *own, uint32_t *pos_of_rank,
+                                                    const uint32_t *edge_of_rank)
 {
     const uint32_t lane = threadIdx.x & 63;
     EW_TILES(m2)
@@ -231,8 +238,9 @@ __global__ __launch_bounds__(256) void k_ew_compact(const uint64_t *off, const u
             const uint64_t gu = uni64(g), mk = mask[gu];
             if ((mk >> lane) & 1) {
                 const uint64_t at = pre[gu] + (uint64_t)__popcll(mk & ((1ull << lane) - 1));
-                nbr[at] = w.nb;
+                nbr[at] = edge_of_rank ? edge_of_rank[w.rk] : w.nb;
                 if (own) own[at] = w.u;
+                if (pos_of_rank) pos_of_rank[w.rk] = (uint32_t)at;
             }
         }
     }
@@ -419,7 +427,7 @@ constexpr uint32_t kTriChunkHeavy = 8192;       // in-list entries per block wor
 constexpr uint32_t kTriHeavyThreads = 1024;
 constexpr int kTriWaveBits = 11, kTriBlockBits = 14;
 constexpr uint64_t kTriLight = 1u << (kTriWaveBits - 2);   // out-degree limit of a wavefront item
-constexpr int kTriUnroll = 8, kTriUnrollHeavy = 8;   // loads per lane per step (two steps in flight)
+constexpr int kTriUnroll = 8, kTriUnrollHeavy = 8, kK4Unroll = 4;   // loads per lane per step (two steps in flight)
 
 // work items of head v: ceil(din / kTriChunk) when v has out-neighbours in
 // the class (HEAVY: more than kTriLight of them)
@@ -558,13 +566,13 @@ struct TriCursor {
     }
 };
 
-// entries x[k] with bit k of ok set found in the table T (2^bits slots) or,
+// bit k: x[k] (with bit k of ok set) is in the table T (2^bits slots) or,
 // when T is null, in the sorted list vl of dv entries
 template <int U>
-__device__ __forceinline__ unsigned long long tri_probe(const uint32_t (&x)[U], uint32_t ok, const uint32_t *T,
-                                                        uint32_t bits, const uint32_t *vl, uint64_t dv)
+__device__ __forceinline__ uint32_t tri_found(const uint32_t (&x)[U], uint32_t ok, const uint32_t *T, uint32_t bits,
+                                              const uint32_t *vl, uint64_t dv)
 {
-    unsigned long long c = 0;
+    uint32_t f = 0;
     if (T) {
         // two reads per entry; a lane with nothing to look up reads slot 0,
         // which all such lanes share (a broadcast, no bank conflict)
@@ -578,13 +586,13 @@ __device__ __forceinline__ unsigned long long tri_probe(const uint32_t (&x)[U], 
             y2[k] = T[v ? h2 : 0];
         }
 #pragma unroll
-        for (int k = 0; k < U; ++k) c += (((ok >> k) & 1) && (y1[k] == x[k] || y2[k] == x[k])) ? 1 : 0;
+        for (int k = 0; k < U; ++k) f |= ((((ok >> k) & 1) && (y1[k] == x[k] || y2[k] == x[k])) ? 1u : 0u) << k;
     } else {
 #pragma unroll
         for (int k = 0; k < U; ++k)
-            if ((ok >> k) & 1) c += bsearch_u32(vl, dv, x[k]) ? 1 : 0;
+            if (((ok >> k) & 1) && bsearch_u32(vl, dv, x[k])) f |= 1u << k;
     }
-    return c;
+    return f;
 }
 
 // Tails sb + lane (< ie) of the item: entries of their out-lists found in the
@@ -612,62 +620,205 @@ __device__ __forceinline__ unsigned long long tri_stream(const uint64_t *ooff, c
     for (;;) {   // two buffers in turn, no copies (a copy would wait for the loads)
         bool more = cur.more();
         okb = cur.fill<U>(onb, xb);
-        c += tri_probe<U>(xa, oka, T, bits, vl, dv);
+        c += __popc(tri_found<U>(xa, oka, T, bits, vl, dv));
         if (!more) break;
         more = cur.more();
         oka = cur.fill<U>(onb, xa);
-        c += tri_probe<U>(xb, okb, T, bits, vl, dv);
+        c += __popc(tri_found<U>(xb, okb, T, bits, vl, dv));
         if (!more) break;
     }
     return c;
 }
 
+// ---- 4-clique rows from the same stream ----------------------------------------
+// Row i of vertex u (the row of oriented edge u -> v, v = out(u)[i]) is the set
+// of positions p of out(u) with out(u)[p] in out(v): the entries of out(u)
+// found in v's table.  So the rows come from the triangle stream: head v's
+// table, tails u from v's in-list (each entry the index e of edge u -> v in
+// the out-lists, so i = e - ooff[u]), and the hits of every 64-entry word of
+// out(u) stored as one row word.  Windows hold whole words of the lists (a
+// word that does not fit the rest of a window opens the next one), so each
+// row word is stored once, by the lane at the word's first position.
+struct RowCursor {
+    uint64_t nz, b, cb;   // as TriCursor
+    uint32_t d, rb, cs, cp, cd, crb;   // rb: lane s's row (its first word in the row buffer)
+
+    __device__ RowCursor(uint64_t b_, uint32_t d_, uint32_t rb_) : b(b_), d(d_), rb(rb_)
+    {
+        nz = __ballot(d > 0);
+        cs = nz ? (uint32_t)__builtin_ctzll(nz) : 64;
+        cp = 0;
+        cd = cs < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)d, (int)cs) : 0;
+        cb = cs < 64 ? readlane64(b, cs) : 0;
+        crb = cs < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)rb, (int)cs) : 0;
+    }
+    __device__ bool more() const { return cs < 64; }
+
+    // window k: x[k] loaded, wa[k] the row word of the lane's position; bit k
+    // of ok: the lane has a position, of st: it is a word's first position
+    template <int U>
+    __device__ __forceinline__ void fill(const uint32_t *onb, uint32_t (&x)[U], uint32_t (&wa)[U], uint32_t &ok,
+                                         uint32_t &st)
+    {
+        const uint32_t lane = threadIdx.x & 63;
+        ok = 0;
+        st = 0;
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            uint64_t at = 0;
+            uint32_t w = 0;
+            bool has = false, first = false;
+            uint32_t filled = 0;
+            while (cs < 64) {
+                const uint32_t take = cd - cp < 64 ? cd - cp : 64;   // the rest of one word (cp is a multiple of 64)
+                if (take > 64 - filled) break;
+                if (lane >= filled && lane < filled + take) {
+                    at = cb + cp + (lane - filled);
+                    w = crb + (cp >> 6);
+                    has = true;
+                    first = lane == filled;
+                }
+                filled += take;
+                cp += take;
+                if (cp == cd) {
+                    const uint64_t rest = cs < 63 ? nz >> (cs + 1) : 0;
+                    cs = rest ? cs + 1 + (uint32_t)__builtin_ctzll(rest) : 64;
+                    cp = 0;
+                    if (cs < 64) {
+                        cd = (uint32_t)__builtin_amdgcn_readlane((int)d, (int)cs);
+                        cb = readlane64(b, cs);
+                        crb = (uint32_t)__builtin_amdgcn_readlane((int)rb, (int)cs);
+                    }
+                }
+                if (filled == 64) break;
+            }
+            x[k] = onb[has ? at : 0];
+            wa[k] = w;
+            ok |= (has ? 1u : 0u) << k;
+            st |= (first ? 1u : 0u) << k;
+        }
+    }
+};
+
+// store each word's hits: the word's first lane takes the hit bits from its
+// own lane up to the next word's first lane
+template <int U>
+__device__ __forceinline__ void k4_store(uint64_t *rows, const uint32_t (&wa)[U], uint32_t found, uint32_t st)
+{
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        const uint64_t hit = __ballot((found >> k) & 1), starts = __ballot((st >> k) & 1);
+        if ((st >> k) & 1) {
+            const uint64_t above = lane == 63 ? 0 : starts & (~0ull << (lane + 1));
+            const uint32_t len = (above ? (uint32_t)__builtin_ctzll(above) : 64u) - lane;
+            const uint64_t word = (hit >> lane) & (len == 64 ? ~0ull : ((1ull << len) - 1));
+            if (word) rows[wa[k]] = word;
+        }
+    }
+}
+
+// Arguments of the head-table kernels: the triangle count (MODE 0) or the
+// 4-clique rows of tails in [u0, u1) (MODE 1, in-lists holding edge indices).
+struct HeadArgs {
+    const uint64_t *ooff;
+    const uint32_t *onb;
+    const uint64_t *iin;
+    const uint32_t *inb;
+    const uint64_t *items;
+    uint64_t nitems;
+    const uint32_t *osrc;
+    const uint64_t *boff;
+    uint64_t bbase;
+    uint32_t u0, u1;
+    uint64_t *rows;
+};
+
+template <int U>
+__device__ __forceinline__ void k4_stream(const HeadArgs &A, uint64_t sb, uint64_t ie, const uint32_t *T,
+                                          uint32_t bits, const uint32_t *vl, uint64_t dv)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t si = sb + lane;
+    uint64_t b = 0;
+    uint32_t d = 0, rb = 0;
+    if (si < ie) {
+        const uint32_t e = A.inb[si], u = A.osrc[e];
+        if (u >= A.u0 && u < A.u1) {
+            const uint64_t lb = A.ooff[u], D = A.ooff[u + 1] - lb;
+            if (D >= 3) {
+                b = lb;
+                d = (uint32_t)D;
+                rb = (uint32_t)(A.boff[u] - A.bbase + (e - lb) * ((D + 63) >> 6));
+            }
+        }
+    }
+    RowCursor cur(b, d, rb);
+    if (!cur.more()) return;
+    uint32_t xa[U], xb[U], wa[U], wb[U], oka, sta, okb, stb;
+    cur.fill<U>(A.onb, xa, wa, oka, sta);
+    for (;;) {   // two buffers in turn, as in tri_stream
+        bool more = cur.more();
+        cur.fill<U>(A.onb, xb, wb, okb, stb);
+        k4_store<U>(A.rows, wa, tri_found<U>(xa, oka, T, bits, vl, dv), sta);
+        if (!more) break;
+        more = cur.more();
+        cur.fill<U>(A.onb, xa, wa, oka, sta);
+        k4_store<U>(A.rows, wb, tri_found<U>(xb, okb, T, bits, vl, dv), stb);
+        if (!more) break;
+    }
+}
+
 // one wavefront per work item (heads with up to kTriLight out-neighbours)
-__global__ __launch_bounds__(256) void k_tri_light(const uint64_t *ooff, const uint32_t *onb, const uint64_t *iin,
-                                                   const uint32_t *inb, const uint64_t *items, uint64_t nitems,
-                                                   unsigned long long *partial)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_head_light(HeadArgs A, unsigned long long *partial)
 {
     __shared__ uint32_t tab[4][1u << kTriWaveBits];
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t *const T = tab[wv];
     unsigned long long c = 0;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; it < nitems; it += nwaves) {
-        const uint64_t item = uni64(items[uni64(it)]);
+    for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; it < A.nitems; it += nwaves) {
+        const uint64_t item = uni64(A.items[uni64(it)]);
         const uint32_t v = (uint32_t)(item >> 32);
-        const uint64_t vb = ooff[v], dv = ooff[v + 1] - vb;
-        const uint64_t ib = iin[v] + (item & 0xffffffffull) * kTriChunk, iend = iin[v + 1];
+        const uint64_t vb = A.ooff[v], dv = A.ooff[v + 1] - vb;
+        const uint64_t ib = A.iin[v] + (item & 0xffffffffull) * kTriChunk, iend = A.iin[v + 1];
         const uint64_t ie = ib + kTriChunk < iend ? ib + kTriChunk : iend;
         const uint32_t bits = kTriWaveBits;
         for (uint32_t t = lane; t < (1u << bits); t += 64) T[t] = kNone;
         wave_sync();
-        const bool built = __ballot(!tri_insert(T, bits, onb + vb, dv, lane, 64)) == 0;
+        const bool built = __ballot(!tri_insert(T, bits, A.onb + vb, dv, lane, 64)) == 0;
         wave_sync();
-        for (uint64_t sb = ib; sb < ie; sb += 64)
-            c += tri_stream<kTriUnroll>(ooff, onb, inb, sb, ie, built ? T : nullptr, bits, onb + vb, dv);
+        for (uint64_t sb = ib; sb < ie; sb += 64) {
+            if (MODE == 0)
+                c += tri_stream<kTriUnroll>(A.ooff, A.onb, A.inb, sb, ie, built ? T : nullptr, bits, A.onb + vb, dv);
+            else
+                k4_stream<kK4Unroll>(A, sb, ie, built ? T : nullptr, bits, A.onb + vb, dv);
+        }
         wave_sync();   // before the table is cleared
     }
-    const unsigned long long t = block_sum(c);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    if (MODE == 0) {
+        const unsigned long long t = block_sum(c);
+        if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    }
 }
 
 // One block of 16 wavefronts per work item of a head with more than kTriLight
 // out-neighbours, sharing one table; the wavefronts claim the item's batches
 // of 64 tails from an LDS counter.
-__global__ __launch_bounds__(kTriHeavyThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tri_heavy(const uint64_t *ooff, const uint32_t *onb,
-                                                                const uint64_t *iin, const uint32_t *inb,
-                                                                const uint64_t *items, uint64_t nitems,
-                                                                unsigned long long *partial)
+template <int MODE>
+__global__ __launch_bounds__(kTriHeavyThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_head_heavy(
+    HeadArgs A, unsigned long long *partial)
 {
     __shared__ uint32_t T[1u << kTriBlockBits];
     __shared__ uint32_t next_batch, failed;
     const uint32_t lane = threadIdx.x & 63;
     unsigned long long c = 0;
-    for (uint64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const uint64_t item = items[it];
+    for (uint64_t it = blockIdx.x; it < A.nitems; it += gridDim.x) {
+        const uint64_t item = A.items[it];
         const uint32_t v = (uint32_t)(item >> 32);
-        const uint64_t vb = ooff[v], dv = ooff[v + 1] - vb;
-        const uint64_t ib = iin[v] + (item & 0xffffffffull) * kTriChunkHeavy, iend = iin[v + 1];
+        const uint64_t vb = A.ooff[v], dv = A.ooff[v + 1] - vb;
+        const uint64_t ib = A.iin[v] + (item & 0xffffffffull) * kTriChunkHeavy, iend = A.iin[v + 1];
         const uint64_t ie = ib + kTriChunkHeavy < iend ? ib + kTriChunkHeavy : iend;
         const bool fits = 4 * dv <= (1u << kTriBlockBits);
         const uint32_t bits = kTriBlockBits;
@@ -675,7 +826,7 @@ __global__ __launch_bounds__(kTriHeavyThreads) __attribute__((amdgpu_waves_per_e
         if (fits)
             for (uint32_t t = threadIdx.x; t < (1u << bits); t += blockDim.x) T[t] = kNone;
         __syncthreads();
-        if (fits && !tri_insert(T, bits, onb + vb, dv, threadIdx.x, blockDim.x)) failed = 1;
+        if (fits && !tri_insert(T, bits, A.onb + vb, dv, threadIdx.x, blockDim.x)) failed = 1;
         __syncthreads();
         const bool hashed = fits && !failed;
         const uint32_t batches = (uint32_t)((ie - ib + 63) >> 6);
@@ -684,12 +835,18 @@ __global__ __launch_bounds__(kTriHeavyThreads) __attribute__((amdgpu_waves_per_e
             if (lane == 0) bi = atomicAdd(&next_batch, 1u);
             bi = __builtin_amdgcn_readfirstlane(bi);
             if (bi >= batches) break;
-            c += tri_stream<kTriUnrollHeavy>(ooff, onb, inb, ib + 64ull * bi, ie, hashed ? T : nullptr, bits, onb + vb, dv);
+            if (MODE == 0)
+                c += tri_stream<kTriUnrollHeavy>(A.ooff, A.onb, A.inb, ib + 64ull * bi, ie, hashed ? T : nullptr,
+                                                 bits, A.onb + vb, dv);
+            else
+                k4_stream<kK4Unroll>(A, ib + 64ull * bi, ie, hashed ? T : nullptr, bits, A.onb + vb, dv);
         }
         __syncthreads();   // before the table and the counter are reset
     }
-    const unsigned long long t = block_sum(c);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    if (MODE == 0) {
+        const unsigned long long t = block_sum(c);
+        if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    }
 }
 
 // 4-cliques by bit rows: the binary-encoded induced subgraphs of Almasri et
@@ -699,21 +856,9 @@ __global__ __launch_bounds__(kTriHeavyThreads) __attribute__((amdgpu_waves_per_e
 // W = out(u) & out(L[i]) over positions of L.  A 4-clique whose lowest vertex
 // is u, next L[i], then L[j], then L[p] is bit p of row_i & row_j, so the
 // count at u is the sum over rows i and set bits j of popcount(row_i & row_j).
-// Vertex u's rows take d * ceil(d / 64) words from boff[u]; a vertex whose
-// out-list is longer than sqrt(2m) cannot exist, so one vertex needs at most
-// 2^16 rows of 2^10 words.
-constexpr int kK4RowLds = 256;   // words of one row staged in LDS (out-lists up to 16384)
-
-__device__ __forceinline__ uint64_t lower_u32(const uint32_t *a, uint64_t n, uint32_t x)
-{
-    uint64_t lo = 0, hi = n;
-    while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (a[mid] < x) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
+// Vertex u's rows take d * ceil(d / 64) words from boff[u]; out-lists stay
+// within sqrt(2m), so one vertex needs at most 2^16 rows of 2^10 words.  The
+// rows themselves come from the head-table stream (k4_stream).
 // words of each vertex's rows (0 below three out-neighbours)
 __global__ void k_k4_words(const uint64_t *ooff, uint32_t n, uint64_t *words)
 {
@@ -721,73 +866,6 @@ __global__ void k_k4_words(const uint64_t *ooff, uint32_t n, uint64_t *words)
     {
         const uint64_t d = u < n ? ooff[u + 1] - ooff[u] : 0;
         words[u] = d >= 3 ? d * ((d + 63) >> 6) : 0;
-    }
-}
-
-// One wavefront per oriented edge of [e0, e1) writes its row, whichever way
-// probes less: ballots over 64 positions of L at a time, each lane searching
-// its entry of L in out(v); or, for a short out(v), each lane searching one
-// entry of out(v) in L and setting its bit in an LDS copy of the row.
-__global__ __launch_bounds__(256) void k_k4_rows(const uint64_t *ooff, const uint32_t *onb, const uint32_t *osrc,
-                                                 const uint64_t *boff, uint64_t bbase, uint64_t e0, uint64_t e1,
-                                                 uint64_t *bits)
-{
-    __shared__ unsigned long long srows[4 * kK4RowLds];
-    const uint32_t lane = threadIdx.x & 63;
-    unsigned long long *const srow = srows + (threadIdx.x >> 6) * kK4RowLds;
-    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    for (uint64_t e = e0 + wave; e < e1; e += nwaves) {
-        const uint32_t u = osrc[e], v = onb[e];
-        const uint64_t lb = ooff[u], d = ooff[u + 1] - lb;
-        if (d < 3) continue;
-        const uint64_t R = (d + 63) >> 6;
-        uint64_t *const row = bits + (boff[u] - bbase) + (e - lb) * R;
-        const uint32_t *const L = onb + lb;
-        const uint64_t vb = ooff[v], dv = ooff[v + 1] - vb;
-        const uint32_t *const ov = onb + vb;
-        const bool staged = R <= (uint64_t)kK4RowLds && ((dv + 63) >> 6) * clog2_u64(d + 1) < R * clog2_u64(dv + 1);
-        if (!staged) {
-            // two positions of L per lane and pass, their searches interleaved
-            for (uint64_t kb = 0; kb < R; kb += 2) {
-                const uint64_t j0 = kb * 64 + lane, j1 = j0 + 64;
-                const uint32_t x0 = j0 < d ? L[j0] : 0, x1 = j1 < d ? L[j1] : 0;
-                uint64_t lo0 = 0, hi0 = j0 < d ? dv : 0, lo1 = 0, hi1 = j1 < d ? dv : 0;
-                while (lo0 < hi0 || lo1 < hi1) {
-                    const uint64_t m0 = (lo0 + hi0) >> 1, m1 = (lo1 + hi1) >> 1;
-                    const uint32_t y0 = lo0 < hi0 ? ov[m0] : 0, y1 = lo1 < hi1 ? ov[m1] : 0;
-                    if (lo0 < hi0) { if (y0 < x0) lo0 = m0 + 1; else hi0 = m0; }
-                    if (lo1 < hi1) { if (y1 < x1) lo1 = m1 + 1; else hi1 = m1; }
-                }
-                const bool h0 = j0 < d && lo0 < dv && ov[lo0] == x0;
-                const bool h1 = j1 < d && lo1 < dv && ov[lo1] == x1;
-                const uint64_t w0 = __ballot(h0), w1 = __ballot(h1);
-                if (lane == 0) {
-                    row[kb] = w0;
-                    if (kb + 1 < R) row[kb + 1] = w1;
-                }
-            }
-            continue;
-        }
-        for (uint64_t t = lane; t < R; t += 64) srow[t] = 0;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        for (uint64_t xb = 0; xb < dv; xb += 64) {
-            const uint64_t t = xb + lane;
-            if (t < dv) {
-                const uint32_t x = ov[t];
-                const uint64_t j = lower_u32(L, d, x);
-                if (j < d && L[j] == x) atomicOr(&srow[j >> 6], 1ull << (j & 63));
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        for (uint64_t t = lane; t < R; t += 64) row[t] = srow[t];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();   // the LDS row is rewritten by the next edge
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
 }
 
@@ -1370,8 +1448,10 @@ int ew_orient(zg_graph *g, const uint16_t *okey, EwMask &out, EwMask *in)
 }
 
 // The lists of the flagged entries, neighbours ascending by id: offsets (n + 1
-// entries), neighbours and, when own is given, owners.
-int ew_lists(zg_graph *g, const EwMask &e, XB<uint64_t> &offs, XB<uint32_t> &nbr, XB<uint32_t> *own)
+// entries), neighbours and, when own is given, owners (pos_of_rank and
+// edge_of_rank as for k_ew_compact).
+int ew_lists(zg_graph *g, const EwMask &e, XB<uint64_t> &offs, XB<uint32_t> &nbr, XB<uint32_t> *own,
+             uint32_t *pos_of_rank = nullptr, const uint32_t *edge_of_rank = nullptr)
 {
     const uint64_t m2 = 2 * (uint64_t)g->m, groups = (m2 + 63) >> 6;
     hipStream_t st = g->stream;
@@ -1381,7 +1461,7 @@ int ew_lists(zg_graph *g, const EwMask &e, XB<uint64_t> &offs, XB<uint32_t> &nbr
     hipLaunchKernelGGL(k_ew_offsets, dim3(gridx((uint64_t)g->n + 1)), dim3(256), 0, st, g->off, g->n, e.mask.p,
                        e.pre.p, offs.p);
     hipLaunchKernelGGL(k_ew_compact, dim3(ew_grid(groups)), dim3(256), 0, st, g->off, g->sadj, g->n, m2, e.mask.p,
-                       e.pre.p, nbr.p, own ? own->p : nullptr);
+                       e.pre.p, nbr.p, own ? own->p : nullptr, pos_of_rank, edge_of_rank);
     ZG_HIP(hipGetLastError());
     ZG_HIP(hipStreamSynchronize(st));   // before a mask is freed
     return ZG_OK;
@@ -1479,20 +1559,28 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         *out = p - 3 * T;
         return ZG_OK;
     }
-    // oriented out-lists (owners for the 4-clique rows and the weighted triangles)
-    XB<uint64_t> ooff;
-    XB<uint32_t> onb, osrc;
+    // oriented out-lists (owners for the 4-clique rows and the weighted
+    // triangles), and for triangles and 4-cliques the in-lists: for every head
+    // v the tails u -> v (for 4-cliques the index of edge u -> v in the
+    // out-lists instead of u)
+    XB<uint64_t> ooff, iin;
+    XB<uint32_t> onb, osrc, inb;
     uint64_t ne = 0, maxout = 0;
-    EwMask imask;   // in-list flags (triangles)
     if (what != 7) {
         XB<uint16_t> okey;
         XALLOC(okey, n);
         hipLaunchKernelGGL(k_order_key, dim3(gridx(n)), dim3(256), 0, st, sdeg.p, n, okey.p);
-        EwMask omask;
-        int rc = ew_orient(g, okey.p, omask, what == 0 ? &imask : nullptr);
-        if (!rc) rc = ew_lists(g, omask, ooff, onb, what == 0 ? nullptr : &osrc);
+        EwMask omask, imask;
+        int rc = ew_orient(g, okey.p, omask, what <= 1 ? &imask : nullptr);
         if (rc) return rc;
+        okey.reset();
         ne = omask.total;
+        if (what == 1 && ne >= (1ull << 32)) return set_error(ZG_EINVAL, "4-clique counts need m < 2^32");
+        XB<uint32_t> pos_of_rank;   // out-list index of each edge rank (4-cliques)
+        if (what == 1) XALLOC(pos_of_rank, g->m);
+        rc = ew_lists(g, omask, ooff, onb, what == 0 ? nullptr : &osrc, pos_of_rank.p);
+        if (!rc && what <= 1) rc = ew_lists(g, imask, iin, inb, nullptr, nullptr, pos_of_rank.p);
+        if (rc) return rc;
         XB<unsigned long long> mx;
         XALLOC(mx, 1);
         ZG_HIP(hipMemsetAsync(mx.p, 0, 8, st));
@@ -1501,19 +1589,14 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         rc = read_u64(mx.p, &maxout);
         if (rc) return rc;
     }
-    if (what == 0) {
-        // in-lists: for every head v the tails u -> v, then work items of up to
-        // kTriChunk in-list entries of one head
-        XB<uint64_t> iin, icnt, ipre, items;
-        XB<uint32_t> inb;
-        uint64_t nitems = 0;
-        int rc = ew_lists(g, imask, iin, inb, nullptr);
-        if (rc) return rc;
-        imask.mask.reset();
-        imask.pre.reset();
+    // work items of the head-table kernels: up to kTriChunk (light) or
+    // kTriChunkHeavy (heavy) in-list entries of one head with out-neighbours
+    XB<uint64_t> items[2];
+    uint64_t nitems[2] = {0, 0};
+    if (what <= 1) {
+        XB<uint64_t> icnt, ipre;
         XALLOC(icnt, (uint64_t)n + 1);
         XALLOC(ipre, (uint64_t)n + 1);
-        unsigned long long total = 0;
         for (int heavy = 0; heavy < 2; ++heavy) {
             if (heavy)
                 hipLaunchKernelGGL(k_tri_items_count<true>, dim3(gridx((uint64_t)n + 1)), dim3(256), 0, st, ooff.p,
@@ -1528,25 +1611,43 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
             XALLOC(tmp, tb);
             ZG_HIP(rocprim::exclusive_scan(tmp.p, tb, icnt.p, ipre.p, (uint64_t)0, (size_t)n + 1,
                                            rocprim::plus<uint64_t>(), st));
-            rc = read_u64(ipre.p + n, &nitems);
+            int rc = read_u64(ipre.p + n, &nitems[heavy]);
             if (rc) return rc;
-            if (nitems == 0) continue;
-            items.reset();
-            XALLOC(items, nitems);
-            hipLaunchKernelGGL(k_tri_items_fill, dim3(gridx(n)), dim3(256), 0, st, ipre.p, n, items.p);
-            const uint64_t per_block = heavy ? 1 : 4, cap = (uint64_t)device_cus() * (heavy ? 2 : 8);
-            const int tgrid = (int)std::min<uint64_t>(cap, std::max<uint64_t>(1, (nitems + per_block - 1) / per_block));
+            XALLOC(items[heavy], nitems[heavy]);
+            if (nitems[heavy])
+                hipLaunchKernelGGL(k_tri_items_fill, dim3(gridx(n)), dim3(256), 0, st, ipre.p, n, items[heavy].p);
+            ZG_HIP(hipGetLastError());
+            ZG_HIP(hipStreamSynchronize(st));   // before the temporaries are freed
+        }
+    }
+    const int hgrid[2] = {
+        (int)std::min<uint64_t>((uint64_t)device_cus() * 8, std::max<uint64_t>(1, (nitems[0] + 3) / 4)),
+        (int)std::min<uint64_t>((uint64_t)device_cus() * 2, std::max<uint64_t>(1, nitems[1]))};
+    auto head_args = [&](int heavy) {
+        HeadArgs A{};
+        A.ooff = ooff.p;
+        A.onb = onb.p;
+        A.iin = iin.p;
+        A.inb = inb.p;
+        A.items = items[heavy].p;
+        A.nitems = nitems[heavy];
+        A.osrc = osrc.p;
+        return A;
+    };
+    if (what == 0) {
+        unsigned long long total = 0;
+        for (int heavy = 0; heavy < 2; ++heavy) {
+            if (nitems[heavy] == 0) continue;
             XB<unsigned long long> tpart;
-            XALLOC(tpart, tgrid);
+            XALLOC(tpart, hgrid[heavy]);
             if (heavy)
-                hipLaunchKernelGGL(k_tri_heavy, dim3(tgrid), dim3(kTriHeavyThreads), 0, st, ooff.p, onb.p, iin.p,
-                                   inb.p, items.p, nitems, tpart.p);
+                hipLaunchKernelGGL(k_head_heavy<0>, dim3(hgrid[1]), dim3(kTriHeavyThreads), 0, st, head_args(1),
+                                   tpart.p);
             else
-                hipLaunchKernelGGL(k_tri_light, dim3(tgrid), dim3(256), 0, st, ooff.p, onb.p, iin.p, inb.p, items.p,
-                                   nitems, tpart.p);
+                hipLaunchKernelGGL(k_head_light<0>, dim3(hgrid[0]), dim3(256), 0, st, head_args(0), tpart.p);
             ZG_HIP(hipGetLastError());
             unsigned long long s = 0;
-            rc = host_sum(tpart.p, tgrid, st, &s);
+            const int rc = host_sum(tpart.p, hgrid[heavy], st, &s);
             if (rc) return rc;
             total += s;
         }
@@ -1800,12 +1901,13 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
     size_t free_b = 0, total_b = 0;
     ZG_HIP(hipMemGetInfo(&free_b, &total_b));
     const uint64_t one_vertex = maxout * ((maxout + 63) >> 6);
-    uint64_t budget = std::min<uint64_t>(16ull << 30, std::max<uint64_t>(1ull << 30, free_b / 4)) / 8;
+    // row words per pass: a third of the free memory, at most 2^32 (32 GiB, the
+    // row-word offsets are 32-bit)
+    uint64_t budget = std::min<uint64_t>(1ull << 32, std::max<uint64_t>(1ull << 27, free_b / 24));
     if (const char *ev = getenv("ZGRAPH_K4_SCRATCH_WORDS")) budget = strtoull(ev, nullptr, 10);   // smaller passes
     budget = std::max(budget, one_vertex);
     XB<uint64_t> bits;
     XALLOC(bits, std::min(total_words, budget));
-    const int rgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16, std::max<uint64_t>(1, (ne + 3) / 4));
     const int cgrid = gridx(ne);
     const int wgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16, std::max<uint64_t>(1, (ne + 255) / 256));
     XB<unsigned long long> part, wpart;
@@ -1832,9 +1934,24 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         if (!rc) rc = read_u64(ooff.p + u1, &e1);
         if (rc) return rc;
         if (e1 > e0) {
-            hipLaunchKernelGGL(k_k4_rows, dim3(rgrid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, boff.p, b0, e0, e1,
-                               bits.p);
-            ZG_HIP(hipGetLastError());
+            uint64_t b1 = 0;
+            rc = read_u64(boff.p + u1, &b1);
+            if (rc) return rc;
+            ZG_HIP(hipMemsetAsync(bits.p, 0, 8 * (b1 - b0), st));   // rows of edges into heads with no out-list stay 0
+            for (int heavy = 0; heavy < 2; ++heavy) {
+                if (nitems[heavy] == 0) continue;
+                HeadArgs A = head_args(heavy);
+                A.boff = boff.p;
+                A.bbase = b0;
+                A.u0 = u0;
+                A.u1 = u1;
+                A.rows = bits.p;
+                if (heavy)
+                    hipLaunchKernelGGL(k_head_heavy<1>, dim3(hgrid[1]), dim3(kTriHeavyThreads), 0, st, A, nullptr);
+                else
+                    hipLaunchKernelGGL(k_head_light<1>, dim3(hgrid[0]), dim3(256), 0, st, A, nullptr);
+                ZG_HIP(hipGetLastError());
+            }
             hipLaunchKernelGGL(k_k4_count, dim3(cgrid), dim3(256), 0, st, ooff.p, osrc.p, boff.p, b0, e0, e1, bits.p,
                                part.p);
             ZG_HIP(hipGetLastError());
