@@ -222,11 +222,12 @@ __global__ __launch_bounds__(256) void k_ew_orient(const uint64_t *off, const ui
 // pos_of_rank[edge rank] = k; with edge_of_rank, nbr[k] = edge_of_rank[edge
 // rank] instead of the neighbour (an in-list entry becomes the index of its
 // twin in the out-lists: both copies of an edge carry its rank, and a run of
-// parallel copies keeps its lowest rank first in both lists).
+// parallel copies keeps its lowest rank first in both lists); with rank_out,
+// rank_out[k] = the edge rank.
 __global__ __launch_bounds__(256) void k_ew_compact(const uint64_t *off, const uint64_t *sadj, uint32_t n, uint64_t m2,
                                                     const uint64_t *mask, const uint64_t *pre, uint32_t *nbr,
                                                     uint32_t *own, uint32_t *pos_of_rank,
-                                                    const uint32_t *edge_of_rank)
+                                                    const uint32_t *edge_of_rank, uint32_t *rank_out)
 {
     const uint32_t lane = threadIdx.x & 63;
     EW_TILES(m2)
@@ -241,6 +242,7 @@ __global__ __launch_bounds__(256) void k_ew_compact(const uint64_t *off, const u
                 nbr[at] = edge_of_rank ? edge_of_rank[w.rk] : w.nb;
                 if (own) own[at] = w.u;
                 if (pos_of_rank) pos_of_rank[w.rk] = (uint32_t)at;
+                if (rank_out) rank_out[at] = w.rk;
             }
         }
     }
@@ -255,6 +257,79 @@ __global__ void k_ew_offsets(const uint64_t *off, uint32_t n, const uint64_t *ma
     {
         const uint64_t i = off[w], g = i >> 6;
         out[w] = pre[g] + (uint64_t)__popcll(mask[g] & ((1ull << (i & 63)) - 1));
+    }
+}
+
+// ---- out-lists in the orientation's order ------------------------------------
+// The 4-clique counter takes out-lists of kSortMin < d <= kSortMax entries
+// sorted by the orientation's order instead of by id: every entry of out(v)
+// comes after v, so in a sorted out(u) the entries that can lie in out(v) for
+// v = out(u)[i] are those after position i (half the row stream, rows that are
+// zero below their diagonal).  Shorter lists stay by id (their work is small),
+// longer ones too (a table holds at most kSortMax keys).  The triangle counter
+// keeps every list by id: there the saved stream costs as much as the sort and
+// the in-list transposition it needs.
+constexpr uint64_t kSortMin = 64, kSortMax = 4096;
+
+__device__ __forceinline__ bool order_sorted(uint64_t d) { return d > kSortMin && d <= kSortMax; }
+
+// vertices whose out-list gets sorted, in any order
+__global__ void k_collect_sorted(const uint64_t *ooff, uint32_t n, uint32_t *list, unsigned int *count)
+{
+    GRID_STRIDE(v, (uint64_t)n)
+    {
+        if (order_sorted(ooff[v + 1] - ooff[v])) list[atomicAdd(count, 1u)] = (uint32_t)v;
+    }
+}
+
+// One block per listed vertex: bitonic sort of its out-list in LDS by
+// (order key, id), carrying the edge ranks, and pos_of_rank of each entry's
+// edge to its new position.
+__global__ __launch_bounds__(256) void k_sort_lists(const uint64_t *ooff, const uint32_t *list, uint32_t nlist,
+                                                    const uint16_t *okey, uint32_t *onb, const uint32_t *orank,
+                                                    uint32_t *pos_of_rank)
+{
+    __shared__ uint64_t key[kSortMax];
+    __shared__ uint32_t val[kSortMax];
+    for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+        const uint32_t u = list[li];
+        const uint64_t b = ooff[u], d = ooff[u + 1] - b;
+        uint32_t P = 2 * kSortMin;
+        while (P < d) P <<= 1;
+        for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
+            if (t < d) {
+                const uint32_t x = onb[b + t];
+                key[t] = ((uint64_t)okey[x] << 32) | x;
+                val[t] = orank[b + t];
+            } else {
+                key[t] = ~0ull;
+                val[t] = 0;
+            }
+        }
+        __syncthreads();
+        for (uint32_t k = 2; k <= P; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
+                    const uint32_t o = t ^ j;
+                    if (o > t) {
+                        const uint64_t a = key[t], c = key[o];
+                        if ((a > c) == ((t & k) == 0)) {
+                            key[t] = c;
+                            key[o] = a;
+                            const uint32_t va = val[t];
+                            val[t] = val[o];
+                            val[o] = va;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t t = threadIdx.x; t < d; t += blockDim.x) {
+            onb[b + t] = (uint32_t)key[t];
+            pos_of_rank[val[t]] = (uint32_t)(b + t);
+        }
+        __syncthreads();
     }
 }
 
@@ -461,30 +536,37 @@ __global__ void k_tri_items_fill(const uint64_t *ipre, uint32_t n, uint64_t *ite
 // GPU scheme); a key still homeless after kCuckooMoves moves fails the build.
 constexpr int kCuckooMoves = 256;
 
-// Both slots from one multiplicative hash p = x * 0x9E3779B1 (bits <= 16): h1
-// is its top `bits` bits and h2 = h1 plus an odd step taken from the next
-// `bits` bits, so h2 != h1 in a table of a power-of-two size.
-__device__ __forceinline__ void cuckoo_slots(uint32_t x, uint32_t bits, uint32_t &h1, uint32_t &h2)
+// Both slots from one multiplicative hash p = x * mult (mult odd, bits <= 16):
+// h1 is its top `bits` bits and h2 = h1 plus an odd step taken from the next
+// `bits` bits, so h2 != h1 in a table of a power-of-two size.  A build fails
+// with probability of order 1 / 2^bits however empty the table (a small
+// component of the slot graph with more keys than slots), so a failed build
+// is redone with the next multiplier, up to kCuckooTries of them.
+constexpr int kCuckooTries = 4;
+
+__device__ __forceinline__ uint32_t cuckoo_mult(int attempt) { return 0x9E3779B1u + 0x6A09E668u * (uint32_t)attempt; }
+
+__device__ __forceinline__ void cuckoo_slots(uint32_t x, uint32_t bits, uint32_t mult, uint32_t &h1, uint32_t &h2)
 {
-    const uint32_t p = x * 0x9E3779B1u, hm = (1u << bits) - 1;
+    const uint32_t p = x * mult, hm = (1u << bits) - 1;
     h1 = p >> (32 - bits);
     h2 = (h1 + (((p >> (32 - 2 * bits)) & hm) | 1u)) & hm;
 }
 
 // insert list[t], list[t + lanes], ...; false if one of them found no slot
-__device__ __forceinline__ bool tri_insert(uint32_t *T, uint32_t bits, const uint32_t *list, uint64_t dv, uint32_t t,
-                                           uint32_t lanes)
+__device__ __forceinline__ bool tri_insert(uint32_t *T, uint32_t bits, uint32_t mult, const uint32_t *list, uint64_t dv,
+                                           uint32_t t, uint32_t lanes)
 {
     bool placed = true;
     for (uint64_t k = t; k < dv; k += lanes) {
         uint32_t cur = list[k], pos, other;
-        cuckoo_slots(cur, bits, pos, other);
+        cuckoo_slots(cur, bits, mult, pos, other);
         int moves = 0;
         for (; moves < kCuckooMoves; ++moves) {
             cur = atomicExch(&T[pos], cur);
             if (cur == kNone) break;
             uint32_t p1, p2;
-            cuckoo_slots(cur, bits, p1, p2);
+            cuckoo_slots(cur, bits, mult, p1, p2);
             pos = pos == p1 ? p2 : p1;
         }
         if (moves == kCuckooMoves) placed = false;
@@ -567,10 +649,11 @@ struct TriCursor {
 };
 
 // bit k: x[k] (with bit k of ok set) is in the table T (2^bits slots) or,
-// when T is null, in the sorted list vl of dv entries
+// when T is null, in the list vl of dv entries (binary search when it is by
+// id, a scan when it is in the orientation's order)
 template <int U>
 __device__ __forceinline__ uint32_t tri_found(const uint32_t (&x)[U], uint32_t ok, const uint32_t *T, uint32_t bits,
-                                              const uint32_t *vl, uint64_t dv)
+                                              uint32_t mult, const uint32_t *vl, uint64_t dv)
 {
     uint32_t f = 0;
     if (T) {
@@ -581,16 +664,23 @@ __device__ __forceinline__ uint32_t tri_found(const uint32_t (&x)[U], uint32_t o
         for (int k = 0; k < U; ++k) {
             const bool v = (ok >> k) & 1;
             uint32_t h1, h2;
-            cuckoo_slots(x[k], bits, h1, h2);
+            cuckoo_slots(x[k], bits, mult, h1, h2);
             y1[k] = T[v ? h1 : 0];
             y2[k] = T[v ? h2 : 0];
         }
 #pragma unroll
         for (int k = 0; k < U; ++k) f |= ((((ok >> k) & 1) && (y1[k] == x[k] || y2[k] == x[k])) ? 1u : 0u) << k;
-    } else {
+    } else if (!order_sorted(dv)) {
 #pragma unroll
         for (int k = 0; k < U; ++k)
             if (((ok >> k) & 1) && bsearch_u32(vl, dv, x[k])) f |= 1u << k;
+    } else {   // only should a table build fail
+        for (uint64_t t = 0; t < dv; ++t) {
+            const uint32_t y = vl[t];
+#pragma unroll
+            for (int k = 0; k < U; ++k)
+                if (((ok >> k) & 1) && x[k] == y) f |= 1u << k;
+        }
     }
     return f;
 }
@@ -599,9 +689,10 @@ __device__ __forceinline__ uint32_t tri_found(const uint32_t (&x)[U], uint32_t o
 // head's table.  Software-pipelined: the next U windows' loads are in flight
 // while the current ones probe the table.
 template <int U>
-__device__ __forceinline__ unsigned long long tri_stream(const uint64_t *ooff, const uint32_t *onb, const uint32_t *inb, uint64_t sb,
-                                         uint64_t ie, const uint32_t *T, uint32_t bits, const uint32_t *vl,
-                                         uint64_t dv)
+__device__ __forceinline__ unsigned long long tri_stream(const uint64_t *ooff, const uint32_t *onb,
+                                                         const uint32_t *inb, uint64_t sb,
+                                                         uint64_t ie, const uint32_t *T, uint32_t bits,
+                                                         uint32_t mult, const uint32_t *vl, uint64_t dv)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t si = sb + lane;
@@ -620,11 +711,11 @@ __device__ __forceinline__ unsigned long long tri_stream(const uint64_t *ooff, c
     for (;;) {   // two buffers in turn, no copies (a copy would wait for the loads)
         bool more = cur.more();
         okb = cur.fill<U>(onb, xb);
-        c += __popc(tri_found<U>(xa, oka, T, bits, vl, dv));
+        c += __popc(tri_found<U>(xa, oka, T, bits, mult, vl, dv));
         if (!more) break;
         more = cur.more();
         oka = cur.fill<U>(onb, xa);
-        c += __popc(tri_found<U>(xb, okb, T, bits, vl, dv));
+        c += __popc(tri_found<U>(xb, okb, T, bits, mult, vl, dv));
         if (!more) break;
     }
     return c;
@@ -736,7 +827,7 @@ struct HeadArgs {
 
 template <int U>
 __device__ __forceinline__ void k4_stream(const HeadArgs &A, uint64_t sb, uint64_t ie, const uint32_t *T,
-                                          uint32_t bits, const uint32_t *vl, uint64_t dv)
+                                          uint32_t bits, uint32_t mult, const uint32_t *vl, uint64_t dv)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t si = sb + lane;
@@ -747,9 +838,12 @@ __device__ __forceinline__ void k4_stream(const HeadArgs &A, uint64_t sb, uint64
         if (u >= A.u0 && u < A.u1) {
             const uint64_t lb = A.ooff[u], D = A.ooff[u + 1] - lb;
             if (D >= 3) {
-                b = lb;
-                d = (uint32_t)D;
-                rb = (uint32_t)(A.boff[u] - A.bbase + (e - lb) * ((D + 63) >> 6));
+                // a sorted out(u): from the word holding position i + 1 (the
+                // words before it stay zero)
+                const uint64_t s0 = order_sorted(D) ? (e - lb + 1) & ~63ull : 0;
+                b = lb + s0;
+                d = (uint32_t)(D - s0);
+                rb = (uint32_t)(A.boff[u] - A.bbase + (e - lb) * ((D + 63) >> 6) + (s0 >> 6));
             }
         }
     }
@@ -760,11 +854,11 @@ __device__ __forceinline__ void k4_stream(const HeadArgs &A, uint64_t sb, uint64
     for (;;) {   // two buffers in turn, as in tri_stream
         bool more = cur.more();
         cur.fill<U>(A.onb, xb, wb, okb, stb);
-        k4_store<U>(A.rows, wa, tri_found<U>(xa, oka, T, bits, vl, dv), sta);
+        k4_store<U>(A.rows, wa, tri_found<U>(xa, oka, T, bits, mult, vl, dv), sta);
         if (!more) break;
         more = cur.more();
         cur.fill<U>(A.onb, xa, wa, oka, sta);
-        k4_store<U>(A.rows, wb, tri_found<U>(xb, okb, T, bits, vl, dv), stb);
+        k4_store<U>(A.rows, wb, tri_found<U>(xb, okb, T, bits, mult, vl, dv), stb);
         if (!more) break;
     }
 }
@@ -785,15 +879,21 @@ __global__ __launch_bounds__(256) void k_head_light(HeadArgs A, unsigned long lo
         const uint64_t ib = A.iin[v] + (item & 0xffffffffull) * kTriChunk, iend = A.iin[v + 1];
         const uint64_t ie = ib + kTriChunk < iend ? ib + kTriChunk : iend;
         const uint32_t bits = kTriWaveBits;
-        for (uint32_t t = lane; t < (1u << bits); t += 64) T[t] = kNone;
-        wave_sync();
-        const bool built = __ballot(!tri_insert(T, bits, A.onb + vb, dv, lane, 64)) == 0;
-        wave_sync();
+        bool built = false;
+        uint32_t mult = 0;
+        for (int attempt = 0; attempt < kCuckooTries && !built; ++attempt) {
+            mult = cuckoo_mult(attempt);
+            for (uint32_t t = lane; t < (1u << bits); t += 64) T[t] = kNone;
+            wave_sync();
+            built = __ballot(!tri_insert(T, bits, mult, A.onb + vb, dv, lane, 64)) == 0;
+            wave_sync();
+        }
         for (uint64_t sb = ib; sb < ie; sb += 64) {
             if (MODE == 0)
-                c += tri_stream<kTriUnroll>(A.ooff, A.onb, A.inb, sb, ie, built ? T : nullptr, bits, A.onb + vb, dv);
+                c += tri_stream<kTriUnroll>(A.ooff, A.onb, A.inb, sb, ie, built ? T : nullptr, bits, mult,
+                                            A.onb + vb, dv);
             else
-                k4_stream<kK4Unroll>(A, sb, ie, built ? T : nullptr, bits, A.onb + vb, dv);
+                k4_stream<kK4Unroll>(A, sb, ie, built ? T : nullptr, bits, mult, A.onb + vb, dv);
         }
         wave_sync();   // before the table is cleared
     }
@@ -822,13 +922,20 @@ __global__ __launch_bounds__(kTriHeavyThreads) __attribute__((amdgpu_waves_per_e
         const uint64_t ie = ib + kTriChunkHeavy < iend ? ib + kTriChunkHeavy : iend;
         const bool fits = 4 * dv <= (1u << kTriBlockBits);
         const uint32_t bits = kTriBlockBits;
-        if (threadIdx.x == 0) { next_batch = 0; failed = 0; }
-        if (fits)
+        bool hashed = false;
+        uint32_t mult = 0;
+        for (int attempt = 0; fits && attempt < kCuckooTries && !hashed; ++attempt) {
+            mult = cuckoo_mult(attempt);
+            if (threadIdx.x == 0) failed = 0;
             for (uint32_t t = threadIdx.x; t < (1u << bits); t += blockDim.x) T[t] = kNone;
+            __syncthreads();
+            if (!tri_insert(T, bits, mult, A.onb + vb, dv, threadIdx.x, blockDim.x)) failed = 1;
+            __syncthreads();
+            hashed = !failed;
+            __syncthreads();   // every thread has read the flag before the next attempt clears it
+        }
+        if (threadIdx.x == 0) next_batch = 0;
         __syncthreads();
-        if (fits && !tri_insert(T, bits, A.onb + vb, dv, threadIdx.x, blockDim.x)) failed = 1;
-        __syncthreads();
-        const bool hashed = fits && !failed;
         const uint32_t batches = (uint32_t)((ie - ib + 63) >> 6);
         for (;;) {
             uint32_t bi = 0;
@@ -837,9 +944,9 @@ __global__ __launch_bounds__(kTriHeavyThreads) __attribute__((amdgpu_waves_per_e
             if (bi >= batches) break;
             if (MODE == 0)
                 c += tri_stream<kTriUnrollHeavy>(A.ooff, A.onb, A.inb, ib + 64ull * bi, ie, hashed ? T : nullptr,
-                                                 bits, A.onb + vb, dv);
+                                                 bits, mult, A.onb + vb, dv);
             else
-                k4_stream<kK4Unroll>(A, ib + 64ull * bi, ie, hashed ? T : nullptr, bits, A.onb + vb, dv);
+                k4_stream<kK4Unroll>(A, ib + 64ull * bi, ie, hashed ? T : nullptr, bits, mult, A.onb + vb, dv);
         }
         __syncthreads();   // before the table and the counter are reset
     }
@@ -893,10 +1000,12 @@ __global__ void k_k4_count(const uint64_t *ooff, const uint32_t *osrc, const uin
             for (uint64_t wb = w0; wb; wb &= wb - 1) c += __popcll(w0 & rb[__builtin_ctzll(wb)]);
             continue;
         }
+        const bool tri = order_sorted(d);   // row j zero below word (j + 1) / 64
         for (uint64_t kb = 0; kb < R; ++kb) {
             for (uint64_t wb = ri[kb]; wb; wb &= wb - 1) {
-                const uint64_t *const rj = rb + (kb * 64 + __builtin_ctzll(wb)) * R;
-                for (uint64_t t = 0; t < R; ++t) c += __popcll(ri[t] & rj[t]);
+                const uint64_t j = kb * 64 + __builtin_ctzll(wb);
+                const uint64_t *const rj = rb + j * R;
+                for (uint64_t t = tri ? (j + 1) >> 6 : 0; t < R; ++t) c += __popcll(ri[t] & rj[t]);
             }
         }
     }
@@ -904,14 +1013,23 @@ __global__ void k_k4_count(const uint64_t *ooff, const uint32_t *osrc, const uin
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-// One wavefront per wide row of [e0, e1), the lanes sharing its words (held
-// in registers up to 64 words).  The wave takes 64 consecutive rows at a time
-// and walks the wide ones among them, so a hub's rows spread over many waves.
+// One wavefront per wide row i of [e0, e1).  Lanes form groups of G lanes (G =
+// R rounded up to a power of two), lane w of a group holding word w of row i;
+// the set bits j of each word of row i are compacted into LDS, and each group
+// takes one j per step (64 / G of them at once, kK4CountUnroll steps of loads
+// in flight) and adds popcount(row_i[w] & row_j[w]).  A sorted vertex's row j
+// is zero below word (j + 1) / 64, so those words are not read.  The wave
+// takes 64 consecutive rows at a time and walks the wide ones among them, so a
+// hub's rows spread over many waves.
+constexpr int kK4CountUnroll = 4;
+
 __global__ __launch_bounds__(256) void k_k4_count_wide(const uint64_t *ooff, const uint32_t *osrc,
                                                        const uint64_t *boff, uint64_t bbase, uint64_t e0,
                                                        uint64_t e1, const uint64_t *bits,
                                                        unsigned long long *partial)
 {
+    __shared__ uint32_t jls[4][64];
+    uint32_t *const jl = jls[threadIdx.x >> 6];
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -925,28 +1043,33 @@ __global__ __launch_bounds__(256) void k_k4_count_wide(const uint64_t *ooff, con
         for (uint64_t m = __ballot(((d + 63) >> 6) > kK4NarrowWords); m; m &= m - 1) {
             const uint64_t e = b + __builtin_ctzll(m);
             const uint32_t u = osrc[e];
-            const uint64_t lb = ooff[u], R = (ooff[u + 1] - lb + 63) >> 6;
+            const uint64_t lb = ooff[u], D = ooff[u + 1] - lb, R = (D + 63) >> 6;
             const uint64_t *const rb = bits + (boff[u] - bbase);
             const uint64_t *const ri = rb + (e - lb) * R;
             if (R <= 64) {
-                // four set bits per pass, so four row loads are in flight
-                const uint64_t mine = lane < R ? ri[lane] : 0;
-                for (uint64_t kb = 0; kb < R; ++kb) {
-                    for (uint64_t wb = ri[kb]; wb;) {
-                        uint64_t j[4];
-                        bool ok[4];
+                const bool tri = order_sorted(D);
+                const uint32_t G = R <= 16 ? 16 : (R <= 32 ? 32 : 64), per = 64 / G;
+                const uint32_t w = lane & (G - 1), grp = lane / G;
+                const uint64_t mine = w < R ? ri[w] : 0;
+                for (uint64_t kb = tri ? (e - lb + 1) >> 6 : 0; kb < R; ++kb) {
+                    const uint64_t word = ri[kb];   // uniform
+                    if (!word) continue;
+                    if ((word >> lane) & 1) jl[__popcll(word & ((1ull << lane) - 1))] = (uint32_t)(kb * 64 + lane);
+                    wave_sync();
+                    const uint32_t cnt = (uint32_t)__popcll(word);
+                    for (uint32_t q0 = 0; q0 < cnt; q0 += per * kK4CountUnroll) {
+                        uint64_t x[kK4CountUnroll];
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            ok[q] = wb != 0;
-                            j[q] = kb * 64 + (ok[q] ? (uint64_t)__builtin_ctzll(wb) : 0);
-                            wb &= wb - 1;
+                        for (int k = 0; k < kK4CountUnroll; ++k) {
+                            const uint32_t q = q0 + per * k + grp;
+                            const uint32_t j = q < cnt ? jl[q] : 0;
+                            const bool v = q < cnt && w < R && (!tri || w >= ((j + 1) >> 6));
+                            x[k] = v ? rb[(uint64_t)j * R + w] : 0;
                         }
-                        uint64_t x[4];
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) x[q] = lane < R ? rb[(ok[q] ? j[q] : j[0]) * R + lane] : 0;
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) c += ok[q] ? __popcll(mine & x[q]) : 0;
+                        for (int k = 0; k < kK4CountUnroll; ++k) c += __popcll(mine & x[k]);
                     }
+                    wave_sync();   // before jl is rewritten
                 }
                 continue;
             }
@@ -1451,7 +1574,7 @@ int ew_orient(zg_graph *g, const uint16_t *okey, EwMask &out, EwMask *in)
 // entries), neighbours and, when own is given, owners (pos_of_rank and
 // edge_of_rank as for k_ew_compact).
 int ew_lists(zg_graph *g, const EwMask &e, XB<uint64_t> &offs, XB<uint32_t> &nbr, XB<uint32_t> *own,
-             uint32_t *pos_of_rank = nullptr, const uint32_t *edge_of_rank = nullptr)
+             uint32_t *pos_of_rank = nullptr, const uint32_t *edge_of_rank = nullptr, uint32_t *rank_out = nullptr)
 {
     const uint64_t m2 = 2 * (uint64_t)g->m, groups = (m2 + 63) >> 6;
     hipStream_t st = g->stream;
@@ -1461,7 +1584,7 @@ int ew_lists(zg_graph *g, const EwMask &e, XB<uint64_t> &offs, XB<uint32_t> &nbr
     hipLaunchKernelGGL(k_ew_offsets, dim3(gridx((uint64_t)g->n + 1)), dim3(256), 0, st, g->off, g->n, e.mask.p,
                        e.pre.p, offs.p);
     hipLaunchKernelGGL(k_ew_compact, dim3(ew_grid(groups)), dim3(256), 0, st, g->off, g->sadj, g->n, m2, e.mask.p,
-                       e.pre.p, nbr.p, own ? own->p : nullptr, pos_of_rank, edge_of_rank);
+                       e.pre.p, nbr.p, own ? own->p : nullptr, pos_of_rank, edge_of_rank, rank_out);
     ZG_HIP(hipGetLastError());
     ZG_HIP(hipStreamSynchronize(st));   // before a mask is freed
     return ZG_OK;
@@ -1573,14 +1696,41 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         EwMask omask, imask;
         int rc = ew_orient(g, okey.p, omask, what <= 1 ? &imask : nullptr);
         if (rc) return rc;
-        okey.reset();
         ne = omask.total;
         if (what == 1 && ne >= (1ull << 32)) return set_error(ZG_EINVAL, "4-clique counts need m < 2^32");
-        XB<uint32_t> pos_of_rank;   // out-list index of each edge rank (4-cliques)
-        if (what == 1) XALLOC(pos_of_rank, g->m);
-        rc = ew_lists(g, omask, ooff, onb, what == 0 ? nullptr : &osrc, pos_of_rank.p);
-        if (!rc && what <= 1) rc = ew_lists(g, imask, iin, inb, nullptr, nullptr, pos_of_rank.p);
+        XB<uint32_t> pos_of_rank, orank;   // 4-cliques: out-list index and edge rank of each out-list entry
+        if (what == 1) {
+            XALLOC(pos_of_rank, g->m);
+            XALLOC(orank, ne);
+        }
+        rc = ew_lists(g, omask, ooff, onb, what == 0 ? nullptr : &osrc, pos_of_rank.p, nullptr, orank.p);
         if (rc) return rc;
+        omask.mask.reset();
+        omask.pre.reset();
+        if (what == 0) {
+            rc = ew_lists(g, imask, iin, inb, nullptr);
+            if (rc) return rc;
+        }
+        if (what == 1) {   // out-lists of kSortMin < d <= kSortMax entries into the orientation's order
+            XB<uint32_t> slist;
+            XB<unsigned int> scount;
+            XALLOC(slist, n);
+            XALLOC(scount, 1);
+            ZG_HIP(hipMemsetAsync(scount.p, 0, sizeof(unsigned int), st));
+            hipLaunchKernelGGL(k_collect_sorted, dim3(gridx(n)), dim3(256), 0, st, ooff.p, n, slist.p, scount.p);
+            unsigned int ns = 0;
+            ZG_HIP(hipMemcpyAsync(&ns, scount.p, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+            ZG_HIP(hipStreamSynchronize(st));
+            if (ns)
+                hipLaunchKernelGGL(k_sort_lists, dim3((int)std::min<uint64_t>(ns, (uint64_t)device_cus() * 8)),
+                                   dim3(256), 0, st, ooff.p, slist.p, ns, okey.p, onb.p, orank.p, pos_of_rank.p);
+            ZG_HIP(hipGetLastError());
+            ZG_HIP(hipStreamSynchronize(st));   // before the temporaries are freed
+            orank.reset();
+            rc = ew_lists(g, imask, iin, inb, nullptr, nullptr, pos_of_rank.p);
+            if (rc) return rc;
+        }
+        okey.reset();
         XB<unsigned long long> mx;
         XALLOC(mx, 1);
         ZG_HIP(hipMemsetAsync(mx.p, 0, 8, st));
