@@ -12,13 +12,16 @@
 //   4-cycles   no reference program (the 5-chain closed form needs them)
 // Method: orient every undirected pair from lower to higher (degree, id) and
 // keep the out-lists sorted by neighbour id (taken from the neighbour-ordered
-// copy, parallel edges collapsed).  Triangles: one wavefront per oriented edge
-// u->v binary-searches the entries of one out-list in the other, whichever
-// takes fewer passes times search depth.  4-cliques:
-// per vertex u, bit rows over out(u) (row i: which of out(u) are in
-// out(out(u)[i])) and the sum of popcount(row_i & row_j) over the set bits j
-// of each row.  4-cycles: at their highest vertex, counting the wedges below
-// it per opposite vertex (LDS hashes, dense counters for hubs).
+// copy, parallel edges collapsed), with in-lists of the tails of each head.
+// Triangles by head: out(v) in an LDS cuckoo table, the out-lists of v's
+// tails streamed through it (k_head_light / k_head_heavy).  4-cliques: per
+// vertex u, bit rows over out(u) (row i: which of out(u) are in
+// out(out(u)[i])), built from the same stream, and the sum of
+// popcount(row_i & row_j) over the set bits j of each row.  The 5-chain
+// count's per-vertex weighted triangle sum: one wavefront per oriented edge
+// binary-searching one out-list in the other (k_tri_wave).  4-cycles: at
+// their highest vertex, counting the wedges below it per opposite vertex (LDS
+// hashes, sorted wedge keys and dense counters for hubs).
 #include <cstdlib>
 #include <cstring>
 
