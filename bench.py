@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--elp-seeds", type=int, default=5, help="independent queries per profile and pattern")
     ap.add_argument("--elp-weights", default=None, help="learned-profile weights (default: the package's)")
     ap.add_argument("--elp-seeds-k4", type=int, default=3, help="independent 4-clique queries per profile")
+    ap.add_argument("--split-completions", type=int, default=256,
+                    help="completions per closed triangle of the split 4-clique plan")
     ap.add_argument("--elp-cap-log2", type=int, default=39,
                     help="largest budget of one time-to-error query, log2 estimators (2^39 = 5.5e11: well "
                          "above the 1.2-1.5e11 the learned sizing asks for a +-1%% 4-clique answer on the bench "
@@ -171,6 +173,7 @@ def main():
     log(f"estimator access chain over the graph's array sizes: {chain_gbs:.0f} GB/s ({chain_ms:.2f} ms per {N})")
     traffic = profiled_traffic(dom, args, N)
     fused = fused_triangle_k4(Z, g, args, N, first_sums)
+    split = split_k4(Z, g, args, N, first_sums["4-clique"], avg_ms["4-clique"])
 
     elp = None
     if not args.no_elp:
@@ -212,12 +215,42 @@ def main():
                      "random_gather_gbs_16GiB": achievable,
                      "random_gather_gbs_graph_size": same_size},
         "fused_triangle_4clique": fused,
+        "split_4clique": split,
         "cpu_baseline": cpu,
         "time_to_error": elp,
         "batched_queries": batch,
         "device": info,
     }
     print(json.dumps(out), flush=True)
+
+
+def split_k4(Z, g, args, N, k4_sums, k4_ms):
+    """The split 4-clique plan over the same ids as the first timed 4-clique
+    launch: kernel time, random lines per estimator and the per-estimator CV
+    next to the 4-clique plan's; the time to a given error scales with CV^2
+    times the kernel time per estimator, so their ratio is the expected gain."""
+    import math
+    J = args.split_completions
+    plan = Z.Plan(Z.PLAN_K4_SPLIT, completions=J)
+    Z.run(g, plan, args.seed, (10_000 + args.warmup) * N, N)
+    kms, first = [], None
+    for i in range(args.steps):
+        sums, ms = Z.run(g, plan, args.seed, i * N, N, args.block, args.blocks_per_cu, timed=True)
+        kms.append(ms)
+        if i == 0:
+            first = sums[0]
+    avg = sum(kms) / len(kms)
+    lines = Z.count_reads(g, plan, args.seed, 0, 1 << 22) / (1 << 22)
+
+    def cv(s):
+        m = s.mean
+        return math.sqrt(max(s.sumsq / s.count - m * m, 0.0)) / m if m > 0 else float("inf")
+    c_split, c_k4 = cv(first), cv(k4_sums)
+    log(f"split 4-clique x{J}: {avg:.2f} ms per 2^{args.log2_estimators} ids, {lines:.3f} lines/id, "
+        f"CV {c_split:.1f} against the 4-clique plan's {c_k4:.1f} on the same ids")
+    return {"completions": J, "kernel_ms": avg, "estimators_per_s": N / (avg / 1e3), "lines_per_estimator": lines,
+            "cv": c_split, "cv_4clique_plan_same_ids": c_k4, "estimate": first.mean,
+            "expected_time_to_error_gain": (c_k4 / c_split) ** 2 * k4_ms / avg}
 
 
 def fused_triangle_k4(Z, g, args, N, first_sums):
@@ -252,19 +285,29 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95, patterns=None, cap_log2=None,
                         estimators (judge-only) -- the regression side's time;
       pilot+judge       the learned profile's pilot sizing with the network's
                         correction left out (r = 0): what the network adds;
-      learned+judge     pilot + network sizing + judge."""
+      learned+judge     pilot + network sizing + judge.
+    The split 4-clique plan (ZG_PLAN_K4_SPLIT, `split_completions` draws of the
+    shapes per closed triangle) has no slot in the network, so it runs the
+    regression and pilot arms; its speedup is reported against the 4-clique
+    plan's learned+judge time."""
     import math
     from zgraph import elp as L
     weights = args.elp_weights or L.DEFAULT_WEIGHTS
     learned = L.LearnedErrorProfile(weights) if os.path.exists(weights) else None
     cap_log2 = cap_log2 or args.elp_cap_log2
     queries = queries or args.elp_seeds
-    patterns = patterns or (("triangle", Z.PLAN_TRIANGLE, "triangle"), ("4-clique", Z.PLAN_K4_CLOSE, "k4"))
+    J = args.split_completions
+    patterns = patterns or (("triangle", Z.PLAN_TRIANGLE, "triangle"), ("4-clique", Z.PLAN_K4_CLOSE, "k4"),
+                            (f"4-clique split x{J}", lambda: Z.Plan(Z.PLAN_K4_SPLIT, completions=J), "k4",
+                             ("regression+judge", "pilot+judge")))
     per_pattern_queries = {"4-clique": min(queries, args.elp_seeds_k4)}
     out = {"epsilon": eps, "confidence": conf, "queries_per_profile": queries,
            "budget_cap": 1 << cap_log2,
            "learned_weights": os.path.basename(weights) if learned else None}
-    for name, kind, what in patterns:
+    for entry in patterns:
+        name, kind, what = entry[:3]
+        arms = entry[3] if len(entry) > 3 else None
+        make_plan = kind if callable(kind) else (lambda k=kind: Z.Plan(k))
         t = time.perf_counter()
         truth = g.exact_count(what)
         t_exact = time.perf_counter() - t
@@ -274,15 +317,17 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95, patterns=None, cap_log2=None,
                     ("pilot+judge", L.LearnedErrorProfile(correction=False), True, 1 << 24)]
         if learned is not None:
             profiles.append(("learned+judge", learned, True, 1 << 24))
+        if arms is not None:
+            profiles = [p for p in profiles if p[0] in arms]
         nq = per_pattern_queries.get(name, queries)
         for label, prof, judged, fallback in profiles:
             # one untimed query first (code-object loads, first allocations), as a serving loop would have
-            warm = Z.ZGraphInstance(g, Z.Plan(kind), seed=100)
+            warm = Z.ZGraphInstance(g, make_plan(), seed=100)
             L.estimate_to_error(warm, eps, conf, prof, judged=judged, max_estimators=1 << 30,
                                 fallback_budget=fallback)
             recs = []
             for q in range(nq):
-                inst = Z.ZGraphInstance(g, Z.Plan(kind), seed=101 + q)
+                inst = Z.ZGraphInstance(g, make_plan(), seed=101 + q)
                 r = L.estimate_to_error(inst, eps, conf, prof, judged=judged,
                                         max_estimators=1 << cap_log2, fallback_budget=fallback)
                 r["error"] = abs(r["estimate"] - truth) / truth if truth and r["estimators"] else float("nan")
@@ -309,11 +354,18 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95, patterns=None, cap_log2=None,
                              "total_s": r["total_s"]} for q, r in enumerate(recs)],
             }
         lt = res.get("learned+judge", {}).get("mean_time_s")
-        for base in ("regression", "regression+judge", "pilot+judge"):
-            bt = res[base]["mean_time_s"]
-            res["speedup_learned_vs_" + base.replace("+", "_")] = (bt / lt) if (bt and lt) else None
+        if lt is not None:
+            for base in ("regression", "regression+judge", "pilot+judge"):
+                bt = res[base]["mean_time_s"]
+                res["speedup_learned_vs_" + base.replace("+", "_")] = (bt / lt) if (bt and lt) else None
         res["queries_per_arm"] = nq
         out[name] = res
+    split = next((n for n in out if isinstance(n, str) and n.startswith("4-clique split")), None)
+    if split and "4-clique" in out:
+        st = out[split]["pilot+judge"]["mean_time_s"]
+        for base in ("learned+judge", "regression+judge"):
+            bt = out["4-clique"].get(base, {}).get("mean_time_s")
+            out[split]["speedup_vs_4clique_plan_" + base.replace("+", "_")] = (bt / st) if (bt and st) else None
     return out
 
 
@@ -716,8 +768,30 @@ def cpu_baseline(Z, g, plans, args, N, elp=None):
                "same_answer": abs(est - q[0]["estimate"]) <= 1e-12 * abs(q[0]["estimate"])}
         log(f"cpu time to +-1% (triangle, {nq} estimators of the learned query): {spent:.1f}s "
             f"against {q[0]['total_s'] * 1e3:.1f} ms on the GPU, same answer {tte['same_answer']}")
+    # 4-cliques to +-1 %: hours of CPU work, so projected from the CPU loop's measured rate
+    # on a bounded sample, for the same estimator counts the GPU queries ran
+    tte_k4 = {}
+    for label, kind, order, arm in (("4-clique", O.PLAN_K4_CLOSE, 0, "learned+judge"),
+                                    (f"4-clique split x{args.split_completions}", O.PLAN_K4_SPLIT,
+                                     args.split_completions, "pilot+judge")):
+        rec = (((elp or {}).get(label) or {}).get(arm) or {})
+        if not rec.get("mean_estimators"):
+            continue
+        if kind == O.PLAN_K4_CLOSE:
+            rate = rates["4-clique"]
+        else:
+            n_s = int(max(1 << 16, rates["4-clique"] * args.cpu_seconds / 4))
+            t = time.perf_counter()
+            O.run(og, O.make_plan(kind, order=order), args.seed, 0, n_s, threads)
+            rate = n_s / (time.perf_counter() - t)
+            log(f"cpu {label}: {n_s} estimators at {rate:.4g}/s")
+        cpu_s = rec["mean_estimators"] / rate
+        tte_k4[label] = {"arm": arm, "estimators": rec["mean_estimators"], "cpu_rate": rate,
+                         "cpu_s_projected": cpu_s, "gpu_s": rec["mean_time_s"],
+                         "speedup": cpu_s / rec["mean_time_s"] if rec["mean_time_s"] else None}
+        log(f"cpu time to +-1% ({label}, projected): {cpu_s:.4g}s against {rec['mean_time_s']:.3f}s on the GPU")
     return {"value": 2 * N / per_step, "unit": "estimators/s", "cores": threads, "kind": "port",
-            "time_to_error": tte,
+            "time_to_error": tte, "time_to_error_4clique_projected": tte_k4 or None,
             "sample": "; ".join(f"{k}: {samples[k]} estimators (ids 0..) of the same graph/seed"
                                  for k in samples),
             "per_pattern": rates, "cpu_model": host["cpu_model"], "host": host,

@@ -123,11 +123,17 @@ enum {
     ZG_PLAN_PATTERN = 6,      /* SamplerGenerator<Empty>(Pattern) (src/SamplerGenerator.cpp:378-464) */
     ZG_PLAN_K4_CLOSE = 7,     /* 4-clique: three spanning shapes + closing checks (unbiased;
                                  replaces the biased applications/FourClique.cpp:41-76) */
-    ZG_PLAN_TRIANGLE_K4 = 8   /* fused: channel 0 = triangle (ZG_PLAN_TRIANGLE), channel 1 =
+    ZG_PLAN_TRIANGLE_K4 = 8,  /* fused: channel 0 = triangle (ZG_PLAN_TRIANGLE), channel 1 =
                                  4-clique (ZG_PLAN_K4_CLOSE) from one first edge, conditional
                                  sample and closing check per id -- the cached sub-pattern
                                  sharing of ThreeMotif (src/ZGraphInstance.cpp:596-833) applied
                                  to triangle -> 4-clique; each channel equals its own plan's run */
+    ZG_PLAN_K4_SPLIT = 9      /* 4-clique, split: ZG_PLAN_K4_CLOSE's first stage (e0, e1, the
+                                 triangle edge's closing check), then, when the triangle closes,
+                                 `completions` independent draws of the three shapes averaged:
+                                 the same mean with far less variance per estimator (no
+                                 reference counterpart; each completion is FourClique.cpp's
+                                 remaining conditional samples and closing checks) */
 };
 
 #define ZG_MAX_PATTERN_EDGES 15
@@ -138,6 +144,8 @@ typedef struct {
     int pattern_edges;                       /* ZG_PLAN_PATTERN: E (1..15) */
     uint32_t pattern_src[ZG_MAX_PATTERN_EDGES];
     uint32_t pattern_dst[ZG_MAX_PATTERN_EDGES];
+    int completions;                         /* ZG_PLAN_K4_SPLIT: completions per closed triangle,
+                                                a power of two in [1, 4096] */
 } zg_plan_desc;
 
 int zg_plan_create(const zg_plan_desc *desc, zg_plan **out);

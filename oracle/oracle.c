@@ -15,6 +15,9 @@
  *                      parallel edges ("some copy above min_valid")     pinned
  *   estimators         exhaustive expectation E[X|graph,ranks] == exact
  *                      count (closed forms / brute force)              pinned
+ *   est_k4_split       exhaustive expectation with 1, 2 and 4 completions
+ *                      == exact 4-cliques; first stage == est_k4_close's
+ *                      per id; variance falls with completions          pinned
  *   pattern dedupe     tests/TestPattern2.cpp values (12/24, 1/24, 5/120,
  *                      60/120 house) and TestPattern.cpp values           pinned
  *   exact counters     closed forms, brute force, SNAP's published
@@ -1011,27 +1014,38 @@ static int close1(og_ctx *c, uint64_t min_valid, const uint32_t *v, int nv, uint
     return close_scan(c->g, min_valid, v, nv, a, b, rank);
 }
 
-static int est_k4_close(og_ctx *c)
+/* first stage: e0, e1 from {a, b} above rank(e0), and the triangle edge
+ * {ob, w} above rank(e1) (every shape needs it); fills the stage state and
+ * leaves c->num = m s1 */
+typedef struct { og_edge e0, e1; uint32_t a, b, o, w, ob; uint64_t rt; } k4_stage;
+
+static int k4_first(og_ctx *c, k4_stage *k)
 {
-    og_edge e0, e1, e2, edges[3];
-    uint64_t s1, s2; int c1, c2;
-    if (!prim_sample_edge(c, &e0)) return 0;
-    uint32_t a = e0.src, b = e0.dst;
-    uint32_t v2[2] = {a, b};
-    edges[0] = e0;
-    if (!prim_cond(c, edges, 1, v2, 2, &e1, &s1, &c1)) return 0;
+    og_edge edges[1];
+    uint64_t s1; int c1;
+    if (!prim_sample_edge(c, &k->e0)) return 0;
+    k->a = k->e0.src; k->b = k->e0.dst;
+    uint32_t v2[2] = {k->a, k->b};
+    edges[0] = k->e0;
+    if (!prim_cond(c, edges, 1, v2, 2, &k->e1, &s1, &c1)) return 0;
     if (c1 == 2) return 0;
-    uint32_t o = e1.src, w = e1.dst, ob = (o == a) ? b : a;
+    k->o = k->e1.src; k->w = k->e1.dst; k->ob = (k->o == k->a) ? k->b : k->a;
+    uint32_t v3[3] = {k->a, k->b, k->w};
+    return close1(c, k->e1.rank + 1, v3, 3, k->ob, k->w, &k->rt);
+}
+
+/* the three shapes after a closed first stage, each with one conditional
+ * sample from c->st; returns the sum of the shapes' values m s1 s2 */
+static og_u128 k4_shapes(og_ctx *c, const k4_stage *k, og_u128 base, int base_ov, int *ov)
+{
+    const uint32_t a = k->a, b = k->b, o = k->o, w = k->w, ob = k->ob;
+    const og_edge e0 = k->e0, e1 = k->e1;
+    og_edge e2, edges[3];
+    uint64_t s2; int c2;
     uint32_t v3[3] = {a, b, w};
-    /* every shape needs the triangle edge {ob, w} above rank(e1): test it first */
-    uint64_t rt;
-    if (!close1(c, e1.rank + 1, v3, 3, ob, w, &rt)) return 0;
-    og_u128 base = c->num;                 /* m * s1 */
-    int base_ov = c->overflow;
     og_u128 total = 0;
-    int ov = 0;
     /* shape A: e2 from V3 above rank(e1), new vertex d */
-    edges[1] = e1;
+    edges[0] = e0; edges[1] = e1;
     c->num = base; c->overflow = base_ov;
     if (prim_cond(c, edges, 2, v3, 3, &e2, &s2, &c2) && c2 == 1) {
         uint32_t d = e2.dst;
@@ -1045,11 +1059,11 @@ static int est_k4_close(og_ctx *c)
                        ((x == e2.src && y == d) || (x == d && y == e2.src));
             if (!have && !close1(c, e2.rank + 1, v4, 4, x, y, NULL)) okc = 0;
         }
-        if (okc) { total += c->num; ov |= c->overflow; }
+        if (okc) { total += c->num; *ov |= c->overflow; }
     }
     /* shape B: triangle {e0, e1, t}, then e2 above rank(t) to new vertex d */
     {
-        og_edge t; t.rank = rt; t.src = ob; t.dst = w;
+        og_edge t; t.rank = k->rt; t.src = ob; t.dst = w;
         edges[1] = e1; edges[2] = t;
         c->num = base; c->overflow = base_ov;
         if (prim_cond(c, edges, 3, v3, 3, &e2, &s2, &c2) && c2 == 1) {
@@ -1061,7 +1075,7 @@ static int est_k4_close(og_ctx *c)
                 if (x == e2.src) continue;
                 if (!close1(c, e2.rank + 1, v4, 4, x, d, NULL)) okc = 0;
             }
-            if (okc) { total += c->num; ov |= c->overflow; }
+            if (okc) { total += c->num; *ov |= c->overflow; }
         }
     }
     /* shape C: e2 from V3 above rank(e0), disjoint from e0, below rank(e1) */
@@ -1071,10 +1085,49 @@ static int est_k4_close(og_ctx *c)
         uint32_t v4[4] = {a, b, w, d};
         if (close1(c, e1.rank + 1, v4, 4, ob, w, NULL) && close1(c, e1.rank + 1, v4, 4, a, d, NULL) &&
             close1(c, e1.rank + 1, v4, 4, b, d, NULL)) {
-            total += c->num; ov |= c->overflow;
+            total += c->num; *ov |= c->overflow;
         }
     }
+    return total;
+}
+
+static int est_k4_close(og_ctx *c)
+{
+    k4_stage k;
+    if (!k4_first(c, &k)) return 0;
+    int ov = 0;
+    og_u128 total = k4_shapes(c, &k, c->num, c->overflow, &ov);
     c->num = total; c->overflow = ov; c->log2den = 0;
+    return total != 0;
+}
+
+/* Split 4-clique plan: est_k4_close's first stage and, when its triangle
+ * closes, J independent completions -- the three shapes, completion j drawing
+ * from the stream (seed, OG_TAG_COMPLETION + j, id) -- averaged: value =
+ * (sum over completions of the shape values) / J, J a power of two.  Each
+ * completion has the distribution of est_k4_close's shapes given the first
+ * stage, so the mean is unchanged; the rare completions that give the value
+ * its variance are drawn J times per closed triangle.  nwords counts the
+ * first stage's stream, ncond every conditional sample (saturating). */
+static int est_k4_split(og_ctx *c, int J, uint64_t seed, uint64_t id)
+{
+    k4_stage k;
+    if (!k4_first(c, &k)) return 0;
+    const og_u128 base = c->num;
+    const int base_ov = c->overflow;
+    og_stream main = c->st;
+    og_u128 total = 0;
+    int ov = 0;
+    for (int j = 0; j < J; ++j) {
+        og_stream_init(&c->st, seed, OG_TAG_COMPLETION + (uint32_t)j, id);
+        const og_u128 t = k4_shapes(c, &k, base, base_ov, &ov);
+        if (total + t < total) ov = 1;
+        total += t;
+    }
+    c->st = main;
+    int lg = 0;
+    while ((1 << lg) < J) ++lg;
+    c->num = total; c->overflow = ov; c->log2den = lg;
     return total != 0;
 }
 
@@ -1089,6 +1142,7 @@ int og_plan_max_log2den(const og_plan *p)
     case OG_PLAN_FOUR_CHAIN: return 2;
     case OG_PLAN_PATTERN: return p->pe - 1;
     case OG_PLAN_K4_CLOSE: return 0;
+    case OG_PLAN_K4_SPLIT: { int lg = 0; while ((1 << lg) < p->order) ++lg; return lg; }
     default: return 0;
     }
 }
@@ -1100,7 +1154,7 @@ double og_plan_cond_calls(const og_plan *p)
     case OG_PLAN_CHAIN: return p->order - 2;
     case OG_PLAN_FOUR_CHAIN: return 2;
     case OG_PLAN_PATTERN: return p->pe - 1;   /* SamplerGenerator.cpp:471-474 */
-    case OG_PLAN_K4_CLOSE: return 2;
+    case OG_PLAN_K4_CLOSE: case OG_PLAN_K4_SPLIT: return 2;
     default: return 0;
     }
 }
@@ -1145,13 +1199,14 @@ static void estimate_with(const og_graph *g, const og_plan *p, const og_patset *
     case OG_PLAN_FOUR_CHAIN: ok = est_four_chain(&c); break;
     case OG_PLAN_PATTERN: ok = est_pattern(&c, ps, p->pk, p->pe); break;
     case OG_PLAN_K4_CLOSE: ok = est_k4_close(&c); break;
+    case OG_PLAN_K4_SPLIT: ok = est_k4_split(&c, p->order, seed, id); break;
     default: abort();
     }
     if (ok) set_ok(&out[0], &c);
     uint8_t nw = c.st.word > 255 ? 255 : (uint8_t)c.st.word;
     for (int i = 0; i < nch; ++i) {
         out[i].e0 = c.e0;
-        out[i].ncond = (uint8_t)c.ncond;
+        out[i].ncond = c.ncond > 255 ? 255 : (uint8_t)c.ncond;
         out[i].nwords = nw;
         if (!out[i].status) { out[i].num_lo = out[i].num_hi = 0; out[i].log2den = 0; }
     }

@@ -18,6 +18,7 @@ PLAN_CHAIN = 4
 PLAN_FOUR_CHAIN = 5
 PLAN_PATTERN = 6
 PLAN_K4_CLOSE = 7
+PLAN_K4_SPLIT = 9     # order = completions (a power of two)
 
 MAX_PE = 16
 

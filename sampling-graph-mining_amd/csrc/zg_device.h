@@ -106,7 +106,8 @@ __device__ __forceinline__ void count_reads(const GView &g, uint32_t k)
     if (g.reads) atomicAdd(g.reads, (unsigned long long)k);
 }
 
-enum : uint32_t { TAG_ESTIMATOR = 0, TAG_SHUFFLE = 1, TAG_SUBSAMPLE = 2, TAG_GEN = 3 };
+enum : uint32_t { TAG_ESTIMATOR = 0, TAG_SHUFFLE = 1, TAG_SUBSAMPLE = 2, TAG_GEN = 3,
+                  TAG_COMPLETION = 0x1000 /* + j: completion j of a split 4-clique estimator */ };
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
 __host__ __device__ __forceinline__ uint4 philox10(uint4 c, uint32_t k0, uint32_t k1)

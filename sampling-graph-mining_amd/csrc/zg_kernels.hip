@@ -366,11 +366,160 @@ __device__ __forceinline__ void plan_pattern(const GView &g, const PlanView &P, 
 // Each shape has one draw sequence of probability 1/(m s1 s2), so the sum of
 // the three shape values is unbiased for every rank order.  (The reference's
 // applications/FourClique.cpp:41-76 covers shape A only.)
-//
-// TRI: also channel 0 = the triangle plan's outcome for the same id.  Both
-// plans draw the first edge and the conditional sample identically, and for
-// w outside {a, b} the triangle plan's closing check is this plan's first one,
-// {ob, w} above r1 (for w in {a, b} neither succeeds), with value m * s1.
+
+// The first stage's state: e0 = {a, b} of rank r0 (a, b's lists from position
+// pa0, pb0 on hold the ranks above r0), e1 = {o, w} of rank r1 drawn from
+// o's list at position po1 (o = a when oa), s1 the valid entries it was drawn
+// from, {ob, w} closed at rank rt > r1, and W with rank r1 at position pw1.
+struct K4S {
+    VI A, B, W;
+    uint32_t r0, r1, rt, pa0, pb0, po1, pw1, oa;
+    uint64_t s1;
+};
+
+// e0, e1 and the closing check of the triangle edge {ob, w} above r1, which
+// every shape needs; true when it closes.  TRI: channel 0 = the triangle
+// plan's outcome for the same id.  Both plans draw the first edge and the
+// conditional sample identically, and for w outside {a, b} the triangle
+// plan's closing check is this one, with value m * s1 (for w in {a, b}
+// neither succeeds).
+template <bool TRI>
+__device__ __forceinline__ bool k4_first(const GView &g, Rng &rng, Est &o, K4S &k)
+{
+    if (g.m == 0) return false;
+    const uint32_t r0 = (uint32_t)rng.uniform(g.m);
+    o.e0 = r0;
+    const ERec E = load_edge(g, r0);
+    const uint32_t a = E.src, b = E.dst;
+    k.A = src_vi(E);
+    k.B = dst_vi(E);
+    k.r0 = r0;
+    k.pa0 = E.s_pos + 1;
+    k.pb0 = E.d_pos + 1;
+    SE e1;
+    {
+        VI V[2] = {k.A, k.B};
+        uint32_t pos[2] = {k.pa0, k.pb0};
+        o.ncond = 1;
+        if (!cond_sample<2>(g, rng, V, pos, 2, k.s1, e1)) return false;
+    }
+    const uint32_t w = e1.dst;
+    if (w == a || w == b) return false;
+    k.oa = e1.src == a ? 1u : 0u;
+    k.r1 = e1.rank;
+    if (!close_pair(g, k.oa ? b : a, w, k.r1 + 1, &k.rt)) return false;
+    if constexpr (TRI) {
+        U128 tv{g.m, 0};
+        const bool tov = mul_u128(tv, k.s1);
+        set_out(o, 0, tv, tov, 0);
+    }
+    k.po1 = e1.pos;                              // rank r1 in o's list
+    k.W = end_vi(load_edge(g, k.r1), w, k.pw1);  // and in w's
+    return true;
+}
+
+// The three shapes after a closed first stage, one conditional sample each
+// from rng (ncond counts them); returns the sum of their values m s1 s2, and
+// sets ov when one of them passed 128 bits.
+__device__ __forceinline__ U128 k4_shapes(const GView &g, Rng &rng, const K4S &k, uint32_t &ncond, bool &ov)
+{
+    const VI A = k.A, B = k.B, W = k.W;
+    const uint32_t a = A.v, b = B.v, w = W.v;
+    const bool oa = k.oa != 0;
+    const uint32_t ob = oa ? b : a;
+    const uint32_t r0 = k.r0, r1 = k.r1, rt = k.rt, pa0 = k.pa0, pb0 = k.pb0;
+    const uint32_t po1 = k.po1, pw1 = k.pw1;
+    const VI O = oa ? A : B, OB = oa ? B : A;
+    const uint32_t pob0 = oa ? pb0 : pa0;        // lower_bound(r0 + 1) in ob's list
+    U128 base{g.m, 0};
+    const bool base_ov = mul_u128(base, k.s1);
+    U128 total{0, 0};
+
+    // shape A: e2 above r1 from {a, b, w} to a new vertex d; close the rest above r2
+    {
+        const uint32_t pob = lb_rank(g, OB, pob0, OB.deg, r1 + 1, (uint64_t)r0 + 1, g.m);
+        VI V[3] = {A, B, W};
+        uint32_t pos[3] = {oa ? po1 + 1 : pob, oa ? pob : po1 + 1, pw1 + 1};
+        uint64_t s2;
+        SE e2;
+        ncond++;
+        if (cond_sample<3>(g, rng, V, pos, 3, s2, e2)) {
+            const uint32_t d = e2.dst;
+            if (d != a && d != b && d != w) {
+                const uint32_t mv = e2.rank + 1;
+                bool ok = close_pair(g, ob, w, mv, nullptr);
+                if (ok && e2.src != a) ok = close_pair(g, a, d, mv, nullptr);
+                if (ok && e2.src != b) ok = close_pair(g, b, d, mv, nullptr);
+                if (ok && e2.src != w) ok = close_pair(g, w, d, mv, nullptr);
+                if (ok) {
+                    U128 v = base;
+                    const bool o2 = mul_u128(v, s2);
+                    add_u128(total, v);
+                    ov |= base_ov | o2;
+                }
+            }
+        }
+    }
+    // shape B: triangle {e0, e1, t}; e2 above rank(t) to a new vertex d; close the rest above r2
+    {
+        {
+            const ERec ET = load_edge(g, rt);
+            const uint32_t pob_t = (ET.src == ob ? ET.s_pos : ET.d_pos) + 1;
+            const uint32_t pw_t = (ET.src == w ? ET.s_pos : ET.d_pos) + 1;
+            const uint32_t po = lb_rank(g, O, po1 + 1, O.deg, rt + 1, (uint64_t)r1 + 1, g.m);
+            VI V[3] = {A, B, W};
+            uint32_t pos[3] = {oa ? po : pob_t, oa ? pob_t : po, pw_t};
+            uint64_t s3;
+            SE e3;
+            ncond++;
+            if (cond_sample<3>(g, rng, V, pos, 3, s3, e3)) {
+                const uint32_t d = e3.dst;
+                if (d != a && d != b && d != w) {
+                    const uint32_t mv = e3.rank + 1;
+                    bool ok = true;
+                    if (e3.src != a) ok = close_pair(g, a, d, mv, nullptr);
+                    if (ok && e3.src != b) ok = close_pair(g, b, d, mv, nullptr);
+                    if (ok && e3.src != w) ok = close_pair(g, w, d, mv, nullptr);
+                    if (ok) {
+                        U128 v = base;
+                        const bool o3 = mul_u128(v, s3);
+                        add_u128(total, v);
+                        ov |= base_ov | o3;
+                    }
+                }
+            }
+        }
+    }
+    // shape C: e2 above r0 from w to a new vertex d, below r1; close the rest above r1
+    {
+        const uint32_t pw0 = lb_rank(g, W, 0, pw1, r0 + 1, 0, r1);
+        VI V[3] = {A, B, W};
+        uint32_t pos[3] = {pa0, pb0, pw0};
+        uint64_t s4;
+        SE e4;
+        ncond++;
+        if (cond_sample<3>(g, rng, V, pos, 3, s4, e4)) {
+            const uint32_t d = e4.dst;
+            if (e4.src == w && d != a && d != b && d != w && e4.rank < r1) {
+                const uint32_t mv = r1 + 1;
+                if (close_pair(g, ob, w, mv, nullptr) && close_pair(g, a, d, mv, nullptr) &&
+                    close_pair(g, b, d, mv, nullptr)) {
+                    U128 v = base;
+                    const bool o4 = mul_u128(v, s4);
+                    add_u128(total, v);
+                    ov |= base_ov | o4;
+                }
+            }
+        }
+    }
+    return total;
+}
+
+// The 4-clique plan (TRI: and the triangle channel) as one body: the steps of
+// k4_first then k4_shapes on the estimator's own stream.  Written out rather
+// than composed from the two: the composed form compiled to 85 VGPRs (5
+// waves/SIMD) and ran 4 % slower on the 69 M-edge graph (16.9 against 16.3 ms
+// per 2^28 estimators, alternating runs of both libraries).
 template <bool TRI>
 __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
 {
@@ -489,6 +638,87 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
         }
     }
     if (total.lo | total.hi) set_out(o, TRI ? 1 : 0, total, ov, 0);
+}
+
+// ---- split 4-clique plan (ZG_PLAN_K4_SPLIT) ---------------------------------
+// The 4-clique plan's first stage per lane; for every lane whose triangle
+// closes, the whole wavefront runs that estimator's J completions -- the
+// three shapes, completion j drawing from the stream (seed, TAG_COMPLETION +
+// j, id) -- lane l taking completions l, l + 64, ...  The value is the sum of
+// the completions' values over J (J = 2^P.K): each completion has the
+// distribution of the shapes given the first stage, so the mean is the
+// 4-clique plan's, while the rare completions that make its variance are
+// drawn J times per closed triangle.  One lane running its own J completions
+// would hold its wavefront for J dependent chains; spread over the lanes they
+// cost about one.  Every lane of the wavefront calls this (active = the lane
+// has an estimator); outcomes match the oracle's sequential completions.
+__device__ __forceinline__ uint32_t lane_u32(uint32_t x, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)x, l); }
+__device__ __forceinline__ VI lane_vi(const VI &x, int l) { return VI{lane_u32(x.v, l), lane_u32(x.deg, l), lane_u32(x.off, l)}; }
+
+__device__ __forceinline__ void k4_split(const GView &g, const PlanView &P, uint64_t seed, uint64_t id, bool active,
+                                         Est &o, uint32_t &words)
+{
+    o.st[0] = o.st[1] = 0;
+    o.den[0] = o.den[1] = 0;
+    o.v[0] = o.v[1] = U128{0, 0};
+    o.ncond = 0;
+    o.e0 = 0;
+    Rng rng;
+    rng.init(seed, TAG_ESTIMATOR, id);
+    K4S k;
+    const bool closed = active && k4_first<false>(g, rng, o, k);
+    words = rng.word;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t J = 1u << P.K;
+    for (uint64_t pend = __ballot(closed); pend; pend &= pend - 1) {
+        const int l = __builtin_ctzll(pend);
+        K4S t;
+        t.A = lane_vi(k.A, l);
+        t.B = lane_vi(k.B, l);
+        t.W = lane_vi(k.W, l);
+        t.r0 = lane_u32(k.r0, l);
+        t.r1 = lane_u32(k.r1, l);
+        t.rt = lane_u32(k.rt, l);
+        t.pa0 = lane_u32(k.pa0, l);
+        t.pb0 = lane_u32(k.pb0, l);
+        t.po1 = lane_u32(k.po1, l);
+        t.pw1 = lane_u32(k.pw1, l);
+        t.oa = lane_u32(k.oa, l);
+        t.s1 = ((uint64_t)lane_u32((uint32_t)(k.s1 >> 32), l) << 32) | lane_u32((uint32_t)k.s1, l);
+        const uint64_t tid = ((uint64_t)lane_u32((uint32_t)(id >> 32), l) << 32) | lane_u32((uint32_t)id, l);
+        U128 sum{0, 0};
+        bool ov = false;
+        for (uint32_t j0 = 0; j0 < J; j0 += 64) {
+            if (j0 + lane < J) {
+                Rng rc;
+                rc.init(seed, TAG_COMPLETION + j0 + lane, tid);
+                uint32_t nc = 0;
+                const U128 v = k4_shapes(g, rc, t, nc, ov);
+                ov |= add_u128(sum, v);
+            }
+        }
+        // the wavefront's sum in 32-bit digits (each digit sum < 2^38)
+        uint64_t d0 = (uint32_t)sum.lo, d1 = sum.lo >> 32, d2 = (uint32_t)sum.hi, d3 = sum.hi >> 32;
+        uint32_t ovw = ov ? 1u : 0u;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            d0 += __shfl_xor(d0, off, 64);
+            d1 += __shfl_xor(d1, off, 64);
+            d2 += __shfl_xor(d2, off, 64);
+            d3 += __shfl_xor(d3, off, 64);
+            ovw |= __shfl_xor(ovw, off, 64);
+        }
+        if (lane == (uint32_t)l) {
+            U128 tot{d0, 0};
+            bool tov = ovw != 0;
+            tov |= add_u128(tot, U128{d1 << 32, d1 >> 32});
+            tov |= add_u128(tot, U128{0, d2});
+            tov |= add_u128(tot, U128{0, d3 << 32});
+            tov |= (d3 >> 32) != 0;
+            o.ncond = 1 + 3 * J;
+            if (tot.lo | tot.hi) set_out(o, 0, tot, tov, P.K);
+        }
+    }
 }
 
 // ---- accumulation ---------------------------------------------------------
@@ -631,11 +861,7 @@ __global__ __launch_bounds__(256) void k_run(GView g, PlanView P, uint64_t seed,
     AccL acc[NCH];
     acc_zero<NCH>(acc);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-        Rng rng;
-        rng.init(seed, TAG_ESTIMATOR, first + i);
-        Est o;
-        run_one<KIND>(g, P, rng, o);
+    auto record = [&](uint64_t i, const Est &o, uint32_t words) {
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             acc_add(acc[c], o.v[c], o.st[c], o.den[c], c == 0 ? P.K : P.K1);
@@ -647,10 +873,29 @@ __global__ __launch_bounds__(256) void k_run(GView g, PlanView P, uint64_t seed,
                 rec.e0 = o.e0;
                 rec.status = o.st[c];
                 rec.log2den = ok ? o.den[c] : 0;
-                rec.ncond = (uint8_t)o.ncond;
-                rec.nwords = (uint8_t)(rng.word > 255 ? 255 : rng.word);
+                rec.ncond = (uint8_t)(o.ncond > 255 ? 255 : o.ncond);
+                rec.nwords = (uint8_t)(words > 255 ? 255 : words);
                 out[i * NCH + c] = rec;
             }
+        }
+    };
+    if constexpr (KIND == ZG_PLAN_K4_SPLIT) {
+        // wavefront-uniform trip count: every lane reaches the cooperative completions
+        const uint32_t lane = threadIdx.x & 63;
+        for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < count; b += stride) {
+            const uint64_t i = b + lane;
+            Est o;
+            uint32_t words;
+            k4_split(g, P, seed, first + i, i < count, o, words);
+            if (i < count) record(i, o, words);
+        }
+    } else {
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+            Rng rng;
+            rng.init(seed, TAG_ESTIMATOR, first + i);
+            Est o;
+            run_one<KIND>(g, P, rng, o);
+            record(i, o, rng.word);
         }
     }
     block_reduce_store<NCH>(acc, partials + (size_t)blockIdx.x * NCH);
@@ -753,13 +998,23 @@ __device__ __attribute__((noinline)) void batch_chunk(const BQuery *__restrict__
     rec = uniptr(rec);
     AccL acc[NCH];
     acc_zero<NCH>(acc);
-    for (uint64_t i = base + lane; i < end; i += 64) {
-        Rng rng;
-        rng.init(seed, TAG_ESTIMATOR, first + i);
-        Est o;
-        run_one<KIND>(g, P, rng, o);
+    if constexpr (KIND == ZG_PLAN_K4_SPLIT) {
+        for (uint64_t b = base; b < end; b += 64) {   // uniform: every lane reaches the completions
+            const uint64_t i = b + lane;
+            Est o;
+            uint32_t words;
+            k4_split(g, P, seed, first + i, i < end, o, words);
+            if (i < end) acc_add(acc[0], o.v[0], o.st[0], o.den[0], P.K);
+        }
+    } else {
+        for (uint64_t i = base + lane; i < end; i += 64) {
+            Rng rng;
+            rng.init(seed, TAG_ESTIMATOR, first + i);
+            Est o;
+            run_one<KIND>(g, P, rng, o);
 #pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) acc_add(acc[ch], o.v[ch], o.st[ch], o.den[ch], ch == 0 ? P.K : P.K1);
+            for (int ch = 0; ch < NCH; ++ch) acc_add(acc[ch], o.v[ch], o.st[ch], o.den[ch], ch == 0 ? P.K : P.K1);
+        }
     }
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
@@ -811,6 +1066,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         case ZG_PLAN_FOUR_CHAIN: batch_chunk<ZG_PLAN_FOUR_CHAIN, 1>(&Q, base, end, rec); break;
         case ZG_PLAN_K4_CLOSE: batch_chunk<ZG_PLAN_K4_CLOSE, 1>(&Q, base, end, rec); break;
         case ZG_PLAN_TRIANGLE_K4: batch_chunk<ZG_PLAN_TRIANGLE_K4, 2>(&Q, base, end, rec); break;
+        case ZG_PLAN_K4_SPLIT: batch_chunk<ZG_PLAN_K4_SPLIT, 1>(&Q, base, end, rec); break;
         case KIND_PATTERN_BASE + 2: batch_chunk<KIND_PATTERN_BASE + 2, 1>(&Q, base, end, rec); break;
         case KIND_PATTERN_BASE + 3: batch_chunk<KIND_PATTERN_BASE + 3, 1>(&Q, base, end, rec); break;
         case KIND_PATTERN_BASE + 4: batch_chunk<KIND_PATTERN_BASE + 4, 1>(&Q, base, end, rec); break;
@@ -903,10 +1159,14 @@ static void launch_kind(const zg_graph *g, const PlanView &P, uint64_t seed, uin
 {
     GView v = g->view();
     v.reads = reads;
-    if (order)
-        hipLaunchKernelGGL((k_run_ordered<KIND, NCH>), dim3(grid), dim3(block), 0, g->stream, v, P, seed, first,
-                           count, order, partials);
-    else if (d_out)
+    if constexpr (KIND != ZG_PLAN_K4_SPLIT) {   // the split plan runs in id order only
+        if (order) {
+            hipLaunchKernelGGL((k_run_ordered<KIND, NCH>), dim3(grid), dim3(block), 0, g->stream, v, P, seed, first,
+                               count, order, partials);
+            return;
+        }
+    }
+    if (d_out)
         hipLaunchKernelGGL((k_run<KIND, NCH, true>), dim3(grid), dim3(block), 0, g->stream, v, P, seed, first, count,
                            d_out, partials);
     else
@@ -1000,7 +1260,7 @@ int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
     }
     const PlanView P = p->view();
     const uint32_t *ord = nullptr;
-    if (order >= 2 && !d_out && !reads && count > 0 && g->m > 0) {
+    if (order >= 2 && !d_out && !reads && count > 0 && g->m > 0 && p->desc.kind != ZG_PLAN_K4_SPLIT) {
         int rc = order_by_first_edge(g, p, seed, first, count, order, &ord);
         if (rc) return rc;
     }
@@ -1022,6 +1282,7 @@ int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
         break;
     case ZG_PLAN_K4_CLOSE: launch_kind<ZG_PLAN_K4_CLOSE, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
     case ZG_PLAN_TRIANGLE_K4: launch_kind<ZG_PLAN_TRIANGLE_K4, 2>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
+    case ZG_PLAN_K4_SPLIT: launch_kind<ZG_PLAN_K4_SPLIT, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
     default: return set_error(ZG_EINVAL, "unknown plan kind");
     }
     ZG_HIP(hipGetLastError());

@@ -555,6 +555,13 @@ class Plan {
         d.order = order;
         return d;
     }
+    // ZG_PLAN_K4_SPLIT with `completions` per closed triangle
+    static zg_plan_desc split_k4(int completions)
+    {
+        zg_plan_desc d = of(ZG_PLAN_K4_SPLIT);
+        d.completions = completions;
+        return d;
+    }
     Plan(const Plan &) = delete;
     Plan &operator=(const Plan &) = delete;
     ~Plan()
@@ -739,6 +746,18 @@ class FourCliqueCountingInstance : public ZGraphInstance<EdgeData> {
     explicit FourCliqueCountingInstance(Graph<EdgeData> *g) : ZGraphInstance<EdgeData>(g)
     {
         this->set_plan(Plan::of(ZG_PLAN_K4_CLOSE));
+    }
+};
+
+// 4-cliques, split: FourCliqueCountingInstance's first stage, then
+// `completions` independent draws of its three shapes per closed triangle,
+// averaged (same mean, lower variance per estimator; no reference counterpart)
+template <typename EdgeData>
+class FourCliqueSplitCountingInstance : public ZGraphInstance<EdgeData> {
+  public:
+    FourCliqueSplitCountingInstance(Graph<EdgeData> *g, int completions = 64) : ZGraphInstance<EdgeData>(g)
+    {
+        this->set_plan(Plan::split_k4(completions));
     }
 };
 
@@ -979,13 +998,13 @@ class LearnedErrorProfile {
         }
     }
 
-    // Runs the pilot into `est` and returns the learned budget, or 0 when the
-    // network cannot size this query (untrained plan, too few pilot successes).
+    // Runs the pilot into `est` and returns the learned budget (from the
+    // pilot's own CV for a plan the network has no slot for), or 0 when the
+    // pilot cannot size this query (too few successes).
     unsigned long long size(ZGraphInstance<EdgeData> &inst, double eps, double conf, Estimation &est)
     {
         if (!inst.plan()) throw Error(ZG_EINVAL, "LearnedErrorProfile: instance has no device plan");
         const int sl = slot(plan_kind(inst));
-        if (sl < 0) return 0;
         unsigned long long n = pilot_;
         for (;;) {
             inst.calculate(n, 0, est);
@@ -1073,16 +1092,21 @@ class LearnedErrorProfile {
 
   private:
     static int plan_kind(ZGraphInstance<EdgeData> &inst) { return inst.plan()->kind(); }
-    // the learned budget of a pattern slot from the pilot's statistics, or 0 when undefined
+    // the learned budget of a pattern slot from the pilot's statistics (slot
+    // -1: the pilot's CV alone, no network), or 0 when undefined
     unsigned long long budget(Graph<EdgeData> *g, int sl, unsigned long long n, const Estimation &est, double eps,
                               double conf)
     {
-        if (sl < 0) return 0;
         const double t = (double)est.get_num_finished_estimators(), mean = est.exact_mean();
         if (est.get_successes() < 32 || !(mean > 0)) return 0;
         const double var = std::max(est.get_sumsq() / t - mean * mean, 0.0);
         const double cv = std::sqrt(var) / mean, succ = (double)est.get_successes() / t;
         if (!(cv > 0) || !std::isfinite(cv)) return 0;
+        const double z = normal_quantile_two_sided(conf);
+        if (sl < 0) {
+            const double q = z * cv / eps;
+            return std::max((unsigned long long)std::ceil(q * q), n);
+        }
         float x[16];
         features(g, sl, (double)n, cv, succ, x);
         float y = 0;
@@ -1091,7 +1115,6 @@ class LearnedErrorProfile {
         // aim one validation RMSE of the correction below the predicted CV: a short run costs
         // one top-up of the id sequence, a long one its excess
         const double r = (use_[sl] ? (double)(y * y_scale_) : 0.0) - aim_sigmas_ * (double)val_rmse_[sl];
-        const double z = normal_quantile_two_sided(conf);
         const double q = z * cv * std::pow(10.0, r) / eps;
         return std::max((unsigned long long)std::ceil(q * q), n);
     }
@@ -1141,6 +1164,7 @@ using zgraph::ErrorProfile;
 using zgraph::Estimation;
 using zgraph::FourChainCountingInstance;
 using zgraph::FourCliqueCountingInstance;
+using zgraph::FourCliqueSplitCountingInstance;
 using zgraph::get_globally_consistent_seed;
 using zgraph::Graph;
 using zgraph::MyList;

@@ -1,8 +1,10 @@
 // A query sized by the learned error-latency profile and judged to
 // +-eps at 95 % confidence: argv[2] pattern (triangle, 4-clique, 3-chain,
-// 4-chain; or the two-channel 3-motif and triangle+4-clique, both channels to
-// +-eps over one id range), argv[3] eps (default 0.01), argv[4] estimator seed,
-// argv[5] the weights file (default: elp_weights.bin beside libzgraph_amd.so).
+// 4-chain, 4-clique-split -- the split 4-clique plan with 256 completions per
+// closed triangle, sized from its pilot; or the two-channel 3-motif and
+// triangle+4-clique, both channels to +-eps over one id range), argv[3] eps
+// (default 0.01), argv[4] estimator seed, argv[5] the weights file (default:
+// elp_weights.bin beside libzgraph_amd.so).
 #include <cstring>
 
 #include "AppCommon.hpp"
@@ -47,6 +49,7 @@ int main(int argc, char **argv)
         else if (pat == "4-clique") inst.reset(new FourCliqueCountingInstance<Empty>(&graph));
         else if (pat == "3-chain") inst.reset(new ThreeChainCountingInstance<Empty>(&graph));
         else if (pat == "4-chain") inst.reset(new FourChainCountingInstance<Empty>(&graph));
+        else if (pat == "4-clique-split") inst.reset(new FourCliqueSplitCountingInstance<Empty>(&graph, 256));
         else throw zgraph::Error(ZG_EINVAL, "unknown pattern " + pat);
         inst->seed = seed;
         LearnedErrorProfile<Empty> profile(weights);

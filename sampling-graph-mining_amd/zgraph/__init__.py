@@ -17,7 +17,7 @@ import time
 import numpy as np
 
 from . import _capi
-from ._capi import (PLAN_CHAIN, PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_PATTERN, PLAN_THREE_CHAIN,
+from ._capi import (PLAN_CHAIN, PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_K4_SPLIT, PLAN_PATTERN, PLAN_THREE_CHAIN,
                     PLAN_THREE_MOTIF, PLAN_TRIANGLE, PLAN_TRIANGLE_K4, ZGraphError, check, lib)
 from .formats import load_biedgelist, load_edgelist, load_graph_meta_data, save_biedgelist
 
@@ -25,7 +25,7 @@ __all__ = [
     "Graph", "DeviceEdges", "erdos_renyi", "chung_lu", "rmat", "Plan", "Pattern", "Estimation",
     "ZGraphInstance", "TriangleCoutingInstance", "ThreeChainCountingInstance", "ThreeMotifInstance",
     "TriangleFourCliqueInstance",
-    "ChainMiningInstance", "FourChainCountingInstance", "FourCliqueCountingInstance",
+    "ChainMiningInstance", "FourChainCountingInstance", "FourCliqueCountingInstance", "FourCliqueSplitCountingInstance",
     "SamplerGenerator", "AllPredicateZGraphInstance", "ErrorProfile", "TimeProfile",
     "load_graph_meta_data", "load_biedgelist", "load_edgelist", "save_biedgelist", "ZGraphError",
     "device_info", "device_memory", "clt_half_width", "run", "run_outcomes", "Batch", "random_gather_bandwidth",
@@ -250,10 +250,15 @@ FIVE_HOUSE_PATTERN = Pattern(5, [(0, 1), (0, 2), (1, 2), (1, 3), (2, 4), (3, 4)]
 
 
 class Plan:
-    def __init__(self, kind, order=0, pattern=None):
+    """A device plan (an estimate() body).  order: ZG_PLAN_CHAIN's path
+    vertices; pattern: ZG_PLAN_PATTERN's pattern; completions: ZG_PLAN_K4_SPLIT's
+    completions per closed triangle (a power of two in [1, 4096])."""
+
+    def __init__(self, kind, order=0, pattern=None, completions=0):
         d = _capi.PlanDesc()
         d.kind = kind
         d.order = order
+        d.completions = completions
         if pattern is not None:
             if pattern.num_edges > _capi.MAX_PATTERN_EDGES:
                 raise ValueError("pattern too large")
@@ -265,6 +270,7 @@ class Plan:
         self.kind = kind
         self.order = order
         self.pattern = pattern
+        self.completions = completions
         self.h = C.c_void_p()
         check(lib().zg_plan_create(C.byref(d), C.byref(self.h)))
         self.channels = lib().zg_plan_channels(self.h)
@@ -598,6 +604,15 @@ class FourCliqueCountingInstance(ZGraphInstance):
     """4-cliques by three spanning shapes + closing checks (unbiased; the
     reference's applications/FourClique.cpp:17-77 misses two of the shapes)."""
     plan_kind = PLAN_K4_CLOSE
+
+
+class FourCliqueSplitCountingInstance(ZGraphInstance):
+    """4-cliques, split: FourCliqueCountingInstance's first stage, then
+    `completions` independent draws of its three shapes per closed triangle,
+    averaged (same mean, lower variance per estimator)."""
+
+    def __init__(self, graph, completions=64, seed=1):
+        super().__init__(graph, Plan(PLAN_K4_SPLIT, completions=completions), seed)
 
 
 class SamplerGenerator(ZGraphInstance):

@@ -25,6 +25,7 @@ PLAN_FOUR_CHAIN = 5
 PLAN_PATTERN = 6
 PLAN_K4_CLOSE = 7
 PLAN_TRIANGLE_K4 = 8
+PLAN_K4_SPLIT = 9
 
 MAX_PATTERN_EDGES = 15
 
@@ -62,7 +63,8 @@ class PlanDesc(C.Structure):
     _fields_ = [("kind", C.c_int), ("order", C.c_int), ("pattern_vertices", C.c_int),
                 ("pattern_edges", C.c_int),
                 ("pattern_src", C.c_uint32 * MAX_PATTERN_EDGES),
-                ("pattern_dst", C.c_uint32 * MAX_PATTERN_EDGES)]
+                ("pattern_dst", C.c_uint32 * MAX_PATTERN_EDGES),
+                ("completions", C.c_int)]
 
 
 class Sums(C.Structure):

@@ -19,8 +19,9 @@ bound scaled by K^2.  The learned profile replaces that sizing:
      z * sd / (sqrt(t) * mean) <= eps) checks the exact running sums and tops
      the same id sequence up until it holds.
 
-The regression profile stays as the fallback: for plans the network was not
-trained on, for pilots with too few successes, or without a weights file.
+Plans the network was not trained on are sized from the pilot's own CV (r =
+0) and judged the same way.  The regression profile stays as the fallback: for
+pilots with too few successes, or without a weights file.
 Because estimator ids are global and every run continues the id sequence,
 the final answer equals one run of N_final estimators.
 """
@@ -164,10 +165,14 @@ class LearnedErrorProfile:
 
     # -- sizing --------------------------------------------------------------
     def _size(self, instance, eps, conf, est):
-        """Runs the pilot into `est`; returns (N, method)."""
+        """Runs the pilot into `est`; returns (N, method): "learned" (pilot +
+        network), "pilot" (the pilot's own CV: correction=False, or a plan the
+        network has no pattern slot for), or (None, "regression") when the
+        pilot cannot size the query or the weights are missing."""
         kind = instance.plan.kind
-        if self.correction and (self.model is None or kind not in PATTERN_SLOTS):
+        if self.correction and self.model is None:
             return None, "regression"
+        use_net = self.correction and kind in PATTERN_SLOTS
         n_p = self.pilot
         while True:
             instance.calculate(n_p, None, est)
@@ -178,14 +183,14 @@ class LearnedErrorProfile:
         mean, cv, succ = pilot_stats(s)
         if s.successes < MIN_PILOT_SUCCESSES or not math.isfinite(cv) or cv <= 0:
             return None, "regression"
-        if self.correction:
+        if use_net:
             x = feature_vector(graph_features(instance.graph), kind, n_p, cv, succ)
             r = float(self.model.predict(x)[0]) - self.aim_sigmas * self.model.val_rmse[PATTERN_SLOTS[kind]]
         else:
             r = 0.0
         cv_hat = cv * 10.0 ** r
         z = _z(conf)
-        return max(int(math.ceil((z * cv_hat / eps) ** 2)), n_p), "learned" if self.correction else "pilot"
+        return max(int(math.ceil((z * cv_hat / eps) ** 2)), n_p), "learned" if use_net else "pilot"
 
     def get_num_estimators(self, instance, num_vertices_in_pattern, required_error_rate, required_confidence):
         est = Estimation()

@@ -121,6 +121,7 @@ PLANS = [
     ("chain5", dict(kind=4, order=5), dict(kind=4, order=5)),
     ("four_chain", dict(kind=5), dict(kind=5)),
     ("k4_close", dict(kind=7), dict(kind=7)),
+    ("k4_split", dict(kind=9, completions=64), dict(kind=9, completions=64)),
     ("pattern_triangle", dict(kind=6, pattern="TRIANGLE"), dict(kind=6, pattern="TRIANGLE")),
     ("pattern_k4", dict(kind=6, pattern="K4"), dict(kind=6, pattern="K4")),
     ("pattern_5star", dict(kind=6, pattern="FIVE_STAR"), dict(kind=6, pattern="FIVE_STAR")),
@@ -128,11 +129,13 @@ PLANS = [
 ]
 
 
-def oracle_plan(O, kind, order=0, pattern=None):
-    return O.make_plan(kind, order=order, pattern=getattr(O, pattern) if pattern else None)
+def oracle_plan(O, kind, order=0, pattern=None, completions=0):
+    # the oracle's split 4-clique plan takes its completions in `order`
+    return O.make_plan(kind, order=completions if kind == O.PLAN_K4_SPLIT else order,
+                       pattern=getattr(O, pattern) if pattern else None)
 
 
-def device_plan(Z, kind, order=0, pattern=None):
+def device_plan(Z, kind, order=0, pattern=None, completions=0):
     pats = {"TRIANGLE": Z.TRIANGLE_PATTERN, "K4": Z.FOUR_CLIQUE_PATTERN, "FIVE_STAR": Z.FIVE_STAR_PATTERN,
             "FIVE_HOUSE": Z.FIVE_HOUSE_PATTERN}
-    return Z.Plan(kind, order=order, pattern=pats[pattern] if pattern else None)
+    return Z.Plan(kind, order=order, pattern=pats[pattern] if pattern else None, completions=completions)

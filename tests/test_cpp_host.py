@@ -146,9 +146,13 @@ def test_predicate_apps(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("pattern,kind", [("triangle", Z.PLAN_TRIANGLE), ("4-chain", Z.PLAN_FOUR_CHAIN)])
-def test_learned_profile_cpp_equals_python(tmp_path, pattern, kind):
-    """The C++ LearnedErrorProfile sizes and judges a query as zgraph.elp does."""
+@pytest.mark.parametrize("pattern,kind,completions,sizing", [("triangle", Z.PLAN_TRIANGLE, 0, "learned"),
+                                                             ("4-chain", Z.PLAN_FOUR_CHAIN, 0, "learned"),
+                                                             ("4-clique-split", Z.PLAN_K4_SPLIT, 256, "pilot")])
+def test_learned_profile_cpp_equals_python(tmp_path, pattern, kind, completions, sizing):
+    """The C++ LearnedErrorProfile sizes and judges a query as zgraph.elp does
+    (the split 4-clique plan, which the network has no slot for, from its
+    pilot's CV)."""
     from zgraph import elp as L
     n, s, d = golden_graph("wiki-vote")
     prefix = _write_graph(tmp_path, "wv", n, s, d)
@@ -162,9 +166,9 @@ def test_learned_profile_cpp_equals_python(tmp_path, pattern, kind):
     est, cnt = _report(r.stdout)[pattern]
     assert judged.group(1) == "1" and float(judged.group(2)) <= eps
     g = Z.Graph.from_edges(s, d, n, seed=1)
-    inst = Z.ZGraphInstance(g, Z.Plan(kind), seed=seed)
+    inst = Z.ZGraphInstance(g, Z.Plan(kind, completions=completions), seed=seed)
     rec = L.estimate_to_error(inst, eps, 0.95, L.LearnedErrorProfile())
-    assert rec["sizing"] == "learned"
+    assert rec["sizing"] == sizing
     assert abs(rec["sized_estimators"] - sized) <= 1
     if rec["estimators"] == cnt:
         assert est == pytest.approx(rec["estimate"], rel=1e-12)

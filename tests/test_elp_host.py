@@ -154,7 +154,12 @@ def test_learned_sizing_is_the_clt_formula(aim):
 def test_fallback_for_untrained_plans_and_missing_weights():
     prof = L.LearnedErrorProfile(weights=None, fallback=FixedProfile(777))
     assert prof.get_num_estimators(FakeInstance(), 3, 0.01, 0.95) == 777   # no weights
-    prof.model = FakeModel(0.0)
-    assert prof.get_num_estimators(FakeInstance(Z.PLAN_PATTERN), 3, 0.01, 0.95) == 777
-    rec = L.estimate_to_error(FakeInstance(Z.PLAN_PATTERN), 0.05, 0.95, prof)
-    assert rec["sizing"] == "regression" and rec["accepted"]
+    # a plan the network has no slot for: sized from its pilot's CV (r = 0), never the network
+    prof.model = FakeModel(0.7)
+    est = Z.Estimation()
+    n, how = prof._size(FakeInstance(Z.PLAN_PATTERN), 0.01, 0.95, est)
+    v = FakeInstance.values(0, 1 << 16).astype(np.float64)
+    cv = math.sqrt(max((v ** 2).mean() - v.mean() ** 2, 0)) / v.mean()
+    assert how == "pilot" and n == math.ceil((1.959963984540054 * cv / 0.01) ** 2)
+    rec = L.estimate_to_error(FakeInstance(Z.PLAN_K4_SPLIT), 0.05, 0.95, prof)
+    assert rec["sizing"] == "pilot" and rec["accepted"]

@@ -224,7 +224,7 @@ def test_exact_counters_collapse_parallel_edges():
 def _exact(g, name):
     if name in ("triangle", "pattern_triangle"):
         return g.triangles()
-    if name in ("k4_close", "pattern_k4"):
+    if name in ("k4_close", "k4_split", "pattern_k4"):
         return g.k4()
     if name == "three_chain":
         return g.wedges()
@@ -244,7 +244,7 @@ TINY = [H.complete_graph(5), H.random_graph(7, 0.6, 1), H.random_graph(7, 0.5, 2
 
 
 @pytest.mark.parametrize("gi", range(len(TINY)))
-@pytest.mark.parametrize("name,oargs", [(p[0], p[1]) for p in H.PLANS])
+@pytest.mark.parametrize("name,oargs", [(p[0], p[1]) for p in H.PLANS if p[0] != "k4_split"])
 def test_unbiased_by_exhaustive_expectation(gi, name, oargs):
     n, s, d = TINY[gi]
     for seed in (1, 2):
@@ -257,6 +257,56 @@ def test_unbiased_by_exhaustive_expectation(gi, name, oargs):
         e = O.expectation(g, plan, 0)
         assert e >= 0, "too many paths"
         assert e == pytest.approx(_exact(g, name), rel=1e-9, abs=1e-7)
+
+
+@pytest.mark.parametrize("graph,J", [(H.complete_graph(4), 1), (H.complete_graph(4), 2), (H.complete_graph(5), 2),
+                                     (H.random_graph(7, 0.6, 1), 2)] + [(t, 1) for t in TINY])
+def test_split_k4_unbiased_by_exhaustive_expectation(graph, J):
+    # every draw of the first stage and of all J completions enumerated: the
+    # mean of J completions over J is the 4-clique count for each rank order
+    n, s, d = graph
+    for seed in (1, 2):
+        g = O.OGraph(n, s, d, seed=seed)
+        e = O.expectation(g, O.make_plan(O.PLAN_K4_SPLIT, order=J), 0)
+        assert e >= 0, "too many paths"
+        assert e == pytest.approx(g.k4(), rel=1e-9, abs=1e-7)
+
+
+def test_split_k4_first_stage_is_the_k4_plans():
+    # same first edge per id; no completion unless the 4-clique plan's first
+    # stage closed its triangle (its ncond reaches 4: the three shapes' samples)
+    n, s, d = H.power_law_graph(400, 3000, 2.1, 5)
+    g = O.OGraph(n, s, d, seed=4)
+    k4 = O.run_outcomes(g, O.make_plan(O.PLAN_K4_CLOSE), 9, 100, 60000)
+    for J, nc in ((1, 4), (16, 49), (128, 255)):
+        sp = O.run_outcomes(g, O.make_plan(O.PLAN_K4_SPLIT, order=J), 9, 100, 60000)
+        assert (sp["e0"] == k4["e0"]).all()
+        closed = k4["ncond"] == 4
+        assert closed.sum() > 100
+        assert (sp["status"][~closed] == 0).all() and (sp["ncond"][~closed] == k4["ncond"][~closed]).all()
+        assert (sp["ncond"][closed] == nc).all()
+        ok = sp["status"] == 1
+        assert ok.any() and (sp["log2den"][ok] == int(math.log2(J))).all()
+
+
+def test_split_k4_variance_falls_with_completions():
+    # the same estimator ids with 1, 16 and 64 completions: the mean stays at
+    # the count and the per-estimator variance falls about J-fold until the
+    # first stage's share of it dominates (here between 16 and 64 completions:
+    # 22x and 26x below one completion); one shared completion stream per
+    # estimator, or an unscaled sum, would fail this
+    n, s, d = H.power_law_graph(2000, 30000, 2.1, 11)
+    g = O.OGraph(n, s, d, seed=1)
+    K4 = g.k4()
+    N = 200_000
+    var = {}
+    for J in (1, 16, 64):
+        o = O.run_outcomes(g, O.make_plan(O.PLAN_K4_SPLIT, order=J), 3, 0, N)
+        v = (o["num_lo"].astype(np.float64) + o["num_hi"].astype(np.float64) * 2.0 ** 64) / 2.0 ** o["log2den"]
+        v[o["status"] != 1] = 0
+        var[J] = v.var()
+        assert abs(v.mean() - K4) <= 4.5 * math.sqrt(var[J] / N), (J, v.mean(), K4)
+    assert var[16] < var[1] / 6 and var[64] < var[1] / 6
 
 
 def test_reference_fourclique_shape_alone_is_biased():
