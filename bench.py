@@ -49,6 +49,9 @@ def main():
     ap.add_argument("--elp-seeds-k4", type=int, default=3, help="independent 4-clique queries per profile")
     ap.add_argument("--split-completions", type=int, default=256,
                     help="completions per closed triangle of the split 4-clique plan")
+    ap.add_argument("--elp-patterns", default="",
+                    help="--workload elp: comma-separated pattern-name prefixes to run (default: all)")
+    ap.add_argument("--no-coverage", action="store_true", help="--workload elp: skip the 40-query coverage runs")
     ap.add_argument("--elp-cap-log2", type=int, default=39,
                     help="largest budget of one time-to-error query, log2 estimators (2^39 = 5.5e11: well "
                          "above the 1.2-1.5e11 the learned sizing asks for a +-1%% 4-clique answer on the bench "
@@ -486,8 +489,14 @@ def elp_sweep(Z, args):
               ("erdos-renyi 10M/100M", lambda: Z.erdos_renyi(10_000_000, 100_000_000, args.seed)),
               ("chunglu 2M/20M b2.5", lambda: Z.chung_lu(2_000_000, 20_000_000, 2.5, 5000.0, args.seed)),
               ("rmat26", lambda: Z.rmat(26, 16 << 26, 0.57, 0.19, 0.19, args.seed))]
+    J = args.split_completions
     patterns = (("triangle", Z.PLAN_TRIANGLE, "triangle"), ("4-clique", Z.PLAN_K4_CLOSE, "k4"),
-                ("3-chain", Z.PLAN_THREE_CHAIN, "wedge"), ("4-chain", Z.PLAN_FOUR_CHAIN, "path3"))
+                ("3-chain", Z.PLAN_THREE_CHAIN, "wedge"), ("4-chain", Z.PLAN_FOUR_CHAIN, "path3"),
+                (f"4-clique split x{J}", lambda: Z.Plan(Z.PLAN_K4_SPLIT, completions=J), "k4",
+                 ("regression+judge", "pilot+judge")))
+    if args.elp_patterns:
+        keep = [x.strip() for x in args.elp_patterns.split(",")]
+        patterns = tuple(p for p in patterns if any(p[0].startswith(k) for k in keep))
     rows = {}
     for gname, make in graphs:
         e = make()
@@ -496,8 +505,9 @@ def elp_sweep(Z, args):
         log(f"elp sweep {gname}: m={g.num_local_edges} maxdeg={g.max_degree}")
         rows[gname] = time_to_error(Z, g, args, patterns=patterns, cap_log2=36 if gname == "rmat26" else 34,
                                     queries=3)
-        if gname in ("chunglu 4.8M/69M b2.1", "rmat22"):
-            rows[gname]["coverage"] = elp_coverage(Z, g, args, patterns[0:1] + patterns[2:])
+        cov = [p for p in patterns if p[0] in ("triangle", "3-chain", "4-chain")]
+        if gname in ("chunglu 4.8M/69M b2.1", "rmat22") and cov and not args.no_coverage:
+            rows[gname]["coverage"] = elp_coverage(Z, g, args, cov)
         g.close()
     print(json.dumps({"elp_sweep": rows}), flush=True)
 
