@@ -128,12 +128,15 @@ enum {
                                  sample and closing check per id -- the cached sub-pattern
                                  sharing of ThreeMotif (src/ZGraphInstance.cpp:596-833) applied
                                  to triangle -> 4-clique; each channel equals its own plan's run */
-    ZG_PLAN_K4_SPLIT = 9      /* 4-clique, split: ZG_PLAN_K4_CLOSE's first stage (e0, e1, the
+    ZG_PLAN_K4_SPLIT = 9,     /* 4-clique, split: ZG_PLAN_K4_CLOSE's first stage (e0, e1, the
                                  triangle edge's closing check), then, when the triangle closes,
                                  `completions` independent draws of the three shapes averaged:
                                  the same mean with far less variance per estimator (no
                                  reference counterpart; each completion is FourClique.cpp's
                                  remaining conditional samples and closing checks) */
+    ZG_PLAN_TRIANGLE_K4_SPLIT = 10 /* fused: channel 0 = triangle (ZG_PLAN_TRIANGLE), channel 1 =
+                                 the split 4-clique (ZG_PLAN_K4_SPLIT, `completions`) from one
+                                 first stage per id; each channel equals its own plan's run */
 };
 
 #define ZG_MAX_PATTERN_EDGES 15
@@ -144,13 +147,14 @@ typedef struct {
     int pattern_edges;                       /* ZG_PLAN_PATTERN: E (1..15) */
     uint32_t pattern_src[ZG_MAX_PATTERN_EDGES];
     uint32_t pattern_dst[ZG_MAX_PATTERN_EDGES];
-    int completions;                         /* ZG_PLAN_K4_SPLIT: completions per closed triangle,
-                                                a power of two in [1, 4096] */
+    int completions;                         /* ZG_PLAN_K4_SPLIT, ZG_PLAN_TRIANGLE_K4_SPLIT: completions
+                                                per closed triangle, a power of two in [1, 4096] */
 } zg_plan_desc;
 
 int zg_plan_create(const zg_plan_desc *desc, zg_plan **out);
 int zg_plan_destroy(zg_plan *p);
-/* outputs per estimator (2 for ZG_PLAN_THREE_MOTIF and ZG_PLAN_TRIANGLE_K4, else 1) */
+/* outputs per estimator (2 for ZG_PLAN_THREE_MOTIF, ZG_PLAN_TRIANGLE_K4 and
+ * ZG_PLAN_TRIANGLE_K4_SPLIT, else 1) */
 int zg_plan_channels(const zg_plan *p);
 /* conditional_sample_edge calls per successful estimate
  * (get_num_conditional_sample_edge_calls, src/ZGraphInstance.cpp:470-488;

@@ -655,6 +655,8 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
 __device__ __forceinline__ uint32_t lane_u32(uint32_t x, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)x, l); }
 __device__ __forceinline__ VI lane_vi(const VI &x, int l) { return VI{lane_u32(x.v, l), lane_u32(x.deg, l), lane_u32(x.off, l)}; }
 
+// TRI: channel 0 = the triangle plan's outcome (k4_first<true>), channel 1 the split 4-clique's.
+template <bool TRI>
 __device__ __forceinline__ void k4_split(const GView &g, const PlanView &P, uint64_t seed, uint64_t id, bool active,
                                          Est &o, uint32_t &words)
 {
@@ -666,10 +668,11 @@ __device__ __forceinline__ void k4_split(const GView &g, const PlanView &P, uint
     Rng rng;
     rng.init(seed, TAG_ESTIMATOR, id);
     K4S k;
-    const bool closed = active && k4_first<false>(g, rng, o, k);
+    const bool closed = active && k4_first<TRI>(g, rng, o, k);
     words = rng.word;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t J = 1u << P.K;
+    const int lg = TRI ? P.K1 : P.K;
+    const uint32_t J = 1u << lg;
     for (uint64_t pend = __ballot(closed); pend; pend &= pend - 1) {
         const int l = __builtin_ctzll(pend);
         K4S t;
@@ -716,7 +719,7 @@ __device__ __forceinline__ void k4_split(const GView &g, const PlanView &P, uint
             tov |= add_u128(tot, U128{0, d3 << 32});
             tov |= (d3 >> 32) != 0;
             o.ncond = 1 + 3 * J;
-            if (tot.lo | tot.hi) set_out(o, 0, tot, tov, P.K);
+            if (tot.lo | tot.hi) set_out(o, TRI ? 1 : 0, tot, tov, lg);
         }
     }
 }
@@ -879,14 +882,14 @@ __global__ __launch_bounds__(256) void k_run(GView g, PlanView P, uint64_t seed,
             }
         }
     };
-    if constexpr (KIND == ZG_PLAN_K4_SPLIT) {
+    if constexpr (KIND == ZG_PLAN_K4_SPLIT || KIND == ZG_PLAN_TRIANGLE_K4_SPLIT) {
         // wavefront-uniform trip count: every lane reaches the cooperative completions
         const uint32_t lane = threadIdx.x & 63;
         for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < count; b += stride) {
             const uint64_t i = b + lane;
             Est o;
             uint32_t words;
-            k4_split(g, P, seed, first + i, i < count, o, words);
+            k4_split<KIND == ZG_PLAN_TRIANGLE_K4_SPLIT>(g, P, seed, first + i, i < count, o, words);
             if (i < count) record(i, o, words);
         }
     } else {
@@ -998,13 +1001,16 @@ __device__ __attribute__((noinline)) void batch_chunk(const BQuery *__restrict__
     rec = uniptr(rec);
     AccL acc[NCH];
     acc_zero<NCH>(acc);
-    if constexpr (KIND == ZG_PLAN_K4_SPLIT) {
+    if constexpr (KIND == ZG_PLAN_K4_SPLIT || KIND == ZG_PLAN_TRIANGLE_K4_SPLIT) {
         for (uint64_t b = base; b < end; b += 64) {   // uniform: every lane reaches the completions
             const uint64_t i = b + lane;
             Est o;
             uint32_t words;
-            k4_split(g, P, seed, first + i, i < end, o, words);
-            if (i < end) acc_add(acc[0], o.v[0], o.st[0], o.den[0], P.K);
+            k4_split<KIND == ZG_PLAN_TRIANGLE_K4_SPLIT>(g, P, seed, first + i, i < end, o, words);
+            if (i < end) {
+#pragma unroll
+                for (int ch = 0; ch < NCH; ++ch) acc_add(acc[ch], o.v[ch], o.st[ch], o.den[ch], ch == 0 ? P.K : P.K1);
+            }
         }
     } else {
         for (uint64_t i = base + lane; i < end; i += 64) {
@@ -1067,6 +1073,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         case ZG_PLAN_K4_CLOSE: batch_chunk<ZG_PLAN_K4_CLOSE, 1>(&Q, base, end, rec); break;
         case ZG_PLAN_TRIANGLE_K4: batch_chunk<ZG_PLAN_TRIANGLE_K4, 2>(&Q, base, end, rec); break;
         case ZG_PLAN_K4_SPLIT: batch_chunk<ZG_PLAN_K4_SPLIT, 1>(&Q, base, end, rec); break;
+        case ZG_PLAN_TRIANGLE_K4_SPLIT: batch_chunk<ZG_PLAN_TRIANGLE_K4_SPLIT, 2>(&Q, base, end, rec); break;
         case KIND_PATTERN_BASE + 2: batch_chunk<KIND_PATTERN_BASE + 2, 1>(&Q, base, end, rec); break;
         case KIND_PATTERN_BASE + 3: batch_chunk<KIND_PATTERN_BASE + 3, 1>(&Q, base, end, rec); break;
         case KIND_PATTERN_BASE + 4: batch_chunk<KIND_PATTERN_BASE + 4, 1>(&Q, base, end, rec); break;
@@ -1159,7 +1166,7 @@ static void launch_kind(const zg_graph *g, const PlanView &P, uint64_t seed, uin
 {
     GView v = g->view();
     v.reads = reads;
-    if constexpr (KIND != ZG_PLAN_K4_SPLIT) {   // the split plan runs in id order only
+    if constexpr (KIND != ZG_PLAN_K4_SPLIT && KIND != ZG_PLAN_TRIANGLE_K4_SPLIT) {   // split plans: id order only
         if (order) {
             hipLaunchKernelGGL((k_run_ordered<KIND, NCH>), dim3(grid), dim3(block), 0, g->stream, v, P, seed, first,
                                count, order, partials);
@@ -1260,7 +1267,8 @@ int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
     }
     const PlanView P = p->view();
     const uint32_t *ord = nullptr;
-    if (order >= 2 && !d_out && !reads && count > 0 && g->m > 0 && p->desc.kind != ZG_PLAN_K4_SPLIT) {
+    if (order >= 2 && !d_out && !reads && count > 0 && g->m > 0 && p->desc.kind != ZG_PLAN_K4_SPLIT &&
+        p->desc.kind != ZG_PLAN_TRIANGLE_K4_SPLIT) {
         int rc = order_by_first_edge(g, p, seed, first, count, order, &ord);
         if (rc) return rc;
     }
@@ -1283,6 +1291,7 @@ int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
     case ZG_PLAN_K4_CLOSE: launch_kind<ZG_PLAN_K4_CLOSE, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
     case ZG_PLAN_TRIANGLE_K4: launch_kind<ZG_PLAN_TRIANGLE_K4, 2>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
     case ZG_PLAN_K4_SPLIT: launch_kind<ZG_PLAN_K4_SPLIT, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
+    case ZG_PLAN_TRIANGLE_K4_SPLIT: launch_kind<ZG_PLAN_TRIANGLE_K4_SPLIT, 2>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
     default: return set_error(ZG_EINVAL, "unknown plan kind");
     }
     ZG_HIP(hipGetLastError());

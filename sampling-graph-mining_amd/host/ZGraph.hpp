@@ -555,10 +555,11 @@ class Plan {
         d.order = order;
         return d;
     }
-    // ZG_PLAN_K4_SPLIT with `completions` per closed triangle
-    static zg_plan_desc split_k4(int completions)
+    // ZG_PLAN_K4_SPLIT (or, fused with the triangle channel, ZG_PLAN_TRIANGLE_K4_SPLIT)
+    // with `completions` per closed triangle
+    static zg_plan_desc split_k4(int completions, int kind = ZG_PLAN_K4_SPLIT)
     {
-        zg_plan_desc d = of(ZG_PLAN_K4_SPLIT);
+        zg_plan_desc d = of(kind);
         d.completions = completions;
         return d;
     }
@@ -704,6 +705,13 @@ class TriangleFourCliqueInstance : public ZGraphInstance<EdgeData> {
     explicit TriangleFourCliqueInstance(Graph<EdgeData> *g, int channel_ = 0) : ZGraphInstance<EdgeData>(g)
     {
         this->set_plan(Plan::of(ZG_PLAN_TRIANGLE_K4));
+        this->channel = channel_;
+    }
+    // channel 1 by the split 4-clique plan with `completions` per closed triangle
+    // (ZG_PLAN_TRIANGLE_K4_SPLIT)
+    TriangleFourCliqueInstance(Graph<EdgeData> *g, int channel_, int completions) : ZGraphInstance<EdgeData>(g)
+    {
+        this->set_plan(Plan::split_k4(completions, ZG_PLAN_TRIANGLE_K4_SPLIT));
         this->channel = channel_;
     }
     // both counts from one pass over ids [0, N)
@@ -1026,6 +1034,7 @@ class LearnedErrorProfile {
         switch (plan_kind(inst)) {
         case ZG_PLAN_THREE_MOTIF: k0 = ZG_PLAN_THREE_CHAIN; k1 = ZG_PLAN_TRIANGLE; break;
         case ZG_PLAN_TRIANGLE_K4: k0 = ZG_PLAN_TRIANGLE; k1 = ZG_PLAN_K4_CLOSE; break;
+        case ZG_PLAN_TRIANGLE_K4_SPLIT: k0 = ZG_PLAN_TRIANGLE; k1 = ZG_PLAN_K4_SPLIT; break;
         default: throw Error(ZG_EINVAL, "LearnedErrorProfile::run_both: not a two-channel plan");
         }
         Graph<EdgeData> *g = inst.get_run_graph();

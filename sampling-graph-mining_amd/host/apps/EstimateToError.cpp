@@ -1,8 +1,9 @@
 // A query sized by the learned error-latency profile and judged to
 // +-eps at 95 % confidence: argv[2] pattern (triangle, 4-clique, 3-chain,
 // 4-chain, 4-clique-split -- the split 4-clique plan with 256 completions per
-// closed triangle, sized from its pilot; or the two-channel 3-motif and
-// triangle+4-clique, both channels to +-eps over one id range), argv[3] eps
+// closed triangle, sized from its pilot; or the two-channel 3-motif,
+// triangle+4-clique and triangle+4-clique-split, both channels to +-eps over
+// one id range), argv[3] eps
 // (default 0.01), argv[4] estimator seed, argv[5] the weights file (default:
 // elp_weights.bin beside libzgraph_amd.so).
 #include <cstring>
@@ -26,10 +27,11 @@ int main(int argc, char **argv)
             const std::string self = argv[0];
             weights = self.substr(0, self.rfind('/') + 1) + "../../zgraph/elp_weights.bin";
         }
-        if (pat == "3-motif" || pat == "triangle+4-clique") {
+        if (pat == "3-motif" || pat == "triangle+4-clique" || pat == "triangle+4-clique-split") {
             std::unique_ptr<ZGraphInstance<Empty>> both;
             if (pat == "3-motif") both.reset(new ThreeMotifInstance<Empty>(&graph));
-            else both.reset(new TriangleFourCliqueInstance<Empty>(&graph));
+            else if (pat == "triangle+4-clique") both.reset(new TriangleFourCliqueInstance<Empty>(&graph));
+            else both.reset(new TriangleFourCliqueInstance<Empty>(&graph, 0, 256));
             both->seed = seed;
             LearnedErrorProfile<Empty> profile(weights);
             Estimation e0, e1;

@@ -18,13 +18,14 @@ import numpy as np
 
 from . import _capi
 from ._capi import (PLAN_CHAIN, PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_K4_SPLIT, PLAN_PATTERN, PLAN_THREE_CHAIN,
-                    PLAN_THREE_MOTIF, PLAN_TRIANGLE, PLAN_TRIANGLE_K4, ZGraphError, check, lib)
+                    PLAN_THREE_MOTIF, PLAN_TRIANGLE, PLAN_TRIANGLE_K4, PLAN_TRIANGLE_K4_SPLIT, ZGraphError, check,
+                    lib)
 from .formats import load_biedgelist, load_edgelist, load_graph_meta_data, save_biedgelist
 
 __all__ = [
     "Graph", "DeviceEdges", "erdos_renyi", "chung_lu", "rmat", "Plan", "Pattern", "Estimation",
     "ZGraphInstance", "TriangleCoutingInstance", "ThreeChainCountingInstance", "ThreeMotifInstance",
-    "TriangleFourCliqueInstance",
+    "TriangleFourCliqueInstance", "TriangleFourCliqueSplitInstance",
     "ChainMiningInstance", "FourChainCountingInstance", "FourCliqueCountingInstance", "FourCliqueSplitCountingInstance",
     "SamplerGenerator", "AllPredicateZGraphInstance", "ErrorProfile", "TimeProfile",
     "load_graph_meta_data", "load_biedgelist", "load_edgelist", "save_biedgelist", "ZGraphError",
@@ -585,6 +586,15 @@ class TriangleFourCliqueInstance(_TwoChannelInstance):
     (src/ZGraphInstance.cpp:596-833) applied to triangle -> 4-clique.  Each
     channel equals its own plan's run on the same ids."""
     plan_kind = PLAN_TRIANGLE_K4
+
+
+class TriangleFourCliqueSplitInstance(_TwoChannelInstance):
+    """Triangles (channel 0) and 4-cliques by the split plan (channel 1,
+    FourCliqueSplitCountingInstance's estimator with `completions`) from one
+    first stage per id; each channel equals its own plan's run on the same ids."""
+
+    def __init__(self, graph, completions=256, seed=1):
+        super().__init__(graph, Plan(PLAN_TRIANGLE_K4_SPLIT, completions=completions), seed)
 
 
 class ChainMiningInstance(ZGraphInstance):

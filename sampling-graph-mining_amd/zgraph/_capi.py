@@ -26,6 +26,7 @@ PLAN_PATTERN = 6
 PLAN_K4_CLOSE = 7
 PLAN_TRIANGLE_K4 = 8
 PLAN_K4_SPLIT = 9
+PLAN_TRIANGLE_K4_SPLIT = 10
 
 MAX_PATTERN_EDGES = 15
 

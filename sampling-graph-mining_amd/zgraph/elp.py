@@ -32,8 +32,8 @@ import time
 
 import numpy as np
 
-from . import (PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_THREE_CHAIN, PLAN_THREE_MOTIF, PLAN_TRIANGLE,
-               PLAN_TRIANGLE_K4, ErrorProfile, Estimation, ZGraphError, _z, run)
+from . import (PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_K4_SPLIT, PLAN_THREE_CHAIN, PLAN_THREE_MOTIF, PLAN_TRIANGLE,
+               PLAN_TRIANGLE_K4, PLAN_TRIANGLE_K4_SPLIT, ErrorProfile, Estimation, ZGraphError, _z, run)
 
 IN_DIM = 16
 PATTERN_SLOTS = {PLAN_TRIANGLE: 0, PLAN_K4_CLOSE: 1, PLAN_THREE_CHAIN: 2, PLAN_FOUR_CHAIN: 3}
@@ -41,7 +41,8 @@ DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "elp_
 MIN_PILOT_SUCCESSES = 32
 # the single plan whose estimator each channel of a two-channel plan carries
 CHANNEL_KINDS = {PLAN_THREE_MOTIF: (PLAN_THREE_CHAIN, PLAN_TRIANGLE),
-                 PLAN_TRIANGLE_K4: (PLAN_TRIANGLE, PLAN_K4_CLOSE)}
+                 PLAN_TRIANGLE_K4: (PLAN_TRIANGLE, PLAN_K4_CLOSE),
+                 PLAN_TRIANGLE_K4_SPLIT: (PLAN_TRIANGLE, PLAN_K4_SPLIT)}
 MAX_PILOT = 1 << 28
 
 
@@ -278,10 +279,12 @@ def estimate_to_error(instance, eps, conf, profile, init_estimation=None, num_ve
 
 def estimate_both_to_error(instance, eps, conf, profile, max_estimators=1 << 40, max_rounds=8,
                            fallback_budget=1 << 24):
-    """Both channels of a two-channel plan (3-motif; fused triangle + 4-clique)
-    to +-eps at `conf`: one pilot gives both channels' statistics, each channel
-    is sized by the network with its own pattern slot (judge-only start where it
-    cannot be), the query runs to the larger size over one id range, and the
+    """Both channels of a two-channel plan (3-motif; fused triangle + 4-clique,
+    with the three-shape or the split 4-clique plan) to +-eps at `conf`: one
+    pilot gives both channels' statistics, each channel is sized by the network
+    with its own pattern slot (from its pilot's CV for a channel the network
+    has no slot for; judge-only start where the pilot holds too few successes),
+    the query runs to the larger size over one id range, and the
     judge tops up until both channels' intervals hold.  Returns one record per
     channel (estimate, estimators, half-width, accepted) and the shared times."""
     kinds = CHANNEL_KINDS.get(instance.plan.kind)
@@ -314,10 +317,14 @@ def estimate_both_to_error(instance, eps, conf, profile, max_estimators=1 << 40,
         mean, cv, succ = pilot_stats(ests[c].sums_view())
         if (profile.model is not None and ests[c].successes >= MIN_PILOT_SUCCESSES and math.isfinite(cv)
                 and cv > 0):
-            r = float(profile.model.predict(feature_vector(gf, kinds[c], n_p, cv, succ))[0]) - \
-                profile.aim_sigmas * profile.model.val_rmse[PATTERN_SLOTS[kinds[c]]]
+            if kinds[c] in PATTERN_SLOTS:
+                r = float(profile.model.predict(feature_vector(gf, kinds[c], n_p, cv, succ))[0]) - \
+                    profile.aim_sigmas * profile.model.val_rmse[PATTERN_SLOTS[kinds[c]]]
+                how.append("learned")
+            else:   # no network slot: the pilot's own CV
+                r = 0.0
+                how.append("pilot")
             sizes.append(max(int(math.ceil((z * cv * 10.0 ** r / eps) ** 2)), n_p))
-            how.append("learned")
         else:
             sizes.append(max(fallback_budget, n_p))
             how.append("judge-only")

@@ -176,6 +176,8 @@ def test_learned_profile_cpp_equals_python(tmp_path, pattern, kind, completions,
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("pattern,cls,names", [("triangle+4-clique", "TriangleFourCliqueInstance", ("triangle", "4-clique")),
+                                               ("triangle+4-clique-split", "TriangleFourCliqueSplitInstance",
+                                                ("triangle", "4-clique")),
                                                ("3-motif", "ThreeMotifInstance", ("3-chain", "triangle"))])
 def test_learned_two_channel_cpp_equals_python(tmp_path, pattern, cls, names):
     """The C++ LearnedErrorProfile::run_both sizes, runs and judges a

@@ -35,7 +35,8 @@ def plans():
             "star": Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_STAR_PATTERN),
             "house": Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_HOUSE_PATTERN),
             "w3": Z.Plan(Z.PLAN_THREE_CHAIN), "tk4": Z.Plan(Z.PLAN_TRIANGLE_K4),
-            "k4s": Z.Plan(Z.PLAN_K4_SPLIT, completions=128)}
+            "k4s": Z.Plan(Z.PLAN_K4_SPLIT, completions=128),
+            "tk4s": Z.Plan(Z.PLAN_TRIANGLE_K4_SPLIT, completions=32)}
 
 
 def _queries(graphs, plans):

@@ -71,7 +71,8 @@ def test_sizing_failure_falls_back_to_the_judge():
 
 
 @pytest.mark.parametrize("inst_cls,whats", [(Z.ThreeMotifInstance, ("wedge", "triangle")),
-                                            (Z.TriangleFourCliqueInstance, ("triangle", "k4"))])
+                                            (Z.TriangleFourCliqueInstance, ("triangle", "k4")),
+                                            (Z.TriangleFourCliqueSplitInstance, ("triangle", "k4"))])
 def test_two_channel_query_reaches_the_error_on_both(inst_cls, whats):
     # one id range serves both channels; each channel's answer equals its own plan over those ids
     n, s, d = H.golden_graph("wiki-vote")
@@ -87,5 +88,6 @@ def test_two_channel_query_reaches_the_error_on_both(inst_cls, whats):
     for c, (chan, truth) in enumerate(zip(r["channels"], truths)):
         assert chan["accepted"] and chan["rel_half_width"] <= eps
         assert abs(chan["estimate"] - truth) <= 2 * chan["half_width"]
-        single = Z.run(g, Z.Plan(kinds[c]), 41, 0, r["estimators"])[0]
+        single = Z.run(g, Z.Plan(kinds[c], completions=inst.plan.completions if c else 0), 41, 0,
+                       r["estimators"])[0]
         assert chan["estimate"] == pytest.approx(single.mean, rel=1e-12)

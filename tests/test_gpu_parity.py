@@ -178,19 +178,23 @@ def test_outcomes_bit_exact(gname, name, oargs):
 
 
 @pytest.mark.parametrize("gname", ["wiki-vote", "powerlaw", "gnp-60", "K7", "multi"])
-def test_fused_triangle_k4_channels_equal_the_single_plans(gname):
-    # channel 0 is the triangle plan's outcome and channel 1 the 4-clique plan's, id for id
+@pytest.mark.parametrize("fused,completions", [(Z.PLAN_TRIANGLE_K4, 0), (Z.PLAN_TRIANGLE_K4_SPLIT, 64)])
+def test_fused_triangle_k4_channels_equal_the_single_plans(gname, fused, completions):
+    # channel 0 is the triangle plan's outcome and channel 1 the 4-clique plan's
+    # (three-shape or split), id for id
     n, s, d = GRAPHS[gname]()
     zg, og = both(n, s, d, seed=3)
     count = 40000 if gname in ("wiki-vote", "powerlaw") else 20000
-    say(f"[parity] fused triangle + 4-clique on {gname}, {count} ids")
-    zo, zs = Z.run_outcomes(zg, Z.Plan(Z.PLAN_TRIANGLE_K4), 17, 123, count)
+    say(f"[parity] fused triangle + 4-clique (plan {fused}) on {gname}, {count} ids")
+    zo, zs = Z.run_outcomes(zg, Z.Plan(fused, completions=completions), 17, 123, count)
     zo = zo.reshape(count, 2)
-    for ch, kind in ((0, Z.PLAN_TRIANGLE), (1, Z.PLAN_K4_CLOSE)):
-        oo = O.run_outcomes(og, H.oracle_plan(O, kind=kind), 17, 123, count).reshape(count, -1)[:, 0]
+    k4 = Z.PLAN_K4_CLOSE if fused == Z.PLAN_TRIANGLE_K4 else Z.PLAN_K4_SPLIT
+    for ch, kind in ((0, Z.PLAN_TRIANGLE), (1, k4)):
+        c = completions if kind == Z.PLAN_K4_SPLIT else 0
+        oo = O.run_outcomes(og, H.oracle_plan(O, kind=kind, completions=c), 17, 123, count).reshape(count, -1)[:, 0]
         for f in ("status", "num_lo", "num_hi", "log2den", "e0"):
             assert (zo[:, ch][f] == oo[f]).all(), f"channel {ch} differs from the oracle in {f}"
-        single = Z.run(zg, Z.Plan(kind), 17, 123, count)[0]
+        single = Z.run(zg, Z.Plan(kind, completions=c), 17, 123, count)[0]
         assert zs[ch].total() == single.total() and zs[ch].successes == single.successes
         assert zs[ch].sumsq == pytest.approx(single.sumsq, rel=1e-12)
     if gname == "K7":
