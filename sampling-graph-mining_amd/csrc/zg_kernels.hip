@@ -418,31 +418,64 @@ __device__ __forceinline__ bool k4_first(const GView &g, Rng &rng, Est &o, K4S &
     return true;
 }
 
-// The three shapes after a closed first stage, one conditional sample each
-// from rng (ncond counts them); returns the sum of their values m s1 s2, and
-// sets ov when one of them passed 128 bits.
-__device__ __forceinline__ U128 k4_shapes(const GView &g, Rng &rng, const K4S &k, uint32_t &ncond, bool &ov)
-{
-    const VI A = k.A, B = k.B, W = k.W;
-    const uint32_t a = A.v, b = B.v, w = W.v;
-    const bool oa = k.oa != 0;
-    const uint32_t ob = oa ? b : a;
-    const uint32_t r0 = k.r0, r1 = k.r1, rt = k.rt, pa0 = k.pa0, pb0 = k.pb0;
-    const uint32_t po1 = k.po1, pw1 = k.pw1;
-    const VI O = oa ? A : B, OB = oa ? B : A;
-    const uint32_t pob0 = oa ? pb0 : pa0;        // lower_bound(r0 + 1) in ob's list
-    U128 base{g.m, 0};
-    const bool base_ov = mul_u128(base, k.s1);
-    U128 total{0, 0};
+// What every completion of a closed first stage shares: the three shapes'
+// starting positions in the lists of a, b, w (shape A: ranks above r1; B:
+// above the triangle edge's rank rt; C: above r0), found once per estimator
+// by the searches of the shapes (rank r1 in ob's list, rt in o's, r0 in w's)
+// and the triangle edge's record.
+struct K4Q {
+    VI A, B, W;
+    uint32_t r0, r1, oa;
+    uint32_t pA[3], pB[3], pC[3];
+    uint64_t s1;
+};
 
+__device__ __forceinline__ void k4_prepare(const GView &g, const K4S &k, K4Q &q)
+{
+    const bool oa = k.oa != 0;
+    const uint32_t ob = oa ? k.B.v : k.A.v, w = k.W.v;
+    const VI O = oa ? k.A : k.B, OB = oa ? k.B : k.A;
+    const uint32_t pob0 = oa ? k.pb0 : k.pa0;   // lower_bound(r0 + 1) in ob's list
+    q.A = k.A;
+    q.B = k.B;
+    q.W = k.W;
+    q.r0 = k.r0;
+    q.r1 = k.r1;
+    q.oa = k.oa;
+    q.s1 = k.s1;
+    const uint32_t pob = lb_rank(g, OB, pob0, OB.deg, k.r1 + 1, (uint64_t)k.r0 + 1, g.m);
+    q.pA[0] = oa ? k.po1 + 1 : pob;
+    q.pA[1] = oa ? pob : k.po1 + 1;
+    q.pA[2] = k.pw1 + 1;
+    const ERec ET = load_edge(g, k.rt);
+    const uint32_t pob_t = (ET.src == ob ? ET.s_pos : ET.d_pos) + 1;
+    const uint32_t pw_t = (ET.src == w ? ET.s_pos : ET.d_pos) + 1;
+    const uint32_t po = lb_rank(g, O, k.po1 + 1, O.deg, k.rt + 1, (uint64_t)k.r1 + 1, g.m);
+    q.pB[0] = oa ? po : pob_t;
+    q.pB[1] = oa ? pob_t : po;
+    q.pB[2] = pw_t;
+    q.pC[0] = k.pa0;
+    q.pC[1] = k.pb0;
+    q.pC[2] = lb_rank(g, k.W, 0, k.pw1, k.r0 + 1, 0, k.r1);
+}
+
+// One completion: the three shapes from their prepared positions, one
+// conditional sample each from rng, and their closing checks; returns the sum
+// of their values m s1 s2, and sets ov when one of them passed 128 bits.
+__device__ __forceinline__ U128 k4_complete(const GView &g, Rng &rng, const K4Q &q, bool &ov)
+{
+    const VI A = q.A, B = q.B, W = q.W;
+    const uint32_t a = A.v, b = B.v, w = W.v;
+    const uint32_t ob = q.oa ? b : a;
+    U128 base{g.m, 0};
+    const bool base_ov = mul_u128(base, q.s1);
+    U128 total{0, 0};
+    VI V[3] = {A, B, W};
     // shape A: e2 above r1 from {a, b, w} to a new vertex d; close the rest above r2
     {
-        const uint32_t pob = lb_rank(g, OB, pob0, OB.deg, r1 + 1, (uint64_t)r0 + 1, g.m);
-        VI V[3] = {A, B, W};
-        uint32_t pos[3] = {oa ? po1 + 1 : pob, oa ? pob : po1 + 1, pw1 + 1};
+        uint32_t pos[3] = {q.pA[0], q.pA[1], q.pA[2]};
         uint64_t s2;
         SE e2;
-        ncond++;
         if (cond_sample<3>(g, rng, V, pos, 3, s2, e2)) {
             const uint32_t d = e2.dst;
             if (d != a && d != b && d != w) {
@@ -462,46 +495,35 @@ __device__ __forceinline__ U128 k4_shapes(const GView &g, Rng &rng, const K4S &k
     }
     // shape B: triangle {e0, e1, t}; e2 above rank(t) to a new vertex d; close the rest above r2
     {
-        {
-            const ERec ET = load_edge(g, rt);
-            const uint32_t pob_t = (ET.src == ob ? ET.s_pos : ET.d_pos) + 1;
-            const uint32_t pw_t = (ET.src == w ? ET.s_pos : ET.d_pos) + 1;
-            const uint32_t po = lb_rank(g, O, po1 + 1, O.deg, rt + 1, (uint64_t)r1 + 1, g.m);
-            VI V[3] = {A, B, W};
-            uint32_t pos[3] = {oa ? po : pob_t, oa ? pob_t : po, pw_t};
-            uint64_t s3;
-            SE e3;
-            ncond++;
-            if (cond_sample<3>(g, rng, V, pos, 3, s3, e3)) {
-                const uint32_t d = e3.dst;
-                if (d != a && d != b && d != w) {
-                    const uint32_t mv = e3.rank + 1;
-                    bool ok = true;
-                    if (e3.src != a) ok = close_pair(g, a, d, mv, nullptr);
-                    if (ok && e3.src != b) ok = close_pair(g, b, d, mv, nullptr);
-                    if (ok && e3.src != w) ok = close_pair(g, w, d, mv, nullptr);
-                    if (ok) {
-                        U128 v = base;
-                        const bool o3 = mul_u128(v, s3);
-                        add_u128(total, v);
-                        ov |= base_ov | o3;
-                    }
+        uint32_t pos[3] = {q.pB[0], q.pB[1], q.pB[2]};
+        uint64_t s3;
+        SE e3;
+        if (cond_sample<3>(g, rng, V, pos, 3, s3, e3)) {
+            const uint32_t d = e3.dst;
+            if (d != a && d != b && d != w) {
+                const uint32_t mv = e3.rank + 1;
+                bool ok = true;
+                if (e3.src != a) ok = close_pair(g, a, d, mv, nullptr);
+                if (ok && e3.src != b) ok = close_pair(g, b, d, mv, nullptr);
+                if (ok && e3.src != w) ok = close_pair(g, w, d, mv, nullptr);
+                if (ok) {
+                    U128 v = base;
+                    const bool o3 = mul_u128(v, s3);
+                    add_u128(total, v);
+                    ov |= base_ov | o3;
                 }
             }
         }
     }
     // shape C: e2 above r0 from w to a new vertex d, below r1; close the rest above r1
     {
-        const uint32_t pw0 = lb_rank(g, W, 0, pw1, r0 + 1, 0, r1);
-        VI V[3] = {A, B, W};
-        uint32_t pos[3] = {pa0, pb0, pw0};
+        uint32_t pos[3] = {q.pC[0], q.pC[1], q.pC[2]};
         uint64_t s4;
         SE e4;
-        ncond++;
         if (cond_sample<3>(g, rng, V, pos, 3, s4, e4)) {
             const uint32_t d = e4.dst;
-            if (e4.src == w && d != a && d != b && d != w && e4.rank < r1) {
-                const uint32_t mv = r1 + 1;
+            if (e4.src == w && d != a && d != b && d != w && e4.rank < q.r1) {
+                const uint32_t mv = q.r1 + 1;
                 if (close_pair(g, ob, w, mv, nullptr) && close_pair(g, a, d, mv, nullptr) &&
                     close_pair(g, b, d, mv, nullptr)) {
                     U128 v = base;
@@ -516,10 +538,10 @@ __device__ __forceinline__ U128 k4_shapes(const GView &g, Rng &rng, const K4S &k
 }
 
 // The 4-clique plan (TRI: and the triangle channel) as one body: the steps of
-// k4_first then k4_shapes on the estimator's own stream.  Written out rather
-// than composed from the two: the composed form compiled to 85 VGPRs (5
-// waves/SIMD) and ran 4 % slower on the 69 M-edge graph (16.9 against 16.3 ms
-// per 2^28 estimators, alternating runs of both libraries).
+// k4_first, k4_prepare and k4_complete on the estimator's own stream.  Written
+// out rather than composed from them: the composed form compiled to 85 VGPRs
+// (5 waves/SIMD) and ran 4 % slower on the 69 M-edge graph (16.9 against 16.3
+// ms per 2^28 estimators, alternating runs of both libraries).
 template <bool TRI>
 __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
 {
@@ -650,8 +672,11 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
 // 4-clique plan's, while the rare completions that make its variance are
 // drawn J times per closed triangle.  One lane running its own J completions
 // would hold its wavefront for J dependent chains; spread over the lanes they
-// cost about one.  Every lane of the wavefront calls this (active = the lane
-// has an estimator); outcomes match the oracle's sequential completions.
+// cost about one.  The searches that fix the shapes' starting positions depend
+// only on the first stage, so the owning lane runs them once (k4_prepare) and
+// the completions share the result.  Every lane of the wavefront calls this
+// (active = the lane has an estimator); outcomes match the oracle's sequential
+// completions.
 __device__ __forceinline__ uint32_t lane_u32(uint32_t x, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)x, l); }
 __device__ __forceinline__ VI lane_vi(const VI &x, int l) { return VI{lane_u32(x.v, l), lane_u32(x.deg, l), lane_u32(x.off, l)}; }
 
@@ -670,24 +695,27 @@ __device__ __forceinline__ void k4_split(const GView &g, const PlanView &P, uint
     K4S k;
     const bool closed = active && k4_first<TRI>(g, rng, o, k);
     words = rng.word;
+    K4Q q;
+    if (closed) k4_prepare(g, k, q);   // the completions' shared searches, once per estimator
     const uint32_t lane = threadIdx.x & 63;
     const int lg = TRI ? P.K1 : P.K;
     const uint32_t J = 1u << lg;
     for (uint64_t pend = __ballot(closed); pend; pend &= pend - 1) {
         const int l = __builtin_ctzll(pend);
-        K4S t;
-        t.A = lane_vi(k.A, l);
-        t.B = lane_vi(k.B, l);
-        t.W = lane_vi(k.W, l);
-        t.r0 = lane_u32(k.r0, l);
-        t.r1 = lane_u32(k.r1, l);
-        t.rt = lane_u32(k.rt, l);
-        t.pa0 = lane_u32(k.pa0, l);
-        t.pb0 = lane_u32(k.pb0, l);
-        t.po1 = lane_u32(k.po1, l);
-        t.pw1 = lane_u32(k.pw1, l);
-        t.oa = lane_u32(k.oa, l);
-        t.s1 = ((uint64_t)lane_u32((uint32_t)(k.s1 >> 32), l) << 32) | lane_u32((uint32_t)k.s1, l);
+        K4Q t;
+        t.A = lane_vi(q.A, l);
+        t.B = lane_vi(q.B, l);
+        t.W = lane_vi(q.W, l);
+        t.r0 = lane_u32(q.r0, l);
+        t.r1 = lane_u32(q.r1, l);
+        t.oa = lane_u32(q.oa, l);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            t.pA[i] = lane_u32(q.pA[i], l);
+            t.pB[i] = lane_u32(q.pB[i], l);
+            t.pC[i] = lane_u32(q.pC[i], l);
+        }
+        t.s1 = ((uint64_t)lane_u32((uint32_t)(q.s1 >> 32), l) << 32) | lane_u32((uint32_t)q.s1, l);
         const uint64_t tid = ((uint64_t)lane_u32((uint32_t)(id >> 32), l) << 32) | lane_u32((uint32_t)id, l);
         U128 sum{0, 0};
         bool ov = false;
@@ -695,8 +723,7 @@ __device__ __forceinline__ void k4_split(const GView &g, const PlanView &P, uint
             if (j0 + lane < J) {
                 Rng rc;
                 rc.init(seed, TAG_COMPLETION + j0 + lane, tid);
-                uint32_t nc = 0;
-                const U128 v = k4_shapes(g, rc, t, nc, ov);
+                const U128 v = k4_complete(g, rc, t, ov);
                 ov |= add_u128(sum, v);
             }
         }
