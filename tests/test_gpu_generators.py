@@ -143,6 +143,38 @@ def test_device_k4_count_with_long_out_lists(nh, nl):
     assert g.exact_count("triangle") == nh + 2 * pairs + pairs * nl
 
 
+@pytest.mark.parametrize("n", [600, 1500, 4097])
+def test_device_k4_count_wide_rows_on_complete_graphs(n):
+    """K_n with n - 1 >= 513: the high vertices' out-lists (513..4096 entries,
+    in the orientation's order) have triangular bit rows of 9..64 words,
+    counted one wavefront per row with the lanes packed in groups; K_4097's
+    vertex 0 has 4096 out-neighbours, the longest sorted list.  C(n, 4)
+    4-cliques."""
+    from math import comb
+    import helpers as H
+    n, s, d = H.complete_graph(n)
+    g = Z.Graph.from_edges(s, d, n, seed=2)
+    assert g.exact_count("k4") == comb(n, 4)
+
+
+_DENSE_K4 = {}
+
+
+@pytest.mark.parametrize("scratch_words", [None, "300000"])
+def test_device_k4_count_wide_rows_equals_oracle(monkeypatch, scratch_words):
+    """A dense random graph (1200 vertices, p = 0.6): out-lists of up to ~700
+    entries (wide triangular rows); the count equals the oracle's, in one
+    pass and in many vertex ranges."""
+    import helpers as H
+    if scratch_words:
+        monkeypatch.setenv("ZGRAPH_K4_SCRATCH_WORDS", scratch_words)
+    n, s, d = H.random_graph(1200, 0.6, 31)
+    if "k4" not in _DENSE_K4:   # the oracle's count once for both scratch sizes
+        _DENSE_K4["k4"] = O.OGraph(n, s, d, seed=1).k4()
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    assert g.exact_count("k4") == _DENSE_K4["k4"] == 4007890405
+
+
 @pytest.mark.parametrize("scratch_words", [None, "20000"])
 def test_device_k4_count_rmat16_in_one_and_in_many_passes(monkeypatch, scratch_words):
     """R-MAT scale 16 (1 M edges): the 4-clique count equals the oracle's,
