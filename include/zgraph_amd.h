@@ -286,9 +286,9 @@ int zg_count_reads(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_
  * that exceed 64 bits fail with ZG_EOVERFLOW (5-stars, 5-chains; 4-cycles are
  * summed in 128 bits and checked too, though at most C(m, 2) < 2^61 of them
  * exist for m < 2^31).  Scratch: 4-cliques work in vertex ranges
- * whose bit rows fit min(16 GiB, max(1 GiB, free / 4)) (the environment
+ * whose bit rows fit min(32 GiB, max(1 GiB, free / 3)) (the environment
  * variable ZGRAPH_K4_SCRATCH_WORDS lowers it, in 8-byte words); 4-cycles and
- * 5-chains sort the hubs' wedges in batches of up to budget / 24 wedges
+ * 5-chains sort the hubs' wedges in batches of up to min(16 GiB, max(1 GiB, free / 4)) / 24 wedges
  * (ZGRAPH_C4_SORT_WEDGES lowers it; a hub with more wedges is counted with
  * a dense counter of n entries instead; ZGRAPH_C4_WIDE_KEYS=1 and
  * ZGRAPH_C4_BATCH_VERTICES force the 64-bit sort keys and cap the vertices of
