@@ -667,7 +667,8 @@ __device__ __forceinline__ void plan_k4_close(const GView &g, Rng &rng, Est &o)
 // closes, the whole wavefront runs that estimator's J completions -- the
 // three shapes, completion j drawing from the stream (seed, TAG_COMPLETION +
 // j, id) -- lane l taking completions l, l + 64, ...  The value is the sum of
-// the completions' values over J (J = 2^P.K): each completion has the
+// the completions' values over J (a power of two: the channel's denominator
+// exponent, P.K, or P.K1 beside the triangle channel): each completion has the
 // distribution of the shapes given the first stage, so the mean is the
 // 4-clique plan's, while the rare completions that make its variance are
 // drawn J times per closed triangle.  One lane running its own J completions
