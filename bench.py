@@ -69,6 +69,9 @@ def main():
     ap.add_argument("--line-sweep", action="store_true",
                     help="only measure the random 128-byte line rate by load shape (1, 2, 5, 8 loads of 16 bytes "
                          "per lane, or 8 lanes per line) on 2, 8 and 32 GiB buffers")
+    ap.add_argument("--region-sweep", action="store_true",
+                    help="only measure the random 128-byte line rate with each workgroup's reads confined to a "
+                         "region (1..16 MiB, matched to the XCDs or not) against the whole 8 GiB buffer")
     args = ap.parse_args()
 
     if args.calibrate_copy_mib:
@@ -93,6 +96,22 @@ def main():
                              "gbs": round(gbs, 1), "ms": round(ms, 3)})
                 log(f"line sweep: {gib} GiB, {rows[-1]['shape']}: {gbs:.0f} GB/s")
         print(json.dumps({"line_sweep": rows, "lines_per_point": 1 << 28}), flush=True)
+        return
+
+    if args.region_sweep:
+        import zgraph as Z
+        lines = 1 << 30
+        gbs, ms = Z.random_line_bandwidth(8 << 30, 1, False, lines)
+        rows = [{"region": "whole 8 GiB buffer", "gbs": round(gbs, 1), "ms": round(ms, 3)}]
+        log(f"region sweep: whole buffer {gbs:.0f} GB/s")
+        for mib in (1, 2, 4, 16):
+            for matched in (True, False):
+                for rpp in (64, 256):
+                    gbs, ms = Z.random_region_bandwidth(mib << 20, matched, 8 << 30, rpp, lines)
+                    rows.append({"region_mib": mib, "xcd_matched": matched, "rounds_per_phase": rpp,
+                                 "gbs": round(gbs, 1), "ms": round(ms, 3)})
+                    log(f"region sweep: {mib} MiB, xcd_matched={matched}, {rpp} reads per phase: {gbs:.0f} GB/s")
+        print(json.dumps({"region_sweep": rows, "lines_per_point": lines}), flush=True)
         return
 
     if args.gpus != 1:

@@ -324,6 +324,16 @@ int zg_random_chain_bw(uint64_t m, uint64_t estimators, int block_size, int bloc
  * hash's one-load header. */
 int zg_random_line_bw(uint64_t buffer_bytes, int loads_per_line, int cooperative, uint64_t lines, int block_size,
                       int blocks_per_cu, double *gbs, double *ms);
+/* Random lines confined to regions: `lines` random 16-byte reads, one per
+ * 128-byte line, each address taken from the previous line, inside regions of
+ * region_bytes of a buffer_bytes buffer; workgroup b reads region
+ * (b mod 8 + 8 * phase) when xcd_matched (the workgroups of one XCD share a
+ * region: L2-resident when it fits), ((b / 8) mod 8 + 8 * phase) otherwise,
+ * with phase = round / rounds_per_phase.  *gbs counts 128 bytes per line.
+ * Measurement helper (the reuse each cache level would give the estimator
+ * reads); no reference counterpart. */
+int zg_random_region_bw(uint64_t buffer_bytes, uint64_t region_bytes, int xcd_matched, uint32_t rounds_per_phase,
+                        uint64_t lines, int block_size, int blocks_per_cu, double *gbs, double *ms);
 
 /* ---- error-latency profile (src/ErrorLatencyProfile.cpp) --------------- */
 /* ErrorProfile<E>::get_num_estimators (:71-207) on the device: subsample at

@@ -75,6 +75,28 @@ __global__ __launch_bounds__(256) void k_mb_lines_coop(const uint4 *buf, uint64_
     if (x == 0x0123456789ABCDEFull) sink[0] = x;
 }
 
+// One random 16-byte read per 128-byte line, each address from the previous
+// line, confined to regions of region_lines lines: workgroup b reads region
+// (sel + 8 * phase) mod nreg, phase = round / rounds_per_phase, with sel =
+// b mod 8 (XCD_MATCHED: the workgroups of one XCD -- consecutive workgroups go
+// to the eight XCDs in turn -- share a region, so it can live in that XCD's
+// L2) or sel = (b / 8) mod 8 (the workgroups of one XCD spread over eight
+// regions).  Shows what the L2, the memory-side cache and the DRAM each serve
+// for random lines.
+template <bool XCD_MATCHED>
+__global__ __launch_bounds__(256) void k_mb_region(const uint4 *buf, uint64_t region_lines, uint64_t nreg,
+                                                   uint32_t rounds, uint32_t rounds_per_phase, uint64_t *sink)
+{
+    uint64_t x = hmix((uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1);
+    const uint32_t sel = XCD_MATCHED ? (blockIdx.x & 7u) : ((blockIdx.x >> 3) & 7u);
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint64_t reg = (sel + 8ull * (r / rounds_per_phase)) % nreg;
+        const uint4 v = buf[(reg * region_lines + __umul64hi(x, region_lines)) * 8];
+        x = hmix(x ^ (((uint64_t)v.x << 32) | v.y) ^ v.z);
+    }
+    if (x == 0x0123456789ABCDEFull) sink[0] = x;
+}
+
 // One estimator's access chain without the estimator: a random 32-byte record
 // (two 16-byte loads), then an 8-byte entry of a second array at an index
 // taken from the record, then the 16-byte header of a 128-byte bucket of a
@@ -272,6 +294,64 @@ extern "C" int zg_random_line_bw(uint64_t buffer_bytes, int loads_per_line, int 
         ZG_HIP(hipEventElapsedTime(&t, e0, e1));
         const double done = (double)readers * rounds;
         *gbs = done * 128.0 / (t * 1e-3) / 1e9;
+        if (ms) *ms = t;
+        return ZG_OK;
+    };
+    rc = run();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (sink) (void)hipFree(sink);
+    (void)hipFree(buf);
+    return rc;
+}
+
+extern "C" int zg_random_region_bw(uint64_t buffer_bytes, uint64_t region_bytes, int xcd_matched,
+                                   uint32_t rounds_per_phase, uint64_t lines, int block_size, int blocks_per_cu,
+                                   double *gbs, double *ms)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!gbs || lines == 0 || rounds_per_phase == 0 || region_bytes < 128 || buffer_bytes < region_bytes)
+        return set_error(ZG_EINVAL, "bad region benchmark arguments");
+    const int bs = block_size > 0 ? block_size : 256;
+    if (bs > 256 || bs % 64) return set_error(ZG_EINVAL, "block size must be a multiple of 64 up to 256");
+    const int bpc = blocks_per_cu > 0 ? blocks_per_cu : 16;
+    const uint64_t grid = (uint64_t)device_cus() * bpc;
+    const uint64_t readers = grid * bs;
+    const uint64_t per_reader = (lines + readers - 1) / readers;
+    if (per_reader > 0xffffffffull) return set_error(ZG_EOVERFLOW, "too many lines");
+    const uint32_t rounds = (uint32_t)per_reader;
+    const uint64_t region_lines = region_bytes / 128, nreg = buffer_bytes / 128 / region_lines;
+    uint4 *buf = nullptr;
+    uint64_t *sink = nullptr;
+    if (hipMalloc(&buf, nreg * region_lines * 128) != hipSuccess)
+        return set_error(ZG_ENOMEM, "region buffer allocation failed");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto run = [&]() -> int {
+        ZG_HIP(hipMalloc(&sink, 8));
+        ZG_HIP(hipEventCreate(&e0));
+        ZG_HIP(hipEventCreate(&e1));
+        hipLaunchKernelGGL(k_mb_fill, dim3(device_cus() * 64), dim3(256), 0, 0, reinterpret_cast<uint64_t *>(buf),
+                           nreg * region_lines * 16);
+        ZG_HIP(hipGetLastError());
+        auto launch = [&]() {
+            if (xcd_matched)
+                hipLaunchKernelGGL(k_mb_region<true>, dim3(grid), dim3(bs), 0, 0, buf, region_lines, nreg, rounds,
+                                   rounds_per_phase, sink);
+            else
+                hipLaunchKernelGGL(k_mb_region<false>, dim3(grid), dim3(bs), 0, 0, buf, region_lines, nreg, rounds,
+                                   rounds_per_phase, sink);
+        };
+        launch();   // warm-up
+        ZG_HIP(hipGetLastError());
+        ZG_HIP(hipEventRecord(e0, 0));
+        launch();
+        ZG_HIP(hipGetLastError());
+        ZG_HIP(hipEventRecord(e1, 0));
+        ZG_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        ZG_HIP(hipEventElapsedTime(&t, e0, e1));
+        *gbs = (double)readers * rounds * 128.0 / (t * 1e-3) / 1e9;
         if (ms) *ms = t;
         return ZG_OK;
     };

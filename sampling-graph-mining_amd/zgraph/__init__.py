@@ -30,7 +30,7 @@ __all__ = [
     "SamplerGenerator", "AllPredicateZGraphInstance", "ErrorProfile", "TimeProfile",
     "load_graph_meta_data", "load_biedgelist", "load_edgelist", "save_biedgelist", "ZGraphError",
     "device_info", "device_memory", "clt_half_width", "run", "run_outcomes", "Batch", "random_gather_bandwidth",
-    "random_line_bandwidth", "random_chain_bandwidth",
+    "random_line_bandwidth", "random_chain_bandwidth", "random_region_bandwidth",
 ]
 
 OUTCOME_DTYPE = np.dtype([("num_lo", "<u8"), ("num_hi", "<u8"), ("e0", "<u4"), ("status", "u1"),
@@ -402,6 +402,17 @@ def random_line_bandwidth(buffer_bytes=16 << 30, loads_per_line=1, cooperative=F
     gbs, ms = C.c_double(), C.c_double()
     check(lib().zg_random_line_bw(buffer_bytes, loads_per_line, int(bool(cooperative)), lines, block_size,
                                   blocks_per_cu, C.byref(gbs), C.byref(ms)))
+    return gbs.value, ms.value
+
+
+def random_region_bandwidth(region_bytes, xcd_matched=True, buffer_bytes=8 << 30, rounds_per_phase=256,
+                            lines=1 << 30, block_size=0, blocks_per_cu=0):
+    """Random 128-byte line rate in GB/s when each workgroup's reads stay in
+    one region of `region_bytes` (changing every `rounds_per_phase` reads),
+    the regions matched to the XCDs (each XCD's workgroups share one) or not."""
+    gbs, ms = C.c_double(), C.c_double()
+    check(lib().zg_random_region_bw(buffer_bytes, region_bytes, int(bool(xcd_matched)), rounds_per_phase, lines,
+                                    block_size, blocks_per_cu, C.byref(gbs), C.byref(ms)))
     return gbs.value, ms.value
 
 

@@ -42,7 +42,7 @@ EXPORTED = [
     "zg_run", "zg_run_noalloc", "zg_run_outcomes", "zg_count_reads", "zg_exact_counts",
     "zg_batch_create", "zg_batch_records", "zg_batch_run", "zg_batch_set_ids", "zg_batch_autotune",
     "zg_batch_config_get", "zg_batch_destroy", "zg_run_batch",
-    "zg_graph_degree_moments", "zg_random_gather_bw", "zg_random_line_bw", "zg_random_chain_bw",
+    "zg_graph_degree_moments", "zg_random_gather_bw", "zg_random_line_bw", "zg_random_chain_bw", "zg_random_region_bw",
     "zg_elp_regression", "zg_elp_bound", "zg_regression_k", "zg_elp_from_pilots",
     "zg_mlp_num_params", "zg_mlp_input_dim", "zg_mlp_create", "zg_mlp_destroy", "zg_mlp_get_params",
     "zg_mlp_set_params", "zg_mlp_forward", "zg_mlp_gradient", "zg_mlp_train",
@@ -162,6 +162,8 @@ def lib():
     L.zg_graph_degree_moments.argtypes = [vp, P(C.c_double), P(C.c_double)]
     L.zg_random_gather_bw.argtypes = [C.c_uint64, C.c_int, C.c_uint64, C.c_int, C.c_int, P(C.c_double),
                                       P(C.c_double)]
+    L.zg_random_region_bw.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_uint32, C.c_uint64, C.c_int, C.c_int,
+                                      P(C.c_double), P(C.c_double)]
     L.zg_random_line_bw.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, P(C.c_double),
                                     P(C.c_double)]
     L.zg_random_chain_bw.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, P(C.c_double), P(C.c_double)]
