@@ -6,7 +6,9 @@ the device has 200 GiB of HBM free, as an MI355X does):
   * the fused 3-motif plan's two channels equal the separate 3-chain and
     triangle plans exactly (same draws per id);
   * 3-chain, triangle and 4-chain (FourChain) estimates lie in the 99.9 %
-    intervals of the exact wedge, triangle and 3-edge path counts;
+    intervals of the exact wedge, triangle and 3-edge path counts, and the
+    split 4-clique plan's (fused with the triangle channel, which equals the
+    triangle plan) in that of the exact 4-clique count;
   * predicate triangles: the all-edges estimate on the w >= 0.5 graph lies in
     the interval of that graph's exact triangle count, and the at-least-one
     estimate (all triangles minus those of the w < 0.5 graph,
@@ -56,6 +58,11 @@ def _check(scale, N):
     _in_interval("triangles", tri, N, T)
     four = Z.run(g, Z.Plan(Z.PLAN_FOUR_CHAIN), 11, 0, N)[0]
     _in_interval("3-edge paths (4-chain)", four, N, g.exact_count("path3"))
+    # 4-cliques by the split plan (256 completions per closed triangle): the first stage's
+    # triangles are the triangle plan's, and the estimate lies in the interval of the exact count
+    split = Z.run(g, Z.Plan(Z.PLAN_TRIANGLE_K4_SPLIT, completions=256), 11, 0, N // 4)
+    assert split[0].total() == Z.run(g, Z.Plan(Z.PLAN_TRIANGLE), 11, 0, N // 4)[0].total()
+    _in_interval("4-cliques (split plan)", split[1], N // 4, g.exact_count("k4"))
     half = Z.run(g, Z.Plan(Z.PLAN_FOUR_CHAIN), 11, 0, N // 2)[0]
     rest = Z.run(g, Z.Plan(Z.PLAN_FOUR_CHAIN), 11, N // 2, N - N // 2)[0]
     assert half.total() + rest.total() == four.total()
