@@ -51,6 +51,8 @@ def main():
                     help="completions per closed triangle of the split 4-clique plan")
     ap.add_argument("--elp-patterns", default="",
                     help="--workload elp: comma-separated pattern-name prefixes to run (default: all)")
+    ap.add_argument("--elp-graphs", default="",
+                    help="--workload elp: comma-separated graph-name prefixes to run (default: all five)")
     ap.add_argument("--no-coverage", action="store_true", help="--workload elp: skip the 40-query coverage runs")
     ap.add_argument("--elp-cap-log2", type=int, default=39,
                     help="largest budget of one time-to-error query, log2 estimators (2^39 = 5.5e11: well "
@@ -383,6 +385,9 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95, patterns=None, cap_log2=None,
         res["queries_per_arm"] = nq
         out[name] = res
     if patterns_default and learned is not None:
+        # 4-cliques by the split plan with its completions chosen by two pilots
+        out["4-clique split, completions chosen"] = split_auto_arm(Z, L, g, learned, eps, conf, queries, cap_log2)
+    if patterns_default and learned is not None:
         # triangles and 4-cliques (split plan) both to +-eps by one query over one id range
         tri, k4 = g.exact_count("triangle"), g.exact_count("k4")
         inst = Z.TriangleFourCliqueSplitInstance(g, completions=J, seed=100)
@@ -408,6 +413,28 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95, patterns=None, cap_log2=None,
             bt = out["4-clique"].get(base, {}).get("mean_time_s")
             out[split]["speedup_vs_4clique_plan_" + base.replace("+", "_")] = (bt / st) if (bt and st) else None
     return out
+
+
+def split_auto_arm(Z, L, g, prof, eps, conf, queries, cap_log2):
+    """Time to +-eps for 4-cliques by the split plan whose completions per
+    closed triangle are chosen by elp.choose_split_completions (two pilots,
+    their time included), then sized by `prof` and judged."""
+    truth = g.exact_count("k4")
+    L.split_k4_to_error(g, eps, conf, prof, seed=100, max_estimators=1 << 30)   # untimed first query
+    recs = []
+    for q in range(queries):
+        r = L.split_k4_to_error(g, eps, conf, prof, seed=101 + q, max_estimators=1 << cap_log2)
+        err = abs(r["estimate"] - truth) / truth if truth and r["estimators"] else float("nan")
+        recs.append({"seed": 101 + q, "completions": r["completions"], "estimators": r["estimators"],
+                     "choice_s": r["choice_s"], "total_s": r["total_s"], "error": err, "accepted": r["accepted"],
+                     "sizing": r["sizing"]})
+        log(f"4-clique split, completions chosen, q{q}: {r['total_s'] * 1e3:.1f} ms (choice "
+            f"{r['choice_s'] * 1e3:.1f} ms, {r['completions']} completions) N={r['estimators']} err={err * 100:.3f}%")
+    ok = [r for r in recs if r["estimators"]]
+    return {"mean_time_s": sum(r["total_s"] for r in ok) / len(ok) if ok else None,
+            "mean_choice_s": sum(r["choice_s"] for r in ok) / len(ok) if ok else None,
+            "within_eps": sum(r["error"] <= eps for r in ok) / len(recs),
+            "completions": sorted({r["completions"] for r in recs}), "queries": recs}
 
 
 def batched_workload(Z, seed=1):
@@ -520,13 +547,16 @@ def elp_sweep(Z, args):
     10 M vertices / 100 M edges, Chung-Lu with 2 M vertices / 20 M edges and
     beta 2.5, R-MAT scale 26 with 1.07 B edges): the reference's regression
     sizing against pilot-only and pilot + network sizing with the judge, 3
-    queries each, budgets capped at 2^34 (2^36 on R-MAT 26), errors against the
-    device exact counts."""
+    queries each, budgets capped at 2^34 (2^36 on R-MAT 26, 2^37 on the beta 2.5
+    graph), errors against the device exact counts; with the split pattern,
+    also the split plan with its completions chosen by two pilots."""
     graphs = [("chunglu 4.8M/69M b2.1", lambda: Z.chung_lu(4_800_000, 69_000_000, 2.1, 20000.0, args.seed)),
               ("rmat22", lambda: Z.rmat(22, 16 << 22, 0.57, 0.19, 0.19, args.seed)),
               ("erdos-renyi 10M/100M", lambda: Z.erdos_renyi(10_000_000, 100_000_000, args.seed)),
               ("chunglu 2M/20M b2.5", lambda: Z.chung_lu(2_000_000, 20_000_000, 2.5, 5000.0, args.seed)),
               ("rmat26", lambda: Z.rmat(26, 16 << 26, 0.57, 0.19, 0.19, args.seed))]
+    # 4-cliques are rare on the beta 2.5 graph (1.6e5 of them): +-1 % takes ~1e11 estimators there
+    caps = {"rmat26": 36, "chunglu 2M/20M b2.5": 37}
     J = args.split_completions
     patterns = (("triangle", Z.PLAN_TRIANGLE, "triangle"), ("4-clique", Z.PLAN_K4_CLOSE, "k4"),
                 ("3-chain", Z.PLAN_THREE_CHAIN, "wedge"), ("4-chain", Z.PLAN_FOUR_CHAIN, "path3"),
@@ -535,14 +565,21 @@ def elp_sweep(Z, args):
     if args.elp_patterns:
         keep = [x.strip() for x in args.elp_patterns.split(",")]
         patterns = tuple(p for p in patterns if any(p[0].startswith(k) for k in keep))
+    if args.elp_graphs:
+        keep = [x.strip() for x in args.elp_graphs.split(",")]
+        graphs = [x for x in graphs if any(x[0].startswith(k) for k in keep)]
     rows = {}
     for gname, make in graphs:
         e = make()
         g = Z.Graph.from_device(e, seed=args.seed)
         e.free()
         log(f"elp sweep {gname}: m={g.num_local_edges} maxdeg={g.max_degree}")
-        rows[gname] = time_to_error(Z, g, args, patterns=patterns, cap_log2=36 if gname == "rmat26" else 34,
-                                    queries=3)
+        cap = caps.get(gname, 34)
+        rows[gname] = time_to_error(Z, g, args, patterns=patterns, cap_log2=cap, queries=3)
+        if any(p[0].startswith("4-clique split") for p in patterns):
+            from zgraph import elp as L
+            rows[gname]["4-clique split, completions chosen"] = split_auto_arm(
+                Z, L, g, L.LearnedErrorProfile(args.elp_weights or L.DEFAULT_WEIGHTS), 0.01, 0.95, 3, cap)
         cov = [p for p in patterns if p[0] in ("triangle", "3-chain", "4-chain")]
         if gname in ("chunglu 4.8M/69M b2.1", "rmat22") and cov and not args.no_coverage:
             rows[gname]["coverage"] = elp_coverage(Z, g, args, cov)

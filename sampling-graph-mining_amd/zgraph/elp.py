@@ -350,3 +350,67 @@ def estimate_both_to_error(instance, eps, conf, profile, max_estimators=1 << 40,
                       "accepted": judge(ests[c], eps, conf)})
     return {"channels": chans, "estimators": done, "topup_rounds": rounds, "capped": max(sizes) > max_estimators,
             "sizing_s": t_size, "total_s": t1 - t0}
+
+
+def choose_split_completions(graph, seed=1, j_lo=64, j_hi=1024, min_successes=32, pilot=1 << 22,
+                             max_pilot=1 << 30, first_id=1 << 60, fused=False):
+    """Completions per closed triangle for the split 4-clique plan, from two
+    pilots of the same ids with j_lo and j_hi completions (each at least
+    `pilot` ids, enough for the kernel times to be launch-independent, grown
+    4x until it holds `min_successes` successes or `max_pilot` ids).  Per estimator,
+    Var(J) = Vb + Vw / J (Vb: the first stage's share, Vw: the completions')
+    and the kernel time t(J) = c0 + c1 J; the time to a given error is
+    proportional to Var(J) t(J), least at J* = sqrt(Vw c0 / (Vb c1)), returned
+    as the nearest power of two in [16, 4096] (16 when Vw <= 0, 4096 when
+    Vb <= 0).  The pilots run ids from `first_id` on, away from the query's
+    own.  fused: size the 4-clique channel of ZG_PLAN_TRIANGLE_K4_SPLIT.
+    Returns (J, record)."""
+    from . import PLAN_K4_SPLIT as _SPLIT, PLAN_TRIANGLE_K4_SPLIT as _FUSED, Plan
+    kind, ch = (_FUSED, 1) if fused else (_SPLIT, 0)
+    t0 = time.perf_counter()
+    stats = {}
+    for J in (j_lo, j_hi):
+        plan = Plan(kind, completions=J)
+        n = pilot
+        while True:
+            run(graph, plan, seed, first_id, min(n, 1 << 16))   # code objects loaded before timing
+            sums, ms = run(graph, plan, seed, first_id, n, timed=True)
+            s = sums[ch]
+            if s.successes >= min_successes or n >= max_pilot:
+                break
+            n = min(n * 4, max_pilot)
+        mean = s.mean
+        stats[J] = {"ids": n, "successes": s.successes, "mean": mean,
+                    "var": max(s.sumsq / n - mean * mean, 0.0), "ms_per_id": ms / n}
+    a, b = stats[j_lo], stats[j_hi]
+    vw = (a["var"] - b["var"]) / (1.0 / j_lo - 1.0 / j_hi)
+    vb = b["var"] - vw / j_hi
+    c1 = (b["ms_per_id"] - a["ms_per_id"]) / (j_hi - j_lo)
+    c0 = a["ms_per_id"] - c1 * j_lo
+    if vw <= 0:
+        jstar = 16.0
+    elif vb <= 0 or c1 <= 0:
+        jstar = 4096.0
+    else:
+        jstar = math.sqrt(vw * max(c0, 1e-12) / (vb * c1))
+    J = int(min(4096, max(16, 2 ** round(math.log2(max(jstar, 1.0))))))
+    rec = {"completions": J, "j_star": jstar, "v_between": vb, "v_within": vw, "c0_ms_per_id": c0,
+           "c1_ms_per_id_per_completion": c1, "pilots": stats, "seconds": time.perf_counter() - t0}
+    return J, rec
+
+
+def split_k4_to_error(graph, eps, conf, profile, seed=1, max_estimators=1 << 40, **choose):
+    """4-cliques to +-eps at `conf` by the split plan with its completions
+    chosen by choose_split_completions (pilots on ids away from the query's),
+    then sized by `profile` and judged as estimate_to_error does.  The record
+    adds the completions, the choice's record and its time (in total_s)."""
+    from . import FourCliqueSplitCountingInstance
+    t0 = time.perf_counter()
+    J, choice = choose_split_completions(graph, seed=seed, **choose)
+    inst = FourCliqueSplitCountingInstance(graph, completions=J, seed=seed)
+    rec = estimate_to_error(inst, eps, conf, profile, max_estimators=max_estimators)
+    rec["completions"] = J
+    rec["choice"] = choice
+    rec["choice_s"] = choice["seconds"]
+    rec["total_s"] = time.perf_counter() - t0
+    return rec
