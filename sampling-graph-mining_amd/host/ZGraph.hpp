@@ -1157,6 +1157,71 @@ class LearnedErrorProfile {
     ErrorProfile<EdgeData> fallback_;
 };
 
+// ---- completions of the split 4-clique plan (no reference counterpart) ------
+// Two pilots over the same ids (from first_id on, away from a query's own)
+// with j_lo and j_hi completions per closed triangle, each at least `pilot`
+// ids and grown 4x until it holds min_successes successes or max_pilot ids.
+// Per estimator Var(J) = Vb + Vw / J and the kernel time t(J) = c0 + c1 J, so
+// the time to a given error, proportional to Var(J) t(J), is least at
+// J* = sqrt(Vw c0 / (Vb c1)); the choice is the nearest power of two in
+// [16, 4096] (16 when Vw <= 0, 4096 when Vb <= 0 or c1 <= 0).  fused: size the
+// 4-clique channel of ZG_PLAN_TRIANGLE_K4_SPLIT.  Same rule as
+// zgraph.elp.choose_split_completions.
+struct SplitChoice {
+    int completions = 0;
+    double j_star = 0, v_between = 0, v_within = 0, c0_ms_per_id = 0, c1_ms_per_id_per_completion = 0,
+           seconds = 0;
+    unsigned long long pilot_ids[2] = {0, 0}, pilot_successes[2] = {0, 0};
+    double pilot_var[2] = {0, 0};
+};
+
+template <typename EdgeData>
+SplitChoice choose_split_completions(Graph<EdgeData> *g, uint64_t seed = 1, bool fused = false, int j_lo = 64,
+                                     int j_hi = 1024, unsigned long long min_successes = 32,
+                                     unsigned long long pilot = 1ull << 22, unsigned long long max_pilot = 1ull << 30,
+                                     uint64_t first_id = 1ull << 60)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const int kind = fused ? ZG_PLAN_TRIANGLE_K4_SPLIT : ZG_PLAN_K4_SPLIT, ch = fused ? 1 : 0;
+    SplitChoice c;
+    double ms_per_id[2] = {0, 0};
+    const int js[2] = {j_lo, j_hi};
+    for (int i = 0; i < 2; ++i) {
+        Plan plan(Plan::split_k4(js[i], kind));
+        zg_launch launch{0, 0, 0};
+        zg_sums s[2];
+        unsigned long long n = pilot;
+        for (;;) {
+            // code objects loaded before timing
+            check(zg_run(g->handle(), plan.handle(), seed, first_id, std::min(n, 1ull << 16), &launch, s, nullptr));
+            float ms = 0;
+            check(zg_run(g->handle(), plan.handle(), seed, first_id, n, &launch, s, &ms));
+            if (s[ch].successes >= min_successes || n >= max_pilot) {
+                ms_per_id[i] = ms / (double)n;
+                break;
+            }
+            n = std::min(n * 4, max_pilot);
+        }
+        const double mean = s[ch].mean;
+        c.pilot_ids[i] = n;
+        c.pilot_successes[i] = s[ch].successes;
+        c.pilot_var[i] = std::max(s[ch].sumsq / (double)n - mean * mean, 0.0);
+    }
+    c.v_within = (c.pilot_var[0] - c.pilot_var[1]) / (1.0 / j_lo - 1.0 / j_hi);
+    c.v_between = c.pilot_var[1] - c.v_within / j_hi;
+    c.c1_ms_per_id_per_completion = (ms_per_id[1] - ms_per_id[0]) / (j_hi - j_lo);
+    c.c0_ms_per_id = ms_per_id[0] - c.c1_ms_per_id_per_completion * j_lo;
+    if (c.v_within <= 0) c.j_star = 16;
+    else if (c.v_between <= 0 || c.c1_ms_per_id_per_completion <= 0) c.j_star = 4096;
+    else
+        c.j_star = std::sqrt(c.v_within * std::max(c.c0_ms_per_id, 1e-12) /
+                             (c.v_between * c.c1_ms_per_id_per_completion));
+    const double l = std::nearbyint(std::log2(std::max(c.j_star, 1.0)));
+    c.completions = (int)std::min(4096.0, std::max(16.0, std::exp2(l)));
+    c.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return c;
+}
+
 }  // namespace zgraph
 
 #ifndef ZGRAPH_NO_GLOBAL_NAMES

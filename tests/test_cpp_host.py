@@ -175,6 +175,35 @@ def test_learned_profile_cpp_equals_python(tmp_path, pattern, kind, completions,
 
 
 @pytest.mark.gpu
+def test_split_completions_choice_cpp_equals_python(tmp_path):
+    """zgraph::choose_split_completions fits the same variance model from the
+    same pilots as zgraph.elp.choose_split_completions (the kernel times, and
+    so the choice itself, may differ between runs), and the query it sizes
+    reaches the error."""
+    from zgraph import elp as L
+    n, s, d = golden_graph("wiki-vote")
+    prefix = _write_graph(tmp_path, "wv", n, s, d)
+    eps, seed = 0.02, 13
+    print("[cpp] EstimateToError 4-clique-split-auto", flush=True)
+    r = subprocess.run([_bin("EstimateToError"), prefix, "4-clique-split-auto", str(eps), str(seed)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"^completions (\d+) (\S+) (\S+) (\d+) (\d+)$", r.stdout, re.M)
+    J, vb, vw, n0, n1 = int(m.group(1)), float(m.group(2)), float(m.group(3)), int(m.group(4)), int(m.group(5))
+    assert J in [1 << k for k in range(4, 13)]
+    judged = re.search(r"^judged (\d) (\S+)$", r.stdout, re.M)
+    assert judged.group(1) == "1" and float(judged.group(2)) <= eps
+    g = Z.Graph.from_edges(s, d, n, seed=1)
+    _, rec = L.choose_split_completions(g, seed=seed)
+    assert (rec["pilots"][64]["ids"], rec["pilots"][1024]["ids"]) == (n0, n1)
+    assert vb == pytest.approx(rec["v_between"], rel=1e-9)
+    assert vw == pytest.approx(rec["v_within"], rel=1e-9)
+    truth = g.exact_count("k4")
+    est, _ = _report(r.stdout)["4-clique-split-auto"]
+    assert abs(est - truth) <= 2 * float(judged.group(2)) * est
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("pattern,cls,names", [("triangle+4-clique", "TriangleFourCliqueInstance", ("triangle", "4-clique")),
                                                ("triangle+4-clique-split", "TriangleFourCliqueSplitInstance",
                                                 ("triangle", "4-clique")),
