@@ -136,7 +136,15 @@ enum {
                                  remaining conditional samples and closing checks) */
     ZG_PLAN_TRIANGLE_K4_SPLIT = 10 /* fused: channel 0 = triangle (ZG_PLAN_TRIANGLE), channel 1 =
                                  the split 4-clique (ZG_PLAN_K4_SPLIT, `completions`) from one
-                                 first stage per id; each channel equals its own plan's run */
+                                 first stage per id; each channel equals its own plan's run */,
+    ZG_PLAN_HOUSE_SPLIT = 11  /* 5-house (the pattern of applications/FiveHouse.cpp:18-27), split:
+                                 base edge e0 and its roof apex from the smaller-degree endpoint's
+                                 list, closed by one pair lookup, then `completions` draws of the
+                                 square's two other vertices per closed roof, averaged.  Counts
+                                 each house once per set of edge copies (the simple graph's count
+                                 when there are no parallel edges).  No reference counterpart:
+                                 SamplerGenerator's house estimator rarely succeeds on large
+                                 graphs (none in 2^26 on a 69 M-edge Chung-Lu graph) */
 };
 
 #define ZG_MAX_PATTERN_EDGES 15
@@ -148,7 +156,8 @@ typedef struct {
     uint32_t pattern_src[ZG_MAX_PATTERN_EDGES];
     uint32_t pattern_dst[ZG_MAX_PATTERN_EDGES];
     int completions;                         /* ZG_PLAN_K4_SPLIT, ZG_PLAN_TRIANGLE_K4_SPLIT: completions
-                                                per closed triangle, a power of two in [1, 4096] */
+                                                per closed triangle; ZG_PLAN_HOUSE_SPLIT: per closed
+                                                roof; a power of two in [1, 4096] */
 } zg_plan_desc;
 
 int zg_plan_create(const zg_plan_desc *desc, zg_plan **out);

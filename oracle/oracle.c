@@ -18,6 +18,10 @@
  *   est_k4_split       exhaustive expectation with 1, 2 and 4 completions
  *                      == exact 4-cliques; first stage == est_k4_close's
  *                      per id; variance falls with completions          pinned
+ *   est_house_split    exhaustive expectation with 1 and 2 completions ==
+ *                      brute-force house counts (K5: 60, C(n,5) 60 on K_n;
+ *                      copy-weighted on multigraphs); the reference's
+ *                      wiki-vote ground truth inside the CLT interval   pinned
  *   pattern dedupe     tests/TestPattern2.cpp values (12/24, 1/24, 5/120,
  *                      60/120 house) and TestPattern.cpp values           pinned
  *   exact counters     closed forms, brute force, SNAP's published
@@ -1131,6 +1135,70 @@ static int est_k4_split(og_ctx *c, int J, uint64_t seed, uint64_t id)
     return total != 0;
 }
 
+/* Multiplicity of the pair {x, y}: the stored edges joining them (parallel
+ * copies count one each), by a scan of the shorter of the two lists. */
+static uint32_t pair_mult(const og_graph *g, uint32_t x, uint32_t y)
+{
+    if (g->off[y + 1] - g->off[y] < g->off[x + 1] - g->off[x]) { uint32_t t = x; x = y; y = t; }
+    uint32_t n = 0;
+    for (uint64_t i = g->off[x]; i < g->off[x + 1]; ++i) n += g->adj_dst[i] == y;
+    return n;
+}
+
+/* 5-house, split (no reference counterpart: the replacement of
+ * SamplerGenerator's house estimator for time to a target error, as
+ * est_k4_split is for 4-cliques).  The house of applications/FiveHouse.cpp:18-27
+ * is the roof triangle {0, 1, 2} on the square 1-3-4-2; exactly one of its
+ * edges, the base {1, 2}, lies in both its triangle and its 4-cycle, so a
+ * house is one tuple (base edge {a, b}, apex w ~ a, b; square vertices x ~ a
+ * and y ~ b with x ~ y; five distinct vertices), and the copy-weighted count
+ * weighs it by the multiplicities of its six pairs.
+ * First stage: e0 of rank r0 ~ U[m]; a = its endpoint of smaller degree (the
+ * source on ties), b the other; an entry ~ U[deg a] of a's list gives the apex
+ * w; the roof closes iff w != b and mult(w, b) > 0.  Completion j (stream
+ * (seed, OG_TAG_COMPLETION + j, id)): an entry ~ U[deg a] of a's list gives x,
+ * then an entry ~ U[deg b] of b's list gives y; it counts mult(x, y) when
+ * x != b, y != a, x != y and w is neither.  value = m deg(a) mult(w, b)
+ * deg(a) deg(b) (sum_j count_j) / J, so E[value | e0] = m times the
+ * copy-weighted houses on base e0.  ncond: 1 (the roof) + 2 J once it closes. */
+static int est_house_split(og_ctx *c, int J, uint64_t seed, uint64_t id)
+{
+    const og_graph *g = c->g;
+    if (g->m == 0) return 0;
+    const uint64_t r0 = og_uniform(&c->st, g->m);
+    c->e0 = (uint32_t)r0;
+    uint32_t a = g->src[r0], b = g->dst[r0];
+    uint64_t da = g->off[a + 1] - g->off[a], db = g->off[b + 1] - g->off[b];
+    if (db < da) {
+        uint32_t t = a; a = b; b = t;
+        uint64_t u = da; da = db; db = u;
+    }
+    c->ncond = 1;
+    const uint32_t w = g->adj_dst[g->off[a] + og_uniform(&c->st, da)];
+    if (w == b) return 0;
+    const uint32_t mwb = pair_mult(g, w, b);
+    if (!mwb) return 0;
+    og_stream main = c->st;
+    uint64_t S = 0;
+    for (int j = 0; j < J; ++j) {
+        og_stream_init(&c->st, seed, OG_TAG_COMPLETION + (uint32_t)j, id);
+        const uint32_t x = g->adj_dst[g->off[a] + og_uniform(&c->st, da)];
+        const uint32_t y = g->adj_dst[g->off[b] + og_uniform(&c->st, db)];
+        if (x == b || y == a || x == y || x == w || y == w) continue;
+        S += pair_mult(g, x, y);
+    }
+    c->st = main;
+    c->ncond = 1 + 2 * J;
+    if (!S) return 0;
+    c->num = g->m;
+    c->overflow = 0;
+    ctx_mul(c, da); ctx_mul(c, mwb); ctx_mul(c, da); ctx_mul(c, db); ctx_mul(c, S);
+    int lg = 0;
+    while ((1 << lg) < J) ++lg;
+    c->log2den = lg;
+    return 1;
+}
+
 /* ---- plan dispatch ----------------------------------------------------- */
 int og_plan_channels(const og_plan *p) { return p->kind == OG_PLAN_THREE_MOTIF ? 2 : 1; }
 
@@ -1142,7 +1210,7 @@ int og_plan_max_log2den(const og_plan *p)
     case OG_PLAN_FOUR_CHAIN: return 2;
     case OG_PLAN_PATTERN: return p->pe - 1;
     case OG_PLAN_K4_CLOSE: return 0;
-    case OG_PLAN_K4_SPLIT: { int lg = 0; while ((1 << lg) < p->order) ++lg; return lg; }
+    case OG_PLAN_K4_SPLIT: case OG_PLAN_HOUSE_SPLIT: { int lg = 0; while ((1 << lg) < p->order) ++lg; return lg; }
     default: return 0;
     }
 }
@@ -1155,6 +1223,7 @@ double og_plan_cond_calls(const og_plan *p)
     case OG_PLAN_FOUR_CHAIN: return 2;
     case OG_PLAN_PATTERN: return p->pe - 1;   /* SamplerGenerator.cpp:471-474 */
     case OG_PLAN_K4_CLOSE: case OG_PLAN_K4_SPLIT: return 2;
+    case OG_PLAN_HOUSE_SPLIT: return 3;   /* the roof, x and y */
     default: return 0;
     }
 }
@@ -1200,6 +1269,7 @@ static void estimate_with(const og_graph *g, const og_plan *p, const og_patset *
     case OG_PLAN_PATTERN: ok = est_pattern(&c, ps, p->pk, p->pe); break;
     case OG_PLAN_K4_CLOSE: ok = est_k4_close(&c); break;
     case OG_PLAN_K4_SPLIT: ok = est_k4_split(&c, p->order, seed, id); break;
+    case OG_PLAN_HOUSE_SPLIT: ok = est_house_split(&c, p->order, seed, id); break;
     default: abort();
     }
     if (ok) set_ok(&out[0], &c);

@@ -19,6 +19,7 @@ PLAN_FOUR_CHAIN = 5
 PLAN_PATTERN = 6
 PLAN_K4_CLOSE = 7
 PLAN_K4_SPLIT = 9     # order = completions (a power of two)
+PLAN_HOUSE_SPLIT = 11 # order = completions (a power of two)
 
 MAX_PE = 16
 

@@ -36,7 +36,7 @@ void og_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 
 /* stream tags (counter word 1) */
 enum { OG_TAG_ESTIMATOR = 0, OG_TAG_SHUFFLE = 1, OG_TAG_SUBSAMPLE = 2, OG_TAG_GEN = 3,
-       OG_TAG_COMPLETION = 0x1000 /* + j: completion j of a split 4-clique estimator */ };
+       OG_TAG_COMPLETION = 0x1000 /* + j: completion j of a split estimator */ };
 
 typedef struct {
     uint32_t key[2];
@@ -108,7 +108,8 @@ enum {
     OG_PLAN_PATTERN = 6,       /* src/SamplerGenerator.cpp:378-464 */
     OG_PLAN_K4_CLOSE = 7,      /* 4-clique: three spanning shapes + closing checks */
     OG_PLAN_FOUR_CLIQUE_REF = 8,/* applications/FourClique.cpp:41-76 (biased, kept for the record) */
-    OG_PLAN_K4_SPLIT = 9       /* OG_PLAN_K4_CLOSE's first stage + `order` (a power of two) completions */
+    OG_PLAN_K4_SPLIT = 9,      /* OG_PLAN_K4_CLOSE's first stage + `order` (a power of two) completions */
+    OG_PLAN_HOUSE_SPLIT = 11   /* 5-house: base edge + roof, then `order` (a power of two) square completions */
 };
 
 #define OG_MAX_PATTERN_V 8

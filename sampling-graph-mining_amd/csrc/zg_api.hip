@@ -319,13 +319,13 @@ extern "C" int zg_plan_create(const zg_plan_desc *d, zg_plan **out)
     case ZG_PLAN_K4_CLOSE: p->K = 0; p->cond_calls = 2; break;
     // each channel keeps its own plan's exponent, so its sums equal that plan's word for word
     case ZG_PLAN_TRIANGLE_K4: p->K = 1; p->K1 = 0; p->cond_calls = 2; p->nch = 2; break;
-    case ZG_PLAN_K4_SPLIT: case ZG_PLAN_TRIANGLE_K4_SPLIT: {
+    case ZG_PLAN_K4_SPLIT: case ZG_PLAN_TRIANGLE_K4_SPLIT: case ZG_PLAN_HOUSE_SPLIT: {
         const int J = d->completions;
         if (J < 1 || J > 4096 || (J & (J - 1)))
             return set_error(ZG_EINVAL, "completions must be a power of two in [1, 4096]");
-        const int lg = __builtin_ctz((unsigned)J);   // every 4-clique success has denominator J
-        p->cond_calls = 2;
-        if (d->kind == ZG_PLAN_K4_SPLIT) {
+        const int lg = __builtin_ctz((unsigned)J);   // every split success has denominator J
+        p->cond_calls = d->kind == ZG_PLAN_HOUSE_SPLIT ? 3 : 2;   // house: the roof, x and y
+        if (d->kind != ZG_PLAN_TRIANGLE_K4_SPLIT) {
             p->K = lg;
         } else {
             p->K = 1;   // the triangle channel keeps the triangle plan's exponent

@@ -96,6 +96,7 @@ double lines_per_estimator(const zg_plan *p)
     case ZG_PLAN_THREE_CHAIN: return 2.0;
     case ZG_PLAN_TRIANGLE: case ZG_PLAN_THREE_MOTIF: case ZG_PLAN_K4_CLOSE: case ZG_PLAN_TRIANGLE_K4: return 3.0;
     case ZG_PLAN_K4_SPLIT: case ZG_PLAN_TRIANGLE_K4_SPLIT: return 3.0 + 0.01 * p->desc.completions;   // ~0.1 % of first stages close, ~10 lines per completion
+    case ZG_PLAN_HOUSE_SPLIT: return 3.0 + 0.003 * p->desc.completions;   // ~0.1 % of roofs close, 3 lines per completion
     case ZG_PLAN_FOUR_CHAIN: return 4.5;
     case ZG_PLAN_CHAIN: return 1.0 + 2.3 * (p->desc.order - 2);
     case ZG_PLAN_PATTERN: return 1.0 + 2.1 * (p->desc.pattern_edges - 1);

@@ -315,6 +315,36 @@ __device__ __forceinline__ bool close_pair(const GView &g, uint32_t x, uint32_t 
     return true;
 }
 
+// multiplicity of the pair {x, y}, x != y: the stored copies of the edge (0 if
+// none), from the same bucket chain as close_pair -- one line in the common case
+__device__ __forceinline__ uint32_t close_count(const GView &g, uint32_t x, uint32_t y)
+{
+    const uint64_t key = x < y ? ((uint64_t)x << 32) | y : ((uint64_t)y << 32) | x;
+    const uint64_t h = hmix(key);
+    const uint32_t t = htag_of(h);
+    uint32_t b = hbucket_of(h, g.hnb);
+    uint32_t n = 0, buckets = 0;
+    for (;;) {
+        const uint4 hd = *reinterpret_cast<const uint4 *>(g.htab[b].tag);
+        ++buckets;
+        const uint32_t w[4] = {hd.x, hd.y, hd.z, hd.w};
+        uint32_t match = 0;
+#pragma unroll
+        for (int s = 0; s < kBucketSlots; ++s)
+            match |= (uint32_t)(((w[s >> 1] >> (16 * (s & 1))) & 0x7fffu) == t) << s;
+        while (match) {
+            const int s = __builtin_ctz(match);
+            match &= match - 1;
+            const HSlot sl = g.htab[b].slot[s];
+            n += sl.key_lo == (uint32_t)key && sl.key_hi == (uint32_t)(key >> 32);
+        }
+        if (!(hd.x & kBucketPassed)) break;
+        b = (b + 1 == g.hnb) ? 0 : b + 1;
+    }
+    count_reads(g, buckets);
+    return n;
+}
+
 __device__ __forceinline__ bool close_edge(const GView &g, const VI &x, const VI &y, uint32_t min_valid,
                                            uint32_t *rank)
 {

@@ -752,6 +752,109 @@ __device__ __forceinline__ void k4_split(const GView &g, const PlanView &P, uint
     }
 }
 
+// ---- split 5-house plan (ZG_PLAN_HOUSE_SPLIT) -------------------------------
+// A house (the pattern of applications/FiveHouse.cpp:18-27: roof triangle
+// {0, 1, 2} on the square 1-3-4-2) has exactly one edge in both its triangle
+// and its 4-cycle, the base {1, 2}.  First stage: e0 of rank r0 ~ U[m]; a = its
+// endpoint of smaller degree (the source on ties), b the other; an entry ~
+// U[deg a] of a's list gives the apex w, and the roof closes iff w != b and the
+// pair {w, b} has copies.  For every lane whose roof closes, the wavefront runs
+// that estimator's J completions (stream (seed, TAG_COMPLETION + j, id), lane
+// l taking j = l, l + 64, ...): an entry ~ U[deg a] of a's list gives x, an
+// entry ~ U[deg b] of b's list gives y, counting the copies of {x, y} when
+// x != b, y != a, x != y and w is neither.  value = m deg(a) mult(w, b) deg(a)
+// deg(b) (sum of the counts) / J: its mean is the number of houses, each once
+// per choice of copies of its six edges.
+struct HQ {
+    VI A, B;
+    uint32_t w;
+};
+
+// the first stage; on a closed roof, q and the factor m deg(a) mult(w, b) deg(a) deg(b)
+__device__ __forceinline__ bool house_first(const GView &g, Rng &rng, Est &o, HQ &q, U128 &base, bool &bov)
+{
+    if (g.m == 0) return false;
+    const uint32_t r0 = (uint32_t)rng.uniform(g.m);
+    o.e0 = r0;
+    const ERec E = load_edge(g, r0);
+    VI A = src_vi(E), B = dst_vi(E);
+    if (B.deg < A.deg) { const VI t = A; A = B; B = t; }
+    o.ncond = 1;
+    const uint32_t w = (uint32_t)g.adj[(uint64_t)A.off + rng.uniform(A.deg)];
+    count_reads(g, 1);
+    if (w == B.v) return false;
+    const uint32_t mwb = close_count(g, w, B.v);
+    if (!mwb) return false;
+    q.A = A; q.B = B; q.w = w;
+    base = U128{g.m, 0};
+    bov = mul_u128(base, A.deg);
+    bov |= mul_u128(base, mwb);
+    bov |= mul_u128(base, A.deg);
+    bov |= mul_u128(base, B.deg);
+    return true;
+}
+
+__device__ __forceinline__ uint32_t house_complete(const GView &g, Rng &rc, const HQ &q)
+{
+    const uint32_t x = (uint32_t)g.adj[(uint64_t)q.A.off + rc.uniform(q.A.deg)];
+    const uint32_t y = (uint32_t)g.adj[(uint64_t)q.B.off + rc.uniform(q.B.deg)];
+    count_reads(g, 2);
+    if (x == q.B.v || y == q.A.v || x == y || x == q.w || y == q.w) return 0;
+    return close_count(g, x, y);
+}
+
+__device__ __forceinline__ void house_split(const GView &g, const PlanView &P, uint64_t seed, uint64_t id,
+                                            bool active, Est &o, uint32_t &words)
+{
+    o.st[0] = o.st[1] = 0;
+    o.den[0] = o.den[1] = 0;
+    o.v[0] = o.v[1] = U128{0, 0};
+    o.ncond = 0;
+    o.e0 = 0;
+    Rng rng;
+    rng.init(seed, TAG_ESTIMATOR, id);
+    HQ q;
+    U128 base{0, 0};
+    bool bov = false;
+    const bool closed = active && house_first(g, rng, o, q, base, bov);
+    words = rng.word;
+    const uint32_t lane = threadIdx.x & 63;
+    const int lg = P.K;
+    const uint32_t J = 1u << lg;
+    for (uint64_t pend = __ballot(closed); pend; pend &= pend - 1) {
+        const int l = __builtin_ctzll(pend);
+        HQ t;
+        t.A = lane_vi(q.A, l);
+        t.B = lane_vi(q.B, l);
+        t.w = lane_u32(q.w, l);
+        const uint64_t tid = ((uint64_t)lane_u32((uint32_t)(id >> 32), l) << 32) | lane_u32((uint32_t)id, l);
+        uint64_t n = 0;   // at most J / 64 counts of at most 2^32 - 1 each
+        for (uint32_t j0 = 0; j0 < J; j0 += 64) {
+            if (j0 + lane < J) {
+                Rng rc;
+                rc.init(seed, TAG_COMPLETION + j0 + lane, tid);
+                n += house_complete(g, rc, t);
+            }
+        }
+        // the wavefront's sum in 32-bit digits
+        uint64_t d0 = (uint32_t)n, d1 = n >> 32;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            d0 += __shfl_xor(d0, off, 64);
+            d1 += __shfl_xor(d1, off, 64);
+        }
+        if (lane == (uint32_t)l) {
+            o.ncond = 1 + 2 * J;
+            const uint64_t S = d0 + (d1 << 32);   // d0 < 2^38, d1 < 2^12: S < 2^45
+            if (S) {
+                U128 tot = base;
+                const bool tov = bov | mul_u128(tot, S);
+                set_out(o, 0, tot, tov, lg);
+            }
+        }
+    }
+}
+
 // ---- accumulation ---------------------------------------------------------
 // per lane: a 192-bit running sum (three u64 limbs); per wave and beyond: sums
 // of 32-bit digits, which add without carries
@@ -853,6 +956,19 @@ inline int kernel_kind(const PlanView &P)
     return P.kind == ZG_PLAN_PATTERN ? KIND_PATTERN_BASE + P.pk : P.kind;
 }
 
+// plans whose estimators finish their completions wavefront-cooperatively
+__host__ __device__ constexpr bool split_kind(int kind)
+{
+    return kind == ZG_PLAN_K4_SPLIT || kind == ZG_PLAN_TRIANGLE_K4_SPLIT || kind == ZG_PLAN_HOUSE_SPLIT;
+}
+template <int KIND>
+__device__ __forceinline__ void run_split(const GView &g, const PlanView &P, uint64_t seed, uint64_t id, bool active,
+                                          Est &o, uint32_t &words)
+{
+    if constexpr (KIND == ZG_PLAN_HOUSE_SPLIT) house_split(g, P, seed, id, active, o, words);
+    else k4_split<KIND == ZG_PLAN_TRIANGLE_K4_SPLIT>(g, P, seed, id, active, o, words);
+}
+
 // one estimator: the plan body for estimator id `id`
 template <int KIND>
 __device__ __forceinline__ void run_one(const GView &g, const PlanView &P, Rng &rng, Est &o)
@@ -910,14 +1026,14 @@ __global__ __launch_bounds__(256) void k_run(GView g, PlanView P, uint64_t seed,
             }
         }
     };
-    if constexpr (KIND == ZG_PLAN_K4_SPLIT || KIND == ZG_PLAN_TRIANGLE_K4_SPLIT) {
+    if constexpr (split_kind(KIND)) {
         // wavefront-uniform trip count: every lane reaches the cooperative completions
         const uint32_t lane = threadIdx.x & 63;
         for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < count; b += stride) {
             const uint64_t i = b + lane;
             Est o;
             uint32_t words;
-            k4_split<KIND == ZG_PLAN_TRIANGLE_K4_SPLIT>(g, P, seed, first + i, i < count, o, words);
+            run_split<KIND>(g, P, seed, first + i, i < count, o, words);
             if (i < count) record(i, o, words);
         }
     } else {
@@ -1029,12 +1145,12 @@ __device__ __attribute__((noinline)) void batch_chunk(const BQuery *__restrict__
     rec = uniptr(rec);
     AccL acc[NCH];
     acc_zero<NCH>(acc);
-    if constexpr (KIND == ZG_PLAN_K4_SPLIT || KIND == ZG_PLAN_TRIANGLE_K4_SPLIT) {
+    if constexpr (split_kind(KIND)) {
         for (uint64_t b = base; b < end; b += 64) {   // uniform: every lane reaches the completions
             const uint64_t i = b + lane;
             Est o;
             uint32_t words;
-            k4_split<KIND == ZG_PLAN_TRIANGLE_K4_SPLIT>(g, P, seed, first + i, i < end, o, words);
+            run_split<KIND>(g, P, seed, first + i, i < end, o, words);
             if (i < end) {
 #pragma unroll
                 for (int ch = 0; ch < NCH; ++ch) acc_add(acc[ch], o.v[ch], o.st[ch], o.den[ch], ch == 0 ? P.K : P.K1);
@@ -1102,6 +1218,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         case ZG_PLAN_TRIANGLE_K4: batch_chunk<ZG_PLAN_TRIANGLE_K4, 2>(&Q, base, end, rec); break;
         case ZG_PLAN_K4_SPLIT: batch_chunk<ZG_PLAN_K4_SPLIT, 1>(&Q, base, end, rec); break;
         case ZG_PLAN_TRIANGLE_K4_SPLIT: batch_chunk<ZG_PLAN_TRIANGLE_K4_SPLIT, 2>(&Q, base, end, rec); break;
+        case ZG_PLAN_HOUSE_SPLIT: batch_chunk<ZG_PLAN_HOUSE_SPLIT, 1>(&Q, base, end, rec); break;
         case KIND_PATTERN_BASE + 2: batch_chunk<KIND_PATTERN_BASE + 2, 1>(&Q, base, end, rec); break;
         case KIND_PATTERN_BASE + 3: batch_chunk<KIND_PATTERN_BASE + 3, 1>(&Q, base, end, rec); break;
         case KIND_PATTERN_BASE + 4: batch_chunk<KIND_PATTERN_BASE + 4, 1>(&Q, base, end, rec); break;
@@ -1194,7 +1311,7 @@ static void launch_kind(const zg_graph *g, const PlanView &P, uint64_t seed, uin
 {
     GView v = g->view();
     v.reads = reads;
-    if constexpr (KIND != ZG_PLAN_K4_SPLIT && KIND != ZG_PLAN_TRIANGLE_K4_SPLIT) {   // split plans: id order only
+    if constexpr (!split_kind(KIND)) {   // split plans: id order only
         if (order) {
             hipLaunchKernelGGL((k_run_ordered<KIND, NCH>), dim3(grid), dim3(block), 0, g->stream, v, P, seed, first,
                                count, order, partials);
@@ -1295,8 +1412,7 @@ int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
     }
     const PlanView P = p->view();
     const uint32_t *ord = nullptr;
-    if (order >= 2 && !d_out && !reads && count > 0 && g->m > 0 && p->desc.kind != ZG_PLAN_K4_SPLIT &&
-        p->desc.kind != ZG_PLAN_TRIANGLE_K4_SPLIT) {
+    if (order >= 2 && !d_out && !reads && count > 0 && g->m > 0 && !split_kind(p->desc.kind)) {
         int rc = order_by_first_edge(g, p, seed, first, count, order, &ord);
         if (rc) return rc;
     }
@@ -1320,6 +1436,7 @@ int launch_run(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first, uin
     case ZG_PLAN_TRIANGLE_K4: launch_kind<ZG_PLAN_TRIANGLE_K4, 2>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
     case ZG_PLAN_K4_SPLIT: launch_kind<ZG_PLAN_K4_SPLIT, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
     case ZG_PLAN_TRIANGLE_K4_SPLIT: launch_kind<ZG_PLAN_TRIANGLE_K4_SPLIT, 2>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
+    case ZG_PLAN_HOUSE_SPLIT: launch_kind<ZG_PLAN_HOUSE_SPLIT, 1>(g, P, seed, first, count, d_out, (int)grid, block, reads, ord, partials); break;
     default: return set_error(ZG_EINVAL, "unknown plan kind");
     }
     ZG_HIP(hipGetLastError());

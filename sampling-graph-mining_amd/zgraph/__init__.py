@@ -17,7 +17,8 @@ import time
 import numpy as np
 
 from . import _capi
-from ._capi import (PLAN_CHAIN, PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_K4_SPLIT, PLAN_PATTERN, PLAN_THREE_CHAIN,
+from ._capi import (PLAN_CHAIN, PLAN_FOUR_CHAIN, PLAN_HOUSE_SPLIT, PLAN_K4_CLOSE, PLAN_K4_SPLIT, PLAN_PATTERN,
+                    PLAN_THREE_CHAIN,
                     PLAN_THREE_MOTIF, PLAN_TRIANGLE, PLAN_TRIANGLE_K4, PLAN_TRIANGLE_K4_SPLIT, ZGraphError, check,
                     lib)
 from .formats import load_biedgelist, load_edgelist, load_graph_meta_data, save_biedgelist
@@ -252,8 +253,9 @@ FIVE_HOUSE_PATTERN = Pattern(5, [(0, 1), (0, 2), (1, 2), (1, 3), (2, 4), (3, 4)]
 
 class Plan:
     """A device plan (an estimate() body).  order: ZG_PLAN_CHAIN's path
-    vertices; pattern: ZG_PLAN_PATTERN's pattern; completions: ZG_PLAN_K4_SPLIT's
-    completions per closed triangle (a power of two in [1, 4096])."""
+    vertices; pattern: ZG_PLAN_PATTERN's pattern; completions: the split plans'
+    completions per closed triangle (ZG_PLAN_K4_SPLIT, ZG_PLAN_TRIANGLE_K4_SPLIT)
+    or per closed roof (ZG_PLAN_HOUSE_SPLIT), a power of two in [1, 4096]."""
 
     def __init__(self, kind, order=0, pattern=None, completions=0):
         d = _capi.PlanDesc()
@@ -634,6 +636,18 @@ class FourCliqueSplitCountingInstance(ZGraphInstance):
 
     def __init__(self, graph, completions=64, seed=1):
         super().__init__(graph, Plan(PLAN_K4_SPLIT, completions=completions), seed)
+
+
+class FiveHouseSplitCountingInstance(ZGraphInstance):
+    """5-houses (the pattern of applications/FiveHouse.cpp:18-27), split: a
+    base edge and its roof apex, closed by one pair lookup, then `completions`
+    draws of the square's two other vertices per closed roof, averaged.  Its
+    mean is the house count that SamplerGenerator(FIVE_HOUSE_PATTERN) estimates
+    (each house once per choice of copies of its edges), with far less
+    variance per estimator."""
+
+    def __init__(self, graph, completions=256, seed=1):
+        super().__init__(graph, Plan(PLAN_HOUSE_SPLIT, completions=completions), seed)
 
 
 class SamplerGenerator(ZGraphInstance):

@@ -27,6 +27,7 @@ PLAN_K4_CLOSE = 7
 PLAN_TRIANGLE_K4 = 8
 PLAN_K4_SPLIT = 9
 PLAN_TRIANGLE_K4_SPLIT = 10
+PLAN_HOUSE_SPLIT = 11
 
 MAX_PATTERN_EDGES = 15
 

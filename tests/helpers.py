@@ -122,6 +122,7 @@ PLANS = [
     ("four_chain", dict(kind=5), dict(kind=5)),
     ("k4_close", dict(kind=7), dict(kind=7)),
     ("k4_split", dict(kind=9, completions=64), dict(kind=9, completions=64)),
+    ("house_split", dict(kind=11, completions=16), dict(kind=11, completions=16)),
     ("pattern_triangle", dict(kind=6, pattern="TRIANGLE"), dict(kind=6, pattern="TRIANGLE")),
     ("pattern_k4", dict(kind=6, pattern="K4"), dict(kind=6, pattern="K4")),
     ("pattern_5star", dict(kind=6, pattern="FIVE_STAR"), dict(kind=6, pattern="FIVE_STAR")),
@@ -130,8 +131,8 @@ PLANS = [
 
 
 def oracle_plan(O, kind, order=0, pattern=None, completions=0):
-    # the oracle's split 4-clique plan takes its completions in `order`
-    return O.make_plan(kind, order=completions if kind == O.PLAN_K4_SPLIT else order,
+    # the oracle's split plans take their completions in `order`
+    return O.make_plan(kind, order=completions if kind in (O.PLAN_K4_SPLIT, O.PLAN_HOUSE_SPLIT) else order,
                        pattern=getattr(O, pattern) if pattern else None)
 
 

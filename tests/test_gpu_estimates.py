@@ -78,6 +78,8 @@ def test_three_motif_is_both_halves():
     ("wiki-vote", "four_chain", lambda og: og.paths3(), 2_000_000),
     ("wiki-vote", "k4_close", lambda og: og.k4(), 16_000_000),
     ("wiki-vote", "k4_split", lambda og: og.k4(), 2_000_000),
+    # the house count applications/FiveHouse.cpp:68 records for this fixture
+    ("wiki-vote", "house_split", lambda og: 2134981304, 4_000_000),
     # SamplerGenerator patterns need E-1 lucky conditional samples (K4: about one
     # success per 10^9 estimators on wiki-vote), so they are checked on a dense graph
     ("dense-14", "pattern_k4", lambda og: og.k4(), 64_000_000),

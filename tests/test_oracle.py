@@ -244,7 +244,7 @@ TINY = [H.complete_graph(5), H.random_graph(7, 0.6, 1), H.random_graph(7, 0.5, 2
 
 
 @pytest.mark.parametrize("gi", range(len(TINY)))
-@pytest.mark.parametrize("name,oargs", [(p[0], p[1]) for p in H.PLANS if p[0] != "k4_split"])
+@pytest.mark.parametrize("name,oargs", [(p[0], p[1]) for p in H.PLANS if p[0] not in ("k4_split", "house_split")])
 def test_unbiased_by_exhaustive_expectation(gi, name, oargs):
     n, s, d = TINY[gi]
     for seed in (1, 2):
@@ -307,6 +307,94 @@ def test_split_k4_variance_falls_with_completions():
         var[J] = v.var()
         assert abs(v.mean() - K4) <= 4.5 * math.sqrt(var[J] / N), (J, v.mean(), K4)
     assert var[16] < var[1] / 6 and var[64] < var[1] / 6
+
+
+def _houses_copy_weighted(n, s, d):
+    """Brute force, independent of the estimator's decomposition: every
+    injective map of the house pattern (applications/FiveHouse.cpp:18-27) into
+    the graph, weighted by the product of its six pairs' multiplicities, over
+    the pattern's two automorphisms."""
+    from collections import Counter
+    from itertools import permutations
+    mult = Counter()
+    for a, b in zip(s.tolist(), d.tolist()):
+        mult[(a, b)] += 1
+        mult[(b, a)] += 1
+    pat = [(0, 1), (0, 2), (1, 2), (1, 3), (2, 4), (3, 4)]
+    tot = 0
+    for phi in permutations(range(n), 5):
+        w = 1
+        for u, v in pat:
+            w *= mult[(phi[u], phi[v])]
+            if not w:
+                break
+        tot += w
+    return tot // 2
+
+
+def test_house_brute_force_closed_forms():
+    # K_n holds C(n, 5) * 5! / 2 houses (the pattern has two automorphisms);
+    # doubling every edge of K5 multiplies each house's weight by 2^6
+    for k in (5, 6, 7):
+        n, s, d = H.complete_graph(k)
+        assert _houses_copy_weighted(n, s, d) == math.comb(k, 5) * 60
+        assert O.OGraph(n, s, d).house_brute() == math.comb(k, 5) * 60
+    n, s, d = H.complete_graph(5)
+    assert _houses_copy_weighted(n, np.concatenate([s, d]), np.concatenate([d, s])) == 60 * 64
+    for gi in range(1, len(TINY)):
+        n, s, d = TINY[gi]
+        assert _houses_copy_weighted(n, s, d) == O.OGraph(n, s, d).house_brute()
+
+
+HOUSE_GRAPHS = [H.complete_graph(5), H.random_graph(7, 0.6, 1), H.random_graph(8, 0.5, 2),
+                H.random_graph(8, 0.45, 3), H.multigraph(6, 0.7, 4, max_copies=2, heavy_copies=3)]
+
+
+@pytest.mark.parametrize("gi", range(len(HOUSE_GRAPHS)))
+@pytest.mark.parametrize("J", [1, 2])
+def test_split_house_unbiased_by_exhaustive_expectation(gi, J):
+    # every draw of the base edge, the roof and all J completions enumerated:
+    # the expectation is the copy-weighted house count (the simple count
+    # without parallel edges), for either rank order
+    n, s, d = HOUSE_GRAPHS[gi]
+    want = _houses_copy_weighted(n, s, d)
+    assert want > 0
+    for seed in (1, 2):
+        g = O.OGraph(n, s, d, seed=seed)
+        e = O.expectation(g, O.make_plan(O.PLAN_HOUSE_SPLIT, order=J), 0, max_paths=200_000_000)
+        assert e >= 0, "too many paths"
+        assert e == pytest.approx(want, rel=1e-9)
+
+
+def test_split_house_outcome_fields():
+    # the first stage's words and ncond: 1 after the roof, 1 + 2 J once it
+    # closes; successes carry denominator J
+    n, s, d = H.power_law_graph(400, 3000, 2.1, 5)
+    g = O.OGraph(n, s, d, seed=4)
+    base = O.run_outcomes(g, O.make_plan(O.PLAN_HOUSE_SPLIT, order=1), 9, 100, 60000)
+    for J in (4, 128):
+        o = O.run_outcomes(g, O.make_plan(O.PLAN_HOUSE_SPLIT, order=J), 9, 100, 60000)
+        assert (o["e0"] == base["e0"]).all() and (o["nwords"] == base["nwords"]).all()
+        closed = o["ncond"] > 1
+        assert closed.sum() > 100 and (closed == (base["ncond"] > 1)).all()
+        assert (o["ncond"][closed] == min(1 + 2 * J, 255)).all() and (o["ncond"][~closed] == 1).all()
+        ok = o["status"] == 1
+        assert ok.any() and (o["log2den"][ok] == int(math.log2(J))).all()
+
+
+def test_split_house_reference_ground_truth_on_wiki_vote():
+    # applications/FiveHouse.cpp:68 records 2,134,981,304 houses for the
+    # reference's wiki-vote fixture; 2^20 split estimators put it inside
+    # their 99.9 % interval
+    n, s, d = H.golden_graph("wiki-vote")
+    g = O.OGraph(n, s, d, seed=1)
+    N = 1 << 20
+    r = O.run(g, O.make_plan(O.PLAN_HOUSE_SPLIT, order=64), 5, 0, N, threads=8)[0]
+    mean = O.mean(r)
+    hw = 3.29 * math.sqrt(max(r.sumsq / N - mean * mean, 0.0) / N)
+    assert r.successes > 1000
+    assert abs(mean - 2134981304) <= hw, (mean, hw)
+    assert hw < 0.05 * 2134981304
 
 
 def test_reference_fourclique_shape_alone_is_biased():
