@@ -58,14 +58,16 @@ def main():
                     help="largest budget of one time-to-error query, log2 estimators (2^39 = 5.5e11: well "
                          "above the 1.2-1.5e11 the learned sizing asks for a +-1%% 4-clique answer on the bench "
                          "graph, so no query is capped)")
-    ap.add_argument("--workload", default="chunglu69m", choices=["chunglu69m", "rmat26", "sizes", "elp", "exact"],
+    ap.add_argument("--workload", default="chunglu69m",
+                    choices=["chunglu69m", "rmat26", "sizes", "elp", "exact", "house"],
                     help="chunglu69m: the headline triangle + 4-clique workload; rmat26: R-MAT scale 26, "
                          "edge factor 16 with the 3-motif, 3-chain, 4-chain and predicate-triangle plans; "
                          "sizes: 3-chain, triangle and 4-clique on R-MAT 22/24/26 and Chung-Lu 69M/500M "
                          "(how the rate holds as the graph outgrows the translation caches); elp: time to "
                          "+-1%% for triangle, 4-clique, 3-chain and 4-chain on four graphs, learned against "
                          "regression sizing; exact: the device exact counters on R-MAT 20 and 22, with the "
-                         "CPU oracle's triangle and 4-clique counts beside them")
+                         "CPU oracle's triangle and 4-clique counts beside them; house: time to +-1%% for 5-houses, "
+                         "SamplerGenerator against the split house plan, on three graphs")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this size (rocprof counter calibration)")
     ap.add_argument("--line-sweep", action="store_true",
@@ -131,6 +133,9 @@ def main():
         return
     if args.workload == "elp":
         elp_sweep(Z, args)
+        return
+    if args.workload == "house":
+        house_sweep(Z, args)
         return
     if args.workload == "exact":
         exact_sweep(Z, args)
@@ -333,7 +338,7 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95, patterns=None, cap_log2=None,
         arms = entry[3] if len(entry) > 3 else None
         make_plan = kind if callable(kind) else (lambda k=kind: Z.Plan(k))
         t = time.perf_counter()
-        truth = g.exact_count(what)
+        truth = exact_count(g, what)
         t_exact = time.perf_counter() - t
         log(f"exact {name}: {truth} in {t_exact:.2f}s")
         res = {"exact": truth, "exact_count_s": t_exact}
@@ -415,20 +420,34 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95, patterns=None, cap_log2=None,
     return out
 
 
-def split_auto_arm(Z, L, g, prof, eps, conf, queries, cap_log2):
-    """Time to +-eps for 4-cliques by the split plan whose completions per
-    closed triangle are chosen by elp.choose_split_completions (two pilots,
-    their time included), then sized by `prof` and judged."""
-    truth = g.exact_count("k4")
-    L.split_k4_to_error(g, eps, conf, prof, seed=100, max_estimators=1 << 30)   # untimed first query
+_EXACT = {}
+
+
+def exact_count(g, what):
+    """The device exact count, computed once per graph object and pattern (the
+    cache holds the object, so its id is not reused)."""
+    key = (id(g), what)
+    if key not in _EXACT:
+        _EXACT[key] = (g, g.exact_count(what))
+    return _EXACT[key][1]
+
+
+def split_auto_arm(Z, L, g, prof, eps, conf, queries, cap_log2, kind=None, what="k4"):
+    """Time to +-eps by a split plan (4-cliques by default; 5-houses with
+    kind=PLAN_HOUSE_SPLIT, what="house") whose completions per closed first
+    stage are chosen by elp.choose_split_completions (two pilots, their time
+    included), then sized by `prof` and judged."""
+    kind = kind or Z.PLAN_K4_SPLIT
+    truth = exact_count(g, what)
+    L.split_to_error(g, eps, conf, prof, kind=kind, seed=100, max_estimators=1 << 30)   # untimed first query
     recs = []
     for q in range(queries):
-        r = L.split_k4_to_error(g, eps, conf, prof, seed=101 + q, max_estimators=1 << cap_log2)
+        r = L.split_to_error(g, eps, conf, prof, kind=kind, seed=101 + q, max_estimators=1 << cap_log2)
         err = abs(r["estimate"] - truth) / truth if truth and r["estimators"] else float("nan")
         recs.append({"seed": 101 + q, "completions": r["completions"], "estimators": r["estimators"],
                      "choice_s": r["choice_s"], "total_s": r["total_s"], "error": err, "accepted": r["accepted"],
                      "sizing": r["sizing"]})
-        log(f"4-clique split, completions chosen, q{q}: {r['total_s'] * 1e3:.1f} ms (choice "
+        log(f"{what} split, completions chosen, q{q}: {r['total_s'] * 1e3:.1f} ms (choice "
             f"{r['choice_s'] * 1e3:.1f} ms, {r['completions']} completions) N={r['estimators']} err={err * 100:.3f}%")
     ok = [r for r in recs if r["estimators"]]
     return {"mean_time_s": sum(r["total_s"] for r in ok) / len(ok) if ok else None,
@@ -585,6 +604,38 @@ def elp_sweep(Z, args):
             rows[gname]["coverage"] = elp_coverage(Z, g, args, cov)
         g.close()
     print(json.dumps({"elp_sweep": rows}), flush=True)
+
+
+def house_sweep(Z, args):
+    """Time to +-1 % at 95 % for 5-houses (the pattern of the reference's
+    FiveHouse application) on three seeded graphs whose exact house counts the
+    device counter finds in seconds (Chung-Lu 200 k / 2 M with beta 2.1,
+    Chung-Lu 1 M / 10 M with beta 2.5, R-MAT scale 16): the reference's
+    SamplerGenerator house estimator against the split house plan with 256
+    completions per closed roof and with its completions chosen by two pilots;
+    regression and pilot sizing with the judge, 3 queries each, budgets capped
+    at 2^34, errors against the exact counts."""
+    graphs = [("chunglu 200k/2M b2.1", lambda: Z.chung_lu(200_000, 2_000_000, 2.1, 2000.0, args.seed)),
+              ("chunglu 1M/10M b2.5", lambda: Z.chung_lu(1_000_000, 10_000_000, 2.5, 2000.0, args.seed)),
+              ("rmat16", lambda: Z.rmat(16, 16 << 16, 0.57, 0.19, 0.19, args.seed))]
+    J = args.split_completions
+    arms = ("regression+judge", "pilot+judge")
+    patterns = (("5-house, SamplerGenerator", lambda: Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_HOUSE_PATTERN), "house",
+                 arms),
+                (f"5-house split x{J}", lambda: Z.Plan(Z.PLAN_HOUSE_SPLIT, completions=J), "house", arms))
+    from zgraph import elp as L
+    prof = L.LearnedErrorProfile(args.elp_weights or L.DEFAULT_WEIGHTS)
+    rows = {}
+    for gname, make in graphs:
+        e = make()
+        g = Z.Graph.from_device(e, seed=args.seed)
+        e.free()
+        log(f"house sweep {gname}: m={g.num_local_edges} maxdeg={g.max_degree}")
+        rows[gname] = time_to_error(Z, g, args, patterns=patterns, cap_log2=34, queries=3)
+        rows[gname]["5-house split, completions chosen"] = split_auto_arm(
+            Z, L, g, prof, 0.01, 0.95, 3, 34, kind=Z.PLAN_HOUSE_SPLIT, what="house")
+        g.close()
+    print(json.dumps({"house_sweep": rows}), flush=True)
 
 
 def elp_coverage(Z, g, args, patterns, queries=40, eps=0.01, conf=0.95):

@@ -556,7 +556,7 @@ class Plan {
         return d;
     }
     // ZG_PLAN_K4_SPLIT (or, fused with the triangle channel, ZG_PLAN_TRIANGLE_K4_SPLIT)
-    // with `completions` per closed triangle
+    // with `completions` per closed triangle; also ZG_PLAN_HOUSE_SPLIT (per closed roof)
     static zg_plan_desc split_k4(int completions, int kind = ZG_PLAN_K4_SPLIT)
     {
         zg_plan_desc d = of(kind);
@@ -766,6 +766,20 @@ class FourCliqueSplitCountingInstance : public ZGraphInstance<EdgeData> {
     FourCliqueSplitCountingInstance(Graph<EdgeData> *g, int completions = 64) : ZGraphInstance<EdgeData>(g)
     {
         this->set_plan(Plan::split_k4(completions));
+    }
+};
+
+// 5-houses (the pattern of applications/FiveHouse.cpp:18-27), split: a base
+// edge and its roof apex, closed by one pair lookup, then `completions` draws
+// of the square's two other vertices per closed roof, averaged -- the house
+// count SamplerGenerator estimates, with far less variance per estimator (no
+// reference counterpart)
+template <typename EdgeData>
+class FiveHouseSplitCountingInstance : public ZGraphInstance<EdgeData> {
+  public:
+    FiveHouseSplitCountingInstance(Graph<EdgeData> *g, int completions = 256) : ZGraphInstance<EdgeData>(g)
+    {
+        this->set_plan(Plan::split_k4(completions, ZG_PLAN_HOUSE_SPLIT));
     }
 };
 
@@ -1157,67 +1171,88 @@ class LearnedErrorProfile {
     ErrorProfile<EdgeData> fallback_;
 };
 
-// ---- completions of the split 4-clique plan (no reference counterpart) ------
-// Two pilots over the same ids (from first_id on, away from a query's own)
-// with j_lo and j_hi completions per closed triangle, each at least `pilot`
-// ids and grown 4x until it holds min_successes successes or max_pilot ids.
-// Per estimator Var(J) = Vb + Vw / J and the kernel time t(J) = c0 + c1 J, so
-// the time to a given error, proportional to Var(J) t(J), is least at
-// J* = sqrt(Vw c0 / (Vb c1)); the choice is the nearest power of two in
-// [16, 4096] (16 when Vw <= 0, 4096 when Vb <= 0 or c1 <= 0).  fused: size the
-// 4-clique channel of ZG_PLAN_TRIANGLE_K4_SPLIT.  Same rule as
-// zgraph.elp.choose_split_completions.
+// ---- completions of the split plans (no reference counterpart) -------------
+// Completions for a split plan (ZG_PLAN_K4_SPLIT, the 4-clique channel of
+// ZG_PLAN_TRIANGLE_K4_SPLIT, ZG_PLAN_HOUSE_SPLIT), from two pilots over the
+// same ids (from first_id on, away from a query's own) with j_lo and j_hi
+// completions.  The j_lo pilot starts at `pilot` ids and grows (at least 4x,
+// more when far short) until it holds min_successes successes and its kernel
+// runs min_ms milliseconds (a launch-independent time), or reaches max_pilot
+// ids.  If a +-eps answer at `conf` with j_lo completions, projected from that
+// pilot, would take less time than the j_hi pilot can (j_hi / j_lo times the
+// j_lo pilot's), the choice is j_lo (eps <= 0 turns this off); otherwise the
+// j_hi pilot runs the same ids.  Per estimator Var(J) = Vb + Vw / J
+// and the kernel time t(J) = c0 + c1 J, so the time to a given error,
+// proportional to Var(J) t(J), is least at J* = sqrt(Vw c0 / (Vb c1)); the
+// choice is the nearest power of two in [1, 4096] (1 when Vw <= 0, 4096 when
+// Vb <= 0 or c1 <= 0).  Same rule as zgraph.elp.choose_split_completions.
 struct SplitChoice {
     int completions = 0;
     double j_star = 0, v_between = 0, v_within = 0, c0_ms_per_id = 0, c1_ms_per_id_per_completion = 0,
            seconds = 0;
-    unsigned long long pilot_ids[2] = {0, 0}, pilot_successes[2] = {0, 0};
+    unsigned long long pilot_ids = 0, pilot_successes[2] = {0, 0};
     double pilot_var[2] = {0, 0};
+    bool hi_pilot_skipped = false;
+    double projected_ms = 0;
 };
 
 template <typename EdgeData>
-SplitChoice choose_split_completions(Graph<EdgeData> *g, uint64_t seed = 1, bool fused = false, int j_lo = 64,
-                                     int j_hi = 1024, unsigned long long min_successes = 32,
-                                     unsigned long long pilot = 1ull << 22, unsigned long long max_pilot = 1ull << 30,
-                                     uint64_t first_id = 1ull << 60)
+SplitChoice choose_split_completions(Graph<EdgeData> *g, uint64_t seed = 1, int kind = ZG_PLAN_K4_SPLIT, int j_lo = 64,
+                                     int j_hi = 1024, unsigned long long min_successes = 32, double min_ms = 1.0,
+                                     unsigned long long pilot = 1ull << 18, unsigned long long max_pilot = 1ull << 30,
+                                     uint64_t first_id = 1ull << 60, double eps = 0.01, double conf = 0.95)
 {
+    if (kind != ZG_PLAN_K4_SPLIT && kind != ZG_PLAN_TRIANGLE_K4_SPLIT && kind != ZG_PLAN_HOUSE_SPLIT)
+        throw Error(ZG_EINVAL, "choose_split_completions: not a split plan kind");
     const auto t0 = std::chrono::steady_clock::now();
-    const int kind = fused ? ZG_PLAN_TRIANGLE_K4_SPLIT : ZG_PLAN_K4_SPLIT, ch = fused ? 1 : 0;
+    const int ch = kind == ZG_PLAN_TRIANGLE_K4_SPLIT ? 1 : 0;
+    Plan lo(Plan::split_k4(j_lo, kind)), hi(Plan::split_k4(j_hi, kind));
+    zg_launch launch{0, 0, 0};
+    zg_sums s[2];
+    for (const Plan *p : {&lo, &hi})   // code objects loaded before timing
+        check(zg_run(g->handle(), p->handle(), seed, first_id, 1ull << 12, &launch, s, nullptr));
     SplitChoice c;
-    double ms_per_id[2] = {0, 0};
-    const int js[2] = {j_lo, j_hi};
-    for (int i = 0; i < 2; ++i) {
-        Plan plan(Plan::split_k4(js[i], kind));
-        zg_launch launch{0, 0, 0};
-        zg_sums s[2];
-        unsigned long long n = pilot;
-        for (;;) {
-            // code objects loaded before timing
-            check(zg_run(g->handle(), plan.handle(), seed, first_id, std::min(n, 1ull << 16), &launch, s, nullptr));
-            float ms = 0;
-            check(zg_run(g->handle(), plan.handle(), seed, first_id, n, &launch, s, &ms));
-            if (s[ch].successes >= min_successes || n >= max_pilot) {
-                ms_per_id[i] = ms / (double)n;
-                break;
-            }
-            n = std::min(n * 4, max_pilot);
-        }
-        const double mean = s[ch].mean;
-        c.pilot_ids[i] = n;
-        c.pilot_successes[i] = s[ch].successes;
-        c.pilot_var[i] = std::max(s[ch].sumsq / (double)n - mean * mean, 0.0);
+    unsigned long long n = pilot;
+    float ms = 0;
+    for (;;) {
+        check(zg_run(g->handle(), lo.handle(), seed, first_id, n, &launch, s, &ms));
+        if ((s[ch].successes >= min_successes && ms >= min_ms) || n >= max_pilot) break;
+        const double short_by = std::max((double)min_successes / (double)std::max<uint64_t>(s[ch].successes, 1),
+                                         min_ms / std::max((double)ms, 1e-3));
+        n = std::min((unsigned long long)((double)n * std::min(std::max(1.5 * short_by, 4.0), 256.0)), max_pilot);
     }
+    double ms_per_id[2];
+    auto take = [&](int i, const zg_sums &x, float t) {
+        const double mean = x.mean;
+        c.pilot_successes[i] = x.successes;
+        c.pilot_var[i] = std::max(x.sumsq / (double)n - mean * mean, 0.0);
+        ms_per_id[i] = t / (double)n;
+    };
+    take(0, s[ch], ms);
+    c.pilot_ids = n;
+    if (eps > 0 && s[ch].mean > 0) {
+        const double z = normal_quantile_two_sided(conf), mu = s[ch].mean;
+        c.projected_ms = z * z * c.pilot_var[0] / (eps * mu * eps * mu) * ms_per_id[0];
+        if (c.projected_ms < (double)ms * j_hi / j_lo) {
+            c.hi_pilot_skipped = true;
+            c.completions = j_lo;
+            c.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            return c;
+        }
+    }
+    check(zg_run(g->handle(), hi.handle(), seed, first_id, n, &launch, s, &ms));
+    take(1, s[ch], ms);
     c.v_within = (c.pilot_var[0] - c.pilot_var[1]) / (1.0 / j_lo - 1.0 / j_hi);
     c.v_between = c.pilot_var[1] - c.v_within / j_hi;
     c.c1_ms_per_id_per_completion = (ms_per_id[1] - ms_per_id[0]) / (j_hi - j_lo);
     c.c0_ms_per_id = ms_per_id[0] - c.c1_ms_per_id_per_completion * j_lo;
-    if (c.v_within <= 0) c.j_star = 16;
+    if (c.v_within <= 0) c.j_star = 1;
     else if (c.v_between <= 0 || c.c1_ms_per_id_per_completion <= 0) c.j_star = 4096;
     else
         c.j_star = std::sqrt(c.v_within * std::max(c.c0_ms_per_id, 1e-12) /
                              (c.v_between * c.c1_ms_per_id_per_completion));
     const double l = std::nearbyint(std::log2(std::max(c.j_star, 1.0)));
-    c.completions = (int)std::min(4096.0, std::max(16.0, std::exp2(l)));
+    c.completions = (int)std::min(4096.0, std::max(1.0, std::exp2(l)));
     c.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return c;
 }
@@ -1239,6 +1274,7 @@ using zgraph::Estimation;
 using zgraph::FourChainCountingInstance;
 using zgraph::FourCliqueCountingInstance;
 using zgraph::FourCliqueSplitCountingInstance;
+using zgraph::FiveHouseSplitCountingInstance;
 using zgraph::get_globally_consistent_seed;
 using zgraph::Graph;
 using zgraph::MyList;

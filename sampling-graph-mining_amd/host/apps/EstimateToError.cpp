@@ -2,7 +2,9 @@
 // +-eps at 95 % confidence: argv[2] pattern (triangle, 4-clique, 3-chain,
 // 4-chain, 4-clique-split -- the split 4-clique plan with 256 completions per
 // closed triangle, sized from its pilot; 4-clique-split-auto -- the same with
-// its completions chosen by two pilots; or the two-channel 3-motif,
+// its completions chosen by two pilots; 5-house-split and 5-house-split-auto
+// -- the split 5-house plan, 256 or chosen completions per closed roof; or the
+// two-channel 3-motif,
 // triangle+4-clique and triangle+4-clique-split, both channels to +-eps over
 // one id range), argv[3] eps
 // (default 0.01), argv[4] estimator seed, argv[5] the weights file (default:
@@ -53,11 +55,16 @@ int main(int argc, char **argv)
         else if (pat == "3-chain") inst.reset(new ThreeChainCountingInstance<Empty>(&graph));
         else if (pat == "4-chain") inst.reset(new FourChainCountingInstance<Empty>(&graph));
         else if (pat == "4-clique-split") inst.reset(new FourCliqueSplitCountingInstance<Empty>(&graph, 256));
-        else if (pat == "4-clique-split-auto") {
-            const zgraph::SplitChoice c = zgraph::choose_split_completions(&graph, seed);
-            std::printf("completions %d %.17g %.17g %llu %llu\n", c.completions, c.v_between, c.v_within,
-                        c.pilot_ids[0], c.pilot_ids[1]);
-            inst.reset(new FourCliqueSplitCountingInstance<Empty>(&graph, c.completions));
+        else if (pat == "5-house-split") inst.reset(new FiveHouseSplitCountingInstance<Empty>(&graph, 256));
+        else if (pat == "4-clique-split-auto" || pat == "5-house-split-auto") {
+            const bool house = pat == "5-house-split-auto";
+            const zgraph::SplitChoice c = zgraph::choose_split_completions(
+                &graph, seed, house ? ZG_PLAN_HOUSE_SPLIT : ZG_PLAN_K4_SPLIT, 64, 1024, 32, 1.0, 1ull << 18,
+                1ull << 30, 1ull << 60, eps, 0.95);
+            std::printf("completions %d %d %.17g %.17g %llu\n", c.completions, c.hi_pilot_skipped ? 1 : 0,
+                        c.v_between, c.v_within, c.pilot_ids);
+            if (house) inst.reset(new FiveHouseSplitCountingInstance<Empty>(&graph, c.completions));
+            else inst.reset(new FourCliqueSplitCountingInstance<Empty>(&graph, c.completions));
         }
         else throw zgraph::Error(ZG_EINVAL, "unknown pattern " + pat);
         inst->seed = seed;

@@ -32,8 +32,8 @@ import time
 
 import numpy as np
 
-from . import (PLAN_FOUR_CHAIN, PLAN_K4_CLOSE, PLAN_K4_SPLIT, PLAN_THREE_CHAIN, PLAN_THREE_MOTIF, PLAN_TRIANGLE,
-               PLAN_TRIANGLE_K4, PLAN_TRIANGLE_K4_SPLIT, ErrorProfile, Estimation, ZGraphError, _z, run)
+from . import (PLAN_FOUR_CHAIN, PLAN_HOUSE_SPLIT, PLAN_K4_CLOSE, PLAN_K4_SPLIT, PLAN_THREE_CHAIN, PLAN_THREE_MOTIF,
+               PLAN_TRIANGLE, PLAN_TRIANGLE_K4, PLAN_TRIANGLE_K4_SPLIT, ErrorProfile, Estimation, ZGraphError, _z, run)
 
 IN_DIM = 16
 PATTERN_SLOTS = {PLAN_TRIANGLE: 0, PLAN_K4_CLOSE: 1, PLAN_THREE_CHAIN: 2, PLAN_FOUR_CHAIN: 3}
@@ -352,62 +352,96 @@ def estimate_both_to_error(instance, eps, conf, profile, max_estimators=1 << 40,
             "sizing_s": t_size, "total_s": t1 - t0}
 
 
-def choose_split_completions(graph, seed=1, j_lo=64, j_hi=1024, min_successes=32, pilot=1 << 22,
-                             max_pilot=1 << 30, first_id=1 << 60, fused=False):
-    """Completions per closed triangle for the split 4-clique plan, from two
-    pilots of the same ids with j_lo and j_hi completions (each at least
-    `pilot` ids, enough for the kernel times to be launch-independent, grown
-    4x until it holds `min_successes` successes or `max_pilot` ids).  Per estimator,
+def choose_split_completions(graph, seed=1, kind=PLAN_K4_SPLIT, j_lo=64, j_hi=1024, min_successes=32,
+                             min_ms=1.0, pilot=1 << 18, max_pilot=1 << 30, first_id=1 << 60, eps=0.01, conf=0.95):
+    """Completions for a split plan (ZG_PLAN_K4_SPLIT, the 4-clique channel of
+    ZG_PLAN_TRIANGLE_K4_SPLIT, or ZG_PLAN_HOUSE_SPLIT), from two pilots over
+    the same ids with j_lo and j_hi completions.  The j_lo pilot starts at
+    `pilot` ids and grows (at least 4x, more when far short) until it holds
+    `min_successes` successes and its kernel runs `min_ms` milliseconds --
+    long enough for a launch-independent time -- or reaches `max_pilot` ids.
+    If a +-eps answer at `conf` with j_lo completions, projected from that
+    pilot, would take less time than the j_hi pilot can (j_hi / j_lo times the
+    j_lo pilot's), the choice is j_lo: it could not repay its own cost
+    (eps=None turns this off).  Otherwise the j_hi pilot runs the same ids
+    (its completions include the j_lo pilot's).  Per estimator
     Var(J) = Vb + Vw / J (Vb: the first stage's share, Vw: the completions')
     and the kernel time t(J) = c0 + c1 J; the time to a given error is
     proportional to Var(J) t(J), least at J* = sqrt(Vw c0 / (Vb c1)), returned
-    as the nearest power of two in [16, 4096] (16 when Vw <= 0, 4096 when
-    Vb <= 0).  The pilots run ids from `first_id` on, away from the query's
-    own.  fused: size the 4-clique channel of ZG_PLAN_TRIANGLE_K4_SPLIT.
-    Returns (J, record)."""
-    from . import PLAN_K4_SPLIT as _SPLIT, PLAN_TRIANGLE_K4_SPLIT as _FUSED, Plan
-    kind, ch = (_FUSED, 1) if fused else (_SPLIT, 0)
+    as the nearest power of two in [1, 4096] (1 when Vw <= 0, 4096 when
+    Vb <= 0 or c1 <= 0).  The pilots run ids from `first_id` on, away from the
+    query's own.  Returns (J, record)."""
+    from . import Plan
+    if kind not in (PLAN_K4_SPLIT, PLAN_TRIANGLE_K4_SPLIT, PLAN_HOUSE_SPLIT):
+        raise ValueError("not a split plan kind")
+    ch = 1 if kind == PLAN_TRIANGLE_K4_SPLIT else 0
     t0 = time.perf_counter()
-    stats = {}
-    for J in (j_lo, j_hi):
-        plan = Plan(kind, completions=J)
-        n = pilot
-        while True:
-            run(graph, plan, seed, first_id, min(n, 1 << 16))   # code objects loaded before timing
-            sums, ms = run(graph, plan, seed, first_id, n, timed=True)
-            s = sums[ch]
-            if s.successes >= min_successes or n >= max_pilot:
-                break
-            n = min(n * 4, max_pilot)
-        mean = s.mean
-        stats[J] = {"ids": n, "successes": s.successes, "mean": mean,
-                    "var": max(s.sumsq / n - mean * mean, 0.0), "ms_per_id": ms / n}
+    plans = {J: Plan(kind, completions=J) for J in (j_lo, j_hi)}
+    for p in plans.values():
+        run(graph, p, seed, first_id, 1 << 12)   # code objects loaded before timing
+    n = pilot
+    while True:
+        sums, ms = run(graph, plans[j_lo], seed, first_id, n, timed=True)
+        s = sums[ch]
+        if (s.successes >= min_successes and ms >= min_ms) or n >= max_pilot:
+            break
+        short = max(min_successes / max(s.successes, 1), min_ms / max(ms, 1e-3))
+        n = min(int(n * min(max(1.5 * short, 4.0), 256.0)), max_pilot)
+    stats = {j_lo: split_pilot_record(s, n, ms)}
+    if eps is not None and s.mean > 0:
+        z = _z(conf)
+        projected_ms = (z * z * stats[j_lo]["var"] / (eps * s.mean) ** 2) * stats[j_lo]["ms_per_id"]
+        if projected_ms < ms * j_hi / j_lo:
+            return j_lo, {"completions": j_lo, "hi_pilot_skipped": True, "projected_ms": projected_ms,
+                          "pilots": stats, "seconds": time.perf_counter() - t0}
+    sums, ms = run(graph, plans[j_hi], seed, first_id, n, timed=True)
+    stats[j_hi] = split_pilot_record(sums[ch], n, ms)
+    rec = fit_split_model(stats, j_lo, j_hi)
+    rec["hi_pilot_skipped"] = False
+    rec["pilots"] = stats
+    rec["seconds"] = time.perf_counter() - t0
+    return rec["completions"], rec
+
+
+def split_pilot_record(sums, n, ms):
+    """A split pilot's record from its channel sums over n ids and its kernel ms."""
+    mean = sums.mean
+    return {"ids": n, "successes": sums.successes, "mean": mean, "var": max(sums.sumsq / n - mean * mean, 0.0),
+            "ms_per_id": ms / n}
+
+
+def fit_split_model(stats, j_lo, j_hi):
+    """Var(J) = Vb + Vw / J and t(J) = c0 + c1 J through the two pilots
+    (stats[j]: split_pilot_record), and the completions that minimise Var(J) t(J):
+    the power of two nearest J* = sqrt(Vw c0 / (Vb c1)) in [1, 4096]."""
     a, b = stats[j_lo], stats[j_hi]
     vw = (a["var"] - b["var"]) / (1.0 / j_lo - 1.0 / j_hi)
     vb = b["var"] - vw / j_hi
     c1 = (b["ms_per_id"] - a["ms_per_id"]) / (j_hi - j_lo)
     c0 = a["ms_per_id"] - c1 * j_lo
     if vw <= 0:
-        jstar = 16.0
+        jstar = 1.0
     elif vb <= 0 or c1 <= 0:
         jstar = 4096.0
     else:
         jstar = math.sqrt(vw * max(c0, 1e-12) / (vb * c1))
-    J = int(min(4096, max(16, 2 ** round(math.log2(max(jstar, 1.0))))))
-    rec = {"completions": J, "j_star": jstar, "v_between": vb, "v_within": vw, "c0_ms_per_id": c0,
-           "c1_ms_per_id_per_completion": c1, "pilots": stats, "seconds": time.perf_counter() - t0}
-    return J, rec
+    J = int(min(4096, max(1, 2 ** round(math.log2(max(jstar, 1.0))))))
+    return {"completions": J, "j_star": jstar, "v_between": vb, "v_within": vw, "c0_ms_per_id": c0,
+            "c1_ms_per_id_per_completion": c1}
 
 
-def split_k4_to_error(graph, eps, conf, profile, seed=1, max_estimators=1 << 40, **choose):
-    """4-cliques to +-eps at `conf` by the split plan with its completions
+def split_to_error(graph, eps, conf, profile, kind=PLAN_K4_SPLIT, seed=1, max_estimators=1 << 40, **choose):
+    """A count to +-eps at `conf` by a one-channel split plan (4-cliques:
+    ZG_PLAN_K4_SPLIT; 5-houses: ZG_PLAN_HOUSE_SPLIT) with its completions
     chosen by choose_split_completions (pilots on ids away from the query's),
     then sized by `profile` and judged as estimate_to_error does.  The record
     adds the completions, the choice's record and its time (in total_s)."""
-    from . import FourCliqueSplitCountingInstance
+    from . import Plan, ZGraphInstance
+    if kind not in (PLAN_K4_SPLIT, PLAN_HOUSE_SPLIT):
+        raise ValueError("not a one-channel split plan kind")
     t0 = time.perf_counter()
-    J, choice = choose_split_completions(graph, seed=seed, **choose)
-    inst = FourCliqueSplitCountingInstance(graph, completions=J, seed=seed)
+    J, choice = choose_split_completions(graph, seed=seed, kind=kind, eps=eps, conf=conf, **choose)
+    inst = ZGraphInstance(graph, Plan(kind, completions=J), seed=seed)
     rec = estimate_to_error(inst, eps, conf, profile, max_estimators=max_estimators)
     rec["completions"] = J
     rec["choice"] = choice

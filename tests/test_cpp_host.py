@@ -148,7 +148,8 @@ def test_predicate_apps(tmp_path):
 @pytest.mark.gpu
 @pytest.mark.parametrize("pattern,kind,completions,sizing", [("triangle", Z.PLAN_TRIANGLE, 0, "learned"),
                                                              ("4-chain", Z.PLAN_FOUR_CHAIN, 0, "learned"),
-                                                             ("4-clique-split", Z.PLAN_K4_SPLIT, 256, "pilot")])
+                                                             ("4-clique-split", Z.PLAN_K4_SPLIT, 256, "pilot"),
+                                                             ("5-house-split", Z.PLAN_HOUSE_SPLIT, 256, "pilot")])
 def test_learned_profile_cpp_equals_python(tmp_path, pattern, kind, completions, sizing):
     """The C++ LearnedErrorProfile sizes and judges a query as zgraph.elp does
     (the split 4-clique plan, which the network has no slot for, from its
@@ -175,7 +176,9 @@ def test_learned_profile_cpp_equals_python(tmp_path, pattern, kind, completions,
 
 
 @pytest.mark.gpu
-def test_split_completions_choice_cpp_equals_python(tmp_path):
+@pytest.mark.parametrize("pattern,kind", [("4-clique-split-auto", Z.PLAN_K4_SPLIT),
+                                          ("5-house-split-auto", Z.PLAN_HOUSE_SPLIT)])
+def test_split_completions_choice_cpp_equals_python(tmp_path, pattern, kind):
     """zgraph::choose_split_completions fits the same variance model from the
     same pilots as zgraph.elp.choose_split_completions (the kernel times, and
     so the choice itself, may differ between runs), and the query it sizes
@@ -184,22 +187,30 @@ def test_split_completions_choice_cpp_equals_python(tmp_path):
     n, s, d = golden_graph("wiki-vote")
     prefix = _write_graph(tmp_path, "wv", n, s, d)
     eps, seed = 0.02, 13
-    print("[cpp] EstimateToError 4-clique-split-auto", flush=True)
-    r = subprocess.run([_bin("EstimateToError"), prefix, "4-clique-split-auto", str(eps), str(seed)],
+    print(f"[cpp] EstimateToError {pattern}", flush=True)
+    r = subprocess.run([_bin("EstimateToError"), prefix, pattern, str(eps), str(seed)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    m = re.search(r"^completions (\d+) (\S+) (\S+) (\d+) (\d+)$", r.stdout, re.M)
-    J, vb, vw, n0, n1 = int(m.group(1)), float(m.group(2)), float(m.group(3)), int(m.group(4)), int(m.group(5))
-    assert J in [1 << k for k in range(4, 13)]
+    m = re.search(r"^completions (\d+) (\d) (\S+) (\S+) (\d+)$", r.stdout, re.M)
+    J, skipped, vb, vw, ids = int(m.group(1)), m.group(2) == "1", float(m.group(3)), float(m.group(4)), int(m.group(5))
+    assert J in [1 << k for k in range(13)]
     judged = re.search(r"^judged (\d) (\S+)$", r.stdout, re.M)
     assert judged.group(1) == "1" and float(judged.group(2)) <= eps
     g = Z.Graph.from_edges(s, d, n, seed=1)
-    _, rec = L.choose_split_completions(g, seed=seed)
-    assert (rec["pilots"][64]["ids"], rec["pilots"][1024]["ids"]) == (n0, n1)
-    assert vb == pytest.approx(rec["v_between"], rel=1e-9)
-    assert vw == pytest.approx(rec["v_within"], rel=1e-9)
-    truth = g.exact_count("k4")
-    est, _ = _report(r.stdout)["4-clique-split-auto"]
+    # the pilots' sizes (and whether the second one runs) depend on kernel
+    # times; refit the C++ pilots' ids in Python
+    stats = {j: L.split_pilot_record(Z.run(g, Z.Plan(kind, completions=j), seed, 1 << 60, ids)[0], ids, 1.0)
+             for j in (64, 1024)}
+    assert stats[64]["successes"] >= 32
+    if skipped:
+        assert J == 64
+    else:
+        rec = L.fit_split_model(stats, 64, 1024)
+        assert vb == pytest.approx(rec["v_between"], rel=1e-9)
+        assert vw == pytest.approx(rec["v_within"], rel=1e-9)
+    # 4-cliques: the exact count; houses: the reference's own (FiveHouse.cpp:68)
+    truth = g.exact_count("k4") if kind == Z.PLAN_K4_SPLIT else 2134981304
+    est, _ = _report(r.stdout)[pattern]
     assert abs(est - truth) <= 2 * float(judged.group(2)) * est
 
 

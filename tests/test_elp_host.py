@@ -163,3 +163,25 @@ def test_fallback_for_untrained_plans_and_missing_weights():
     assert how == "pilot" and n == math.ceil((1.959963984540054 * cv / 0.01) ** 2)
     rec = L.estimate_to_error(FakeInstance(Z.PLAN_K4_SPLIT), 0.05, 0.95, prof)
     assert rec["sizing"] == "pilot" and rec["accepted"]
+
+
+def test_split_model_fit_recovers_the_model_and_minimises_time_to_error():
+    # pilots drawn from known Vb, Vw, c0, c1: the fit returns them, and the
+    # chosen completions minimise Var(J) t(J) over every power of two in
+    # [1, 4096] (brute force); edge cases clamp to 1 and 4096
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        vb, vw = 10 ** rng.uniform(-2, 4), 10 ** rng.uniform(-1, 7)
+        c0, c1 = 10 ** rng.uniform(-8, -5), 10 ** rng.uniform(-11, -7)
+        stats = {j: {"var": vb + vw / j, "ms_per_id": c0 + c1 * j} for j in (64, 1024)}
+        r = L.fit_split_model(stats, 64, 1024)
+        assert r["v_between"] == pytest.approx(vb, rel=1e-6, abs=1e-9 * vw)
+        assert r["v_within"] == pytest.approx(vw, rel=1e-9)
+        assert r["c1_ms_per_id_per_completion"] == pytest.approx(c1, rel=1e-6)
+        cost = {1 << k: (vb + vw / (1 << k)) * (c0 + c1 * (1 << k)) for k in range(13)}
+        best = min(cost, key=cost.get)
+        assert cost[r["completions"]] <= cost[best] * (1 + 1e-9)
+    flat = {64: {"var": 5.0, "ms_per_id": 1e-6}, 1024: {"var": 5.0, "ms_per_id": 2e-6}}
+    assert L.fit_split_model(flat, 64, 1024)["completions"] == 1          # completions add no information
+    pure = {64: {"var": 1.0 / 64, "ms_per_id": 1e-6}, 1024: {"var": 1.0 / 1024, "ms_per_id": 2e-6}}
+    assert L.fit_split_model(pure, 64, 1024)["completions"] == 4096       # all variance in the completions

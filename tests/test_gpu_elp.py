@@ -93,26 +93,29 @@ def test_two_channel_query_reaches_the_error_on_both(inst_cls, whats):
         assert chan["estimate"] == pytest.approx(single.mean, rel=1e-12)
 
 
-@pytest.mark.parametrize("fused", [False, True])
-def test_split_completions_choice(fused):
+@pytest.mark.parametrize("kind", [Z.PLAN_K4_SPLIT, Z.PLAN_TRIANGLE_K4_SPLIT, Z.PLAN_HOUSE_SPLIT])
+def test_split_completions_choice(kind):
     # two pilots fit Var(J) = Vb + Vw / J and t(J) = c0 + c1 J; the choice is a
-    # power of two in [16, 4096] and the pilots' ids stay clear of the query's
+    # power of two in [1, 4096] and the pilots' ids stay clear of the query's
     n, s, d = H.golden_graph("wiki-vote")
     g = Z.Graph.from_edges(s, d, n, seed=1)
-    J, rec = L.choose_split_completions(g, seed=7, fused=fused)
-    say(f"[elp] split choice fused={fused}: J={J} J*={rec['j_star']:.1f} Vb={rec['v_between']:.4g} "
+    J, rec = L.choose_split_completions(g, seed=7, kind=kind, eps=None)   # both pilots
+    say(f"[elp] split choice kind={kind}: J={J} J*={rec['j_star']:.1f} Vb={rec['v_between']:.4g} "
         f"Vw={rec['v_within']:.4g} c0={rec['c0_ms_per_id']:.3g} c1={rec['c1_ms_per_id_per_completion']:.3g} "
         f"{rec['seconds'] * 1e3:.1f} ms")
-    assert J in [1 << k for k in range(4, 13)]
-    assert rec["pilots"][64]["successes"] >= 32 and rec["pilots"][1024]["successes"] >= 32
-    # on wiki-vote most of a closed triangle's variance is in its completions
+    assert J in [1 << k for k in range(13)]
+    lo, hi = (rec["pilots"][j] for j in sorted(rec["pilots"]))
+    assert lo["ids"] == hi["ids"] and lo["successes"] >= 32 and hi["successes"] >= lo["successes"]
+    assert lo["ids"] * lo["ms_per_id"] >= 1.0
+    # on wiki-vote part of a closed first stage's variance is in its completions
     assert rec["v_within"] > 0
-    if not fused:
-        truth = g.exact_count("k4")
-        r = L.split_k4_to_error(g, 0.02, 0.95, L.LearnedErrorProfile(), seed=7, max_estimators=1 << 32)
-        say(f"[elp] split auto: J={r['completions']} N={r['estimators']} err="
+    if kind != Z.PLAN_TRIANGLE_K4_SPLIT:
+        # 4-cliques against the exact count, houses against the reference's own (FiveHouse.cpp:68)
+        truth = g.exact_count("k4") if kind == Z.PLAN_K4_SPLIT else 2134981304
+        r = L.split_to_error(g, 0.02, 0.95, L.LearnedErrorProfile(), kind=kind, seed=7, max_estimators=1 << 32)
+        say(f"[elp] split auto kind={kind}: J={r['completions']} N={r['estimators']} err="
             f"{abs(r['estimate'] - truth) / truth:.4f} hw={r['rel_half_width']:.4f}")
         assert r["accepted"] and r["rel_half_width"] <= 0.02
         assert abs(r["estimate"] - truth) <= 2 * r["half_width"]
-        direct = Z.run(g, Z.Plan(Z.PLAN_K4_SPLIT, completions=r["completions"]), 7, 0, r["estimators"])[0]
+        direct = Z.run(g, Z.Plan(kind, completions=r["completions"]), 7, 0, r["estimators"])[0]
         assert r["estimate"] == pytest.approx(direct.mean, rel=1e-12)

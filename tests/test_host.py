@@ -198,14 +198,14 @@ def test_plan_kinds_and_channels_without_gpu():
         p = Z.Plan(kind)
         assert (p.channels, p.cond_calls) == (nch, calls)
     assert Z.Plan(Z.PLAN_CHAIN, order=6).cond_calls == 4
-    for kind, nch in ((Z.PLAN_K4_SPLIT, 1), (Z.PLAN_TRIANGLE_K4_SPLIT, 2)):
+    for kind, nch, calls in ((Z.PLAN_K4_SPLIT, 1, 2), (Z.PLAN_TRIANGLE_K4_SPLIT, 2, 2), (Z.PLAN_HOUSE_SPLIT, 1, 3)):
         for J in (1, 64, 4096):
             p = Z.Plan(kind, completions=J)
-            assert (p.channels, p.cond_calls) == (nch, 2)
+            assert (p.channels, p.cond_calls) == (nch, calls)
         for J in (0, 3, 8192):   # completions: a power of two in [1, 4096]
             with pytest.raises(Z.ZGraphError):
                 Z.Plan(kind, completions=J)
-    for bad in (0, 11, 100):
+    for bad in (0, 12, 100):
         with pytest.raises(Z.ZGraphError):
             Z.Plan(bad)
     with pytest.raises(Z.ZGraphError):
