@@ -634,8 +634,39 @@ def house_sweep(Z, args):
         rows[gname] = time_to_error(Z, g, args, patterns=patterns, cap_log2=34, queries=3)
         rows[gname]["5-house split, completions chosen"] = split_auto_arm(
             Z, L, g, prof, 0.01, 0.95, 3, 34, kind=Z.PLAN_HOUSE_SPLIT, what="house")
+        rows[gname]["kernels"] = house_kernels(Z, g, gname)
+        g.close()
+    # the same kernels on the two large graphs (no exact house count there)
+    for gname, make in (("chunglu 4.8M/69M b2.1", lambda: Z.chung_lu(4_800_000, 69_000_000, 2.1, 20000.0, args.seed)),
+                        ("rmat22", lambda: Z.rmat(22, 16 << 22, 0.57, 0.19, 0.19, args.seed))):
+        e = make()
+        g = Z.Graph.from_device(e, seed=args.seed)
+        e.free()
+        rows[gname] = {"kernels": house_kernels(Z, g, gname)}
         g.close()
     print(json.dumps({"house_sweep": rows}), flush=True)
+
+
+def house_kernels(Z, g, gname, log2n=26):
+    """Kernel time, successes and per-estimator CV of the split house plan
+    with 1, 16, 256 and 4096 completions and of SamplerGenerator's house plan
+    over 2^log2n ids, and the time to +-1 % at 95 % they project."""
+    import math
+    N = 1 << log2n
+    out = {}
+    plans = [(f"split x{J}", Z.Plan(Z.PLAN_HOUSE_SPLIT, completions=J)) for J in (1, 16, 256, 4096)]
+    plans.append(("SamplerGenerator", Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_HOUSE_PATTERN)))
+    for name, p in plans:
+        Z.run(g, p, 1, 1 << 50, 1 << 16)
+        sums, ms = Z.run(g, p, 1, 0, N, timed=True)
+        s = sums[0]
+        var = max(s.sumsq / N - s.mean * s.mean, 0.0)
+        cv = math.sqrt(var) / s.mean if s.mean > 0 else None
+        proj = (1.959964 * cv / 0.01) ** 2 / N * ms / 1e3 if cv else None
+        out[name] = {"ms": ms, "successes": s.successes, "mean": s.mean, "cv": cv, "projected_s_to_1pct": proj}
+        log(f"house kernels {gname} {name}: {ms:.2f} ms per 2^{log2n}, {s.successes} successes, CV "
+            f"{cv if cv is None else round(cv, 1)}, projected +-1% {proj if proj is None else round(proj, 4)} s")
+    return out
 
 
 def elp_coverage(Z, g, args, patterns, queries=40, eps=0.01, conf=0.95):
