@@ -663,9 +663,12 @@ def house_kernels(Z, g, gname, log2n=26):
         var = max(s.sumsq / N - s.mean * s.mean, 0.0)
         cv = math.sqrt(var) / s.mean if s.mean > 0 else None
         proj = (1.959964 * cv / 0.01) ** 2 / N * ms / 1e3 if cv else None
-        out[name] = {"ms": ms, "successes": s.successes, "mean": s.mean, "cv": cv, "projected_s_to_1pct": proj}
+        lines = Z.count_reads(g, p, 1, 0, 1 << 20) / (1 << 20)
+        out[name] = {"ms": ms, "successes": s.successes, "mean": s.mean, "cv": cv, "projected_s_to_1pct": proj,
+                     "lines_per_estimator": lines, "line_rate_gbs": lines * LINE * N / (ms / 1e3) / 1e9}
         log(f"house kernels {gname} {name}: {ms:.2f} ms per 2^{log2n}, {s.successes} successes, CV "
-            f"{cv if cv is None else round(cv, 1)}, projected +-1% {proj if proj is None else round(proj, 4)} s")
+            f"{cv if cv is None else round(cv, 1)}, projected +-1% {proj if proj is None else round(proj, 4)} s, "
+            f"{lines:.3f} lines/estimator, {out[name]['line_rate_gbs']:.0f} GB/s")
     return out
 
 

@@ -27,7 +27,8 @@
  *     slots, which does not depend on the insertion order).
  * Plans covered: triangle, 3-chain (no entry read on a graph without
  * parallel edges: the entry can only set the parallel-copy denominator),
- * 3-motif and the three-shape 4-clique plan.
+ * 3-motif, the three-shape 4-clique plan and (og_device_lines_house) the
+ * split 5-house plan.
  */
 #include "zg_oracle.h"
 
@@ -323,6 +324,54 @@ static void lines_k4(lm *L, og_stream *st)
             }
         }
     }
+}
+
+/* split 5-house (est_house_split's draws): e0's record, the roof's entry and
+ * its pair's buckets; per completion of a closed roof, two entries, and the
+ * pair's buckets only when x and y pass the distinctness checks */
+static void lines_house(lm *L, og_stream *st, int J, uint64_t seed, uint64_t id)
+{
+    const og_graph *g = L->g;
+    const uint64_t r0 = og_uniform(st, g->m);
+    L->lines += 1;   /* e0's record */
+    uint32_t a = g->src[r0], b = g->dst[r0];
+    uint64_t da = g->off[a + 1] - g->off[a], db = g->off[b + 1] - g->off[b];
+    if (db < da) {
+        uint32_t t = a; a = b; b = t;
+        uint64_t u = da; da = db; db = u;
+    }
+    const uint32_t w = g->adj_dst[g->off[a] + og_uniform(st, da)];
+    L->lines += 1;   /* the roof's entry */
+    if (w == b) return;
+    L->lines += walk(L->h, w, b);
+    if (!pair_rank(g, w, b, 0, NULL)) return;
+    for (int j = 0; j < J; ++j) {
+        og_stream c;
+        og_stream_init(&c, seed, OG_TAG_COMPLETION + (uint32_t)j, id);
+        const uint32_t x = g->adj_dst[g->off[a] + og_uniform(&c, da)];
+        const uint32_t y = g->adj_dst[g->off[b] + og_uniform(&c, db)];
+        L->lines += 2;
+        if (x == b || y == a || x == y || x == w || y == w) continue;
+        L->lines += walk(L->h, x, y);
+    }
+}
+
+uint64_t og_device_lines_house(const og_graph *g, int completions, uint64_t seed, uint64_t first, uint64_t count)
+{
+    if (g->m == 0) return 0;
+    lm_hash h;
+    if (hash_build(g, &h)) return UINT64_MAX;
+    uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : total)
+    for (uint64_t i = 0; i < count; ++i) {
+        lm L = {g, &h, 0};
+        og_stream st;
+        og_stream_init(&st, seed, OG_TAG_ESTIMATOR, first + i);
+        lines_house(&L, &st, completions, seed, first + i);
+        total += L.lines;
+    }
+    free(h.passed);
+    return total;
 }
 
 uint64_t og_device_lines(const og_graph *g, int kind, uint64_t seed, uint64_t first, uint64_t count)

@@ -35,6 +35,8 @@
  *   og_device_lines    (lines.c) closed costs on CPU (3-chain = 1 line on a
  *                      simple graph, 3-motif = triangle); equals the device's
  *                      own read count on eleven graphs (GPU tests)         pinned
+ *   og_device_lines_house  equals the device's read count of the split
+ *                      house plan on the parity graphs (GPU tests)       pinned
  *   generators         restated from the device generators' definition
  *                      (not the reference's); the device output is
  *                      compared element by element                     pinned

@@ -81,6 +81,8 @@ def lib():
         L.og_graph_subsample.argtypes = [P(Graph), P(Graph), C.c_double, C.c_uint64]
         L.og_device_lines.argtypes = [P(Graph), C.c_int, C.c_uint64, C.c_uint64, C.c_uint64]
         L.og_device_lines.restype = C.c_uint64
+        L.og_device_lines_house.argtypes = [P(Graph), C.c_int, C.c_uint64, C.c_uint64, C.c_uint64]
+        L.og_device_lines_house.restype = C.c_uint64
         L.og_graph_filter.argtypes = [P(Graph), P(Graph), C.c_float, C.c_int, C.c_uint64]
         L.og_estimate.argtypes = [P(Graph), P(Plan), C.c_uint64, C.c_uint64, P(Outcome)]
         L.og_run.argtypes = [P(Graph), P(Plan), C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, P(Sums)]
@@ -323,6 +325,11 @@ def device_lines(g, kind, seed, first, count):
     if r == (1 << 64) - 1:
         raise ValueError("no line model for this plan kind")
     return r
+
+
+def device_lines_house(g, completions, seed, first, count):
+    """device_lines for the split 5-house plan with `completions` per closed roof."""
+    return lib().og_device_lines_house(C.byref(g.g), completions, seed, first, count)
 
 
 def mean(s):

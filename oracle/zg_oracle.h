@@ -184,6 +184,8 @@ int og_close(const og_graph *g, uint64_t min_valid, const uint32_t *verts, int n
  * 3-motif, 4-clique) reads for estimators [first, first+count) under the
  * device layout; UINT64_MAX for other kinds. */
 uint64_t og_device_lines(const og_graph *g, int kind, uint64_t seed, uint64_t first, uint64_t count);
+/* the same for OG_PLAN_HOUSE_SPLIT with `completions` per closed roof */
+uint64_t og_device_lines_house(const og_graph *g, int completions, uint64_t seed, uint64_t first, uint64_t count);
 
 /* ---- exact counts (naive_implementation semantics) --------------------- */
 uint64_t og_exact_triangles(const og_graph *g, int threads);

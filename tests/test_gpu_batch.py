@@ -36,7 +36,8 @@ def plans():
             "house": Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_HOUSE_PATTERN),
             "w3": Z.Plan(Z.PLAN_THREE_CHAIN), "tk4": Z.Plan(Z.PLAN_TRIANGLE_K4),
             "k4s": Z.Plan(Z.PLAN_K4_SPLIT, completions=128),
-            "tk4s": Z.Plan(Z.PLAN_TRIANGLE_K4_SPLIT, completions=32)}
+            "tk4s": Z.Plan(Z.PLAN_TRIANGLE_K4_SPLIT, completions=32),
+            "hs": Z.Plan(Z.PLAN_HOUSE_SPLIT, completions=64)}
 
 
 def _queries(graphs, plans):

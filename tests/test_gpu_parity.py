@@ -85,6 +85,10 @@ def test_read_count_equals_the_cpu_line_model(name):
         dev = Z.count_reads(zg, Z.Plan(kind), 9, 1 << 20, k)
         cpu = O.device_lines(og, kind, 9, 1 << 20, k)
         assert dev == cpu, (name, kind, dev, cpu)
+    for J in (1, 16):
+        dev = Z.count_reads(zg, Z.Plan(Z.PLAN_HOUSE_SPLIT, completions=J), 9, 1 << 20, k)
+        cpu = O.device_lines_house(og, J, 9, 1 << 20, k)
+        assert dev == cpu, (name, "house", J, dev, cpu)
 
 
 @pytest.mark.parametrize("name", sorted(GRAPHS))

@@ -47,7 +47,8 @@ def graphs():
 @pytest.mark.parametrize("plan_args", [dict(kind=Z.PLAN_TRIANGLE), dict(kind=Z.PLAN_TRIANGLE_K4),
                                        dict(kind=Z.PLAN_PATTERN, pattern=Z.FIVE_STAR_PATTERN),
                                        dict(kind=Z.PLAN_THREE_MOTIF),
-                                       dict(kind=Z.PLAN_TRIANGLE_K4_SPLIT, completions=64)])
+                                       dict(kind=Z.PLAN_TRIANGLE_K4_SPLIT, completions=64),
+                                       dict(kind=Z.PLAN_HOUSE_SPLIT, completions=16)])
 def test_noalloc_equals_run_on_shrinking_calls(graphs, plan_args):
     big, small = graphs
     plan = Z.Plan(**plan_args)
