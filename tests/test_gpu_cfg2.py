@@ -156,3 +156,39 @@ def test_cfg2_split_k4_interval_and_parity(graph, exported):
         for f in ("status", "num_lo", "num_hi", "log2den", "e0", "ncond", "nwords"):
             assert zo[f][i] == o[f][0], (int(i), f)
     say(f"[cfg2] split 4-clique: first 2^16 ids and all {len(ids)} successes among 2^22 bit-exact")
+
+
+def test_cfg2_split_house_parity_and_consistency(graph, exported):
+    # the split 5-house plan on the headline graph (its exact house count is
+    # out of the exact counter's reach): the first 2^16 ids and every success
+    # among the first 2^20 equal the oracle's outcomes, the device read count
+    # equals the CPU line model, and the estimates with 16 and 256 completions
+    # (same first stages, other completion draws) agree within their 99.9 %
+    # intervals
+    a = exported
+    og = O.OGraph.from_csr(graph.num_vertices, a["src"], a["dst"], a["off"], a["adj_rank"], a["adj_nbr"])
+    plan = Z.Plan(Z.PLAN_HOUSE_SPLIT, completions=16)
+    oplan = O.make_plan(O.PLAN_HOUSE_SPLIT, order=16)
+    zo, zs = Z.run_outcomes(graph, plan, 1, 0, 1 << 16)
+    oo = O.run_outcomes(og, oplan, 1, 0, 1 << 16)
+    for f in ("status", "num_lo", "num_hi", "log2den", "e0", "ncond", "nwords"):
+        assert (zo[f] == oo[f]).all(), f
+    zo, _ = Z.run_outcomes(graph, plan, 1, 0, 1 << 20)
+    ids = np.flatnonzero(zo["status"] != 0)
+    assert len(ids) >= 1000
+    for i in ids:
+        o = O.run_outcomes(og, oplan, 1, int(i), 1)
+        for f in ("status", "num_lo", "num_hi", "log2den", "e0", "ncond", "nwords"):
+            assert zo[f][i] == o[f][0], (int(i), f)
+    k = 1 << 16
+    assert Z.count_reads(graph, plan, 1, 0, k) == O.device_lines_house(og, 16, 1, 0, k)
+    n = 1 << 26
+    est = {}
+    for J in (16, 256):
+        s = Z.run(graph, Z.Plan(Z.PLAN_HOUSE_SPLIT, completions=J), 1, 0, n)[0]
+        est[J] = (s.mean, Z.clt_half_width(s.mean, s.sumsq, n, 0.999))
+        assert s.status == 0 and s.successes > 10000
+    say(f"[cfg2] split house: first 2^16 ids and all {len(ids)} successes among 2^20 bit-exact; estimates "
+        + ", ".join(f"J={J}: {m:.6g} +- {h:.3g}" for J, (m, h) in est.items()))
+    assert abs(est[16][0] - est[256][0]) <= est[16][1] + est[256][1]
+    assert est[256][1] < 0.05 * est[256][0]
