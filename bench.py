@@ -203,6 +203,7 @@ def main():
     traffic = profiled_traffic(dom, args, N)
     fused = fused_triangle_k4(Z, g, args, N, first_sums)
     split = split_k4(Z, g, args, N, first_sums["4-clique"], avg_ms["4-clique"])
+    house = split_house(Z, g, args, N)
 
     elp = None
     if not args.no_elp:
@@ -245,6 +246,7 @@ def main():
                      "random_gather_gbs_graph_size": same_size},
         "fused_triangle_4clique": fused,
         "split_4clique": split,
+        "split_5house": house,
         "cpu_baseline": cpu,
         "time_to_error": elp,
         "batched_queries": batch,
@@ -280,6 +282,34 @@ def split_k4(Z, g, args, N, k4_sums, k4_ms):
     return {"completions": J, "kernel_ms": avg, "estimators_per_s": N / (avg / 1e3), "lines_per_estimator": lines,
             "cv": c_split, "cv_4clique_plan_same_ids": c_k4, "estimate": first.mean,
             "expected_time_to_error_gain": (c_k4 / c_split) ** 2 * k4_ms / avg}
+
+
+def split_house(Z, g, args, N, J=16):
+    """The split 5-house plan over the timed steps' ids (J completions per
+    closed roof; 16 is near the best J on this graph): kernel time, random
+    lines per estimator, their rate, the per-estimator CV and the time to
+    +-1 % at 95 % it projects.  SamplerGenerator's house plan finds no house
+    in 2^26 ids on this graph."""
+    import math
+    plan = Z.Plan(Z.PLAN_HOUSE_SPLIT, completions=J)
+    Z.run(g, plan, args.seed, (10_000 + args.warmup) * N, N)
+    kms, first = [], None
+    for i in range(args.steps):
+        sums, ms = Z.run(g, plan, args.seed, i * N, N, args.block, args.blocks_per_cu, timed=True)
+        kms.append(ms)
+        if i == 0:
+            first = sums[0]
+    avg = sum(kms) / len(kms)
+    lines = Z.count_reads(g, plan, args.seed, 0, 1 << 22) / (1 << 22)
+    m = first.mean
+    cv = math.sqrt(max(first.sumsq / first.count - m * m, 0.0)) / m if m > 0 else float("inf")
+    proj = (1.959964 * cv / 0.01) ** 2 * avg / N / 1e3
+    gbs = lines * LINE * N / (avg / 1e3) / 1e9
+    log(f"split 5-house x{J}: {avg:.2f} ms per 2^{args.log2_estimators} ids, {lines:.3f} lines/id ({gbs:.0f} GB/s), "
+        f"CV {cv:.1f}, estimate {m:.6g}, +-1% projected in {proj * 1e3:.1f} ms")
+    return {"completions": J, "kernel_ms": avg, "estimators_per_s": N / (avg / 1e3), "lines_per_estimator": lines,
+            "line_rate_gbs": gbs, "frac_of_hbm_peak": gbs / PEAK_HBM_GBS, "cv": cv, "estimate": m,
+            "projected_s_to_1pct": proj}
 
 
 def fused_triangle_k4(Z, g, args, N, first_sums):
