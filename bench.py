@@ -638,16 +638,19 @@ def elp_sweep(Z, args):
 
 def house_sweep(Z, args):
     """Time to +-1 % at 95 % for 5-houses (the pattern of the reference's
-    FiveHouse application) on three seeded graphs whose exact house counts the
-    device counter finds in seconds (Chung-Lu 200 k / 2 M with beta 2.1,
-    Chung-Lu 1 M / 10 M with beta 2.5, R-MAT scale 16): the reference's
-    SamplerGenerator house estimator against the split house plan with 256
-    completions per closed roof and with its completions chosen by two pilots;
-    regression and pilot sizing with the judge, 3 queries each, budgets capped
-    at 2^34, errors against the exact counts."""
+    FiveHouse application) on five seeded graphs (Chung-Lu 200 k / 2 M with
+    beta 2.1, Chung-Lu 1 M / 10 M with beta 2.5, R-MAT scale 16, the 69 M-edge
+    bench graph, R-MAT scale 22): the reference's SamplerGenerator house
+    estimator against the split house plan with 256 completions per closed
+    roof and with its completions chosen by two pilots; regression and pilot
+    sizing with the judge, 3 queries each, budgets capped at 2^34, errors
+    against the device exact counts; and the split plan's kernels with 1 to
+    4096 completions."""
     graphs = [("chunglu 200k/2M b2.1", lambda: Z.chung_lu(200_000, 2_000_000, 2.1, 2000.0, args.seed)),
               ("chunglu 1M/10M b2.5", lambda: Z.chung_lu(1_000_000, 10_000_000, 2.5, 2000.0, args.seed)),
-              ("rmat16", lambda: Z.rmat(16, 16 << 16, 0.57, 0.19, 0.19, args.seed))]
+              ("rmat16", lambda: Z.rmat(16, 16 << 16, 0.57, 0.19, 0.19, args.seed)),
+              ("chunglu 4.8M/69M b2.1", lambda: Z.chung_lu(4_800_000, 69_000_000, 2.1, 20000.0, args.seed)),
+              ("rmat22", lambda: Z.rmat(22, 16 << 22, 0.57, 0.19, 0.19, args.seed))]
     J = args.split_completions
     arms = ("regression+judge", "pilot+judge")
     patterns = (("5-house, SamplerGenerator", lambda: Z.Plan(Z.PLAN_PATTERN, pattern=Z.FIVE_HOUSE_PATTERN), "house",
@@ -665,14 +668,6 @@ def house_sweep(Z, args):
         rows[gname]["5-house split, completions chosen"] = split_auto_arm(
             Z, L, g, prof, 0.01, 0.95, 3, 34, kind=Z.PLAN_HOUSE_SPLIT, what="house")
         rows[gname]["kernels"] = house_kernels(Z, g, gname)
-        g.close()
-    # the same kernels on the two large graphs (no exact house count there)
-    for gname, make in (("chunglu 4.8M/69M b2.1", lambda: Z.chung_lu(4_800_000, 69_000_000, 2.1, 20000.0, args.seed)),
-                        ("rmat22", lambda: Z.rmat(22, 16 << 22, 0.57, 0.19, 0.19, args.seed))):
-        e = make()
-        g = Z.Graph.from_device(e, seed=args.seed)
-        e.free()
-        rows[gname] = {"kernels": house_kernels(Z, g, gname)}
         g.close()
     print(json.dumps({"house_sweep": rows}), flush=True)
 
@@ -768,7 +763,7 @@ def exact_sweep(Z, args):
     os.environ.setdefault("OMP_PROC_BIND", "close")
     import pyoracle as O
     threads = min(16, os.cpu_count() or 1)
-    kinds = ["triangle", "k4", "wedge", "path3", "c4", "path4", "star5"]
+    kinds = ["triangle", "k4", "wedge", "path3", "c4", "path4", "star5", "house"]
     rows = []
     for scale in (20, 22):
         e = Z.rmat(scale, 16 << scale, 0.57, 0.19, 0.19, args.seed)

@@ -30,6 +30,9 @@
  *   og_exact_c4        brute force of the 4-cycle pattern; K_n, C_n      pinned
  *   og_exact_paths4    og_exact_paths_brute(5), itself equal to the
  *                      reference's FiveChain built from source            pinned
+ *   og_exact_house     og_exact_house_brute on random graphs and
+ *                      multigraphs, K_n closed forms, FiveHouse.cpp:68's
+ *                      wiki-vote count                                   pinned
  *   og_regression_k    tests/TestRegression.cpp known answer k^2=609.8565 pinned
  *   og_elp_*           hand-computed closed form on 1-3 points           pinned
  *   og_device_lines    (lines.c) closed costs on CPU (3-chain = 1 line on a
@@ -1602,6 +1605,61 @@ uint64_t og_exact_c4(const og_graph *g)
     }
     free(cnt); free(seen); simple_free(&s);
     return pairs / 2;
+}
+
+/* 5-houses by a closed form: with t_e the triangles and c4_e the 4-cycles
+ * through edge e of the simple graph, a house (roof triangle on a square) is
+ * an edge e in both, an apex from e's t_e common neighbours and a 4-cycle
+ * through e, minus the choices whose apex lies on the cycle; over the
+ * house's two automorphisms,
+ *   houses = sum_e t_e c4_e - 2 sum_e t_e (t_e - 1)
+ * (K5: 10 * 3 * 6 - 2 * 10 * 6 = 60).  sum_e t_e c4_e = the sum over 4-cycles
+ * of their edges' t, counted here from both diagonals {x, y} of each cycle:
+ * k common neighbours v_i of x and y close C(k, 2) cycles whose weights sum
+ * to (k - 1) sum_i (t(x, v_i) + t(v_i, y)).  Pinned against
+ * og_exact_house_brute and FiveHouse.cpp:68's wiki-vote count. */
+uint64_t og_exact_house(const og_graph *g)
+{
+    og_simple s; simple_build(g, &s);
+    const uint64_t ne = s.off[s.n];
+    uint32_t *te = (uint32_t *)malloc(4 * (ne ? ne : 1));
+    unsigned __int128 tt = 0;   /* sum over entries (each edge twice) of t (t - 1) */
+    for (uint32_t v = 0; v < s.n; ++v)
+        for (uint64_t i = s.off[v]; i < s.off[v + 1]; ++i) {
+            const uint32_t u = s.nb[i];
+            uint64_t a = s.off[v], b = s.off[u], c = 0;
+            while (a < s.off[v + 1] && b < s.off[u + 1]) {
+                if (s.nb[a] < s.nb[b]) ++a;
+                else if (s.nb[a] > s.nb[b]) ++b;
+                else { ++c; ++a; ++b; }
+            }
+            te[i] = (uint32_t)c;
+            tt += (unsigned __int128)c * (c ? c - 1 : 0);
+        }
+    uint64_t *cnt = (uint64_t *)calloc((size_t)s.n + 1, 8);
+    unsigned __int128 *wsum = (unsigned __int128 *)calloc((size_t)s.n + 1, sizeof(unsigned __int128));
+    uint32_t *seen = (uint32_t *)malloc(4 * ((size_t)s.n + 1));
+    unsigned __int128 w2 = 0;   /* every 4-cycle's weight, once per diagonal */
+    for (uint32_t x = 0; x < s.n; ++x) {
+        uint64_t ns = 0;
+        for (uint64_t i = s.off[x]; i < s.off[x + 1]; ++i) {
+            const uint32_t v = s.nb[i];
+            for (uint64_t j = s.off[v]; j < s.off[v + 1]; ++j) {
+                const uint32_t y = s.nb[j];
+                if (y <= x) continue;
+                if (cnt[y]++ == 0) seen[ns++] = y;
+                wsum[y] += (unsigned __int128)te[i] + te[j];
+            }
+        }
+        for (uint64_t k = 0; k < ns; ++k) {
+            const uint32_t y = seen[k];
+            w2 += (unsigned __int128)(cnt[y] - 1) * wsum[y];
+            cnt[y] = 0;
+            wsum[y] = 0;
+        }
+    }
+    free(cnt); free(wsum); free(seen); free(te); simple_free(&s);
+    return (uint64_t)(w2 / 2 - tt);
 }
 
 /* 5-vertex chains (simple paths of four edges, FiveChain.cpp's count) by a

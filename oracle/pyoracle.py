@@ -121,6 +121,8 @@ def lib():
         L.og_exact_stars64.argtypes = [P(Graph), C.c_int, P(C.c_uint64), P(C.c_uint64)]
         L.og_exact_house_brute.restype = C.c_uint64
         L.og_exact_house_brute.argtypes = [P(Graph)]
+        L.og_exact_house.restype = C.c_uint64
+        L.og_exact_house.argtypes = [P(Graph)]
         L.og_exact_pattern_brute.restype = C.c_uint64
         L.og_exact_pattern_brute.argtypes = [P(Graph), C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         L.og_regression_k.restype = C.c_longdouble
@@ -279,6 +281,9 @@ class OGraph:
 
     def house_brute(self):
         return lib().og_exact_house_brute(C.byref(self.g))
+
+    def house(self):
+        return lib().og_exact_house(C.byref(self.g))
 
     def pattern_brute(self, k, edges):
         s = np.array([e[0] for e in edges], np.uint32)

@@ -198,6 +198,7 @@ og_u128 og_exact_stars(const og_graph *g, int leaves);     /* FiveStar.cpp:46-60
 void og_exact_stars64(const og_graph *g, int leaves, uint64_t *lo, uint64_t *hi);
 uint64_t og_exact_paths_brute(const og_graph *g, int nverts); /* simple paths with nverts vertices */
 uint64_t og_exact_house_brute(const og_graph *g);          /* 5-house, non-induced */
+uint64_t og_exact_house(const og_graph *g);                /* 5-house by per-edge triangles and 4-cycles */
 uint64_t og_exact_pattern_brute(const og_graph *g, int k, int e, const uint32_t *ps, const uint32_t *pd);
 
 /* ---- regression error-latency profile (src/ErrorLatencyProfile.cpp) --- */

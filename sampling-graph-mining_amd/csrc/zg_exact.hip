@@ -6,7 +6,8 @@
 //   wedges     naive_implementation/ThreeChain.cpp:44-51 (sum of C(d, 2))
 //   3-paths    naive_implementation/FourChain.cpp:46-66 (sum over edges (du-1)(dv-1) - 3T)
 //   5-stars    naive_implementation/FiveStar.cpp:51-59 (sum of C(d, 4))
-//   5-houses   naive_implementation/FiveHouse.cpp (per ordered edge, k_house)
+//   5-houses   naive_implementation/FiveHouse.cpp's count as sum_e t_e c4_e - 2 sum_e t_e (t_e - 1):
+//              per-edge triangles (k_tri_edges) weighting the 4-cycle listing
 //   5-chains   naive_implementation/FiveChain.cpp:45-72 (closed form over degrees,
 //              per-vertex triangles and 4-cycles)
 //   4-cycles   no reference program (the 5-chain closed form needs them)
@@ -1153,51 +1154,6 @@ __device__ __forceinline__ uint64_t isect_count(const uint32_t *a, uint64_t na, 
     return c;
 }
 
-// 5-vertex houses (apex x over the edge {a, b} of the square a-c-d-b):
-// one wavefront per ordered edge (a, b) counts the embeddings with a, b as
-// the roof's base, sum over P = {(c, d): c in N(a)\b, d in N(b)\a, c ~ d}
-// of (t_ab - [c in T] - [d in T]) with T = N(a) & N(b), t_ab = |T|:
-//   t_ab * (S - (d_a - 1)) - sum_{c in T}(|N(c) & N(b)| - 1) - sum_{d in T}(|N(d) & N(a)| - 1),
-//   S = sum_{c in N(a)\b} |N(c) & N(b)|.
-// The house count is the embedding total over 2 (its automorphisms).
-__global__ __launch_bounds__(256) void k_house(const uint64_t *foff, const uint32_t *fnb, const uint32_t *fsrc,
-                                               uint64_t ne, unsigned long long *partial)
-{
-    const int lane = threadIdx.x & 63;
-    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    unsigned long long acc = 0;
-    for (uint64_t e = wave; e < ne; e += nwaves) {
-        const uint32_t a = fsrc[e], b = fnb[e];
-        const uint32_t *Na = fnb + foff[a], *Nb = fnb + foff[b];
-        const uint64_t da = foff[a + 1] - foff[a], db = foff[b + 1] - foff[b];
-        unsigned long long S = 0, t = 0, X = 0, Y = 0;
-        for (uint64_t i = lane; i < da; i += 64) {
-            const uint32_t c = Na[i];
-            if (c == b) continue;
-            const uint32_t *Nc = fnb + foff[c];
-            const uint64_t dc = foff[c + 1] - foff[c];
-            const uint64_t k = isect_count(Nc, dc, Nb, db);
-            S += k;
-            if (bsearch_u32(Nb, db, c)) {
-                t += 1;
-                X += k - 1;
-                Y += isect_count(Nc, dc, Na, da) - 1;
-            }
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            S += __shfl_xor(S, o, 64);
-            t += __shfl_xor(t, o, 64);
-            X += __shfl_xor(X, o, 64);
-            Y += __shfl_xor(Y, o, 64);
-        }
-        if (lane == 0) acc += t * (S - (da - 1)) - X - Y;
-    }
-    const unsigned long long tot = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
-}
-
 // degree moments: sum d^2 and sum d^3 (as doubles) per block
 __global__ void k_deg_moments(const uint64_t *off, uint32_t n, double *p2, double *p3)
 {
@@ -1260,16 +1216,28 @@ constexpr uint32_t kC4Empty = 0xffffffffu;
 
 __device__ __forceinline__ uint32_t c4_hash(uint32_t w, uint32_t bits) { return (w * 0x9E3779B1u) >> (32 - bits); }
 
+// W (houses): each wedge u - v - w also adds t(u, v) + t(v, w) (tf: the
+// triangles of each full-list entry's edge) to w's weight, and the top vertex
+// u with opposite w adds (cnt[w] - 1) * weight[w] -- the sum over its
+// C(cnt[w], 2) cycles of their four edges' triangle counts -- instead of
+// C(cnt[w], 2).
+template <bool W>
 __global__ __launch_bounds__(256) void k_c4_lds(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg,
                                                 uint32_t n, unsigned long long *partial, uint32_t *ovf_list,
-                                                unsigned int *ovf_count)
+                                                unsigned int *ovf_count, const uint32_t *tf)
 {
     __shared__ uint32_t keys[4 * kC4Slots], vals[4 * kC4Slots];
+    __shared__ unsigned long long wts[W ? 4 * kC4Slots : 1];
     __shared__ uint32_t fill[4], full[4];
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t *const K = keys + wv * kC4Slots;
     uint32_t *const V = vals + wv * kC4Slots;
-    for (uint32_t t = lane; t < kC4Slots; t += 64) { K[t] = kC4Empty; V[t] = 0; }
+    unsigned long long *const WT = W ? wts + wv * kC4Slots : wts;
+    for (uint32_t t = lane; t < kC4Slots; t += 64) {
+        K[t] = kC4Empty;
+        V[t] = 0;
+        if (W) WT[t] = 0;
+    }
     if (lane == 0) { fill[wv] = 0; full[wv] = 0; }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -1283,6 +1251,7 @@ __global__ __launch_bounds__(256) void k_c4_lds(const uint64_t *foff, const uint
         for (uint64_t jb = 0; jb < pu; jb += 64) {
             if (jb + lane < pu) {
                 const uint32_t v = (uint32_t)fkey[ub + jb + lane];
+                const uint32_t tuv = W ? tf[ub + jb + lane] : 0;
                 const uint64_t vb = foff[v];
                 const uint64_t qv = lower_u64(fkey + vb, foff[v + 1] - vb, ku);   // neighbours of v below u
                 for (uint64_t k = 0; k < qv; ++k) {
@@ -1294,6 +1263,7 @@ __global__ __launch_bounds__(256) void k_c4_lds(const uint64_t *foff, const uint
                         const uint32_t old = atomicCAS(&K[h], kC4Empty, w);
                         if (old == kC4Empty || old == w) {
                             atomicAdd(&V[h], 1u);
+                            if (W) atomicAdd(&WT[h], (unsigned long long)tuv + tf[vb + k]);
                             if (old == kC4Empty && atomicAdd(&fill[wv], 1u) + 1 >= kC4Slots * 3 / 4)
                                 atomicOr(&full[wv], 1u);
                             done = true;
@@ -1312,7 +1282,12 @@ __global__ __launch_bounds__(256) void k_c4_lds(const uint64_t *foff, const uint
         if (!full[wv]) {
             for (uint32_t t = lane; t < kC4Slots; t += 64) {
                 const unsigned long long x = V[t];
-                add128(clo, chi, x * (x - (x > 0)) / 2, 0);
+                if (W) {
+                    const unsigned __int128 q = (unsigned __int128)(x - (x > 0)) * WT[t];
+                    add128(clo, chi, (unsigned long long)q, (unsigned long long)(q >> 64));
+                } else {
+                    add128(clo, chi, x * (x - (x > 0)) / 2, 0);
+                }
             }
         } else if (lane == 0) {
             ovf_list[atomicAdd(ovf_count, 1u)] = (uint32_t)u;
@@ -1320,7 +1295,11 @@ __global__ __launch_bounds__(256) void k_c4_lds(const uint64_t *foff, const uint
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        for (uint32_t t = lane; t < kC4Slots; t += 64) { K[t] = kC4Empty; V[t] = 0; }
+        for (uint32_t t = lane; t < kC4Slots; t += 64) {
+            K[t] = kC4Empty;
+            V[t] = 0;
+            if (W) WT[t] = 0;
+        }
         if (lane == 0) { fill[wv] = 0; full[wv] = 0; }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
@@ -1409,9 +1388,11 @@ __global__ void k_c4d_clear(uint32_t *cnt, uint64_t total)
 // the lanes adding 1 at every neighbour of v below u (atomics whose results
 // are not read).  ipre is the exclusive prefix of the vertices' neighbour
 // counts below them.
+template <bool W>
 __global__ __launch_bounds__(256) void k_c4d_insert(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg,
                                                     const uint32_t *list, const uint64_t *ipre, uint32_t nb,
-                                                    uint32_t n, uint32_t *cnt)
+                                                    uint32_t n, uint32_t *cnt, const uint32_t *tf,
+                                                    unsigned long long *wsum)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -1426,20 +1407,94 @@ __global__ __launch_bounds__(256) void k_c4d_insert(const uint64_t *foff, const 
         const uint32_t u = list[lo];
         const uint64_t ub = foff[u], ku = ((uint64_t)sdeg[u] << 32) | u;
         const uint32_t v = (uint32_t)fkey[ub + (t - ipre[lo])];
+        const uint32_t tuv = W ? tf[ub + (t - ipre[lo])] : 0;
         const uint64_t vb = foff[v];
         const uint64_t qv = lower_u64(fkey + vb, foff[v + 1] - vb, ku);
         uint32_t *const cu = cnt + (uint64_t)lo * n;
-        for (uint64_t k = lane; k < qv; k += 64) atomicAdd(&cu[(uint32_t)fkey[vb + k]], 1u);
+        unsigned long long *const wu = W ? wsum + (uint64_t)lo * n : wsum;
+        for (uint64_t k = lane; k < qv; k += 64) {
+            const uint32_t w = (uint32_t)fkey[vb + k];
+            atomicAdd(&cu[w], 1u);
+            if (W) atomicAdd(&wu[w], (unsigned long long)tuv + tf[vb + k]);
+        }
     }
 }
 
-__global__ void k_c4d_sum(const uint32_t *cnt, uint64_t total, unsigned long long *partial)
+template <bool W>
+__global__ void k_c4d_sum(const uint32_t *cnt, uint64_t total, const unsigned long long *wsum,
+                          unsigned long long *partial)
 {
     unsigned long long lo = 0, hi = 0;
     GRID_STRIDE(t, total)
     {
         const unsigned long long x = cnt[t];
-        add128(lo, hi, x * (x - (x > 0)) / 2, 0);
+        if (W) {
+            const unsigned __int128 q = (unsigned __int128)(x - (x > 0)) * wsum[t];
+            add128(lo, hi, (unsigned long long)q, (unsigned long long)(q >> 64));
+        } else {
+            add128(lo, hi, x * (x - (x > 0)) / 2, 0);
+        }
+    }
+    block_sum128_store(lo, hi, partial);
+}
+
+__global__ void k_clear_u64(unsigned long long *p, uint64_t total)
+{
+    GRID_STRIDE(t, total) p[t] = 0;
+}
+
+// 5-houses: triangles per edge.  One wavefront per oriented edge u -> v finds
+// the common out-neighbours x of u and v (each triangle once, at the edge
+// between its two lowest vertices) and adds 1 to the counts of its three
+// edges u -> v, u -> x and v -> x (te: one count per out-list entry).
+__global__ __launch_bounds__(256) void k_tri_edges(const uint64_t *ooff, const uint32_t *onb, const uint32_t *osrc,
+                                                   uint64_t ne, uint32_t *te)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t e = wave; e < ne; e += nwaves) {
+        const uint32_t u = osrc[e], v = onb[e];
+        const uint64_t ub = ooff[u], du = ooff[u + 1] - ub, vb = ooff[v], dv = ooff[v + 1] - vb;
+        // search the entries of the list that costs fewer passes times depth in the other
+        const bool from_u = ((du + 63) >> 6) * clog2_u64(dv + 1) <= ((dv + 63) >> 6) * clog2_u64(du + 1);
+        const uint64_t abase = from_u ? ub : vb, na = from_u ? du : dv;
+        const uint64_t bbase = from_u ? vb : ub, nb = from_u ? dv : du;
+        unsigned long long c = 0;
+        for (uint64_t i = lane; i < na; i += 64) {
+            const uint32_t x = onb[abase + i];
+            uint64_t lo = 0, hi = nb;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                if (onb[bbase + mid] < x) lo = mid + 1; else hi = mid;
+            }
+            if (lo < nb && onb[bbase + lo] == x) {
+                atomicAdd(&te[abase + i], 1u);
+                atomicAdd(&te[bbase + lo], 1u);
+                ++c;
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0 && c) atomicAdd(&te[e], (uint32_t)c);
+    }
+}
+
+// The per-edge triangle counts onto the full sorted lists: edge u -> v's
+// count at v's entry in u's list and at u's entry in v's list (found by key);
+// with the sum over edges of t (t - 1) in 128 bits.
+__global__ void k_tf_scatter(const uint32_t *onb, const uint32_t *osrc, uint64_t ne, const uint32_t *te,
+                             const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg, uint32_t *tf,
+                             unsigned long long *partial)
+{
+    unsigned long long lo = 0, hi = 0;
+    GRID_STRIDE(e, ne)
+    {
+        const uint32_t u = osrc[e], v = onb[e], t = te[e];
+        const uint64_t kv = ((uint64_t)sdeg[v] << 32) | v, ku = ((uint64_t)sdeg[u] << 32) | u;
+        tf[foff[u] + lower_u64(fkey + foff[u], foff[u + 1] - foff[u], kv)] = t;
+        tf[foff[v] + lower_u64(fkey + foff[v], foff[v + 1] - foff[v], ku)] = t;
+        add128(lo, hi, (unsigned long long)t * (t ? t - 1 : 0), 0);
     }
     block_sum128_store(lo, hi, partial);
 }
@@ -1653,19 +1708,6 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         *out = (uint64_t)total;
         return ZG_OK;
     }
-    if (what == 5) {
-        const uint64_t ne = nf;
-        const int grid = (int)std::min<uint64_t>((uint64_t)device_cus() * 8, std::max<uint64_t>(1, (ne + 3) / 4));
-        XB<unsigned long long> part;
-        XALLOC(part, grid);
-        hipLaunchKernelGGL(k_house, dim3(grid), dim3(256), 0, st, foff.p, fnb.p, fsrc.p, ne, part.p);
-        ZG_HIP(hipGetLastError());
-        unsigned long long s = 0;
-        int rc = host_sum(part.p, grid, st, &s);
-        if (rc) return rc;
-        *out = s / 2;
-        return ZG_OK;
-    }
     if (what == 2 || what == 3) {
         const int grid = gridx(n), pgrid = ew_grid((2 * (uint64_t)g->m + 63) >> 6);
         XB<unsigned long long> pw, pp;
@@ -1807,9 +1849,9 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         *out = total;
         return ZG_OK;
     }
-    if (what == 6 || what == 7) {
+    if (what >= 5) {
         // full simple adjacency as order keys, every list sorted by key
-        if (nf >= (1ull << 32)) return set_error(ZG_EINVAL, "4-cycle and 5-chain counts need 2m < 2^32");
+        if (nf >= (1ull << 32)) return set_error(ZG_EINVAL, "house, 4-cycle and 5-chain counts need 2m < 2^32");
         XB<uint64_t> fkey;
         XALLOC(fkey, nf);
         {
@@ -1828,6 +1870,31 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         }
         fnb.reset();
         fsrc.reset();
+        // 5-houses = sum_e t_e c4_e - 2 sum_e t_e (t_e - 1), t_e and c4_e the
+        // triangles and 4-cycles through edge e (a house is an edge in both a
+        // triangle and a 4-cycle whose apex is off the cycle; over its two
+        // automorphisms).  sum_e t_e c4_e is the 4-cycle count below with each
+        // cycle weighted by its four edges' triangle counts (tf).
+        const bool weighted = what == 5;
+        XB<uint32_t> tf;
+        unsigned __int128 tt = 0;   // sum over edges of t_e (t_e - 1)
+        if (weighted) {
+            XB<uint32_t> te;
+            XALLOC(te, ne ? ne : 1);
+            XALLOC(tf, nf);
+            hipLaunchKernelGGL(k_c4d_clear, dim3(gridx(ne)), dim3(256), 0, st, te.p, ne);
+            const int tgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16, std::max<uint64_t>(1, (ne + 3) / 4));
+            hipLaunchKernelGGL(k_tri_edges, dim3(tgrid), dim3(256), 0, st, ooff.p, onb.p, osrc.p, ne, te.p);
+            ZG_HIP(hipGetLastError());
+            const int sgrid = gridx(ne);
+            XB<unsigned long long> tpart;
+            XALLOC(tpart, 2 * (uint64_t)sgrid);
+            hipLaunchKernelGGL(k_tf_scatter, dim3(sgrid), dim3(256), 0, st, onb.p, osrc.p, ne, te.p, foff.p, fkey.p,
+                               sdeg.p, tf.p, tpart.p);
+            ZG_HIP(hipGetLastError());
+            const int rc = host_sum128(tpart.p, sgrid, st, &tt);   // synchronizes before te is freed
+            if (rc) return rc;
+        }
         // 4-cycles: LDS hashes, then the vertices that overflowed them with global hashes
         const int cgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 8, std::max<uint64_t>(1, ((uint64_t)n + 3) / 4));
         XB<unsigned long long> cpart;
@@ -1835,10 +1902,14 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
         XB<unsigned int> novf_d;
         XALLOC(cpart, 2 * (uint64_t)cgrid); XALLOC(ovf, n); XALLOC(novf_d, 1);
         ZG_HIP(hipMemsetAsync(novf_d.p, 0, sizeof(unsigned int), st));
-        hipLaunchKernelGGL(k_c4_lds, dim3(cgrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p, n, cpart.p, ovf.p,
-                           novf_d.p);
+        if (weighted)
+            hipLaunchKernelGGL(k_c4_lds<true>, dim3(cgrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p, n, cpart.p,
+                               ovf.p, novf_d.p, tf.p);
+        else
+            hipLaunchKernelGGL(k_c4_lds<false>, dim3(cgrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p, n, cpart.p,
+                               ovf.p, novf_d.p, nullptr);
         ZG_HIP(hipGetLastError());
-        unsigned __int128 c4 = 0;   // 4-cycles, summed in 128 bits (see the bound at the end)
+        unsigned __int128 c4 = 0;   // 4-cycles (houses: their weights), summed in 128 bits
         int rc = host_sum128(cpart.p, cgrid, st, &c4);
         if (rc) return rc;
         unsigned int novf = 0;
@@ -1879,7 +1950,8 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
             uint64_t tmax = 0;
             uint32_t nbmax = 0;
             for (uint32_t i = 0; i < novf;) {
-                if (hwedge[i] > cap) { dense.push_back(hlist[i]); ++i; continue; }
+                // the sorted batches count without weights: houses take the dense counters
+                if (weighted || hwedge[i] > cap) { dense.push_back(hlist[i]); ++i; continue; }
                 uint64_t tot = 0;
                 uint32_t j = i;
                 while (j < novf && j - i < nbcap && hwedge[j] <= cap && tot + hwedge[j] <= cap) tot += hwedge[j++];
@@ -1968,14 +2040,17 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
             if (!dense.empty()) {
                 // one dense counter over the n vertices per vertex, `per` vertices at a time
                 const uint32_t nd = (uint32_t)dense.size();
-                const uint32_t per = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nd, budget / (4ull * n)));
+                const uint64_t per_vertex = (weighted ? 12ull : 4ull) * n;   // counts (+ weights) over the n vertices
+                const uint32_t per = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nd, budget / per_vertex));
                 XB<uint32_t> cnt, dlist;
+                XB<unsigned long long> wsum;
                 XB<uint64_t> d_ipre;
                 XALLOC(cnt, (uint64_t)per * n); XALLOC(dlist, nd); XALLOC(d_ipre, (uint64_t)per + 1);
+                if (weighted) XALLOC(wsum, (uint64_t)per * n);
                 ZG_HIP(hipMemcpyAsync(dlist.p, dense.data(), 4ull * nd, hipMemcpyHostToDevice, st));
                 std::vector<uint64_t> dbelow(nd);
                 for (uint32_t i = 0, k = 0; i < novf; ++i)
-                    if (hwedge[i] > cap) dbelow[k++] = hbelow[i];
+                    if (weighted || hwedge[i] > cap) dbelow[k++] = hbelow[i];
                 const int sgrid = gridx((uint64_t)per * n);
                 XB<unsigned long long> gpart;
                 XALLOC(gpart, 2 * (uint64_t)sgrid);
@@ -1986,14 +2061,23 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                     const uint64_t total = (uint64_t)nb * n;
                     ZG_HIP(hipMemcpyAsync(d_ipre.p, ipre.data(), 8ull * (nb + 1), hipMemcpyHostToDevice, st));
                     hipLaunchKernelGGL(k_c4d_clear, dim3(gridx(total)), dim3(256), 0, st, cnt.p, total);
+                    if (weighted) hipLaunchKernelGGL(k_clear_u64, dim3(gridx(total)), dim3(256), 0, st, wsum.p, total);
                     ZG_HIP(hipGetLastError());
                     const int igrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16,
                                                               std::max<uint64_t>(1, (ipre[nb] + 3) / 4));
-                    hipLaunchKernelGGL(k_c4d_insert, dim3(igrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p,
-                                       dlist.p + i0, d_ipre.p, nb, n, cnt.p);
+                    if (weighted)
+                        hipLaunchKernelGGL(k_c4d_insert<true>, dim3(igrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p,
+                                           dlist.p + i0, d_ipre.p, nb, n, cnt.p, tf.p, wsum.p);
+                    else
+                        hipLaunchKernelGGL(k_c4d_insert<false>, dim3(igrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p,
+                                           dlist.p + i0, d_ipre.p, nb, n, cnt.p, nullptr, nullptr);
                     ZG_HIP(hipGetLastError());
                     const int g2 = gridx(total);
-                    hipLaunchKernelGGL(k_c4d_sum, dim3(g2), dim3(256), 0, st, cnt.p, total, gpart.p);
+                    if (weighted)
+                        hipLaunchKernelGGL(k_c4d_sum<true>, dim3(g2), dim3(256), 0, st, cnt.p, total, wsum.p, gpart.p);
+                    else
+                        hipLaunchKernelGGL(k_c4d_sum<false>, dim3(g2), dim3(256), 0, st, cnt.p, total, nullptr,
+                                           gpart.p);
                     ZG_HIP(hipGetLastError());
                     unsigned __int128 g4 = 0;
                     rc = host_sum128(gpart.p, g2, st, &g4);   // synchronizes before ipre changes
@@ -2001,6 +2085,13 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                     c4 += g4;
                 }
             }
+        }
+        if (weighted) {
+            if (c4 < 2 * tt) return set_error(ZG_EINVAL, "house count: inconsistent per-edge counts");
+            const unsigned __int128 h = c4 - 2 * tt;
+            if (h >> 64) return set_error(ZG_EOVERFLOW, "5-house count exceeds 64 bits");
+            *out = (uint64_t)h;
+            return ZG_OK;
         }
         // A simple graph with m edges has at most C(m, 2) 4-cycles (each holds two pairs of
         // disjoint edges, and each such pair closes at most two 4-cycles), so with 2m < 2^32

@@ -598,3 +598,19 @@ def test_line_model_basic_costs(gi):
     assert k < tri < (4 if gi < 2 else 8) * k   # the multigraph's copies overflow buckets: longer walks
     with pytest.raises(ValueError):
         O.device_lines(g, O.PLAN_FOUR_CHAIN, 1, 0, 16)
+
+
+def test_house_closed_form_equals_brute_force():
+    # houses = sum_e t_e c4_e - 2 sum_e t_e (t_e - 1): equal to the brute force
+    # over pattern maps on random graphs, multigraphs (their simple graphs) and
+    # K_n, and to FiveHouse.cpp:68's count for the reference's wiki-vote fixture
+    for k in (5, 6, 8):
+        n, s, d = H.complete_graph(k)
+        assert O.OGraph(n, s, d).house() == math.comb(k, 5) * 60
+    graphs = [H.random_graph(20 + 5 * i, 0.3, i) for i in range(6)]
+    graphs += [H.multigraph(25, 0.3, 7), H.power_law_graph(60, 300, 2.1, 3), H.random_graph(40, 0.8, 9)]
+    for n, s, d in graphs:
+        g = O.OGraph(n, s, d)
+        assert g.house() == g.house_brute()
+    n, s, d = H.golden_graph("wiki-vote")
+    assert O.OGraph(n, s, d).house() == 2134981304
