@@ -1366,6 +1366,62 @@ __global__ __launch_bounds__(256) void k_c4s_fill(const uint64_t *foff, const ui
     }
 }
 
+// Weighted sorted counting (5-houses): every wedge u - v - w of a batch's
+// vertex u writes the key (i << 32 | w) with the value {1, t(u, v) + t(v, w)};
+// after sorting the pairs and reducing them by key, a run {c, s} adds
+// (c - 1) * s.
+struct CW {
+    unsigned long long c, s;
+};
+struct CWPlus {
+    __host__ __device__ CW operator()(const CW &a, const CW &b) const { return CW{a.c + b.c, a.s + b.s}; }
+};
+
+__global__ __launch_bounds__(256) void k_c4s_fill_w(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg,
+                                                    const uint32_t *list, const uint64_t *ipre, uint32_t nb,
+                                                    unsigned long long *cursor, const uint32_t *tf, uint64_t *keys,
+                                                    CW *vals)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t items = ipre[nb];
+    for (uint64_t t = wave; t < items; t += nwaves) {
+        uint32_t lo = 0, hi = nb;   // ipre[lo] <= t < ipre[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (ipre[mid] <= t) lo = mid; else hi = mid;
+        }
+        const uint32_t u = list[lo];
+        const uint64_t ub = foff[u], ku = ((uint64_t)sdeg[u] << 32) | u;
+        const uint64_t j = ub + (t - ipre[lo]);
+        const uint32_t v = (uint32_t)fkey[j];
+        const unsigned long long tuv = tf[j];
+        const uint64_t vb = foff[v];
+        const uint64_t qv = lower_u64(fkey + vb, foff[v + 1] - vb, ku);
+        unsigned long long base = 0;
+        if (lane == 0 && qv) base = atomicAdd(&cursor[lo], (unsigned long long)qv);
+        base = __shfl(base, 0, 64);
+        for (uint64_t k = lane; k < qv; k += 64) {
+            keys[base + k] = ((uint64_t)lo << 32) | (uint32_t)fkey[vb + k];
+            vals[base + k] = CW{1ull, tuv + tf[vb + k]};
+        }
+    }
+}
+
+__global__ void k_c4s_pairs_w(const CW *runs, const unsigned int *nruns, unsigned long long *partial)
+{
+    const uint64_t n = *nruns;
+    unsigned long long lo = 0, hi = 0;
+    GRID_STRIDE(t, n)
+    {
+        const CW r = runs[t];
+        const unsigned __int128 q = (unsigned __int128)(r.c - (r.c > 0)) * r.s;
+        add128(lo, hi, (unsigned long long)q, (unsigned long long)(q >> 64));
+    }
+    block_sum128_store(lo, hi, partial);
+}
+
 __global__ void k_c4s_pairs(const unsigned int *counts, const unsigned int *nruns, unsigned long long *partial)
 {
     const uint64_t runs = *nruns;
@@ -1932,7 +1988,8 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
             const uint64_t budget = std::min<uint64_t>(16ull << 30, std::max<uint64_t>(1ull << 30, free_b / 4));
             // sorted batches of consecutive vertices, up to `cap` wedges each (about 24 bytes per
             // wedge); a vertex with more wedges than that goes to the dense counters instead
-            uint64_t cap = std::min<uint64_t>(budget / 24, 0xF0000000ull);
+            // (houses: 64-bit keys and 16-byte {count, weight} values, about 88 bytes per wedge)
+            uint64_t cap = std::min<uint64_t>(budget / (weighted ? 88 : 24), 0xF0000000ull);
             if (const char *ev = getenv("ZGRAPH_C4_SORT_WEDGES")) cap = std::min<uint64_t>(cap, strtoull(ev, nullptr, 10));
             // keys (batch index << wbits | w): 32-bit while ceil(log2 n) + the batch index's bits fit,
             // which caps a batch at 2^(32 - wbits) vertices; 64-bit keys past n = 2^30
@@ -1941,7 +1998,7 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
             // ZGRAPH_C4_WIDE_KEYS=1 forces the 64-bit keys and ZGRAPH_C4_BATCH_VERTICES caps the
             // vertices per batch (tests reach those branches on small graphs with them)
             const char *ew = getenv("ZGRAPH_C4_WIDE_KEYS");
-            const bool narrow = wbits <= 30 && !(ew && ew[0] == '1');
+            const bool narrow = wbits <= 30 && !(ew && ew[0] == '1') && !weighted;
             uint32_t nbcap = narrow ? (1u << (32 - wbits)) : 0xffffffffu;
             if (const char *ev = getenv("ZGRAPH_C4_BATCH_VERTICES"))
                 nbcap = std::max<uint32_t>(1, std::min<uint32_t>(nbcap, (uint32_t)strtoul(ev, nullptr, 10)));
@@ -1950,8 +2007,7 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
             uint64_t tmax = 0;
             uint32_t nbmax = 0;
             for (uint32_t i = 0; i < novf;) {
-                // the sorted batches count without weights: houses take the dense counters
-                if (weighted || hwedge[i] > cap) { dense.push_back(hlist[i]); ++i; continue; }
+                if (hwedge[i] > cap) { dense.push_back(hlist[i]); ++i; continue; }
                 uint64_t tot = 0;
                 uint32_t j = i;
                 while (j < novf && j - i < nbcap && hwedge[j] <= cap && tot + hwedge[j] <= cap) tot += hwedge[j++];
@@ -1960,7 +2016,63 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                 nbmax = std::max(nbmax, j - i);
                 i = j;
             }
-            if (!batches.empty() && tmax > 0) {
+            if (weighted && !batches.empty() && tmax > 0) {
+                XB<uint64_t> ka, kb, uk, d_ipre;
+                XB<CW> va, vb, agg;
+                XB<unsigned int> nruns;
+                XB<unsigned long long> cursor;
+                XALLOC(ka, tmax); XALLOC(kb, tmax); XALLOC(uk, tmax);
+                XALLOC(va, tmax); XALLOC(vb, tmax); XALLOC(agg, tmax); XALLOC(nruns, 1);
+                XALLOC(d_ipre, (uint64_t)nbmax + 1); XALLOC(cursor, nbmax);
+                size_t ts = 0, tr = 0;
+                ZG_HIP(rocprim::radix_sort_pairs(nullptr, ts, ka.p, kb.p, va.p, vb.p, (size_t)tmax, 0, 64, st));
+                ZG_HIP(rocprim::reduce_by_key(nullptr, tr, kb.p, vb.p, (size_t)tmax, uk.p, agg.p, nruns.p, CWPlus(),
+                                              rocprim::equal_to<uint64_t>(), st));
+                XB<char> tmps;
+                XALLOC(tmps, std::max(ts, tr));
+                const int pgrid = gridx(tmax);
+                XB<unsigned long long> spart;
+                XALLOC(spart, 2 * (uint64_t)pgrid);
+                for (const auto &b : batches) {
+                    const uint32_t nb = b.second - b.first;
+                    std::vector<uint64_t> ipre(nb + 1, 0), cur(nb, 0);
+                    uint64_t tot = 0;
+                    for (uint32_t i = 0; i < nb; ++i) {
+                        ipre[i + 1] = ipre[i] + hbelow[b.first + i];
+                        cur[i] = tot;
+                        tot += hwedge[b.first + i];
+                    }
+                    if (tot == 0) continue;
+                    uint32_t kbits = 1;
+                    while ((1ull << kbits) < nb) ++kbits;
+                    ZG_HIP(hipMemcpyAsync(d_ipre.p, ipre.data(), 8ull * (nb + 1), hipMemcpyHostToDevice, st));
+                    ZG_HIP(hipMemcpyAsync(cursor.p, cur.data(), 8ull * nb, hipMemcpyHostToDevice, st));
+                    const int fgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16,
+                                                              std::max<uint64_t>(1, (ipre[nb] + 3) / 4));
+                    hipLaunchKernelGGL(k_c4s_fill_w, dim3(fgrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p,
+                                       ovf.p + b.first, d_ipre.p, nb, cursor.p, tf.p, ka.p, va.p);
+                    ZG_HIP(hipGetLastError());
+                    size_t t1 = 0, t2 = 0;
+                    ZG_HIP(rocprim::radix_sort_pairs(nullptr, t1, ka.p, kb.p, va.p, vb.p, (size_t)tot, 0,
+                                                     32 + kbits, st));
+                    ZG_HIP(rocprim::reduce_by_key(nullptr, t2, kb.p, vb.p, (size_t)tot, uk.p, agg.p, nruns.p,
+                                                  CWPlus(), rocprim::equal_to<uint64_t>(), st));
+                    if (std::max(t1, t2) > std::max(ts, tr))
+                        return set_error(ZG_ENOMEM, "house count: sort scratch larger than sized");
+                    t1 = t2 = std::max(ts, tr);
+                    ZG_HIP(rocprim::radix_sort_pairs(tmps.p, t1, ka.p, kb.p, va.p, vb.p, (size_t)tot, 0,
+                                                     32 + kbits, st));
+                    ZG_HIP(rocprim::reduce_by_key(tmps.p, t2, kb.p, vb.p, (size_t)tot, uk.p, agg.p, nruns.p,
+                                                  CWPlus(), rocprim::equal_to<uint64_t>(), st));
+                    const int g2 = gridx(tot);
+                    hipLaunchKernelGGL(k_c4s_pairs_w, dim3(g2), dim3(256), 0, st, agg.p, nruns.p, spart.p);
+                    ZG_HIP(hipGetLastError());
+                    unsigned __int128 g4 = 0;
+                    rc = host_sum128(spart.p, g2, st, &g4);   // synchronizes before ipre and cur change
+                    if (rc) return rc;
+                    c4 += g4;
+                }
+            } else if (!batches.empty() && tmax > 0) {
                 XB<uint64_t> ka, kb, d_ipre;
                 XB<unsigned int> counts, nruns;
                 XB<unsigned long long> cursor;
@@ -2050,7 +2162,7 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                 ZG_HIP(hipMemcpyAsync(dlist.p, dense.data(), 4ull * nd, hipMemcpyHostToDevice, st));
                 std::vector<uint64_t> dbelow(nd);
                 for (uint32_t i = 0, k = 0; i < novf; ++i)
-                    if (weighted || hwedge[i] > cap) dbelow[k++] = hbelow[i];
+                    if (hwedge[i] > cap) dbelow[k++] = hbelow[i];
                 const int sgrid = gridx((uint64_t)per * n);
                 XB<unsigned long long> gpart;
                 XALLOC(gpart, 2 * (uint64_t)sgrid);

@@ -208,6 +208,8 @@ def test_device_c4_and_five_chains_of_complete_graphs(monkeypatch, n, sort_wedge
     g = Z.Graph.from_edges(s, d, n, seed=2)
     assert g.exact_count("c4") == 3 * comb(n, 4)
     assert g.exact_count("path4") == perm(n, 5) // 2
+    # houses: 5! / 2 per 5 vertices, through the same LDS, sorted (weighted) and dense paths
+    assert g.exact_count("house") == 60 * comb(n, 5)
 
 
 @pytest.mark.parametrize("a,b,sort_wedges", [(2, 60000, None), (300, 300, None), (300, 300, "5000")])
@@ -222,6 +224,7 @@ def test_device_c4_complete_bipartite(monkeypatch, a, b, sort_wedges):
     u, v = np.meshgrid(np.arange(a), np.arange(a, a + b), indexing="ij")
     g = Z.Graph.from_edges(u.ravel().astype(np.uint32), v.ravel().astype(np.uint32), a + b, seed=3)
     assert g.exact_count("c4") == comb(a, 2) * comb(b, 2)
+    assert g.exact_count("house") == 0   # no triangles
 
 
 @pytest.mark.parametrize("env", [{}, {"ZGRAPH_C4_WIDE_KEYS": "1"}, {"ZGRAPH_C4_BATCH_VERTICES": "3"},
@@ -240,6 +243,18 @@ def test_device_c4_and_five_chains_rmat14(monkeypatch, env):
     g = Z.Graph.from_edges(s, d, 1 << 14, seed=1)
     assert g.exact_count("c4") == og.c4()
     assert g.exact_count("path4") == og.paths4()
+    assert g.exact_count("house") == og.house()
+
+
+def test_device_house_count_through_the_dense_weighted_counters(monkeypatch):
+    """R-MAT 14 with the hub batches limited to 2 000 wedges: the larger hubs'
+    weighted wedges go to the dense counters, the rest to the sorted batches;
+    the count equals the oracle's closed form."""
+    monkeypatch.setenv("ZGRAPH_C4_SORT_WEDGES", "2000")
+    s, d = O.gen_rmat(14, 16 << 14, 0.57, 0.19, 0.19, 3)
+    og = O.OGraph(1 << 14, s, d, seed=1)
+    g = Z.Graph.from_edges(s, d, 1 << 14, seed=1)
+    assert g.exact_count("house") == og.house()
 
 
 @pytest.mark.parametrize("order,what", [(4, "path3"), (5, "path4")])
