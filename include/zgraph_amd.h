@@ -288,16 +288,18 @@ int zg_count_reads(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_
  * Non-induced counts on the simple graph underlying the edge list: what = 0
  * triangles (TriangleCounting.cpp:44-72), 1 4-cliques (FourClique.cpp:44-78),
  * 2 wedges (ThreeChain.cpp:44-51), 3 three-edge paths (FourChain.cpp:46-66),
- * 4 5-stars (FiveStar.cpp:51-59), 5 5-houses (FiveHouse.cpp; cost grows as
- * the sum over edges of d_a * d_c: meant for graphs up to a few million edges),
- * 6 5-vertex chains (FiveChain.cpp:45-72, simple paths of four edges), 7
- * 4-cycles (no reference program; the 5-chain count needs them).  Counts
+ * 4 5-stars (FiveStar.cpp:51-59), 5 5-houses (FiveHouse.cpp's count, as
+ * sum_e t_e c4_e - 2 sum_e t_e (t_e - 1) with t_e and c4_e the triangles and
+ * 4-cycles through edge e; needs 2m < 2^32 like the 4-cycles), 6 5-vertex
+ * chains (FiveChain.cpp:45-72, simple paths of four edges), 7 4-cycles (no
+ * reference program; the 5-chain and house counts need them).  Counts
  * that exceed 64 bits fail with ZG_EOVERFLOW (5-stars, 5-chains; 4-cycles are
  * summed in 128 bits and checked too, though at most C(m, 2) < 2^61 of them
  * exist for m < 2^31).  Scratch: 4-cliques work in vertex ranges
  * whose bit rows fit min(32 GiB, max(1 GiB, free / 3)) (the environment
  * variable ZGRAPH_K4_SCRATCH_WORDS lowers it, in 8-byte words); 4-cycles and
  * 5-chains sort the hubs' wedges in batches of up to min(16 GiB, max(1 GiB, free / 4)) / 24 wedges
+ * (houses: / 88, their wedges carrying weights)
  * (ZGRAPH_C4_SORT_WEDGES lowers it; a hub with more wedges is counted with
  * a dense counter of n entries instead; ZGRAPH_C4_WIDE_KEYS=1 and
  * ZGRAPH_C4_BATCH_VERTICES force the 64-bit sort keys and cap the vertices of
