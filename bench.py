@@ -215,7 +215,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(Z, g, plans, args, N, elp)
+        cpu = cpu_baseline(Z, g, plans, args, N, elp, house)
 
     out = {
         "metric": "estimators/s; time to ±1% err @95% conf, learned vs regression ELP",
@@ -906,7 +906,7 @@ def profiled_traffic(kernel, args, N):
     return per if same else None
 
 
-def cpu_baseline(Z, g, plans, args, N, elp=None):
+def cpu_baseline(Z, g, plans, args, N, elp=None, house=None):
     """The oracle's OpenMP restatement of ZGraphInstance::calculate on this
     host, on a bounded number of estimators of the same workload.  With the
     time-to-error results, also the CPU time of the first learned triangle
@@ -985,8 +985,24 @@ def cpu_baseline(Z, g, plans, args, N, elp=None):
                          "cpu_s_projected": cpu_s, "gpu_s": rec["mean_time_s"],
                          "speedup": cpu_s / rec["mean_time_s"] if rec["mean_time_s"] else None}
         log(f"cpu time to +-1% ({label}, projected): {cpu_s:.4g}s against {rec['mean_time_s']:.3f}s on the GPU")
+    # 5-houses to +-1 % with the split plan: the estimators the GPU kernel's CV asks for, at the
+    # CPU loop's rate on a bounded sample of the same plan
+    tte_house = None
+    if house and house.get("cv") and house["cv"] != float("inf"):
+        J = house["completions"]
+        n_s = int(max(1 << 16, rates["4-clique"] * args.cpu_seconds / 4))
+        t = time.perf_counter()
+        O.run(og, O.make_plan(O.PLAN_HOUSE_SPLIT, order=J), args.seed, 0, n_s, threads)
+        rate = n_s / (time.perf_counter() - t)
+        need = (1.959964 * house["cv"] / 0.01) ** 2
+        tte_house = {"plan": f"5-house split x{J}", "estimators": need, "cpu_rate": rate, "cpu_s_projected": need / rate,
+                     "gpu_s_projected": house["projected_s_to_1pct"],
+                     "speedup": need / rate / house["projected_s_to_1pct"]}
+        log(f"cpu time to +-1% (5-house split x{J}, projected): {need / rate:.4g}s at {rate:.4g}/s against "
+            f"{house['projected_s_to_1pct'] * 1e3:.1f} ms on the GPU")
     return {"value": 2 * N / per_step, "unit": "estimators/s", "cores": threads, "kind": "port",
             "time_to_error": tte, "time_to_error_4clique_projected": tte_k4 or None,
+            "time_to_error_5house_projected": tte_house,
             "sample": "; ".join(f"{k}: {samples[k]} estimators (ids 0..) of the same graph/seed"
                                  for k in samples),
             "per_pattern": rates, "cpu_model": host["cpu_model"], "host": host,
