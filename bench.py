@@ -348,7 +348,10 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95, patterns=None, cap_log2=None,
     The split 4-clique plan (ZG_PLAN_K4_SPLIT, `split_completions` draws of the
     shapes per closed triangle) has no slot in the network, so the learned
     profile sizes it from its pilot (learned+judge = pilot+judge there); its
-    speedup is also reported against the 4-clique plan's times."""
+    speedup is also reported against the 4-clique plan's times.  With the
+    default patterns, 4-cliques and 5-houses also run by the split plans with
+    their completions chosen per query (split_auto_arm), errors against the
+    device exact counts."""
     import math
     from zgraph import elp as L
     weights = args.elp_weights or L.DEFAULT_WEIGHTS
@@ -420,8 +423,10 @@ def time_to_error(Z, g, args, eps=0.01, conf=0.95, patterns=None, cap_log2=None,
         res["queries_per_arm"] = nq
         out[name] = res
     if patterns_default and learned is not None:
-        # 4-cliques by the split plan with its completions chosen by two pilots
+        # 4-cliques and 5-houses by the split plans with their completions chosen by pilots
         out["4-clique split, completions chosen"] = split_auto_arm(Z, L, g, learned, eps, conf, queries, cap_log2)
+        out["5-house split, completions chosen"] = split_auto_arm(Z, L, g, learned, eps, conf, queries, cap_log2,
+                                                                 kind=Z.PLAN_HOUSE_SPLIT, what="house")
     if patterns_default and learned is not None:
         # triangles and 4-cliques (split plan) both to +-eps by one query over one id range
         tri, k4 = g.exact_count("triangle"), g.exact_count("k4")
