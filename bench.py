@@ -673,8 +673,32 @@ def house_sweep(Z, args):
         rows[gname]["5-house split, completions chosen"] = split_auto_arm(
             Z, L, g, prof, 0.01, 0.95, 3, 34, kind=Z.PLAN_HOUSE_SPLIT, what="house")
         rows[gname]["kernels"] = house_kernels(Z, g, gname)
+        if gname in ("chunglu 4.8M/69M b2.1", "rmat22"):
+            rows[gname]["coverage"] = split_coverage(Z, L, g, prof, Z.PLAN_HOUSE_SPLIT, "house")
+            if gname.startswith("chunglu"):
+                rows[gname]["coverage_4clique"] = split_coverage(Z, L, g, prof, Z.PLAN_K4_SPLIT, "k4")
         g.close()
     print(json.dumps({"house_sweep": rows}), flush=True)
+
+
+def split_coverage(Z, L, g, prof, kind, what, queries=40, eps=0.01, conf=0.95):
+    """Fraction of split-plan answers (completions chosen per query, pilot
+    sizing, judge) within +-eps of the exact count over `queries` independent
+    queries (seeds 1000..): the judge's stopping rule should hold the nominal
+    confidence on these heavy-tailed estimators too."""
+    truth = exact_count(g, what)
+    errs, acc, ns, ts = [], 0, [], []
+    for q in range(queries):
+        r = L.split_to_error(g, eps, conf, prof, kind=kind, seed=1000 + q, max_estimators=1 << 36)
+        errs.append(abs(r["estimate"] - truth) / truth)
+        acc += bool(r["accepted"])
+        ns.append(r["estimators"])
+        ts.append(r["total_s"])
+    within = sum(e <= eps for e in errs)
+    log(f"coverage {what} split: {within}/{queries} within +-{eps * 100:.0f}%, judge accepted {acc}/{queries}, "
+        f"max error {max(errs) * 100:.2f}%, median {sorted(ts)[queries // 2] * 1e3:.1f} ms")
+    return {"queries": queries, "within_eps": within, "judge_accepted": acc, "max_error": max(errs),
+            "median_estimators": sorted(ns)[queries // 2], "median_time_s": sorted(ts)[queries // 2]}
 
 
 def house_kernels(Z, g, gname, log2n=26):
