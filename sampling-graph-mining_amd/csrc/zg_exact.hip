@@ -1367,7 +1367,7 @@ __global__ __launch_bounds__(256) void k_c4s_fill(const uint64_t *foff, const ui
 }
 
 // Weighted sorted counting (5-houses): every wedge u - v - w of a batch's
-// vertex u writes the key (i << 32 | w) with the value {1, t(u, v) + t(v, w)};
+// vertex u writes the key (i << wbits | w) with the value {1, t(u, v) + t(v, w)};
 // after sorting the pairs and reducing them by key, a run {c, s} adds
 // (c - 1) * s.
 struct CW {
@@ -1377,10 +1377,11 @@ struct CWPlus {
     __host__ __device__ CW operator()(const CW &a, const CW &b) const { return CW{a.c + b.c, a.s + b.s}; }
 };
 
+template <class Key>
 __global__ __launch_bounds__(256) void k_c4s_fill_w(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg,
                                                     const uint32_t *list, const uint64_t *ipre, uint32_t nb,
-                                                    unsigned long long *cursor, const uint32_t *tf, uint64_t *keys,
-                                                    CW *vals)
+                                                    uint32_t wbits, unsigned long long *cursor, const uint32_t *tf,
+                                                    Key *keys, CW *vals)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -1403,7 +1404,7 @@ __global__ __launch_bounds__(256) void k_c4s_fill_w(const uint64_t *foff, const 
         if (lane == 0 && qv) base = atomicAdd(&cursor[lo], (unsigned long long)qv);
         base = __shfl(base, 0, 64);
         for (uint64_t k = lane; k < qv; k += 64) {
-            keys[base + k] = ((uint64_t)lo << 32) | (uint32_t)fkey[vb + k];
+            keys[base + k] = ((Key)lo << wbits) | (Key)(uint32_t)fkey[vb + k];
             vals[base + k] = CW{1ull, tuv + tf[vb + k]};
         }
     }
@@ -1988,7 +1989,7 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
             const uint64_t budget = std::min<uint64_t>(16ull << 30, std::max<uint64_t>(1ull << 30, free_b / 4));
             // sorted batches of consecutive vertices, up to `cap` wedges each (about 24 bytes per
             // wedge); a vertex with more wedges than that goes to the dense counters instead
-            // (houses: 64-bit keys and 16-byte {count, weight} values, about 88 bytes per wedge)
+            // (houses: keys, 16-byte {count, weight} values and the reduction's outputs, up to 88 bytes per wedge)
             uint64_t cap = std::min<uint64_t>(budget / (weighted ? 88 : 24), 0xF0000000ull);
             if (const char *ev = getenv("ZGRAPH_C4_SORT_WEDGES")) cap = std::min<uint64_t>(cap, strtoull(ev, nullptr, 10));
             // keys (batch index << wbits | w): 32-bit while ceil(log2 n) + the batch index's bits fit,
@@ -1998,7 +1999,7 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
             // ZGRAPH_C4_WIDE_KEYS=1 forces the 64-bit keys and ZGRAPH_C4_BATCH_VERTICES caps the
             // vertices per batch (tests reach those branches on small graphs with them)
             const char *ew = getenv("ZGRAPH_C4_WIDE_KEYS");
-            const bool narrow = wbits <= 30 && !(ew && ew[0] == '1') && !weighted;
+            const bool narrow = wbits <= 30 && !(ew && ew[0] == '1');
             uint32_t nbcap = narrow ? (1u << (32 - wbits)) : 0xffffffffu;
             if (const char *ev = getenv("ZGRAPH_C4_BATCH_VERTICES"))
                 nbcap = std::max<uint32_t>(1, std::min<uint32_t>(nbcap, (uint32_t)strtoul(ev, nullptr, 10)));
@@ -2016,18 +2017,24 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                 nbmax = std::max(nbmax, j - i);
                 i = j;
             }
-            if (weighted && !batches.empty() && tmax > 0) {
-                XB<uint64_t> ka, kb, uk, d_ipre;
+            // houses: the hubs' weighted wedges, (key, {count, weight}) pairs sorted and reduced by key;
+            // 32-bit keys (batch index << wbits | w) while they fit, else 64-bit ones
+            auto weighted_batches = [&](auto key0) -> int {
+                using Key = decltype(key0);
+                XB<Key> ka, kb, uk;
+                XB<uint64_t> d_ipre;
                 XB<CW> va, vb, agg;
                 XB<unsigned int> nruns;
                 XB<unsigned long long> cursor;
                 XALLOC(ka, tmax); XALLOC(kb, tmax); XALLOC(uk, tmax);
                 XALLOC(va, tmax); XALLOC(vb, tmax); XALLOC(agg, tmax); XALLOC(nruns, 1);
                 XALLOC(d_ipre, (uint64_t)nbmax + 1); XALLOC(cursor, nbmax);
+                const uint32_t kw = sizeof(Key) == 4 ? wbits : 32u;   // bits of w in the key
                 size_t ts = 0, tr = 0;
-                ZG_HIP(rocprim::radix_sort_pairs(nullptr, ts, ka.p, kb.p, va.p, vb.p, (size_t)tmax, 0, 64, st));
+                ZG_HIP(rocprim::radix_sort_pairs(nullptr, ts, ka.p, kb.p, va.p, vb.p, (size_t)tmax, 0,
+                                                 (unsigned)(8 * sizeof(Key)), st));
                 ZG_HIP(rocprim::reduce_by_key(nullptr, tr, kb.p, vb.p, (size_t)tmax, uk.p, agg.p, nruns.p, CWPlus(),
-                                              rocprim::equal_to<uint64_t>(), st));
+                                              rocprim::equal_to<Key>(), st));
                 XB<char> tmps;
                 XALLOC(tmps, std::max(ts, tr));
                 const int pgrid = gridx(tmax);
@@ -2045,33 +2052,37 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                     if (tot == 0) continue;
                     uint32_t kbits = 1;
                     while ((1ull << kbits) < nb) ++kbits;
+                    const uint32_t ebits = kw + kbits;   // <= 32 for 32-bit keys (nbcap)
                     ZG_HIP(hipMemcpyAsync(d_ipre.p, ipre.data(), 8ull * (nb + 1), hipMemcpyHostToDevice, st));
                     ZG_HIP(hipMemcpyAsync(cursor.p, cur.data(), 8ull * nb, hipMemcpyHostToDevice, st));
                     const int fgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16,
                                                               std::max<uint64_t>(1, (ipre[nb] + 3) / 4));
-                    hipLaunchKernelGGL(k_c4s_fill_w, dim3(fgrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p,
-                                       ovf.p + b.first, d_ipre.p, nb, cursor.p, tf.p, ka.p, va.p);
+                    hipLaunchKernelGGL(k_c4s_fill_w<Key>, dim3(fgrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p,
+                                       ovf.p + b.first, d_ipre.p, nb, kw, cursor.p, tf.p, ka.p, va.p);
                     ZG_HIP(hipGetLastError());
                     size_t t1 = 0, t2 = 0;
-                    ZG_HIP(rocprim::radix_sort_pairs(nullptr, t1, ka.p, kb.p, va.p, vb.p, (size_t)tot, 0,
-                                                     32 + kbits, st));
+                    ZG_HIP(rocprim::radix_sort_pairs(nullptr, t1, ka.p, kb.p, va.p, vb.p, (size_t)tot, 0, ebits, st));
                     ZG_HIP(rocprim::reduce_by_key(nullptr, t2, kb.p, vb.p, (size_t)tot, uk.p, agg.p, nruns.p,
-                                                  CWPlus(), rocprim::equal_to<uint64_t>(), st));
+                                                  CWPlus(), rocprim::equal_to<Key>(), st));
                     if (std::max(t1, t2) > std::max(ts, tr))
                         return set_error(ZG_ENOMEM, "house count: sort scratch larger than sized");
                     t1 = t2 = std::max(ts, tr);
-                    ZG_HIP(rocprim::radix_sort_pairs(tmps.p, t1, ka.p, kb.p, va.p, vb.p, (size_t)tot, 0,
-                                                     32 + kbits, st));
+                    ZG_HIP(rocprim::radix_sort_pairs(tmps.p, t1, ka.p, kb.p, va.p, vb.p, (size_t)tot, 0, ebits, st));
                     ZG_HIP(rocprim::reduce_by_key(tmps.p, t2, kb.p, vb.p, (size_t)tot, uk.p, agg.p, nruns.p,
-                                                  CWPlus(), rocprim::equal_to<uint64_t>(), st));
+                                                  CWPlus(), rocprim::equal_to<Key>(), st));
                     const int g2 = gridx(tot);
                     hipLaunchKernelGGL(k_c4s_pairs_w, dim3(g2), dim3(256), 0, st, agg.p, nruns.p, spart.p);
                     ZG_HIP(hipGetLastError());
                     unsigned __int128 g4 = 0;
-                    rc = host_sum128(spart.p, g2, st, &g4);   // synchronizes before ipre and cur change
-                    if (rc) return rc;
+                    const int rc2 = host_sum128(spart.p, g2, st, &g4);   // synchronizes before ipre and cur change
+                    if (rc2) return rc2;
                     c4 += g4;
                 }
+                return ZG_OK;
+            };
+            if (weighted && !batches.empty() && tmax > 0) {
+                rc = narrow ? weighted_batches(uint32_t(0)) : weighted_batches(uint64_t(0));
+                if (rc) return rc;
             } else if (!batches.empty() && tmax > 0) {
                 XB<uint64_t> ka, kb, d_ipre;
                 XB<unsigned int> counts, nruns;
