@@ -303,7 +303,8 @@ int zg_count_reads(zg_graph *g, const zg_plan *p, uint64_t seed, uint64_t first_
  * (ZGRAPH_C4_SORT_WEDGES lowers it; a hub with more wedges is counted with
  * a dense counter of n entries instead; ZGRAPH_C4_WIDE_KEYS=1 and
  * ZGRAPH_C4_BATCH_VERTICES force the 64-bit sort keys and cap the vertices of
- * a batch, for tests). */
+ * a batch, and ZGRAPH_HOUSE_WIDE_VALUES=1 the houses' unpacked 16-byte
+ * {count, weight} values, for tests). */
 int zg_exact_counts(zg_graph *g, int what, uint64_t *out);
 /* Degree moments sum d^2 and sum d^3 over all vertices (ELP features). */
 int zg_graph_degree_moments(zg_graph *g, double *sum_d2, double *sum_d3);

@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include <type_traits>
 #include <utility>
 
 #include <rocprim/rocprim.hpp>
@@ -1376,12 +1377,26 @@ struct CW {
 struct CWPlus {
     __host__ __device__ CW operator()(const CW &a, const CW &b) const { return CW{a.c + b.c, a.s + b.s}; }
 };
+// A packed value holds the count above bit sb and the weight below it; sums of
+// packed values stay exact while every run's weight stays below 2^sb and its
+// count below 2^(64 - sb), which the host checks from the maximum degree (a
+// run counts at most max_degree wedges, each weighing at most 2 max_degree).
+__device__ __forceinline__ void cw_make(CW &v, unsigned long long w, uint32_t) { v = CW{1ull, w}; }
+__device__ __forceinline__ void cw_make(unsigned long long &v, unsigned long long w, uint32_t sb)
+{
+    v = (1ull << sb) + w;
+}
+__device__ __forceinline__ CW cw_get(const CW &v, uint32_t) { return v; }
+__device__ __forceinline__ CW cw_get(unsigned long long v, uint32_t sb)
+{
+    return CW{v >> sb, v & ((1ull << sb) - 1)};
+}
 
-template <class Key>
+template <class Key, class Val>
 __global__ __launch_bounds__(256) void k_c4s_fill_w(const uint64_t *foff, const uint64_t *fkey, const uint32_t *sdeg,
                                                     const uint32_t *list, const uint64_t *ipre, uint32_t nb,
                                                     uint32_t wbits, unsigned long long *cursor, const uint32_t *tf,
-                                                    Key *keys, CW *vals)
+                                                    uint32_t sb, Key *keys, Val *vals)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -1405,18 +1420,19 @@ __global__ __launch_bounds__(256) void k_c4s_fill_w(const uint64_t *foff, const 
         base = __shfl(base, 0, 64);
         for (uint64_t k = lane; k < qv; k += 64) {
             keys[base + k] = ((Key)lo << wbits) | (Key)(uint32_t)fkey[vb + k];
-            vals[base + k] = CW{1ull, tuv + tf[vb + k]};
+            cw_make(vals[base + k], tuv + tf[vb + k], sb);
         }
     }
 }
 
-__global__ void k_c4s_pairs_w(const CW *runs, const unsigned int *nruns, unsigned long long *partial)
+template <class Val>
+__global__ void k_c4s_pairs_w(const Val *runs, const unsigned int *nruns, uint32_t sb, unsigned long long *partial)
 {
     const uint64_t n = *nruns;
     unsigned long long lo = 0, hi = 0;
     GRID_STRIDE(t, n)
     {
-        const CW r = runs[t];
+        const CW r = cw_get(runs[t], sb);
         const unsigned __int128 q = (unsigned __int128)(r.c - (r.c > 0)) * r.s;
         add128(lo, hi, (unsigned long long)q, (unsigned long long)(q >> 64));
     }
@@ -2019,11 +2035,19 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
             }
             // houses: the hubs' weighted wedges, (key, {count, weight}) pairs sorted and reduced by key;
             // 32-bit keys (batch index << wbits | w) while they fit, else 64-bit ones
-            auto weighted_batches = [&](auto key0) -> int {
+            // values packed in 8 bytes while the maximum degree bounds them (see cw_make)
+            uint32_t cbits = 1;
+            while ((1ull << cbits) <= g->max_degree) ++cbits;   // a run's count < 2^cbits
+            const uint32_t sb = 1 + 2 * cbits;                  // a run's weight < 2 maxdeg^2 < 2^sb
+            // (ZGRAPH_HOUSE_WIDE_VALUES=1 forces the 16-byte values, for tests)
+            const char *hv = getenv("ZGRAPH_HOUSE_WIDE_VALUES");
+            const bool packed = cbits + sb <= 64 && !(hv && hv[0] == '1');
+            auto weighted_batches = [&](auto key0, auto val0) -> int {
                 using Key = decltype(key0);
+                using Val = decltype(val0);
                 XB<Key> ka, kb, uk;
                 XB<uint64_t> d_ipre;
-                XB<CW> va, vb, agg;
+                XB<Val> va, vb, agg;
                 XB<unsigned int> nruns;
                 XB<unsigned long long> cursor;
                 XALLOC(ka, tmax); XALLOC(kb, tmax); XALLOC(uk, tmax);
@@ -2033,7 +2057,9 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                 size_t ts = 0, tr = 0;
                 ZG_HIP(rocprim::radix_sort_pairs(nullptr, ts, ka.p, kb.p, va.p, vb.p, (size_t)tmax, 0,
                                                  (unsigned)(8 * sizeof(Key)), st));
-                ZG_HIP(rocprim::reduce_by_key(nullptr, tr, kb.p, vb.p, (size_t)tmax, uk.p, agg.p, nruns.p, CWPlus(),
+                using Op = typename std::conditional<std::is_same<Val, CW>::value, CWPlus,
+                                                     rocprim::plus<unsigned long long>>::type;
+                ZG_HIP(rocprim::reduce_by_key(nullptr, tr, kb.p, vb.p, (size_t)tmax, uk.p, agg.p, nruns.p, Op(),
                                               rocprim::equal_to<Key>(), st));
                 XB<char> tmps;
                 XALLOC(tmps, std::max(ts, tr));
@@ -2057,21 +2083,21 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                     ZG_HIP(hipMemcpyAsync(cursor.p, cur.data(), 8ull * nb, hipMemcpyHostToDevice, st));
                     const int fgrid = (int)std::min<uint64_t>((uint64_t)device_cus() * 16,
                                                               std::max<uint64_t>(1, (ipre[nb] + 3) / 4));
-                    hipLaunchKernelGGL(k_c4s_fill_w<Key>, dim3(fgrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p,
-                                       ovf.p + b.first, d_ipre.p, nb, kw, cursor.p, tf.p, ka.p, va.p);
+                    hipLaunchKernelGGL((k_c4s_fill_w<Key, Val>), dim3(fgrid), dim3(256), 0, st, foff.p, fkey.p, sdeg.p,
+                                       ovf.p + b.first, d_ipre.p, nb, kw, cursor.p, tf.p, sb, ka.p, va.p);
                     ZG_HIP(hipGetLastError());
                     size_t t1 = 0, t2 = 0;
                     ZG_HIP(rocprim::radix_sort_pairs(nullptr, t1, ka.p, kb.p, va.p, vb.p, (size_t)tot, 0, ebits, st));
                     ZG_HIP(rocprim::reduce_by_key(nullptr, t2, kb.p, vb.p, (size_t)tot, uk.p, agg.p, nruns.p,
-                                                  CWPlus(), rocprim::equal_to<Key>(), st));
+                                                  Op(), rocprim::equal_to<Key>(), st));
                     if (std::max(t1, t2) > std::max(ts, tr))
                         return set_error(ZG_ENOMEM, "house count: sort scratch larger than sized");
                     t1 = t2 = std::max(ts, tr);
                     ZG_HIP(rocprim::radix_sort_pairs(tmps.p, t1, ka.p, kb.p, va.p, vb.p, (size_t)tot, 0, ebits, st));
                     ZG_HIP(rocprim::reduce_by_key(tmps.p, t2, kb.p, vb.p, (size_t)tot, uk.p, agg.p, nruns.p,
-                                                  CWPlus(), rocprim::equal_to<Key>(), st));
+                                                  Op(), rocprim::equal_to<Key>(), st));
                     const int g2 = gridx(tot);
-                    hipLaunchKernelGGL(k_c4s_pairs_w, dim3(g2), dim3(256), 0, st, agg.p, nruns.p, spart.p);
+                    hipLaunchKernelGGL(k_c4s_pairs_w<Val>, dim3(g2), dim3(256), 0, st, agg.p, nruns.p, sb, spart.p);
                     ZG_HIP(hipGetLastError());
                     unsigned __int128 g4 = 0;
                     const int rc2 = host_sum128(spart.p, g2, st, &g4);   // synchronizes before ipre and cur change
@@ -2081,7 +2107,10 @@ extern "C" int zg_exact_counts(zg_graph *g, int what, uint64_t *out)
                 return ZG_OK;
             };
             if (weighted && !batches.empty() && tmax > 0) {
-                rc = narrow ? weighted_batches(uint32_t(0)) : weighted_batches(uint64_t(0));
+                const CW cw0{0, 0};
+                const unsigned long long pv0 = 0;
+                if (narrow) rc = packed ? weighted_batches(uint32_t(0), pv0) : weighted_batches(uint32_t(0), cw0);
+                else rc = packed ? weighted_batches(uint64_t(0), pv0) : weighted_batches(uint64_t(0), cw0);
                 if (rc) return rc;
             } else if (!batches.empty() && tmax > 0) {
                 XB<uint64_t> ka, kb, d_ipre;

@@ -228,8 +228,11 @@ def test_device_c4_complete_bipartite(monkeypatch, a, b, sort_wedges):
 
 
 @pytest.mark.parametrize("env", [{}, {"ZGRAPH_C4_WIDE_KEYS": "1"}, {"ZGRAPH_C4_BATCH_VERTICES": "3"},
-                                 {"ZGRAPH_C4_WIDE_KEYS": "1", "ZGRAPH_C4_BATCH_VERTICES": "2"}],
-                         ids=["default", "wide-keys", "3-vertex-batches", "wide-keys-2-vertex-batches"])
+                                 {"ZGRAPH_C4_WIDE_KEYS": "1", "ZGRAPH_C4_BATCH_VERTICES": "2"},
+                                 {"ZGRAPH_HOUSE_WIDE_VALUES": "1"},
+                                 {"ZGRAPH_HOUSE_WIDE_VALUES": "1", "ZGRAPH_C4_WIDE_KEYS": "1"}],
+                         ids=["default", "wide-keys", "3-vertex-batches", "wide-keys-2-vertex-batches",
+                              "house-wide-values", "house-wide-values-wide-keys"])
 def test_device_c4_and_five_chains_rmat14(monkeypatch, env):
     """R-MAT scale 14 (262 k edges, hubs of a few thousand): device counts
     equal the oracle's pair-counting 4-cycles and closed-form chains -- also
