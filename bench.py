@@ -248,11 +248,37 @@ def main():
         "split_4clique": split,
         "split_5house": house,
         "cpu_baseline": cpu,
+        "time_to_error_summary": tte_summary(elp),
         "time_to_error": elp,
         "batched_queries": batch,
         "device": info,
     }
     print(json.dumps(out), flush=True)
+
+
+def tte_summary(elp):
+    """Mean seconds to +-1 % per pattern and arm (and the fraction of queries
+    within +-1 %), read from time_to_error's record; None where an arm did not
+    run.  Never raises: a malformed record gives an empty summary."""
+    if not isinstance(elp, dict):
+        return None
+    out = {}
+    try:
+        for name, rec in elp.items():
+            if not isinstance(rec, dict):
+                continue
+            row = {}
+            for arm in ("regression+judge", "pilot+judge", "learned+judge"):
+                a = rec.get(arm)
+                if isinstance(a, dict) and a.get("mean_time_s") is not None:
+                    row[arm] = {"s": a["mean_time_s"], "within_eps": a.get("within_eps")}
+            if rec.get("mean_time_s") is not None:   # completions-chosen and one-query records
+                row["query"] = {"s": rec["mean_time_s"], "within_eps": rec.get("within_eps")}
+            if row:
+                out[name] = row
+    except Exception:   # the summary is a convenience; the full record follows it
+        return {}
+    return out
 
 
 def split_k4(Z, g, args, N, k4_sums, k4_ms):

@@ -43,3 +43,18 @@ def test_profiled_traffic_ignores_other_workloads():
     assert bench.profiled_traffic("triangle", _args(edges=1000), N) is None
     assert bench.profiled_traffic("triangle", _args(), N // 2) is None
     assert bench.profiled_traffic("no-such-kernel", _args(), N) is None
+
+
+def test_time_to_error_summary_reads_every_arm_and_never_raises():
+    elp = {"epsilon": 0.01,
+           "triangle": {"regression+judge": {"mean_time_s": 0.17, "within_eps": 1.0},
+                        "learned+judge": {"mean_time_s": 0.005, "within_eps": 0.8},
+                        "pilot+judge": {"mean_time_s": None}},
+           "5-house split, completions chosen": {"mean_time_s": 0.02, "within_eps": 1.0}}
+    s = bench.tte_summary(elp)
+    assert s["triangle"] == {"regression+judge": {"s": 0.17, "within_eps": 1.0},
+                             "learned+judge": {"s": 0.005, "within_eps": 0.8}}
+    assert s["5-house split, completions chosen"] == {"query": {"s": 0.02, "within_eps": 1.0}}
+    assert bench.tte_summary(None) is None
+    # records of other shapes are skipped, not fatal
+    assert bench.tte_summary({"x": 5, "y": {"learned+judge": 3}, "z": {"queries": []}}) == {}
